@@ -1,0 +1,275 @@
+/*
+ * ltpg.h - C ABI of the MI355X deterministic concurrency-control path.
+ *
+ * The path is LTPG's large-batch deterministic concurrency control: a batch
+ * (epoch) of transactions goes through four phases,
+ *
+ *   EXECUTE    each op registers its TID with an atomic minimum on a per-row
+ *              register (R for reads, W for writes) and stages the columns it
+ *              reads from the table snapshot;
+ *   CHECK      each op reads back the opposite register(s) and counts raw /
+ *              war / waw conflicts; the transaction commits iff
+ *              waw == 0 && (raw == 0 || war == 0);
+ *   WRITEBACK  committed transactions write their write set to the table;
+ *   CID        (order-entry only) Payment by-name customer resolution.
+ *
+ * Every call here replaces a piece of the reference's implicit seam: the
+ * __global__ kernels of LTPG/TPCC-allopt-final/tp.cu:11-117 and
+ * LTPG/YCSB/0/ycsb.cu:9-58, the Database/Query lifecycle of
+ * LTPG/TPCC-allopt-final/Database.h:37-102 and Query.h:330-383,532-631, and
+ * the epoch loop of tp.cu:136-261.  Each declaration cites the part it
+ * replaces.
+ *
+ * Conventions
+ *   - Every function returns int: LTPG_OK (0) or a negative LTPG_ERR_* code.
+ *     ltpg_last_error() returns a thread-local message for the last failure.
+ *     No C++ exception crosses this boundary.
+ *   - A context is single-threaded: the caller serialises calls on one
+ *     context.  Separate contexts are independent.
+ *   - The library owns every device allocation.  Batches given with
+ *     LTPG_MEM_HOST are copied to the device before the call returns;
+ *     batches given with LTPG_MEM_DEVICE are read in place and must stay
+ *     valid until ltpg_sync() returns.
+ *   - TIDs are u32 priorities; a lower TID wins.  TIDs must be distinct
+ *     within one batch (the reference's TID = (epoch & 0xfff) << 20 | seq,
+ *     TPCC-allopt-final/Query.h:500, satisfies this).
+ */
+#ifndef LTPG_H
+#define LTPG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LTPG_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define LTPG_OK 0
+#define LTPG_ERR_INVALID (-1)     /* bad argument or shape                   */
+#define LTPG_ERR_NOMEM (-2)       /* device or host allocation failed        */
+#define LTPG_ERR_DEVICE (-3)      /* HIP runtime error / no device           */
+#define LTPG_ERR_STATE (-4)       /* call out of order (e.g. no batch)       */
+#define LTPG_ERR_RANGE (-5)       /* a key / row id outside its table        */
+#define LTPG_ERR_BUDGET (-6)      /* admission refused: HBM budget exceeded  */
+#define LTPG_ERR_UNSUPPORTED (-7) /* feature not configured in this context  */
+
+/* ---- rule sets (which register each op slot registers and checks) ------ */
+/* REFERENCE: the slots that actually dispatch in the reference, including
+ *   the NewOrder check dispatch that never runs check_3_0
+ *   (TPCC-allopt-final/Execute_neworder.h:635-646) and the NEWORDER insert
+ *   registering R (Execute_neworder.h:115).
+ * INTENDED: the NEWORDER insert registers W and checks R->war and W->waw. */
+#define LTPG_RULES_REFERENCE 0u
+#define LTPG_RULES_INTENDED 1u
+
+/* ---- value semantics of write-back ------------------------------------ */
+/* COPYBACK: a committed write stores the snapshot value it staged, as the
+ *   reference does (TPCC-allopt-final/Genericfunction.h:93-107,416-430):
+ *   the table never changes.
+ * RMW: key-value writes add (TID | 1) to every u32 word of their field;
+ *   order-entry writes store the values of the order-entry rules (NewOrder
+ *   inserts, stock update, Payment customer update, W_YTD/D_YTD reduce). */
+#define LTPG_VALUES_COPYBACK 0u
+#define LTPG_VALUES_RMW 1u
+
+/* ---- memory kinds ------------------------------------------------------ */
+#define LTPG_MEM_HOST 0u
+#define LTPG_MEM_DEVICE 1u
+
+/* ---- table layouts ----------------------------------------------------- */
+/* COLUMN_MAJOR is the reference's SNAPSHOT layout, tbl[col * rows + row]
+ * (TPCC-allopt-final/Datastructure.h:119-130, YCSB/0/Datastructure.h:16-19).
+ * ROW_MAJOR is the device layout, tbl[row * cols + col]. */
+#define LTPG_LAYOUT_ROW_MAJOR 0u
+#define LTPG_LAYOUT_COLUMN_MAJOR 1u
+
+/* ---- workloads --------------------------------------------------------- */
+#define LTPG_WORKLOAD_KV 1u
+#define LTPG_WORKLOAD_TPCC 2u
+
+/* ---- transaction types (result queries) --------------------------------- */
+#define LTPG_TYPE_KV 0u
+#define LTPG_TYPE_NEWORDER 1u
+#define LTPG_TYPE_PAYMENT 2u
+
+/* ---- phases ------------------------------------------------------------ */
+#define LTPG_PHASE_CID 1u
+#define LTPG_PHASE_EXECUTE 2u
+#define LTPG_PHASE_CHECK 4u
+#define LTPG_PHASE_WRITEBACK 8u
+#define LTPG_PHASE_ALL 15u
+
+/* ---- key-value op encoding (meta word of one op slot) ------------------ */
+/* bits 0-1: kind; bits 8-23: field index.  A slot of kind NONE is empty
+ * (ragged transactions).  A READ registers R and stages its field; a WRITE
+ * is a read-modify-write of its field that registers W (YCSB/0/Execute.h:
+ * 11-78). */
+#define LTPG_KV_NONE 0u
+#define LTPG_KV_READ 1u
+#define LTPG_KV_WRITE 2u
+#define LTPG_KV_META(kind, field) ((uint32_t)(kind) | ((uint32_t)(field) << 8))
+#define LTPG_KV_KIND(meta) ((meta) & 3u)
+#define LTPG_KV_FIELD(meta) (((meta) >> 8) & 0xffffu)
+
+#define LTPG_NO_HOT_ROW UINT64_MAX
+
+/* ---- order-entry schema (TPCC-allopt-final/Predefine.h:48-77) ---------- */
+#define LTPG_TPCC_WAREHOUSE 0u
+#define LTPG_TPCC_DISTRICT 1u
+#define LTPG_TPCC_CUSTOMER 2u
+#define LTPG_TPCC_HISTORY 3u
+#define LTPG_TPCC_NEWORDER 4u
+#define LTPG_TPCC_ORDER 5u
+#define LTPG_TPCC_ORDERLINE 6u
+#define LTPG_TPCC_STOCK 7u
+#define LTPG_TPCC_ITEM 8u
+#define LTPG_TPCC_NTABLES 9u
+#define LTPG_TPCC_MAX_LINES 15u
+
+/* ---- context configuration --------------------------------------------- */
+typedef struct ltpg_config {
+    uint32_t size;          /* sizeof(ltpg_config), set by ltpg_config_init   */
+    int32_t device;         /* HIP device ordinal                              */
+    uint32_t workload;      /* LTPG_WORKLOAD_KV or LTPG_WORKLOAD_TPCC          */
+    uint32_t rules;         /* LTPG_RULES_*                                    */
+    uint32_t values;        /* LTPG_VALUES_*                                   */
+    uint32_t max_txns;      /* capacity: transactions per batch (per type)     */
+    /* key-value table (YCSB/0/Predefine.h:9-10,21) */
+    uint64_t kv_rows;
+    uint32_t kv_fields;
+    uint32_t kv_field_bytes; /* multiple of 4, at most 1024                    */
+    uint32_t kv_max_slots;   /* capacity: op slots per transaction (<= 64)     */
+    uint32_t reserved0;
+    uint64_t kv_hot_row;     /* LTPG_NO_HOT_ROW, or a row whose writes go      */
+                             /* through the commutative reduce                 */
+    /* order-entry (TPCC-allopt-final/Predefine.h:9) */
+    uint32_t tpcc_warehouses;
+    uint32_t reserved1;
+    uint64_t hbm_budget;     /* serving: bytes of access-set buffers admitted  */
+} ltpg_config;
+
+/* Fill cfg with defaults (device 0, KV, reference rules, RMW values). */
+void ltpg_config_init(ltpg_config *cfg);
+
+/* ---- lifecycle (Database()/Query() ctors, TPCC-allopt-final/Database.h:37-57,
+ *      Query.h:330-355; clear_LOG, Database.h:78-102) ----------------------- */
+typedef struct ltpg_ctx ltpg_ctx;
+
+int ltpg_create(const ltpg_config *cfg, ltpg_ctx **out);
+int ltpg_destroy(ltpg_ctx *ctx);
+const char *ltpg_last_error(void);
+int ltpg_abi_version(void);
+/* Number of HIP devices visible (0 when none).  Never initialises a context. */
+int ltpg_device_count(void);
+
+/* Use the caller's hipStream_t (NULL: the context's own stream). */
+int ltpg_set_stream(ltpg_ctx *ctx, void *hip_stream);
+/* Block until all work queued by this context has finished; returns any
+ * deferred error (e.g. LTPG_ERR_RANGE raised by a kernel bounds check). */
+int ltpg_sync(ltpg_ctx *ctx);
+
+/* ---- device memory for LTPG_MEM_DEVICE batches (the reference's
+ *      cudaMalloc'd query slots, TPCC-allopt-final/Query.h:344-351, and their
+ *      H2D copies, Query.h:544-578) ---------------------------------------- */
+int ltpg_device_alloc(ltpg_ctx *ctx, uint64_t bytes, void **dptr);
+int ltpg_device_free(ltpg_ctx *ctx, void *dptr);
+/* Synchronous host -> device / device -> host copies on the context stream. */
+int ltpg_copy_to_device(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- key-value tables (YCSB/0/Database.h:14-131) ------------------------ */
+/* Load rows x kv_fields x kv_field_bytes.  COLUMN_MAJOR requires
+ * kv_field_bytes == 4 (the reference's u32 snapshot). */
+int ltpg_kv_table_load(ltpg_ctx *ctx, const void *host, uint64_t rows, uint32_t layout);
+int ltpg_kv_table_read(ltpg_ctx *ctx, void *host, uint64_t rows, uint32_t layout);
+/* Device-side fill with the reference's initial contents: every u32 word of
+ * field c holds c (YCSB/0/Database.h:72-99). */
+int ltpg_kv_table_fill_reference(ltpg_ctx *ctx);
+
+/* ---- key-value batch (YCSB_A_QUERY, YCSB/0/Datastructure.h:36-41) ------- */
+/* Slot-major, as the reference kernel indexes LOC[B * op + NO]
+ * (YCSB/0/Execute.h:20,47): slot s of transaction t is at [s * n_txns + t]. */
+typedef struct ltpg_kv_batch {
+    uint32_t n_txns;         /* B                                              */
+    uint32_t n_slots;        /* S (<= kv_max_slots)                            */
+    const uint64_t *keys;    /* [S * B] row ids                                */
+    const uint32_t *meta;    /* [S * B] LTPG_KV_META(kind, field)              */
+    const uint32_t *tids;    /* [B]                                            */
+} ltpg_kv_batch;
+
+/* Make `batch` the context's current epoch (copy_to_device,
+ * YCSB/0/Query.h:197-216).  Starts a new epoch: registers of the previous
+ * epoch become stale without a memset. */
+int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *batch, uint32_t mem);
+
+/* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
+ *      tp.cu:24-117, YCSB/0/ycsb.cu:9-58; launched by tp.cu:154-236) ------- */
+/* Run the phases in phase_mask on the current batch, in the order CID,
+ * EXECUTE, CHECK, WRITEBACK.  Asynchronous on the context's stream; errors
+ * raised by kernels are returned by the next ltpg_sync(). */
+int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask);
+/* Make every per-row register read as "no registrant" (clear_LOG,
+ * TPCC-allopt-final/Database.h:78-102).  O(1): registers carry an epoch tag,
+ * so this advances the tag instead of writing 0xFFFFFFFF over every row. */
+int ltpg_epoch_reset(ltpg_ctx *ctx);
+
+/* ---- results (statistic_query / copy_to_host, TPCC-allopt-final/Query.h:
+ *      606-631,697-725; COMMIT_AND_ABORT, YCSB/0/Execute.h:176-192) ------- */
+/* Commit bitmap of the last epoch: bit t of word t/64 is 1 iff transaction t
+ * (of `type`) committed.  Synchronises the context. */
+int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_words);
+/* Number of committed transactions of `type` in the last epoch. */
+int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out);
+/* Per-transaction conflict counters raw | war << 8 | waw << 16 (the
+ * reference's raw/war/waw[NO], TPCC-allopt-final/Datastructure.h:231-233). */
+int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t n);
+/* Staged read values of the last key-value epoch (set_local_set,
+ * YCSB/0/Genericfunction.h:31-45): field words of slot s of transaction t at
+ * out[((s * B) + t) * field_words + w].  Only read slots are written. */
+int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words);
+/* Running totals since the last reset: committed and attempted (checked)
+ * transactions of `type`, kept on the device so a pipelined epoch loop needs
+ * no per-epoch readback.  reset != 0 zeroes them after reading. */
+int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t *attempts,
+                       int reset);
+/* Enable (1) or disable (0) per-phase HIP events (off by default).  While
+ * enabled every ltpg_epoch_run records events around each phase on the
+ * context's stream. */
+int ltpg_set_timing(ltpg_ctx *ctx, int enable);
+/* Sum of the device time in milliseconds of each phase [CID, EXECUTE, CHECK,
+ * WRITEBACK] over the epochs timed since the previous call, and their
+ * number; then starts a new measurement.  Synchronises the context. */
+int ltpg_phase_times(ltpg_ctx *ctx, float ms[4], uint32_t *n_epochs);
+
+/* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
+ *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:
+ *      17-121, YCSB/0/Random.h:12,52) ----------------------------------- */
+typedef struct ltpg_kv_gen {
+    uint64_t seed;
+    uint64_t rows;           /* keys are row ids in [0, rows)                  */
+    uint32_t fields;         /* field index uniform in [0, fields)             */
+    uint32_t n_txns;
+    uint32_t n_slots;        /* S of the slot-major output                     */
+    uint32_t min_ops;        /* ops per txn uniform in [min_ops, max_ops]      */
+    uint32_t max_ops;        /* <= n_slots; slots beyond a txn's count are     */
+                             /* LTPG_KV_NONE                                   */
+    uint32_t read_pct;       /* percent of ops that are reads                  */
+    double theta;            /* Zipf skew; 0 = uniform                         */
+    uint32_t tid_base;       /* tids[t] = tid_base + t                         */
+    uint32_t distinct_keys;  /* 1: keys within one transaction are distinct    */
+    uint64_t txn_base;       /* global index of transaction 0 (stream offset)  */
+} ltpg_kv_gen;
+
+void ltpg_kv_gen_init(ltpg_kv_gen *g);
+/* Fill keys[S*B], meta[S*B], tids[B].  Deterministic in (seed, txn_base + t,
+ * slot); uses up to `threads` host threads (0 = hardware concurrency). */
+int ltpg_kv_generate(const ltpg_kv_gen *g, uint64_t *keys, uint32_t *meta, uint32_t *tids,
+                     uint32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LTPG_H */

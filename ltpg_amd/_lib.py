@@ -1,0 +1,116 @@
+"""ctypes declarations for libltpg.so (include/ltpg.h).
+
+The shared library is the product: this module only loads it and declares
+the C signatures.  Importing fails loudly when the library has not been built
+(there is no Python or CPU fallback for the hot path).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libltpg.so")
+
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class Config(ctypes.Structure):
+    """struct ltpg_config."""
+
+    _fields_ = [
+        ("size", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("workload", ctypes.c_uint32),
+        ("rules", ctypes.c_uint32),
+        ("values", ctypes.c_uint32),
+        ("max_txns", ctypes.c_uint32),
+        ("kv_rows", ctypes.c_uint64),
+        ("kv_fields", ctypes.c_uint32),
+        ("kv_field_bytes", ctypes.c_uint32),
+        ("kv_max_slots", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("kv_hot_row", ctypes.c_uint64),
+        ("tpcc_warehouses", ctypes.c_uint32),
+        ("reserved1", ctypes.c_uint32),
+        ("hbm_budget", ctypes.c_uint64),
+    ]
+
+
+class KvBatch(ctypes.Structure):
+    """struct ltpg_kv_batch (slot-major, slot s of txn t at [s*B + t])."""
+
+    _fields_ = [
+        ("n_txns", ctypes.c_uint32),
+        ("n_slots", ctypes.c_uint32),
+        ("keys", ctypes.c_void_p),
+        ("meta", ctypes.c_void_p),
+        ("tids", ctypes.c_void_p),
+    ]
+
+
+class KvGen(ctypes.Structure):
+    """struct ltpg_kv_gen."""
+
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("rows", ctypes.c_uint64),
+        ("fields", ctypes.c_uint32),
+        ("n_txns", ctypes.c_uint32),
+        ("n_slots", ctypes.c_uint32),
+        ("min_ops", ctypes.c_uint32),
+        ("max_ops", ctypes.c_uint32),
+        ("read_pct", ctypes.c_uint32),
+        ("theta", ctypes.c_double),
+        ("tid_base", ctypes.c_uint32),
+        ("distinct_keys", ctypes.c_uint32),
+        ("txn_base", ctypes.c_uint64),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point.
+SIGNATURES = [
+    ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
+    ("ltpg_create", ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]),
+    ("ltpg_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_last_error", ctypes.c_char_p, []),
+    ("ltpg_abi_version", ctypes.c_int, []),
+    ("ltpg_device_count", ctypes.c_int, []),
+    ("ltpg_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("ltpg_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_device_alloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("ltpg_device_free", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("ltpg_copy_to_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    ("ltpg_copy_to_host", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    ("ltpg_kv_table_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
+    ("ltpg_kv_table_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
+    ("ltpg_kv_table_fill_reference", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_kv_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KvBatch), ctypes.c_uint32]),
+    ("ltpg_epoch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("ltpg_epoch_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_commit_bitmap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, ctypes.c_uint64]),
+    ("ltpg_commit_count", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p]),
+    ("ltpg_conflict_counters", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint32]),
+    ("ltpg_kv_read_values", ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_uint64]),
+    ("ltpg_phase_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), c_u32p]),
+    ("ltpg_commit_totals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u64p, ctypes.c_int]),
+    ("ltpg_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
+    ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
+]
+
+
+def load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C ltpg_amd/csrc` "
+            "(or __graft_entry__.build()); there is no fallback for the HIP path")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load()

@@ -1,0 +1,561 @@
+// api.cpp - the extern "C" entry points of libltpg (include/ltpg.h).
+//
+// Host-side counterpart of the reference's Database / Query lifecycle
+// (LTPG/TPCC-allopt-final/Database.h:37-102, Query.h:330-383,532-631;
+// LTPG/YCSB/0/Database.h:27-50, Query.h:145-225) and of the per-epoch kernel
+// launches of tp.cu:136-261 / ycsb.cu:78-152.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "ltpg_internal.h"
+
+namespace ltpg {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+template <typename T>
+int dev_alloc(T **p, uint64_t count, const char *what)
+{
+    *p = nullptr;
+    if (count == 0) return LTPG_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error(std::string("hipMalloc(") + what + ", " + std::to_string(count * sizeof(T)) +
+                  " bytes): " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? LTPG_ERR_NOMEM : LTPG_ERR_DEVICE;
+    }
+    return LTPG_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int check_ctx(ltpg_ctx *ctx)
+{
+    if (!ctx) LTPG_FAIL(LTPG_ERR_INVALID, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return LTPG_OK;
+}
+
+void free_kv(KvState &kv)
+{
+    dev_free(kv.table);
+    dev_free(kv.regs);
+    dev_free(kv.read_values);
+    dev_free(kv.bitmap);
+    dev_free(kv.conflicts);
+    dev_free(kv.keys_buf);
+    dev_free(kv.meta_buf);
+    dev_free(kv.tids_buf);
+}
+
+int init_kv(ltpg_ctx *ctx)
+{
+    const ltpg_config &c = ctx->cfg;
+    KvState &kv = ctx->kv;
+    if (c.kv_rows == 0 || c.kv_fields == 0) LTPG_FAIL(LTPG_ERR_INVALID, "kv_rows and kv_fields must be > 0");
+    if (c.kv_field_bytes == 0 || c.kv_field_bytes % 4 != 0 || c.kv_field_bytes > 1024)
+        LTPG_FAIL(LTPG_ERR_INVALID, "kv_field_bytes must be a multiple of 4 in [4, 1024]");
+    if (c.kv_fields > 65536) LTPG_FAIL(LTPG_ERR_INVALID, "kv_fields must be <= 65536");
+    if (c.kv_max_slots == 0 || c.kv_max_slots > 64)
+        LTPG_FAIL(LTPG_ERR_INVALID, "kv_max_slots must be in [1, 64]");
+    if (c.kv_hot_row != LTPG_NO_HOT_ROW)
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "kv_hot_row is not supported by this build");
+    kv.rows = c.kv_rows;
+    kv.fields = c.kv_fields;
+    kv.field_words = c.kv_field_bytes / 4;
+    kv.max_slots = c.kv_max_slots;
+    const uint64_t B = c.max_txns, S = c.kv_max_slots;
+    int rc;
+    if ((rc = dev_alloc(&kv.table, kv.rows * kv.fields * kv.field_words, "kv table"))) return rc;
+    if ((rc = dev_alloc(&kv.regs, kv.rows * 2, "kv registers"))) return rc;
+    if ((rc = dev_alloc(&kv.read_values, S * B * kv.field_words, "kv read values"))) return rc;
+    if ((rc = dev_alloc(&kv.bitmap, (B + 63) / 64, "kv bitmap"))) return rc;
+    if ((rc = dev_alloc(&kv.conflicts, B, "kv conflicts"))) return rc;
+    if ((rc = dev_alloc(&kv.keys_buf, S * B, "kv keys"))) return rc;
+    if ((rc = dev_alloc(&kv.meta_buf, S * B, "kv meta"))) return rc;
+    if ((rc = dev_alloc(&kv.tids_buf, B, "kv tids"))) return rc;
+    // Registers start as all-ones: tag ~0 belongs to epoch 0, which is never
+    // current (epochs start at 1), so every row reads as "no registrant".
+    LTPG_HIP_TRY(launch_fill_u64(kv.regs, kv.rows * 2, ~0ull, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(kv.table, 0, kv.rows * kv.fields * kv.field_words * 4, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(kv.bitmap, 0, (B + 63) / 64 * 8, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(kv.conflicts, 0, B * 4, ctx->stream));
+    return LTPG_OK;
+}
+
+// Advance the register tag; at wrap-around refill every register.
+int next_epoch(ltpg_ctx *ctx)
+{
+    ctx->epoch += 1;
+    if (ctx->epoch == 0) {
+        if (ctx->kv.regs) LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
+        ctx->epoch = 1;
+    }
+    return LTPG_OK;
+}
+
+}  // namespace
+}  // namespace ltpg
+
+using namespace ltpg;
+
+extern "C" {
+
+void ltpg_config_init(ltpg_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->size = sizeof(ltpg_config);
+    cfg->device = 0;
+    cfg->workload = LTPG_WORKLOAD_KV;
+    cfg->rules = LTPG_RULES_REFERENCE;
+    cfg->values = LTPG_VALUES_RMW;
+    cfg->max_txns = 65536;
+    cfg->kv_rows = 1000000;
+    cfg->kv_fields = 10;
+    cfg->kv_field_bytes = 4;
+    cfg->kv_max_slots = 16;
+    cfg->kv_hot_row = LTPG_NO_HOT_ROW;
+    cfg->tpcc_warehouses = 64;
+    cfg->hbm_budget = 0;
+}
+
+int ltpg_abi_version(void) { return LTPG_ABI_VERSION; }
+
+const char *ltpg_last_error(void) { return g_last_error.c_str(); }
+
+int ltpg_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int ltpg_create(const ltpg_config *cfg, ltpg_ctx **out)
+{
+    if (!cfg || !out) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->size != sizeof(ltpg_config)) LTPG_FAIL(LTPG_ERR_INVALID, "ltpg_config.size mismatch (call ltpg_config_init)");
+    if (cfg->workload != LTPG_WORKLOAD_KV && cfg->workload != LTPG_WORKLOAD_TPCC)
+        LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload");
+    if (cfg->rules > LTPG_RULES_INTENDED) LTPG_FAIL(LTPG_ERR_INVALID, "unknown rule set");
+    if (cfg->values > LTPG_VALUES_RMW) LTPG_FAIL(LTPG_ERR_INVALID, "unknown value mode");
+    if (cfg->max_txns == 0 || cfg->max_txns > (1u << 26)) LTPG_FAIL(LTPG_ERR_INVALID, "max_txns must be in [1, 2^26]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) LTPG_FAIL(LTPG_ERR_DEVICE, "no HIP device visible");
+    if (cfg->device < 0 || cfg->device >= ndev) LTPG_FAIL(LTPG_ERR_INVALID, "device ordinal out of range");
+
+    ltpg_ctx *ctx = new (std::nothrow) ltpg_ctx();
+    if (!ctx) LTPG_FAIL(LTPG_ERR_NOMEM, "host allocation of context failed");
+    ctx->cfg = *cfg;
+    ctx->device = cfg->device;
+    int rc = LTPG_OK;
+    do {
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) {
+            set_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+            rc = LTPG_ERR_DEVICE;
+            break;
+        }
+        e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+            rc = LTPG_ERR_DEVICE;
+            break;
+        }
+        ctx->stream = ctx->own_stream;
+        if ((rc = dev_alloc(&ctx->counters, 1, "counters"))) break;
+        e = hipMemsetAsync(ctx->counters, 0, sizeof(DeviceCounters), ctx->stream);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+            rc = LTPG_ERR_DEVICE;
+            break;
+        }
+        if (cfg->workload == LTPG_WORKLOAD_KV) {
+            if ((rc = init_kv(ctx))) break;
+        } else {
+            set_error("order-entry workload is not supported by this build");
+            rc = LTPG_ERR_UNSUPPORTED;
+            break;
+        }
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+            rc = LTPG_ERR_DEVICE;
+            break;
+        }
+    } while (0);
+    if (rc) {
+        std::string msg = g_last_error;
+        ltpg_destroy(ctx);
+        set_error(msg);
+        return rc;
+    }
+    *out = ctx;
+    return LTPG_OK;
+}
+
+int ltpg_destroy(ltpg_ctx *ctx)
+{
+    if (!ctx) return LTPG_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_kv(ctx->kv);
+    dev_free(ctx->counters);
+    for (auto &ev : ctx->ev_pool)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return LTPG_OK;
+}
+
+int ltpg_set_stream(ltpg_ctx *ctx, void *hip_stream)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+    return LTPG_OK;
+}
+
+int ltpg_sync(ltpg_ctx *ctx)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t errors = 0;
+    LTPG_HIP_TRY(hipMemcpy(&errors, &ctx->counters->range_errors, sizeof(errors), hipMemcpyDeviceToHost));
+    if (errors) {
+        LTPG_HIP_TRY(hipMemset(&ctx->counters->range_errors, 0, sizeof(uint32_t)));
+        LTPG_FAIL(LTPG_ERR_RANGE, std::to_string(errors) + " op(s) named a row or field outside the table");
+    }
+    return LTPG_OK;
+}
+
+int ltpg_set_timing(ltpg_ctx *ctx, int enable)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    ctx->timing = enable != 0;
+    return LTPG_OK;
+}
+
+int ltpg_phase_times(ltpg_ctx *ctx, float ms[4], uint32_t *n_epochs)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ms) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; k++) ms[k] = 0.f;
+    const uint32_t n = ctx->ev_used / 5;
+    for (uint32_t e = 0; e < n; e++) {
+        for (int k = 0; k < 4; k++) {
+            float v = 0.f;
+            LTPG_HIP_TRY(hipEventElapsedTime(&v, ctx->ev_pool[5 * e + k], ctx->ev_pool[5 * e + k + 1]));
+            ms[k] += v;
+        }
+    }
+    if (n_epochs) *n_epochs = n;
+    ctx->ev_used = 0;
+    return LTPG_OK;
+}
+
+int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t *attempts, int reset)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    DeviceCounters h;
+    LTPG_HIP_TRY(hipMemcpy(&h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost));
+    if (commits) *commits = h.commits_total[type];
+    if (attempts) *attempts = h.attempts_total[type];
+    if (reset) {
+        LTPG_HIP_TRY(hipMemset(&ctx->counters->commits_total[type], 0, sizeof(unsigned long long)));
+        LTPG_HIP_TRY(hipMemset(&ctx->counters->attempts_total[type], 0, sizeof(unsigned long long)));
+    }
+    return LTPG_OK;
+}
+
+// ---- device memory ---------------------------------------------------------
+
+int ltpg_device_alloc(ltpg_ctx *ctx, uint64_t bytes, void **dptr)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dptr) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    *dptr = nullptr;
+    if (bytes == 0) LTPG_FAIL(LTPG_ERR_INVALID, "zero-byte allocation");
+    uint8_t *p = nullptr;
+    if ((rc = dev_alloc(&p, bytes, "caller buffer"))) return rc;
+    *dptr = p;
+    return LTPG_OK;
+}
+
+int ltpg_device_free(ltpg_ctx *ctx, void *dptr)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (dptr) {
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        LTPG_HIP_TRY(hipFree(dptr));
+    }
+    return LTPG_OK;
+}
+
+int ltpg_copy_to_device(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dst || !src) LTPG_FAIL(LTPG_ERR_INVALID, "null pointer");
+    if (bytes) {
+        LTPG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return LTPG_OK;
+}
+
+int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dst || !src) LTPG_FAIL(LTPG_ERR_INVALID, "null pointer");
+    if (bytes) {
+        LTPG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return LTPG_OK;
+}
+
+// ---- key-value tables ---------------------------------------------------
+
+int ltpg_kv_table_load(ltpg_ctx *ctx, const void *host, uint64_t rows, uint32_t layout)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    if (!host) LTPG_FAIL(LTPG_ERR_INVALID, "null host table");
+    if (rows != kv.rows) LTPG_FAIL(LTPG_ERR_INVALID, "rows does not match the configured kv_rows");
+    const uint64_t words = kv.rows * kv.fields * kv.field_words;
+    if (layout == LTPG_LAYOUT_ROW_MAJOR) {
+        LTPG_HIP_TRY(hipMemcpyAsync(kv.table, host, words * 4, hipMemcpyHostToDevice, ctx->stream));
+    } else if (layout == LTPG_LAYOUT_COLUMN_MAJOR) {
+        if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major load needs kv_field_bytes == 4");
+        uint32_t *tmp = nullptr;
+        if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
+        hipError_t e = hipMemcpyAsync(tmp, host, words * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_kv_transpose(tmp, kv.table, kv.rows, kv.fields, 1, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        dev_free(tmp);
+        if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major load: ") + hipGetErrorString(e));
+    } else {
+        LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
+    }
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return LTPG_OK;
+}
+
+int ltpg_kv_table_read(ltpg_ctx *ctx, void *host, uint64_t rows, uint32_t layout)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    if (!host) LTPG_FAIL(LTPG_ERR_INVALID, "null host table");
+    if (rows != kv.rows) LTPG_FAIL(LTPG_ERR_INVALID, "rows does not match the configured kv_rows");
+    const uint64_t words = kv.rows * kv.fields * kv.field_words;
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (layout == LTPG_LAYOUT_ROW_MAJOR) {
+        LTPG_HIP_TRY(hipMemcpy(host, kv.table, words * 4, hipMemcpyDeviceToHost));
+    } else if (layout == LTPG_LAYOUT_COLUMN_MAJOR) {
+        if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major read needs kv_field_bytes == 4");
+        uint32_t *tmp = nullptr;
+        if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
+        hipError_t e = launch_kv_transpose(kv.table, tmp, kv.rows, kv.fields, 0, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(host, tmp, words * 4, hipMemcpyDeviceToHost);
+        dev_free(tmp);
+        if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major read: ") + hipGetErrorString(e));
+    } else {
+        LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
+    }
+    return LTPG_OK;
+}
+
+int ltpg_kv_table_fill_reference(ltpg_ctx *ctx)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    LTPG_HIP_TRY(launch_kv_fill_reference(kv.table, kv.rows, kv.fields, kv.field_words, ctx->stream));
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return LTPG_OK;
+}
+
+// ---- key-value batches --------------------------------------------------
+
+int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null batch");
+    if (b->n_txns > ctx->cfg.max_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n_txns exceeds max_txns");
+    if (b->n_slots > kv.max_slots) LTPG_FAIL(LTPG_ERR_INVALID, "n_slots exceeds kv_max_slots");
+    if (b->n_txns > 0 && (!b->keys || !b->meta || !b->tids)) LTPG_FAIL(LTPG_ERR_INVALID, "null batch array");
+    if (mem != LTPG_MEM_HOST && mem != LTPG_MEM_DEVICE) LTPG_FAIL(LTPG_ERR_INVALID, "unknown memory kind");
+    if ((rc = next_epoch(ctx))) return rc;
+    KvBatchDev d;
+    d.n_txns = b->n_txns;
+    d.n_slots = b->n_slots;
+    if (mem == LTPG_MEM_DEVICE) {
+        d.keys = b->keys;
+        d.meta = b->meta;
+        d.tids = b->tids;
+    } else {
+        const uint64_t n = (uint64_t)b->n_txns * b->n_slots;
+        if (n) {
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.keys_buf, b->keys, n * 8, hipMemcpyHostToDevice, ctx->stream));
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.meta_buf, b->meta, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (b->n_txns)
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.tids_buf, b->tids, (uint64_t)b->n_txns * 4, hipMemcpyHostToDevice, ctx->stream));
+        // pageable source: the copy must finish before the caller may reuse it
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        d.keys = kv.keys_buf;
+        d.meta = kv.meta_buf;
+        d.tids = kv.tids_buf;
+    }
+    kv.cur = d;
+    kv.has_batch = true;
+    return LTPG_OK;
+}
+
+// ---- epoch phases --------------------------------------------------------
+
+namespace {
+// Take the next 5 timing events of this epoch from the pool.
+int take_events(ltpg_ctx *ctx, hipEvent_t **out)
+{
+    if (ctx->ev_used + 5 > ctx->ev_pool.size()) {
+        for (int k = 0; k < 40; k++) {
+            hipEvent_t e = nullptr;
+            LTPG_HIP_TRY(hipEventCreate(&e));
+            ctx->ev_pool.push_back(e);
+        }
+    }
+    *out = &ctx->ev_pool[ctx->ev_used];
+    ctx->ev_used += 5;
+    return LTPG_OK;
+}
+}  // namespace
+
+int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (phase_mask & ~LTPG_PHASE_ALL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown phase bit");
+    KvState &kv = ctx->kv;
+    if (!kv.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "no batch submitted");
+    const uint32_t inv = ~ctx->epoch;
+    hipStream_t s = ctx->stream;
+    hipEvent_t *ev = nullptr;
+    if (ctx->timing) {
+        if ((rc = take_events(ctx, &ev))) return rc;
+        LTPG_HIP_TRY(hipEventRecord(ev[0], s));
+    }
+    // CID has no key-value work: its bracket is empty.
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
+    if (phase_mask & LTPG_PHASE_EXECUTE) LTPG_HIP_TRY(launch_kv_execute(kv, kv.cur, inv, ctx->counters, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
+    if (phase_mask & LTPG_PHASE_CHECK) {
+        LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
+        LTPG_HIP_TRY(launch_kv_check(kv, kv.cur, inv, ctx->counters, s));
+    }
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[3], s));
+    if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_writeback(kv, kv.cur, ctx->cfg.values, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[4], s));
+    return LTPG_OK;
+}
+
+int ltpg_epoch_reset(ltpg_ctx *ctx)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    return next_epoch(ctx);
+}
+
+// ---- results --------------------------------------------------------------
+
+int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_words)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    if (type != LTPG_TYPE_KV || !ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "unknown transaction type");
+    const uint64_t need = ((uint64_t)ctx->kv.cur.n_txns + 63) / 64;
+    if (n_words < need) LTPG_FAIL(LTPG_ERR_INVALID, "bitmap buffer too small");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (need) LTPG_HIP_TRY(hipMemcpy(out, ctx->kv.bitmap, need * 8, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    LTPG_HIP_TRY(hipMemcpy(out, &ctx->counters->commits[type], sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t n)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    if (type != LTPG_TYPE_KV || !ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "unknown transaction type");
+    if (n > ctx->kv.cur.n_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the batch size");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) LTPG_HIP_TRY(hipMemcpy(out, ctx->kv.conflicts, (uint64_t)n * 4, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    const uint64_t have = (uint64_t)kv.cur.n_slots * kv.cur.n_txns * kv.field_words;
+    if (n_words > have) LTPG_FAIL(LTPG_ERR_INVALID, "n_words exceeds the staged read values");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n_words) LTPG_HIP_TRY(hipMemcpy(out, kv.read_values, n_words * 4, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+}  // extern "C"

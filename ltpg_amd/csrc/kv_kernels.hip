@@ -1,0 +1,386 @@
+// kv_kernels.hip - key-value execute / check / write-back kernels for gfx950.
+//
+// Replaces KERNEL_EXECUTE / KERNEL_CHECK / KERNEL_WRITEBACK of
+// LTPG/YCSB/0/ycsb.cu:9-58 and the per-op bodies of YCSB/0/Execute.h:11-252
+// and Genericfunction.h:31-158.
+//
+// Mapping.  The batch is slot-major (slot s of transaction t at [s*B + t],
+// the reference's LOC[B*op + NO]).  One lane owns one transaction and walks
+// its slots, so every slot-level load of a wave is 64 consecutive elements
+// (coalesced), conflict counts accumulate in registers (no per-op atomicAdd
+// on raw/war/waw[NO]), and the commit decision is fused into the check: a
+// wave64 ballot of the commit predicate is exactly one 64-bit word of the
+// commit bitmap.
+//
+// Registers.  regs[2*row] is R and regs[2*row+1] is W, each a u64
+// (~epoch << 32 | TID).  A newer epoch has a smaller tag, so one atomicMin
+// both registers a TID and evicts a stale value; a register whose tag is not
+// the current one reads as 0xFFFFFFFF (the reference's cleared value,
+// Database.h:45-50).  R and W of a row share one 16-byte line segment, so a
+// write op's check is a single 16-byte load.
+//
+// Field copies.  A field is field_words u32 words inside a row-major record.
+// A wave moves the fields of its (up to) 64 ops cooperatively: LP lanes per
+// op (LP = next power of two >= field_words, at most 64), so a 100-byte field
+// is one contiguous run of 25 lanes instead of 25 scattered loads per lane.
+// The op's addresses come from its owner lane by __shfl.
+#include "ltpg_internal.h"
+
+namespace ltpg {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t reg_tid(unsigned long long v, uint32_t inv_epoch)
+{
+    return (uint32_t)(v >> 32) == inv_epoch ? (uint32_t)v : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Copy (or read-modify-write) the fields of the active ops of a wave.
+// Lane j's op: src_base[src_off + w] (+ delta) -> dst_base[dst_off + w] for
+// w < fw.  Requires fw <= LP.  Ops of one call never alias each other
+// (different transactions of one wave never write one row in the same
+// slot), so the loads of U steps are issued before their stores.  A lane
+// always handles the same word positions of the same transaction, so
+// successive calls on one row by one transaction are ordered.
+template <int LP, int U>
+__device__ __forceinline__ void wave_copy(unsigned long long mask, uint64_t src_off,
+                                          uint64_t dst_off, uint32_t delta,
+                                          const uint32_t *src_base, uint32_t *dst_base,
+                                          uint32_t fw)
+{
+    constexpr int PER = 64 / LP;  // ops per step
+    const int lane = lane_id();
+    const int sub = lane / LP;
+    const uint32_t w0 = (uint32_t)(lane % LP);
+    const bool word_ok = w0 < fw;
+#pragma unroll 1
+    for (int s0 = 0; s0 < LP; s0 += U) {
+        uint32_t v[U];
+        uint64_t d[U];
+        bool a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = (s0 + u) * PER + sub;
+            const uint64_t so = __shfl(src_off, j);
+            d[u] = __shfl(dst_off, j);
+            const uint32_t dl = __shfl(delta, j);
+            a[u] = word_ok && ((mask >> j) & 1ull);
+            v[u] = a[u] ? src_base[so + w0] + dl : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (a[u]) dst_base[d[u] + w0] = v[u];
+    }
+}
+
+// Fields wider than 64 words: one op at a time, 64 words per instruction.
+__device__ __forceinline__ void wave_copy_wide(unsigned long long mask, uint64_t src_off,
+                                               uint64_t dst_off, uint32_t delta,
+                                               const uint32_t *src_base, uint32_t *dst_base,
+                                               uint32_t fw)
+{
+    const int lane = lane_id();
+    while (mask) {
+        const int j = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+        const uint64_t so = __shfl(src_off, j);
+        const uint64_t d = __shfl(dst_off, j);
+        const uint32_t dl = __shfl(delta, j);
+        for (uint32_t w = (uint32_t)lane; w < fw; w += 64) dst_base[d + w] = src_base[so + w] + dl;
+    }
+}
+
+template <int LP>
+__device__ __forceinline__ void copy_fields(unsigned long long mask, uint64_t src_off,
+                                            uint64_t dst_off, uint32_t delta,
+                                            const uint32_t *src_base, uint32_t *dst_base,
+                                            uint32_t fw)
+{
+    if constexpr (LP == 0)
+        wave_copy_wide(mask, src_off, dst_off, delta, src_base, dst_base, fw);
+    else
+        wave_copy<LP, (LP < 8 ? LP : 8)>(mask, src_off, dst_off, delta, src_base, dst_base, fw);
+}
+
+// Zero the per-epoch commit counts (a kernel rather than a memset so the
+// epoch can be captured in a graph).
+__global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
+{
+    if (threadIdx.x < 3) ctr->commits[threadIdx.x] = 0;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Reference initial contents: every word of field c holds c
+// (YCSB/0/Database.h:77-86 writes Y_c = c into column c).
+__global__ void kv_fill_reference_kernel(uint32_t *table, uint64_t n_words, uint32_t fields,
+                                         uint32_t fw)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += stride)
+        table[i] = (uint32_t)((i / fw) % fields);
+}
+
+// Column-major <-> row-major for u32 tables (rows x cols).
+__global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                 uint64_t rows, uint32_t cols, int to_row_major)
+{
+    const uint64_t n = rows * cols;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        // i indexes the destination
+        if (to_row_major) {
+            const uint64_t r = i / cols, c = i % cols;
+            dst[i] = src[c * rows + r];
+        } else {
+            const uint64_t c = i / rows, r = i % rows;
+            dst[i] = src[r * cols + c];
+        }
+    }
+}
+
+// EXECUTE: register every valid op's TID on R (read) or W (write) and stage
+// read fields from the snapshot (read_0 / read_1 / write_0,
+// YCSB/0/Execute.h:11-53; __REGISTER / __READ, Genericfunction.h:31-100).
+// A write is a read-modify-write applied at write-back, so it stages nothing.
+template <int LP>
+__global__ __launch_bounds__(kBlock) void kv_execute_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    uint32_t fw, const uint32_t *__restrict__ table, unsigned long long *__restrict__ regs,
+    uint32_t inv_epoch, uint32_t *__restrict__ read_values, DeviceCounters *__restrict__ ctr)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = t < B;
+    const unsigned long long tagged =
+        live ? (((unsigned long long)inv_epoch << 32) | tids[t]) : 0ull;
+    uint32_t bad = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint64_t i = (uint64_t)s * B + t;
+        uint32_t m = 0;
+        uint64_t row = 0;
+        if (live) {
+            m = meta[i];
+            row = keys[i];
+        }
+        const uint32_t kind = m & 3u;
+        const uint32_t field = (m >> 8) & 0xffffu;
+        const bool is_op = kind == LTPG_KV_READ || kind == LTPG_KV_WRITE;
+        const bool valid = is_op && row < rows && field < fields;
+        bad += (is_op && !valid) ? 1u : 0u;
+        if (valid) atomicMin(&regs[2 * row + (kind == LTPG_KV_WRITE ? 1 : 0)], tagged);
+        const unsigned long long rd = __ballot(valid && kind == LTPG_KV_READ);
+        if (rd) {
+            const uint64_t src = (row * fields + field) * fw;
+            copy_fields<LP>(rd, src, i * fw, 0u, table, read_values, fw);
+        }
+    }
+    if (bad) atomicAdd(&ctr->range_errors, bad);
+}
+
+// CHECK + COMMIT: per op read back the opposite register(s) and count
+// conflicts when TID > registered (check_r / check_w_0 / check_w_1 and
+// __CHECK, YCSB/0/Execute.h:109-175, Genericfunction.h:101-144); commit iff
+// waw == 0 && (raw == 0 || war == 0) (writeback_0, Execute.h:184-191).
+__global__ __launch_bounds__(kBlock) void kv_check_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    const unsigned long long *__restrict__ regs, uint32_t inv_epoch,
+    uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts,
+    DeviceCounters *__restrict__ ctr)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = lane_id();
+    const bool live = t < B;
+    const uint32_t tid = live ? tids[t] : 0u;
+    uint32_t raw = 0, war = 0, waw = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint64_t i = (uint64_t)s * B + t;
+        uint32_t m = 0;
+        uint64_t row = 0;
+        if (live) {
+            m = meta[i];
+            row = keys[i];
+        }
+        const uint32_t kind = m & 3u;
+        const uint32_t field = (m >> 8) & 0xffffu;
+        const bool valid =
+            (kind == LTPG_KV_READ || kind == LTPG_KV_WRITE) && row < rows && field < fields;
+        if (valid) {
+            if (kind == LTPG_KV_READ) {
+                raw += tid > reg_tid(regs[2 * row + 1], inv_epoch) ? 1u : 0u;
+            } else {
+                const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * row);
+                war += tid > reg_tid(rw.x, inv_epoch) ? 1u : 0u;
+                waw += tid > reg_tid(rw.y, inv_epoch) ? 1u : 0u;
+            }
+        }
+    }
+    const bool commit = live && waw == 0 && (raw == 0 || war == 0);
+    const unsigned long long word = __ballot(commit);
+    const uint32_t wave_base = t - (uint32_t)lane;
+    if (lane == 0 && wave_base < B) {
+        bitmap[wave_base / 64] = word;
+        const uint32_t n = (uint32_t)__popcll(word);
+        const uint32_t live_n = (B - wave_base) < 64u ? (B - wave_base) : 64u;
+        if (n) {
+            atomicAdd(&ctr->commits[LTPG_TYPE_KV], n);
+            atomicAdd(&ctr->commits_total[LTPG_TYPE_KV], (unsigned long long)n);
+        }
+        atomicAdd(&ctr->attempts_total[LTPG_TYPE_KV], (unsigned long long)live_n);
+    }
+    if (live) conflicts[t] = raw | (war << 8) | (waw << 16);
+}
+
+// WRITEBACK: committed transactions apply their write slots in slot order
+// (writeback_w / __WRITEBACK, YCSB/0/Execute.h:193-206,
+// Genericfunction.h:145-158).  Value: word + delta, delta = TID | 1 for RMW
+// and 0 for copy-back.  The committed writer of a row is unique (the min-TID
+// W registrant), so reading the table here reads the snapshot.
+template <int LP>
+__global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    uint32_t fw, uint32_t *table, const uint64_t *__restrict__ bitmap, uint32_t values)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = lane_id();
+    const uint32_t wave_base = t - (uint32_t)lane;
+    if (wave_base >= B) return;
+    const uint64_t word = bitmap[wave_base / 64];
+    if (word == 0) return;
+    // t < B guards a bitmap word left by an earlier, larger batch when this
+    // phase runs without a fresh CHECK.
+    const bool committed = t < B && ((word >> lane) & 1ull);
+    const uint32_t delta = (committed && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint64_t i = (uint64_t)s * B + t;
+        uint32_t m = 0;
+        uint64_t row = 0;
+        if (committed) {
+            m = meta[i];
+            row = keys[i];
+        }
+        const uint32_t field = (m >> 8) & 0xffffu;
+        const bool wr = committed && (m & 3u) == LTPG_KV_WRITE && row < rows && field < fields;
+        const unsigned long long mask = __ballot(wr);
+        if (mask) {
+            const uint64_t off = (row * fields + field) * fw;
+            copy_fields<LP>(mask, off, off, delta, table, table, fw);
+        }
+    }
+}
+
+template <template <int> class K>
+struct Dispatch;
+
+int lanes_for(uint32_t fw)
+{
+    if (fw > 64) return 0;
+    int lp = 1;
+    while ((uint32_t)lp < fw) lp <<= 1;
+    return lp;
+}
+
+inline dim3 grid_for(uint64_t n_threads)
+{
+    return dim3((unsigned)((n_threads + kBlock - 1) / kBlock));
+}
+
+inline dim3 grid_stride(uint64_t n)
+{
+    uint64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return dim3((unsigned)g);
+}
+
+#define LTPG_LP_SWITCH(lp, CALL) \
+    switch (lp) {                \
+    case 1: CALL(1); break;      \
+    case 2: CALL(2); break;      \
+    case 4: CALL(4); break;      \
+    case 8: CALL(8); break;      \
+    case 16: CALL(16); break;    \
+    case 32: CALL(32); break;    \
+    case 64: CALL(64); break;    \
+    default: CALL(0); break;     \
+    }
+
+}  // namespace
+
+hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(64), 0, s, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_u64_kernel, grid_stride(n), dim3(kBlock), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_fill_reference(uint32_t *table, uint64_t rows, uint32_t fields,
+                                    uint32_t field_words, hipStream_t s)
+{
+    const uint64_t n = rows * fields * field_words;
+    hipLaunchKernelGGL(kv_fill_reference_kernel, grid_stride(n), dim3(kBlock), 0, s, table, n,
+                       fields, field_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
+                               int to_row_major, hipStream_t s)
+{
+    hipLaunchKernelGGL(transpose_kernel, grid_stride(rows * cols), dim3(kBlock), 0, s, src, dst,
+                       rows, cols, to_row_major);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+                             DeviceCounters *ctr, hipStream_t s)
+{
+    if (b.n_txns == 0) return hipSuccess;
+    const int lp = lanes_for(kv.field_words);
+#define CALL(LPV)                                                                         \
+    hipLaunchKernelGGL(kv_execute_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,   \
+                       b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,  \
+                       kv.field_words, kv.table, kv.regs, inv_epoch, kv.read_values, ctr)
+    LTPG_LP_SWITCH(lp, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+                           DeviceCounters *ctr, hipStream_t s)
+{
+    if (b.n_txns == 0) return hipSuccess;
+    hipLaunchKernelGGL(kv_check_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
+                       b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, inv_epoch,
+                       kv.bitmap, kv.conflicts, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
+                               hipStream_t s)
+{
+    if (b.n_txns == 0) return hipSuccess;
+    const int lp = lanes_for(kv.field_words);
+#define CALL(LPV)                                                                           \
+    hipLaunchKernelGGL(kv_writeback_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,   \
+                       b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,    \
+                       kv.field_words, kv.table, kv.bitmap, values)
+    LTPG_LP_SWITCH(lp, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+}  // namespace ltpg

@@ -1,0 +1,101 @@
+// ltpg_internal.h - context state and helpers shared by the host side of
+// libltpg.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "ltpg.h"
+
+namespace ltpg {
+
+// Thread-local message for ltpg_last_error().
+void set_error(const std::string &msg);
+
+#define LTPG_HIP_TRY(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::ltpg::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));             \
+            return e_ == hipErrorOutOfMemory ? LTPG_ERR_NOMEM : LTPG_ERR_DEVICE;              \
+        }                                                                                     \
+    } while (0)
+
+#define LTPG_FAIL(code, msg)                                                                 \
+    do {                                                                                      \
+        ::ltpg::set_error(msg);                                                               \
+        return (code);                                                                        \
+    } while (0)
+
+// Device-resident counters written by kernels.
+struct DeviceCounters {
+    uint32_t range_errors;                  // ops whose row / field is outside its table
+    uint32_t commits[3];                    // per LTPG_TYPE_*, last epoch
+    unsigned long long commits_total[3];    // per LTPG_TYPE_*, since the last reset
+    unsigned long long attempts_total[3];   // transactions checked since the last reset
+};
+
+// One key-value batch as the kernels see it (device pointers).
+struct KvBatchDev {
+    uint32_t n_txns = 0;
+    uint32_t n_slots = 0;
+    const uint64_t *keys = nullptr;
+    const uint32_t *meta = nullptr;
+    const uint32_t *tids = nullptr;
+};
+
+struct KvState {
+    uint64_t rows = 0;
+    uint32_t fields = 0;
+    uint32_t field_words = 0;
+    uint32_t max_slots = 0;
+    uint32_t *table = nullptr;          // rows * fields * field_words, row-major
+    unsigned long long *regs = nullptr; // 2 * rows: [R, W] per row, epoch-tagged
+    uint32_t *read_values = nullptr;    // max_slots * max_txns * field_words
+    uint64_t *bitmap = nullptr;         // ceil(max_txns / 64)
+    uint32_t *conflicts = nullptr;      // max_txns
+    // library-owned copy of host-submitted batches
+    uint64_t *keys_buf = nullptr;
+    uint32_t *meta_buf = nullptr;
+    uint32_t *tids_buf = nullptr;
+    KvBatchDev cur;
+    bool has_batch = false;
+};
+
+}  // namespace ltpg
+
+struct ltpg_ctx {
+    ltpg_config cfg;
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t epoch = 1;  // register tag is ~epoch; 0 is the tag of the fill value
+    ltpg::DeviceCounters *counters = nullptr;
+    ltpg::KvState kv;
+    // Per-phase timing: each timed epoch takes 5 events from the pool
+    // (start, after CID, after EXECUTE, after CHECK, after WRITEBACK).
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    uint32_t ev_used = 0;
+};
+
+namespace ltpg {
+
+// Kernel launchers (kv_kernels.hip).  All asynchronous on `s`.
+hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_kv_fill_reference(uint32_t *table, uint64_t rows, uint32_t fields,
+                                    uint32_t field_words, hipStream_t s);
+hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
+                               int to_row_major, hipStream_t s);
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+                             DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+                           DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
+                               hipStream_t s);
+
+}  // namespace ltpg

@@ -1,0 +1,156 @@
+/*
+ * ltpg_oracle_kv.c - key-value rules, restated on the CPU.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; never linked into the product).
+ *
+ * Restates LTPG/YCSB/0 (identical to YCSB/1): Execute.h:11-271 and
+ * Genericfunction.h:31-158.  Per op slot s of transaction t
+ * (slot-major index s*B + t, as the kernels index LOC_R[YCSB_A_SIZE*OP + NO],
+ * Execute.h:20,47):
+ *   read  : register R (read_0 -> __REGISTER, Execute.h:25;
+ *           Genericfunction.h:71-85), stage the field (read_1 -> __READ,
+ *           Execute.h:36; Genericfunction.h:31-45), check W -> raw
+ *           (check_r, Execute.h:109-122; __CHECK, Genericfunction.h:101-144)
+ *   write : register W (write_0, Execute.h:38-53), check R -> war and
+ *           W -> waw (check_w_0 / check_w_1, Execute.h:123-150)
+ *   commit: waw == 0 && (raw == 0 || war == 0)  (writeback_0, Execute.h:
+ *           184-191)
+ *   write-back of committed writes (writeback_w, Execute.h:193-206).
+ *
+ * Build-defined where the reference is degenerate (its writes store the
+ * value they staged, so the table never changes):
+ *   values == 0 (copy-back): write-back stores the snapshot value: no change.
+ *   values == 1 (RMW): each u32 word of the written field becomes
+ *     word + (TID | 1), applied per write op in slot order.
+ * A slot whose meta kind is NONE does nothing.  A key >= rows is counted in
+ * range_errors and the op does nothing.
+ */
+#include "ltpg_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#define KIND(m) ((m) & 3u)
+#define FIELD(m) (((m) >> 8) & 0xffffu)
+#define K_READ 1u
+#define K_WRITE 2u
+
+/* An op takes part in the epoch iff its kind is READ or WRITE and its row
+ * and field exist; anything else is skipped by every phase (and counted as a
+ * range error when the kind is READ or WRITE). */
+static int op_valid(uint32_t m, uint64_t row, uint64_t rows, uint32_t fields)
+{
+    uint32_t k = KIND(m);
+    return (k == K_READ || k == K_WRITE) && row < rows && FIELD(m) < fields;
+}
+
+/* __REGISTER: atomicMin(&log->YCSB_{R,W}[Loc], TID) -> a plain minimum. */
+void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
+                        const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
+                        uint32_t *reg_r, uint32_t *reg_w)
+{
+    for (uint32_t s = 0; s < S; s++) {
+        for (uint32_t t = 0; t < B; t++) {
+            uint64_t i = (uint64_t)s * B + t;
+            uint64_t row = keys[i];
+            if (!op_valid(meta[i], row, rows, fields)) continue;
+            uint32_t *reg = (KIND(meta[i]) == K_READ) ? reg_r : reg_w;
+            if (tids[t] < reg[row]) reg[row] = tids[t];
+        }
+    }
+}
+
+/* __CHECK: existed = register; if (TID > existed) mark[NO]++ (Generic-
+ * function.h:140-143).  Reads consult W (raw); writes consult R (war) and
+ * W (waw). */
+void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
+                     const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
+                     const uint32_t *reg_r, const uint32_t *reg_w, uint32_t *conflicts)
+{
+    for (uint32_t t = 0; t < B; t++) {
+        uint32_t raw = 0, war = 0, waw = 0, tid = tids[t];
+        for (uint32_t s = 0; s < S; s++) {
+            uint64_t i = (uint64_t)s * B + t;
+            uint64_t row = keys[i];
+            if (!op_valid(meta[i], row, rows, fields)) continue;
+            if (KIND(meta[i]) == K_READ) {
+                if (tid > reg_w[row]) raw++;
+            } else {
+                if (tid > reg_r[row]) war++;
+                if (tid > reg_w[row]) waw++;
+            }
+        }
+        conflicts[t] = (raw & 0xffu) | ((war & 0xffu) << 8) | ((waw & 0xffu) << 16);
+    }
+}
+
+/* writeback_0 (Execute.h:184-191): commit iff waw == 0 && (raw == 0 ||
+ * war == 0); bit t of the bitmap, and the COMMIT count. */
+uint32_t oracle_commit(uint32_t B, const uint32_t *conflicts, uint64_t *bitmap)
+{
+    uint32_t count = 0;
+    memset(bitmap, 0, ((size_t)B + 63) / 64 * sizeof(uint64_t));
+    for (uint32_t t = 0; t < B; t++) {
+        uint32_t raw = conflicts[t] & 0xffu;
+        uint32_t war = (conflicts[t] >> 8) & 0xffu;
+        uint32_t waw = (conflicts[t] >> 16) & 0xffu;
+        if (waw == 0 && (raw == 0 || war == 0)) {
+            bitmap[t / 64] |= 1ull << (t % 64);
+            count++;
+        }
+    }
+    return count;
+}
+
+/* writeback_w / __WRITEBACK (Execute.h:193-206, Genericfunction.h:145-158):
+ * committed transactions write their write slots, in slot order. */
+void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                         uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                         const uint32_t *tids, const uint64_t *bitmap, uint32_t values)
+{
+    if (values == 0) return; /* copy-back stores the unchanged snapshot value */
+    for (uint32_t t = 0; t < B; t++) {
+        if (!((bitmap[t / 64] >> (t % 64)) & 1ull)) continue;
+        uint32_t delta = tids[t] | 1u;
+        for (uint32_t s = 0; s < S; s++) {
+            uint64_t i = (uint64_t)s * B + t;
+            uint32_t m = meta[i];
+            uint64_t row = keys[i];
+            if (KIND(m) != K_WRITE || !op_valid(m, row, rows, fields)) continue;
+            uint32_t *w = table + (row * fields + FIELD(m)) * (uint64_t)field_words;
+            for (uint32_t j = 0; j < field_words; j++) w[j] += delta;
+        }
+    }
+}
+
+int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                    uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                    const uint32_t *tids, uint32_t values, uint32_t *reg_scratch,
+                    oracle_kv_result *res)
+{
+    uint32_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
+    uint32_t errors = 0;
+    /* clear_LOG: every register back to 0xFFFFFFFF (YCSB/0/Database.h:45-50) */
+    memset(reg_scratch, 0xff, (size_t)rows * 2 * sizeof(uint32_t));
+
+    /* execute: register, then stage read fields from the snapshot */
+    oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w);
+    for (uint64_t i = 0; i < (uint64_t)S * B; i++) {
+        uint32_t k = KIND(meta[i]);
+        if (!op_valid(meta[i], keys[i], rows, fields)) {
+            if (k == K_READ || k == K_WRITE) errors++;
+            continue;
+        }
+        if (k == K_READ && res->read_values) {
+            const uint32_t *src = table + (keys[i] * fields + FIELD(meta[i])) * (uint64_t)field_words;
+            memcpy(res->read_values + i * field_words, src, field_words * sizeof(uint32_t));
+        }
+    }
+
+    oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts);
+    res->commit_count = oracle_commit(B, res->conflicts, res->commit_bitmap);
+    oracle_kv_writeback(rows, fields, field_words, table, B, S, keys, meta, tids,
+                        res->commit_bitmap, values);
+    res->range_errors = errors;
+    return 0;
+}

@@ -1,0 +1,171 @@
+"""Pins the key-value oracle (CPU only).
+
+The reference ships no tests, fixtures or golden vectors for this path, so
+the oracle is pinned by (1) hand-derived known-answer epochs
+(tests/golden/kv_kat.json), (2) an independent pairwise computation of the
+conflict counters, (3) Aria's serializability guarantee checked on the
+committed set with a serial replay that also fixes the final table, and
+(4) brute force over every TID permutation of tiny batches.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests import kvcases as kc
+
+KAT = kc.load_kat()
+
+
+@pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
+def test_oracle_known_answers(case):
+    rows, fields, fw = case["rows"], case["fields"], case["field_words"]
+    keys, meta, tids, S = kc.case_arrays(case)
+    values = 1 if case["values"] == "rmw" else 0
+    ep = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values)
+    B = len(tids)
+    exp = case["expect"]
+    assert kc.bitmap_to_list(ep.bitmap, B) == exp["commit"]
+    assert kc.unpack_conflicts(ep.conflicts).tolist() == exp["conflicts"]
+    assert ep.range_errors == exp["range_errors"]
+    assert ep.commit_count == sum(exp["commit"])
+    np.testing.assert_array_equal(ep.table, kc.expected_table(case))
+    for t, s, words in exp["reads"]:
+        i = s * B + t
+        assert ep.read_values[i * fw:(i + 1) * fw].tolist() == words
+
+
+def _run(rows, fields, fw, keys, meta, tids, S, values=1):
+    return oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_oracle_matches_pairwise_and_is_serializable(seed):
+    rng = np.random.default_rng(seed)
+    B = int(rng.integers(1, 40))
+    S = int(rng.integers(1, 6))
+    rows = int(rng.integers(1, 12))
+    fields, fw = 3, int(rng.integers(1, 4))
+    keys, meta, tids = kc.random_batch(rng, B, S, rows, fields)
+    ep = _run(rows, fields, fw, keys, meta, tids, S)
+    conf = kc.unpack_conflicts(ep.conflicts)
+    np.testing.assert_array_equal(conf, kc.pairwise_conflicts(keys, meta, tids, B, S, rows, fields))
+    committed = kc.bitmap_to_list(ep.bitmap, B)
+    assert committed == kc.aria_commit(conf).tolist()
+    ok, msg, final = kc.serial_check(kc.initial_table(rows, fields, fw), keys, meta, tids, B, S, rows,
+                                     fields, fw, committed)
+    assert ok, msg
+    np.testing.assert_array_equal(ep.table, final)
+    # reads return the snapshot
+    snap = kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)
+    for s in range(S):
+        for t in range(B):
+            m = int(meta[s * B + t])
+            if m & 3 == 1:
+                i = s * B + t
+                np.testing.assert_array_equal(ep.read_values[i * fw:(i + 1) * fw],
+                                              snap[int(keys[i]), (m >> 8) & 0xFFFF])
+
+
+def test_oracle_bruteforce_all_tid_permutations():
+    """Every TID order of small access patterns (<= 6 txns on <= 4 rows)."""
+    rng = np.random.default_rng(1234)
+    checked = 0
+    for pattern in range(12):
+        B = int(rng.integers(2, 7))
+        S = int(rng.integers(1, 4))
+        rows, fields, fw = int(rng.integers(1, 5)), 2, 1
+        keys, meta, _ = kc.random_batch(rng, B, S, rows, fields, p_empty=0.0)
+        for perm in itertools.permutations(range(1, B + 1)):
+            tids = np.array(perm, dtype=np.uint32) * 3
+            ep = _run(rows, fields, fw, keys, meta, tids, S)
+            conf = kc.unpack_conflicts(ep.conflicts)
+            np.testing.assert_array_equal(conf, kc.pairwise_conflicts(keys, meta, tids, B, S, rows, fields))
+            committed = kc.bitmap_to_list(ep.bitmap, B)
+            ok, msg, final = kc.serial_check(kc.initial_table(rows, fields, fw), keys, meta, tids, B, S,
+                                             rows, fields, fw, committed)
+            assert ok, msg
+            np.testing.assert_array_equal(ep.table, final)
+            # the lowest TID always commits
+            assert committed[int(np.argmin(tids))] == 1
+            checked += 1
+    assert checked > 1000
+
+
+def test_oracle_copyback_never_changes_table():
+    rng = np.random.default_rng(7)
+    rows, fields, fw, B, S = 16, 4, 3, 64, 6
+    keys, meta, tids = kc.random_batch(rng, B, S, rows, fields)
+    ep = _run(rows, fields, fw, keys, meta, tids, S, values=0)
+    np.testing.assert_array_equal(ep.table, kc.initial_table(rows, fields, fw))
+    ep1 = _run(rows, fields, fw, keys, meta, tids, S, values=1)
+    np.testing.assert_array_equal(ep.bitmap, ep1.bitmap)
+
+
+def _rules_with_mistake(keys, meta, tids, B, S, rows, fields, fw, table0, mistake):
+    """A CPU copy of the rules with one planted mistake (test-only)."""
+    ops = kc.ops_of(keys, meta, B, S, rows, fields)
+    R, W = {}, {}
+    for t in range(B):
+        for _, kind, row, _ in ops[t]:
+            reg = R if (kind == 1) != (mistake == "swap_registers") else W
+            reg[row] = min(reg.get(row, 0xFFFFFFFF), int(tids[t]))
+    conf = np.zeros((B, 3), dtype=np.uint32)
+    gt = (lambda a, b: a >= b) if mistake == "ge" else (lambda a, b: a > b)
+    for t in range(B):
+        me = int(tids[t])
+        for _, kind, row, _ in ops[t]:
+            if kind == 1:
+                conf[t, 0] += gt(me, W.get(row, 0xFFFFFFFF))
+            else:
+                conf[t, 1] += gt(me, (W if mistake == "war_from_w" else R).get(row, 0xFFFFFFFF))
+                conf[t, 2] += gt(me, W.get(row, 0xFFFFFFFF))
+    raw, war, waw = conf[:, 0], conf[:, 1], conf[:, 2]
+    if mistake == "waw_only":
+        commit = waw == 0
+    elif mistake == "or_rule":
+        commit = (waw == 0) | (raw == 0) | (war == 0)
+    else:
+        commit = (waw == 0) & ((raw == 0) | (war == 0))
+    tab = np.array(table0, dtype=np.uint32).reshape(rows, fields, fw).copy()
+    for t in range(B):
+        if commit[t]:
+            seen = set()
+            for _, kind, row, field in ops[t]:
+                if kind == 2:
+                    if mistake == "apply_once" and (row, field) in seen:
+                        continue
+                    seen.add((row, field))
+                    tab[row, field] += np.uint32(int(tids[t]) | 1)
+    return commit.astype(int).tolist(), conf, tab.reshape(-1)
+
+
+@pytest.mark.parametrize("mistake", ["swap_registers", "ge", "war_from_w", "waw_only", "or_rule", "apply_once"])
+def test_pins_catch_planted_mistakes(mistake):
+    """Each plausible mistake fails a known answer or the serializability
+    replay, so the pins above would catch it in the oracle."""
+    caught = False
+    for case in KAT["cases"]:
+        if case["values"] != "rmw" or case["expect"]["range_errors"]:
+            continue
+        rows, fields, fw = case["rows"], case["fields"], case["field_words"]
+        keys, meta, tids, S = kc.case_arrays(case)
+        commit, conf, tab = _rules_with_mistake(keys, meta, tids, len(tids), S, rows, fields, fw,
+                                                kc.initial_table(rows, fields, fw), mistake)
+        if (commit != case["expect"]["commit"] or conf.tolist() != case["expect"]["conflicts"]
+                or not np.array_equal(tab, kc.expected_table(case))):
+            caught = True
+            break
+    rng = np.random.default_rng(5)
+    for _ in range(300 if not caught else 0):
+        rows, fields, fw, B, S = 5, 2, 1, 12, 3
+        keys, meta, tids = kc.random_batch(rng, B, S, rows, fields, p_empty=0.0)
+        commit, _, tab = _rules_with_mistake(keys, meta, tids, B, S, rows, fields, fw,
+                                             kc.initial_table(rows, fields, fw), mistake)
+        ok, _, final = kc.serial_check(kc.initial_table(rows, fields, fw), keys, meta, tids, B, S, rows,
+                                       fields, fw, commit)
+        if not ok or not np.array_equal(final, tab):
+            caught = True
+            break
+    assert caught, f"planted mistake {mistake!r} passed every pin"
