@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 METRIC = "committed txns/sec at 1 GPU + abort rate; p99 commit latency vs plain form"
 PHASES = ["cid", "execute", "check", "writeback"]
+T_START = time.perf_counter()
+
+
+def log(msg):
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def kv_workload(name, theta):
@@ -90,12 +95,14 @@ def run_kv(args):
     w = kv_workload(args.workload, args.theta)
     rows, fields, fw, B, S = w["rows"], w["fields"], w["fw"], w["B"], w["S"]
     P = args.batches
+    log(f"generating {P} batches: {w['desc']}")
     gen = [L.kv_generate(seed=0x4C545047 + i, rows=rows, fields=fields, n_txns=B, n_slots=S,
                          read_pct=w["read_pct"], theta=w["theta"], tid_base=0) for i in range(P)]
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                     max_txns=B, kv_max_slots=S, values=L.VALUES_RMW)
     ctx.kv_fill_reference()
     dev = [L.DeviceKvBatch(ctx, k, m, t, S) for (k, m, t) in gen]
+    log("batches resident in HBM")
 
     # one pass per batch: commit sets depend only on the batch, which fixes
     # the committed-write bytes of every later run of it
@@ -111,6 +118,7 @@ def run_kv(args):
         ctx.epoch_run()
     ctx.sync()
 
+    log(f"warmup done; timing {args.steps} steps")
     ctx.commit_totals(reset=True)
     ctx.set_timing(True)
     ctx.phase_times()  # start a fresh measurement
@@ -124,6 +132,7 @@ def run_kv(args):
     ctx.set_timing(False)
     commits, attempts = ctx.commit_totals()
 
+    log(f"timed region: {elapsed:.3f} s; latency loop")
     # reference-style synchronous tp loop: submit, phases, commit readback
     lat = []
     for k in range(max(5, args.steps // 2)):
@@ -171,6 +180,7 @@ def run_kv(args):
         d.free()
     ctx.close()
     if not args.no_cpu_baseline:
+        log("cpu baseline (oracle)")
         k0, m0, t0_ = gen[0]
         result["cpu_baseline"] = cpu_baseline_kv(w, k0, m0, t0_)
     return result
