@@ -36,6 +36,7 @@ def log(msg):
 
 def kv_workload(name, theta):
     if name == "kv-c2":
+        theta = 0.6 if theta is None else theta
         return dict(rows=10_000_000, fields=10, fw=25, B=1 << 20, S=16, theta=theta, read_pct=50,
                     desc=f"key-value plain form: 10M rows x 10 fields x 100 B, 2^20 txns x 16 ops, "
                          f"50/50 read/RMW, Zipf {theta}")
@@ -63,36 +64,40 @@ def kv_phase_bytes(keys, meta, bitmap, B, S, fw):
     return {"execute": execute, "check": check, "writeback": writeback}, committed
 
 
-def cpu_baseline_kv(w, keys, meta, tids, budget_s=12.0):
-    """Time the oracle (single thread) on a bounded sample: the first n
-    transactions of the batch over the full-size table."""
+def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
+    """Time the oracle (single-threaded C replay of the same rules, read
+    staging included) on a bounded sample of the same workload: prefixes of
+    the batch over the full-size table, growing until about budget_s of CPU
+    work (then whole-batch epochs repeat until the budget is spent)."""
     from oracle import oracle
 
     rows, fields, fw, B, S = w["rows"], w["fields"], w["fw"], w["B"], w["S"]
     table = np.empty(rows * fields * fw, dtype=np.uint32)
     table.reshape(rows, fields, fw)[:] = np.arange(fields, dtype=np.uint32)[None, :, None]
     n = min(B, 1 << 14)
-    total_s, total_commits, total_txns = 0.0, 0, 0
-    while True:
+    total_s, total_commits, total_txns, epochs = 0.0, 0, 0, 0
+    while total_s < budget_s:
         k = keys.reshape(S, B)[:, :n].copy()
         m = meta.reshape(S, B)[:, :n].copy()
         t = tids[:n].copy()
         t0 = time.perf_counter()
-        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, t, S, values=1, want_reads=False, inplace=True)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, t, S, values=1, want_reads=True, inplace=True)
         dt = time.perf_counter() - t0
-        total_s, total_commits, total_txns = total_s + dt, total_commits + ep.commit_count, total_txns + n
-        if total_s > budget_s or n >= B:
-            break
-        n = min(B, int(n * max(2.0, min(8.0, (budget_s - total_s) / max(dt, 1e-3)))))
+        total_s, total_commits, total_txns, epochs = (total_s + dt, total_commits + ep.commit_count,
+                                                      total_txns + n, epochs + 1)
+        if n < B:
+            n = min(B, int(n * max(2.0, min(8.0, (budget_s - total_s) / max(dt, 1e-3)))))
     return {"value": total_commits / total_s, "unit": "txn/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (single-threaded C replay) on prefixes of the batch totalling {total_txns} txns "
-                      f"x {S} ops over the full {rows}-row table, {total_s:.1f} s"}
+            "attempted_txn_per_s": total_txns / total_s,
+            "sample": f"oracle (single-threaded C replay, read staging included) over the full {rows}-row "
+                      f"table: {epochs} epochs on prefixes of the batch, {total_txns} txns x {S} ops in "
+                      f"{total_s:.1f} s"}
 
 
-def run_kv(args):
+def run_kv(args, theta, cpu_baseline=True):
     import ltpg_amd as L
 
-    w = kv_workload(args.workload, args.theta)
+    w = kv_workload(args.workload, theta)
     rows, fields, fw, B, S = w["rows"], w["fields"], w["fw"], w["B"], w["S"]
     P = args.batches
     log(f"generating {P} batches: {w['desc']}")
@@ -179,7 +184,7 @@ def run_kv(args):
     for d in dev:
         d.free()
     ctx.close()
-    if not args.no_cpu_baseline:
+    if cpu_baseline and not args.no_cpu_baseline:
         log("cpu baseline (oracle)")
         k0, m0, t0_ = gen[0]
         result["cpu_baseline"] = cpu_baseline_kv(w, k0, m0, t0_)
@@ -197,15 +202,38 @@ def measured_traffic(phase):
         return None
 
 
+def copy_calibration(gib):
+    """Copy `gib` GiB device -> device five times with the library's copy
+    kernel: the known-size traffic that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import ltpg_amd as L
+
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=64, kv_fields=1, kv_field_bytes=4, max_txns=64,
+                    kv_max_slots=1)
+    n = int(gib * (1 << 30))
+    a, b = L.DeviceBuffer(ctx, n), L.DeviceBuffer(ctx, n)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b.copy_from(a)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 5, "gbs": 5 * 2 * n / dt / 1e9}))
+    a.free()
+    b.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="kv-c2")
-    ap.add_argument("--theta", type=float, default=0.99)
+    ap.add_argument("--theta", type=float, default=None,
+                    help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
+                    help="only run the known-size copy used to calibrate traffic counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
@@ -213,7 +241,19 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         raise SystemExit("--steps must be >= 1")
-    result = run_kv(args)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    thetas = [args.theta] if args.theta is not None else ([0.6, 0.99] if args.workload == "kv-c2" else [None])
+    result = run_kv(args, thetas[0])
+    variants = []
+    for th in thetas[1:]:
+        v = run_kv(args, th, cpu_baseline=False)
+        variants.append({k: v[k] for k in ("value", "unit", "ms_per_step", "abort_rate", "latency_ms",
+                                           "phase_ms", "phase_algorithmic_gbs", "roofline")}
+                        | {"config": v["config"]})
+    if variants:
+        result["variants"] = variants
     print(json.dumps(result))
 
 

@@ -179,6 +179,11 @@ int ltpg_device_free(ltpg_ctx *ctx, void *dptr);
 /* Synchronous host -> device / device -> host copies on the context stream. */
 int ltpg_copy_to_device(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* Device -> device copy by a streaming kernel on the context stream
+ * (asynchronous; bytes must be a multiple of 16 and both pointers 16-byte
+ * aligned).  Used to snapshot tables and as the known-size copy that
+ * calibrates HBM traffic counters. */
+int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 /* ---- key-value tables (YCSB/0/Database.h:14-131) ------------------------ */
 /* Load rows x kv_fields x kv_field_bytes.  COLUMN_MAJOR requires

@@ -94,6 +94,12 @@ class DeviceBuffer:
         _check(lib.ltpg_copy_to_host(self._ctx.handle, _ptr(arr), ctypes.c_void_p(self.ptr), arr.nbytes),
                "ltpg_copy_to_host")
 
+    def copy_from(self, other, nbytes=None):
+        """Device -> device copy (asynchronous on the context stream)."""
+        n = min(self.nbytes, other.nbytes) if nbytes is None else int(nbytes)
+        _check(lib.ltpg_device_copy(self._ctx.handle, ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), n),
+               "ltpg_device_copy")
+
     def free(self):
         if self.ptr:
             _check(lib.ltpg_device_free(self._ctx.handle, ctypes.c_void_p(self.ptr)), "ltpg_device_free")

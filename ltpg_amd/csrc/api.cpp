@@ -345,6 +345,17 @@ int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     return LTPG_OK;
 }
 
+int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dst || !src) LTPG_FAIL(LTPG_ERR_INVALID, "null pointer");
+    if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+        LTPG_FAIL(LTPG_ERR_INVALID, "device copy needs 16-byte multiples and alignment");
+    if (bytes) LTPG_HIP_TRY(launch_copy_u128(dst, src, bytes / 16, ctx->stream));
+    return LTPG_OK;
+}
+
 // ---- key-value tables ---------------------------------------------------
 
 int ltpg_kv_table_load(ltpg_ctx *ctx, const void *host, uint64_t rows, uint32_t layout)

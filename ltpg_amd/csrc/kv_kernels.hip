@@ -38,6 +38,17 @@ __device__ __forceinline__ uint32_t reg_tid(unsigned long long v, uint32_t inv_e
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// atomicMin that skips the atomic when the register already holds a value
+// <= v.  A register only ever decreases (atomicMin within an epoch, and a
+// newer epoch's tag is smaller than every older one), so any value a plain
+// load returns -- even a stale cached one -- is >= the register's current
+// value: if it is <= v the atomic could not change anything.  On hot rows
+// this turns a serialized chain of same-address atomics into cached loads.
+__device__ __forceinline__ void register_min(unsigned long long *reg, unsigned long long v)
+{
+    if (__atomic_load_n(reg, __ATOMIC_RELAXED) > v) atomicMin(reg, v);
+}
+
 // Copy (or read-modify-write) the fields of the active ops of a wave.
 // Lane j's op: src_base[src_off + w] (+ delta) -> dst_base[dst_off + w] for
 // w < fw.  Requires fw <= LP.  Ops of one call never alias each other
@@ -112,6 +123,13 @@ __global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
     if (threadIdx.x < 3) ctr->commits[threadIdx.x] = 0;
 }
 
+// Streaming 16-byte copy, grid-stride.
+__global__ void copy_u128_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long long v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -175,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
         const bool is_op = kind == LTPG_KV_READ || kind == LTPG_KV_WRITE;
         const bool valid = is_op && row < rows && field < fields;
         bad += (is_op && !valid) ? 1u : 0u;
-        if (valid) atomicMin(&regs[2 * row + (kind == LTPG_KV_WRITE ? 1 : 0)], tagged);
+        if (valid) register_min(&regs[2 * row + (kind == LTPG_KV_WRITE ? 1 : 0)], tagged);
         const unsigned long long rd = __ballot(valid && kind == LTPG_KV_READ);
         if (rd) {
             const uint64_t src = (row * fields + field) * fw;
@@ -319,6 +337,16 @@ inline dim3 grid_stride(uint64_t n)
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s)
 {
     hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(64), 0, s, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s)
+{
+    uint64_t g = (n16 + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(copy_u128_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, static_cast<uint4 *>(dst),
+                       static_cast<const uint4 *>(src), n16);
     return hipGetLastError();
 }
 

@@ -86,6 +86,7 @@ namespace ltpg {
 
 // Kernel launchers (kv_kernels.hip).  All asynchronous on `s`.
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_kv_fill_reference(uint32_t *table, uint64_t rows, uint32_t fields,
                                     uint32_t field_words, hipStream_t s);
