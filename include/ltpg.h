@@ -30,9 +30,10 @@
  *     LTPG_MEM_HOST are copied to the device before the call returns;
  *     batches given with LTPG_MEM_DEVICE are read in place and must stay
  *     valid until ltpg_sync() returns.
- *   - TIDs are u32 priorities; a lower TID wins.  TIDs must be distinct
- *     within one batch (the reference's TID = (epoch & 0xfff) << 20 | seq,
- *     TPCC-allopt-final/Query.h:500, satisfies this).
+ *   - TIDs are u32 priorities; a lower TID wins.  The reference relies on
+ *     TIDs being distinct within a batch (TID = (epoch & 0xfff) << 20 | seq,
+ *     TPCC-allopt-final/Query.h:500); here equal TIDs are ordered by batch
+ *     position, so priority is the pair (TID, position).
  */
 #ifndef LTPG_H
 #define LTPG_H
@@ -99,7 +100,7 @@ extern "C" {
 #define LTPG_PHASE_CID 1u
 #define LTPG_PHASE_EXECUTE 2u
 #define LTPG_PHASE_CHECK 4u
-#define LTPG_PHASE_WRITEBACK 8u
+#define LTPG_PHASE_WRITEBACK 8u /* committed reads are delivered, then writes applied */
 #define LTPG_PHASE_ALL 15u
 
 /* ---- key-value op encoding (meta word of one op slot) ------------------ */
@@ -115,6 +116,13 @@ extern "C" {
 #define LTPG_KV_FIELD(meta) (((meta) >> 8) & 0xffffu)
 
 #define LTPG_NO_HOT_ROW UINT64_MAX
+
+/* ---- context flags ------------------------------------------------------ */
+/* CONFLICT_COUNTS: CHECK counts raw / war / waw of every transaction (the
+ * reference's per-transaction counters) for ltpg_conflict_counters().
+ * Without it CHECK skips transactions already known to abort (waw found
+ * during EXECUTE) and the counters are not produced. */
+#define LTPG_FLAG_CONFLICT_COUNTS 1u
 
 /* ---- order-entry schema (TPCC-allopt-final/Predefine.h:48-77) ---------- */
 #define LTPG_TPCC_WAREHOUSE 0u
@@ -136,13 +144,13 @@ typedef struct ltpg_config {
     uint32_t workload;      /* LTPG_WORKLOAD_KV or LTPG_WORKLOAD_TPCC          */
     uint32_t rules;         /* LTPG_RULES_*                                    */
     uint32_t values;        /* LTPG_VALUES_*                                   */
-    uint32_t max_txns;      /* capacity: transactions per batch (per type)     */
+    uint32_t max_txns;      /* capacity: transactions per batch (<= 2^24)      */
     /* key-value table (YCSB/0/Predefine.h:9-10,21) */
     uint64_t kv_rows;
     uint32_t kv_fields;
     uint32_t kv_field_bytes; /* multiple of 4, at most 1024                    */
     uint32_t kv_max_slots;   /* capacity: op slots per transaction (<= 64)     */
-    uint32_t reserved0;
+    uint32_t flags;          /* LTPG_FLAG_*                                    */
     uint64_t kv_hot_row;     /* LTPG_NO_HOT_ROW, or a row whose writes go      */
                              /* through the commutative reduce                 */
     /* order-entry (TPCC-allopt-final/Predefine.h:9) */
@@ -229,11 +237,14 @@ int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_w
 /* Number of committed transactions of `type` in the last epoch. */
 int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out);
 /* Per-transaction conflict counters raw | war << 8 | waw << 16 (the
- * reference's raw/war/waw[NO], TPCC-allopt-final/Datastructure.h:231-233). */
+ * reference's raw/war/waw[NO], TPCC-allopt-final/Datastructure.h:231-233).
+ * Needs LTPG_FLAG_CONFLICT_COUNTS (else LTPG_ERR_UNSUPPORTED). */
 int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t n);
-/* Staged read values of the last key-value epoch (set_local_set,
+/* Read values of the last key-value epoch (set_local_set,
  * YCSB/0/Genericfunction.h:31-45): field words of slot s of transaction t at
- * out[((s * B) + t) * field_words + w].  Only read slots are written. */
+ * out[((s * B) + t) * field_words + w], taken from the snapshot (the table
+ * before the epoch's write-back).  Only read slots of committed transactions
+ * are written: an aborted transaction's reads are discarded. */
 int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words);
 /* Running totals since the last reset: committed and attempted (checked)
  * transactions of `type`, kept on the device so a pipelined epoch loop needs
@@ -244,10 +255,13 @@ int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t
  * enabled every ltpg_epoch_run records events around each phase on the
  * context's stream. */
 int ltpg_set_timing(ltpg_ctx *ctx, int enable);
-/* Sum of the device time in milliseconds of each phase [CID, EXECUTE, CHECK,
- * WRITEBACK] over the epochs timed since the previous call, and their
- * number; then starts a new measurement.  Synchronises the context. */
-int ltpg_phase_times(ltpg_ctx *ctx, float ms[4], uint32_t *n_epochs);
+/* Timing slots: CID, EXECUTE, CHECK, WRITEBACK's read-out of committed
+ * reads, WRITEBACK's writes. */
+#define LTPG_TIMING_SLOTS 5
+/* Sum of the device time in milliseconds of each timing slot over the
+ * epochs timed since the previous call, and their number; then starts a new
+ * measurement.  Synchronises the context. */
+int ltpg_phase_times(ltpg_ctx *ctx, float ms[LTPG_TIMING_SLOTS], uint32_t *n_epochs);
 
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:

@@ -34,6 +34,9 @@ TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT = 0, 1, 2
 PHASE_CID, PHASE_EXECUTE, PHASE_CHECK, PHASE_WRITEBACK, PHASE_ALL = 1, 2, 4, 8, 15
 KV_NONE, KV_READ, KV_WRITE = 0, 1, 2
 NO_HOT_ROW = (1 << 64) - 1
+FLAG_CONFLICT_COUNTS = 1
+TIMING_SLOTS = 5
+TIMING_NAMES = ["cid", "execute", "check", "readout", "writeback"]
 
 
 def kv_meta(kind, field):
@@ -212,8 +215,9 @@ class Context:
         _check(lib.ltpg_set_timing(self.handle, 1 if on else 0), "ltpg_set_timing")
 
     def phase_times(self):
-        """(summed ms per phase [CID, EXECUTE, CHECK, WRITEBACK], epochs)."""
-        ms = (ctypes.c_float * 4)()
+        """(summed ms per timing slot [cid, execute, check, readout, writeback],
+        number of timed epochs); starts a new measurement."""
+        ms = (ctypes.c_float * TIMING_SLOTS)()
         n = ctypes.c_uint32()
         _check(lib.ltpg_phase_times(self.handle, ms, ctypes.byref(n)), "ltpg_phase_times")
         return list(ms), n.value
@@ -246,6 +250,8 @@ class Context:
         return out
 
     def kv_read_values(self):
+        """Read values of committed transactions' read slots (slot-major,
+        field_words per op); other slots are left as they were."""
         _, _, fw = self.kv_shape
         n = self._cur_txns * self._cur_slots * fw
         out = np.zeros(max(n, 1), dtype=np.uint32)

@@ -29,7 +29,7 @@ class Config(ctypes.Structure):
         ("kv_fields", ctypes.c_uint32),
         ("kv_field_bytes", ctypes.c_uint32),
         ("kv_max_slots", ctypes.c_uint32),
-        ("reserved0", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
         ("kv_hot_row", ctypes.c_uint64),
         ("tpcc_warehouses", ctypes.c_uint32),
         ("reserved1", ctypes.c_uint32),

@@ -18,6 +18,7 @@ namespace ltpg {
 
 namespace {
 thread_local std::string g_last_error;
+constexpr uint32_t kEv = LTPG_TIMING_SLOTS + 1;  // events per timed epoch
 }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -58,6 +59,7 @@ void free_kv(KvState &kv)
 {
     dev_free(kv.table);
     dev_free(kv.regs);
+    dev_free(kv.doomed);
     dev_free(kv.read_values);
     dev_free(kv.bitmap);
     dev_free(kv.conflicts);
@@ -81,11 +83,15 @@ int init_kv(ltpg_ctx *ctx)
     kv.rows = c.kv_rows;
     kv.fields = c.kv_fields;
     kv.field_words = c.kv_field_bytes / 4;
+    // Fields of 64 bytes or more start on 64-byte boundaries, so a random
+    // field read touches the fewest 64-byte memory segments.
+    kv.field_stride = kv.field_words >= 16 ? (kv.field_words + 15) / 16 * 16 : kv.field_words;
     kv.max_slots = c.kv_max_slots;
     const uint64_t B = c.max_txns, S = c.kv_max_slots;
     int rc;
-    if ((rc = dev_alloc(&kv.table, kv.rows * kv.fields * kv.field_words, "kv table"))) return rc;
+    if ((rc = dev_alloc(&kv.table, kv.rows * kv.fields * kv.field_stride, "kv table"))) return rc;
     if ((rc = dev_alloc(&kv.regs, kv.rows * 2, "kv registers"))) return rc;
+    if ((rc = dev_alloc(&kv.doomed, B, "kv doomed flags"))) return rc;
     if ((rc = dev_alloc(&kv.read_values, S * B * kv.field_words, "kv read values"))) return rc;
     if ((rc = dev_alloc(&kv.bitmap, (B + 63) / 64, "kv bitmap"))) return rc;
     if ((rc = dev_alloc(&kv.conflicts, B, "kv conflicts"))) return rc;
@@ -95,20 +101,71 @@ int init_kv(ltpg_ctx *ctx)
     // Registers start as all-ones: tag ~0 belongs to epoch 0, which is never
     // current (epochs start at 1), so every row reads as "no registrant".
     LTPG_HIP_TRY(launch_fill_u64(kv.regs, kv.rows * 2, ~0ull, ctx->stream));
-    LTPG_HIP_TRY(hipMemsetAsync(kv.table, 0, kv.rows * kv.fields * kv.field_words * 4, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(kv.doomed, B, 0, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(kv.table, 0, kv.rows * kv.fields * kv.field_stride * 4, ctx->stream));
     LTPG_HIP_TRY(hipMemsetAsync(kv.bitmap, 0, (B + 63) / 64 * 8, ctx->stream));
     LTPG_HIP_TRY(hipMemsetAsync(kv.conflicts, 0, B * 4, ctx->stream));
     return LTPG_OK;
 }
 
-// Advance the register tag; at wrap-around refill every register.
+// Advance the epoch (1..255).  At wrap-around every register goes back to
+// all-ones (the tag of epoch 0, never current) and the doomed flags to 0.
 int next_epoch(ltpg_ctx *ctx)
 {
     ctx->epoch += 1;
-    if (ctx->epoch == 0) {
-        if (ctx->kv.regs) LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
+    if (ctx->epoch > 255) {
+        if (ctx->kv.regs) {
+            LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
+            LTPG_HIP_TRY(launch_fill_u8(ctx->kv.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+        }
         ctx->epoch = 1;
     }
+    return LTPG_OK;
+}
+
+EpochTag epoch_tag(const ltpg_ctx *ctx)
+{
+    EpochTag e;
+    e.epoch = (uint8_t)ctx->epoch;
+    e.tag = 255u - ctx->epoch;
+    return e;
+}
+
+// Host <-> device table copies between the caller's dense row-major layout
+// (field_words apart) and the device's strided one, in chunks through a
+// device staging buffer.
+int kv_copy_table(ltpg_ctx *ctx, void *host, bool to_device)
+{
+    KvState &kv = ctx->kv;
+    const uint64_t n_fields = kv.rows * kv.fields;
+    const uint64_t fw = kv.field_words, fsw = kv.field_stride;
+    if (fw == fsw) {
+        if (to_device)
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.table, host, n_fields * fw * 4, hipMemcpyHostToDevice, ctx->stream));
+        else
+            LTPG_HIP_TRY(hipMemcpyAsync(host, kv.table, n_fields * fw * 4, hipMemcpyDeviceToHost, ctx->stream));
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return LTPG_OK;
+    }
+    const uint64_t chunk_fields = std::max<uint64_t>(1, (256ull << 20) / (fw * 4));
+    uint32_t *tmp = nullptr;
+    int rc = dev_alloc(&tmp, std::min(chunk_fields, n_fields) * fw, "repack staging");
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    for (uint64_t f0 = 0; f0 < n_fields && e == hipSuccess; f0 += chunk_fields) {
+        const uint64_t nf = std::min(chunk_fields, n_fields - f0);
+        uint32_t *h = static_cast<uint32_t *>(host) + f0 * fw;
+        if (to_device) {
+            e = hipMemcpyAsync(tmp, h, nf * fw * 4, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = launch_repack(tmp, kv.table + f0 * fsw, nf, (uint32_t)fw, (uint32_t)fsw, 1, ctx->stream);
+        } else {
+            e = launch_repack(kv.table + f0 * fsw, tmp, nf, (uint32_t)fw, (uint32_t)fsw, 0, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(h, tmp, nf * fw * 4, hipMemcpyDeviceToHost, ctx->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    dev_free(tmp);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("table copy: ") + hipGetErrorString(e));
     return LTPG_OK;
 }
 
@@ -158,7 +215,8 @@ int ltpg_create(const ltpg_config *cfg, ltpg_ctx **out)
         LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload");
     if (cfg->rules > LTPG_RULES_INTENDED) LTPG_FAIL(LTPG_ERR_INVALID, "unknown rule set");
     if (cfg->values > LTPG_VALUES_RMW) LTPG_FAIL(LTPG_ERR_INVALID, "unknown value mode");
-    if (cfg->max_txns == 0 || cfg->max_txns > (1u << 26)) LTPG_FAIL(LTPG_ERR_INVALID, "max_txns must be in [1, 2^26]");
+    if (cfg->max_txns == 0 || cfg->max_txns > (1u << 24)) LTPG_FAIL(LTPG_ERR_INVALID, "max_txns must be in [1, 2^24]");
+    if (cfg->flags & ~LTPG_FLAG_CONFLICT_COUNTS) LTPG_FAIL(LTPG_ERR_INVALID, "unknown flag");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) LTPG_FAIL(LTPG_ERR_DEVICE, "no HIP device visible");
     if (cfg->device < 0 || cfg->device >= ndev) LTPG_FAIL(LTPG_ERR_INVALID, "device ordinal out of range");
@@ -258,18 +316,18 @@ int ltpg_set_timing(ltpg_ctx *ctx, int enable)
     return LTPG_OK;
 }
 
-int ltpg_phase_times(ltpg_ctx *ctx, float ms[4], uint32_t *n_epochs)
+int ltpg_phase_times(ltpg_ctx *ctx, float ms[LTPG_TIMING_SLOTS], uint32_t *n_epochs)
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ms) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int k = 0; k < 4; k++) ms[k] = 0.f;
-    const uint32_t n = ctx->ev_used / 5;
+    for (int k = 0; k < LTPG_TIMING_SLOTS; k++) ms[k] = 0.f;
+    const uint32_t n = ctx->ev_used / kEv;
     for (uint32_t e = 0; e < n; e++) {
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < LTPG_TIMING_SLOTS; k++) {
             float v = 0.f;
-            LTPG_HIP_TRY(hipEventElapsedTime(&v, ctx->ev_pool[5 * e + k], ctx->ev_pool[5 * e + k + 1]));
+            LTPG_HIP_TRY(hipEventElapsedTime(&v, ctx->ev_pool[kEv * e + k], ctx->ev_pool[kEv * e + k + 1]));
             ms[k] += v;
         }
     }
@@ -366,22 +424,18 @@ int ltpg_kv_table_load(ltpg_ctx *ctx, const void *host, uint64_t rows, uint32_t 
     if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
     if (!host) LTPG_FAIL(LTPG_ERR_INVALID, "null host table");
     if (rows != kv.rows) LTPG_FAIL(LTPG_ERR_INVALID, "rows does not match the configured kv_rows");
-    const uint64_t words = kv.rows * kv.fields * kv.field_words;
-    if (layout == LTPG_LAYOUT_ROW_MAJOR) {
-        LTPG_HIP_TRY(hipMemcpyAsync(kv.table, host, words * 4, hipMemcpyHostToDevice, ctx->stream));
-    } else if (layout == LTPG_LAYOUT_COLUMN_MAJOR) {
-        if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major load needs kv_field_bytes == 4");
-        uint32_t *tmp = nullptr;
-        if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
-        hipError_t e = hipMemcpyAsync(tmp, host, words * 4, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = launch_kv_transpose(tmp, kv.table, kv.rows, kv.fields, 1, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        dev_free(tmp);
-        if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major load: ") + hipGetErrorString(e));
-    } else {
-        LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
-    }
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (layout == LTPG_LAYOUT_ROW_MAJOR) return kv_copy_table(ctx, const_cast<void *>(host), true);
+    if (layout != LTPG_LAYOUT_COLUMN_MAJOR) LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
+    if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major load needs kv_field_bytes == 4");
+    const uint64_t words = kv.rows * kv.fields;
+    uint32_t *tmp = nullptr;
+    if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
+    hipError_t e = hipMemcpyAsync(tmp, host, words * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_kv_transpose(tmp, kv.table, kv.rows, kv.fields, 1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(tmp);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major load: ") + hipGetErrorString(e));
     return LTPG_OK;
 }
 
@@ -393,22 +447,18 @@ int ltpg_kv_table_read(ltpg_ctx *ctx, void *host, uint64_t rows, uint32_t layout
     if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
     if (!host) LTPG_FAIL(LTPG_ERR_INVALID, "null host table");
     if (rows != kv.rows) LTPG_FAIL(LTPG_ERR_INVALID, "rows does not match the configured kv_rows");
-    const uint64_t words = kv.rows * kv.fields * kv.field_words;
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (layout == LTPG_LAYOUT_ROW_MAJOR) {
-        LTPG_HIP_TRY(hipMemcpy(host, kv.table, words * 4, hipMemcpyDeviceToHost));
-    } else if (layout == LTPG_LAYOUT_COLUMN_MAJOR) {
-        if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major read needs kv_field_bytes == 4");
-        uint32_t *tmp = nullptr;
-        if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
-        hipError_t e = launch_kv_transpose(kv.table, tmp, kv.rows, kv.fields, 0, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(host, tmp, words * 4, hipMemcpyDeviceToHost);
-        dev_free(tmp);
-        if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major read: ") + hipGetErrorString(e));
-    } else {
-        LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
-    }
+    if (layout == LTPG_LAYOUT_ROW_MAJOR) return kv_copy_table(ctx, host, false);
+    if (layout != LTPG_LAYOUT_COLUMN_MAJOR) LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
+    if (kv.field_words != 1) LTPG_FAIL(LTPG_ERR_INVALID, "column-major read needs kv_field_bytes == 4");
+    const uint64_t words = kv.rows * kv.fields;
+    uint32_t *tmp = nullptr;
+    if ((rc = dev_alloc(&tmp, words, "transpose staging"))) return rc;
+    hipError_t e = launch_kv_transpose(kv.table, tmp, kv.rows, kv.fields, 0, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(host, tmp, words * 4, hipMemcpyDeviceToHost);
+    dev_free(tmp);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("column-major read: ") + hipGetErrorString(e));
     return LTPG_OK;
 }
 
@@ -418,7 +468,7 @@ int ltpg_kv_table_fill_reference(ltpg_ctx *ctx)
     if (rc) return rc;
     KvState &kv = ctx->kv;
     if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    LTPG_HIP_TRY(launch_kv_fill_reference(kv.table, kv.rows, kv.fields, kv.field_words, ctx->stream));
+    LTPG_HIP_TRY(launch_kv_fill_reference(kv, ctx->stream));
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return LTPG_OK;
 }
@@ -466,18 +516,18 @@ int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
 // ---- epoch phases --------------------------------------------------------
 
 namespace {
-// Take the next 5 timing events of this epoch from the pool.
+// Take the next LTPG_TIMING_SLOTS + 1 timing events of this epoch.
 int take_events(ltpg_ctx *ctx, hipEvent_t **out)
 {
-    if (ctx->ev_used + 5 > ctx->ev_pool.size()) {
-        for (int k = 0; k < 40; k++) {
+    if (ctx->ev_used + kEv > ctx->ev_pool.size()) {
+        for (int k = 0; k < 8 * kEv; k++) {
             hipEvent_t e = nullptr;
             LTPG_HIP_TRY(hipEventCreate(&e));
             ctx->ev_pool.push_back(e);
         }
     }
     *out = &ctx->ev_pool[ctx->ev_used];
-    ctx->ev_used += 5;
+    ctx->ev_used += kEv;
     return LTPG_OK;
 }
 }  // namespace
@@ -489,7 +539,7 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     if (phase_mask & ~LTPG_PHASE_ALL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown phase bit");
     KvState &kv = ctx->kv;
     if (!kv.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "no batch submitted");
-    const uint32_t inv = ~ctx->epoch;
+    const EpochTag e = epoch_tag(ctx);
     hipStream_t s = ctx->stream;
     hipEvent_t *ev = nullptr;
     if (ctx->timing) {
@@ -498,15 +548,19 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     }
     // CID has no key-value work: its bracket is empty.
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
-    if (phase_mask & LTPG_PHASE_EXECUTE) LTPG_HIP_TRY(launch_kv_execute(kv, kv.cur, inv, ctx->counters, s));
+    if (phase_mask & LTPG_PHASE_EXECUTE) LTPG_HIP_TRY(launch_kv_execute(kv, kv.cur, e, ctx->counters, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
     if (phase_mask & LTPG_PHASE_CHECK) {
-        LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
-        LTPG_HIP_TRY(launch_kv_check(kv, kv.cur, inv, ctx->counters, s));
+        if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
+        LTPG_HIP_TRY(launch_kv_check(kv, kv.cur, e, (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS) != 0,
+                                     ctx->counters, s));
     }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[3], s));
-    if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_writeback(kv, kv.cur, ctx->cfg.values, s));
+    // committed reads see the snapshot: deliver them before any write
+    if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_readout(kv, kv.cur, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[4], s));
+    if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_writeback(kv, kv.cur, ctx->cfg.values, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[5], s));
     return LTPG_OK;
 }
 
@@ -549,6 +603,8 @@ int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t
     if (rc) return rc;
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
     if (type != LTPG_TYPE_KV || !ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "unknown transaction type");
+    if (!(ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS))
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "conflict counters need LTPG_FLAG_CONFLICT_COUNTS");
     if (n > ctx->kv.cur.n_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the batch size");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n) LTPG_HIP_TRY(hipMemcpy(out, ctx->kv.conflicts, (uint64_t)n * 4, hipMemcpyDeviceToHost));

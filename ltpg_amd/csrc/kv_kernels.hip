@@ -1,4 +1,5 @@
-// kv_kernels.hip - key-value execute / check / write-back kernels for gfx950.
+// kv_kernels.hip - key-value execute / check / readout / write-back kernels
+// for gfx950.
 //
 // Replaces KERNEL_EXECUTE / KERNEL_CHECK / KERNEL_WRITEBACK of
 // LTPG/YCSB/0/ycsb.cu:9-58 and the per-op bodies of YCSB/0/Execute.h:11-252
@@ -7,46 +8,50 @@
 // Mapping.  The batch is slot-major (slot s of transaction t at [s*B + t],
 // the reference's LOC[B*op + NO]).  One lane owns one transaction and walks
 // its slots, so every slot-level load of a wave is 64 consecutive elements
-// (coalesced), conflict counts accumulate in registers (no per-op atomicAdd
-// on raw/war/waw[NO]), and the commit decision is fused into the check: a
-// wave64 ballot of the commit predicate is exactly one 64-bit word of the
+// (coalesced), per-transaction state stays in registers (no per-op
+// atomicAdd on raw/war/waw[NO]), and the commit decision is fused into the
+// check: a wave64 ballot of the commit predicate is one 64-bit word of the
 // commit bitmap.
 //
-// Registers.  regs[2*row] is R and regs[2*row+1] is W, each a u64
-// (~epoch << 32 | TID).  A newer epoch has a smaller tag, so one atomicMin
-// both registers a TID and evicts a stale value; a register whose tag is not
-// the current one reads as 0xFFFFFFFF (the reference's cleared value,
-// Database.h:45-50).  R and W of a row share one 16-byte line segment, so a
-// write op's check is a single 16-byte load.
+// Registers.  regs[2*row] is R and regs[2*row+1] is W, each a u64 priority
+//     tag:8 | TID:32 | batch position:24
+// where tag = 255 - epoch (epochs 1..255; the library refills every register
+// with all-ones when the epoch counter wraps).  Lower value = higher
+// priority, and every value of an older epoch is larger than every value of
+// the current one, so a single atomicMin both registers and evicts a stale
+// value, and "register < my priority" already implies "registered in this
+// epoch" -- stale registers never produce a conflict.
 //
-// Field copies.  A field is field_words u32 words inside a row-major record.
-// A wave moves the fields of its (up to) 64 ops cooperatively: LP lanes per
-// op (LP = next power of two >= field_words, at most 64), so a 100-byte field
-// is one contiguous run of 25 lanes instead of 25 scattered loads per lane.
-// The op's addresses come from its owner lane by __shfl.
+// Early waw.  atomicMin returns the old value.  A write whose registration
+// finds a lower priority already there has waw > 0; a write that lowers the
+// register from v displaces the writer whose position is in v, which then
+// has waw > 0.  Every writer except the minimum one is caught by exactly one
+// of the two (the first registration that goes below a writer sees exactly
+// that writer's value), so after EXECUTE the doomed[] bytes mark precisely
+// the transactions with waw > 0, and CHECK only walks the others.
+//
+// Field copies.  A field is field_words u32 words inside a row-major record
+// whose fields start at multiples of field_stride words (field_stride =
+// field_words rounded up to 16 words = 64 bytes for fields of 64 bytes or
+// more, so a 100-byte field touches exactly two 64-byte segments).  A wave
+// moves the fields of its ops cooperatively: LP lanes per op (LP = next
+// power of two >= field_words, at most 64), the op's addresses fetched from
+// its owner lane by __shfl.
 #include "ltpg_internal.h"
 
 namespace ltpg {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint32_t reg_tid(unsigned long long v, uint32_t inv_epoch)
-{
-    return (uint32_t)(v >> 32) == inv_epoch ? (uint32_t)v : 0xFFFFFFFFu;
-}
+constexpr int kTagShift = 56;
+constexpr int kTidShift = 24;
+constexpr unsigned long long kPosMask = 0xFFFFFFull;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// atomicMin that skips the atomic when the register already holds a value
-// <= v.  A register only ever decreases (atomicMin within an epoch, and a
-// newer epoch's tag is smaller than every older one), so any value a plain
-// load returns -- even a stale cached one -- is >= the register's current
-// value: if it is <= v the atomic could not change anything.  On hot rows
-// this turns a serialized chain of same-address atomics into cached loads.
-__device__ __forceinline__ void register_min(unsigned long long *reg, unsigned long long v)
+__device__ __forceinline__ unsigned long long load_relaxed(const unsigned long long *p)
 {
-    if (__atomic_load_n(reg, __ATOMIC_RELAXED) > v) atomicMin(reg, v);
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
 }
 
 // Copy (or read-modify-write) the fields of the active ops of a wave.
@@ -116,6 +121,27 @@ __device__ __forceinline__ void copy_fields(unsigned long long mask, uint64_t sr
         wave_copy<LP, (LP < 8 ? LP : 8)>(mask, src_off, dst_off, delta, src_base, dst_base, fw);
 }
 
+// Per-op decode shared by every phase.
+struct Op {
+    uint64_t row;
+    uint32_t kind;
+    uint32_t field;
+    bool valid;
+};
+
+__device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
+                                      const uint32_t *__restrict__ meta, uint64_t i, bool live,
+                                      uint64_t rows, uint32_t fields)
+{
+    Op o;
+    const uint32_t m = live ? meta[i] : 0u;
+    o.row = live ? keys[i] : 0ull;
+    o.kind = m & 3u;
+    o.field = (m >> 8) & 0xffffu;
+    o.valid = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && o.row < rows && o.field < fields;
+    return o;
+}
+
 // Zero the per-epoch commit counts (a kernel rather than a memset so the
 // epoch can be captured in a graph).
 __global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
@@ -136,14 +162,39 @@ __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-// Reference initial contents: every word of field c holds c
-// (YCSB/0/Database.h:77-86 writes Y_c = c into column c).
-__global__ void kv_fill_reference_kernel(uint32_t *table, uint64_t n_words, uint32_t fields,
-                                         uint32_t fw)
+__global__ void fill_u8_kernel(uint8_t *p, uint64_t n, uint8_t v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += stride)
-        table[i] = (uint32_t)((i / fw) % fields);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Reference initial contents: every word of field c holds c
+// (YCSB/0/Database.h:77-86 writes Y_c = c into column c); padding words 0.
+__global__ void kv_fill_reference_kernel(uint32_t *table, uint64_t n_words, uint32_t fields,
+                                         uint32_t fw, uint32_t fsw)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += stride) {
+        const uint64_t f = i / fsw;
+        const uint32_t w = (uint32_t)(i - f * fsw);
+        table[i] = w < fw ? (uint32_t)(f % fields) : 0u;
+    }
+}
+
+// Dense fields (field_words apart) <-> strided fields (field_stride apart).
+__global__ void repack_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                              uint64_t n_fields, uint32_t fw, uint32_t fsw, int to_strided)
+{
+    const uint64_t n = n_fields * fw;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t f = i / fw;
+        const uint64_t w = i - f * fw;
+        if (to_strided)
+            dst[f * fsw + w] = src[i];
+        else
+            dst[i] = src[f * fsw + w];
+    }
 }
 
 // Column-major <-> row-major for u32 tables (rows x cols).
@@ -164,84 +215,107 @@ __global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__r
     }
 }
 
-// EXECUTE: register every valid op's TID on R (read) or W (write) and stage
-// read fields from the snapshot (read_0 / read_1 / write_0,
-// YCSB/0/Execute.h:11-53; __REGISTER / __READ, Genericfunction.h:31-100).
-// A write is a read-modify-write applied at write-back, so it stages nothing.
-template <int LP>
+// EXECUTE: register every valid op's priority on R (read) or W (write)
+// (read_0 / write_0 -> __REGISTER, YCSB/0/Execute.h:11-53,
+// Genericfunction.h:61-100) and record waw by displacement.  The load before
+// the atomic skips registrations that cannot lower the register (a register
+// only decreases, so even a stale cached value is >= its current value); on
+// hot rows this turns a serialized chain of same-address atomics into
+// cached loads.
 __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, const uint32_t *__restrict__ table, unsigned long long *__restrict__ regs,
-    uint32_t inv_epoch, uint32_t *__restrict__ read_values, DeviceCounters *__restrict__ ctr)
+    unsigned long long *__restrict__ regs, uint32_t tag, uint8_t *__restrict__ doomed,
+    uint8_t epoch, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
-    const unsigned long long tagged =
-        live ? (((unsigned long long)inv_epoch << 32) | tids[t]) : 0ull;
+    // the epoch's commit counts start at zero (CHECK adds to them)
+    if (blockIdx.x == 0 && threadIdx.x < 3) ctr->commits[threadIdx.x] = 0;
+    const unsigned long long me =
+        live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
+             : 0ull;
+    bool waw = false;
     uint32_t bad = 0;
     for (uint32_t s = 0; s < S; ++s) {
-        const uint64_t i = (uint64_t)s * B + t;
-        uint32_t m = 0;
-        uint64_t row = 0;
-        if (live) {
-            m = meta[i];
-            row = keys[i];
-        }
-        const uint32_t kind = m & 3u;
-        const uint32_t field = (m >> 8) & 0xffffu;
-        const bool is_op = kind == LTPG_KV_READ || kind == LTPG_KV_WRITE;
-        const bool valid = is_op && row < rows && field < fields;
-        bad += (is_op && !valid) ? 1u : 0u;
-        if (valid) register_min(&regs[2 * row + (kind == LTPG_KV_WRITE ? 1 : 0)], tagged);
-        const unsigned long long rd = __ballot(valid && kind == LTPG_KV_READ);
-        if (rd) {
-            const uint64_t src = (row * fields + field) * fw;
-            copy_fields<LP>(rd, src, i * fw, 0u, table, read_values, fw);
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
+        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
+        if (!o.valid) continue;
+        unsigned long long *r = &regs[2 * o.row + (o.kind == LTPG_KV_WRITE ? 1 : 0)];
+        const unsigned long long cur = load_relaxed(r);
+        if (o.kind == LTPG_KV_WRITE) {
+            if (cur < me) {
+                waw = true;  // a higher-priority writer is already registered
+            } else if (cur > me) {
+                const unsigned long long old = atomicMin(r, me);
+                if (old < me)
+                    waw = true;
+                else if (old > me && (uint32_t)(old >> kTagShift) == tag)
+                    doomed[old & kPosMask] = epoch;  // displaced writer
+            }
+        } else if (cur > me) {
+            atomicMin(r, me);
         }
     }
+    if (waw) doomed[t] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
 }
 
-// CHECK + COMMIT: per op read back the opposite register(s) and count
-// conflicts when TID > registered (check_r / check_w_0 / check_w_1 and
-// __CHECK, YCSB/0/Execute.h:109-175, Genericfunction.h:101-144); commit iff
-// waw == 0 && (raw == 0 || war == 0) (writeback_0, Execute.h:184-191).
+// CHECK + COMMIT (check_r / check_w_0 / check_w_1 and __CHECK,
+// YCSB/0/Execute.h:109-175, Genericfunction.h:101-144; writeback_0,
+// Execute.h:184-191).  Fast form: transactions doomed by waw in EXECUTE
+// abort without touching a register; the others read W for their reads
+// (raw) and R for their writes (war) and stop as soon as both are set.
+// Counting form (kCounts): every transaction counts raw / war / waw over all
+// its ops, the reference's counters, for parity checks of the counters.
+template <bool kCounts>
 __global__ __launch_bounds__(kBlock) void kv_check_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    const unsigned long long *__restrict__ regs, uint32_t inv_epoch,
-    uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts,
-    DeviceCounters *__restrict__ ctr)
+    const unsigned long long *__restrict__ regs, uint32_t tag,
+    const uint8_t *__restrict__ doomed, uint8_t epoch, uint64_t *__restrict__ bitmap,
+    uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
     const bool live = t < B;
-    const uint32_t tid = live ? tids[t] : 0u;
-    uint32_t raw = 0, war = 0, waw = 0;
-    for (uint32_t s = 0; s < S; ++s) {
-        const uint64_t i = (uint64_t)s * B + t;
-        uint32_t m = 0;
-        uint64_t row = 0;
-        if (live) {
-            m = meta[i];
-            row = keys[i];
-        }
-        const uint32_t kind = m & 3u;
-        const uint32_t field = (m >> 8) & 0xffffu;
-        const bool valid =
-            (kind == LTPG_KV_READ || kind == LTPG_KV_WRITE) && row < rows && field < fields;
-        if (valid) {
-            if (kind == LTPG_KV_READ) {
-                raw += tid > reg_tid(regs[2 * row + 1], inv_epoch) ? 1u : 0u;
+    const unsigned long long me =
+        live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
+             : 0ull;
+    bool commit;
+    if constexpr (kCounts) {
+        uint32_t raw = 0, war = 0, waw = 0;
+        for (uint32_t s = 0; s < S; ++s) {
+            const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
+            if (!o.valid) continue;
+            if (o.kind == LTPG_KV_READ) {
+                raw += regs[2 * o.row + 1] < me ? 1u : 0u;
             } else {
-                const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * row);
-                war += tid > reg_tid(rw.x, inv_epoch) ? 1u : 0u;
-                waw += tid > reg_tid(rw.y, inv_epoch) ? 1u : 0u;
+                const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * o.row);
+                war += rw.x < me ? 1u : 0u;
+                waw += rw.y < me ? 1u : 0u;
             }
         }
+        commit = live && waw == 0 && (raw == 0 || war == 0);
+        if (live) conflicts[t] = raw | (war << 8) | (waw << 16);
+    } else {
+        const bool alive = live && doomed[t] != epoch;
+        bool raw = false, war = false;
+        if (__ballot(alive)) {
+            for (uint32_t s = 0; s < S; ++s) {
+                const bool todo = alive && !(raw && war);
+                if (!__ballot(todo)) break;
+                if (!todo) continue;
+                const Op o = load_op(keys, meta, (uint64_t)s * B + t, true, rows, fields);
+                if (!o.valid) continue;
+                if (o.kind == LTPG_KV_READ)
+                    raw |= regs[2 * o.row + 1] < me;
+                else
+                    war |= regs[2 * o.row] < me;
+            }
+        }
+        commit = alive && !(raw && war);
     }
-    const bool commit = live && waw == 0 && (raw == 0 || war == 0);
     const unsigned long long word = __ballot(commit);
     const uint32_t wave_base = t - (uint32_t)lane;
     if (lane == 0 && wave_base < B) {
@@ -254,19 +328,45 @@ __global__ __launch_bounds__(kBlock) void kv_check_kernel(
         }
         atomicAdd(&ctr->attempts_total[LTPG_TYPE_KV], (unsigned long long)live_n);
     }
-    if (live) conflicts[t] = raw | (war << 8) | (waw << 16);
+}
+
+// READOUT: committed transactions' read fields, gathered from the snapshot
+// before any write-back of the epoch (read_1 / __READ, YCSB/0/Execute.h:
+// 27-37, Genericfunction.h:31-45).  Reads of aborted transactions are never
+// delivered, so they are not gathered.
+template <int LP>
+__global__ __launch_bounds__(kBlock) void kv_readout_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, uint32_t B, uint32_t S,
+    uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint32_t *__restrict__ table,
+    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ read_values)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = lane_id();
+    const uint32_t wave_base = t - (uint32_t)lane;
+    if (wave_base >= B) return;
+    const uint64_t word = bitmap[wave_base / 64];
+    if (word == 0) return;
+    const bool committed = t < B && ((word >> lane) & 1ull);
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint64_t i = (uint64_t)s * B + t;
+        const Op o = load_op(keys, meta, i, committed, rows, fields);
+        const unsigned long long mask = __ballot(o.valid && o.kind == LTPG_KV_READ);
+        if (mask)
+            copy_fields<LP>(mask, (o.row * fields + o.field) * fsw, i * fw, 0u, table, read_values, fw);
+    }
 }
 
 // WRITEBACK: committed transactions apply their write slots in slot order
 // (writeback_w / __WRITEBACK, YCSB/0/Execute.h:193-206,
 // Genericfunction.h:145-158).  Value: word + delta, delta = TID | 1 for RMW
-// and 0 for copy-back.  The committed writer of a row is unique (the min-TID
-// W registrant), so reading the table here reads the snapshot.
+// and 0 for copy-back.  The committed writer of a row is unique (the
+// minimum W registrant), so reading the table here reads the snapshot.
 template <int LP>
 __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, uint32_t *table, const uint64_t *__restrict__ bitmap, uint32_t values)
+    uint32_t fw, uint32_t fsw, uint32_t *table, const uint64_t *__restrict__ bitmap,
+    uint32_t values)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -279,25 +379,14 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const bool committed = t < B && ((word >> lane) & 1ull);
     const uint32_t delta = (committed && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
     for (uint32_t s = 0; s < S; ++s) {
-        const uint64_t i = (uint64_t)s * B + t;
-        uint32_t m = 0;
-        uint64_t row = 0;
-        if (committed) {
-            m = meta[i];
-            row = keys[i];
-        }
-        const uint32_t field = (m >> 8) & 0xffffu;
-        const bool wr = committed && (m & 3u) == LTPG_KV_WRITE && row < rows && field < fields;
-        const unsigned long long mask = __ballot(wr);
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, committed, rows, fields);
+        const unsigned long long mask = __ballot(o.valid && o.kind == LTPG_KV_WRITE);
         if (mask) {
-            const uint64_t off = (row * fields + field) * fw;
+            const uint64_t off = (o.row * fields + o.field) * fsw;
             copy_fields<LP>(mask, off, off, delta, table, table, fw);
         }
     }
 }
-
-template <template <int> class K>
-struct Dispatch;
 
 int lanes_for(uint32_t fw)
 {
@@ -356,12 +445,25 @@ hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long
     return hipGetLastError();
 }
 
-hipError_t launch_kv_fill_reference(uint32_t *table, uint64_t rows, uint32_t fields,
-                                    uint32_t field_words, hipStream_t s)
+hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s)
 {
-    const uint64_t n = rows * fields * field_words;
-    hipLaunchKernelGGL(kv_fill_reference_kernel, grid_stride(n), dim3(kBlock), 0, s, table, n,
-                       fields, field_words);
+    hipLaunchKernelGGL(fill_u8_kernel, grid_stride(n), dim3(kBlock), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_fill_reference(const KvState &kv, hipStream_t s)
+{
+    const uint64_t n = kv.rows * kv.fields * kv.field_stride;
+    hipLaunchKernelGGL(kv_fill_reference_kernel, grid_stride(n), dim3(kBlock), 0, s, kv.table, n,
+                       kv.fields, kv.field_words, kv.field_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, uint32_t fw,
+                         uint32_t fsw, int to_strided, hipStream_t s)
+{
+    hipLaunchKernelGGL(repack_kernel, grid_stride(n_fields * fw), dim3(kBlock), 0, s, src, dst, n_fields, fw,
+                       fsw, to_strided);
     return hipGetLastError();
 }
 
@@ -373,27 +475,40 @@ hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows
     return hipGetLastError();
 }
 
-hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e,
                              DeviceCounters *ctr, hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-    const int lp = lanes_for(kv.field_words);
-#define CALL(LPV)                                                                         \
-    hipLaunchKernelGGL(kv_execute_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,   \
-                       b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,  \
-                       kv.field_words, kv.table, kv.regs, inv_epoch, kv.read_values, ctr)
-    LTPG_LP_SWITCH(lp, CALL)
-#undef CALL
+    hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
+                       b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed, e.epoch, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-    hipLaunchKernelGGL(kv_check_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
-                       b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, inv_epoch,
-                       kv.bitmap, kv.conflicts, ctr);
+    if (counts)
+        hipLaunchKernelGGL(kv_check_kernel<true>, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
+                           b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed,
+                           e.epoch, kv.bitmap, kv.conflicts, ctr);
+    else
+        hipLaunchKernelGGL(kv_check_kernel<false>, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
+                           b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed,
+                           e.epoch, kv.bitmap, kv.conflicts, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s)
+{
+    if (b.n_txns == 0) return hipSuccess;
+    const int lp = lanes_for(kv.field_words);
+#define CALL(LPV)                                                                          \
+    hipLaunchKernelGGL(kv_readout_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,    \
+                       b.keys, b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields,           \
+                       kv.field_words, kv.field_stride, kv.table, kv.bitmap, kv.read_values)
+    LTPG_LP_SWITCH(lp, CALL)
+#undef CALL
     return hipGetLastError();
 }
 
@@ -405,7 +520,7 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 #define CALL(LPV)                                                                           \
     hipLaunchKernelGGL(kv_writeback_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,   \
                        b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,    \
-                       kv.field_words, kv.table, kv.bitmap, values)
+                       kv.field_words, kv.field_stride, kv.table, kv.bitmap, values)
     LTPG_LP_SWITCH(lp, CALL)
 #undef CALL
     return hipGetLastError();

@@ -47,13 +47,22 @@ struct KvBatchDev {
     const uint32_t *tids = nullptr;
 };
 
+// Register tag of the current epoch: registers hold tag:8 | TID:32 | pos:24
+// with tag = 255 - epoch, epochs 1..255.
+struct EpochTag {
+    uint32_t tag;
+    uint8_t epoch;
+};
+
 struct KvState {
     uint64_t rows = 0;
     uint32_t fields = 0;
     uint32_t field_words = 0;
+    uint32_t field_stride = 0;          // words between field starts (>= field_words)
     uint32_t max_slots = 0;
-    uint32_t *table = nullptr;          // rows * fields * field_words, row-major
+    uint32_t *table = nullptr;          // rows * fields * field_stride, row-major
     unsigned long long *regs = nullptr; // 2 * rows: [R, W] per row, epoch-tagged
+    uint8_t *doomed = nullptr;          // max_txns: epoch number if waw > 0
     uint32_t *read_values = nullptr;    // max_slots * max_txns * field_words
     uint64_t *bitmap = nullptr;         // ceil(max_txns / 64)
     uint32_t *conflicts = nullptr;      // max_txns
@@ -72,11 +81,12 @@ struct ltpg_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    uint32_t epoch = 1;  // register tag is ~epoch; 0 is the tag of the fill value
+    uint32_t epoch = 1;  // 1..255; 0 owns the all-ones fill value of the registers
     ltpg::DeviceCounters *counters = nullptr;
     ltpg::KvState kv;
-    // Per-phase timing: each timed epoch takes 5 events from the pool
-    // (start, after CID, after EXECUTE, after CHECK, after WRITEBACK).
+    // Per-step timing: each timed epoch takes LTPG_TIMING_SLOTS + 1 events
+    // (start, after CID, after EXECUTE, after CHECK, after the read-out of
+    // WRITEBACK, after WRITEBACK).
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     uint32_t ev_used = 0;
@@ -88,14 +98,17 @@ namespace ltpg {
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
-hipError_t launch_kv_fill_reference(uint32_t *table, uint64_t rows, uint32_t fields,
-                                    uint32_t field_words, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s);
+hipError_t launch_kv_fill_reference(const KvState &kv, hipStream_t s);
+hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, uint32_t fw,
+                         uint32_t fsw, int to_strided, hipStream_t s);
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                int to_row_major, hipStream_t s);
-hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e,
                              DeviceCounters *ctr, hipStream_t s);
-hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, uint32_t inv_epoch,
+hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s);
 hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
                                hipStream_t s);
 

@@ -24,6 +24,17 @@
  *     word + (TID | 1), applied per write op in slot order.
  * A slot whose meta kind is NONE does nothing.  A key >= rows is counted in
  * range_errors and the op does nothing.
+ *
+ * Priority.  The reference compares TIDs only and relies on them being
+ * distinct (TID = (epoch & 0xfff) << 20 | seq, TPCC-allopt-final/Query.h:
+ * 500).  Here priority is the pair (TID, batch position) in lexicographic
+ * order, which is the reference's order whenever TIDs are distinct and a
+ * total order when they are not.  Registers hold the minimum pair.
+ *
+ * Read values.  Only committed transactions' reads are staged: an aborted
+ * transaction's reads are discarded (it re-executes in a later epoch), so
+ * the build does not produce them.  They come from the snapshot, i.e. the
+ * table before this epoch's write-back.
  */
 #include "ltpg_oracle.h"
 
@@ -44,18 +55,21 @@ static int op_valid(uint32_t m, uint64_t row, uint64_t rows, uint32_t fields)
     return (k == K_READ || k == K_WRITE) && row < rows && FIELD(m) < fields;
 }
 
-/* __REGISTER: atomicMin(&log->YCSB_{R,W}[Loc], TID) -> a plain minimum. */
+static uint64_t prio(const uint32_t *tids, uint32_t t) { return ((uint64_t)tids[t] << 32) | t; }
+
+/* __REGISTER: atomicMin(&log->YCSB_{R,W}[Loc], TID) -> a plain minimum of
+ * the priority pair. */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                        uint32_t *reg_r, uint32_t *reg_w)
+                        uint64_t *reg_r, uint64_t *reg_w)
 {
     for (uint32_t s = 0; s < S; s++) {
         for (uint32_t t = 0; t < B; t++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
             if (!op_valid(meta[i], row, rows, fields)) continue;
-            uint32_t *reg = (KIND(meta[i]) == K_READ) ? reg_r : reg_w;
-            if (tids[t] < reg[row]) reg[row] = tids[t];
+            uint64_t *reg = (KIND(meta[i]) == K_READ) ? reg_r : reg_w;
+            if (prio(tids, t) < reg[row]) reg[row] = prio(tids, t);
         }
     }
 }
@@ -65,19 +79,20 @@ void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
  * W (waw). */
 void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                      const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                     const uint32_t *reg_r, const uint32_t *reg_w, uint32_t *conflicts)
+                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts)
 {
     for (uint32_t t = 0; t < B; t++) {
-        uint32_t raw = 0, war = 0, waw = 0, tid = tids[t];
+        uint32_t raw = 0, war = 0, waw = 0;
+        const uint64_t me = prio(tids, t);
         for (uint32_t s = 0; s < S; s++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
             if (!op_valid(meta[i], row, rows, fields)) continue;
             if (KIND(meta[i]) == K_READ) {
-                if (tid > reg_w[row]) raw++;
+                if (me > reg_w[row]) raw++;
             } else {
-                if (tid > reg_r[row]) war++;
-                if (tid > reg_w[row]) waw++;
+                if (me > reg_r[row]) war++;
+                if (me > reg_w[row]) waw++;
             }
         }
         conflicts[t] = (raw & 0xffu) | ((war & 0xffu) << 8) | ((waw & 0xffu) << 16);
@@ -125,30 +140,37 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
 
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
-                    const uint32_t *tids, uint32_t values, uint32_t *reg_scratch,
+                    const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
                     oracle_kv_result *res)
 {
-    uint32_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
+    uint64_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
     uint32_t errors = 0;
-    /* clear_LOG: every register back to 0xFFFFFFFF (YCSB/0/Database.h:45-50) */
-    memset(reg_scratch, 0xff, (size_t)rows * 2 * sizeof(uint32_t));
+    /* clear_LOG: every register back to "no registrant" (YCSB/0/Database.h:
+     * 45-50 writes 0xFFFFFFFF; here the all-ones priority pair) */
+    memset(reg_scratch, 0xff, (size_t)rows * 2 * sizeof(uint64_t));
 
-    /* execute: register, then stage read fields from the snapshot */
+    /* execute: register */
     oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w);
     for (uint64_t i = 0; i < (uint64_t)S * B; i++) {
         uint32_t k = KIND(meta[i]);
-        if (!op_valid(meta[i], keys[i], rows, fields)) {
-            if (k == K_READ || k == K_WRITE) errors++;
-            continue;
-        }
-        if (k == K_READ && res->read_values) {
-            const uint32_t *src = table + (keys[i] * fields + FIELD(meta[i])) * (uint64_t)field_words;
-            memcpy(res->read_values + i * field_words, src, field_words * sizeof(uint32_t));
-        }
+        if (!op_valid(meta[i], keys[i], rows, fields) && (k == K_READ || k == K_WRITE)) errors++;
     }
 
     oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts);
     res->commit_count = oracle_commit(B, res->conflicts, res->commit_bitmap);
+
+    /* committed reads see the snapshot: stage them before any write-back */
+    if (res->read_values) {
+        for (uint32_t s = 0; s < S; s++) {
+            for (uint32_t t = 0; t < B; t++) {
+                uint64_t i = (uint64_t)s * B + t;
+                if (!((res->commit_bitmap[t / 64] >> (t % 64)) & 1ull)) continue;
+                if (KIND(meta[i]) != K_READ || !op_valid(meta[i], keys[i], rows, fields)) continue;
+                const uint32_t *src = table + (keys[i] * fields + FIELD(meta[i])) * (uint64_t)field_words;
+                memcpy(res->read_values + i * field_words, src, field_words * sizeof(uint32_t));
+            }
+        }
+    }
     oracle_kv_writeback(rows, fields, field_words, table, B, S, keys, meta, tids,
                         res->commit_bitmap, values);
     res->range_errors = errors;

@@ -32,7 +32,7 @@ def _load():
     lib.oracle_kv_epoch.restype = ctypes.c_int
     lib.oracle_kv_epoch.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                     ctypes.c_uint32, ctypes.c_uint32, u64p, u32p, u32p, ctypes.c_uint32,
-                                    u32p, ctypes.POINTER(KvResult)]
+                                    u64p, ctypes.POINTER(KvResult)]
     lib.KvResult = KvResult
     return lib
 
@@ -82,10 +82,10 @@ def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values
     bitmap = np.zeros(max(1, (B + 63) // 64), dtype=np.uint64)
     conflicts = np.zeros(max(B, 1), dtype=np.uint32)
     reads = np.zeros(max(B * S * field_words, 1), dtype=np.uint32) if want_reads else None
-    regs = np.empty(rows * 2, dtype=np.uint32)
+    regs = np.empty(rows * 2, dtype=np.uint64)
     res = _lib.KvResult(_u64p(bitmap), _u32p(conflicts), _u32p(reads) if want_reads else None, 0, 0)
     rc = _lib.oracle_kv_epoch(rows, fields, field_words, _u32p(table), B, S, _u64p(keys), _u32p(meta),
-                              _u32p(tids), values, _u32p(regs), ctypes.byref(res))
+                              _u32p(tids), values, _u64p(regs), ctypes.byref(res))
     assert rc == 0
     return KvEpoch(bitmap[: (B + 63) // 64], conflicts[:B],
                    reads[: B * S * field_words] if want_reads else None,

@@ -76,15 +76,16 @@ def ops_of(keys, meta, B, S, rows, fields):
 
 
 def pairwise_conflicts(keys, meta, tids, B, S, rows, fields):
-    """raw/war/waw by direct pairwise comparison (no registers)."""
+    """raw/war/waw by direct pairwise comparison (no registers).  Priority
+    is (TID, batch position): lower wins."""
     ops = ops_of(keys, meta, B, S, rows, fields)
     readers, writers = {}, {}
     for t in range(B):
         for _, kind, row, _ in ops[t]:
-            (readers if kind == 1 else writers).setdefault(row, []).append(int(tids[t]))
+            (readers if kind == 1 else writers).setdefault(row, []).append((int(tids[t]), t))
     res = np.zeros((B, 3), dtype=np.uint32)
     for t in range(B):
-        me = int(tids[t])
+        me = (int(tids[t]), t)
         for _, kind, row, _ in ops[t]:
             lower_w = any(u < me for u in writers.get(row, []))
             lower_r = any(u < me for u in readers.get(row, []))

@@ -1,7 +1,8 @@
 """Key-value parity tests: the HIP path (through the C ABI) against the oracle.
 
-Bar: bit-exact commit bitmap, conflict counters, staged read values and final
-table on the same seeded batch, and identical results across repeated runs.
+Bar: bit-exact commit bitmap, conflict counters, read values of committed
+transactions and final table on the same seeded batch, and identical results
+across repeated runs.
 """
 import numpy as np
 import pytest
@@ -15,9 +16,10 @@ pytestmark = pytest.mark.gpu
 KAT = kc.load_kat()
 
 
-def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW):
+def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True):
     return L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
-                     max_txns=max_txns, kv_max_slots=max_slots, values=values)
+                     max_txns=max_txns, kv_max_slots=max_slots, values=values,
+                     flags=L.FLAG_CONFLICT_COUNTS if counts else 0)
 
 
 def _gpu_epoch(ctx, keys, meta, tids, S, device=False):
@@ -28,26 +30,41 @@ def _gpu_epoch(ctx, keys, meta, tids, S, device=False):
         ctx.kv_submit(keys, meta, tids, S)
     ctx.epoch_run(L.PHASE_ALL)
     ctx.sync()
-    out = dict(bitmap=ctx.commit_bitmap(), conflicts=ctx.conflict_counters(), reads=ctx.kv_read_values(),
-               count=ctx.commit_count(), table=ctx.kv_table_read())
+    counts = bool(ctx.config.flags & L.FLAG_CONFLICT_COUNTS)
+    out = dict(bitmap=ctx.commit_bitmap(), conflicts=ctx.conflict_counters() if counts else None,
+               reads=ctx.kv_read_values(), count=ctx.commit_count(), table=ctx.kv_table_read())
     if device:
         b.free()
     return out
 
 
-def _assert_reads(gpu_reads, ora_reads, meta, fw):
-    """Compare staged read values on read slots only."""
-    is_read = np.repeat((meta & 3) == 1, fw)
-    np.testing.assert_array_equal(gpu_reads[is_read], ora_reads[is_read])
+def _assert_reads(gpu_reads, ora_reads, meta, tids, fw, bitmap):
+    """Read values must match on the read slots of committed transactions."""
+    B = len(tids)
+    S = meta.size // max(B, 1)
+    committed = np.unpackbits(bitmap.view(np.uint8), bitorder="little")[:B].astype(bool)
+    mask = ((meta.reshape(S, B) & 3) == 1) & committed[None, :]
+    mask = np.repeat(mask.reshape(-1), fw)
+    np.testing.assert_array_equal(gpu_reads[mask], ora_reads[mask])
+
+
+def _assert_epoch(g, ora, meta, tids, fw):
+    np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
+    if g["conflicts"] is not None:
+        np.testing.assert_array_equal(g["conflicts"], ora.conflicts)
+    assert g["count"] == ora.commit_count
+    np.testing.assert_array_equal(g["table"], ora.table)
+    _assert_reads(g["reads"], ora.read_values, meta, tids, fw, ora.bitmap)
 
 
 @pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
-def test_gpu_known_answers(gpu, case):
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_gpu_known_answers(gpu, case, counts):
     rows, fields, fw = case["rows"], case["fields"], case["field_words"]
     keys, meta, tids, S = kc.case_arrays(case)
     values = L.VALUES_RMW if case["values"] == "rmw" else L.VALUES_COPYBACK
     B = len(tids)
-    with _ctx(rows, fields, fw, max(B, 1), S, values) as ctx:
+    with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts) as ctx:
         ctx.kv_table_load(kc.initial_table(rows, fields, fw))
         ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run(L.PHASE_ALL)
@@ -59,7 +76,12 @@ def test_gpu_known_answers(gpu, case):
         else:
             ctx.sync()
         assert kc.bitmap_to_list(ctx.commit_bitmap(), B) == exp["commit"]
-        assert kc.unpack_conflicts(ctx.conflict_counters()).tolist() == exp["conflicts"]
+        if counts:
+            assert kc.unpack_conflicts(ctx.conflict_counters()).tolist() == exp["conflicts"]
+        else:
+            with pytest.raises(L.LtpgError) as ei:
+                ctx.conflict_counters()
+            assert ei.value.code == L.ERR_UNSUPPORTED
         assert ctx.commit_count() == sum(exp["commit"])
         np.testing.assert_array_equal(ctx.kv_table_read(), kc.expected_table(case))
         reads = ctx.kv_read_values()
@@ -71,30 +93,29 @@ def test_gpu_known_answers(gpu, case):
 CASES = [
     # rows, fields, fw, B, S, theta, read_pct, min_ops, seed
     (1000, 4, 1, 5000, 8, 0.9, 50, 8, 1),
-    (50_000, 10, 25, 20_000, 16, 0.99, 50, 16, 2),
+    (50_000, 10, 25, 20_000, 16, 0.99, 50, 16, 2),   # 100-byte fields, stride 64 words
     (50_000, 10, 25, 20_000, 16, 0.6, 50, 16, 3),
-    (4096, 3, 7, 777, 64, 0.8, 30, 1, 4),        # ragged 1..64 ops, odd field width
-    (100_000, 2, 80, 3000, 5, 0.5, 70, 2, 5),     # fields wider than 64 words
-    (100, 2, 1, 65, 3, 0.0, 50, 3, 6),            # uniform keys, partial last wave
+    (4096, 3, 7, 777, 64, 0.8, 30, 1, 4),            # ragged 1..64 ops, odd field width
+    (100_000, 2, 80, 3000, 5, 0.5, 70, 2, 5),        # fields wider than 64 words
+    (100, 2, 1, 65, 3, 0.0, 50, 3, 6),               # uniform keys, partial last wave
+    (20_000, 3, 16, 4000, 10, 0.7, 50, 10, 7),       # exactly 64-byte fields
 ]
 
 
 @pytest.mark.parametrize("rows,fields,fw,B,S,theta,read_pct,min_ops,seed", CASES)
-@pytest.mark.parametrize("device", [False, True])
-def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct, min_ops, seed, device):
+@pytest.mark.parametrize("device", [False, True], ids=["host-batch", "device-batch"])
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct, min_ops, seed, device,
+                                   counts):
     keys, meta, tids = L.kv_generate(seed=seed, rows=rows, fields=fields, n_txns=B, n_slots=S,
                                      min_ops=min_ops, max_ops=S, read_pct=read_pct, theta=theta)
     table0 = np.random.default_rng(seed).integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64)
     table0 = table0.astype(np.uint32)
     ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S, values=1)
-    with _ctx(rows, fields, fw, B, S) as ctx:
+    with _ctx(rows, fields, fw, B, S, counts=counts) as ctx:
         ctx.kv_table_load(table0)
         g = _gpu_epoch(ctx, keys, meta, tids, S, device=device)
-    np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
-    np.testing.assert_array_equal(g["conflicts"], ora.conflicts)
-    assert g["count"] == ora.commit_count
-    np.testing.assert_array_equal(g["table"], ora.table)
-    _assert_reads(g["reads"], ora.read_values, meta, fw)
+    _assert_epoch(g, ora, meta, tids, fw)
 
 
 def test_gpu_config1_reference_table(gpu):
@@ -110,32 +131,48 @@ def test_gpu_config1_reference_table(gpu):
         np.testing.assert_array_equal(table0, ref)
         g = _gpu_epoch(ctx, keys, meta, tids, S)
     ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S, values=1)
-    np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
-    np.testing.assert_array_equal(g["conflicts"], ora.conflicts)
-    np.testing.assert_array_equal(g["table"], ora.table)
-    _assert_reads(g["reads"], ora.read_values, meta, fw)
+    _assert_epoch(g, ora, meta, tids, fw)
     assert 0 < ora.commit_count < B
 
 
+def test_gpu_equal_tids_ordered_by_position(gpu):
+    """Equal TIDs (outside the reference's contract) follow the build's
+    (TID, batch position) priority, on both check forms."""
+    rows, fields, fw, B, S = 500, 2, 2, 6000, 6
+    keys, meta, _ = L.kv_generate(seed=31, rows=rows, fields=fields, n_txns=B, n_slots=S, theta=0.8)
+    tids = (np.random.default_rng(31).integers(0, 50, size=B)).astype(np.uint32)
+    table0 = np.arange(rows * fields * fw, dtype=np.uint32)
+    ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S)
+    for counts in (True, False):
+        with _ctx(rows, fields, fw, B, S, counts=counts) as ctx:
+            ctx.kv_table_load(table0)
+            g = _gpu_epoch(ctx, keys, meta, tids, S)
+        _assert_epoch(g, ora, meta, tids, fw)
+
+
 def test_gpu_epochs_do_not_leak_registers(gpu):
-    """Registers are epoch-tagged instead of cleared: consecutive epochs must
-    behave exactly like the oracle's fresh 0xFFFFFFFF registers."""
-    rows, fields, fw, B, S = 2000, 3, 2, 4000, 6
+    """Registers are epoch-tagged instead of cleared: consecutive epochs,
+    including the refill when the 8-bit epoch counter wraps, must behave
+    exactly like the oracle's fresh registers."""
+    rows, fields, fw, B, S = 300, 3, 2, 400, 6
     rng = np.random.default_rng(11)
     table = rng.integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64).astype(np.uint32)
-    with _ctx(rows, fields, fw, B, S) as ctx:
+    with _ctx(rows, fields, fw, B, S, counts=False) as ctx:
         ctx.kv_table_load(table)
-        for e in range(5):
-            keys, meta, tids = L.kv_generate(seed=100 + e, rows=rows, fields=fields, n_txns=B - 500 * e,
-                                             n_slots=S, theta=0.9, tid_base=(7 - e) * 10000)
-            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S)
-            g = _gpu_epoch(ctx, keys, meta, tids, S)
-            np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
-            np.testing.assert_array_equal(g["conflicts"], ora.conflicts)
-            np.testing.assert_array_equal(g["table"], ora.table)
+        for e in range(300):
+            n = B - (e % 7) * 40
+            keys, meta, tids = L.kv_generate(seed=100 + e, rows=rows, fields=fields, n_txns=n, n_slots=S,
+                                             theta=0.9, tid_base=(e % 5) * 1000)
+            ctx.kv_submit(keys, meta, tids, S)
+            ctx.epoch_run()
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, want_reads=False)
+            if e % 37 == 0 or 250 <= e <= 260:
+                np.testing.assert_array_equal(ctx.commit_bitmap(), ora.bitmap, err_msg=f"epoch {e}")
+                np.testing.assert_array_equal(ctx.kv_table_read(), ora.table, err_msg=f"epoch {e}")
             table = ora.table
-        # an explicit reset between two runs of one batch
-        ctx.epoch_reset()
+            if e == 100:
+                ctx.epoch_reset()  # an extra reset between epochs changes nothing
+        np.testing.assert_array_equal(ctx.kv_table_read(), table)
 
 
 def test_gpu_duplicate_keys_in_txn(gpu):
@@ -146,11 +183,11 @@ def test_gpu_duplicate_keys_in_txn(gpu):
                                      distinct_keys=False)
     table0 = np.arange(rows * fields * fw, dtype=np.uint32)
     ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S)
-    with _ctx(rows, fields, fw, B, S) as ctx:
-        ctx.kv_table_load(table0)
-        g = _gpu_epoch(ctx, keys, meta, tids, S)
-    np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
-    np.testing.assert_array_equal(g["table"], ora.table)
+    for counts in (True, False):
+        with _ctx(rows, fields, fw, B, S, counts=counts) as ctx:
+            ctx.kv_table_load(table0)
+            g = _gpu_epoch(ctx, keys, meta, tids, S)
+        _assert_epoch(g, ora, meta, tids, fw)
 
 
 def test_gpu_copyback_keeps_table(gpu):
@@ -165,7 +202,7 @@ def test_gpu_copyback_keeps_table(gpu):
     np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
 
 
-def test_gpu_column_major_roundtrip(gpu):
+def test_gpu_table_layout_roundtrips(gpu):
     rows, fields = 1234, 10
     colmajor = np.arange(rows * fields, dtype=np.uint32)  # column c of row r at c*rows + r
     with _ctx(rows, fields, 1, 64, 4) as ctx:
@@ -173,6 +210,16 @@ def test_gpu_column_major_roundtrip(gpu):
         rowmajor = ctx.kv_table_read(layout=L.LAYOUT_ROW_MAJOR)
         np.testing.assert_array_equal(rowmajor, colmajor.reshape(fields, rows).T.reshape(-1))
         np.testing.assert_array_equal(ctx.kv_table_read(layout=L.LAYOUT_COLUMN_MAJOR), colmajor)
+    # strided device layout (100-byte fields start every 128 bytes) in chunks
+    rows, fields, fw = 300_000, 3, 25
+    dense = np.random.default_rng(5).integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64)
+    dense = dense.astype(np.uint32)
+    with _ctx(rows, fields, fw, 64, 4) as ctx:
+        ctx.kv_table_load(dense)
+        np.testing.assert_array_equal(ctx.kv_table_read(), dense)
+        ctx.kv_fill_reference()
+        ref = np.repeat(np.tile(np.arange(fields, dtype=np.uint32), rows), fw)
+        np.testing.assert_array_equal(ctx.kv_table_read(), ref)
 
 
 def test_gpu_out_of_range_keys_reported(gpu):
@@ -198,6 +245,9 @@ def test_gpu_rejects_bad_shapes(gpu):
     with pytest.raises(L.LtpgError) as ei:
         _ctx(10, 2, 1, 64, 65)
     assert ei.value.code == L.ERR_INVALID
+    with pytest.raises(L.LtpgError) as ei:
+        _ctx(10, 2, 1, (1 << 24) + 1, 4)
+    assert ei.value.code == L.ERR_INVALID
     with _ctx(10, 2, 1, 64, 4) as ctx:
         keys, meta, tids = L.kv_generate(seed=1, rows=10, fields=2, n_txns=65, n_slots=4, theta=0.0)
         with pytest.raises(L.LtpgError):
@@ -210,19 +260,19 @@ def test_gpu_rejects_bad_shapes(gpu):
 @pytest.mark.parametrize("theta", [0.6, 0.99])
 def test_gpu_config2_full_size(gpu, theta):
     """10M rows x 10 fields x 100 B, 2^20 txns x 16 ops, 50/50: bit-exact
-    against the oracle at full size, and identical across two GPU runs."""
+    against the oracle at full size, identical across two GPU runs, and the
+    fast check agrees with the counting check."""
     rows, fields, fw, B, S = 10_000_000, 10, 25, 1 << 20, 16
     keys, meta, tids = L.kv_generate(seed=0x4C545047, rows=rows, fields=fields, n_txns=B, n_slots=S,
                                      theta=theta)
-    with _ctx(rows, fields, fw, B, S) as ctx:
+    with _ctx(rows, fields, fw, B, S, counts=False) as ctx:
         ctx.kv_fill_reference()
         table0 = ctx.kv_table_read()
         dev = L.DeviceKvBatch(ctx, keys, meta, tids, S)
         ctx.kv_submit_device(dev)
         ctx.epoch_run()
         ctx.sync()
-        bm1, cf1, tab1 = ctx.commit_bitmap(), ctx.conflict_counters(), ctx.kv_table_read()
-        reads1 = ctx.kv_read_values()
+        bm1, tab1, reads1 = ctx.commit_bitmap(), ctx.kv_table_read(), ctx.kv_read_values()
         # second run of the same batch on the same starting table
         ctx.kv_fill_reference()
         ctx.kv_submit_device(dev)
@@ -233,6 +283,12 @@ def test_gpu_config2_full_size(gpu, theta):
         dev.free()
     ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S, values=1)
     np.testing.assert_array_equal(bm1, ora.bitmap)
-    np.testing.assert_array_equal(cf1, ora.conflicts)
     np.testing.assert_array_equal(tab1, ora.table)
-    _assert_reads(reads1, ora.read_values, meta, fw)
+    _assert_reads(reads1, ora.read_values, meta, tids, fw, ora.bitmap)
+    del tab1, table0
+    with _ctx(rows, 1, 1, B, S, counts=True) as ctx:  # counters only need the registers
+        ctx.kv_submit(keys, (meta & 0xFF).astype(np.uint32), tids, S)
+        ctx.epoch_run(L.PHASE_EXECUTE | L.PHASE_CHECK)
+        ctx.sync()
+        np.testing.assert_array_equal(ctx.commit_bitmap(), ora.bitmap)
+        np.testing.assert_array_equal(ctx.conflict_counters(), ora.conflicts)

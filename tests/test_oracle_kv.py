@@ -57,12 +57,12 @@ def test_oracle_matches_pairwise_and_is_serializable(seed):
                                      fields, fw, committed)
     assert ok, msg
     np.testing.assert_array_equal(ep.table, final)
-    # reads return the snapshot
+    # committed reads return the snapshot
     snap = kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)
     for s in range(S):
         for t in range(B):
             m = int(meta[s * B + t])
-            if m & 3 == 1:
+            if m & 3 == 1 and committed[t]:
                 i = s * B + t
                 np.testing.assert_array_equal(ep.read_values[i * fw:(i + 1) * fw],
                                               snap[int(keys[i]), (m >> 8) & 0xFFFF])
@@ -93,6 +93,23 @@ def test_oracle_bruteforce_all_tid_permutations():
     assert checked > 1000
 
 
+@pytest.mark.parametrize("seed", range(10))
+def test_oracle_equal_tids_break_ties_by_position(seed):
+    """Equal TIDs are outside the reference's contract; the build orders
+    them by batch position, and the pins still hold."""
+    rng = np.random.default_rng(1000 + seed)
+    B, S, rows, fields, fw = 30, 3, 6, 2, 1
+    keys, meta, _ = kc.random_batch(rng, B, S, rows, fields)
+    tids = rng.integers(0, 4, size=B).astype(np.uint32)  # many ties
+    ep = _run(rows, fields, fw, keys, meta, tids, S)
+    conf = kc.unpack_conflicts(ep.conflicts)
+    np.testing.assert_array_equal(conf, kc.pairwise_conflicts(keys, meta, tids, B, S, rows, fields))
+    ok, msg, final = kc.serial_check(kc.initial_table(rows, fields, fw), keys, meta, tids, B, S, rows, fields,
+                                     fw, kc.bitmap_to_list(ep.bitmap, B))
+    assert ok, msg
+    np.testing.assert_array_equal(ep.table, final)
+
+
 def test_oracle_copyback_never_changes_table():
     rng = np.random.default_rng(7)
     rows, fields, fw, B, S = 16, 4, 3, 64, 6
@@ -107,20 +124,21 @@ def _rules_with_mistake(keys, meta, tids, B, S, rows, fields, fw, table0, mistak
     """A CPU copy of the rules with one planted mistake (test-only)."""
     ops = kc.ops_of(keys, meta, B, S, rows, fields)
     R, W = {}, {}
+    NONE = (1 << 64,)
     for t in range(B):
         for _, kind, row, _ in ops[t]:
             reg = R if (kind == 1) != (mistake == "swap_registers") else W
-            reg[row] = min(reg.get(row, 0xFFFFFFFF), int(tids[t]))
+            reg[row] = min(reg.get(row, NONE), (int(tids[t]), t))
     conf = np.zeros((B, 3), dtype=np.uint32)
     gt = (lambda a, b: a >= b) if mistake == "ge" else (lambda a, b: a > b)
     for t in range(B):
-        me = int(tids[t])
+        me = (int(tids[t]), t)
         for _, kind, row, _ in ops[t]:
             if kind == 1:
-                conf[t, 0] += gt(me, W.get(row, 0xFFFFFFFF))
+                conf[t, 0] += gt(me, W.get(row, NONE))
             else:
-                conf[t, 1] += gt(me, (W if mistake == "war_from_w" else R).get(row, 0xFFFFFFFF))
-                conf[t, 2] += gt(me, W.get(row, 0xFFFFFFFF))
+                conf[t, 1] += gt(me, (W if mistake == "war_from_w" else R).get(row, NONE))
+                conf[t, 2] += gt(me, W.get(row, NONE))
     raw, war, waw = conf[:, 0], conf[:, 1], conf[:, 2]
     if mistake == "waw_only":
         commit = waw == 0
