@@ -556,8 +556,11 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
                                      ctx->counters, s));
     }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[3], s));
-    // committed reads see the snapshot: deliver them before any write
-    if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_readout(kv, kv.cur, s));
+    // Committed reads see the snapshot, so they are delivered before any
+    // write.  The fast CHECK already gathered them; the counting CHECK
+    // leaves them to a read-out pass here.
+    if ((phase_mask & LTPG_PHASE_WRITEBACK) && (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS))
+        LTPG_HIP_TRY(launch_kv_readout(kv, kv.cur, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[4], s));
     if (phase_mask & LTPG_PHASE_WRITEBACK) LTPG_HIP_TRY(launch_kv_writeback(kv, kv.cur, ctx->cfg.values, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[5], s));

@@ -49,9 +49,11 @@ constexpr unsigned long long kPosMask = 0xFFFFFFull;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-__device__ __forceinline__ unsigned long long load_relaxed(const unsigned long long *p)
+// Probe of a register before the atomic: a work-group-scope relaxed load,
+// served by L2 (a stale value is harmless, see kv_execute_kernel).
+__device__ __forceinline__ unsigned long long probe(const unsigned long long *p)
 {
-    return __atomic_load_n(p, __ATOMIC_RELAXED);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Copy (or read-modify-write) the fields of the active ops of a wave.
@@ -237,43 +239,52 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
              : 0ull;
     bool waw = false;
     uint32_t bad = 0;
-    for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
-        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
-        if (!o.valid) continue;
-        unsigned long long *r = &regs[2 * o.row + (o.kind == LTPG_KV_WRITE ? 1 : 0)];
-        const unsigned long long cur = load_relaxed(r);
-        if (o.kind == LTPG_KV_WRITE) {
-            if (cur < me) {
-                waw = true;  // a higher-priority writer is already registered
-            } else if (cur > me) {
-                const unsigned long long old = atomicMin(r, me);
-                if (old < me)
-                    waw = true;
-                else if (old > me && (uint32_t)(old >> kTagShift) == tag)
-                    doomed[old & kPosMask] = epoch;  // displaced writer
+    // Four slots per step: their op loads, then their register probes, are
+    // issued back to back, so each lane keeps several misses in flight.
+    constexpr uint32_t U = 4;
+    for (uint32_t s0 = 0; s0 < S; s0 += U) {
+        Op o[U];
+        unsigned long long cur[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            o[u] = load_op(keys, meta, (uint64_t)(s0 + u) * B + t, live && s0 + u < S, rows, fields);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            cur[u] = o[u].valid ? probe(&regs[2 * o[u].row + (o[u].kind == LTPG_KV_WRITE ? 1 : 0)]) : 0ull;
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            bad += ((o[u].kind == LTPG_KV_READ || o[u].kind == LTPG_KV_WRITE) && !o[u].valid) ? 1u : 0u;
+            if (!o[u].valid) continue;
+            unsigned long long *r = &regs[2 * o[u].row + (o[u].kind == LTPG_KV_WRITE ? 1 : 0)];
+            if (o[u].kind == LTPG_KV_WRITE) {
+                if (cur[u] < me) {
+                    waw = true;  // a higher-priority writer is already registered
+                } else if (cur[u] > me) {
+                    const unsigned long long old = atomicMin(r, me);
+                    if (old < me)
+                        waw = true;
+                    else if (old > me && (uint32_t)(old >> kTagShift) == tag)
+                        doomed[old & kPosMask] = epoch;  // displaced writer
+                }
+            } else if (cur[u] > me) {
+                atomicMin(r, me);
             }
-        } else if (cur > me) {
-            atomicMin(r, me);
         }
     }
     if (waw) doomed[t] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
 }
 
-// CHECK + COMMIT (check_r / check_w_0 / check_w_1 and __CHECK,
-// YCSB/0/Execute.h:109-175, Genericfunction.h:101-144; writeback_0,
-// Execute.h:184-191).  Fast form: transactions doomed by waw in EXECUTE
-// abort without touching a register; the others read W for their reads
-// (raw) and R for their writes (war) and stop as soon as both are set.
-// Counting form (kCounts): every transaction counts raw / war / waw over all
-// its ops, the reference's counters, for parity checks of the counters.
-template <bool kCounts>
-__global__ __launch_bounds__(kBlock) void kv_check_kernel(
+// CHECK + COMMIT, counting form (check_r / check_w_0 / check_w_1 and
+// __CHECK, YCSB/0/Execute.h:109-175, Genericfunction.h:101-144; writeback_0,
+// Execute.h:184-191): every transaction counts raw (reads whose W register
+// holds a higher priority), war (writes vs R) and waw (writes vs W) over all
+// its ops -- the reference's counters, kept for parity checks of them --
+// and commits iff waw == 0 && (raw == 0 || war == 0).
+__global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    const unsigned long long *__restrict__ regs, uint32_t tag,
-    const uint8_t *__restrict__ doomed, uint8_t epoch, uint64_t *__restrict__ bitmap,
+    const unsigned long long *__restrict__ regs, uint32_t tag, uint64_t *__restrict__ bitmap,
     uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -282,40 +293,20 @@ __global__ __launch_bounds__(kBlock) void kv_check_kernel(
     const unsigned long long me =
         live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
              : 0ull;
-    bool commit;
-    if constexpr (kCounts) {
-        uint32_t raw = 0, war = 0, waw = 0;
-        for (uint32_t s = 0; s < S; ++s) {
-            const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
-            if (!o.valid) continue;
-            if (o.kind == LTPG_KV_READ) {
-                raw += regs[2 * o.row + 1] < me ? 1u : 0u;
-            } else {
-                const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * o.row);
-                war += rw.x < me ? 1u : 0u;
-                waw += rw.y < me ? 1u : 0u;
-            }
+    uint32_t raw = 0, war = 0, waw = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
+        if (!o.valid) continue;
+        if (o.kind == LTPG_KV_READ) {
+            raw += regs[2 * o.row + 1] < me ? 1u : 0u;
+        } else {
+            const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * o.row);
+            war += rw.x < me ? 1u : 0u;
+            waw += rw.y < me ? 1u : 0u;
         }
-        commit = live && waw == 0 && (raw == 0 || war == 0);
-        if (live) conflicts[t] = raw | (war << 8) | (waw << 16);
-    } else {
-        const bool alive = live && doomed[t] != epoch;
-        bool raw = false, war = false;
-        if (__ballot(alive)) {
-            for (uint32_t s = 0; s < S; ++s) {
-                const bool todo = alive && !(raw && war);
-                if (!__ballot(todo)) break;
-                if (!todo) continue;
-                const Op o = load_op(keys, meta, (uint64_t)s * B + t, true, rows, fields);
-                if (!o.valid) continue;
-                if (o.kind == LTPG_KV_READ)
-                    raw |= regs[2 * o.row + 1] < me;
-                else
-                    war |= regs[2 * o.row] < me;
-            }
-        }
-        commit = alive && !(raw && war);
     }
+    const bool commit = live && waw == 0 && (raw == 0 || war == 0);
+    if (live) conflicts[t] = raw | (war << 8) | (waw << 16);
     const unsigned long long word = __ballot(commit);
     const uint32_t wave_base = t - (uint32_t)lane;
     if (lane == 0 && wave_base < B) {
@@ -356,34 +347,146 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     }
 }
 
+// Block-local compaction.  A block owns kChunk consecutive transactions;
+// the transactions that still have work (alive ones for CHECK, committed
+// ones for WRITEBACK) are gathered into an LDS list so that the following
+// rounds keep every lane busy whatever the abort rate.
+constexpr uint32_t kChunk = 1024;
+
+__device__ __forceinline__ void list_append(bool pick, uint32_t t, uint32_t *s_list, uint32_t *s_n)
+{
+    const int lane = lane_id();
+    const unsigned long long m = __ballot(pick);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(s_n, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (pick) s_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+// CHECK + COMMIT + read-out, fast form (rules as kv_check_counting_kernel).
+// Transactions doomed by waw in EXECUTE abort without any further work;
+// the others need raw (reads vs W) and war (writes vs R) only.
+// Lanes are grouped G per transaction, one lane per slot, so all ops of a
+// surviving transaction are probed at once; raw / war are OR-reduced over
+// the group with ballots.  A committed transaction gathers its read fields
+// right away: no write-back has run yet, so the table is the snapshot
+// (read_1 / __READ, YCSB/0/Execute.h:27-37).  Commit bits collect in LDS and
+// leave as whole bitmap words.
+template <int G>
+__global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, uint32_t tag,
+    const uint8_t *__restrict__ doomed, uint8_t epoch, const uint32_t *__restrict__ table,
+    uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
+    DeviceCounters *__restrict__ ctr)
+{
+    __shared__ uint32_t s_list[kChunk];
+    __shared__ unsigned long long s_bits[kChunk / 64];
+    __shared__ uint32_t s_n, s_commits;
+    const int lane = lane_id();
+    const uint32_t chunk0 = blockIdx.x * kChunk;
+    if (threadIdx.x == 0) s_n = s_commits = 0;
+    if (threadIdx.x < kChunk / 64) s_bits[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kChunk / kBlock; ++k) {
+        const uint32_t t = chunk0 + k * kBlock + threadIdx.x;
+        list_append(t < B && doomed[t] != epoch, t, s_list, &s_n);
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    constexpr uint32_t PER = 64 / G;  // transactions per wave per round
+    const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
+    const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (grp * G);
+    uint32_t commits = 0;
+    for (uint32_t j0 = (threadIdx.x / 64) * PER; j0 < n; j0 += (kBlock / 64) * PER) {
+        const uint32_t j = j0 + grp;
+        const bool has = j < n;
+        const uint32_t t = has ? s_list[j] : 0u;
+        const unsigned long long me =
+            ((unsigned long long)tag << kTagShift) | ((unsigned long long)(has ? tids[t] : 0u) << kTidShift) | t;
+        const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
+        const bool raw = o.valid && o.kind == LTPG_KV_READ && regs[2 * o.row + 1] < me;
+        const bool war = o.valid && o.kind == LTPG_KV_WRITE && regs[2 * o.row] < me;
+        const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
+        const unsigned long long rd = __ballot(commit && o.valid && o.kind == LTPG_KV_READ);
+        if (rd) {
+            const uint64_t src = (o.row * fields + o.field) * fsw;
+            for (uint32_t k = 0; k < S && k < (uint32_t)G; ++k) {
+                const int from = (int)(grp * G + k);
+                const uint64_t sk = __shfl(src, from);
+                if ((rd >> from) & 1ull) {
+                    const uint64_t dk = ((uint64_t)k * B + t) * fw;
+                    for (uint32_t w = gl; w < fw; w += G) read_values[dk + w] = table[sk + w];
+                }
+            }
+        }
+        if (commit && gl == 0) {
+            atomicOr(&s_bits[(t - chunk0) / 64], 1ull << ((t - chunk0) % 64));
+            ++commits;
+        }
+    }
+    if (commits) atomicAdd(&s_commits, commits);
+    __syncthreads();
+    const uint32_t nwords = (B + 63) / 64;
+    if (threadIdx.x < kChunk / 64 && chunk0 / 64 + threadIdx.x < nwords)
+        bitmap[chunk0 / 64 + threadIdx.x] = s_bits[threadIdx.x];
+    if (threadIdx.x == 0) {
+        const uint32_t live_n = (B - chunk0) < kChunk ? (B - chunk0) : kChunk;
+        if (s_commits) {
+            atomicAdd(&ctr->commits[LTPG_TYPE_KV], s_commits);
+            atomicAdd(&ctr->commits_total[LTPG_TYPE_KV], (unsigned long long)s_commits);
+        }
+        atomicAdd(&ctr->attempts_total[LTPG_TYPE_KV], (unsigned long long)live_n);
+    }
+}
+
 // WRITEBACK: committed transactions apply their write slots in slot order
 // (writeback_w / __WRITEBACK, YCSB/0/Execute.h:193-206,
 // Genericfunction.h:145-158).  Value: word + delta, delta = TID | 1 for RMW
 // and 0 for copy-back.  The committed writer of a row is unique (the
 // minimum W registrant), so reading the table here reads the snapshot.
-template <int LP>
+// Lanes are grouped G per committed transaction (block-local compaction);
+// lane gl of a group always owns words gl, gl + G, ... of every field the
+// transaction writes, so two writes of one transaction to one field are
+// applied by the same lane in slot order.
+template <int G>
 __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     uint32_t fw, uint32_t fsw, uint32_t *table, const uint64_t *__restrict__ bitmap,
     uint32_t values)
 {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint32_t s_list[kChunk];
+    __shared__ uint32_t s_n;
     const int lane = lane_id();
-    const uint32_t wave_base = t - (uint32_t)lane;
-    if (wave_base >= B) return;
-    const uint64_t word = bitmap[wave_base / 64];
-    if (word == 0) return;
-    // t < B guards a bitmap word left by an earlier, larger batch when this
-    // phase runs without a fresh CHECK.
-    const bool committed = t < B && ((word >> lane) & 1ull);
-    const uint32_t delta = (committed && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
-    for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, committed, rows, fields);
-        const unsigned long long mask = __ballot(o.valid && o.kind == LTPG_KV_WRITE);
-        if (mask) {
-            const uint64_t off = (o.row * fields + o.field) * fsw;
-            copy_fields<LP>(mask, off, off, delta, table, table, fw);
+    const uint32_t chunk0 = blockIdx.x * kChunk;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kChunk / kBlock; ++k) {
+        const uint32_t t = chunk0 + k * kBlock + threadIdx.x;
+        // t < B also guards a bitmap word left by an earlier, larger batch
+        // when this phase runs without a fresh CHECK
+        list_append(t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull), t, s_list, &s_n);
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    constexpr uint32_t PER = 64 / G;
+    const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
+    for (uint32_t j0 = (threadIdx.x / 64) * PER; j0 < n; j0 += (kBlock / 64) * PER) {
+        const uint32_t j = j0 + grp;
+        const bool has = j < n;
+        const uint32_t t = has ? s_list[j] : 0u;
+        const uint32_t delta = (has && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
+        const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
+        const unsigned long long wm = __ballot(o.valid && o.kind == LTPG_KV_WRITE);
+        if (!wm) continue;
+        const uint64_t off = (o.row * fields + o.field) * fsw;
+        for (uint32_t k = 0; k < S && k < (uint32_t)G; ++k) {
+            const int from = (int)(grp * G + k);
+            const uint64_t ok = __shfl(off, from);
+            if ((wm >> from) & 1ull)
+                for (uint32_t w = gl; w < fw; w += G) table[ok + w] += delta;
         }
     }
 }
@@ -484,18 +587,43 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
     return hipGetLastError();
 }
 
+int group_for(uint32_t S)
+{
+    int g = 1;
+    while ((uint32_t)g < S) g <<= 1;
+    return g;
+}
+
+#define LTPG_G_SWITCH(g, CALL)   \
+    switch (g) {                \
+    case 1: CALL(1); break;     \
+    case 2: CALL(2); break;     \
+    case 4: CALL(4); break;     \
+    case 8: CALL(8); break;     \
+    case 16: CALL(16); break;   \
+    case 32: CALL(32); break;   \
+    default: CALL(64); break;   \
+    }
+
+inline dim3 grid_chunks(uint32_t B) { return dim3((B + kChunk - 1) / kChunk); }
+
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-    if (counts)
-        hipLaunchKernelGGL(kv_check_kernel<true>, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
-                           b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed,
-                           e.epoch, kv.bitmap, kv.conflicts, ctr);
-    else
-        hipLaunchKernelGGL(kv_check_kernel<false>, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta,
-                           b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed,
-                           e.epoch, kv.bitmap, kv.conflicts, ctr);
+    if (counts) {
+        hipLaunchKernelGGL(kv_check_counting_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys,
+                           b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag,
+                           kv.bitmap, kv.conflicts, ctr);
+        return hipGetLastError();
+    }
+#define CALL(GV)                                                                                     \
+    hipLaunchKernelGGL(kv_check_fast_kernel<GV>, grid_chunks(b.n_txns), dim3(kBlock), 0, s, b.keys, \
+                       b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,     \
+                       kv.field_stride, kv.regs, e.tag, kv.doomed, e.epoch, kv.table,                \
+                       kv.read_values, kv.bitmap, ctr)
+    LTPG_G_SWITCH(group_for(b.n_slots), CALL)
+#undef CALL
     return hipGetLastError();
 }
 
@@ -516,12 +644,11 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
                                hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-    const int lp = lanes_for(kv.field_words);
-#define CALL(LPV)                                                                           \
-    hipLaunchKernelGGL(kv_writeback_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,   \
-                       b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,    \
-                       kv.field_words, kv.field_stride, kv.table, kv.bitmap, values)
-    LTPG_LP_SWITCH(lp, CALL)
+#define CALL(GV)                                                                                      \
+    hipLaunchKernelGGL(kv_writeback_kernel<GV>, grid_chunks(b.n_txns), dim3(kBlock), 0, s, b.keys,   \
+                       b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,      \
+                       kv.field_stride, kv.table, kv.bitmap, values)
+    LTPG_G_SWITCH(group_for(b.n_slots), CALL)
 #undef CALL
     return hipGetLastError();
 }
