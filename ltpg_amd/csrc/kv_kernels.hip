@@ -49,11 +49,13 @@ constexpr unsigned long long kPosMask = 0xFFFFFFull;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Probe of a register before the atomic: a work-group-scope relaxed load,
-// served by L2 (a stale value is harmless, see kv_execute_kernel).
+// Probe of a register before the atomic: a device-scope relaxed load, so it
+// sees the registrations made from every XCD so far.  (Any value would be
+// safe, see kv_execute_kernel, but a stale L2 copy makes hot rows take the
+// atomic far more often.)
 __device__ __forceinline__ unsigned long long probe(const unsigned long long *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Copy (or read-modify-write) the fields of the active ops of a wave.
@@ -349,9 +351,10 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
 
 // Block-local compaction.  A block owns kChunk consecutive transactions;
 // the transactions that still have work (alive ones for CHECK, committed
-// ones for WRITEBACK) are gathered into an LDS list so that the following
-// rounds keep every lane busy whatever the abort rate.
-constexpr uint32_t kChunk = 1024;
+// ones for WRITEBACK) are gathered into an LDS list, then processed G lanes
+// per transaction (one lane per slot), so lanes stay busy whatever the
+// abort rate and every op load of a transaction is issued at once.
+constexpr uint32_t kChunk = kBlock;
 
 __device__ __forceinline__ void list_append(bool pick, uint32_t t, uint32_t *s_list, uint32_t *s_n)
 {
@@ -363,15 +366,48 @@ __device__ __forceinline__ void list_append(bool pick, uint32_t t, uint32_t *s_l
     if (pick) s_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
 }
 
+// Field copies of a group's flagged slots.  The op of slot k lives in lane
+// grp * G + k; its field is src[src_off + w] -> dst[dst_off + w] (+ delta),
+// w < fw, and lane gl of the group handles words gl, gl + G, ...  Loads of
+// KB slots are issued before their stores, which requires that no two
+// flagged slots of one call write the same word (callers guarantee it).
+template <int G, int KB>
+__device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t src_off,
+                                           uint64_t dst_off, uint32_t delta, const uint32_t *src,
+                                           uint32_t *dst, uint32_t fw, uint32_t S, uint32_t grp,
+                                           uint32_t gl)
+{
+    for (uint32_t k0 = 0; k0 < S; k0 += KB) {
+        uint64_t so[KB], dd[KB];
+        uint32_t dl[KB];
+        bool a[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const uint32_t k = k0 + u;
+            const int from = (int)(grp * G + (k < (uint32_t)G ? k : 0u));
+            so[u] = __shfl(src_off, from);
+            dd[u] = __shfl(dst_off, from);
+            dl[u] = __shfl(delta, from);
+            a[u] = k < S && ((flags >> from) & 1ull);
+        }
+        for (uint32_t w = gl; w < fw; w += G) {
+            uint32_t v[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) v[u] = a[u] ? src[so[u] + w] + dl[u] : 0u;
+#pragma unroll
+            for (int u = 0; u < KB; ++u)
+                if (a[u]) dst[dd[u] + w] = v[u];
+        }
+    }
+}
+
 // CHECK + COMMIT + read-out, fast form (rules as kv_check_counting_kernel).
 // Transactions doomed by waw in EXECUTE abort without any further work;
-// the others need raw (reads vs W) and war (writes vs R) only.
-// Lanes are grouped G per transaction, one lane per slot, so all ops of a
-// surviving transaction are probed at once; raw / war are OR-reduced over
-// the group with ballots.  A committed transaction gathers its read fields
-// right away: no write-back has run yet, so the table is the snapshot
-// (read_1 / __READ, YCSB/0/Execute.h:27-37).  Commit bits collect in LDS and
-// leave as whole bitmap words.
+// the others need raw (reads vs W) and war (writes vs R) only, OR-reduced
+// over the transaction's lanes with ballots.  A committed transaction
+// gathers its read fields right away: no write-back has run yet, so the
+// table is the snapshot (read_1 / __READ, YCSB/0/Execute.h:27-37).  Commit
+// bits collect in LDS and leave as whole bitmap words.
 template <int G>
 __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
@@ -389,8 +425,8 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     if (threadIdx.x == 0) s_n = s_commits = 0;
     if (threadIdx.x < kChunk / 64) s_bits[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t k = 0; k < kChunk / kBlock; ++k) {
-        const uint32_t t = chunk0 + k * kBlock + threadIdx.x;
+    {
+        const uint32_t t = chunk0 + threadIdx.x;
         list_append(t < B && doomed[t] != epoch, t, s_list, &s_n);
     }
     __syncthreads();
@@ -405,22 +441,15 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint32_t t = has ? s_list[j] : 0u;
         const unsigned long long me =
             ((unsigned long long)tag << kTagShift) | ((unsigned long long)(has ? tids[t] : 0u) << kTidShift) | t;
-        const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
+        const uint64_t i = (uint64_t)gl * B + t;
+        const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
         const bool raw = o.valid && o.kind == LTPG_KV_READ && regs[2 * o.row + 1] < me;
         const bool war = o.valid && o.kind == LTPG_KV_WRITE && regs[2 * o.row] < me;
         const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
         const unsigned long long rd = __ballot(commit && o.valid && o.kind == LTPG_KV_READ);
-        if (rd) {
-            const uint64_t src = (o.row * fields + o.field) * fsw;
-            for (uint32_t k = 0; k < S && k < (uint32_t)G; ++k) {
-                const int from = (int)(grp * G + k);
-                const uint64_t sk = __shfl(src, from);
-                if ((rd >> from) & 1ull) {
-                    const uint64_t dk = ((uint64_t)k * B + t) * fw;
-                    for (uint32_t w = gl; w < fw; w += G) read_values[dk + w] = table[sk + w];
-                }
-            }
-        }
+        if (rd)
+            group_copy<G, 8>(rd, (o.row * fields + o.field) * fsw, i * fw, 0u, table, read_values, fw, S,
+                             grp, gl);
         if (commit && gl == 0) {
             atomicOr(&s_bits[(t - chunk0) / 64], 1ull << ((t - chunk0) % 64));
             ++commits;
@@ -441,20 +470,20 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     }
 }
 
-// WRITEBACK: committed transactions apply their write slots in slot order
-// (writeback_w / __WRITEBACK, YCSB/0/Execute.h:193-206,
-// Genericfunction.h:145-158).  Value: word + delta, delta = TID | 1 for RMW
-// and 0 for copy-back.  The committed writer of a row is unique (the
+// WRITEBACK: committed transactions apply their write slots (writeback_w /
+// __WRITEBACK, YCSB/0/Execute.h:193-206, Genericfunction.h:145-158).
+// Value: word + delta, delta = TID | 1 for RMW and 0 for copy-back, applied
+// per write op in slot order.  The committed writer of a row is unique (the
 // minimum W registrant), so reading the table here reads the snapshot.
-// Lanes are grouped G per committed transaction (block-local compaction);
-// lane gl of a group always owns words gl, gl + G, ... of every field the
-// transaction writes, so two writes of one transaction to one field are
-// applied by the same lane in slot order.
+// Several write ops of one transaction on one field are folded: the first
+// in slot order writes word + count * delta, the others nothing -- the same
+// result as applying them one by one, and it lets every store of a round go
+// out without ordering between them.
 template <int G>
 __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, uint32_t fsw, uint32_t *table, const uint64_t *__restrict__ bitmap,
+    uint32_t fw, uint32_t fsw, uint32_t *__restrict__ table, const uint64_t *__restrict__ bitmap,
     uint32_t values)
 {
     __shared__ uint32_t s_list[kChunk];
@@ -463,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint32_t chunk0 = blockIdx.x * kChunk;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t k = 0; k < kChunk / kBlock; ++k) {
-        const uint32_t t = chunk0 + k * kBlock + threadIdx.x;
+    {
+        const uint32_t t = chunk0 + threadIdx.x;
         // t < B also guards a bitmap word left by an earlier, larger batch
         // when this phase runs without a fresh CHECK
         list_append(t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull), t, s_list, &s_n);
@@ -477,17 +506,25 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
-        const uint32_t delta = (has && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
+        const uint32_t step = (has && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
         const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
-        const unsigned long long wm = __ballot(o.valid && o.kind == LTPG_KV_WRITE);
+        const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
+        const unsigned long long wm = __ballot(wr);
         if (!wm) continue;
         const uint64_t off = (o.row * fields + o.field) * fsw;
+        // fold duplicates: count this field among the group's write slots
+        uint32_t count = 0;
+        bool first = true;
         for (uint32_t k = 0; k < S && k < (uint32_t)G; ++k) {
             const int from = (int)(grp * G + k);
             const uint64_t ok = __shfl(off, from);
-            if ((wm >> from) & 1ull)
-                for (uint32_t w = gl; w < fw; w += G) table[ok + w] += delta;
+            if (((wm >> from) & 1ull) && ok == off) {
+                ++count;
+                first &= k >= gl;
+            }
         }
+        const unsigned long long am = __ballot(wr && first);
+        group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
     }
 }
 
