@@ -342,14 +342,17 @@ int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t
     if (rc) return rc;
     if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    DeviceCounters h;
-    LTPG_HIP_TRY(hipMemcpy(&h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost));
-    if (commits) *commits = h.commits_total[type];
-    if (attempts) *attempts = h.attempts_total[type];
-    if (reset) {
-        LTPG_HIP_TRY(hipMemset(&ctx->counters->commits_total[type], 0, sizeof(unsigned long long)));
-        LTPG_HIP_TRY(hipMemset(&ctx->counters->attempts_total[type], 0, sizeof(unsigned long long)));
+    static_assert(sizeof(DeviceCounters::totals) == kCounterStripes * 32 * 8, "stripe layout");
+    unsigned long long h[kCounterStripes][32];
+    LTPG_HIP_TRY(hipMemcpy(h, ctx->counters->totals, sizeof(h), hipMemcpyDeviceToHost));
+    uint64_t c = 0, a = 0;
+    for (int k = 0; k < kCounterStripes; k++) {
+        c += h[k][type];
+        a += h[k][3 + type];
     }
+    if (commits) *commits = c;
+    if (attempts) *attempts = a;
+    if (reset) LTPG_HIP_TRY(hipMemset(ctx->counters->totals, 0, sizeof(h)));
     return LTPG_OK;
 }
 
@@ -596,7 +599,11 @@ int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out)
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
     if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    LTPG_HIP_TRY(hipMemcpy(out, &ctx->counters->commits[type], sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t h[kCounterStripes][64];
+    LTPG_HIP_TRY(hipMemcpy(h, ctx->counters->commits, sizeof(h), hipMemcpyDeviceToHost));
+    uint32_t sum = 0;
+    for (int k = 0; k < kCounterStripes; k++) sum += h[k][type];
+    *out = sum;
     return LTPG_OK;
 }
 

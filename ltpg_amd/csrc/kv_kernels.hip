@@ -49,13 +49,14 @@ constexpr unsigned long long kPosMask = 0xFFFFFFull;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Probe of a register before the atomic: a device-scope relaxed load, so it
-// sees the registrations made from every XCD so far.  (Any value would be
-// safe, see kv_execute_kernel, but a stale L2 copy makes hot rows take the
-// atomic far more often.)
+// Probe of a register before the atomic: a system-scope relaxed load, which
+// reads the coherent copy the atomics update.  Any value would be safe (see
+// kv_execute_kernel), but on hot rows a stale cached copy sends far more
+// lanes into the contended atomic; measured on MI355X at Zipf 0.99 the
+// coherent probe is clearly the fastest.
 __device__ __forceinline__ unsigned long long probe(const unsigned long long *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Copy (or read-modify-write) the fields of the active ops of a wave.
@@ -125,6 +126,19 @@ __device__ __forceinline__ void copy_fields(unsigned long long mask, uint64_t sr
         wave_copy<LP, (LP < 8 ? LP : 8)>(mask, src_off, dst_off, delta, src_base, dst_base, fw);
 }
 
+// Commit and attempt counts go to one of kCounterStripes stripes (256 bytes
+// apart): thousands of blocks adding to one address serialize in the
+// memory-side atomic unit (4096 same-address atomics cost ~50 us on MI355X).
+__device__ __forceinline__ void add_counts(DeviceCounters *ctr, uint32_t stripe, uint32_t type,
+                                           uint32_t commits, uint32_t attempts)
+{
+    if (commits) {
+        atomicAdd(&ctr->commits[stripe][type], commits);
+        atomicAdd(&ctr->totals[stripe][type], (unsigned long long)commits);
+    }
+    if (attempts) atomicAdd(&ctr->totals[stripe][3 + type], (unsigned long long)attempts);
+}
+
 // Per-op decode shared by every phase.
 struct Op {
     uint64_t row;
@@ -133,13 +147,20 @@ struct Op {
     bool valid;
 };
 
+// Loads are unconditional on a clamped index (element 0 of a non-empty batch
+// when the lane has no op) and the result is selected afterwards: a load
+// under a lane-dependent branch sits in its own basic block and the compiler
+// waits for it before the next one, which would serialize the misses.
 __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
                                       const uint32_t *__restrict__ meta, uint64_t i, bool live,
                                       uint64_t rows, uint32_t fields)
 {
     Op o;
-    const uint32_t m = live ? meta[i] : 0u;
-    o.row = live ? keys[i] : 0ull;
+    const uint64_t j = live ? i : 0ull;
+    const uint32_t m0 = meta[j];
+    const uint64_t r0 = keys[j];
+    const uint32_t m = live ? m0 : 0u;
+    o.row = live ? r0 : 0ull;
     o.kind = m & 3u;
     o.field = (m >> 8) & 0xffffu;
     o.valid = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && o.row < rows && o.field < fields;
@@ -150,7 +171,7 @@ __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
 // epoch can be captured in a graph).
 __global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
 {
-    if (threadIdx.x < 3) ctr->commits[threadIdx.x] = 0;
+    if (threadIdx.x < kCounterStripes * 3) ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
 }
 
 // Streaming 16-byte copy, grid-stride.
@@ -235,42 +256,31 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
     // the epoch's commit counts start at zero (CHECK adds to them)
-    if (blockIdx.x == 0 && threadIdx.x < 3) ctr->commits[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
     const unsigned long long me =
         live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
              : 0ull;
     bool waw = false;
     uint32_t bad = 0;
-    // Four slots per step: their op loads, then their register probes, are
-    // issued back to back, so each lane keeps several misses in flight.
-    constexpr uint32_t U = 4;
-    for (uint32_t s0 = 0; s0 < S; s0 += U) {
-        Op o[U];
-        unsigned long long cur[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            o[u] = load_op(keys, meta, (uint64_t)(s0 + u) * B + t, live && s0 + u < S, rows, fields);
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            cur[u] = o[u].valid ? probe(&regs[2 * o[u].row + (o[u].kind == LTPG_KV_WRITE ? 1 : 0)]) : 0ull;
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            bad += ((o[u].kind == LTPG_KV_READ || o[u].kind == LTPG_KV_WRITE) && !o[u].valid) ? 1u : 0u;
-            if (!o[u].valid) continue;
-            unsigned long long *r = &regs[2 * o[u].row + (o[u].kind == LTPG_KV_WRITE ? 1 : 0)];
-            if (o[u].kind == LTPG_KV_WRITE) {
-                if (cur[u] < me) {
-                    waw = true;  // a higher-priority writer is already registered
-                } else if (cur[u] > me) {
-                    const unsigned long long old = atomicMin(r, me);
-                    if (old < me)
-                        waw = true;
-                    else if (old > me && (uint32_t)(old >> kTagShift) == tag)
-                        doomed[old & kPosMask] = epoch;  // displaced writer
-                }
-            } else if (cur[u] > me) {
-                atomicMin(r, me);
+    for (uint32_t s = 0; s < S; ++s) {
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
+        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
+        if (!o.valid) continue;
+        unsigned long long *r = &regs[2 * o.row + (o.kind == LTPG_KV_WRITE ? 1 : 0)];
+        const unsigned long long cur = probe(r);  // after the continue: o is valid
+        if (o.kind == LTPG_KV_WRITE) {
+            if (cur < me) {
+                waw = true;  // a higher-priority writer is already registered
+            } else if (cur > me) {
+                const unsigned long long old = atomicMin(r, me);
+                if (old < me)
+                    waw = true;
+                else if (old > me && (uint32_t)(old >> kTagShift) == tag)
+                    doomed[old & kPosMask] = epoch;  // displaced writer
             }
+        } else if (cur > me) {
+            atomicMin(r, me);
         }
     }
     if (waw) doomed[t] = epoch;
@@ -315,11 +325,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
         bitmap[wave_base / 64] = word;
         const uint32_t n = (uint32_t)__popcll(word);
         const uint32_t live_n = (B - wave_base) < 64u ? (B - wave_base) : 64u;
-        if (n) {
-            atomicAdd(&ctr->commits[LTPG_TYPE_KV], n);
-            atomicAdd(&ctr->commits_total[LTPG_TYPE_KV], (unsigned long long)n);
-        }
-        atomicAdd(&ctr->attempts_total[LTPG_TYPE_KV], (unsigned long long)live_n);
+        add_counts(ctr, (blockIdx.x * (kBlock / 64) + threadIdx.x / 64) % kCounterStripes, LTPG_TYPE_KV, n, live_n);
     }
 }
 
@@ -390,13 +396,19 @@ __device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t sr
             dl[u] = __shfl(delta, from);
             a[u] = k < S && ((flags >> from) & 1ull);
         }
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < KB; ++u) any |= a[u];
+        if (!__ballot(any)) continue;
         for (uint32_t w = gl; w < fw; w += G) {
             uint32_t v[KB];
+            // unconditional loads (inactive slots read word 0) so that all KB
+            // misses are in flight together; only active slots store
 #pragma unroll
-            for (int u = 0; u < KB; ++u) v[u] = a[u] ? src[so[u] + w] + dl[u] : 0u;
+            for (int u = 0; u < KB; ++u) v[u] = src[a[u] ? so[u] + w : 0ull];
 #pragma unroll
             for (int u = 0; u < KB; ++u)
-                if (a[u]) dst[dd[u] + w] = v[u];
+                if (a[u]) dst[dd[u] + w] = v[u] + dl[u];
         }
     }
 }
@@ -443,8 +455,10 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
             ((unsigned long long)tag << kTagShift) | ((unsigned long long)(has ? tids[t] : 0u) << kTidShift) | t;
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
-        const bool raw = o.valid && o.kind == LTPG_KV_READ && regs[2 * o.row + 1] < me;
-        const bool war = o.valid && o.kind == LTPG_KV_WRITE && regs[2 * o.row] < me;
+        // a read consults W (raw), a write consults R (war): one load
+        const unsigned long long rv = regs[o.valid ? 2 * o.row + (o.kind == LTPG_KV_READ ? 1 : 0) : 0ull];
+        const bool raw = o.valid && o.kind == LTPG_KV_READ && rv < me;
+        const bool war = o.valid && o.kind == LTPG_KV_WRITE && rv < me;
         const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
         const unsigned long long rd = __ballot(commit && o.valid && o.kind == LTPG_KV_READ);
         if (rd)
@@ -462,11 +476,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         bitmap[chunk0 / 64 + threadIdx.x] = s_bits[threadIdx.x];
     if (threadIdx.x == 0) {
         const uint32_t live_n = (B - chunk0) < kChunk ? (B - chunk0) : kChunk;
-        if (s_commits) {
-            atomicAdd(&ctr->commits[LTPG_TYPE_KV], s_commits);
-            atomicAdd(&ctr->commits_total[LTPG_TYPE_KV], (unsigned long long)s_commits);
-        }
-        atomicAdd(&ctr->attempts_total[LTPG_TYPE_KV], (unsigned long long)live_n);
+        add_counts(ctr, blockIdx.x % kCounterStripes, LTPG_TYPE_KV, s_commits, live_n);
     }
 }
 
@@ -565,7 +575,7 @@ inline dim3 grid_stride(uint64_t n)
 
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s)
 {
-    hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(64), 0, s, ctr);
+    hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(kCounterStripes * 3), 0, s, ctr);
     return hipGetLastError();
 }
 

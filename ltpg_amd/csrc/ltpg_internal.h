@@ -30,12 +30,17 @@ void set_error(const std::string &msg);
         return (code);                                                                        \
     } while (0)
 
-// Device-resident counters written by kernels.
+// Device-resident counters written by kernels.  Commit / attempt counts are
+// striped: kernels add to stripe (block % kCounterStripes) and the host sums
+// the stripes, so no single address takes an atomic from every block.
+constexpr int kCounterStripes = 64;
+
 struct DeviceCounters {
-    uint32_t range_errors;                  // ops whose row / field is outside its table
-    uint32_t commits[3];                    // per LTPG_TYPE_*, last epoch
-    unsigned long long commits_total[3];    // per LTPG_TYPE_*, since the last reset
-    unsigned long long attempts_total[3];   // transactions checked since the last reset
+    uint32_t range_errors;                            // ops outside their table
+    uint32_t pad0[63];
+    uint32_t commits[kCounterStripes][64];            // [stripe][type]: last epoch
+    unsigned long long totals[kCounterStripes][32];   // [stripe][type]: commits,
+                                                      // [stripe][3 + type]: attempts
 };
 
 // One key-value batch as the kernels see it (device pointers).
