@@ -25,36 +25,6 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 namespace {
 
-template <typename T>
-int dev_alloc(T **p, uint64_t count, const char *what)
-{
-    *p = nullptr;
-    if (count == 0) return LTPG_OK;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        set_error(std::string("hipMalloc(") + what + ", " + std::to_string(count * sizeof(T)) +
-                  " bytes): " + hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? LTPG_ERR_NOMEM : LTPG_ERR_DEVICE;
-    }
-    return LTPG_OK;
-}
-
-template <typename T>
-void dev_free(T *&p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-int check_ctx(ltpg_ctx *ctx)
-{
-    if (!ctx) LTPG_FAIL(LTPG_ERR_INVALID, "null context");
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
-    return LTPG_OK;
-}
-
 void free_kv(KvState &kv)
 {
     dev_free(kv.table);

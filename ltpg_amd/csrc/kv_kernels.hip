@@ -37,27 +37,15 @@
 // moves the fields of its ops cooperatively: LP lanes per op (LP = next
 // power of two >= field_words, at most 64), the op's addresses fetched from
 // its owner lane by __shfl.
+#include "device_common.h"
 #include "ltpg_internal.h"
 
 namespace ltpg {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTagShift = 56;
-constexpr int kTidShift = 24;
-constexpr unsigned long long kPosMask = 0xFFFFFFull;
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
-
-// Probe of a register before the atomic: a system-scope relaxed load, which
-// reads the coherent copy the atomics update.  Any value would be safe (see
-// kv_execute_kernel), but on hot rows a stale cached copy sends far more
-// lanes into the contended atomic; measured on MI355X at Zipf 0.99 the
-// coherent probe is clearly the fastest.
-__device__ __forceinline__ unsigned long long probe(const unsigned long long *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 // Copy (or read-modify-write) the fields of the active ops of a wave.
 // Lane j's op: src_base[src_off + w] (+ delta) -> dst_base[dst_off + w] for
@@ -258,30 +246,17 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     // the epoch's commit counts start at zero (CHECK adds to them)
     if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
         ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
-    const unsigned long long me =
-        live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
-             : 0ull;
+    const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     bool waw = false;
     uint32_t bad = 0;
     for (uint32_t s = 0; s < S; ++s) {
         const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
         bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
         if (!o.valid) continue;
-        unsigned long long *r = &regs[2 * o.row + (o.kind == LTPG_KV_WRITE ? 1 : 0)];
-        const unsigned long long cur = probe(r);  // after the continue: o is valid
-        if (o.kind == LTPG_KV_WRITE) {
-            if (cur < me) {
-                waw = true;  // a higher-priority writer is already registered
-            } else if (cur > me) {
-                const unsigned long long old = atomicMin(r, me);
-                if (old < me)
-                    waw = true;
-                else if (old > me && (uint32_t)(old >> kTagShift) == tag)
-                    doomed[old & kPosMask] = epoch;  // displaced writer
-            }
-        } else if (cur > me) {
-            atomicMin(r, me);
-        }
+        if (o.kind == LTPG_KV_WRITE)
+            waw |= register_write(&regs[2 * o.row + 1], me, tag, doomed, epoch);
+        else
+            register_read(&regs[2 * o.row], me);
     }
     if (waw) doomed[t] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
@@ -302,9 +277,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
     const bool live = t < B;
-    const unsigned long long me =
-        live ? (((unsigned long long)tag << kTagShift) | ((unsigned long long)tids[t] << kTidShift) | t)
-             : 0ull;
+    const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     uint32_t raw = 0, war = 0, waw = 0;
     for (uint32_t s = 0; s < S; ++s) {
         const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
@@ -451,8 +424,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
-        const unsigned long long me =
-            ((unsigned long long)tag << kTagShift) | ((unsigned long long)(has ? tids[t] : 0u) << kTidShift) | t;
+        const unsigned long long me = priority(tag, has ? tids[t] : 0u, t);
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
         // a read consults W (raw), a write consults R (war): one load

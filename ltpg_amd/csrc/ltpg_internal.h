@@ -99,6 +99,38 @@ struct ltpg_ctx {
 
 namespace ltpg {
 
+// ---- host helpers shared by the API translation units --------------------
+
+template <typename T>
+int dev_alloc(T **p, uint64_t count, const char *what)
+{
+    *p = nullptr;
+    if (count == 0) return LTPG_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error(std::string("hipMalloc(") + what + ", " + std::to_string(count * sizeof(T)) +
+                  " bytes): " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? LTPG_ERR_NOMEM : LTPG_ERR_DEVICE;
+    }
+    return LTPG_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+inline int check_ctx(ltpg_ctx *ctx)
+{
+    if (!ctx) LTPG_FAIL(LTPG_ERR_INVALID, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return LTPG_OK;
+}
+
 // Kernel launchers (kv_kernels.hip).  All asynchronous on `s`.
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
