@@ -218,6 +218,103 @@ typedef struct ltpg_kv_batch {
  * epoch become stale without a memset. */
 int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *batch, uint32_t mem);
 
+/* ---- order-entry tables (SNAPSHOT, TPCC-allopt-final/Datastructure.h:119-130;
+ *      sizes Predefine.h:48-65) --------------------------------------------- */
+/* Rows and columns of table `table_id` (LTPG_TPCC_*) for the context's
+ * warehouse count: W, 10W, 30000W, 30000W, 30000W, 30000W, 450000W,
+ * 100000W, 100000 rows of 9, 11, 21, 5, 3, 8, 10, 17, 5 u32 columns. */
+int ltpg_tpcc_table_shape(ltpg_ctx *ctx, uint32_t table_id, uint64_t *rows, uint32_t *cols);
+/* Load / read one table; COLUMN_MAJOR is the reference's snapshot layout
+ * (tbl[col * rows + row]).  Loading the customer table rebuilds the
+ * by-name index (C_LAST, column 13) used by the CID phase. */
+int ltpg_tpcc_table_load(ltpg_ctx *ctx, uint32_t table_id, const uint32_t *host, uint64_t rows,
+                         uint32_t layout);
+int ltpg_tpcc_table_read(ltpg_ctx *ctx, uint32_t table_id, uint32_t *host, uint64_t rows,
+                         uint32_t layout);
+/* Fill every table on the device with the reference loader's contents
+ * (create_warehouse .. create_item, TPCC-allopt-final/Database.h:104-327);
+ * its uniform draws (C_CREDIT, O_C_ID, OL_I_ID, I_PRICE, I_DATA) come from a
+ * counter-based hash of (seed, table, row).  Rebuilds the by-name index. */
+int ltpg_tpcc_populate(ltpg_ctx *ctx, uint64_t seed);
+
+/* ---- order-entry batches (NEWORDER_QUERY / PAYMENT_QUERY,
+ *      TPCC-allopt-final/Datastructure.h:211-249) ---------------------------- */
+typedef struct ltpg_neworder_batch {
+    uint32_t n;               /* transactions                                  */
+    uint32_t line_stride;     /* >= n: line l of txn q is [l * line_stride + q] */
+    const uint32_t *W_ID, *D_ID, *C_ID, *O_ID, *N_O_ID, *O_OL_CNT, *O_OL_ID; /* [n] */
+    const uint32_t *OL_I_ID, *OL_SUPPLY_W_ID, *OL_QUANTITY; /* [15 * line_stride] */
+    const uint32_t *TID;      /* [n]                                           */
+} ltpg_neworder_batch;
+
+typedef struct ltpg_payment_batch {
+    uint32_t n;
+    uint32_t reserved;
+    const uint32_t *W_ID, *D_ID, *C_ID, *C_LAST, *isName, *C_D_ID, *C_W_ID, *H_AMOUNT, *H_ID;
+    const uint32_t *TID;      /* all [n]                                       */
+} ltpg_payment_batch;
+
+/* Views of the reference's packed structs: a NEWORDER_QUERY / PAYMENT_QUERY
+ * whose arrays hold `cap` entries each (NEWORDER_CNT / PAYMENT_CNT), of which
+ * the first n are used.  The packed NewOrder block is 53 * cap u32 (7 arrays,
+ * 3 x 15 line arrays, TID); the packed Payment block is 10 * cap u32. */
+void ltpg_neworder_view(const uint32_t *block, uint32_t cap, uint32_t n, ltpg_neworder_batch *out);
+void ltpg_payment_view(const uint32_t *block, uint32_t cap, uint32_t n, ltpg_payment_batch *out);
+
+/* Make (neworder, payment) the current epoch (copy_to_device,
+ * TPCC-allopt-final/Query.h:532-605).  Either may be NULL or empty.
+ * NewOrder q has batch position q, Payment p has position n_neworder + p. */
+int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *neworder,
+                     const ltpg_payment_batch *payment, uint32_t mem);
+/* C_ID used by each Payment of the current batch: the query's C_ID, or,
+ * once the CID phase has run, the by-name resolution for isName Payments
+ * (KERNEL_CID writes it back into PAYMENT_QUERY.C_ID, TPCC-allopt-final/
+ * tp.cu:11-23, Execute_payment.h:13-36): the C_ID column of the k-th
+ * (0-based, k = C_LAST & 0xff) customer in row order of district
+ * C_W_ID * 10 + C_D_ID whose C_LAST equals C_LAST >> 8, or 0 if none. */
+int ltpg_tpcc_payment_cid(ltpg_ctx *ctx, uint32_t *out, uint32_t n);
+/* Read values of the last epoch's committed transactions (the read slots of
+ * set_local_set, TPCC-allopt-final/Execute_neworder.h:30-202,
+ * Execute_payment.h:55-239), value-major: value v of transaction q at
+ * out[v * n + q], n = the type's batch size, taken from the snapshot.
+ *   NEWORDER (51): W_TAX; D_TAX, D_NEXT_O_ID; C_DISCOUNT, C_CREDIT, C_LAST;
+ *     then I_PRICE, I_DATA, I_NAME of lines 0..14 (0 beyond O_OL_CNT).
+ *   PAYMENT (24): WAREHOUSE cols 3..8; DISTRICT cols 5..10; CUSTOMER cols
+ *     3, 4, 11..20.
+ * A slot whose row is outside its table reads 0; aborted transactions'
+ * entries are not written. */
+int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t n_words);
+
+/* ---- seeded order-entry generator (MakeNeworder / MakePayment,
+ *      TPCC-allopt-final/Query.h:16-212, with its commented-out options
+ *      enabled: NURand items, remote supply warehouse, remote customer) ---- */
+typedef struct ltpg_tpcc_gen_config {
+    uint64_t seed;
+    uint32_t warehouses;
+    uint32_t neworder_pct;        /* share of NewOrder in a batch (exact)      */
+    uint32_t remote_supply_pct;   /* per order line (1)                         */
+    uint32_t remote_customer_pct; /* per Payment (15)                           */
+    uint32_t byname_pct;          /* per Payment (60)                           */
+    uint32_t nurand_items;        /* 1: item = NURand(8191, 1, 100000) - 1      */
+    uint32_t min_lines, max_lines;/* O_OL_CNT uniform (5, 15)                   */
+    uint32_t max_amount;          /* H_AMOUNT uniform in [1, max_amount] (5)    */
+    uint32_t max_quantity;        /* OL_QUANTITY uniform in [1, max_quantity]   */
+} ltpg_tpcc_gen_config;
+typedef struct ltpg_tpcc_gen ltpg_tpcc_gen;
+
+void ltpg_tpcc_gen_config_init(ltpg_tpcc_gen_config *c);
+int ltpg_tpcc_gen_create(const ltpg_tpcc_gen_config *c, ltpg_tpcc_gen **out);
+int ltpg_tpcc_gen_destroy(ltpg_tpcc_gen *g);
+/* Next batch of n_txns transactions (TIDs tid_base .. tid_base + n_txns - 1
+ * in arrival order, types interleaved by a seeded shuffle with exact
+ * counts) into packed reference-layout blocks of capacity no_cap / pay_cap.
+ * The per-warehouse order / history id counters continue across calls
+ * (wrapping as the reference does, Query.h:54-88,157-166, with order lines
+ * kept inside their warehouse). */
+int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uint32_t *no_block,
+                       uint32_t no_cap, uint32_t *pay_block, uint32_t pay_cap, uint32_t *n_neworder,
+                       uint32_t *n_payment);
+
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
  *      tp.cu:24-117, YCSB/0/ycsb.cu:9-58; launched by tp.cu:154-236) ------- */
 /* Run the phases in phase_mask on the current batch, in the order CID,
@@ -232,7 +329,8 @@ int ltpg_epoch_reset(ltpg_ctx *ctx);
 /* ---- results (statistic_query / copy_to_host, TPCC-allopt-final/Query.h:
  *      606-631,697-725; COMMIT_AND_ABORT, YCSB/0/Execute.h:176-192) ------- */
 /* Commit bitmap of the last epoch: bit t of word t/64 is 1 iff transaction t
- * (of `type`) committed.  Synchronises the context. */
+ * of `type` (LTPG_TYPE_KV, or LTPG_TYPE_NEWORDER / LTPG_TYPE_PAYMENT indexed
+ * within their own batch) committed.  Synchronises the context. */
 int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_words);
 /* Number of committed transactions of `type` in the last epoch. */
 int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out);

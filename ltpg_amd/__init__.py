@@ -9,8 +9,16 @@ libltpg.so (include/ltpg.h).  The reference's host objects map as follows:
   YCSBQuery / Query (batch upload, commit readback)
       -> Context.kv_submit / commit_bitmap / commit_count
                                      (LTPG/YCSB/0/Query.h:126-231)
-  KERNEL_EXECUTE / KERNEL_CHECK / KERNEL_WRITEBACK
-      -> Context.epoch_run(phases)   (LTPG/YCSB/0/ycsb.cu:9-58)
+  KERNEL_CID / KERNEL_EXECUTE / KERNEL_CHECK / KERNEL_WRITEBACK
+      -> Context.epoch_run(phases)   (LTPG/YCSB/0/ycsb.cu:9-58,
+                                      LTPG/TPCC-allopt-final/tp.cu:11-117)
+  Database (order-entry snapshot, create_*, customer name index)
+      -> Context.tpcc_table_load / tpcc_table_read / tpcc_populate
+                                     (TPCC-allopt-final/Database.h:13-451)
+  Query (NEWORDER_QUERY / PAYMENT_QUERY upload) and MakeNeworder /
+  MakePayment
+      -> Context.tpcc_submit, TpccGenerator
+                                     (TPCC-allopt-final/Query.h:16-212,532-631)
 
 Every call goes to the HIP implementation; nothing here computes results.
 """
@@ -19,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import Config, KvBatch, KvGen, lib
+from ._lib import Config, KvBatch, KvGen, NeworderBatch, PaymentBatch, TpccGenConfig, lib
 
 OK = 0
 ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_RANGE, ERR_BUDGET, ERR_UNSUPPORTED = (
@@ -37,6 +45,13 @@ NO_HOT_ROW = (1 << 64) - 1
 FLAG_CONFLICT_COUNTS = 1
 TIMING_SLOTS = 5
 TIMING_NAMES = ["cid", "execute", "check", "readout", "writeback"]
+TPCC_WAREHOUSE, TPCC_DISTRICT, TPCC_CUSTOMER, TPCC_HISTORY, TPCC_NEWORDER, TPCC_ORDER, TPCC_ORDERLINE, \
+    TPCC_STOCK, TPCC_ITEM = range(9)
+TPCC_TABLES = ["warehouse", "district", "customer", "history", "neworder", "order", "orderline", "stock", "item"]
+TPCC_MAX_LINES = 15
+NEWORDER_READS, PAYMENT_READS = 51, 24
+# u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
+NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 
 
 def kv_meta(kind, field):
@@ -144,6 +159,8 @@ class Context:
         self.handle = h
         self._cur_txns = 0
         self._cur_slots = 0
+        self._n_no = 0
+        self._n_pay = 0
 
     def close(self):
         if self.handle:
@@ -249,6 +266,63 @@ class Context:
                "ltpg_conflict_counters")
         return out
 
+    # ---- order-entry ------------------------------------------------------
+    def tpcc_shape(self, table):
+        rows, cols = ctypes.c_uint64(), ctypes.c_uint32()
+        _check(lib.ltpg_tpcc_table_shape(self.handle, table, ctypes.byref(rows), ctypes.byref(cols)),
+               "ltpg_tpcc_table_shape")
+        return rows.value, cols.value
+
+    def tpcc_table_load(self, table, data, layout=LAYOUT_COLUMN_MAJOR):
+        rows, cols = self.tpcc_shape(table)
+        data = np.ascontiguousarray(data, dtype=np.uint32)
+        if data.size != rows * cols:
+            raise ValueError("table size does not match its shape")
+        _check(lib.ltpg_tpcc_table_load(self.handle, table, data.ctypes.data_as(_lib.c_u32p), rows, layout),
+               "ltpg_tpcc_table_load")
+
+    def tpcc_table_read(self, table, layout=LAYOUT_COLUMN_MAJOR):
+        rows, cols = self.tpcc_shape(table)
+        out = np.empty(rows * cols, dtype=np.uint32)
+        _check(lib.ltpg_tpcc_table_read(self.handle, table, out.ctypes.data_as(_lib.c_u32p), rows, layout),
+               "ltpg_tpcc_table_read")
+        return out
+
+    def tpcc_populate(self, seed=0x4C545047):
+        _check(lib.ltpg_tpcc_populate(self.handle, seed), "ltpg_tpcc_populate")
+
+    def tpcc_submit(self, batch, mem=MEM_HOST):
+        """Submit a TpccBatch (host blocks), or with mem=MEM_DEVICE a pair of
+        (NeworderBatch, PaymentBatch) structs of device pointers."""
+        if mem == MEM_HOST:
+            nb = neworder_struct(batch.no, batch.no_cap, batch.n_no)
+            pb = payment_struct(batch.pay, batch.pay_cap, batch.n_pay)
+            n_no, n_pay = batch.n_no, batch.n_pay
+        else:
+            nb, pb = batch
+            n_no, n_pay = nb.n, pb.n
+        _check(lib.ltpg_tpcc_submit(self.handle, ctypes.byref(nb), ctypes.byref(pb), mem), "ltpg_tpcc_submit")
+        self._n_no, self._n_pay = n_no, n_pay
+
+    def tpcc_payment_cid(self):
+        out = np.zeros(max(self._n_pay, 1), dtype=np.uint32)
+        _check(lib.ltpg_tpcc_payment_cid(self.handle, out.ctypes.data_as(_lib.c_u32p), self._n_pay),
+               "ltpg_tpcc_payment_cid")
+        return out[:self._n_pay]
+
+    def tpcc_read_values(self, type_):
+        n = self._n_no * NEWORDER_READS if type_ == TYPE_NEWORDER else self._n_pay * PAYMENT_READS
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(lib.ltpg_tpcc_read_values(self.handle, type_, out.ctypes.data_as(_lib.c_u32p), n),
+               "ltpg_tpcc_read_values")
+        return out[:n]
+
+    def tpcc_bitmap(self, type_):
+        return self.commit_bitmap(type_, self._n_no if type_ == TYPE_NEWORDER else self._n_pay)
+
+    def tpcc_conflicts(self, type_):
+        return self.conflict_counters(type_, self._n_no if type_ == TYPE_NEWORDER else self._n_pay)
+
     def kv_read_values(self):
         """Read values of committed transactions' read slots (slot-major,
         field_words per op); other slots are left as they were."""
@@ -257,6 +331,93 @@ class Context:
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib.ltpg_kv_read_values(self.handle, out.ctypes.data_as(_lib.c_u32p), n), "ltpg_kv_read_values")
         return out[:n]
+
+
+def neworder_struct(block, cap, n):
+    """ltpg_neworder_batch viewing a packed NEWORDER_QUERY block (53 * cap u32)."""
+    b = NeworderBatch()
+    lib.ltpg_neworder_view(block.ctypes.data_as(_lib.c_u32p), cap, n, ctypes.byref(b))
+    return b
+
+
+def payment_struct(block, cap, n):
+    """ltpg_payment_batch viewing a packed PAYMENT_QUERY block (10 * cap u32)."""
+    b = PaymentBatch()
+    lib.ltpg_payment_view(block.ctypes.data_as(_lib.c_u32p), cap, n, ctypes.byref(b))
+    return b
+
+
+class TpccBatch:
+    """One order-entry batch in the reference's packed layout: `no` is a
+    NEWORDER_QUERY block [53 * no_cap] and `pay` a PAYMENT_QUERY block
+    [10 * pay_cap] (TPCC-allopt-final/Datastructure.h:211-249)."""
+
+    NO_FIELDS = {"W_ID": 0, "D_ID": 1, "C_ID": 2, "O_ID": 3, "N_O_ID": 4, "O_OL_CNT": 5, "O_OL_ID": 6,
+                 "OL_I_ID": 7, "OL_SUPPLY_W_ID": 22, "OL_QUANTITY": 37, "TID": 52}
+    PAY_FIELDS = {"W_ID": 0, "D_ID": 1, "C_ID": 2, "C_LAST": 3, "isName": 4, "C_D_ID": 5, "C_W_ID": 6,
+                  "H_AMOUNT": 7, "H_ID": 8, "TID": 9}
+
+    def __init__(self, no, no_cap, n_no, pay, pay_cap, n_pay):
+        self.no, self.no_cap, self.n_no = no, int(no_cap), int(n_no)
+        self.pay, self.pay_cap, self.n_pay = pay, int(pay_cap), int(n_pay)
+
+    @classmethod
+    def empty(cls, n_no, n_pay, no_cap=None, pay_cap=None):
+        no_cap = max(1, n_no if no_cap is None else no_cap)
+        pay_cap = max(1, n_pay if pay_cap is None else pay_cap)
+        return cls(np.zeros(NEWORDER_WORDS * no_cap, np.uint32), no_cap, n_no,
+                   np.zeros(PAYMENT_WORDS * pay_cap, np.uint32), pay_cap, n_pay)
+
+    def nf(self, name, line=None):
+        """View of NewOrder field `name` (a line of it for OL_* fields)."""
+        k = self.NO_FIELDS[name]
+        if name.startswith("OL_"):
+            if line is None:
+                return self.no[k * self.no_cap:(k + 15) * self.no_cap].reshape(15, self.no_cap)[:, :self.n_no]
+            k += line
+        return self.no[k * self.no_cap:k * self.no_cap + self.n_no]
+
+    def pf(self, name):
+        """View of Payment field `name`."""
+        k = self.PAY_FIELDS[name]
+        return self.pay[k * self.pay_cap:k * self.pay_cap + self.n_pay]
+
+
+class TpccGenerator:
+    """Seeded NewOrder / Payment batches (ltpg_tpcc_gen_*)."""
+
+    def __init__(self, **kw):
+        cfg = TpccGenConfig()
+        lib.ltpg_tpcc_gen_config_init(ctypes.byref(cfg))
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise AttributeError(f"ltpg_tpcc_gen_config has no field {k}")
+            setattr(cfg, k, v)
+        self.config = cfg
+        h = ctypes.c_void_p()
+        _check(lib.ltpg_tpcc_gen_create(ctypes.byref(cfg), ctypes.byref(h)), "ltpg_tpcc_gen_create")
+        self.handle = h
+
+    def next(self, n_txns, tid_base=0):
+        n_no = n_txns * self.config.neworder_pct // 100
+        b = TpccBatch.empty(n_no, n_txns - n_no)
+        a, c = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.ltpg_tpcc_gen_next(self.handle, n_txns, tid_base, b.no.ctypes.data_as(_lib.c_u32p), b.no_cap,
+                                      b.pay.ctypes.data_as(_lib.c_u32p), b.pay_cap, ctypes.byref(a),
+                                      ctypes.byref(c)), "ltpg_tpcc_gen_next")
+        assert a.value == b.n_no and c.value == b.n_pay
+        return b
+
+    def close(self):
+        if self.handle:
+            lib.ltpg_tpcc_gen_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def kv_generate(seed=0x4C545047, rows=1_000_000, fields=10, n_txns=65536, n_slots=10, min_ops=None,

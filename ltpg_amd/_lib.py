@@ -68,6 +68,31 @@ class KvGen(ctypes.Structure):
     ]
 
 
+class NeworderBatch(ctypes.Structure):
+    """struct ltpg_neworder_batch (line l of txn q at [l * line_stride + q])."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("line_stride", ctypes.c_uint32)] + [
+        (name, ctypes.c_void_p) for name in ("W_ID", "D_ID", "C_ID", "O_ID", "N_O_ID", "O_OL_CNT", "O_OL_ID",
+                                             "OL_I_ID", "OL_SUPPLY_W_ID", "OL_QUANTITY", "TID")]
+
+
+class PaymentBatch(ctypes.Structure):
+    """struct ltpg_payment_batch."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)] + [
+        (name, ctypes.c_void_p) for name in ("W_ID", "D_ID", "C_ID", "C_LAST", "isName", "C_D_ID", "C_W_ID",
+                                             "H_AMOUNT", "H_ID", "TID")]
+
+
+class TpccGenConfig(ctypes.Structure):
+    """struct ltpg_tpcc_gen_config."""
+
+    _fields_ = [("seed", ctypes.c_uint64)] + [
+        (name, ctypes.c_uint32) for name in ("warehouses", "neworder_pct", "remote_supply_pct",
+                                             "remote_customer_pct", "byname_pct", "nurand_items", "min_lines",
+                                             "max_lines", "max_amount", "max_quantity")]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
@@ -96,6 +121,26 @@ SIGNATURES = [
     ("ltpg_phase_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), c_u32p]),
     ("ltpg_commit_totals", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u64p, ctypes.c_int]),
     ("ltpg_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("ltpg_tpcc_table_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u32p]),
+    ("ltpg_tpcc_table_load", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint64, ctypes.c_uint32]),
+    ("ltpg_tpcc_table_read", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint64, ctypes.c_uint32]),
+    ("ltpg_tpcc_populate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    ("ltpg_neworder_view", None,
+     [c_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(NeworderBatch)]),
+    ("ltpg_payment_view", None,
+     [c_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(PaymentBatch)]),
+    ("ltpg_tpcc_submit", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), ctypes.c_uint32]),
+    ("ltpg_tpcc_payment_cid", ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_uint32]),
+    ("ltpg_tpcc_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint64]),
+    ("ltpg_tpcc_gen_config_init", None, [ctypes.POINTER(TpccGenConfig)]),
+    ("ltpg_tpcc_gen_create", ctypes.c_int, [ctypes.POINTER(TpccGenConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    ("ltpg_tpcc_gen_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_tpcc_gen_next", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_uint32, c_u32p, ctypes.c_uint32,
+      c_u32p, c_u32p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
     ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
 ]

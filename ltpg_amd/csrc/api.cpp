@@ -78,6 +78,84 @@ int init_kv(ltpg_ctx *ctx)
     return LTPG_OK;
 }
 
+// ---- order-entry state ------------------------------------------------------
+
+// Rows per table at W warehouses (TPCC-allopt-final/Predefine.h:48-56) and
+// columns (Predefine.h:57-65); device rows are padded to a multiple of 4.
+const uint64_t kRowsPerW[kTpccTables] = {1, 10, 30000, 30000, 30000, 30000, 450000, 100000, 0};
+const uint32_t kTpccCols[kTpccTables] = {9, 11, 21, 5, 3, 8, 10, 17, 5};
+
+uint64_t tpcc_rows(uint32_t W, uint32_t t) { return t == LTPG_TPCC_ITEM ? 100000 : kRowsPerW[t] * W; }
+
+void free_tpcc(TpccState &tp)
+{
+    for (auto &t : tp.d.t) dev_free(t.data);
+    dev_free(tp.d.regs);
+    dev_free(tp.d.name_index);
+    dev_free(tp.d.new_last);
+    dev_free(tp.d.doomed);
+    dev_free(tp.d.bitmap_no);
+    dev_free(tp.d.bitmap_pay);
+    dev_free(tp.d.conf_no);
+    dev_free(tp.d.conf_pay);
+    dev_free(tp.d.reads_no);
+    dev_free(tp.d.reads_pay);
+    dev_free(tp.d.cid);
+    dev_free(tp.no_buf);
+    dev_free(tp.pay_buf);
+}
+
+int init_tpcc(ltpg_ctx *ctx)
+{
+    const ltpg_config &c = ctx->cfg;
+    TpccState &tp = ctx->tpcc;
+    TpccDev &d = tp.d;
+    const uint32_t W = c.tpcc_warehouses;
+    if (W == 0 || W > 65536) LTPG_FAIL(LTPG_ERR_INVALID, "tpcc_warehouses must be in [1, 65536]");
+    d.warehouses = W;
+    d.rules = c.rules;
+    int rc;
+    for (uint32_t t = 0; t < (uint32_t)kTpccTables; t++) {
+        TpccTable &tb = d.t[t];
+        tb.rows = tpcc_rows(W, t);
+        tb.cols = kTpccCols[t];
+        tb.stride = (tb.cols + 3) / 4 * 4;
+        if ((rc = dev_alloc(&tb.data, tb.rows * tb.stride, "order-entry table"))) return rc;
+        LTPG_HIP_TRY(hipMemsetAsync(tb.data, 0, tb.rows * tb.stride * 4, ctx->stream));
+    }
+    const uint64_t cust = d.t[LTPG_TPCC_CUSTOMER].rows;
+    d.off_order = 2 * cust;
+    d.off_stock = d.off_order + d.t[LTPG_TPCC_ORDER].rows;
+    d.off_line = d.off_stock + d.t[LTPG_TPCC_STOCK].rows;
+    d.off_hist = d.off_line + d.t[LTPG_TPCC_ORDERLINE].rows;
+    d.off_new = d.off_hist + d.t[LTPG_TPCC_HISTORY].rows;
+    d.n_regs = d.off_new + d.t[LTPG_TPCC_NEWORDER].rows;
+    const uint64_t B = c.max_txns;
+    if ((rc = dev_alloc(&d.regs, d.n_regs, "order-entry registers"))) return rc;
+    if ((rc = dev_alloc(&d.name_index, cust, "by-name index"))) return rc;
+    if ((rc = dev_alloc(&d.new_last, d.t[LTPG_TPCC_NEWORDER].rows, "NEWORDER inserters"))) return rc;
+    if ((rc = dev_alloc(&d.doomed, B, "doomed flags"))) return rc;
+    if ((rc = dev_alloc(&d.bitmap_no, (B + 63) / 64, "NewOrder bitmap"))) return rc;
+    if ((rc = dev_alloc(&d.bitmap_pay, (B + 63) / 64, "Payment bitmap"))) return rc;
+    if ((rc = dev_alloc(&d.conf_no, B, "NewOrder conflicts"))) return rc;
+    if ((rc = dev_alloc(&d.conf_pay, B, "Payment conflicts"))) return rc;
+    if ((rc = dev_alloc(&d.reads_no, (uint64_t)kNoReads * B, "NewOrder read values"))) return rc;
+    if ((rc = dev_alloc(&d.reads_pay, (uint64_t)kPayReads * B, "Payment read values"))) return rc;
+    if ((rc = dev_alloc(&d.cid, B, "Payment C_ID"))) return rc;
+    if ((rc = dev_alloc(&tp.no_buf, 53 * B, "NewOrder batch"))) return rc;
+    if ((rc = dev_alloc(&tp.pay_buf, 10 * B, "Payment batch"))) return rc;
+    LTPG_HIP_TRY(launch_fill_u64(d.regs, d.n_regs, ~0ull, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(d.doomed, B, 0, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u64(d.new_last, d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(d.bitmap_no, 0, (B + 63) / 64 * 8, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(d.bitmap_pay, 0, (B + 63) / 64 * 8, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(d.conf_no, 0, B * 4, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(d.conf_pay, 0, B * 4, ctx->stream));
+    LTPG_HIP_TRY(hipMemsetAsync(d.cid, 0, B * 4, ctx->stream));
+    LTPG_HIP_TRY(launch_tpcc_name_index(d, ctx->stream));
+    return LTPG_OK;
+}
+
 // Advance the epoch (1..255).  At wrap-around every register goes back to
 // all-ones (the tag of epoch 0, never current) and the doomed flags to 0.
 int next_epoch(ltpg_ctx *ctx)
@@ -87,6 +165,11 @@ int next_epoch(ltpg_ctx *ctx)
         if (ctx->kv.regs) {
             LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
             LTPG_HIP_TRY(launch_fill_u8(ctx->kv.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+        }
+        if (ctx->tpcc.d.regs) {
+            LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.regs, ctx->tpcc.d.n_regs, ~0ull, ctx->stream));
+            LTPG_HIP_TRY(launch_fill_u8(ctx->tpcc.d.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+            LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.new_last, ctx->tpcc.d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
         }
         ctx->epoch = 1;
     }
@@ -220,9 +303,7 @@ int ltpg_create(const ltpg_config *cfg, ltpg_ctx **out)
         if (cfg->workload == LTPG_WORKLOAD_KV) {
             if ((rc = init_kv(ctx))) break;
         } else {
-            set_error("order-entry workload is not supported by this build");
-            rc = LTPG_ERR_UNSUPPORTED;
-            break;
+            if ((rc = init_tpcc(ctx))) break;
         }
         e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
@@ -247,6 +328,7 @@ int ltpg_destroy(ltpg_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_kv(ctx->kv);
+    free_tpcc(ctx->tpcc);
     dev_free(ctx->counters);
     for (auto &ev : ctx->ev_pool)
         if (ev) (void)hipEventDestroy(ev);
@@ -490,7 +572,7 @@ int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
 
 namespace {
 // Take the next LTPG_TIMING_SLOTS + 1 timing events of this epoch.
-int take_events(ltpg_ctx *ctx, hipEvent_t **out)
+static int take_events(ltpg_ctx *ctx, hipEvent_t **out)
 {
     if (ctx->ev_used + kEv > ctx->ev_pool.size()) {
         for (int k = 0; k < 8 * kEv; k++) {
@@ -503,6 +585,40 @@ int take_events(ltpg_ctx *ctx, hipEvent_t **out)
     ctx->ev_used += kEv;
     return LTPG_OK;
 }
+
+// CID, EXECUTE, CHECK (with the committed reads), WRITEBACK of the current
+// order-entry batch; the read-out timing slot stays empty.
+static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
+{
+    TpccState &tp = ctx->tpcc;
+    if (!tp.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "no batch submitted");
+    const EpochTag e = epoch_tag(ctx);
+    const bool counts = (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS) != 0;
+    hipStream_t s = ctx->stream;
+    hipEvent_t *ev = nullptr;
+    int rc;
+    if (ctx->timing) {
+        if ((rc = take_events(ctx, &ev))) return rc;
+        LTPG_HIP_TRY(hipEventRecord(ev[0], s));
+    }
+    if (phase_mask & LTPG_PHASE_CID) LTPG_HIP_TRY(launch_tpcc_cid(tp.d, tp.pay, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
+    if (phase_mask & LTPG_PHASE_EXECUTE)
+        LTPG_HIP_TRY(launch_tpcc_execute(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
+    if (phase_mask & LTPG_PHASE_CHECK) {
+        if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
+        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
+    }
+    if (ev) {
+        LTPG_HIP_TRY(hipEventRecord(ev[3], s));
+        LTPG_HIP_TRY(hipEventRecord(ev[4], s));
+    }
+    if (phase_mask & LTPG_PHASE_WRITEBACK)
+        LTPG_HIP_TRY(launch_tpcc_writeback(tp.d, tp.no, tp.pay, e, ctx->cfg.values, s));
+    if (ev) LTPG_HIP_TRY(hipEventRecord(ev[5], s));
+    return LTPG_OK;
+}
 }  // namespace
 
 int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
@@ -510,6 +626,7 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (phase_mask & ~LTPG_PHASE_ALL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown phase bit");
+    if (ctx->cfg.workload == LTPG_WORKLOAD_TPCC) return tpcc_epoch_run(ctx, phase_mask);
     KvState &kv = ctx->kv;
     if (!kv.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "no batch submitted");
     const EpochTag e = epoch_tag(ctx);
@@ -554,11 +671,24 @@ int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_w
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
-    if (type != LTPG_TYPE_KV || !ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "unknown transaction type");
-    const uint64_t need = ((uint64_t)ctx->kv.cur.n_txns + 63) / 64;
+    const uint64_t *src = nullptr;
+    uint64_t n = 0;
+    if (type == LTPG_TYPE_KV && ctx->kv.table) {
+        src = ctx->kv.bitmap;
+        n = ctx->kv.cur.n_txns;
+    } else if (type == LTPG_TYPE_NEWORDER && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.bitmap_no;
+        n = ctx->tpcc.no.n;
+    } else if (type == LTPG_TYPE_PAYMENT && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.bitmap_pay;
+        n = ctx->tpcc.pay.n;
+    } else {
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "transaction type not in this context's workload");
+    }
+    const uint64_t need = (n + 63) / 64;
     if (n_words < need) LTPG_FAIL(LTPG_ERR_INVALID, "bitmap buffer too small");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (need) LTPG_HIP_TRY(hipMemcpy(out, ctx->kv.bitmap, need * 8, hipMemcpyDeviceToHost));
+    if (need) LTPG_HIP_TRY(hipMemcpy(out, src, need * 8, hipMemcpyDeviceToHost));
     return LTPG_OK;
 }
 
@@ -582,12 +712,25 @@ int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
-    if (type != LTPG_TYPE_KV || !ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "unknown transaction type");
+    const uint32_t *src = nullptr;
+    uint32_t have = 0;
+    if (type == LTPG_TYPE_KV && ctx->kv.table) {
+        src = ctx->kv.conflicts;
+        have = ctx->kv.cur.n_txns;
+    } else if (type == LTPG_TYPE_NEWORDER && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.conf_no;
+        have = ctx->tpcc.no.n;
+    } else if (type == LTPG_TYPE_PAYMENT && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.conf_pay;
+        have = ctx->tpcc.pay.n;
+    } else {
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "transaction type not in this context's workload");
+    }
     if (!(ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS))
         LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "conflict counters need LTPG_FLAG_CONFLICT_COUNTS");
-    if (n > ctx->kv.cur.n_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the batch size");
+    if (n > have) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the batch size");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (n) LTPG_HIP_TRY(hipMemcpy(out, ctx->kv.conflicts, (uint64_t)n * 4, hipMemcpyDeviceToHost));
+    if (n) LTPG_HIP_TRY(hipMemcpy(out, src, (uint64_t)n * 4, hipMemcpyDeviceToHost));
     return LTPG_OK;
 }
 
@@ -602,6 +745,191 @@ int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words)
     if (n_words > have) LTPG_FAIL(LTPG_ERR_INVALID, "n_words exceeds the staged read values");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n_words) LTPG_HIP_TRY(hipMemcpy(out, kv.read_values, n_words * 4, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+// ---- order-entry tables, batches, results -----------------------------------
+
+namespace {
+static int tpcc_ctx(ltpg_ctx *ctx)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->tpcc.d.regs) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no order-entry tables");
+    return LTPG_OK;
+}
+
+static int tpcc_copy_table(ltpg_ctx *ctx, uint32_t table_id, uint32_t *host, uint64_t rows, uint32_t layout,
+                    bool to_device)
+{
+    TpccState &tp = ctx->tpcc;
+    if (table_id >= (uint32_t)kTpccTables) LTPG_FAIL(LTPG_ERR_INVALID, "unknown table id");
+    if (!host) LTPG_FAIL(LTPG_ERR_INVALID, "null host table");
+    if (layout != LTPG_LAYOUT_ROW_MAJOR && layout != LTPG_LAYOUT_COLUMN_MAJOR) LTPG_FAIL(LTPG_ERR_INVALID, "unknown layout");
+    const TpccTable &tb = tp.d.t[table_id];
+    if (rows != tb.rows) LTPG_FAIL(LTPG_ERR_INVALID, "rows does not match the table's row count");
+    const uint64_t words = tb.rows * tb.cols;
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t *tmp = nullptr;
+    int rc = dev_alloc(&tmp, words, "table staging");
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    if (to_device) {
+        e = hipMemcpyAsync(tmp, host, words * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = launch_tpcc_relayout(tmp, tb.data, tb.rows, tb.cols, tb.stride, layout, 1, ctx->stream);
+        if (e == hipSuccess && table_id == LTPG_TPCC_CUSTOMER) e = launch_tpcc_name_index(tp.d, ctx->stream);
+    } else {
+        e = launch_tpcc_relayout(tb.data, tmp, tb.rows, tb.cols, tb.stride, layout, 0, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, words * 4, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(tmp);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("order-entry table copy: ") + hipGetErrorString(e));
+    return LTPG_OK;
+}
+}  // namespace
+
+int ltpg_tpcc_table_shape(ltpg_ctx *ctx, uint32_t table_id, uint64_t *rows, uint32_t *cols)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    if (table_id >= (uint32_t)kTpccTables) LTPG_FAIL(LTPG_ERR_INVALID, "unknown table id");
+    if (rows) *rows = ctx->tpcc.d.t[table_id].rows;
+    if (cols) *cols = ctx->tpcc.d.t[table_id].cols;
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_table_load(ltpg_ctx *ctx, uint32_t table_id, const uint32_t *host, uint64_t rows, uint32_t layout)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    return tpcc_copy_table(ctx, table_id, const_cast<uint32_t *>(host), rows, layout, true);
+}
+
+int ltpg_tpcc_table_read(ltpg_ctx *ctx, uint32_t table_id, uint32_t *host, uint64_t rows, uint32_t layout)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    return tpcc_copy_table(ctx, table_id, host, rows, layout, false);
+}
+
+int ltpg_tpcc_populate(ltpg_ctx *ctx, uint64_t seed)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    LTPG_HIP_TRY(launch_tpcc_populate(ctx->tpcc.d, seed, ctx->stream));
+    LTPG_HIP_TRY(launch_tpcc_name_index(ctx->tpcc.d, ctx->stream));
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_payment_batch *pb, uint32_t mem)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    TpccState &tp = ctx->tpcc;
+    if (mem != LTPG_MEM_HOST && mem != LTPG_MEM_DEVICE) LTPG_FAIL(LTPG_ERR_INVALID, "unknown memory kind");
+    const uint32_t n_no = nb ? nb->n : 0, n_pay = pb ? pb->n : 0;
+    if ((uint64_t)n_no + n_pay > ctx->cfg.max_txns) LTPG_FAIL(LTPG_ERR_INVALID, "batch exceeds max_txns");
+    if (n_no) {
+        if (nb->line_stride < n_no) LTPG_FAIL(LTPG_ERR_INVALID, "line_stride must be >= n");
+        if (!nb->W_ID || !nb->D_ID || !nb->C_ID || !nb->O_ID || !nb->N_O_ID || !nb->O_OL_CNT || !nb->O_OL_ID ||
+            !nb->OL_I_ID || !nb->OL_SUPPLY_W_ID || !nb->OL_QUANTITY || !nb->TID)
+            LTPG_FAIL(LTPG_ERR_INVALID, "null NewOrder array");
+    }
+    if (n_pay) {
+        if (!pb->W_ID || !pb->D_ID || !pb->C_ID || !pb->C_LAST || !pb->isName || !pb->C_D_ID || !pb->C_W_ID ||
+            !pb->H_AMOUNT || !pb->H_ID || !pb->TID)
+            LTPG_FAIL(LTPG_ERR_INVALID, "null Payment array");
+    }
+    if ((rc = next_epoch(ctx))) return rc;
+    NeworderDev no;
+    PaymentDev pay;
+    no.n = n_no;
+    pay.n = n_pay;
+    if (mem == LTPG_MEM_DEVICE) {
+        if (n_no) {
+            no.line_stride = nb->line_stride;
+            no.W_ID = nb->W_ID; no.D_ID = nb->D_ID; no.C_ID = nb->C_ID; no.O_ID = nb->O_ID; no.N_O_ID = nb->N_O_ID;
+            no.O_OL_CNT = nb->O_OL_CNT; no.O_OL_ID = nb->O_OL_ID; no.OL_I_ID = nb->OL_I_ID;
+            no.OL_SUPPLY_W_ID = nb->OL_SUPPLY_W_ID; no.OL_QUANTITY = nb->OL_QUANTITY; no.TID = nb->TID;
+        }
+        if (n_pay) {
+            pay.W_ID = pb->W_ID; pay.D_ID = pb->D_ID; pay.C_ID = pb->C_ID; pay.C_LAST = pb->C_LAST;
+            pay.isName = pb->isName; pay.C_D_ID = pb->C_D_ID; pay.C_W_ID = pb->C_W_ID; pay.H_AMOUNT = pb->H_AMOUNT;
+            pay.H_ID = pb->H_ID; pay.TID = pb->TID;
+        }
+    } else {
+        // pack into the library's NEWORDER_QUERY / PAYMENT_QUERY blocks
+        const uint64_t C = ctx->cfg.max_txns;
+        hipStream_t s = ctx->stream;
+        if (n_no) {
+            uint32_t *b = tp.no_buf;
+            const uint32_t *heads[7] = {nb->W_ID, nb->D_ID, nb->C_ID, nb->O_ID, nb->N_O_ID, nb->O_OL_CNT, nb->O_OL_ID};
+            for (int k = 0; k < 7; k++)
+                LTPG_HIP_TRY(hipMemcpyAsync(b + k * C, heads[k], (uint64_t)n_no * 4, hipMemcpyHostToDevice, s));
+            const uint32_t *lines[3] = {nb->OL_I_ID, nb->OL_SUPPLY_W_ID, nb->OL_QUANTITY};
+            for (int k = 0; k < 3; k++)
+                LTPG_HIP_TRY(hipMemcpy2DAsync(b + (7 + 15 * k) * C, C * 4, lines[k], (uint64_t)nb->line_stride * 4,
+                                              (uint64_t)n_no * 4, LTPG_TPCC_MAX_LINES, hipMemcpyHostToDevice, s));
+            LTPG_HIP_TRY(hipMemcpyAsync(b + 52 * C, nb->TID, (uint64_t)n_no * 4, hipMemcpyHostToDevice, s));
+            no.line_stride = (uint32_t)C;
+            no.W_ID = b; no.D_ID = b + C; no.C_ID = b + 2 * C; no.O_ID = b + 3 * C; no.N_O_ID = b + 4 * C;
+            no.O_OL_CNT = b + 5 * C; no.O_OL_ID = b + 6 * C; no.OL_I_ID = b + 7 * C; no.OL_SUPPLY_W_ID = b + 22 * C;
+            no.OL_QUANTITY = b + 37 * C; no.TID = b + 52 * C;
+        }
+        if (n_pay) {
+            uint32_t *b = tp.pay_buf;
+            const uint32_t *cols[10] = {pb->W_ID, pb->D_ID, pb->C_ID, pb->C_LAST, pb->isName,
+                                        pb->C_D_ID, pb->C_W_ID, pb->H_AMOUNT, pb->H_ID, pb->TID};
+            for (int k = 0; k < 10; k++)
+                LTPG_HIP_TRY(hipMemcpyAsync(b + k * C, cols[k], (uint64_t)n_pay * 4, hipMemcpyHostToDevice, s));
+            pay.W_ID = b; pay.D_ID = b + C; pay.C_ID = b + 2 * C; pay.C_LAST = b + 3 * C; pay.isName = b + 4 * C;
+            pay.C_D_ID = b + 5 * C; pay.C_W_ID = b + 6 * C; pay.H_AMOUNT = b + 7 * C; pay.H_ID = b + 8 * C;
+            pay.TID = b + 9 * C;
+        }
+        // pageable sources: the copies finish before the caller may reuse them
+        LTPG_HIP_TRY(hipStreamSynchronize(s));
+    }
+    // Until a CID phase runs, each Payment uses its query's C_ID.
+    if (n_pay)
+        LTPG_HIP_TRY(hipMemcpyAsync(tp.d.cid, pay.C_ID, (uint64_t)n_pay * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    tp.no = no;
+    tp.pay = pay;
+    tp.has_batch = true;
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_payment_cid(ltpg_ctx *ctx, uint32_t *out, uint32_t n)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    if (n > ctx->tpcc.pay.n) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the Payment batch size");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) LTPG_HIP_TRY(hipMemcpy(out, ctx->tpcc.d.cid, (uint64_t)n * 4, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t n_words)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    const uint32_t *src = nullptr;
+    uint64_t have = 0;
+    if (type == LTPG_TYPE_NEWORDER) {
+        src = ctx->tpcc.d.reads_no;
+        have = (uint64_t)kNoReads * ctx->tpcc.no.n;
+    } else if (type == LTPG_TYPE_PAYMENT) {
+        src = ctx->tpcc.d.reads_pay;
+        have = (uint64_t)kPayReads * ctx->tpcc.pay.n;
+    } else {
+        LTPG_FAIL(LTPG_ERR_INVALID, "type must be NEWORDER or PAYMENT");
+    }
+    if (n_words > have) LTPG_FAIL(LTPG_ERR_INVALID, "n_words exceeds the batch's read values");
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n_words) LTPG_HIP_TRY(hipMemcpy(out, src, n_words * 4, hipMemcpyDeviceToHost));
     return LTPG_OK;
 }
 

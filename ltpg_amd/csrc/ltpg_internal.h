@@ -79,6 +79,66 @@ struct KvState {
     bool has_batch = false;
 };
 
+// ---- order-entry state -------------------------------------------------
+
+constexpr int kTpccTables = LTPG_TPCC_NTABLES;
+constexpr uint32_t kNoReads = 51;   // read values per NewOrder
+constexpr uint32_t kPayReads = 24;  // read values per Payment
+
+// One table on the device: row-major, `stride` u32 words per row (cols
+// rounded up to a multiple of 4 so rows start 16-byte aligned).
+struct TpccTable {
+    uint32_t *data = nullptr;
+    uint64_t rows = 0;
+    uint32_t cols = 0;
+    uint32_t stride = 0;
+};
+
+// A NewOrder / Payment batch as the kernels see it (device pointers; the
+// fields of ltpg_neworder_batch / ltpg_payment_batch).
+struct NeworderDev {
+    uint32_t n = 0;
+    uint32_t line_stride = 0;
+    const uint32_t *W_ID = nullptr, *D_ID = nullptr, *C_ID = nullptr, *O_ID = nullptr, *N_O_ID = nullptr,
+                   *O_OL_CNT = nullptr, *O_OL_ID = nullptr, *OL_I_ID = nullptr, *OL_SUPPLY_W_ID = nullptr,
+                   *OL_QUANTITY = nullptr, *TID = nullptr;
+};
+struct PaymentDev {
+    uint32_t n = 0;
+    const uint32_t *W_ID = nullptr, *D_ID = nullptr, *C_ID = nullptr, *C_LAST = nullptr, *isName = nullptr,
+                   *C_D_ID = nullptr, *C_W_ID = nullptr, *H_AMOUNT = nullptr, *H_ID = nullptr, *TID = nullptr;
+};
+
+// Everything the order-entry kernels take, passed by value.
+struct TpccDev {
+    uint32_t warehouses = 0;
+    uint32_t rules = 0;
+    TpccTable t[kTpccTables];
+    // Registers (tag:8 | TID:32 | pos:24, lower wins), only those a check
+    // can observe: CUSTOMER R and W (interleaved, 2 per row), then W of
+    // ORDER, STOCK, ORDERLINE, HISTORY and NEWORDER.
+    unsigned long long *regs = nullptr;
+    uint64_t off_order = 0, off_stock = 0, off_line = 0, off_hist = 0, off_new = 0, n_regs = 0;
+    unsigned long long *name_index = nullptr;  // [10W * 3000] sorted C_LAST << 32 | row in district
+    // NEWORDER rows: epoch:8 | TID:32 | pos:24 of the last committed inserter
+    // (largest); only that inserter writes the row.
+    unsigned long long *new_last = nullptr;
+    uint8_t *doomed = nullptr;                 // [max_txns] epoch number if waw > 0
+    uint64_t *bitmap_no = nullptr, *bitmap_pay = nullptr;
+    uint32_t *conf_no = nullptr, *conf_pay = nullptr;
+    uint32_t *reads_no = nullptr, *reads_pay = nullptr;  // [51 * n_no], [24 * n_pay] value-major
+    uint32_t *cid = nullptr;                             // [n_pay] C_ID each Payment uses
+};
+
+struct TpccState {
+    TpccDev d;
+    uint32_t *no_buf = nullptr;   // packed NEWORDER_QUERY, capacity max_txns
+    uint32_t *pay_buf = nullptr;  // packed PAYMENT_QUERY, capacity max_txns
+    NeworderDev no;
+    PaymentDev pay;
+    bool has_batch = false;
+};
+
 }  // namespace ltpg
 
 struct ltpg_ctx {
@@ -89,6 +149,7 @@ struct ltpg_ctx {
     uint32_t epoch = 1;  // 1..255; 0 owns the all-ones fill value of the registers
     ltpg::DeviceCounters *counters = nullptr;
     ltpg::KvState kv;
+    ltpg::TpccState tpcc;
     // Per-step timing: each timed epoch takes LTPG_TIMING_SLOTS + 1 events
     // (start, after CID, after EXECUTE, after CHECK, after the read-out of
     // WRITEBACK, after WRITEBACK).
@@ -148,5 +209,18 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
 hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s);
 hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
                                hipStream_t s);
+
+// Order-entry launchers (tpcc_kernels.hip).
+hipError_t launch_tpcc_populate(const TpccDev &d, uint64_t seed, hipStream_t s);
+hipError_t launch_tpcc_name_index(const TpccDev &d, hipStream_t s);
+hipError_t launch_tpcc_relayout(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
+                                uint32_t stride, uint32_t layout, int to_device, hipStream_t s);
+hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t s);
+hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
+                               const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
+                             const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
+                                 const EpochTag &e, uint32_t values, hipStream_t s);
 
 }  // namespace ltpg

@@ -1,0 +1,685 @@
+// tpcc_kernels.hip - order-entry (NewOrder + Payment) kernels for gfx950.
+//
+// The reference runs one warp "role" per op slot per 32 transactions
+// (KERNEL_EXECUTE / KERNEL_CHECK / KERNEL_WRITEBACK,
+// LTPG/TPCC-allopt-final/tp.cu:24-117, with the role tables of
+// Execute_neworder.h and Execute_payment.h), registering every slot and
+// staging every column into a set_local_set array.  Here:
+//
+//   CID      one thread per Payment; a by-name Payment binary-searches its
+//            district's sorted (C_LAST, row) index.
+//   EXECUTE  registers only the registers some check can observe.  Under the
+//            reference's dispatch (Execute_neworder.h:620-680,
+//            Execute_payment.h:547-595) the WAREHOUSE, DISTRICT and ITEM
+//            registers and the NEWORDER insert's R register are never
+//            consulted, or consulted only on a register nobody lowers, so
+//            they cannot change any outcome.  What remains is W of CUSTOMER
+//            (Payment), ORDER, STOCK, ORDERLINE, HISTORY and NEWORDER, plus
+//            R of CUSTOMER when the raw / war / waw counters are requested.
+//            NewOrders run as 16 waves per 64 transactions (15 line roles and
+//            a header role, wave-uniform), Payments one thread each.  waw is
+//            found here, by displacement (device_common.h).
+//   CHECK    every war of a NewOrder or Payment comes from a register that
+//            nobody lowers except CUSTOMER R, and a Payment's CUSTOMER raw
+//            (check_4_0) reads the same W register as its waw (check_5_1);
+//            so raw && war implies waw and the rule waw == 0 && (raw == 0 ||
+//            war == 0) is waw == 0, which EXECUTE has already decided.  CHECK
+//            reads the doomed flag, writes the commit bitmap and gathers the
+//            committed transactions' read values from the snapshot.  With
+//            conflict counters it recomputes raw / war / waw from the
+//            registers as the reference's check slots do.
+//   WRITEBACK one thread per committed transaction, its lines in order.
+//
+// Tables are row-major, `stride` (cols rounded up to 4) u32 words per row.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "ltpg_internal.h"
+
+namespace ltpg {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+enum : int {
+    WH = LTPG_TPCC_WAREHOUSE,
+    DI = LTPG_TPCC_DISTRICT,
+    CU = LTPG_TPCC_CUSTOMER,
+    HI = LTPG_TPCC_HISTORY,
+    NO = LTPG_TPCC_NEWORDER,
+    OR = LTPG_TPCC_ORDER,
+    OL = LTPG_TPCC_ORDERLINE,
+    ST = LTPG_TPCC_STOCK,
+    IT = LTPG_TPCC_ITEM,
+};
+
+// Row strides (u32 words) of the device layout.
+constexpr uint32_t kStride[kTpccTables] = {12, 12, 24, 8, 4, 8, 12, 20, 8};
+
+__device__ __forceinline__ uint64_t smix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// The loader's uniform draw in [a, b] for (seed, stream, row).
+__device__ __forceinline__ uint32_t pop_uniform(uint64_t seed, uint32_t stream, uint64_t row, uint32_t a,
+                                                uint32_t b)
+{
+    const uint64_t h = smix(smix(seed ^ 0x504F50554C415445ull) ^ ((uint64_t)stream << 48) ^ row);
+    return a + (uint32_t)__umul64hi(h, (uint64_t)b - a + 1);
+}
+
+// ---- loader (create_warehouse .. create_item, Database.h:104-327) -------
+
+template <int TABLE>
+__device__ __forceinline__ void row_values(uint64_t i, uint64_t seed, uint32_t (&v)[kStride[TABLE]])
+{
+#pragma unroll
+    for (uint32_t k = 0; k < kStride[TABLE]; k++) v[k] = 0;
+    if constexpr (TABLE == WH) {
+        v[0] = (uint32_t)i; v[1] = 1; v[2] = 1; v[6] = 10;
+    } else if constexpr (TABLE == DI) {
+        v[0] = (uint32_t)(i % 10); v[1] = (uint32_t)(i / 10); v[2] = 1; v[3] = 1;
+    } else if constexpr (TABLE == CU) {
+        v[0] = (uint32_t)(i % 3000); v[1] = (uint32_t)(i / 30000); v[2] = (uint32_t)((i / 3000) % 10);
+#pragma unroll
+        for (int c = 3; c <= 9; c++) v[c] = 100000;
+        v[10] = pop_uniform(seed, CU, i, 0, 1);
+        v[13] = (uint32_t)(i % 20);
+    } else if constexpr (TABLE == OR) {
+        v[0] = (uint32_t)(i % 30000); v[1] = (uint32_t)(i / 3000); v[2] = (uint32_t)(i / 30000);
+        v[3] = pop_uniform(seed, OR, i, 0, 2999);
+    } else if constexpr (TABLE == OL) {
+        v[0] = (uint32_t)(i % 450000); v[1] = (uint32_t)(i / 45000); v[2] = (uint32_t)(i / 450000);
+        v[4] = pop_uniform(seed, OL, i, 0, 99999); v[5] = (uint32_t)(i / 450000);
+    } else if constexpr (TABLE == ST) {
+        v[0] = (uint32_t)(i % 100000); v[1] = (uint32_t)(i / 100000); v[2] = 100000; v[3] = 100000;
+#pragma unroll
+        for (int dd = 1; dd <= 10; dd++) v[5 + dd] = dd;
+    } else if constexpr (TABLE == IT) {
+        v[0] = (uint32_t)i; v[1] = 1000;
+        v[2] = pop_uniform(seed, IT, i, 1, 40000);
+        v[3] = pop_uniform(seed, IT + 1, i, 1, 10);
+        v[4] = (uint32_t)i;
+    }
+    // HISTORY and NEWORDER start empty.
+}
+
+template <int TABLE>
+__global__ __launch_bounds__(kBlock) void populate_kernel(uint32_t *__restrict__ data, uint64_t rows, uint64_t seed)
+{
+    constexpr uint32_t S = kStride[TABLE];
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < rows; i += step) {
+        uint32_t v[S];
+        row_values<TABLE>(i, seed, v);
+        uint4 *dst = reinterpret_cast<uint4 *>(data + i * S);
+#pragma unroll
+        for (uint32_t k = 0; k < S / 4; k++) dst[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    }
+}
+
+// ---- by-name index -------------------------------------------------------
+
+// One workgroup per district: sort its 3000 (C_LAST, row-in-district) keys
+// in LDS (bitonic, padded to 4096) so that the k-th holder of a name, in row
+// order, is entry lower_bound(name) + k.  The reference keeps C_LAST in row
+// order and scans it (Database.h:182, Genericfunction.h:29-77).
+constexpr uint32_t kPerDistrict = 3000;
+constexpr uint32_t kSortN = 4096;
+
+__global__ __launch_bounds__(kBlock) void name_index_kernel(const uint32_t *__restrict__ customer,
+                                                           unsigned long long *__restrict__ index)
+{
+    __shared__ unsigned long long keys[kSortN];
+    const uint64_t base = (uint64_t)blockIdx.x * kPerDistrict;
+    for (uint32_t i = threadIdx.x; i < kSortN; i += kBlock)
+        keys[i] = i < kPerDistrict
+                      ? ((unsigned long long)customer[(base + i) * kStride[CU] + 13] << 32) | i
+                      : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= kSortN; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < kSortN / 2; t += kBlock) {
+                const uint32_t a = (t / j) * 2 * j + (t % j), b = a + j;
+                const bool up = (a & k) == 0;
+                const unsigned long long x = keys[a], y = keys[b];
+                if ((x > y) == up) {
+                    keys[a] = y;
+                    keys[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < kPerDistrict; i += kBlock) index[base + i] = keys[i];
+}
+
+// ---- layout conversion ------------------------------------------------------
+
+// to_device: dense host layout (column-major src[c * rows + r] or row-major
+// src[r * cols + c]) -> strided device rows (padding zeroed); else back.
+__global__ __launch_bounds__(kBlock) void relayout_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                                         uint64_t rows, uint32_t cols, uint32_t stride,
+                                                         uint32_t layout, int to_device)
+{
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < rows; r += step) {
+        for (uint32_t c = 0; c < stride; c++) {
+            const uint64_t dense = layout == LTPG_LAYOUT_COLUMN_MAJOR ? (uint64_t)c * rows + r : r * cols + c;
+            if (to_device)
+                dst[r * stride + c] = c < cols ? src[dense] : 0u;
+            else if (c < cols)
+                dst[dense] = src[r * stride + c];
+        }
+    }
+}
+
+// ---- CID (KERNEL_CID / get_c_id, tp.cu:11-23, Execute_payment.h:13-36) --
+
+// C_ID of the k-th (0-based, k = C_LAST & 0xff) customer of district
+// C_W_ID * 10 + C_D_ID whose C_LAST is C_LAST >> 8; 0 when there is none.
+__global__ __launch_bounds__(kBlock) void cid_kernel(TpccDev d, PaymentDev pay)
+{
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= pay.n) return;
+    uint32_t c = pay.C_ID[p];
+    if (pay.isName[p]) {
+        c = 0;
+        const uint64_t dist = (uint64_t)pay.C_W_ID[p] * 10 + pay.C_D_ID[p];
+        if (dist < d.t[DI].rows) {
+            const uint32_t cl = pay.C_LAST[p], name = cl >> 8, k = cl & 0xffu;
+            const unsigned long long *idx = d.name_index + dist * kPerDistrict;
+            const unsigned long long key = (unsigned long long)name << 32;
+            uint32_t lo = 0, hi = kPerDistrict;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (idx[mid] < key) lo = mid + 1; else hi = mid;
+            }
+            const uint32_t j = lo + k;
+            if (j < kPerDistrict) {
+                const unsigned long long e = idx[j];
+                if ((uint32_t)(e >> 32) == name)
+                    c = d.t[CU].data[(dist * kPerDistrict + (uint32_t)e) * kStride[CU]];
+            }
+        }
+    }
+    d.cid[p] = c;
+}
+
+// ---- row formulas (u64; the reference's u32 forms, Execute_neworder.h:22-266,
+//      Execute_payment.h:47-286) --------------------------------------------
+
+struct NoRows {
+    uint64_t wh, di, cu, no, od;
+};
+
+__device__ __forceinline__ NoRows no_rows(const NeworderDev &no, uint32_t q)
+{
+    NoRows r;
+    const uint64_t w = no.W_ID[q], dd = no.D_ID[q];
+    r.wh = w;
+    r.di = w * 10 + dd;
+    r.cu = w * 30000 + dd * 3000 + no.C_ID[q];
+    r.no = w * 30000 + no.N_O_ID[q];
+    r.od = w * 30000 + no.O_ID[q];
+    return r;
+}
+
+__device__ __forceinline__ uint32_t no_lines(const NeworderDev &no, uint32_t q)
+{
+    return min(no.O_OL_CNT[q], (uint32_t)LTPG_TPCC_MAX_LINES);
+}
+
+struct PayRows {
+    uint64_t wh, di, cu, hi;
+};
+
+__device__ __forceinline__ PayRows pay_rows(const TpccDev &d, const PaymentDev &pay, uint32_t p)
+{
+    PayRows r;
+    const uint64_t w = pay.W_ID[p];
+    r.wh = w;
+    r.di = w * 10 + pay.D_ID[p];
+    r.cu = (uint64_t)pay.C_W_ID[p] * 30000 + (uint64_t)pay.C_D_ID[p] * 3000 + d.cid[p];
+    r.hi = w * 30000 + pay.H_ID[p];
+    return r;
+}
+
+// Register addresses.
+__device__ __forceinline__ unsigned long long *cust_r(const TpccDev &d, uint64_t row) { return d.regs + 2 * row; }
+__device__ __forceinline__ unsigned long long *cust_w(const TpccDev &d, uint64_t row) { return d.regs + 2 * row + 1; }
+
+// ---- EXECUTE ---------------------------------------------------------------
+
+template <bool COUNTS>
+__global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
+                                                        uint8_t epoch, uint32_t no_waves, DeviceCounters *ctr)
+{
+    // the epoch's commit counts start at zero (CHECK adds to them)
+    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    uint32_t bad = 0;
+    if (wave < no_waves) {
+        const uint32_t role = wave % 16;
+        const uint32_t q = (wave / 16) * 64 + lane;
+        if (q < no.n) {
+            const unsigned long long me = priority(tag, no.TID[q], q);
+            if (role < LTPG_TPCC_MAX_LINES) {
+                // line role: STOCK W (op_6_0), ORDERLINE W (op_7_0); the ITEM
+                // slot (op_5_0) registers a register no check consults
+                if (role < no_lines(no, q)) {
+                    const uint64_t at = (uint64_t)role * no.line_stride + q;
+                    const uint32_t item = no.OL_I_ID[at];
+                    const uint64_t sr = (uint64_t)no.OL_SUPPLY_W_ID[at] * 100000 + item;
+                    const uint64_t lr = (uint64_t)no.W_ID[q] * 450000 + no.O_OL_ID[q] + role;
+                    bad += item >= d.t[IT].rows;
+                    bool waw = false;
+                    if (sr < d.t[ST].rows) waw |= register_write(d.regs + d.off_stock + sr, me, tag, d.doomed, epoch);
+                    else bad++;
+                    if (lr < d.t[OL].rows) waw |= register_write(d.regs + d.off_line + lr, me, tag, d.doomed, epoch);
+                    else bad++;
+                    if (waw) d.doomed[q] = epoch;
+                }
+            } else {
+                // header role: slots 0..4
+                const NoRows r = no_rows(no, q);
+                bad += r.wh >= d.t[WH].rows;
+                bad += r.di >= d.t[DI].rows;
+                if (r.cu < d.t[CU].rows) {
+                    if (COUNTS) register_read(cust_r(d, r.cu), me);
+                } else bad++;
+                // NEWORDER insert: a write (waw) under INTENDED; under
+                // REFERENCE its R register is never consulted
+                bool waw = false;
+                if (r.no < d.t[NO].rows) {
+                    if (d.rules == LTPG_RULES_INTENDED)
+                        waw |= register_write(d.regs + d.off_new + r.no, me, tag, d.doomed, epoch);
+                } else bad++;
+                if (r.od < d.t[OR].rows) waw |= register_write(d.regs + d.off_order + r.od, me, tag, d.doomed, epoch);
+                else bad++;
+                if (waw) d.doomed[q] = epoch;
+            }
+        }
+    } else {
+        const uint32_t p = (wave - no_waves) * 64 + lane;
+        if (p < pay.n) {
+            const unsigned long long me = priority(tag, pay.TID[p], no.n + p);
+            const PayRows r = pay_rows(d, pay, p);
+            bad += r.wh >= d.t[WH].rows ? 2 : 0;  // slots 0, 1
+            bad += r.di >= d.t[DI].rows ? 2 : 0;  // slots 2, 3
+            bool waw = false;
+            if (r.cu < d.t[CU].rows) {            // slots 4, 5
+                if (COUNTS) register_read(cust_r(d, r.cu), me);
+                waw |= register_write(cust_w(d, r.cu), me, tag, d.doomed, epoch);
+            } else bad += 2;
+            if (r.hi < d.t[HI].rows) waw |= register_write(d.regs + d.off_hist + r.hi, me, tag, d.doomed, epoch);
+            else bad++;
+            if (waw) d.doomed[no.n + p] = epoch;
+        }
+    }
+    if (bad) atomicAdd(&ctr->range_errors, bad);
+}
+
+// ---- CHECK + read-out of committed reads -------------------------------------
+
+__device__ __forceinline__ uint32_t lower(const unsigned long long *r, unsigned long long me)
+{
+    return *r < me ? 1u : 0u;
+}
+
+// Committed NewOrder q: W_TAX; D_TAX, D_NEXT_O_ID; C_DISCOUNT, C_CREDIT,
+// C_LAST; per line I_PRICE, I_DATA, I_NAME (op_0_1, op_1_1, op_2_1, op_5_1).
+__device__ __forceinline__ void gather_neworder(const TpccDev &d, const NeworderDev &no, uint32_t q)
+{
+    const NoRows r = no_rows(no, q);
+    const uint32_t L = no_lines(no, q);
+    const uint32_t n = no.n;
+    uint32_t *out = d.reads_no + q;
+    const bool okw = r.wh < d.t[WH].rows, okd = r.di < d.t[DI].rows, okc = r.cu < d.t[CU].rows;
+    const uint32_t *w = d.t[WH].data + (okw ? r.wh : 0) * kStride[WH];
+    const uint32_t *di = d.t[DI].data + (okd ? r.di : 0) * kStride[DI];
+    const uint32_t *cu = d.t[CU].data + (okc ? r.cu : 0) * kStride[CU];
+    uint32_t v[kNoReads];
+    v[0] = w[1];
+    v[1] = di[2];
+    v[2] = di[4];
+    v[3] = cu[4];
+    v[4] = cu[10];
+    v[5] = cu[13];
+    // every line slot exists in the batch arrays (15 * line_stride), so the
+    // item ids load unconditionally and the loads of all lines overlap
+    uint32_t ok_items = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < LTPG_TPCC_MAX_LINES; i++) {
+        const uint32_t item = no.OL_I_ID[(uint64_t)i * no.line_stride + q];
+        const bool ok = i < L && item < d.t[IT].rows;
+        ok_items |= (uint32_t)ok << i;
+        const uint32_t *it = d.t[IT].data + (uint64_t)(ok ? item : 0) * kStride[IT];
+        v[6 + 3 * i] = it[2];
+        v[7 + 3 * i] = it[3];
+        v[8 + 3 * i] = it[4];
+    }
+    if (!okw) v[0] = 0;
+    if (!okd) v[1] = v[2] = 0;
+    if (!okc) v[3] = v[4] = v[5] = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < LTPG_TPCC_MAX_LINES; i++)
+        if (!((ok_items >> i) & 1u)) v[6 + 3 * i] = v[7 + 3 * i] = v[8 + 3 * i] = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kNoReads; k++) out[(uint64_t)k * n] = v[k];
+}
+
+// Committed Payment p: warehouse cols 3..8, district cols 5..10, customer
+// cols 3, 4, 11..20 (op_0_1/0_2, op_2_1/2_2, op_4_1..4_4).
+__device__ __forceinline__ void gather_payment(const TpccDev &d, const PaymentDev &pay, uint32_t p,
+                                               const PayRows &r)
+{
+    const uint32_t n = pay.n;
+    uint32_t *out = d.reads_pay + p;
+    const bool okw = r.wh < d.t[WH].rows, okd = r.di < d.t[DI].rows, okc = r.cu < d.t[CU].rows;
+    const uint32_t *w = d.t[WH].data + (okw ? r.wh : 0) * kStride[WH];
+    const uint32_t *di = d.t[DI].data + (okd ? r.di : 0) * kStride[DI];
+    const uint32_t *cu = d.t[CU].data + (okc ? r.cu : 0) * kStride[CU];
+    uint32_t v[kPayReads];
+#pragma unroll
+    for (uint32_t k = 0; k < 6; k++) v[k] = okw ? w[3 + k] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 6; k++) v[6 + k] = okd ? di[5 + k] : 0u;
+    v[12] = okc ? cu[3] : 0u;
+    v[13] = okc ? cu[4] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 10; k++) v[14 + k] = okc ? cu[11 + k] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kPayReads; k++) out[(uint64_t)k * n] = v[k];
+}
+
+template <bool COUNTS>
+__global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
+                                                      uint8_t epoch, uint32_t no_waves, DeviceCounters *ctr)
+{
+    __shared__ uint32_t s_cnt[4];  // NewOrder commits, attempts, Payment commits, attempts
+    if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    bool commit = false, live = false;
+    const bool is_no = wave < no_waves;
+    if (is_no) {
+        const uint32_t q = wave * 64 + lane;
+        live = q < no.n;
+        if (live) {
+            if (COUNTS) {
+                // check dispatch as it runs (Execute_neworder.h:620-680); the
+                // dropped slots read registers nobody lowers: their count is 0
+                const unsigned long long me = priority(tag, no.TID[q], q);
+                const NoRows r = no_rows(no, q);
+                uint32_t raw = 0, waw = 0;
+                if (r.cu < d.t[CU].rows) raw += lower(cust_w(d, r.cu), me);                      // check_2_0
+                if (d.rules == LTPG_RULES_INTENDED && r.no < d.t[NO].rows)
+                    waw += lower(d.regs + d.off_new + r.no, me);                                 // check_3_1
+                if (r.od < d.t[OR].rows) waw += lower(d.regs + d.off_order + r.od, me);          // check_4_1
+                const uint32_t L = no_lines(no, q);
+                for (uint32_t i = 0; i < L; i++) {
+                    const uint64_t at = (uint64_t)i * no.line_stride + q;
+                    const uint64_t sr = (uint64_t)no.OL_SUPPLY_W_ID[at] * 100000 + no.OL_I_ID[at];
+                    const uint64_t lr = (uint64_t)no.W_ID[q] * 450000 + no.O_OL_ID[q] + i;
+                    if (sr < d.t[ST].rows) waw += lower(d.regs + d.off_stock + sr, me);          // check_6_1
+                    if (lr < d.t[OL].rows) waw += lower(d.regs + d.off_line + lr, me);           // check_7_1
+                }
+                d.conf_no[q] = (raw & 0xffu) | ((waw & 0xffu) << 16);
+                commit = waw == 0;
+            } else {
+                commit = d.doomed[q] != epoch;
+            }
+            if (commit) {
+                gather_neworder(d, no, q);
+                // the last committed inserter of a NEWORDER row writes it
+                const uint64_t nr = (uint64_t)no.W_ID[q] * 30000 + no.N_O_ID[q];
+                if (nr < d.t[NO].rows)
+                    atomicMax(d.new_last + nr, ((unsigned long long)epoch << kTagShift) |
+                                                   ((unsigned long long)no.TID[q] << kTidShift) | q);
+            }
+        }
+    } else {
+        const uint32_t p = (wave - no_waves) * 64 + lane;
+        live = p < pay.n;
+        if (live) {
+            const PayRows r = pay_rows(d, pay, p);
+            if (COUNTS) {
+                const unsigned long long me = priority(tag, pay.TID[p], no.n + p);
+                uint32_t raw = 0, war = 0, waw = 0;
+                if (r.cu < d.t[CU].rows) {
+                    raw += lower(cust_w(d, r.cu), me);                                           // check_4_0
+                    war += lower(cust_r(d, r.cu), me);                                           // check_5_0
+                    waw += lower(cust_w(d, r.cu), me);                                           // check_5_1
+                }
+                if (r.hi < d.t[HI].rows) waw += lower(d.regs + d.off_hist + r.hi, me);           // check_6_1
+                d.conf_pay[p] = (raw & 0xffu) | ((war & 0xffu) << 8) | ((waw & 0xffu) << 16);
+                commit = waw == 0 && (raw == 0 || war == 0);
+            } else {
+                commit = d.doomed[no.n + p] != epoch;
+            }
+            if (commit) gather_payment(d, pay, p, r);
+        }
+    }
+    const unsigned long long bits = __ballot(commit);
+    const unsigned long long lv = __ballot(live);
+    if (lane == 0 && lv) {
+        if (is_no) d.bitmap_no[wave] = bits;
+        else d.bitmap_pay[wave - no_waves] = bits;
+        atomicAdd(&s_cnt[is_no ? 0 : 2], (uint32_t)__popcll(bits));
+        atomicAdd(&s_cnt[is_no ? 1 : 3], (uint32_t)__popcll(lv));
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const uint32_t k = threadIdx.x;  // 0: NewOrder, 1: Payment
+        const uint32_t c = s_cnt[2 * k], a = s_cnt[2 * k + 1];
+        if (a) {
+            const uint32_t stripe = blockIdx.x % kCounterStripes;
+            const uint32_t type = k == 0 ? LTPG_TYPE_NEWORDER : LTPG_TYPE_PAYMENT;
+            if (c) {
+                atomicAdd(&ctr->commits[stripe][type], c);
+                atomicAdd(&ctr->totals[stripe][type], (unsigned long long)c);
+            }
+            atomicAdd(&ctr->totals[stripe][3 + type], (unsigned long long)a);
+        }
+    }
+}
+
+// ---- WRITEBACK ---------------------------------------------------------------
+
+// Committed transactions write their write slots (write_back dispatch,
+// Execute_neworder.h:698-850, Execute_payment.h:613-736).  RMW values:
+//   NEWORDER <- (O_ID, D_ID, W_ID), by its last committed inserter in
+//               priority order only (under the reference rules two inserters
+//               of one row do not conflict);
+//   ORDER    <- (O_ID, D_ID, W_ID);
+//   per line, in line order: STOCK (S_QUANTITY, S_YTD, S_ORDER_CNT) <-
+//     (q >= qty + 10 ? q - qty : q - qty + 91, S_YTD + qty, S_ORDER_CNT + 1),
+//     ORDERLINE <- (O_ID, D_ID, W_ID, line, I_ID, SUPPLY_W_ID, 0, qty,
+//     qty * I_PRICE, I_DATA);
+//   CUSTOMER C_BALANCE -= H_AMOUNT, C_DATA += H_AMOUNT;
+//   HISTORY  <- (C_ID, C_D_ID, C_W_ID, D_ID, W_ID);
+//   W_YTD, D_YTD += H_AMOUNT (commutative hot-row update, no register).
+// Copy-back stores the value each write slot staged from the snapshot, so
+// the rows are rewritten unchanged and the YTD update is not applied.  The
+// committed writer of a row is unique (waw), so every row read here still
+// holds its snapshot value, apart from a NewOrder's own earlier lines.
+__global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint8_t epoch,
+                                                          uint32_t no_waves, int rmw)
+{
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (wave < no_waves) {
+        const uint32_t q = wave * 64 + lane;
+        if (q >= no.n || !((d.bitmap_no[wave] >> lane) & 1ull)) return;
+        const NoRows r = no_rows(no, q);
+        const uint32_t oid = no.O_ID[q], did = no.D_ID[q], wid = no.W_ID[q];
+        const unsigned long long last =
+            ((unsigned long long)epoch << kTagShift) | ((unsigned long long)no.TID[q] << kTidShift) | q;
+        if (r.no < d.t[NO].rows && d.new_last[r.no] == last) {
+            uint32_t *row = d.t[NO].data + r.no * kStride[NO];
+            if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
+            else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
+        }
+        if (r.od < d.t[OR].rows) {
+            uint32_t *row = d.t[OR].data + r.od * kStride[OR];
+            if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
+            else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
+        }
+        const uint32_t L = no_lines(no, q);
+        for (uint32_t i = 0; i < L; i++) {
+            const uint64_t at = (uint64_t)i * no.line_stride + q;
+            const uint32_t item = no.OL_I_ID[at], sw = no.OL_SUPPLY_W_ID[at], qty = no.OL_QUANTITY[at];
+            const uint64_t sr = (uint64_t)sw * 100000 + item;
+            const uint64_t lr = (uint64_t)wid * 450000 + no.O_OL_ID[q] + i;
+            if (sr < d.t[ST].rows) {
+                uint32_t *s = d.t[ST].data + sr * kStride[ST];
+                const uint32_t sq = s[2], ytd = s[3], cnt = s[4];
+                if (rmw) {
+                    s[2] = sq >= qty + 10 ? sq - qty : sq - qty + 91;
+                    s[3] = ytd + qty;
+                    s[4] = cnt + 1;
+                } else {
+                    s[2] = sq; s[3] = ytd; s[4] = cnt;
+                }
+            }
+            if (lr < d.t[OL].rows) {
+                uint4 *row = reinterpret_cast<uint4 *>(d.t[OL].data + lr * kStride[OL]);
+                if (rmw) {
+                    const uint32_t price = d.reads_no[(uint64_t)(6 + 3 * i) * no.n + q];
+                    const uint32_t data = d.reads_no[(uint64_t)(7 + 3 * i) * no.n + q];
+                    row[0] = make_uint4(oid, did, wid, i);
+                    row[1] = make_uint4(item, sw, 0u, qty);
+                    uint2 *tail = reinterpret_cast<uint2 *>(row + 2);
+                    *tail = make_uint2(qty * price, data);
+                } else {
+                    const uint4 a = row[0], b = row[1];
+                    uint2 *tail = reinterpret_cast<uint2 *>(row + 2);
+                    const uint2 c = *tail;
+                    row[0] = a; row[1] = b; *tail = c;
+                }
+            }
+        }
+    } else {
+        const uint32_t p = (wave - no_waves) * 64 + lane;
+        if (p >= pay.n || !((d.bitmap_pay[wave - no_waves] >> lane) & 1ull)) return;
+        const PayRows r = pay_rows(d, pay, p);
+        const uint32_t amt = pay.H_AMOUNT[p];
+        if (r.cu < d.t[CU].rows) {
+            uint32_t *c = d.t[CU].data + r.cu * kStride[CU];
+            const uint32_t bal = c[5], data = c[9];
+            c[5] = rmw ? bal - amt : bal;
+            c[9] = rmw ? data + amt : data;
+        }
+        if (r.hi < d.t[HI].rows) {
+            uint4 *row = reinterpret_cast<uint4 *>(d.t[HI].data + r.hi * kStride[HI]);
+            uint32_t *tail = reinterpret_cast<uint32_t *>(row + 1);
+            if (rmw) {
+                row[0] = make_uint4(d.cid[p], pay.C_D_ID[p], pay.C_W_ID[p], pay.D_ID[p]);
+                *tail = pay.W_ID[p];
+            } else {
+                const uint4 a = row[0];
+                const uint32_t b = *tail;
+                row[0] = a; *tail = b;
+            }
+        }
+        if (rmw) {
+            if (r.wh < d.t[WH].rows) atomicAdd(d.t[WH].data + r.wh * kStride[WH] + 2, amt);
+            if (r.di < d.t[DI].rows) atomicAdd(d.t[DI].data + r.di * kStride[DI] + 3, amt);
+        }
+    }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap = 65536)
+{
+    const uint64_t b = (n + kBlock - 1) / kBlock;
+    return (uint32_t)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+inline uint32_t waves64(uint32_t n) { return (n + 63) / 64; }
+
+}  // namespace
+
+hipError_t launch_tpcc_populate(const TpccDev &d, uint64_t seed, hipStream_t s)
+{
+#define POP(T)                                                                                           \
+    hipLaunchKernelGGL(populate_kernel<T>, dim3(grid_for(d.t[T].rows)), dim3(kBlock), 0, s, d.t[T].data, \
+                       d.t[T].rows, seed)
+    POP(WH); POP(DI); POP(CU); POP(HI); POP(NO); POP(OR); POP(OL); POP(ST); POP(IT);
+#undef POP
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_name_index(const TpccDev &d, hipStream_t s)
+{
+    const uint64_t districts = d.t[DI].rows;
+    if (districts == 0) return hipSuccess;
+    hipLaunchKernelGGL(name_index_kernel, dim3((uint32_t)districts), dim3(kBlock), 0, s, d.t[CU].data, d.name_index);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_relayout(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols, uint32_t stride,
+                                uint32_t layout, int to_device, hipStream_t s)
+{
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(relayout_kernel, dim3(grid_for(rows)), dim3(kBlock), 0, s, src, dst, rows, cols, stride,
+                       layout, to_device);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t s)
+{
+    if (pay.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cid_kernel, dim3((pay.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, pay);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                               bool counts, DeviceCounters *ctr, hipStream_t s)
+{
+    const uint32_t no_waves = waves64(no.n) * 16, waves = no_waves + waves64(pay.n);
+    const uint32_t blocks = waves ? (waves + kWaves - 1) / kWaves : 1;
+    if (counts)
+        hipLaunchKernelGGL(execute_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+                           no_waves, ctr);
+    else
+        hipLaunchKernelGGL(execute_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+                           no_waves, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                             bool counts, DeviceCounters *ctr, hipStream_t s)
+{
+    const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
+    if (waves == 0) return hipSuccess;
+    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+    if (counts)
+        hipLaunchKernelGGL(check_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+                           no_waves, ctr);
+    else
+        hipLaunchKernelGGL(check_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+                           no_waves, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                                 uint32_t values, hipStream_t s)
+{
+    const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
+    if (waves == 0) return hipSuccess;
+    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(writeback_kernel, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_waves,
+                       values == LTPG_VALUES_RMW ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace ltpg

@@ -52,6 +52,42 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
                          uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                          const uint32_t *tids, const uint64_t *bitmap, uint32_t values);
 
+/* ---- order-entry (NewOrder + Payment) --------------------------------- */
+
+/* Result of one order-entry epoch.  Bitmaps and counters are indexed within
+ * each type; read values are value-major (value v of txn q at v*n + q):
+ * 51 per NewOrder, 24 per Payment.  Pointers marked "or NULL" may be NULL. */
+typedef struct oracle_tpcc_result {
+    uint64_t *no_bitmap;      /* [ceil(n_no/64)]                              */
+    uint64_t *pay_bitmap;     /* [ceil(n_pay/64)]                             */
+    uint32_t *no_conflicts;   /* [n_no] raw | war << 8 | waw << 16            */
+    uint32_t *pay_conflicts;  /* [n_pay]                                      */
+    uint32_t *no_reads;       /* [51 * n_no] or NULL                          */
+    uint32_t *pay_reads;      /* [24 * n_pay] or NULL                         */
+    uint32_t *pay_cid;        /* [n_pay] or NULL: C_ID each Payment used      */
+    uint32_t no_commits, pay_commits;
+    uint32_t range_errors;    /* op slots whose row is outside its table      */
+} oracle_tpcc_result;
+
+typedef struct oracle_tpcc oracle_tpcc;
+
+/* Rows and columns of table t (0..8: WAREHOUSE .. ITEM) at W warehouses. */
+void oracle_tpcc_shape(uint32_t warehouses, uint32_t table, uint64_t *rows, uint32_t *cols);
+/* Uniform draw in [a, b] of the loader for (seed, stream, row). */
+uint32_t oracle_tpcc_pop_uniform(uint64_t seed, uint32_t stream, uint64_t row, uint32_t a, uint32_t b);
+/* Fill the nine column-major tables with the loader's contents. */
+void oracle_tpcc_populate(uint32_t warehouses, uint64_t seed, uint32_t *const tables[9]);
+oracle_tpcc *oracle_tpcc_create(uint32_t warehouses);
+void oracle_tpcc_destroy(oracle_tpcc *o);
+/* One epoch over the column-major tables (updated in place).  Batches are
+ * the reference's packed NEWORDER_QUERY / PAYMENT_QUERY blocks of capacity
+ * no_cap / pay_cap (53 * no_cap and 10 * pay_cap u32).  rules: 0 reference,
+ * 1 intended; values: 0 copy-back, 1 RMW. */
+int oracle_tpcc_epoch(oracle_tpcc *o, uint32_t *const tables[9], uint32_t rules, uint32_t values,
+                      const uint32_t *no_block, uint32_t no_cap, uint32_t n_no,
+                      const uint32_t *pay_block, uint32_t pay_cap, uint32_t n_pay,
+                      oracle_tpcc_result *res);
+
 #ifdef __cplusplus
 }
 #endif

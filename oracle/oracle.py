@@ -18,10 +18,12 @@ def build():
     subprocess.run(["make", "-C", _HERE, "-s"], check=True)
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
-        build()
-    lib = ctypes.CDLL(LIB_PATH)
+def _load(path=None):
+    if path is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        path = LIB_PATH
+    lib = ctypes.CDLL(path)
     u32p = ctypes.POINTER(ctypes.c_uint32)
     u64p = ctypes.POINTER(ctypes.c_uint64)
 
@@ -34,10 +36,40 @@ def _load():
                                     ctypes.c_uint32, ctypes.c_uint32, u64p, u32p, u32p, ctypes.c_uint32,
                                     u64p, ctypes.POINTER(KvResult)]
     lib.KvResult = KvResult
+
+    class TpccResult(ctypes.Structure):
+        _fields_ = [("no_bitmap", u64p), ("pay_bitmap", u64p), ("no_conflicts", u32p), ("pay_conflicts", u32p),
+                    ("no_reads", u32p), ("pay_reads", u32p), ("pay_cid", u32p), ("no_commits", ctypes.c_uint32),
+                    ("pay_commits", ctypes.c_uint32), ("range_errors", ctypes.c_uint32)]
+
+    lib.TpccResult = TpccResult
+    tables_t = u32p * 9
+    lib.tables_t = tables_t
+    lib.oracle_tpcc_shape.restype = None
+    lib.oracle_tpcc_shape.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64p, u32p]
+    lib.oracle_tpcc_pop_uniform.restype = ctypes.c_uint32
+    lib.oracle_tpcc_pop_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32]
+    lib.oracle_tpcc_populate.restype = None
+    lib.oracle_tpcc_populate.argtypes = [ctypes.c_uint32, ctypes.c_uint64, tables_t]
+    lib.oracle_tpcc_create.restype = ctypes.c_void_p
+    lib.oracle_tpcc_create.argtypes = [ctypes.c_uint32]
+    lib.oracle_tpcc_destroy.restype = None
+    lib.oracle_tpcc_destroy.argtypes = [ctypes.c_void_p]
+    lib.oracle_tpcc_epoch.restype = ctypes.c_int
+    lib.oracle_tpcc_epoch.argtypes = [ctypes.c_void_p, tables_t, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                      ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(TpccResult)]
     return lib
 
 
 _lib = _load()
+
+
+def load_variant(path):
+    """A separately built copy of the oracle library (tests that plant a
+    mistake in a copy of the oracle load it with this)."""
+    return _load(path)
 
 
 def _u32p(a):
@@ -90,3 +122,84 @@ def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values
     return KvEpoch(bitmap[: (B + 63) // 64], conflicts[:B],
                    reads[: B * S * field_words] if want_reads else None,
                    res.commit_count, res.range_errors, table)
+
+
+# ---- order-entry -----------------------------------------------------------
+
+TPCC_COLS = [9, 11, 21, 5, 3, 8, 10, 17, 5]
+
+
+def tpcc_shape(warehouses, table):
+    rows, cols = ctypes.c_uint64(), ctypes.c_uint32()
+    _lib.oracle_tpcc_shape(warehouses, table, ctypes.byref(rows), ctypes.byref(cols))
+    return rows.value, cols.value
+
+
+def tpcc_empty_tables(warehouses):
+    return [np.zeros(tpcc_shape(warehouses, t)[0] * TPCC_COLS[t], dtype=np.uint32) for t in range(9)]
+
+
+def tpcc_populate(warehouses, seed):
+    """The loader's nine column-major tables (tbl[col * rows + row])."""
+    tables = tpcc_empty_tables(warehouses)
+    _lib.oracle_tpcc_populate(warehouses, seed, _lib.tables_t(*[_u32p(t) for t in tables]))
+    return tables
+
+
+def tpcc_pop_uniform(seed, stream, row, a, b):
+    return int(_lib.oracle_tpcc_pop_uniform(seed, stream, row, a, b))
+
+
+class TpccEpoch:
+    """Outputs of one order-entry epoch of the oracle (per type)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    @staticmethod
+    def bit(bitmap, i):
+        return bool((int(bitmap[i // 64]) >> (i % 64)) & 1)
+
+
+class TpccOracle:
+    """oracle_tpcc: registers for W warehouses, reused across epochs."""
+
+    def __init__(self, warehouses, lib=None):
+        self.warehouses = warehouses
+        self.lib = lib if lib is not None else _lib
+        self.h = self.lib.oracle_tpcc_create(warehouses)
+        assert self.h, "oracle_tpcc_create failed"
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_tpcc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def epoch(self, tables, batch, rules=0, values=1):
+        """One epoch over `tables` (list of 9 column-major u32 arrays, updated
+        in place) for `batch` (an object with no/no_cap/n_no/pay/pay_cap/n_pay:
+        the packed NEWORDER_QUERY / PAYMENT_QUERY blocks)."""
+        for t in tables:
+            assert t.dtype == np.uint32 and t.flags.c_contiguous
+        n_no, n_pay = int(batch.n_no), int(batch.n_pay)
+        nb = np.zeros(max(1, (n_no + 63) // 64), np.uint64)
+        pb = np.zeros(max(1, (n_pay + 63) // 64), np.uint64)
+        nc = np.zeros(max(1, n_no), np.uint32)
+        pc = np.zeros(max(1, n_pay), np.uint32)
+        nr = np.zeros(max(1, 51 * n_no), np.uint32)
+        pr = np.zeros(max(1, 24 * n_pay), np.uint32)
+        cid = np.zeros(max(1, n_pay), np.uint32)
+        lib = self.lib
+        res = lib.TpccResult(_u64p(nb), _u64p(pb), _u32p(nc), _u32p(pc), _u32p(nr), _u32p(pr), _u32p(cid), 0, 0, 0)
+        no = np.ascontiguousarray(batch.no, dtype=np.uint32)
+        pay = np.ascontiguousarray(batch.pay, dtype=np.uint32)
+        rc = lib.oracle_tpcc_epoch(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values, _u32p(no),
+                                    batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay, ctypes.byref(res))
+        assert rc == 0
+        return TpccEpoch(no_bitmap=nb[:(n_no + 63) // 64], pay_bitmap=pb[:(n_pay + 63) // 64],
+                         no_conflicts=nc[:n_no], pay_conflicts=pc[:n_pay], no_reads=nr[:51 * n_no],
+                         pay_reads=pr[:24 * n_pay], pay_cid=cid[:n_pay], no_commits=res.no_commits,
+                         pay_commits=res.pay_commits, range_errors=res.range_errors)

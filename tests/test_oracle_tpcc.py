@@ -1,0 +1,299 @@
+"""Pins the order-entry oracle (CPU only).
+
+The reference ships no tests, fixtures or golden vectors for this path, so
+the oracle is pinned by (1) hand-derived known-answer epochs
+(tests/golden/tpcc_kat.json), (2) conflict counters recomputed pairwise
+from the reference's dispatch tables, (3) Aria's serializability guarantee:
+the committed set is executed one transaction at a time in a precedence
+order, which must reproduce the oracle's tables and read values, (4) brute
+force over every TID order of tiny batches, (5) the closed form of the
+by-name lookup on the loader's data, and (6) copies of the oracle with one
+planted mistake each, which these pins must reject.
+"""
+import itertools
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests import tpcccases as tc
+
+import ltpg_amd as L
+
+KAT = tc.load_kat()
+SEED = KAT["seed"]
+_TABLE_CACHE = {}
+
+
+def _tables(W):
+    if W not in _TABLE_CACHE:
+        _TABLE_CACHE[W] = oracle.tpcc_populate(W, SEED)
+    return [t.copy() for t in _TABLE_CACHE[W]]
+
+
+def _apply(W, tables, cells):
+    for t, row, col, v in cells:
+        tables[t][col * tc.rows_of(W, t) + row] = np.uint32(v)
+
+
+def _kat_batch(case):
+    nos = [dict(n, lines=[tuple(x) for x in n.get("lines", [])]) for n in case["neworders"]]
+    return tc.make_batch(nos, case["payments"])
+
+
+def _check_kat(case, run):
+    """run(tables, batch, rules, values) -> TpccEpoch; compares every
+    expectation of the case, return False on the first mismatch."""
+    W = case["W"]
+    tables = _tables(W)
+    _apply(W, tables, case["set"])
+    initial = [t.copy() for t in tables]
+    b = _kat_batch(case)
+    rules = 0 if case["rules"] == "reference" else 1
+    values = 1 if case["values"] == "rmw" else 0
+    ep = run(tables, b, rules, values)
+    exp = case["expect"]
+    ok = (tc.bits(ep.no_bitmap, b.n_no) == exp["no_commit"] and tc.bits(ep.pay_bitmap, b.n_pay) == exp["pay_commit"]
+          and tc.unpack(ep.no_conflicts).tolist() == exp["no_conf"]
+          and tc.unpack(ep.pay_conflicts).tolist() == exp["pay_conf"]
+          and ep.pay_cid.tolist() == exp["cid"] and ep.range_errors == exp["range_errors"]
+          and ep.no_commits == sum(exp["no_commit"]) and ep.pay_commits == sum(exp["pay_commit"]))
+    expected = [t.copy() for t in initial]
+    _apply(W, expected, exp["cells"])
+    ok = ok and all(np.array_equal(tables[t], expected[t]) for t in range(9))
+    for q, v, val in exp["no_reads"]:
+        ok = ok and int(ep.no_reads[v * b.n_no + q]) == val
+    for p, v, val in exp["pay_reads"]:
+        ok = ok and int(ep.pay_reads[v * b.n_pay + p]) == val
+    return ok
+
+
+@pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
+def test_oracle_known_answers(case):
+    ora = oracle.TpccOracle(case["W"])
+    assert _check_kat(case, lambda tables, b, r, v: ora.epoch(tables, b, r, v))
+
+
+def test_loader_contents():
+    """Spot values of create_* (Database.h:104-327) and the draw ranges."""
+    W = 2
+    t = _tables(W)
+
+    def get(tab, row, col):
+        return int(t[tab][col * tc.rows_of(W, tab) + row])
+
+    assert [get(tc.WH, 1, c) for c in range(9)] == [1, 1, 1, 0, 0, 0, 10, 0, 0]
+    assert [get(tc.DI, 13, c) for c in range(5)] == [3, 1, 1, 1, 0]
+    r = 30000 + 2 * 3000 + 17
+    assert [get(tc.CU, r, c) for c in (0, 1, 2, 3, 9, 13)] == [17, 1, 2, 100000, 100000, r % 20]
+    assert get(tc.OR, 33001, 1) == 11 and get(tc.OR, 33001, 2) == 1
+    assert [get(tc.OL, 450007, c) for c in (0, 1, 2, 5)] == [7, 10, 1, 1]
+    assert [get(tc.ST, 100003, c) for c in (0, 1, 2, 3, 4, 6, 15)] == [3, 1, 100000, 100000, 0, 1, 10]
+    assert [get(tc.IT, 99999, c) for c in (0, 1, 4)] == [99999, 1000, 99999]
+    assert not t[tc.HI].any() and not t[tc.NO].any()
+    price = t[tc.IT][2 * 100000:3 * 100000]
+    data = t[tc.IT][3 * 100000:4 * 100000]
+    credit = t[tc.CU][10 * tc.rows_of(W, tc.CU):11 * tc.rows_of(W, tc.CU)]
+    o_c = t[tc.OR][3 * tc.rows_of(W, tc.OR):4 * tc.rows_of(W, tc.OR)]
+    assert price.min() == 1 and price.max() == 40000 and abs(price.mean() - 20000.5) < 300
+    assert data.min() == 1 and data.max() == 10
+    assert set(np.unique(credit).tolist()) == {0, 1}
+    assert o_c.min() == 0 and o_c.max() == 2999
+
+
+def _random_batch(rng, W, n_no, n_pay, dense=True):
+    """Small batches whose ids collide often."""
+    nos, pays = [], []
+    tids = rng.permutation(n_no + n_pay) * 2 + 1
+    for q in range(n_no):
+        nl = int(rng.integers(0, 4))
+        nos.append(dict(W_ID=int(rng.integers(0, W)), D_ID=int(rng.integers(0, 2)), C_ID=int(rng.integers(0, 4)),
+                        O_ID=int(rng.integers(0, 5)), N_O_ID=int(rng.integers(0, 5)),
+                        O_OL_ID=int(rng.integers(0, 12)), TID=int(tids[q]),
+                        lines=[(int(rng.integers(0, 6)), int(rng.integers(0, W)), int(rng.integers(1, 11)))
+                               for _ in range(nl)]))
+    for p in range(n_pay):
+        by_name = rng.random() < 0.4
+        pays.append(dict(W_ID=int(rng.integers(0, W)), D_ID=int(rng.integers(0, 2)),
+                         C_ID=0 if by_name else int(rng.integers(0, 4)),
+                         C_LAST=(int(rng.integers(0, 4)) << 8) + int(rng.integers(0, 2)) if by_name else 20,
+                         isName=int(by_name), C_D_ID=int(rng.integers(0, 2)), C_W_ID=int(rng.integers(0, W)),
+                         H_AMOUNT=int(rng.integers(1, 6)), H_ID=int(rng.integers(0, 5)),
+                         TID=int(tids[n_no + p])))
+    return tc.make_batch(nos, pays)
+
+
+def _pins_hold(W, tables0, b, ep, rules, values=1):
+    """(ok, why): pairwise conflicts, Aria commit, serial execution."""
+    no_c, pay_c, bad = tc.pairwise_conflicts(b, W, ep.pay_cid, rules)
+    if not np.array_equal(tc.unpack(ep.no_conflicts), no_c) or not np.array_equal(tc.unpack(ep.pay_conflicts), pay_c):
+        return False, "conflict counters differ from the pairwise count"
+    if bad != ep.range_errors:
+        return False, "range errors"
+    no_commit, pay_commit = tc.bits(ep.no_bitmap, b.n_no), tc.bits(ep.pay_bitmap, b.n_pay)
+    if no_commit != tc.aria(no_c).tolist() or pay_commit != tc.aria(pay_c).tolist():
+        return False, "commit is not waw == 0 && (raw == 0 || war == 0)"
+    ok, why, final, reads = tc.serial_check(W, tables0, b, ep.pay_cid, no_commit, pay_commit, rmw=values == 1)
+    if not ok:
+        return False, why
+    for (typ, i), vals in reads.items():
+        flat, n, nv = (ep.no_reads, b.n_no, 51) if typ == "no" else (ep.pay_reads, b.n_pay, 24)
+        if [int(flat[v * n + i]) for v in range(nv)] != vals:
+            return False, f"read values of {typ} {i}"
+    return True, final
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_oracle_pairwise_and_serializable(seed):
+    rng = np.random.default_rng(seed)
+    W = 1 + seed % 2
+    rules = seed % 3 == 2
+    b = _random_batch(rng, W, int(rng.integers(0, 10)), int(rng.integers(0, 10)))
+    tables0 = _tables(W)
+    tables = [t.copy() for t in tables0]
+    ep = oracle.TpccOracle(W).epoch(tables, b, int(rules), 1)
+    ok, res = _pins_hold(W, tables0, b, ep, int(rules))
+    assert ok, res
+    for t in range(9):
+        np.testing.assert_array_equal(tables[t], res[t])
+    # the highest-priority transaction always commits
+    prios = [((int(b.nf("TID")[q]), q), "no", q) for q in range(b.n_no)] + \
+            [((int(b.pf("TID")[p]), b.n_no + p), "pay", p) for p in range(b.n_pay)]
+    if prios:
+        _, typ, i = min(prios)
+        assert tc.bits(ep.no_bitmap if typ == "no" else ep.pay_bitmap, i + 1)[i] == 1
+
+
+def test_oracle_bruteforce_all_tid_orders():
+    """Every TID order of small colliding batches (<= 6 transactions)."""
+    rng = np.random.default_rng(77)
+    W = 1
+    tables0 = _tables(W)
+    ora = oracle.TpccOracle(W)
+    checked = 0
+    for pattern in range(8):
+        n_no, n_pay = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+        base = _random_batch(rng, W, n_no, n_pay)
+        for perm in itertools.permutations(range(n_no + n_pay)):
+            b = L.TpccBatch(base.no.copy(), base.no_cap, base.n_no, base.pay.copy(), base.pay_cap, base.n_pay)
+            b.nf("TID")[:] = np.array(perm[:n_no], dtype=np.uint32) * 5 + 3
+            b.pf("TID")[:] = np.array(perm[n_no:], dtype=np.uint32) * 5 + 3
+            tables = [t.copy() for t in tables0]
+            ep = ora.epoch(tables, b, pattern % 2, 1)
+            ok, res = _pins_hold(W, tables0, b, ep, pattern % 2)
+            assert ok, res
+            for t in range(9):
+                np.testing.assert_array_equal(tables[t], res[t])
+            checked += 1
+    assert checked >= 100
+
+
+def test_cid_closed_form_on_generated_batches():
+    W = 4
+    gen = L.TpccGenerator(warehouses=W, seed=11)
+    b = gen.next(4000)
+    ep = oracle.TpccOracle(W).epoch(_tables(W), b)
+    np.testing.assert_array_equal(ep.pay_cid, tc.cid_closed_form(b, W))
+
+
+def test_copyback_leaves_tables_and_matches_commits():
+    W = 2
+    gen = L.TpccGenerator(warehouses=W, seed=4)
+    b = gen.next(3000)
+    t0, t1 = _tables(W), _tables(W)
+    e0 = oracle.TpccOracle(W).epoch(t0, b, 0, 0)
+    e1 = oracle.TpccOracle(W).epoch(t1, b, 0, 1)
+    for t in range(9):
+        np.testing.assert_array_equal(t0[t], _TABLE_CACHE[W][t])
+    np.testing.assert_array_equal(e0.no_bitmap, e1.no_bitmap)
+    np.testing.assert_array_equal(e0.pay_bitmap, e1.pay_bitmap)
+    np.testing.assert_array_equal(e0.no_reads, e1.no_reads)
+
+
+def test_registers_reset_between_epochs():
+    """A second epoch on a fresh oracle and on a reused one agree."""
+    W = 1
+    gen = L.TpccGenerator(warehouses=W, seed=8)
+    b1, b2 = gen.next(500), gen.next(500, tid_base=500)
+    ta, tb_ = _tables(W), _tables(W)
+    reused = oracle.TpccOracle(W)
+    reused.epoch(ta, b1)
+    ea = reused.epoch(ta, b2)
+    oracle.TpccOracle(W).epoch(tb_, b1)
+    eb = oracle.TpccOracle(W).epoch(tb_, b2)
+    np.testing.assert_array_equal(ea.no_bitmap, eb.no_bitmap)
+    np.testing.assert_array_equal(ea.pay_conflicts, eb.pay_conflicts)
+
+
+# ---- planted mistakes -------------------------------------------------------
+
+ORACLE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+
+MISTAKES = {
+    # check_3_0 made reachable under the reference rules (App. quirk B1)
+    "check_3_0_always": (r"if \(rules != 0\) war \+= lower\(o, NO, R, h.no, me\);",
+                         "war += lower(o, NO, R, h.no, me);"),
+    # the Payment warehouse update registered instead of isFrequent
+    "ytd_registered": (r"if \(!ok\(o, WH, wh\)\) errors\+\+;( +)/\* op_1_0: isFrequent \*/",
+                       "if (ok(o, WH, wh)) reg_min(o, WH, W, wh, me); else errors++;"),
+    # by-name ordinal counted from 1
+    "cid_one_based": (r"if \(seen\+\+ == k\)", "if (++seen == k)"),
+    # restock branch with the wrong sign
+    "restock_sign": (r"\*sq - qty \+ 91", "*sq + qty + 91"),
+    # commit rule with OR
+    "or_rule": (r"if \(waw == 0 && \(raw == 0 \|\| war == 0\)\) \{\n            res->pay_bitmap",
+                "if (waw == 0 || raw == 0 || war == 0) {\n            res->pay_bitmap"),
+    # war read from the W register of the customer
+    "war_from_w": (r"war \+= lower\(o, CU, R, cu, me\);", "war += lower(o, CU, W, cu, me);"),
+    # priority ties broken the wrong way (position descending)
+    "tie_reversed": (r"const uint64_t me = \(\(uint64_t\)pv.TID\[p\] << 32\) \| \(n_no \+ p\);",
+                     "const uint64_t me = ((uint64_t)pv.TID[p] << 32) | (0xffffffu - p);"),
+    # an order line beyond the first not applied
+    "first_line_only": (r"for \(uint32_t i = 0; i < nlines\(&nv, q\); i\+\+\) \{\n                const uint64_t ir = item_row\(&nv, q, i\), sr = stock_row\(&nv, q, i\), lr = line_row\(&nv, q, i\);\n                const uint32_t qty",
+                        "for (uint32_t i = 0; i < 1 && i < nlines(&nv, q); i++) {\n                const uint64_t ir = item_row(&nv, q, i), sr = stock_row(&nv, q, i), lr = line_row(&nv, q, i);\n                const uint32_t qty"),
+}
+
+
+def _mutant(tmp_path, name):
+    pattern, repl = MISTAKES[name]
+    src = open(os.path.join(ORACLE_SRC, "ltpg_oracle_tpcc.c")).read()
+    new, n = re.subn(pattern, lambda m: repl, src, count=1)
+    assert n == 1, f"mutation {name} did not apply"
+    d = tmp_path / name
+    d.mkdir()
+    for f in ("ltpg_oracle.h", "ltpg_oracle_kv.c"):
+        shutil.copy(os.path.join(ORACLE_SRC, f), d / f)
+    (d / "ltpg_oracle_tpcc.c").write_text(new)
+    so = d / "liboracle.so"
+    subprocess.run(["gcc", "-O1", "-std=c99", "-fPIC", "-shared", "-o", str(so), str(d / "ltpg_oracle_kv.c"),
+                    str(d / "ltpg_oracle_tpcc.c")], check=True)
+    return oracle.load_variant(str(so))
+
+
+@pytest.mark.parametrize("mistake", sorted(MISTAKES))
+def test_pins_catch_planted_mistakes(tmp_path, mistake):
+    """Each planted mistake fails a known answer or the pairwise /
+    serializability pins, so those pins would catch it in the oracle."""
+    lib = _mutant(tmp_path, mistake)
+    caught = False
+    for case in KAT["cases"]:
+        ora = oracle.TpccOracle(case["W"], lib=lib)
+        if not _check_kat(case, lambda tables, b, r, v: ora.epoch(tables, b, r, v)):
+            caught = True
+            break
+    rng = np.random.default_rng(99)
+    for i in range(0 if caught else 60):
+        W = 1
+        b = _random_batch(rng, W, int(rng.integers(1, 8)), int(rng.integers(1, 8)))
+        tables0 = _tables(W)
+        tables = [t.copy() for t in tables0]
+        ep = oracle.TpccOracle(W, lib=lib).epoch(tables, b, i % 2, 1)
+        ok, res = _pins_hold(W, tables0, b, ep, i % 2)
+        if not ok or not all(np.array_equal(tables[t], res[t]) for t in range(9)):
+            caught = True
+            break
+    assert caught, f"planted mistake {mistake!r} passed every pin"
