@@ -1,12 +1,14 @@
 """Benchmark of the deterministic concurrency-control hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kv-c2] [--theta 0.99]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kv-c2|kv-c1|tpcc-c3] [--theta T]
 
-A step is one epoch of the plain per-batch form (EXECUTE, CHECK+COMMIT,
-WRITEBACK) over one batch whose inputs are already resident in HBM.  The
-default workload is the key-value configuration the metric is quoted on:
-10M rows x 10 fields x 100 B (row-major 1,000 B records), batches of 2^20
-transactions x 16 ops, 50/50 read / read-modify-write, Zipf theta 0.99.
+A step is one epoch of the plain per-batch form (CID, EXECUTE,
+CHECK+COMMIT, WRITEBACK) over one batch whose inputs are already resident
+in HBM.  The default workload is the key-value configuration the metric is
+quoted on: 10M rows x 10 fields x 100 B (row-major 1,000 B records), batches
+of 2^20 transactions x 16 ops, 50/50 read / read-modify-write, Zipf theta
+0.6, with the Zipf 0.99 run and the order-entry configuration (W = 64,
+2^18 transactions, 50/50 NewOrder / Payment) as variants on the same line.
 
 Prints one JSON line: committed txns/s (whole job), abort rate, p50/p99
 commit latency of the reference-style synchronous loop, per-phase device
@@ -210,13 +212,169 @@ def run_kv(args, theta, cpu_baseline=True):
     return result
 
 
-def measured_traffic(phase):
+TPCC_KERNELS = ["cid", "execute", "check", "writeback"]
+
+
+def tpcc_kernel_bytes(b, no_commit, pay_commit, rules):
+    """Algorithmic bytes of each order-entry kernel for one batch: every
+    query field, register, table word, read value and flag the kernel has
+    to read or write, once.  A u64 register update counts 16 B (8 read + 8
+    written); a bitmap bit 1/8 B."""
+    n_no, n_pay = b.n_no, b.n_pay
+    L_ = np.minimum(b.nf("O_OL_CNT").astype(np.int64), 15)
+    cno = np.asarray(no_commit, dtype=bool)
+    cpay = np.asarray(pay_commit, dtype=bool)
+    by_name = b.pf("isName").astype(bool)
+    lines = int(L_.sum())
+    lines_c = int(L_[cno].sum())
+    nc, pc = int(cno.sum()), int(cpay.sum())
+    reg = 16
+    cid = n_pay * 12 + int(by_name.sum()) * (12 + 12 * 8 + 4) + n_pay * 4
+    execute = (n_no * (8 * 4 + reg * (2 if rules else 1)) + lines * (2 * 4 + 2 * reg) + n_no
+               + n_pay * (7 * 4 + 2 * reg) + n_pay)
+    check = (n_no + n_pay) * (1 + 1 / 8) + nc * (4 * 4 + 6 * 4 + 51 * 4 + reg) + lines_c * (4 + 3 * 4) \
+        + pc * (7 * 4 + 24 * 4 + 24 * 4)
+    writeback = (n_no + n_pay) / 8 + nc * (7 * 4 + 8 + 2 * 12) + lines_c * (3 * 4 + 2 * 4 + 2 * 12 + 40) \
+        + pc * (8 * 4 + 2 * 8 + 20 + 2 * 8)
+    return {"cid": cid, "execute": execute, "check": check, "writeback": writeback}
+
+
+def device_tpcc_batch(ctx, b):
+    """Upload a TpccBatch; (NeworderBatch, PaymentBatch) of device pointers
+    and the buffers that hold them."""
+    import ltpg_amd as L
+
+    nob, pab = L.DeviceBuffer(ctx, max(b.no.nbytes, 16)), L.DeviceBuffer(ctx, max(b.pay.nbytes, 16))
+    nob.upload(b.no)
+    pab.upload(b.pay)
+    hn = L.neworder_struct(b.no, b.no_cap, b.n_no)
+    hp = L.payment_struct(b.pay, b.pay_cap, b.n_pay)
+    nb, pb = L.NeworderBatch(), L.PaymentBatch()
+    nb.n, nb.line_stride, pb.n = hn.n, hn.line_stride, hp.n
+    for name, _ in L.NeworderBatch._fields_[2:]:
+        setattr(nb, name, nob.ptr + (getattr(hn, name) - b.no.ctypes.data))
+    for name, _ in L.PaymentBatch._fields_[2:]:
+        setattr(pb, name, pab.ptr + (getattr(hp, name) - b.pay.ctypes.data))
+    return (nb, pb), (nob, pab)
+
+
+def cpu_baseline_tpcc(W, batches, seed, budget_s=10.0):
+    """The oracle (single-threaded C replay of the reference's rules, every
+    register of every slot, read staging and RMW write-back) on whole batches
+    of the same workload over the full-size tables, until about budget_s."""
+    from oracle import oracle
+
+    tables = oracle.tpcc_populate(W, seed)
+    ora = oracle.TpccOracle(W)
+    total_s, commits, txns, epochs = 0.0, 0, 0, 0
+    while total_s < budget_s:
+        b = batches[epochs % len(batches)]
+        t0 = time.perf_counter()
+        ep = ora.epoch(tables, b, 0, 1)
+        total_s += time.perf_counter() - t0
+        commits += ep.no_commits + ep.pay_commits
+        txns += b.n_no + b.n_pay
+        epochs += 1
+    return {"value": commits / total_s, "unit": "txn/s", "cores": 1, "kind": "port",
+            "attempted_txn_per_s": txns / total_s,
+            "sample": f"oracle (single-threaded C replay) over the full W={W} tables: {epochs} whole "
+                      f"batches, {txns} txns in {total_s:.1f} s"}
+
+
+def run_tpcc(args, cpu_baseline=True):
+    import ltpg_amd as L
+
+    W, n, P = 64, 1 << 18, args.batches
+    seed = 0x4C545047
+    desc = (f"order-entry plain form: W={W}, 2^18 txns, 50/50 NewOrder/Payment, NURand items, 1% remote "
+            f"supply, 15% remote customer, 60% by name")
+    log(f"generating {P} batches: {desc}")
+    gen = L.TpccGenerator(warehouses=W, seed=seed)
+    batches = [gen.next(n, tid_base=i * n) for i in range(P)]
+    ctx = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=n, values=L.VALUES_RMW)
+    ctx.tpcc_populate(seed)
+    dev = [device_tpcc_batch(ctx, b) for b in batches]
+    log("order-entry batches resident in HBM")
+
+    per_batch = []
+    for i in range(P):
+        ctx.tpcc_submit(dev[i][0], mem=L.MEM_DEVICE)
+        ctx.epoch_run()
+        ctx.sync()
+        nc = np.unpackbits(ctx.commit_bitmap(L.TYPE_NEWORDER, n_txns=batches[i].n_no).view(np.uint8),
+                           bitorder="little")[:batches[i].n_no]
+        pc = np.unpackbits(ctx.commit_bitmap(L.TYPE_PAYMENT, n_txns=batches[i].n_pay).view(np.uint8),
+                           bitorder="little")[:batches[i].n_pay]
+        per_batch.append(tpcc_kernel_bytes(batches[i], nc, pc, 0))
+
+    for k in range(args.warmup):
+        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        ctx.epoch_run()
+    ctx.sync()
+    log(f"warmup done; timing {args.steps} steps")
+    for t in (L.TYPE_NEWORDER, L.TYPE_PAYMENT):
+        ctx.commit_totals(t, reset=True)
+    ctx.set_timing(True)
+    ctx.phase_times()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        ctx.epoch_run()
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    phase_ms, n_timed = ctx.phase_times()
+    ctx.set_timing(False)
+    (cn, an), (cp, ap_) = ctx.commit_totals(L.TYPE_NEWORDER), ctx.commit_totals(L.TYPE_PAYMENT)
+
+    lat = []
+    for k in range(max(5, args.steps // 2)):
+        t1 = time.perf_counter()
+        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        ctx.epoch_run()
+        ctx.commit_count(L.TYPE_PAYMENT)
+        lat.append((time.perf_counter() - t1) * 1e3)
+
+    avg_ms = {L.TIMING_NAMES[i]: phase_ms[i] / max(n_timed, 1) for i in range(L.TIMING_SLOTS)}
+    bytes_avg = {k: sum(per_batch[j % P][k] for j in range(args.steps)) / args.steps for k in TPCC_KERNELS}
+    dom = max(TPCC_KERNELS, key=lambda k: avg_ms[k])
+    achieved = bytes_avg[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    result = {
+        "metric": METRIC, "value": (cn + cp) / elapsed, "unit": "txn/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": desc, "warehouses": W, "batch_txns": n, "neworder_pct": 50,
+                   "form": "plain per-batch", "inputs": "resident in HBM", "distinct_batches": P},
+        "abort_rate": 1.0 - (cn + cp) / max(an + ap_, 1),
+        "abort_rate_by_type": {"neworder": 1.0 - cn / max(an, 1), "payment": 1.0 - cp / max(ap_, 1)},
+        "committed_txns": cn + cp, "attempted_txns": an + ap_,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "what": "submit -> commit count on host, one synchronous epoch at a time"},
+        "kernel_ms": avg_ms,
+        "kernel_algorithmic_gbs": {k: bytes_avg[k] / (avg_ms[k] * 1e-3) / 1e9 for k in TPCC_KERNELS
+                                   if avg_ms.get(k, 0) > 0},
+        "kernel_algorithmic_bytes": bytes_avg,
+        "roofline": {"kernel": f"tpcc_{dom}", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "tpcc"),
+                     "algorithmic_bytes_per_launch": bytes_avg[dom]},
+    }
+    for (_, bufs) in dev:
+        for x in bufs:
+            x.free()
+    ctx.close()
+    if cpu_baseline and not args.no_cpu_baseline:
+        log("cpu baseline (order-entry oracle)")
+        result["cpu_baseline"] = cpu_baseline_tpcc(W, batches, seed)
+    return result
+
+
+def measured_traffic(phase, family="kv"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if
-    one exists for this kernel (profiles/kv_c2_traffic.json)."""
-    p = os.path.join(ROOT, "profiles", "kv_c2_traffic.json")
+    one exists for this kernel (profiles/kv_c2_traffic.json,
+    profiles/tpcc_c3_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "kv_c2_traffic.json" if family == "kv" else "tpcc_c3_traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(f"kv_{phase}")
+            return json.load(f).get(f"{family}_{phase}")
     except (OSError, ValueError):
         return None
 
@@ -246,7 +404,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kv-c2")
+    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3"])
     ap.add_argument("--theta", type=float, default=None,
                     help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
@@ -263,14 +421,20 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    thetas = [args.theta] if args.theta is not None else ([0.6, 0.99] if args.workload == "kv-c2" else [None])
-    result = run_kv(args, thetas[0])
+    keep = ("value", "unit", "ms_per_step", "abort_rate", "latency_ms", "kernel_ms", "kernel_algorithmic_gbs",
+            "roofline", "config")
     variants = []
-    for th in thetas[1:]:
-        v = run_kv(args, th, cpu_baseline=False)
-        variants.append({k: v[k] for k in ("value", "unit", "ms_per_step", "abort_rate", "latency_ms",
-                                           "kernel_ms", "kernel_algorithmic_gbs", "roofline")}
-                        | {"config": v["config"]})
+    if args.workload == "tpcc-c3":
+        result = run_tpcc(args)
+    else:
+        thetas = [args.theta] if args.theta is not None else ([0.6, 0.99] if args.workload == "kv-c2" else [None])
+        result = run_kv(args, thetas[0])
+        for th in thetas[1:]:
+            v = run_kv(args, th, cpu_baseline=False)
+            variants.append({k: v[k] for k in keep})
+        if args.workload == "kv-c2" and args.theta is None:
+            v = run_tpcc(args)
+            variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
     if variants:
         result["variants"] = variants
     print(json.dumps(result))
