@@ -33,6 +33,8 @@
 // Tables are row-major, `stride` (cols rounded up to 4) u32 words per row.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "ltpg_internal.h"
 
@@ -256,36 +258,39 @@ __device__ __forceinline__ unsigned long long *cust_w(const TpccDev &d, uint64_t
 
 // ---- EXECUTE ---------------------------------------------------------------
 
+// NewOrders take 16 adjacent lanes each (lane j < 15: line j, lane 15: the
+// header slots), 16 orders per block: an order's consecutive ORDERLINE
+// registers are touched by adjacent lanes, and all its roles share a CU.
+// Payments take one lane each, in the blocks after the NewOrder blocks.
+constexpr uint32_t kNoPerBlock = kBlock / 16;
+
 template <bool COUNTS>
 __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
-                                                        uint8_t epoch, uint32_t no_waves, DeviceCounters *ctr)
+                                                        uint8_t epoch, uint32_t no_blocks, DeviceCounters *ctr)
 {
     // the epoch's commit counts start at zero (CHECK adds to them)
     if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
         ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
-    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
-    const uint32_t lane = threadIdx.x % 64;
     uint32_t bad = 0;
-    if (wave < no_waves) {
-        const uint32_t role = wave % 16;
-        const uint32_t q = (wave / 16) * 64 + lane;
+    if (blockIdx.x < no_blocks) {
+        const uint32_t q = blockIdx.x * kNoPerBlock + threadIdx.x / 16;
+        const uint32_t j = threadIdx.x % 16;
         if (q < no.n) {
             const unsigned long long me = priority(tag, no.TID[q], q);
-            if (role < LTPG_TPCC_MAX_LINES) {
+            bool waw = false;
+            if (j < LTPG_TPCC_MAX_LINES) {
                 // line role: STOCK W (op_6_0), ORDERLINE W (op_7_0); the ITEM
                 // slot (op_5_0) registers a register no check consults
-                if (role < no_lines(no, q)) {
-                    const uint64_t at = (uint64_t)role * no.line_stride + q;
+                if (j < no_lines(no, q)) {
+                    const uint64_t at = (uint64_t)j * no.line_stride + q;
                     const uint32_t item = no.OL_I_ID[at];
                     const uint64_t sr = (uint64_t)no.OL_SUPPLY_W_ID[at] * 100000 + item;
-                    const uint64_t lr = (uint64_t)no.W_ID[q] * 450000 + no.O_OL_ID[q] + role;
+                    const uint64_t lr = (uint64_t)no.W_ID[q] * 450000 + no.O_OL_ID[q] + j;
                     bad += item >= d.t[IT].rows;
-                    bool waw = false;
                     if (sr < d.t[ST].rows) waw |= register_write(d.regs + d.off_stock + sr, me, tag, d.doomed, epoch);
                     else bad++;
                     if (lr < d.t[OL].rows) waw |= register_write(d.regs + d.off_line + lr, me, tag, d.doomed, epoch);
                     else bad++;
-                    if (waw) d.doomed[q] = epoch;
                 }
             } else {
                 // header role: slots 0..4
@@ -297,18 +302,17 @@ __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev 
                 } else bad++;
                 // NEWORDER insert: a write (waw) under INTENDED; under
                 // REFERENCE its R register is never consulted
-                bool waw = false;
                 if (r.no < d.t[NO].rows) {
                     if (d.rules == LTPG_RULES_INTENDED)
                         waw |= register_write(d.regs + d.off_new + r.no, me, tag, d.doomed, epoch);
                 } else bad++;
                 if (r.od < d.t[OR].rows) waw |= register_write(d.regs + d.off_order + r.od, me, tag, d.doomed, epoch);
                 else bad++;
-                if (waw) d.doomed[q] = epoch;
             }
+            if (waw) d.doomed[q] = epoch;
         }
     } else {
-        const uint32_t p = (wave - no_waves) * 64 + lane;
+        const uint32_t p = (blockIdx.x - no_blocks) * kBlock + threadIdx.x;
         if (p < pay.n) {
             const unsigned long long me = priority(tag, pay.TID[p], no.n + p);
             const PayRows r = pay_rows(d, pay, p);
@@ -513,64 +517,85 @@ __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no
 // committed writer of a row is unique (waw), so every row read here still
 // holds its snapshot value, apart from a NewOrder's own earlier lines.
 __global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint8_t epoch,
-                                                          uint32_t no_waves, int rmw)
+                                                          uint32_t no_blocks, int rmw)
 {
-    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
-    const uint32_t lane = threadIdx.x % 64;
-    if (wave < no_waves) {
-        const uint32_t q = wave * 64 + lane;
-        if (q >= no.n || !((d.bitmap_no[wave] >> lane) & 1ull)) return;
-        const NoRows r = no_rows(no, q);
+    if (blockIdx.x < no_blocks) {
+        // 16 lanes per NewOrder, as in EXECUTE: lane j < 15 writes line j,
+        // lane 15 the NEWORDER and ORDER rows.  The group of an order is
+        // active or idle as a whole, so the shuffles below read live lanes.
+        const uint32_t q = blockIdx.x * kNoPerBlock + threadIdx.x / 16;
+        const uint32_t j = threadIdx.x % 16;
+        if (q >= no.n || !((d.bitmap_no[q / 64] >> (q % 64)) & 1ull)) return;
         const uint32_t oid = no.O_ID[q], did = no.D_ID[q], wid = no.W_ID[q];
-        const unsigned long long last =
-            ((unsigned long long)epoch << kTagShift) | ((unsigned long long)no.TID[q] << kTidShift) | q;
-        if (r.no < d.t[NO].rows && d.new_last[r.no] == last) {
-            uint32_t *row = d.t[NO].data + r.no * kStride[NO];
-            if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
-            else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
-        }
-        if (r.od < d.t[OR].rows) {
-            uint32_t *row = d.t[OR].data + r.od * kStride[OR];
-            if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
-            else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
-        }
         const uint32_t L = no_lines(no, q);
-        for (uint32_t i = 0; i < L; i++) {
-            const uint64_t at = (uint64_t)i * no.line_stride + q;
-            const uint32_t item = no.OL_I_ID[at], sw = no.OL_SUPPLY_W_ID[at], qty = no.OL_QUANTITY[at];
-            const uint64_t sr = (uint64_t)sw * 100000 + item;
-            const uint64_t lr = (uint64_t)wid * 450000 + no.O_OL_ID[q] + i;
-            if (sr < d.t[ST].rows) {
-                uint32_t *s = d.t[ST].data + sr * kStride[ST];
-                const uint32_t sq = s[2], ytd = s[3], cnt = s[4];
-                if (rmw) {
-                    s[2] = sq >= qty + 10 ? sq - qty : sq - qty + 91;
-                    s[3] = ytd + qty;
-                    s[4] = cnt + 1;
-                } else {
-                    s[2] = sq; s[3] = ytd; s[4] = cnt;
+        if (j == 15) {
+            const NoRows r = no_rows(no, q);
+            const unsigned long long last =
+                ((unsigned long long)epoch << kTagShift) | ((unsigned long long)no.TID[q] << kTidShift) | q;
+            if (r.no < d.t[NO].rows && d.new_last[r.no] == last) {
+                uint32_t *row = d.t[NO].data + r.no * kStride[NO];
+                if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
+                else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
+            }
+            if (r.od < d.t[OR].rows) {
+                uint32_t *row = d.t[OR].data + r.od * kStride[OR];
+                if (rmw) { row[0] = oid; row[1] = did; row[2] = wid; }
+                else { const uint32_t a = row[0], b = row[1], c = row[2]; row[0] = a; row[1] = b; row[2] = c; }
+            }
+        }
+        const bool has = j < L;
+        const uint64_t at = (uint64_t)(has ? j : 0) * no.line_stride + q;
+        const uint32_t item = no.OL_I_ID[at], sw = no.OL_SUPPLY_W_ID[at], qty = no.OL_QUANTITY[at];
+        const uint64_t sr = (uint64_t)sw * 100000 + item;
+        const unsigned long long key = has && sr < d.t[ST].rows ? sr : ~0ull;
+        // Lines naming the same STOCK row apply in line order: the last of
+        // them applies all of them; the others write nothing.
+        const int base = (int)(threadIdx.x % 64) & ~15;
+        uint32_t match = 0, qk[LTPG_TPCC_MAX_LINES];
+        bool later = false;
+#pragma unroll
+        for (int k = 0; k < (int)LTPG_TPCC_MAX_LINES; k++) {
+            const unsigned long long kk = __shfl(key, base + k);
+            qk[k] = __shfl(qty, base + k);
+            match |= (uint32_t)(kk == key && k <= (int)j) << k;
+            later |= kk == key && k > (int)j;
+        }
+        if (key != ~0ull && !later) {
+            uint32_t *st = d.t[ST].data + sr * kStride[ST];
+            uint32_t sq = st[2], ytd = st[3], cnt = st[4];
+            if (rmw) {
+#pragma unroll
+                for (int k = 0; k < (int)LTPG_TPCC_MAX_LINES; k++) {
+                    if ((match >> k) & 1u) {
+                        sq = sq >= qk[k] + 10 ? sq - qk[k] : sq - qk[k] + 91;
+                        ytd += qk[k];
+                        cnt += 1;
+                    }
                 }
             }
-            if (lr < d.t[OL].rows) {
-                uint4 *row = reinterpret_cast<uint4 *>(d.t[OL].data + lr * kStride[OL]);
-                if (rmw) {
-                    const uint32_t price = d.reads_no[(uint64_t)(6 + 3 * i) * no.n + q];
-                    const uint32_t data = d.reads_no[(uint64_t)(7 + 3 * i) * no.n + q];
-                    row[0] = make_uint4(oid, did, wid, i);
-                    row[1] = make_uint4(item, sw, 0u, qty);
-                    uint2 *tail = reinterpret_cast<uint2 *>(row + 2);
-                    *tail = make_uint2(qty * price, data);
-                } else {
-                    const uint4 a = row[0], b = row[1];
-                    uint2 *tail = reinterpret_cast<uint2 *>(row + 2);
-                    const uint2 c = *tail;
-                    row[0] = a; row[1] = b; *tail = c;
-                }
+            st[2] = sq;
+            st[3] = ytd;
+            st[4] = cnt;
+        }
+        const uint64_t lr = (uint64_t)wid * 450000 + no.O_OL_ID[q] + j;
+        if (has && lr < d.t[OL].rows) {
+            uint4 *row = reinterpret_cast<uint4 *>(d.t[OL].data + lr * kStride[OL]);
+            uint2 *tail = reinterpret_cast<uint2 *>(row + 2);
+            if (rmw) {
+                const uint32_t price = d.reads_no[(uint64_t)(6 + 3 * j) * no.n + q];
+                const uint32_t data = d.reads_no[(uint64_t)(7 + 3 * j) * no.n + q];
+                row[0] = make_uint4(oid, did, wid, j);
+                row[1] = make_uint4(item, sw, 0u, qty);
+                *tail = make_uint2(qty * price, data);
+            } else {
+                const uint4 a = row[0], b = row[1];
+                const uint2 c = *tail;
+                row[0] = a; row[1] = b; *tail = c;
             }
         }
     } else {
-        const uint32_t p = (wave - no_waves) * 64 + lane;
-        if (p >= pay.n || !((d.bitmap_pay[wave - no_waves] >> lane) & 1ull)) return;
+        const uint32_t p = (blockIdx.x - no_blocks) * kBlock + threadIdx.x;
+        if (p >= pay.n || !((d.bitmap_pay[p / 64] >> (p % 64)) & 1ull)) return;
         const PayRows r = pay_rows(d, pay, p);
         const uint32_t amt = pay.H_AMOUNT[p];
         if (r.cu < d.t[CU].rows) {
@@ -591,9 +616,46 @@ __global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDe
                 row[0] = a; *tail = b;
             }
         }
-        if (rmw) {
-            if (r.wh < d.t[WH].rows) atomicAdd(d.t[WH].data + r.wh * kStride[WH] + 2, amt);
-            if (r.di < d.t[DI].rows) atomicAdd(d.t[DI].data + r.di * kStride[DI] + 3, amt);
+    }
+}
+
+// W_YTD / D_YTD += H_AMOUNT of committed Payments (the hot-row reduce,
+// Genericfunction.h:431-518, applied to the table).  A block sums a chunk of
+// Payments into LDS bins (W warehouse bins, then 10W district bins) and adds
+// each non-zero bin to the table once, so a hot row takes one global atomic
+// per block instead of one per Payment.  More bins than LDS holds: direct
+// global atomics.
+constexpr uint32_t kYtdBins = 12288;  // 48 KB of LDS
+
+__global__ __launch_bounds__(kBlock) void ytd_kernel(TpccDev d, PaymentDev pay, uint32_t chunk)
+{
+    __shared__ uint32_t bins[kYtdBins];
+    const uint64_t W = d.t[WH].rows, nd = d.t[DI].rows;
+    const bool lds = W + nd <= kYtdBins;
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < W + nd; i += kBlock) bins[i] = 0;
+    __syncthreads();
+    const uint64_t p0 = (uint64_t)blockIdx.x * chunk, p1 = min<uint64_t>(pay.n, p0 + chunk);
+    for (uint64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        if (!((d.bitmap_pay[p / 64] >> (p % 64)) & 1ull)) continue;
+        const uint64_t w = pay.W_ID[p], di = w * 10 + pay.D_ID[p];
+        const uint32_t amt = pay.H_AMOUNT[p];
+        if (w < W) {
+            if (lds) atomicAdd(&bins[w], amt);
+            else atomicAdd(d.t[WH].data + w * kStride[WH] + 2, amt);
+        }
+        if (di < nd) {
+            if (lds) atomicAdd(&bins[W + di], amt);
+            else atomicAdd(d.t[DI].data + di * kStride[DI] + 3, amt);
+        }
+    }
+    __syncthreads();
+    if (lds) {
+        for (uint32_t i = threadIdx.x; i < W + nd; i += kBlock) {
+            const uint32_t v = bins[i];
+            if (v == 0) continue;
+            if (i < W) atomicAdd(d.t[WH].data + (uint64_t)i * kStride[WH] + 2, v);
+            else atomicAdd(d.t[DI].data + (uint64_t)(i - W) * kStride[DI] + 3, v);
         }
     }
 }
@@ -645,14 +707,14 @@ hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t 
 hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
                                bool counts, DeviceCounters *ctr, hipStream_t s)
 {
-    const uint32_t no_waves = waves64(no.n) * 16, waves = no_waves + waves64(pay.n);
-    const uint32_t blocks = waves ? (waves + kWaves - 1) / kWaves : 1;
+    const uint32_t no_blocks = (no.n + kNoPerBlock - 1) / kNoPerBlock;
+    const uint32_t blocks = std::max(1u, no_blocks + (pay.n + kBlock - 1) / kBlock);
     if (counts)
         hipLaunchKernelGGL(execute_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ctr);
+                           no_blocks, ctr);
     else
         hipLaunchKernelGGL(execute_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ctr);
+                           no_blocks, ctr);
     return hipGetLastError();
 }
 
@@ -674,11 +736,15 @@ hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const Paym
 hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
                                  uint32_t values, hipStream_t s)
 {
-    const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
-    if (waves == 0) return hipSuccess;
-    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(writeback_kernel, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_waves,
-                       values == LTPG_VALUES_RMW ? 1 : 0);
+    const uint32_t no_blocks = (no.n + kNoPerBlock - 1) / kNoPerBlock;
+    const uint32_t blocks = no_blocks + (pay.n + kBlock - 1) / kBlock;
+    if (blocks == 0) return hipSuccess;
+    const int rmw = values == LTPG_VALUES_RMW ? 1 : 0;
+    hipLaunchKernelGGL(writeback_kernel, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_blocks, rmw);
+    if (rmw && pay.n) {
+        const uint32_t chunk = std::max<uint32_t>(4096, (pay.n + 255) / 256);
+        hipLaunchKernelGGL(ytd_kernel, dim3((pay.n + chunk - 1) / chunk), dim3(kBlock), 0, s, d, pay, chunk);
+    }
     return hipGetLastError();
 }
 
