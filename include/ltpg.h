@@ -361,6 +361,45 @@ int ltpg_set_timing(ltpg_ctx *ctx, int enable);
  * measurement.  Synchronises the context. */
 int ltpg_phase_times(ltpg_ctx *ctx, float ms[LTPG_TIMING_SLOTS], uint32_t *n_epochs);
 
+/* ---- batched form: many batches of uneven size and mixed types in one
+ *      submission (replaces the tp driver's per-epoch loop, TPCC-allopt-
+ *      final/tp.cu:136-261, YCSB/0/ycsb.cu:78-152, and its query prefetch
+ *      threads, TPCC-allopt-final/Query.h:640-725) --------------------------- */
+typedef struct ltpg_batch_desc {
+    uint32_t workload;              /* LTPG_WORKLOAD_KV or LTPG_WORKLOAD_TPCC     */
+    uint32_t reserved;
+    ltpg_kv_batch kv;               /* device pointers (workload KV)              */
+    ltpg_neworder_batch neworder;   /* device pointers (workload TPCC)            */
+    ltpg_payment_batch payment;
+} ltpg_batch_desc;
+
+/* PLAIN: one epoch per batch, in list order, each followed by a host
+ *   synchronisation and commit-count readback (the reference's loop).
+ * GRAPH: the batches of each workload, in list order, captured once into a
+ *   HIP graph (register refill, then per batch: submit, CID / EXECUTE /
+ *   CHECK / WRITEBACK, bitmap archive copy); the key-value and order-entry
+ *   graphs run concurrently on their contexts' streams, with one host
+ *   synchronisation at the end.
+ * Both produce identical commit sets and tables: batches of one workload run
+ * in list order, and the two workloads share no table. */
+#define LTPG_RUN_PLAIN 0u
+#define LTPG_RUN_GRAPH 1u
+
+typedef struct ltpg_batched ltpg_batched;
+
+/* A runner over n batches whose arrays stay resident in device memory until
+ * ltpg_batched_destroy.  kv_ctx / tpcc_ctx may be NULL when no batch of that
+ * workload is listed; each context's capacity bounds its batches. */
+int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_desc *descs, uint32_t n,
+                        ltpg_batched **out);
+/* Run every batch once in `mode`.  Blocks until done; range errors of any
+ * batch are returned as LTPG_ERR_RANGE. */
+int ltpg_batched_run(ltpg_batched *b, uint32_t mode);
+/* Commit bitmap of batch i (type LTPG_TYPE_KV, or LTPG_TYPE_NEWORDER /
+ * LTPG_TYPE_PAYMENT for an order-entry batch) from the last run. */
+int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words);
+int ltpg_batched_destroy(ltpg_batched *b);
+
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:
  *      17-121, YCSB/0/Random.h:12,52) ----------------------------------- */

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import Config, KvBatch, KvGen, NeworderBatch, PaymentBatch, TpccGenConfig, lib
+from ._lib import BatchDesc, Config, KvBatch, KvGen, NeworderBatch, PaymentBatch, TpccGenConfig, lib
 
 OK = 0
 ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_RANGE, ERR_BUDGET, ERR_UNSUPPORTED = (
@@ -52,6 +52,7 @@ TPCC_MAX_LINES = 15
 NEWORDER_READS, PAYMENT_READS = 51, 24
 # u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
 NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
+RUN_PLAIN, RUN_GRAPH = 0, 1
 
 
 def kv_meta(kind, field):
@@ -418,6 +419,73 @@ class TpccGenerator:
             self.close()
         except Exception:
             pass
+
+
+def upload_tpcc_batch(ctx, b):
+    """Copy a TpccBatch into device buffers of ctx; returns
+    ((NeworderBatch, PaymentBatch) of device pointers, [buffers])."""
+    nob, pab = DeviceBuffer(ctx, max(b.no.nbytes, 16)), DeviceBuffer(ctx, max(b.pay.nbytes, 16))
+    nob.upload(b.no)
+    pab.upload(b.pay)
+    hn, hp = neworder_struct(b.no, b.no_cap, b.n_no), payment_struct(b.pay, b.pay_cap, b.n_pay)
+    nb, pb = NeworderBatch(), PaymentBatch()
+    nb.n, nb.line_stride, pb.n = hn.n, hn.line_stride, hp.n
+    for name, _ in NeworderBatch._fields_[2:]:
+        setattr(nb, name, nob.ptr + (getattr(hn, name) - b.no.ctypes.data))
+    for name, _ in PaymentBatch._fields_[2:]:
+        setattr(pb, name, pab.ptr + (getattr(hp, name) - b.pay.ctypes.data))
+    return (nb, pb), [nob, pab]
+
+
+class Batched:
+    """ltpg_batched: a list of device-resident batches of both workloads run
+    as one submission (RUN_GRAPH) or one epoch at a time (RUN_PLAIN)."""
+
+    def __init__(self, kv_ctx, tpcc_ctx, descs):
+        arr = (BatchDesc * max(len(descs), 1))(*descs)
+        h = ctypes.c_void_p()
+        _check(lib.ltpg_batched_create(kv_ctx.handle if kv_ctx else None, tpcc_ctx.handle if tpcc_ctx else None,
+                                       arr, len(descs), ctypes.byref(h)), "ltpg_batched_create")
+        self.handle = h
+        self.descs = list(descs)
+
+    def run(self, mode=RUN_GRAPH):
+        _check(lib.ltpg_batched_run(self.handle, mode), "ltpg_batched_run")
+
+    def bitmap(self, i, type_):
+        d = self.descs[i]
+        n = {TYPE_KV: d.kv.n_txns, TYPE_NEWORDER: d.neworder.n, TYPE_PAYMENT: d.payment.n}[type_]
+        out = np.zeros(max(1, (n + 63) // 64), dtype=np.uint64)
+        _check(lib.ltpg_batched_bitmap(self.handle, i, type_, out.ctypes.data_as(_lib.c_u64p), out.size),
+               "ltpg_batched_bitmap")
+        return out[:(n + 63) // 64]
+
+    def close(self):
+        if self.handle:
+            lib.ltpg_batched_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kv_desc(batch):
+    """BatchDesc of a DeviceKvBatch."""
+    d = BatchDesc()
+    d.workload = WORKLOAD_KV
+    d.kv = batch.struct()
+    return d
+
+
+def tpcc_desc(dev):
+    """BatchDesc of an uploaded order-entry batch ((nb, pb) of upload_tpcc_batch)."""
+    d = BatchDesc()
+    d.workload = WORKLOAD_TPCC
+    d.neworder, d.payment = dev
+    return d
 
 
 def kv_generate(seed=0x4C545047, rows=1_000_000, fields=10, n_txns=65536, n_slots=10, min_ops=None,

@@ -93,6 +93,13 @@ class TpccGenConfig(ctypes.Structure):
                                              "max_lines", "max_amount", "max_quantity")]
 
 
+class BatchDesc(ctypes.Structure):
+    """struct ltpg_batch_desc."""
+
+    _fields_ = [("workload", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("kv", KvBatch),
+                ("neworder", NeworderBatch), ("payment", PaymentBatch)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
@@ -141,6 +148,11 @@ SIGNATURES = [
     ("ltpg_tpcc_gen_next", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u32p, ctypes.c_uint32, c_u32p, ctypes.c_uint32,
       c_u32p, c_u32p]),
+    ("ltpg_batched_create", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(BatchDesc), ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("ltpg_batched_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("ltpg_batched_bitmap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u64p, ctypes.c_uint64]),
+    ("ltpg_batched_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
     ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
 ]

@@ -156,21 +156,32 @@ int init_tpcc(ltpg_ctx *ctx)
     return LTPG_OK;
 }
 
+}  // namespace
+
+int refill_registers(ltpg_ctx *ctx)
+{
+    if (ctx->kv.regs) {
+        LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u8(ctx->kv.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+    }
+    if (ctx->tpcc.d.regs) {
+        LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.regs, ctx->tpcc.d.n_regs, ~0ull, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u8(ctx->tpcc.d.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.new_last, ctx->tpcc.d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
+    }
+    return LTPG_OK;
+}
+
+namespace {
+
 // Advance the epoch (1..255).  At wrap-around every register goes back to
 // all-ones (the tag of epoch 0, never current) and the doomed flags to 0.
 int next_epoch(ltpg_ctx *ctx)
 {
     ctx->epoch += 1;
     if (ctx->epoch > 255) {
-        if (ctx->kv.regs) {
-            LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
-            LTPG_HIP_TRY(launch_fill_u8(ctx->kv.doomed, ctx->cfg.max_txns, 0, ctx->stream));
-        }
-        if (ctx->tpcc.d.regs) {
-            LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.regs, ctx->tpcc.d.n_regs, ~0ull, ctx->stream));
-            LTPG_HIP_TRY(launch_fill_u8(ctx->tpcc.d.doomed, ctx->cfg.max_txns, 0, ctx->stream));
-            LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.new_last, ctx->tpcc.d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
-        }
+        int rc = refill_registers(ctx);
+        if (rc) return rc;
         ctx->epoch = 1;
     }
     return LTPG_OK;

@@ -1,0 +1,220 @@
+// batched.cpp - the batched form: many batches of uneven size and mixed
+// types in one submission.
+//
+// The reference drives one epoch at a time from the host: launch a phase,
+// synchronise, launch the next, copy COMMIT back, prefetch the next batch on
+// a helper thread (LTPG/TPCC-allopt-final/tp.cu:136-261, Query.h:640-725).
+// Here the batches are resident in device memory and each workload's whole
+// sequence of epochs is captured once into a HIP graph: a register refill
+// (so that every replay starts from the same register state and uses the
+// same epoch tags), then per batch its submit, the phase kernels and a copy
+// of its commit bitmap(s) into a device archive.  The key-value and the
+// order-entry graphs replay concurrently on their contexts' streams (the two
+// workloads share no table), with one host synchronisation for the lot.
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ltpg_internal.h"
+
+using namespace ltpg;
+
+struct ltpg_batched {
+    ltpg_ctx *ctx[2] = {nullptr, nullptr};  // [0] key-value, [1] order-entry
+    std::vector<ltpg_batch_desc> d;
+    std::vector<uint64_t> off;              // archive offset (u64 words) of batch i's first bitmap
+    uint64_t *archive = nullptr;
+    uint64_t archive_words = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    uint32_t final_epoch[2] = {0, 0};       // host epoch counter after a replay
+};
+
+namespace {
+
+uint64_t words(uint64_t n) { return (n + 63) / 64; }
+
+int slot_of(uint32_t workload) { return workload == LTPG_WORKLOAD_KV ? 0 : 1; }
+
+// One epoch of batch i on its context: submit (device-resident), all
+// phases, archive copy of the commit bitmap(s).  No synchronisation, so it
+// can be captured.
+int enqueue_batch(ltpg_batched *b, uint32_t i)
+{
+    const ltpg_batch_desc &d = b->d[i];
+    ltpg_ctx *ctx = b->ctx[slot_of(d.workload)];
+    int rc;
+    if (d.workload == LTPG_WORKLOAD_KV) {
+        if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
+    } else {
+        if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
+    }
+    if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+    if (d.workload == LTPG_WORKLOAD_KV) {
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i], ctx->kv.bitmap, words(d.kv.n_txns), ctx->stream));
+    } else {
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i], ctx->tpcc.d.bitmap_no, words(d.neworder.n),
+                                     ctx->stream));
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i] + words(d.neworder.n), ctx->tpcc.d.bitmap_pay,
+                                     words(d.payment.n), ctx->stream));
+    }
+    return LTPG_OK;
+}
+
+// Capture the epochs of workload slot k into a graph.
+int capture(ltpg_batched *b, int k)
+{
+    ltpg_ctx *ctx = b->ctx[k];
+    const bool timing = ctx->timing;
+    const uint32_t epoch = ctx->epoch;  // capture runs nothing: the device state keeps this epoch
+    ctx->timing = false;
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    LTPG_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    // every replay starts from all-ones registers and epoch 1
+    ctx->epoch = 0;
+    int rc = refill_registers(ctx);
+    for (uint32_t i = 0; i < b->d.size() && !rc; i++)
+        if (slot_of(b->d[i].workload) == k) rc = enqueue_batch(b, i);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    b->final_epoch[k] = ctx->epoch;
+    ctx->epoch = epoch;
+    ctx->timing = timing;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph[k] = g;
+    LTPG_HIP_TRY(hipGraphInstantiate(&b->exec[k], g, nullptr, nullptr, 0));
+    return LTPG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ltpg_batched_destroy(ltpg_batched *b)
+{
+    if (!b) return LTPG_OK;
+    for (int k = 0; k < 2; k++) {
+        if (b->ctx[k]) (void)hipStreamSynchronize(b->ctx[k]->stream);
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    }
+    dev_free(b->archive);
+    delete b;
+    return LTPG_OK;
+}
+
+int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_desc *descs, uint32_t n,
+                        ltpg_batched **out)
+{
+    if (!out || (!descs && n)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (kv_ctx && !kv_ctx->kv.table) LTPG_FAIL(LTPG_ERR_INVALID, "kv_ctx is not a key-value context");
+    if (tpcc_ctx && !tpcc_ctx->tpcc.d.regs) LTPG_FAIL(LTPG_ERR_INVALID, "tpcc_ctx is not an order-entry context");
+    if (kv_ctx && tpcc_ctx && kv_ctx->device != tpcc_ctx->device)
+        LTPG_FAIL(LTPG_ERR_INVALID, "both contexts must be on one device");
+    ltpg_batched *b = new (std::nothrow) ltpg_batched();
+    if (!b) LTPG_FAIL(LTPG_ERR_NOMEM, "host allocation of batched runner failed");
+    b->ctx[0] = kv_ctx;
+    b->ctx[1] = tpcc_ctx;
+    b->d.assign(descs, descs + n);
+    b->off.resize(n);
+    uint64_t w = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const ltpg_batch_desc &d = b->d[i];
+        b->off[i] = w;
+        if (d.workload == LTPG_WORKLOAD_KV) {
+            if (!kv_ctx) { ltpg_batched_destroy(b); LTPG_FAIL(LTPG_ERR_INVALID, "key-value batch without kv_ctx"); }
+            w += words(d.kv.n_txns);
+        } else if (d.workload == LTPG_WORKLOAD_TPCC) {
+            if (!tpcc_ctx) { ltpg_batched_destroy(b); LTPG_FAIL(LTPG_ERR_INVALID, "order-entry batch without tpcc_ctx"); }
+            w += words(d.neworder.n) + words(d.payment.n);
+        } else {
+            ltpg_batched_destroy(b);
+            LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload in batch descriptor");
+        }
+    }
+    b->archive_words = w;
+    int rc = check_ctx(kv_ctx ? kv_ctx : tpcc_ctx);
+    if (!rc) rc = dev_alloc(&b->archive, w ? w : 1, "bitmap archive");
+    for (int k = 0; k < 2 && !rc; k++) {
+        bool any = false;
+        for (const auto &d : b->d) any |= slot_of(d.workload) == k;
+        if (any) rc = capture(b, k);
+    }
+    if (rc) {
+        const std::string msg = ltpg_last_error();
+        ltpg_batched_destroy(b);
+        set_error(msg);
+        return rc;
+    }
+    *out = b;
+    return LTPG_OK;
+}
+
+int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
+{
+    if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null runner");
+    int rc = check_ctx(b->ctx[0] ? b->ctx[0] : b->ctx[1]);
+    if (rc) return rc;
+    if (mode == LTPG_RUN_PLAIN) {
+        // the reference's loop: per batch its phases, a synchronisation, the
+        // commit count back on the host
+        for (uint32_t i = 0; i < b->d.size(); i++) {
+            ltpg_ctx *ctx = b->ctx[slot_of(b->d[i].workload)];
+            if ((rc = enqueue_batch(b, i))) return rc;
+            if ((rc = ltpg_sync(ctx))) return rc;
+            uint32_t c = 0;
+            if ((rc = ltpg_commit_count(ctx, b->d[i].workload == LTPG_WORKLOAD_KV ? LTPG_TYPE_KV
+                                                                                  : LTPG_TYPE_PAYMENT, &c)))
+                return rc;
+        }
+        return LTPG_OK;
+    }
+    if (mode != LTPG_RUN_GRAPH) LTPG_FAIL(LTPG_ERR_INVALID, "unknown run mode");
+    for (int k = 0; k < 2; k++) {
+        if (!b->exec[k]) continue;
+        LTPG_HIP_TRY(hipGraphLaunch(b->exec[k], b->ctx[k]->stream));
+        // the registers now carry the captured sequence's epoch tags
+        b->ctx[k]->epoch = b->final_epoch[k];
+    }
+    int first = LTPG_OK;
+    for (int k = 0; k < 2; k++) {
+        if (!b->exec[k]) continue;
+        rc = ltpg_sync(b->ctx[k]);
+        if (rc && !first) first = rc;
+    }
+    return first;
+}
+
+int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words)
+{
+    if (!b || !out) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    if (batch >= b->d.size()) LTPG_FAIL(LTPG_ERR_INVALID, "batch index out of range");
+    const ltpg_batch_desc &d = b->d[batch];
+    uint64_t off = b->off[batch], n = 0;
+    if (d.workload == LTPG_WORKLOAD_KV && type == LTPG_TYPE_KV) {
+        n = words(d.kv.n_txns);
+    } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_NEWORDER) {
+        n = words(d.neworder.n);
+    } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_PAYMENT) {
+        off += words(d.neworder.n);
+        n = words(d.payment.n);
+    } else {
+        LTPG_FAIL(LTPG_ERR_INVALID, "type does not match the batch's workload");
+    }
+    if (n_words < n) LTPG_FAIL(LTPG_ERR_INVALID, "bitmap buffer too small");
+    int rc = check_ctx(b->ctx[slot_of(d.workload)]);
+    if (rc) return rc;
+    for (int k = 0; k < 2; k++)
+        if (b->ctx[k]) LTPG_HIP_TRY(hipStreamSynchronize(b->ctx[k]->stream));
+    if (n) LTPG_HIP_TRY(hipMemcpy(out, b->archive + off, n * 8, hipMemcpyDeviceToHost));
+    return LTPG_OK;
+}
+
+}  // extern "C"

@@ -175,6 +175,12 @@ __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+__global__ void copy_u64_kernel(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void fill_u8_kernel(uint8_t *p, uint64_t n, uint8_t v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -564,6 +570,13 @@ hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_u64_kernel, grid_stride(n), dim3(kBlock), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_u64_kernel, grid_stride(n), dim3(kBlock), 0, s, dst, src, n);
     return hipGetLastError();
 }
 

@@ -197,6 +197,12 @@ hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s);
+hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipStream_t s);
+
+// Registers back to "no registrant" (all-ones), doomed flags and NEWORDER
+// inserters to 0, on the context's stream (api.cpp).  Epoch-tag wrap and the
+// start of a captured batch sequence use it.
+int refill_registers(ltpg_ctx *ctx);
 hipError_t launch_kv_fill_reference(const KvState &kv, hipStream_t s);
 hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, uint32_t fw,
                          uint32_t fsw, int to_strided, hipStream_t s);

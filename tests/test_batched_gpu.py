@@ -1,0 +1,111 @@
+"""Batched form: the graph-captured submission of many uneven, mixed
+batches gives the same commit sets and tables as the plain per-batch loop
+(the reference's epoch loop), bit for bit, replay after replay."""
+import numpy as np
+import pytest
+
+import ltpg_amd as L
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x4C545047
+KV = dict(kv_rows=200_000, kv_fields=4, kv_field_bytes=16, kv_max_slots=24, max_txns=6000)
+W = 2
+
+
+def _contexts():
+    kv = L.Context(workload=L.WORKLOAD_KV, **KV)
+    kv.kv_fill_reference()
+    tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=6000)
+    tp.tpcc_populate(SEED)
+    return kv, tp
+
+
+def _batches(rng, n, kv, tp, kinds=None):
+    """n batches, sizes log-uniform in [64, 6000], type from the seed."""
+    descs, keep, meta = [], [], []
+    gen = L.TpccGenerator(warehouses=W, seed=5)
+    for i in range(n):
+        size = int(np.exp(rng.uniform(np.log(64), np.log(6000))))
+        kind = kinds[i] if kinds else ("kv" if rng.random() < 0.5 else "tpcc")
+        if kind == "kv":
+            keys, m, tids = L.kv_generate(seed=SEED + i, rows=KV["kv_rows"], fields=KV["kv_fields"], n_txns=size,
+                                          n_slots=KV["kv_max_slots"], min_ops=1, max_ops=KV["kv_max_slots"],
+                                          theta=0.9, tid_base=i * 10000)
+            db = L.DeviceKvBatch(kv, keys, m, tids, KV["kv_max_slots"])
+            keep.append(db)
+            descs.append(L.kv_desc(db))
+        else:
+            b = gen.next(size, tid_base=i * 10000)
+            dev, bufs = L.upload_tpcc_batch(tp, b)
+            keep.append(bufs)
+            descs.append(L.tpcc_desc(dev))
+        meta.append(kind)
+    return descs, keep, meta
+
+
+def _bitmaps(runner, meta):
+    out = []
+    for i, kind in enumerate(meta):
+        if kind == "kv":
+            out.append([runner.bitmap(i, L.TYPE_KV)])
+        else:
+            out.append([runner.bitmap(i, L.TYPE_NEWORDER), runner.bitmap(i, L.TYPE_PAYMENT)])
+    return out
+
+
+def _tables(kv, tp):
+    return [kv.kv_table_read()] + [tp.tpcc_table_read(t) for t in range(9)]
+
+
+def _run(mode, n, runs, kinds=None):
+    kv, tp = _contexts()
+    descs, keep, meta = _batches(np.random.default_rng(42), n, kv, tp, kinds)
+    r = L.Batched(kv, tp, descs)
+    maps = []
+    for _ in range(runs):
+        r.run(mode)
+        maps.append(_bitmaps(r, meta))
+    tabs = _tables(kv, tp)
+    r.close()
+    return maps, tabs
+
+
+def test_graph_form_matches_plain_loop(gpu):
+    plain_maps, plain_tabs = _run(L.RUN_PLAIN, 40, 2)
+    graph_maps, graph_tabs = _run(L.RUN_GRAPH, 40, 2)
+    for rp, rg in zip(plain_maps, graph_maps):
+        for bp, bg in zip(rp, rg):
+            for x, y in zip(bp, bg):
+                np.testing.assert_array_equal(x, y)
+    for x, y in zip(plain_tabs, graph_tabs):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_graph_replays_match_across_an_epoch_tag_wrap(gpu):
+    """300 key-value epochs in one graph: the captured refill at the tag wrap
+    keeps the results equal to the plain loop."""
+    kinds = ["kv"] * 300
+    plain_maps, plain_tabs = _run(L.RUN_PLAIN, 300, 1, kinds)
+    graph_maps, graph_tabs = _run(L.RUN_GRAPH, 300, 1, kinds)
+    for rp, rg in zip(plain_maps[0], graph_maps[0]):
+        np.testing.assert_array_equal(rp[0], rg[0])
+    np.testing.assert_array_equal(plain_tabs[0], graph_tabs[0])
+
+
+def test_plain_epochs_after_a_graph_run(gpu):
+    """A context used by a graph run keeps working epoch by epoch."""
+    kv, tp = _contexts()
+    descs, keep, meta = _batches(np.random.default_rng(7), 10, kv, tp)
+    r = L.Batched(kv, tp, descs)
+    r.run(L.RUN_GRAPH)
+    r.run(L.RUN_PLAIN)
+    first = _bitmaps(r, meta)
+    r.run(L.RUN_GRAPH)
+    second = _bitmaps(r, meta)
+    # the same batches on the tables as each run leaves them: commit sets
+    # depend only on the batches (and CUSTOMER.C_LAST, never written)
+    for a, b in zip(first, second):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    r.close()
