@@ -315,6 +315,14 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
                        uint32_t no_cap, uint32_t *pay_block, uint32_t pay_cap, uint32_t *n_neworder,
                        uint32_t *n_payment);
 
+/* ---- launch shape of the key-value kernels (replaces the fixed warp-role
+ *      and grid constants, YCSB/0/Predefine.h:27-35) ------------------------ */
+/* execute_per_op: EXECUTE runs one lane per op instead of one lane per
+ * transaction.  split_registers: a row's R and W registers live in two
+ * arrays instead of one 16-byte pair.  Results are identical in every
+ * combination; may change between any two epochs. */
+int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t execute_per_op, uint32_t split_registers);
+
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
  *      tp.cu:24-117, YCSB/0/ycsb.cu:9-58; launched by tp.cu:154-236) ------- */
 /* Run the phases in phase_mask on the current batch, in the order CID,
@@ -399,6 +407,22 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode);
  * LTPG_TYPE_PAYMENT for an order-entry batch) from the last run. */
 int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words);
 int ltpg_batched_destroy(ltpg_batched *b);
+
+/* Autotuner.  For every size class of the runner's key-value batches
+ * (floor(log2(n_txns))) it times each ltpg_kv_set_tuning combination on the
+ * first batch of the class (reps epochs after one warm-up, HIP events on the
+ * context's stream, write-back in copy-back mode so the tables are left as
+ * they were), keeps the fastest for every batch of the class, and captures
+ * the graph again.  One entry per class is written to out. */
+typedef struct ltpg_tuning_entry {
+    uint32_t workload;         /* LTPG_WORKLOAD_KV                                  */
+    uint32_t log2_txns;        /* the size class                                    */
+    uint32_t execute_per_op;   /* chosen                                            */
+    uint32_t split_registers;  /* chosen                                            */
+    float ms[4];               /* epoch time of (per_op, split) = 00, 01, 10, 11    */
+} ltpg_tuning_entry;
+int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
+                          uint32_t *n_out);
 
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:

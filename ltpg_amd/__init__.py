@@ -199,6 +199,10 @@ class Context:
         _check(lib.ltpg_kv_table_read(self.handle, _ptr(out), rows, layout), "ltpg_kv_table_read")
         return out
 
+    def kv_set_tuning(self, execute_per_op=False, split_registers=False):
+        _check(lib.ltpg_kv_set_tuning(self.handle, int(bool(execute_per_op)), int(bool(split_registers))),
+               "ltpg_kv_set_tuning")
+
     def kv_fill_reference(self):
         _check(lib.ltpg_kv_table_fill_reference(self.handle), "ltpg_kv_table_fill_reference")
 
@@ -451,6 +455,15 @@ class Batched:
 
     def run(self, mode=RUN_GRAPH):
         _check(lib.ltpg_batched_run(self.handle, mode), "ltpg_batched_run")
+
+    def autotune(self, reps=3):
+        """Pick the key-value launch shape per size class; returns one dict
+        per class (log2 size, chosen flags, ms of the four shapes)."""
+        out = (_lib.TuningEntry * 33)()
+        n = ctypes.c_uint32()
+        _check(lib.ltpg_batched_autotune(self.handle, reps, out, 33, ctypes.byref(n)), "ltpg_batched_autotune")
+        return [{"log2_txns": e.log2_txns, "execute_per_op": e.execute_per_op, "split_registers": e.split_registers,
+                 "ms": list(e.ms)} for e in out[:n.value]]
 
     def bitmap(self, i, type_):
         d = self.descs[i]

@@ -100,6 +100,13 @@ class BatchDesc(ctypes.Structure):
                 ("neworder", NeworderBatch), ("payment", PaymentBatch)]
 
 
+class TuningEntry(ctypes.Structure):
+    """struct ltpg_tuning_entry."""
+
+    _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("execute_per_op", ctypes.c_uint32),
+                ("split_registers", ctypes.c_uint32), ("ms", ctypes.c_float * 4)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
@@ -118,6 +125,7 @@ SIGNATURES = [
     ("ltpg_kv_table_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     ("ltpg_kv_table_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     ("ltpg_kv_table_fill_reference", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_kv_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("ltpg_kv_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KvBatch), ctypes.c_uint32]),
     ("ltpg_epoch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_epoch_reset", ctypes.c_int, [ctypes.c_void_p]),
@@ -153,6 +161,8 @@ SIGNATURES = [
     ("ltpg_batched_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_batched_bitmap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u64p, ctypes.c_uint64]),
     ("ltpg_batched_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_batched_autotune", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(TuningEntry), ctypes.c_uint32, c_u32p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
     ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
 ]

@@ -539,6 +539,17 @@ int ltpg_kv_table_fill_reference(ltpg_ctx *ctx)
     return LTPG_OK;
 }
 
+int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t execute_per_op, uint32_t split_registers)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    if (execute_per_op > 1 || split_registers > 1) LTPG_FAIL(LTPG_ERR_INVALID, "tuning flags are 0 or 1");
+    ctx->kv.exec_per_op = execute_per_op;
+    ctx->kv.split_regs = split_registers;
+    return LTPG_OK;
+}
+
 // ---- key-value batches --------------------------------------------------
 
 int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)

@@ -30,6 +30,7 @@ struct ltpg_batched {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t final_epoch[2] = {0, 0};       // host epoch counter after a replay
+    std::vector<uint8_t> tune;              // key-value batch i: bit 0 per-op EXECUTE, bit 1 split registers
 };
 
 namespace {
@@ -47,6 +48,7 @@ int enqueue_batch(ltpg_batched *b, uint32_t i)
     ltpg_ctx *ctx = b->ctx[slot_of(d.workload)];
     int rc;
     if (d.workload == LTPG_WORKLOAD_KV) {
+        if ((rc = ltpg_kv_set_tuning(ctx, b->tune[i] & 1u, (b->tune[i] >> 1) & 1u))) return rc;
         if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
     } else {
         if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
@@ -124,6 +126,7 @@ int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_d
     b->ctx[1] = tpcc_ctx;
     b->d.assign(descs, descs + n);
     b->off.resize(n);
+    b->tune.assign(n, 0);
     uint64_t w = 0;
     for (uint32_t i = 0; i < n; i++) {
         const ltpg_batch_desc &d = b->d[i];
@@ -190,6 +193,82 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
         if (rc && !first) first = rc;
     }
     return first;
+}
+
+int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
+                          uint32_t *n_out)
+{
+    if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null runner");
+    if (n_out) *n_out = 0;
+    ltpg_ctx *ctx = b->ctx[0];
+    if (!ctx) return LTPG_OK;  // no key-value batches: nothing to tune
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (reps == 0) reps = 1;
+    // size classes and their first batch
+    std::vector<int> rep(33, -1);
+    for (uint32_t i = 0; i < b->d.size(); i++) {
+        if (b->d[i].workload != LTPG_WORKLOAD_KV || b->d[i].kv.n_txns == 0) continue;
+        const int c = 31 - __builtin_clz(b->d[i].kv.n_txns);
+        if (rep[c] < 0) rep[c] = (int)i;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    LTPG_HIP_TRY(hipEventCreate(&e0));
+    LTPG_HIP_TRY(hipEventCreate(&e1));
+    const uint32_t values = ctx->cfg.values;
+    ctx->cfg.values = LTPG_VALUES_COPYBACK;  // timing runs leave the table as it is
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    uint32_t n = 0;
+    for (int c = 0; c < 33 && !rc; c++) {
+        if (rep[c] < 0) continue;
+        const ltpg_batch_desc &d = b->d[rep[c]];
+        float ms[4] = {0, 0, 0, 0};
+        for (uint32_t cfg = 0; cfg < 4 && !rc; cfg++) {
+            rc = ltpg_kv_set_tuning(ctx, cfg >> 1, cfg & 1u);
+            for (uint32_t k = 0; k <= reps && !rc; k++) {
+                if (k == 1) rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? LTPG_OK : LTPG_ERR_DEVICE;
+                if (!rc) rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE);
+                if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
+            }
+            if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LTPG_ERR_DEVICE;
+            if (!rc) rc = ltpg_sync(ctx);
+            float t = 0;
+            if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = LTPG_ERR_DEVICE;
+            ms[cfg] = t / (float)reps;
+        }
+        if (rc) break;
+        uint32_t best = 0;
+        for (uint32_t cfg = 1; cfg < 4; cfg++)
+            if (ms[cfg] < ms[best]) best = cfg;
+        // bit 0 per-op EXECUTE, bit 1 split registers
+        const uint8_t t = (uint8_t)(((best >> 1) & 1u) | ((best & 1u) << 1));
+        for (uint32_t i = 0; i < b->d.size(); i++)
+            if (b->d[i].workload == LTPG_WORKLOAD_KV && b->d[i].kv.n_txns &&
+                31 - __builtin_clz(b->d[i].kv.n_txns) == c)
+                b->tune[i] = t;
+        if (out && n < max_out) {
+            ltpg_tuning_entry &o = out[n];
+            o.workload = LTPG_WORKLOAD_KV;
+            o.log2_txns = (uint32_t)c;
+            o.execute_per_op = best >> 1;
+            o.split_registers = best & 1u;
+            for (int k = 0; k < 4; k++) o.ms[k] = ms[k];
+        }
+        n++;
+    }
+    ctx->cfg.values = values;
+    ctx->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (n_out) *n_out = n < max_out ? n : max_out;
+    // capture the key-value sequence again with the chosen shapes
+    if (b->exec[0]) (void)hipGraphExecDestroy(b->exec[0]);
+    if (b->graph[0]) (void)hipGraphDestroy(b->graph[0]);
+    b->exec[0] = nullptr;
+    b->graph[0] = nullptr;
+    return capture(b, 0);
 }
 
 int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words)

@@ -58,4 +58,16 @@ __device__ __forceinline__ bool register_write(unsigned long long *r, unsigned l
     return false;
 }
 
+// Where a row's R and W registers live: interleaved (row_mul 2, w_off 1: one
+// 16-byte pair per row) or split (row_mul 1, w_off rows: two arrays).  Either
+// layout may be used in any epoch without clearing: a stale value at any
+// index carries an older epoch's tag and compares greater than every value
+// of the current epoch.
+struct RegLayout {
+    uint64_t row_mul;
+    uint64_t w_off;
+    __host__ __device__ uint64_t r(uint64_t row) const { return row * row_mul; }
+    __host__ __device__ uint64_t w(uint64_t row) const { return row * row_mul + w_off; }
+};
+
 }  // namespace ltpg

@@ -13,7 +13,8 @@
 // check: a wave64 ballot of the commit predicate is one 64-bit word of the
 // commit bitmap.
 //
-// Registers.  regs[2*row] is R and regs[2*row+1] is W, each a u64 priority
+// Registers.  A row's R and W registers sit at rl.r(row) and rl.w(row)
+// (RegLayout: interleaved pairs or two arrays), each a u64 priority
 //     tag:8 | TID:32 | batch position:24
 // where tag = 255 - epoch (epochs 1..255; the library refills every register
 // with all-ones when the epoch counter wraps).  Lower value = higher
@@ -244,7 +245,7 @@ __global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__r
 __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    unsigned long long *__restrict__ regs, uint32_t tag, uint8_t *__restrict__ doomed,
+    unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint8_t *__restrict__ doomed,
     uint8_t epoch, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -260,12 +261,40 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
         bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
         if (!o.valid) continue;
         if (o.kind == LTPG_KV_WRITE)
-            waw |= register_write(&regs[2 * o.row + 1], me, tag, doomed, epoch);
+            waw |= register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch);
         else
-            register_read(&regs[2 * o.row], me);
+            register_read(&regs[rl.r(o.row)], me);
     }
     if (waw) doomed[t] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
+}
+
+// EXECUTE, one lane per op (slot-major op index s * B + t) instead of one
+// lane per transaction: S times the lanes, each with a single registration,
+// for small or ragged batches that would leave the chip idle.
+__global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint8_t *__restrict__ doomed,
+    uint8_t epoch, DeviceCounters *__restrict__ ctr)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < (uint64_t)S * B;
+    const uint64_t s = live ? i / B : 0;
+    const uint32_t t = (uint32_t)(i - s * B);
+    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
+    const bool bad = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid;
+    if (o.valid) {
+        const unsigned long long me = priority(tag, tids[t], t);
+        if (o.kind == LTPG_KV_WRITE) {
+            if (register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch)) doomed[t] = epoch;
+        } else {
+            register_read(&regs[rl.r(o.row)], me);
+        }
+    }
+    if (bad) atomicAdd(&ctr->range_errors, 1u);
 }
 
 // CHECK + COMMIT, counting form (check_r / check_w_0 / check_w_1 and
@@ -277,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
 __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    const unsigned long long *__restrict__ regs, uint32_t tag, uint64_t *__restrict__ bitmap,
+    const unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint64_t *__restrict__ bitmap,
     uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -289,11 +318,10 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
         const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
         if (!o.valid) continue;
         if (o.kind == LTPG_KV_READ) {
-            raw += regs[2 * o.row + 1] < me ? 1u : 0u;
+            raw += regs[rl.w(o.row)] < me ? 1u : 0u;
         } else {
-            const ulonglong2 rw = *reinterpret_cast<const ulonglong2 *>(regs + 2 * o.row);
-            war += rw.x < me ? 1u : 0u;
-            waw += rw.y < me ? 1u : 0u;
+            war += regs[rl.r(o.row)] < me ? 1u : 0u;
+            waw += regs[rl.w(o.row)] < me ? 1u : 0u;
         }
     }
     const bool commit = live && waw == 0 && (raw == 0 || war == 0);
@@ -403,7 +431,7 @@ template <int G>
 __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, uint32_t tag,
+    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag,
     const uint8_t *__restrict__ doomed, uint8_t epoch, const uint32_t *__restrict__ table,
     uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
     DeviceCounters *__restrict__ ctr)
@@ -434,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
         // a read consults W (raw), a write consults R (war): one load
-        const unsigned long long rv = regs[o.valid ? 2 * o.row + (o.kind == LTPG_KV_READ ? 1 : 0) : 0ull];
+        const unsigned long long rv = regs[o.valid ? (o.kind == LTPG_KV_READ ? rl.w(o.row) : rl.r(o.row)) : 0ull];
         const bool raw = o.valid && o.kind == LTPG_KV_READ && rv < me;
         const bool war = o.valid && o.kind == LTPG_KV_WRITE && rv < me;
         const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
@@ -610,12 +638,28 @@ hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows
     return hipGetLastError();
 }
 
+RegLayout reg_layout(const KvState &kv)
+{
+    RegLayout rl;
+    rl.row_mul = kv.split_regs ? 1 : 2;
+    rl.w_off = kv.split_regs ? kv.rows : 1;
+    return rl;
+}
+
 hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e,
                              DeviceCounters *ctr, hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-    hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
-                       b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag, kv.doomed, e.epoch, ctr);
+    if (kv.exec_per_op) {
+        const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
+        hipLaunchKernelGGL(kv_execute_ops_kernel, dim3((unsigned)((ops + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
+                           reg_layout(kv), e.tag, kv.doomed, e.epoch, ctr);
+    } else {
+        hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
+                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), e.tag, kv.doomed,
+                           e.epoch, ctr);
+    }
     return hipGetLastError();
 }
 
@@ -645,14 +689,14 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
     if (b.n_txns == 0) return hipSuccess;
     if (counts) {
         hipLaunchKernelGGL(kv_check_counting_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys,
-                           b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, e.tag,
-                           kv.bitmap, kv.conflicts, ctr);
+                           b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv),
+                           e.tag, kv.bitmap, kv.conflicts, ctr);
         return hipGetLastError();
     }
 #define CALL(GV)                                                                                     \
     hipLaunchKernelGGL(kv_check_fast_kernel<GV>, grid_chunks(b.n_txns), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,     \
-                       kv.field_stride, kv.regs, e.tag, kv.doomed, e.epoch, kv.table,                \
+                       kv.field_stride, kv.regs, reg_layout(kv), e.tag, kv.doomed, e.epoch, kv.table, \
                        kv.read_values, kv.bitmap, ctr)
     LTPG_G_SWITCH(group_for(b.n_slots), CALL)
 #undef CALL

@@ -77,6 +77,10 @@ struct KvState {
     uint32_t *tids_buf = nullptr;
     KvBatchDev cur;
     bool has_batch = false;
+    // tuning (ltpg_kv_set_tuning): EXECUTE one lane per op instead of per
+    // transaction; R and W registers in two arrays instead of pairs
+    uint32_t exec_per_op = 0;
+    uint32_t split_regs = 0;
 };
 
 // ---- order-entry state -------------------------------------------------

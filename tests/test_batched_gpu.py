@@ -109,3 +109,50 @@ def test_plain_epochs_after_a_graph_run(gpu):
         for x, y in zip(a, b):
             np.testing.assert_array_equal(x, y)
     r.close()
+
+
+@pytest.mark.parametrize("per_op", [False, True])
+@pytest.mark.parametrize("split", [False, True])
+def test_kv_launch_shapes_match_oracle(gpu, per_op, split):
+    from oracle import oracle
+    from tests import kvcases as kc
+
+    rows, fields, fw, B, S = 5000, 3, 2, 3000, 12
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw, max_txns=B,
+                    kv_max_slots=S)
+    table = kc.initial_table(rows, fields, fw)
+    ctx.kv_table_load(table)
+    ctx.kv_set_tuning(per_op, split)
+    for ep in range(3):
+        keys, meta, tids = L.kv_generate(seed=900 + ep, rows=rows, fields=fields, n_txns=B, n_slots=S, min_ops=1,
+                                         max_ops=S, theta=0.9, tid_base=ep * B)
+        ctx.kv_submit(keys, meta, tids, S)
+        ctx.epoch_run()
+        ctx.sync()
+        ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, inplace=True)
+        bm = ctx.commit_bitmap()
+        np.testing.assert_array_equal(bm, ora.bitmap)
+        com = np.unpackbits(bm.view(np.uint8), bitorder="little")[:B].astype(bool)
+        mask = np.repeat((((meta.reshape(S, B) & 3) == 1) & com[None, :]).reshape(-1), fw)
+        np.testing.assert_array_equal(ctx.kv_read_values()[mask], ora.read_values[mask])
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+
+
+def test_autotuned_graph_matches_untuned_plain(gpu):
+    kv0, tp0 = _contexts()
+    descs0, keep0, meta = _batches(np.random.default_rng(11), 30, kv0, tp0)
+    plain = L.Batched(kv0, tp0, descs0)
+    plain.run(L.RUN_PLAIN)
+    want = _bitmaps(plain, meta)
+    kv1, tp1 = _contexts()
+    descs1, keep1, _ = _batches(np.random.default_rng(11), 30, kv1, tp1)
+    tuned = L.Batched(kv1, tp1, descs1)
+    table = tuned.autotune(reps=2)
+    assert table and all(len(e["ms"]) == 4 and min(e["ms"]) > 0 for e in table)
+    tuned.run(L.RUN_GRAPH)
+    got = _bitmaps(tuned, meta)
+    for a, b in zip(want, got):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    for x, y in zip(_tables(kv0, tp0), _tables(kv1, tp1)):
+        np.testing.assert_array_equal(x, y)
