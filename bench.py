@@ -367,6 +367,100 @@ def run_tpcc(args, cpu_baseline=True):
     return result
 
 
+def run_batched_c4(args):
+    """Batched form against the plain per-batch loop on the same mixed batch
+    list: 256 batches, sizes floor(exp(U[ln 1e3, ln 1e6])) transactions, each
+    key-value (1-64 ops, Zipf 0.9, the 10M x 10 x 100 B table) or order-entry
+    (W = 256) by a seeded coin.  Commit sets of the two forms must match."""
+    import ltpg_amd as L
+
+    seed = 0x4C545047
+    nb = args.c4_batches
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e6), nb))).astype(np.int64)
+    is_kv = rng.random(nb) < 0.5
+    desc = (f"batched form vs plain loop: {nb} batches, sizes log-uniform in [1e3, 1e6] txns, mixed key-value "
+            f"(1-64 ops, Zipf 0.9, 10M x 10 x 100 B) and order-entry (W=256, 50/50 NewOrder/Payment)")
+    log(f"C4: {desc}; {int(sizes.sum())} txns ({int(sizes[is_kv].sum())} key-value)")
+    rows, fields, fw, S = 10_000_000, 10, 25, 64
+    kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                   max_txns=int(max(sizes.max(), 1)), kv_max_slots=S, values=L.VALUES_RMW)
+    kv.kv_fill_reference()
+    tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=256, max_txns=int(max(sizes.max(), 1)),
+                   values=L.VALUES_RMW)
+    tp.tpcc_populate(seed)
+    gen = L.TpccGenerator(warehouses=256, seed=seed)
+    descs, keep, kinds = [], [], []
+    tid = 0
+    for i in range(nb):
+        n = int(sizes[i])
+        if is_kv[i]:
+            k, m, t = L.kv_generate(seed=seed + i, rows=rows, fields=fields, n_txns=n, n_slots=S, min_ops=1,
+                                    max_ops=S, theta=0.9, read_pct=50, tid_base=tid)
+            db = L.DeviceKvBatch(kv, k, m, t, S)
+            keep.append(db)
+            descs.append(L.kv_desc(db))
+            kinds.append("kv")
+        else:
+            b = gen.next(n, tid_base=tid)
+            dev, bufs = L.upload_tpcc_batch(tp, b)
+            keep.append(bufs)
+            descs.append(L.tpcc_desc(dev))
+            kinds.append("tpcc")
+        tid += n
+        if (i + 1) % 32 == 0:
+            log(f"C4: {i + 1}/{nb} batches resident in HBM")
+    runner = L.Batched(kv, tp, descs)
+
+    def bitmaps():
+        out = []
+        for i, kind in enumerate(kinds):
+            if kind == "kv":
+                out.append(runner.bitmap(i, L.TYPE_KV))
+            else:
+                out.append(np.concatenate([runner.bitmap(i, L.TYPE_NEWORDER), runner.bitmap(i, L.TYPE_PAYMENT)]))
+        return out
+
+    def timed(mode):
+        runner.run(mode)  # warm-up pass
+        t0 = time.perf_counter()
+        runner.run(mode)
+        return time.perf_counter() - t0, runner.times(), bitmaps()
+
+    log("C4: plain per-batch loop (default launch shapes)")
+    plain_s, plain_t, plain_bm = timed(L.RUN_PLAIN)
+    log("C4: autotuning launch shapes")
+    tuning = runner.autotune(reps=3)
+    log("C4: plain per-batch loop with the tuned launch shapes")
+    tuned_s, _, _ = timed(L.RUN_PLAIN)
+    log("C4: batched form (graphs, two streams, tuned shapes)")
+    graph_s, graph_t, graph_bm = timed(L.RUN_GRAPH)
+    same = all(np.array_equal(a, b) for a, b in zip(plain_bm, graph_bm))
+    commits = int(sum(int(np.unpackbits(b.view(np.uint8)).sum()) for b in graph_bm))
+    total = int(sizes.sum())
+    runner.close()
+    for x in keep:
+        for buf in (x if isinstance(x, list) else [x]):
+            buf.free()
+    kv.close()
+    tp.close()
+    return {
+        "value": commits / graph_s, "unit": "txn/s", "ms_per_step": graph_s * 1e3,
+        "plain_value": commits / plain_s, "plain_ms": plain_s * 1e3, "speedup_vs_plain": plain_s / graph_s,
+        "plain_tuned_value": commits / tuned_s, "plain_tuned_ms": tuned_s * 1e3,
+        "commit_sets_identical": bool(same), "committed_txns": commits, "attempted_txns": total,
+        "abort_rate": 1.0 - commits / max(total, 1),
+        "latency_ms": {
+            "what": "completion time of each batch from the start of the run (all batches arrive at once)",
+            "plain_p50": float(np.percentile(plain_t, 50)), "plain_p99": float(np.percentile(plain_t, 99)),
+            "batched_p50": float(np.percentile(graph_t, 50)), "batched_p99": float(np.percentile(graph_t, 99))},
+        "autotuner": tuning,
+        "config": {"workload": desc, "batches": nb, "kv_batches": int(is_kv.sum()), "seed": seed,
+                   "form": "batched (hipGraph per workload, concurrent streams, autotuned) vs plain loop",
+                   "inputs": "resident in HBM"},
+    }
+
+
 def measured_traffic(phase, family="kv"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if
     one exists for this kernel (profiles/kv_c2_traffic.json,
@@ -404,7 +498,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3"])
+    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3", "batched-c4"])
+    ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form variant of the default run")
     ap.add_argument("--theta", type=float, default=None,
                     help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
@@ -426,6 +522,10 @@ def main():
     variants = []
     if args.workload == "tpcc-c3":
         result = run_tpcc(args)
+    elif args.workload == "batched-c4":
+        v = run_batched_c4(args)
+        result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True,
+                  "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic"} | v
     else:
         thetas = [args.theta] if args.theta is not None else ([0.6, 0.99] if args.workload == "kv-c2" else [None])
         result = run_kv(args, thetas[0])
@@ -435,6 +535,8 @@ def main():
         if args.workload == "kv-c2" and args.theta is None:
             v = run_tpcc(args)
             variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
+            if not args.no_batched:
+                variants.append(run_batched_c4(args))
     if variants:
         result["variants"] = variants
     print(json.dumps(result))

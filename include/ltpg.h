@@ -317,11 +317,19 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
 
 /* ---- launch shape of the key-value kernels (replaces the fixed warp-role
  *      and grid constants, YCSB/0/Predefine.h:27-35) ------------------------ */
-/* execute_per_op: EXECUTE runs one lane per op instead of one lane per
- * transaction.  split_registers: a row's R and W registers live in two
- * arrays instead of one 16-byte pair.  Results are identical in every
- * combination; may change between any two epochs. */
-int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t execute_per_op, uint32_t split_registers);
+/* EXEC_PER_OP: EXECUTE runs one lane per op instead of one lane per
+ *   transaction.
+ * SPLIT_REGISTERS: a row's R and W registers live in two arrays instead of
+ *   one 16-byte pair.
+ * SMALL_CHUNKS: CHECK and WRITEBACK blocks own 64 transactions instead of
+ *   256 (more blocks, fewer dependent rounds each).
+ * Results are identical for every combination; may change between any two
+ * epochs. */
+#define LTPG_TUNE_EXEC_PER_OP 1u
+#define LTPG_TUNE_SPLIT_REGISTERS 2u
+#define LTPG_TUNE_SMALL_CHUNKS 4u
+#define LTPG_TUNE_ALL 7u
+int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags);
 
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
  *      tp.cu:24-117, YCSB/0/ycsb.cu:9-58; launched by tp.cu:154-236) ------- */
@@ -406,20 +414,25 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode);
 /* Commit bitmap of batch i (type LTPG_TYPE_KV, or LTPG_TYPE_NEWORDER /
  * LTPG_TYPE_PAYMENT for an order-entry batch) from the last run. */
 int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words);
+/* Completion time of each batch of the last run, in ms from the run's
+ * start (every batch counted as arriving at the start): host clock after the
+ * commit readback for PLAIN, the device clock after the batch's last kernel
+ * for GRAPH. */
+int ltpg_batched_times(ltpg_batched *b, double *ms, uint32_t n);
 int ltpg_batched_destroy(ltpg_batched *b);
 
 /* Autotuner.  For every size class of the runner's key-value batches
- * (floor(log2(n_txns))) it times each ltpg_kv_set_tuning combination on the
- * first batch of the class (reps epochs after one warm-up, HIP events on the
+ * (floor(log2(n_txns))) it times every LTPG_TUNE_* combination on the first
+ * batch of the class (reps epochs after one warm-up, HIP events on the
  * context's stream, write-back in copy-back mode so the tables are left as
  * they were), keeps the fastest for every batch of the class, and captures
  * the graph again.  One entry per class is written to out. */
 typedef struct ltpg_tuning_entry {
     uint32_t workload;         /* LTPG_WORKLOAD_KV                                  */
     uint32_t log2_txns;        /* the size class                                    */
-    uint32_t execute_per_op;   /* chosen                                            */
-    uint32_t split_registers;  /* chosen                                            */
-    float ms[4];               /* epoch time of (per_op, split) = 00, 01, 10, 11    */
+    uint32_t flags;            /* chosen LTPG_TUNE_* combination                    */
+    uint32_t reserved;
+    float ms[8];               /* epoch time of every combination, by flags         */
 } ltpg_tuning_entry;
 int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
                           uint32_t *n_out);

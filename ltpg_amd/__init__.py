@@ -53,6 +53,7 @@ NEWORDER_READS, PAYMENT_READS = 51, 24
 # u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
 NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
+TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_ALL = 1, 2, 4, 7
 
 
 def kv_meta(kind, field):
@@ -199,9 +200,9 @@ class Context:
         _check(lib.ltpg_kv_table_read(self.handle, _ptr(out), rows, layout), "ltpg_kv_table_read")
         return out
 
-    def kv_set_tuning(self, execute_per_op=False, split_registers=False):
-        _check(lib.ltpg_kv_set_tuning(self.handle, int(bool(execute_per_op)), int(bool(split_registers))),
-               "ltpg_kv_set_tuning")
+    def kv_set_tuning(self, flags=0):
+        """LTPG_TUNE_* launch shape of the key-value kernels (results never change)."""
+        _check(lib.ltpg_kv_set_tuning(self.handle, int(flags)), "ltpg_kv_set_tuning")
 
     def kv_fill_reference(self):
         _check(lib.ltpg_kv_table_fill_reference(self.handle), "ltpg_kv_table_fill_reference")
@@ -456,14 +457,20 @@ class Batched:
     def run(self, mode=RUN_GRAPH):
         _check(lib.ltpg_batched_run(self.handle, mode), "ltpg_batched_run")
 
+    def times(self):
+        """Completion time (ms from the run's start) of every batch of the last run."""
+        out = np.zeros(max(len(self.descs), 1), dtype=np.float64)
+        _check(lib.ltpg_batched_times(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      len(self.descs)), "ltpg_batched_times")
+        return out[:len(self.descs)]
+
     def autotune(self, reps=3):
         """Pick the key-value launch shape per size class; returns one dict
-        per class (log2 size, chosen flags, ms of the four shapes)."""
+        per class (log2 size, chosen TUNE_* flags, ms of every combination)."""
         out = (_lib.TuningEntry * 33)()
         n = ctypes.c_uint32()
         _check(lib.ltpg_batched_autotune(self.handle, reps, out, 33, ctypes.byref(n)), "ltpg_batched_autotune")
-        return [{"log2_txns": e.log2_txns, "execute_per_op": e.execute_per_op, "split_registers": e.split_registers,
-                 "ms": list(e.ms)} for e in out[:n.value]]
+        return [{"log2_txns": e.log2_txns, "flags": e.flags, "ms": list(e.ms)} for e in out[:n.value]]
 
     def bitmap(self, i, type_):
         d = self.descs[i]

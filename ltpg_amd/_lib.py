@@ -103,8 +103,8 @@ class BatchDesc(ctypes.Structure):
 class TuningEntry(ctypes.Structure):
     """struct ltpg_tuning_entry."""
 
-    _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("execute_per_op", ctypes.c_uint32),
-                ("split_registers", ctypes.c_uint32), ("ms", ctypes.c_float * 4)]
+    _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 8)]
 
 
 # (name, restype, argtypes) of every exported entry point.
@@ -125,7 +125,7 @@ SIGNATURES = [
     ("ltpg_kv_table_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     ("ltpg_kv_table_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     ("ltpg_kv_table_fill_reference", ctypes.c_int, [ctypes.c_void_p]),
-    ("ltpg_kv_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("ltpg_kv_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_kv_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KvBatch), ctypes.c_uint32]),
     ("ltpg_epoch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_epoch_reset", ctypes.c_int, [ctypes.c_void_p]),
@@ -161,6 +161,7 @@ SIGNATURES = [
     ("ltpg_batched_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_batched_bitmap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u64p, ctypes.c_uint64]),
     ("ltpg_batched_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_batched_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]),
     ("ltpg_batched_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(TuningEntry), ctypes.c_uint32, c_u32p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),

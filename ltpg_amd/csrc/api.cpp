@@ -539,14 +539,15 @@ int ltpg_kv_table_fill_reference(ltpg_ctx *ctx)
     return LTPG_OK;
 }
 
-int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t execute_per_op, uint32_t split_registers)
+int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    if (execute_per_op > 1 || split_registers > 1) LTPG_FAIL(LTPG_ERR_INVALID, "tuning flags are 0 or 1");
-    ctx->kv.exec_per_op = execute_per_op;
-    ctx->kv.split_regs = split_registers;
+    if (flags & ~LTPG_TUNE_ALL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    ctx->kv.exec_per_op = (flags & LTPG_TUNE_EXEC_PER_OP) ? 1 : 0;
+    ctx->kv.split_regs = (flags & LTPG_TUNE_SPLIT_REGISTERS) ? 1 : 0;
+    ctx->kv.small_chunks = (flags & LTPG_TUNE_SMALL_CHUNKS) ? 1 : 0;
     return LTPG_OK;
 }
 

@@ -13,6 +13,7 @@
 // workloads share no table), with one host synchronisation for the lot.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <new>
 #include <string>
 #include <vector>
@@ -30,7 +31,11 @@ struct ltpg_batched {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t final_epoch[2] = {0, 0};       // host epoch counter after a replay
-    std::vector<uint8_t> tune;              // key-value batch i: bit 0 per-op EXECUTE, bit 1 split registers
+    std::vector<uint8_t> tune;              // key-value batch i: LTPG_TUNE_* flags
+    unsigned long long *stamps = nullptr;   // graph runs: device clock after batch i; [n + k] start of graph k
+    std::vector<double> plain_ms;           // plain runs: host ms from the run's start to batch i's readback
+    uint32_t last_mode = LTPG_RUN_PLAIN;
+    bool ran = false;
 };
 
 namespace {
@@ -40,15 +45,15 @@ uint64_t words(uint64_t n) { return (n + 63) / 64; }
 int slot_of(uint32_t workload) { return workload == LTPG_WORKLOAD_KV ? 0 : 1; }
 
 // One epoch of batch i on its context: submit (device-resident), all
-// phases, archive copy of the commit bitmap(s).  No synchronisation, so it
-// can be captured.
-int enqueue_batch(ltpg_batched *b, uint32_t i)
+// phases, archive copy of the commit bitmap(s), and (captured runs) a device
+// clock stamp.  No synchronisation, so it can be captured.
+int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp)
 {
     const ltpg_batch_desc &d = b->d[i];
     ltpg_ctx *ctx = b->ctx[slot_of(d.workload)];
     int rc;
     if (d.workload == LTPG_WORKLOAD_KV) {
-        if ((rc = ltpg_kv_set_tuning(ctx, b->tune[i] & 1u, (b->tune[i] >> 1) & 1u))) return rc;
+        if ((rc = ltpg_kv_set_tuning(ctx, b->tune[i]))) return rc;
         if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
     } else {
         if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
@@ -62,6 +67,7 @@ int enqueue_batch(ltpg_batched *b, uint32_t i)
         LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i] + words(d.neworder.n), ctx->tpcc.d.bitmap_pay,
                                      words(d.payment.n), ctx->stream));
     }
+    if (stamp) LTPG_HIP_TRY(launch_stamp(b->stamps + i, ctx->stream));
     return LTPG_OK;
 }
 
@@ -77,8 +83,9 @@ int capture(ltpg_batched *b, int k)
     // every replay starts from all-ones registers and epoch 1
     ctx->epoch = 0;
     int rc = refill_registers(ctx);
+    if (!rc && launch_stamp(b->stamps + b->d.size() + k, ctx->stream) != hipSuccess) rc = LTPG_ERR_DEVICE;
     for (uint32_t i = 0; i < b->d.size() && !rc; i++)
-        if (slot_of(b->d[i].workload) == k) rc = enqueue_batch(b, i);
+        if (slot_of(b->d[i].workload) == k) rc = enqueue_batch(b, i, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     b->final_epoch[k] = ctx->epoch;
@@ -107,6 +114,7 @@ int ltpg_batched_destroy(ltpg_batched *b)
         if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
     }
     dev_free(b->archive);
+    dev_free(b->stamps);
     delete b;
     return LTPG_OK;
 }
@@ -145,6 +153,8 @@ int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_d
     b->archive_words = w;
     int rc = check_ctx(kv_ctx ? kv_ctx : tpcc_ctx);
     if (!rc) rc = dev_alloc(&b->archive, w ? w : 1, "bitmap archive");
+    if (!rc) rc = dev_alloc(&b->stamps, (uint64_t)n + 2, "batch clock stamps");
+    b->plain_ms.assign(n, 0.0);
     for (int k = 0; k < 2 && !rc; k++) {
         bool any = false;
         for (const auto &d : b->d) any |= slot_of(d.workload) == k;
@@ -168,15 +178,20 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
     if (mode == LTPG_RUN_PLAIN) {
         // the reference's loop: per batch its phases, a synchronisation, the
         // commit count back on the host
+        const auto t0 = std::chrono::steady_clock::now();
         for (uint32_t i = 0; i < b->d.size(); i++) {
             ltpg_ctx *ctx = b->ctx[slot_of(b->d[i].workload)];
-            if ((rc = enqueue_batch(b, i))) return rc;
+            if ((rc = enqueue_batch(b, i, false))) return rc;
             if ((rc = ltpg_sync(ctx))) return rc;
             uint32_t c = 0;
             if ((rc = ltpg_commit_count(ctx, b->d[i].workload == LTPG_WORKLOAD_KV ? LTPG_TYPE_KV
                                                                                   : LTPG_TYPE_PAYMENT, &c)))
                 return rc;
+            b->plain_ms[i] =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
+        b->last_mode = LTPG_RUN_PLAIN;
+        b->ran = true;
         return LTPG_OK;
     }
     if (mode != LTPG_RUN_GRAPH) LTPG_FAIL(LTPG_ERR_INVALID, "unknown run mode");
@@ -192,6 +207,8 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
         rc = ltpg_sync(b->ctx[k]);
         if (rc && !first) first = rc;
     }
+    b->last_mode = LTPG_RUN_GRAPH;
+    b->ran = true;
     return first;
 }
 
@@ -223,9 +240,9 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     for (int c = 0; c < 33 && !rc; c++) {
         if (rep[c] < 0) continue;
         const ltpg_batch_desc &d = b->d[rep[c]];
-        float ms[4] = {0, 0, 0, 0};
-        for (uint32_t cfg = 0; cfg < 4 && !rc; cfg++) {
-            rc = ltpg_kv_set_tuning(ctx, cfg >> 1, cfg & 1u);
+        float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t cfg = 0; cfg <= LTPG_TUNE_ALL && !rc; cfg++) {
+            rc = ltpg_kv_set_tuning(ctx, cfg);
             for (uint32_t k = 0; k <= reps && !rc; k++) {
                 if (k == 1) rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? LTPG_OK : LTPG_ERR_DEVICE;
                 if (!rc) rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE);
@@ -239,10 +256,9 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
         }
         if (rc) break;
         uint32_t best = 0;
-        for (uint32_t cfg = 1; cfg < 4; cfg++)
+        for (uint32_t cfg = 1; cfg <= LTPG_TUNE_ALL; cfg++)
             if (ms[cfg] < ms[best]) best = cfg;
-        // bit 0 per-op EXECUTE, bit 1 split registers
-        const uint8_t t = (uint8_t)(((best >> 1) & 1u) | ((best & 1u) << 1));
+        const uint8_t t = (uint8_t)best;
         for (uint32_t i = 0; i < b->d.size(); i++)
             if (b->d[i].workload == LTPG_WORKLOAD_KV && b->d[i].kv.n_txns &&
                 31 - __builtin_clz(b->d[i].kv.n_txns) == c)
@@ -251,9 +267,9 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
             ltpg_tuning_entry &o = out[n];
             o.workload = LTPG_WORKLOAD_KV;
             o.log2_txns = (uint32_t)c;
-            o.execute_per_op = best >> 1;
-            o.split_registers = best & 1u;
-            for (int k = 0; k < 4; k++) o.ms[k] = ms[k];
+            o.flags = best;
+            o.reserved = 0;
+            for (int k = 0; k < 8; k++) o.ms[k] = ms[k];
         }
         n++;
     }
@@ -269,6 +285,30 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     b->exec[0] = nullptr;
     b->graph[0] = nullptr;
     return capture(b, 0);
+}
+
+int ltpg_batched_times(ltpg_batched *b, double *ms, uint32_t n)
+{
+    if (!b || (!ms && n)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    if (n > b->d.size()) LTPG_FAIL(LTPG_ERR_INVALID, "n exceeds the number of batches");
+    if (!b->ran) LTPG_FAIL(LTPG_ERR_STATE, "no run yet");
+    if (b->last_mode == LTPG_RUN_PLAIN) {
+        for (uint32_t i = 0; i < n; i++) ms[i] = b->plain_ms[i];
+        return LTPG_OK;
+    }
+    ltpg_ctx *ctx = b->ctx[0] ? b->ctx[0] : b->ctx[1];
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    int khz = 0;
+    LTPG_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    if (khz <= 0) LTPG_FAIL(LTPG_ERR_DEVICE, "device reports no wall-clock rate");
+    std::vector<unsigned long long> st(b->d.size() + 2);
+    LTPG_HIP_TRY(hipMemcpy(st.data(), b->stamps, st.size() * 8, hipMemcpyDeviceToHost));
+    unsigned long long start = ~0ull;
+    for (int k = 0; k < 2; k++)
+        if (b->exec[k] && st[b->d.size() + k] < start) start = st[b->d.size() + k];
+    for (uint32_t i = 0; i < n; i++) ms[i] = (double)(st[i] - start) / (double)khz;
+    return LTPG_OK;
 }
 
 int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t *out, uint64_t n_words)

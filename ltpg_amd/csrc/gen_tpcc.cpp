@@ -15,8 +15,10 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ltpg.h"
@@ -208,20 +210,100 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
     // Arrival order: exactly n_no NewOrders among n_txns, each arrival being
     // a NewOrder with probability (NewOrders left) / (arrivals left)
     // (selection sampling), so every interleaving is equally likely.
-    uint32_t left_no = n_no, q = 0, p = 0;
+    // slot[i] = q (NewOrder) or n_txns + p (Payment) of arrival i.
+    std::vector<uint32_t> slot(n_txns);
+    {
+        uint32_t left_no = n_no, q = 0, p = 0;
+        for (uint32_t i = 0; i < n_txns; i++) {
+            const bool is_no = below(draw(seed, g->next_txn + i, F_TYPE, 0), n_txns - i) < left_no;
+            if (is_no) left_no--;
+            slot[i] = is_no ? q++ : n_txns + p++;
+        }
+    }
+    // Every field but the per-warehouse sequence numbers is a pure function
+    // of (seed, arrival): drawn in parallel.
+    auto fields = [&](uint32_t i0, uint32_t i1) {
+        for (uint32_t i = i0; i < i1; i++) {
+            const uint64_t t = g->next_txn + i;
+            if (slot[i] < n_txns) {
+                const uint32_t q = slot[i];
+                const uint32_t w = uniform(draw(seed, t, F_W, 0), 0, W - 1);
+                NW[q] = w;
+                ND[q] = uniform(draw(seed, t, F_D, 0), 0, 9);
+                NC[q] = uniform(draw(seed, t, F_C, 0), 0, 2999);
+                const uint32_t cnt = uniform(draw(seed, t, F_CNT, 0), c.min_lines, c.max_lines);
+                NCNT[q] = cnt;
+                for (uint32_t l = 0; l < LTPG_TPCC_MAX_LINES; l++) {
+                    const uint64_t at = (uint64_t)l * no_cap + q;
+                    if (l >= cnt) {
+                        NI[at] = NSW[at] = NQ[at] = 0;
+                        continue;
+                    }
+                    // distinct items within the order (Query.h:103-113): redraw
+                    // with the next attempt number until unused
+                    uint32_t item = 0;
+                    for (uint32_t k = 0;; k++) {
+                        if (c.nurand_items) {
+                            const uint32_t a = uniform(draw(seed, t, F_ITEM_A, l * 4096 + k), 0, 8191);
+                            const uint32_t b = uniform(draw(seed, t, F_ITEM_B, l * 4096 + k), 1, 100000);
+                            item = (a | b) % 100000;  // NURand(8191, 1, 100000) - 1
+                        } else {
+                            item = uniform(draw(seed, t, F_ITEM_A, l * 4096 + k), 0, 99999);
+                        }
+                        bool dup = false;
+                        for (uint32_t j = 0; j < l; j++) dup |= NI[(uint64_t)j * no_cap + q] == item;
+                        if (!dup) break;
+                    }
+                    NI[at] = item;
+                    uint32_t sw = w;
+                    if (W > 1 && below(draw(seed, t, F_REMOTE, l), 100) < c.remote_supply_pct)
+                        sw = other_warehouse(draw(seed, t, F_RW, l), w, W);
+                    NSW[at] = sw;
+                    NQ[at] = uniform(draw(seed, t, F_QTY, l), 1, c.max_quantity);
+                }
+                NT[q] = tid_base + i;
+            } else {
+                const uint32_t p = slot[i] - n_txns;
+                const uint32_t w = uniform(draw(seed, t, F_W, 0), 0, W - 1);
+                PW[p] = w;
+                PD[p] = uniform(draw(seed, t, F_D, 0), 0, 9);
+                if (below(draw(seed, t, F_ISNAME, 0), 100) < c.byname_pct) {
+                    PN[p] = 1;
+                    PC[p] = 0;
+                    PL[p] = (uniform(draw(seed, t, F_NAME, 0), 0, 19) << 8) + uniform(draw(seed, t, F_ORD, 0), 0, 149);
+                } else {
+                    PN[p] = 0;
+                    PC[p] = uniform(draw(seed, t, F_C, 0), 0, 2999);
+                    PL[p] = 20;
+                }
+                uint32_t cw = w;
+                if (W > 1 && below(draw(seed, t, F_CREMOTE, 0), 100) < c.remote_customer_pct)
+                    cw = other_warehouse(draw(seed, t, F_CW, 0), w, W);
+                PCW[p] = cw;
+                PCD[p] = uniform(draw(seed, t, F_CD, 0), 0, 9);
+                PA[p] = uniform(draw(seed, t, F_AMOUNT, 0), 1, c.max_amount);
+                PT[p] = tid_base + i;
+            }
+        }
+    };
+    const uint32_t nt = n_txns < 4096 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        fields(0, n_txns);
+    } else {
+        std::vector<std::thread> th;
+        const uint32_t per = (n_txns + nt - 1) / nt;
+        for (uint32_t k = 0; k < nt; k++) {
+            const uint32_t i0 = std::min(n_txns, k * per), i1 = std::min(n_txns, i0 + per);
+            th.emplace_back(fields, i0, i1);
+        }
+        for (auto &x : th) x.join();
+    }
+    // Per-warehouse sequence numbers in arrival order, wrapping as
+    // Query.h:54-88,157-166 do; an order's lines stay inside its
+    // warehouse's ORDERLINE range.
     for (uint32_t i = 0; i < n_txns; i++) {
-        const uint64_t t = g->next_txn + i;
-        const bool is_no = below(draw(seed, t, F_TYPE, 0), n_txns - i) < left_no;
-        if (is_no) {
-            left_no--;
-            const uint32_t w = uniform(draw(seed, t, F_W, 0), 0, W - 1);
-            NW[q] = w;
-            ND[q] = uniform(draw(seed, t, F_D, 0), 0, 9);
-            NC[q] = uniform(draw(seed, t, F_C, 0), 0, 2999);
-            const uint32_t cnt = uniform(draw(seed, t, F_CNT, 0), c.min_lines, c.max_lines);
-            NCNT[q] = cnt;
-            // per-warehouse sequence numbers, wrapping as Query.h:54-88 does;
-            // an order's lines stay inside its warehouse's ORDERLINE range
+        if (slot[i] < n_txns) {
+            const uint32_t q = slot[i], w = NW[q], cnt = NCNT[q];
             NNO[q] = g->noid[w];
             g->noid[w] = g->noid[w] + 1 >= kOrdersPerWarehouse ? 0 : g->noid[w] + 1;
             NOI[q] = g->oid[w];
@@ -230,59 +312,10 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
             NOL[q] = g->oolid[w];
             g->oolid[w] += cnt;
             if (g->oolid[w] >= kLinesPerWarehouse) g->oolid[w] = 0;
-            for (uint32_t l = 0; l < LTPG_TPCC_MAX_LINES; l++) {
-                const uint64_t at = (uint64_t)l * no_cap + q;
-                if (l >= cnt) {
-                    NI[at] = NSW[at] = NQ[at] = 0;
-                    continue;
-                }
-                // distinct items within the order (Query.h:103-113): redraw
-                // with the next attempt number until unused
-                uint32_t item = 0;
-                for (uint32_t k = 0;; k++) {
-                    if (c.nurand_items) {
-                        const uint32_t a = uniform(draw(seed, t, F_ITEM_A + 0, l * 4096 + k), 0, 8191);
-                        const uint32_t b = uniform(draw(seed, t, F_ITEM_B + 0, l * 4096 + k), 1, 100000);
-                        item = (a | b) % 100000;  // NURand(8191, 1, 100000) - 1
-                    } else {
-                        item = uniform(draw(seed, t, F_ITEM_A, l * 4096 + k), 0, 99999);
-                    }
-                    bool dup = false;
-                    for (uint32_t j = 0; j < l; j++) dup |= NI[(uint64_t)j * no_cap + q] == item;
-                    if (!dup) break;
-                }
-                NI[at] = item;
-                uint32_t sw = w;
-                if (W > 1 && below(draw(seed, t, F_REMOTE, l), 100) < c.remote_supply_pct)
-                    sw = other_warehouse(draw(seed, t, F_RW, l), w, W);
-                NSW[at] = sw;
-                NQ[at] = uniform(draw(seed, t, F_QTY, l), 1, c.max_quantity);
-            }
-            NT[q] = tid_base + i;
-            q++;
         } else {
-            const uint32_t w = uniform(draw(seed, t, F_W, 0), 0, W - 1);
-            PW[p] = w;
-            PD[p] = uniform(draw(seed, t, F_D, 0), 0, 9);
-            if (below(draw(seed, t, F_ISNAME, 0), 100) < c.byname_pct) {
-                PN[p] = 1;
-                PC[p] = 0;
-                PL[p] = (uniform(draw(seed, t, F_NAME, 0), 0, 19) << 8) + uniform(draw(seed, t, F_ORD, 0), 0, 149);
-            } else {
-                PN[p] = 0;
-                PC[p] = uniform(draw(seed, t, F_C, 0), 0, 2999);
-                PL[p] = 20;
-            }
-            uint32_t cw = w;
-            if (W > 1 && below(draw(seed, t, F_CREMOTE, 0), 100) < c.remote_customer_pct)
-                cw = other_warehouse(draw(seed, t, F_CW, 0), w, W);
-            PCW[p] = cw;
-            PCD[p] = uniform(draw(seed, t, F_CD, 0), 0, 9);
-            PA[p] = uniform(draw(seed, t, F_AMOUNT, 0), 1, c.max_amount);
+            const uint32_t p = slot[i] - n_txns, w = PW[p];
             PH[p] = g->hid[w];
             g->hid[w] = g->hid[w] + 1 >= kHistoryPerWarehouse ? 0 : g->hid[w] + 1;
-            PT[p] = tid_base + i;
-            p++;
         }
     }
     g->next_txn += n_txns;

@@ -182,6 +182,9 @@ __global__ void copy_u64_kernel(uint64_t *__restrict__ dst, const uint64_t *__re
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Device wall-clock stamp (constant-rate counter), one lane.
+__global__ void stamp_kernel(unsigned long long *out) { out[threadIdx.x] = wall_clock64(); }
+
 __global__ void fill_u8_kernel(uint8_t *p, uint64_t n, uint8_t v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -362,7 +365,10 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     }
 }
 
-// Block-local compaction.  A block owns kChunk consecutive transactions;
+// Block-local compaction.  A block owns CH consecutive transactions (CH =
+// 256, or 64 for batches too small to fill the chip: a block walks its
+// list in rounds of 4 * 64 / G transactions, so fewer per block means
+// more blocks and fewer dependent rounds each);
 // the transactions that still have work (alive ones for CHECK, committed
 // ones for WRITEBACK) are gathered into an LDS list, then processed G lanes
 // per transaction (one lane per slot), so lanes stay busy whatever the
@@ -427,7 +433,7 @@ __device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t sr
 // gathers its read fields right away: no write-back has run yet, so the
 // table is the snapshot (read_1 / __READ, YCSB/0/Execute.h:27-37).  Commit
 // bits collect in LDS and leave as whole bitmap words.
-template <int G>
+template <int G, int CH>
 __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
@@ -436,17 +442,17 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
     DeviceCounters *__restrict__ ctr)
 {
-    __shared__ uint32_t s_list[kChunk];
-    __shared__ unsigned long long s_bits[kChunk / 64];
+    __shared__ uint32_t s_list[CH];
+    __shared__ unsigned long long s_bits[CH / 64];
     __shared__ uint32_t s_n, s_commits;
     const int lane = lane_id();
-    const uint32_t chunk0 = blockIdx.x * kChunk;
+    const uint32_t chunk0 = blockIdx.x * CH;
     if (threadIdx.x == 0) s_n = s_commits = 0;
-    if (threadIdx.x < kChunk / 64) s_bits[threadIdx.x] = 0;
+    if (threadIdx.x < CH / 64) s_bits[threadIdx.x] = 0;
     __syncthreads();
     {
         const uint32_t t = chunk0 + threadIdx.x;
-        list_append(t < B && doomed[t] != epoch, t, s_list, &s_n);
+        list_append(threadIdx.x < CH && t < B && doomed[t] != epoch, t, s_list, &s_n);
     }
     __syncthreads();
     const uint32_t n = s_n;
@@ -478,10 +484,10 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     if (commits) atomicAdd(&s_commits, commits);
     __syncthreads();
     const uint32_t nwords = (B + 63) / 64;
-    if (threadIdx.x < kChunk / 64 && chunk0 / 64 + threadIdx.x < nwords)
+    if (threadIdx.x < CH / 64 && chunk0 / 64 + threadIdx.x < nwords)
         bitmap[chunk0 / 64 + threadIdx.x] = s_bits[threadIdx.x];
     if (threadIdx.x == 0) {
-        const uint32_t live_n = (B - chunk0) < kChunk ? (B - chunk0) : kChunk;
+        const uint32_t live_n = (B - chunk0) < (uint32_t)CH ? (B - chunk0) : (uint32_t)CH;
         add_counts(ctr, blockIdx.x % kCounterStripes, LTPG_TYPE_KV, s_commits, live_n);
     }
 }
@@ -495,24 +501,24 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
 // in slot order writes word + count * delta, the others nothing -- the same
 // result as applying them one by one, and it lets every store of a round go
 // out without ordering between them.
-template <int G>
+template <int G, int CH>
 __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     uint32_t fw, uint32_t fsw, uint32_t *__restrict__ table, const uint64_t *__restrict__ bitmap,
     uint32_t values)
 {
-    __shared__ uint32_t s_list[kChunk];
+    __shared__ uint32_t s_list[CH];
     __shared__ uint32_t s_n;
     const int lane = lane_id();
-    const uint32_t chunk0 = blockIdx.x * kChunk;
+    const uint32_t chunk0 = blockIdx.x * CH;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     {
         const uint32_t t = chunk0 + threadIdx.x;
         // t < B also guards a bitmap word left by an earlier, larger batch
         // when this phase runs without a fresh CHECK
-        list_append(t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull), t, s_list, &s_n);
+        list_append(threadIdx.x < CH && t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull), t, s_list, &s_n);
     }
     __syncthreads();
     const uint32_t n = s_n;
@@ -608,6 +614,12 @@ hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipSt
     return hipGetLastError();
 }
 
+hipError_t launch_stamp(unsigned long long *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_u8_kernel, grid_stride(n), dim3(kBlock), 0, s, p, n, v);
@@ -681,7 +693,7 @@ int group_for(uint32_t S)
     default: CALL(64); break;   \
     }
 
-inline dim3 grid_chunks(uint32_t B) { return dim3((B + kChunk - 1) / kChunk); }
+inline dim3 grid_chunks(uint32_t B, uint32_t ch) { return dim3((B + ch - 1) / ch); }
 
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s)
@@ -693,13 +705,19 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
                            e.tag, kv.bitmap, kv.conflicts, ctr);
         return hipGetLastError();
     }
-#define CALL(GV)                                                                                     \
-    hipLaunchKernelGGL(kv_check_fast_kernel<GV>, grid_chunks(b.n_txns), dim3(kBlock), 0, s, b.keys, \
+#define CALL1(GV, CHV)                                                                               \
+    hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,     \
                        kv.field_stride, kv.regs, reg_layout(kv), e.tag, kv.doomed, e.epoch, kv.table, \
                        kv.read_values, kv.bitmap, ctr)
+#define CALL(GV)                    \
+    if (kv.small_chunks)            \
+        CALL1(GV, 64);              \
+    else                            \
+        CALL1(GV, kChunk)
     LTPG_G_SWITCH(group_for(b.n_slots), CALL)
 #undef CALL
+#undef CALL1
     return hipGetLastError();
 }
 
@@ -720,12 +738,18 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
                                hipStream_t s)
 {
     if (b.n_txns == 0) return hipSuccess;
-#define CALL(GV)                                                                                      \
-    hipLaunchKernelGGL(kv_writeback_kernel<GV>, grid_chunks(b.n_txns), dim3(kBlock), 0, s, b.keys,   \
-                       b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,      \
+#define CALL1(GV, CHV)                                                                                      \
+    hipLaunchKernelGGL((kv_writeback_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
+                       b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,             \
                        kv.field_stride, kv.table, kv.bitmap, values)
+#define CALL(GV)         \
+    if (kv.small_chunks) \
+        CALL1(GV, 64);   \
+    else                 \
+        CALL1(GV, kChunk)
     LTPG_G_SWITCH(group_for(b.n_slots), CALL)
 #undef CALL
+#undef CALL1
     return hipGetLastError();
 }
 

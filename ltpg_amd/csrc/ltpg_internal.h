@@ -78,9 +78,11 @@ struct KvState {
     KvBatchDev cur;
     bool has_batch = false;
     // tuning (ltpg_kv_set_tuning): EXECUTE one lane per op instead of per
-    // transaction; R and W registers in two arrays instead of pairs
+    // transaction; R and W registers in two arrays instead of pairs; CHECK
+    // and WRITEBACK blocks of 64 transactions instead of 256
     uint32_t exec_per_op = 0;
     uint32_t split_regs = 0;
+    uint32_t small_chunks = 0;
 };
 
 // ---- order-entry state -------------------------------------------------
@@ -202,6 +204,7 @@ hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s);
 hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipStream_t s);
+hipError_t launch_stamp(unsigned long long *out, hipStream_t s);
 
 // Registers back to "no registrant" (all-ones), doomed flags and NEWORDER
 // inserters to 0, on the context's stream (api.cpp).  Epoch-tag wrap and the
