@@ -437,6 +437,69 @@ typedef struct ltpg_tuning_entry {
 int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
                           uint32_t *n_out);
 
+/* ---- served form: a request loop over resident tables (replaces the
+ *      producer/consumer threads around the tp loop, TPCC-allopt-final/
+ *      Query.h:640-725, tp.cu:136-261, with continuous admission) ---------- */
+typedef struct ltpg_serve_config {
+    uint32_t size;            /* sizeof(ltpg_serve_config)                      */
+    uint32_t slots;           /* epochs in flight per workload (1..4; 2 = the
+                                 next epoch is packed and copied while one runs) */
+    uint32_t kv_ops;          /* op slots per key-value request (<= kv_max_slots) */
+    uint32_t max_epoch_txns;  /* cap on transactions per epoch (0 = context max) */
+    uint64_t hbm_budget;      /* bytes of per-epoch access-set buffers (batch
+                                 slots + read values) admitted per slot; 0 = no cap */
+    uint32_t keep_reads;      /* 1: keep committed requests' read values        */
+    uint32_t record_epochs;   /* 1: keep every epoch's member list (tests)       */
+} ltpg_serve_config;
+
+typedef struct ltpg_serve_done {
+    uint64_t id;              /* request id (submission order, both workloads)   */
+    uint32_t type;            /* LTPG_TYPE_KV / NEWORDER / PAYMENT               */
+    uint32_t tid;             /* priority, assigned at arrival, kept on requeue  */
+    uint32_t attempts;        /* epochs the request took part in                */
+    uint32_t reserved;
+    uint64_t epoch;           /* serving epoch (per workload) it committed in    */
+    uint64_t arrival_ns;      /* steady clock at submission                     */
+    uint64_t commit_ns;       /* steady clock when its epoch's result was read  */
+} ltpg_serve_done;
+
+typedef struct ltpg_serve_stats {
+    uint64_t committed[3];    /* by type                                        */
+    uint64_t attempts[3];     /* transaction-epochs by type                     */
+    uint64_t epochs[2];       /* key-value, order-entry                         */
+    uint64_t max_epoch_txns[2];
+} ltpg_serve_stats;
+
+typedef struct ltpg_server ltpg_server;
+
+void ltpg_serve_config_init(ltpg_serve_config *c);
+/* Start a serving thread over kv_ctx and / or tpcc_ctx (either may be NULL).
+ * While it runs, the contexts belong to the server.  LTPG_ERR_BUDGET when
+ * the budget admits no transaction. */
+int ltpg_serve_start(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_serve_config *cfg, ltpg_server **out);
+/* n key-value requests, request-major: op k of request i at
+ * keys / meta[i * kv_ops + k] (LTPG_KV_META; NONE pads). */
+int ltpg_serve_submit_kv(ltpg_server *sv, uint32_t n, const uint64_t *keys, const uint32_t *meta,
+                         uint64_t *first_id);
+/* Order-entry requests in the batch layouts (TID fields are ignored: the
+ * server assigns priorities in arrival order).  NewOrders get ids
+ * first_id .. first_id + n_no - 1, then the Payments. */
+int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *neworder,
+                           const ltpg_payment_batch *payment, uint64_t *first_id);
+/* Up to max completed (committed) requests, oldest first. */
+int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out);
+/* Read values of a committed request (keep_reads): key-value: the field
+ * words of its read ops in op order; NewOrder 51 / Payment 24 values as
+ * ltpg_tpcc_read_values lays them out for one transaction. */
+int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words);
+/* Request ids of serving epoch e of a workload, in batch order (record_epochs). */
+int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *ids, uint32_t max, uint32_t *n);
+/* Block until every submitted request has committed. */
+int ltpg_serve_drain(ltpg_server *sv);
+int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st);
+/* Drain, stop the thread and free the server (the contexts stay). */
+int ltpg_serve_stop(ltpg_server *sv);
+
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:
  *      17-121, YCSB/0/Random.h:12,52) ----------------------------------- */

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import BatchDesc, Config, KvBatch, KvGen, NeworderBatch, PaymentBatch, TpccGenConfig, lib
+from ._lib import BatchDesc, Config, ServeConfig, ServeDone, ServeStats, KvBatch, KvGen, NeworderBatch, PaymentBatch, TpccGenConfig, lib
 
 OK = 0
 ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_RANGE, ERR_BUDGET, ERR_UNSUPPORTED = (
@@ -506,6 +506,85 @@ def tpcc_desc(dev):
     d.workload = WORKLOAD_TPCC
     d.neworder, d.payment = dev
     return d
+
+
+class Server:
+    """ltpg_serve_*: continuous admission of arriving requests into epochs
+    over resident tables; aborted requests re-queue with their TID."""
+
+    def __init__(self, kv_ctx=None, tpcc_ctx=None, **kw):
+        cfg = ServeConfig()
+        lib.ltpg_serve_config_init(ctypes.byref(cfg))
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise AttributeError(f"ltpg_serve_config has no field {k}")
+            setattr(cfg, k, v)
+        self.config = cfg
+        h = ctypes.c_void_p()
+        _check(lib.ltpg_serve_start(kv_ctx.handle if kv_ctx else None, tpcc_ctx.handle if tpcc_ctx else None,
+                                    ctypes.byref(cfg), ctypes.byref(h)), "ltpg_serve_start")
+        self.handle = h
+
+    def submit_kv(self, keys, meta):
+        """Requests request-major: keys/meta of shape (n, kv_ops); returns the first id."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        meta = np.ascontiguousarray(meta, dtype=np.uint32)
+        n = keys.shape[0]
+        first = ctypes.c_uint64()
+        _check(lib.ltpg_serve_submit_kv(self.handle, n, keys.ctypes.data_as(_lib.c_u64p),
+                                        meta.ctypes.data_as(_lib.c_u32p), ctypes.byref(first)), "ltpg_serve_submit_kv")
+        return first.value
+
+    def submit_tpcc(self, batch):
+        """Requests from a TpccBatch (NewOrders get the first ids, then Payments)."""
+        nb = neworder_struct(batch.no, batch.no_cap, batch.n_no)
+        pb = payment_struct(batch.pay, batch.pay_cap, batch.n_pay)
+        first = ctypes.c_uint64()
+        _check(lib.ltpg_serve_submit_tpcc(self.handle, ctypes.byref(nb), ctypes.byref(pb), ctypes.byref(first)),
+               "ltpg_serve_submit_tpcc")
+        return first.value
+
+    def poll(self, max_n=1 << 16):
+        out = (ServeDone * max_n)()
+        n = ctypes.c_uint32()
+        _check(lib.ltpg_serve_poll(self.handle, out, max_n, ctypes.byref(n)), "ltpg_serve_poll")
+        return [{"id": d.id, "type": d.type, "tid": d.tid, "attempts": d.attempts, "epoch": d.epoch,
+                 "arrival_ns": d.arrival_ns, "commit_ns": d.commit_ns} for d in out[:n.value]]
+
+    def read_values(self, rid, max_words=1 << 16):
+        out = np.zeros(max_words, dtype=np.uint32)
+        n = ctypes.c_uint32()
+        _check(lib.ltpg_serve_read_values(self.handle, rid, out.ctypes.data_as(_lib.c_u32p), max_words,
+                                          ctypes.byref(n)), "ltpg_serve_read_values")
+        return out[:n.value]
+
+    def epoch(self, workload, e, max_n=1 << 24):
+        n = ctypes.c_uint32()
+        _check(lib.ltpg_serve_epoch(self.handle, workload, e, None, 0, ctypes.byref(n)), "ltpg_serve_epoch")
+        ids = np.zeros(max(n.value, 1), dtype=np.uint64)
+        _check(lib.ltpg_serve_epoch(self.handle, workload, e, ids.ctypes.data_as(_lib.c_u64p), n.value,
+                                    ctypes.byref(n)), "ltpg_serve_epoch")
+        return ids[:n.value]
+
+    def drain(self):
+        _check(lib.ltpg_serve_drain(self.handle), "ltpg_serve_drain")
+
+    def stats(self):
+        st = ServeStats()
+        _check(lib.ltpg_serve_stats_get(self.handle, ctypes.byref(st)), "ltpg_serve_stats_get")
+        return {"committed": list(st.committed), "attempts": list(st.attempts), "epochs": list(st.epochs),
+                "max_epoch_txns": list(st.max_epoch_txns)}
+
+    def stop(self):
+        if self.handle:
+            h, self.handle = self.handle, None
+            _check(lib.ltpg_serve_stop(h), "ltpg_serve_stop")
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
 
 
 def kv_generate(seed=0x4C545047, rows=1_000_000, fields=10, n_txns=65536, n_slots=10, min_ops=None,

@@ -107,6 +107,29 @@ class TuningEntry(ctypes.Structure):
                 ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 8)]
 
 
+class ServeConfig(ctypes.Structure):
+    """struct ltpg_serve_config."""
+
+    _fields_ = [("size", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("kv_ops", ctypes.c_uint32),
+                ("max_epoch_txns", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64), ("keep_reads", ctypes.c_uint32),
+                ("record_epochs", ctypes.c_uint32)]
+
+
+class ServeDone(ctypes.Structure):
+    """struct ltpg_serve_done."""
+
+    _fields_ = [("id", ctypes.c_uint64), ("type", ctypes.c_uint32), ("tid", ctypes.c_uint32),
+                ("attempts", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("epoch", ctypes.c_uint64),
+                ("arrival_ns", ctypes.c_uint64), ("commit_ns", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    """struct ltpg_serve_stats."""
+
+    _fields_ = [("committed", ctypes.c_uint64 * 3), ("attempts", ctypes.c_uint64 * 3), ("epochs", ctypes.c_uint64 * 2),
+                ("max_epoch_txns", ctypes.c_uint64 * 2)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
@@ -164,6 +187,19 @@ SIGNATURES = [
     ("ltpg_batched_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]),
     ("ltpg_batched_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(TuningEntry), ctypes.c_uint32, c_u32p]),
+    ("ltpg_serve_config_init", None, [ctypes.POINTER(ServeConfig)]),
+    ("ltpg_serve_start", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ServeConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    ("ltpg_serve_submit_kv", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u32p, c_u64p]),
+    ("ltpg_serve_submit_tpcc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), c_u64p]),
+    ("ltpg_serve_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeDone), ctypes.c_uint32, c_u32p]),
+    ("ltpg_serve_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32p, ctypes.c_uint32, c_u32p]),
+    ("ltpg_serve_epoch", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, c_u64p, ctypes.c_uint32, c_u32p]),
+    ("ltpg_serve_drain", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_serve_stats_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeStats)]),
+    ("ltpg_serve_stop", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
     ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
 ]

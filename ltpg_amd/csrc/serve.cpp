@@ -1,0 +1,622 @@
+// serve.cpp - the served form: a request loop over resident tables.
+//
+// The reference feeds its epoch loop from generator threads that fill
+// pre-allocated query batches, one epoch at a time (LTPG/TPCC-allopt-final/
+// Query.h:640-725, tp.cu:136-261).  Here requests arrive individually
+// (submission = arrival) and a serving thread forms epochs continuously:
+//
+//   admission  pending requests of a workload -- re-queued aborts first, in
+//              priority (TID) order, then new arrivals in arrival order -- fill
+//              the next epoch up to the per-epoch transaction budget, derived
+//              from the HBM budget of the access-set buffers (batch slot and
+//              read values per transaction);
+//   overlap    each workload has `slots` epochs in flight: while one runs on
+//              the context's stream, the next is packed into pinned staging
+//              and copied on a copy stream; key-value and order-entry epochs
+//              run on their own contexts' streams at the same time;
+//   requeue    an aborted transaction keeps the TID given at its arrival and
+//              joins the next epoch formed after its result is read, ahead of
+//              every later arrival, so it commits after finitely many epochs;
+//   latency    commit time (result read on the host) minus arrival time.
+//
+// Tables and registers stay resident in the contexts; nothing is reset
+// between epochs (epoch-tagged registers).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <queue>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ltpg_internal.h"
+
+using namespace ltpg;
+
+namespace {
+
+uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+uint64_t words(uint64_t n) { return (n + 63) / 64; }
+
+constexpr uint32_t kNoFields = 52;  // NewOrder request: 7 head fields + 3 x 15 line fields
+constexpr uint32_t kPayFields = 9;
+
+struct Req {
+    uint64_t id;
+    uint32_t type;      // LTPG_TYPE_*
+    uint32_t tid;
+    uint32_t attempts;
+    uint64_t arrival;
+    uint64_t data;      // index into the workload's request store
+};
+
+struct ByTid {
+    const std::vector<Req> *reqs;
+    bool operator()(uint64_t a, uint64_t b) const { return (*reqs)[a].tid > (*reqs)[b].tid; }
+};
+
+template <typename T>
+int host_alloc(T **p, uint64_t count)
+{
+    *p = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void **>(p), std::max<uint64_t>(count, 1) * sizeof(T), 0) != hipSuccess) {
+        *p = nullptr;
+        LTPG_FAIL(LTPG_ERR_NOMEM, "pinned host allocation failed");
+    }
+    return LTPG_OK;
+}
+
+template <typename T>
+void host_free(T *&p)
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+}
+
+// One epoch in flight (or a free slot).
+struct Flight {
+    bool busy = false;
+    uint64_t epoch = 0;
+    std::vector<uint64_t> members;  // request indices, batch order (NewOrders, then Payments)
+    uint32_t n_no = 0, n_pay = 0;
+    hipEvent_t copied = nullptr, done = nullptr;
+    bool used = false;
+    // key-value staging (pinned) and device slot
+    uint64_t *h_keys = nullptr, *d_keys = nullptr;
+    uint32_t *h_meta = nullptr, *d_meta = nullptr, *h_tids = nullptr, *d_tids = nullptr;
+    // order-entry staging (pinned, packed query blocks) and device slot
+    uint32_t *h_no = nullptr, *d_no = nullptr, *h_pay = nullptr, *d_pay = nullptr;
+    // results (pinned)
+    uint64_t *h_bits = nullptr, *h_bits2 = nullptr;
+    uint32_t *h_reads = nullptr, *h_reads2 = nullptr;
+};
+
+struct Lane {                        // one workload
+    ltpg_ctx *ctx = nullptr;
+    hipStream_t copy = nullptr;
+    uint32_t cap = 0;                // transactions per epoch
+    std::deque<uint64_t> fresh;      // arrival order
+    std::priority_queue<uint64_t, std::vector<uint64_t>, ByTid> requeue;
+    std::vector<Flight> flights;
+    uint64_t epochs = 0;
+    uint32_t next_tid = 0;
+    uint64_t max_epoch = 0;
+    std::vector<std::vector<uint64_t>> log;  // record_epochs: request ids per epoch
+    explicit Lane(const std::vector<Req> *r) : requeue(ByTid{r}) {}
+};
+
+}  // namespace
+
+struct ltpg_server {
+    ltpg_serve_config cfg;
+    std::vector<Req> reqs;
+    std::vector<uint64_t> kv_keys;   // kv_ops per request
+    std::vector<uint32_t> kv_meta;
+    std::vector<uint32_t> no_data;   // kNoFields per NewOrder
+    std::vector<uint32_t> pay_data;  // kPayFields per Payment
+    Lane lane[2];                    // [0] key-value, [1] order-entry
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<ltpg_serve_done> done;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> reads;
+    uint64_t committed[3] = {0, 0, 0}, attempts[3] = {0, 0, 0};
+    uint64_t outstanding = 0;        // submitted, not yet committed
+    bool stop = false;
+    int error = LTPG_OK;
+    std::string error_msg;
+    std::thread th;
+    ltpg_server() : lane{Lane(&reqs), Lane(&reqs)} {}
+};
+
+namespace {
+
+void free_flight(Flight &f)
+{
+    if (f.copied) (void)hipEventDestroy(f.copied);
+    if (f.done) (void)hipEventDestroy(f.done);
+    host_free(f.h_keys);
+    host_free(f.h_meta);
+    host_free(f.h_tids);
+    host_free(f.h_no);
+    host_free(f.h_pay);
+    host_free(f.h_bits);
+    host_free(f.h_bits2);
+    host_free(f.h_reads);
+    host_free(f.h_reads2);
+    dev_free(f.d_keys);
+    dev_free(f.d_meta);
+    dev_free(f.d_tids);
+    dev_free(f.d_no);
+    dev_free(f.d_pay);
+}
+
+int alloc_flight(ltpg_server *sv, int w, Flight &f)
+{
+    const uint64_t C = sv->lane[w].cap;
+    int rc;
+    LTPG_HIP_TRY(hipEventCreateWithFlags(&f.copied, hipEventDisableTiming));
+    LTPG_HIP_TRY(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+    if (w == 0) {
+        const uint64_t S = sv->cfg.kv_ops;
+        const uint64_t fw = sv->lane[0].ctx->kv.field_words;
+        if ((rc = host_alloc(&f.h_keys, S * C)) || (rc = host_alloc(&f.h_meta, S * C)) ||
+            (rc = host_alloc(&f.h_tids, C)) || (rc = host_alloc(&f.h_bits, words(C))))
+            return rc;
+        if (sv->cfg.keep_reads && (rc = host_alloc(&f.h_reads, S * C * fw))) return rc;
+        if ((rc = dev_alloc(&f.d_keys, S * C, "serving key slot")) ||
+            (rc = dev_alloc(&f.d_meta, S * C, "serving meta slot")) ||
+            (rc = dev_alloc(&f.d_tids, C, "serving tid slot")))
+            return rc;
+    } else {
+        if ((rc = host_alloc(&f.h_no, 53 * C)) || (rc = host_alloc(&f.h_pay, 10 * C)) ||
+            (rc = host_alloc(&f.h_bits, words(C))) || (rc = host_alloc(&f.h_bits2, words(C))))
+            return rc;
+        if (sv->cfg.keep_reads &&
+            ((rc = host_alloc(&f.h_reads, (uint64_t)kNoReads * C)) || (rc = host_alloc(&f.h_reads2, (uint64_t)kPayReads * C))))
+            return rc;
+        if ((rc = dev_alloc(&f.d_no, 53 * C, "serving NewOrder slot")) ||
+            (rc = dev_alloc(&f.d_pay, 10 * C, "serving Payment slot")))
+            return rc;
+    }
+    return LTPG_OK;
+}
+
+// Take the next epoch's members (lock held): re-queued first, by TID, then
+// new arrivals.
+void take_members(ltpg_server *sv, int w, Flight &f)
+{
+    Lane &L = sv->lane[w];
+    f.members.clear();
+    while (f.members.size() < L.cap && !L.requeue.empty()) {
+        f.members.push_back(L.requeue.top());
+        L.requeue.pop();
+    }
+    while (f.members.size() < L.cap && !L.fresh.empty()) {
+        f.members.push_back(L.fresh.front());
+        L.fresh.pop_front();
+    }
+    if (w == 1) {  // NewOrders first, then Payments, each in member order
+        std::stable_partition(f.members.begin(), f.members.end(),
+                              [&](uint64_t r) { return sv->reqs[r].type == LTPG_TYPE_NEWORDER; });
+        f.n_no = 0;
+        for (uint64_t r : f.members) f.n_no += sv->reqs[r].type == LTPG_TYPE_NEWORDER;
+        f.n_pay = (uint32_t)f.members.size() - f.n_no;
+    }
+}
+
+// Pack, copy and launch one epoch (lock held: the request store may grow
+// under concurrent submissions).
+int launch(ltpg_server *sv, int w, Flight &f)
+{
+    Lane &L = sv->lane[w];
+    ltpg_ctx *ctx = L.ctx;
+    const uint64_t C = L.cap;
+    int rc;
+    if (f.used) LTPG_HIP_TRY(hipStreamWaitEvent(L.copy, f.done, 0));
+    f.used = true;
+    if (w == 0) {
+        const uint32_t S = sv->cfg.kv_ops, B = (uint32_t)f.members.size();
+        for (uint32_t t = 0; t < B; t++) {
+            const Req &r = sv->reqs[f.members[t]];
+            f.h_tids[t] = r.tid;
+            for (uint32_t s = 0; s < S; s++) {
+                f.h_keys[(uint64_t)s * B + t] = sv->kv_keys[r.data * S + s];
+                f.h_meta[(uint64_t)s * B + t] = sv->kv_meta[r.data * S + s];
+            }
+        }
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_keys, f.h_keys, (uint64_t)S * B * 8, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_meta, f.h_meta, (uint64_t)S * B * 4, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_tids, f.h_tids, (uint64_t)B * 4, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
+        LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
+        ltpg_kv_batch b = {B, S, f.d_keys, f.d_meta, f.d_tids};
+        if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE))) return rc;
+        if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+        LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->kv.bitmap, words(B) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (sv->cfg.keep_reads)
+            LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values, (uint64_t)S * B * ctx->kv.field_words * 4,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay;
+        for (uint32_t q = 0; q < n_no; q++) {
+            const Req &r = sv->reqs[f.members[q]];
+            const uint32_t *src = &sv->no_data[r.data * kNoFields];
+            for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
+            f.h_no[52 * C + q] = r.tid;
+        }
+        for (uint32_t p = 0; p < n_pay; p++) {
+            const Req &r = sv->reqs[f.members[n_no + p]];
+            const uint32_t *src = &sv->pay_data[r.data * kPayFields];
+            for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
+            f.h_pay[9 * C + p] = r.tid;
+        }
+        if (n_no)
+            LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_no, C * 4, f.h_no, C * 4, (uint64_t)n_no * 4, 53,
+                                          hipMemcpyHostToDevice, L.copy));
+        if (n_pay)
+            LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_pay, C * 4, f.h_pay, C * 4, (uint64_t)n_pay * 4, 10,
+                                          hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
+        LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
+        ltpg_neworder_batch nb;
+        ltpg_payment_batch pb;
+        ltpg_neworder_view(f.d_no, (uint32_t)C, n_no, &nb);
+        ltpg_payment_view(f.d_pay, (uint32_t)C, n_pay, &pb);
+        if ((rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_DEVICE))) return rc;
+        if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+        if (n_no)
+            LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->tpcc.d.bitmap_no, words(n_no) * 8, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        if (n_pay)
+            LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits2, ctx->tpcc.d.bitmap_pay, words(n_pay) * 8, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        if (sv->cfg.keep_reads) {
+            if (n_no)
+                LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->tpcc.d.reads_no, (uint64_t)kNoReads * n_no * 4,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+            if (n_pay)
+                LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads2, ctx->tpcc.d.reads_pay, (uint64_t)kPayReads * n_pay * 4,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    LTPG_HIP_TRY(hipEventRecord(f.done, ctx->stream));
+    f.busy = true;
+    return LTPG_OK;
+}
+
+bool bit(const uint64_t *b, uint32_t i) { return (b[i / 64] >> (i % 64)) & 1ull; }
+
+// Read a finished epoch's results (lock held).
+void complete(ltpg_server *sv, int w, Flight &f)
+{
+    Lane &L = sv->lane[w];
+    const uint64_t t_now = now_ns();
+    const uint32_t B = (uint32_t)f.members.size();
+    const uint32_t S = sv->cfg.kv_ops;
+    for (uint32_t t = 0; t < B; t++) {
+        const uint64_t ri = f.members[t];
+        Req &r = sv->reqs[ri];
+        r.attempts++;
+        sv->attempts[r.type]++;
+        bool ok;
+        if (w == 0) ok = bit(f.h_bits, t);
+        else if (t < f.n_no) ok = bit(f.h_bits, t);
+        else ok = bit(f.h_bits2, t - f.n_no);
+        if (!ok) {
+            L.requeue.push(ri);
+            continue;
+        }
+        sv->committed[r.type]++;
+        sv->outstanding--;
+        ltpg_serve_done d;
+        d.id = r.id;
+        d.type = r.type;
+        d.tid = r.tid;
+        d.attempts = r.attempts;
+        d.reserved = 0;
+        d.epoch = f.epoch;
+        d.arrival_ns = r.arrival;
+        d.commit_ns = t_now;
+        sv->done.push_back(d);
+        if (sv->cfg.keep_reads) {
+            std::vector<uint32_t> v;
+            if (w == 0) {
+                const uint32_t fw = L.ctx->kv.field_words;
+                for (uint32_t s = 0; s < S; s++) {
+                    if (LTPG_KV_KIND(sv->kv_meta[r.data * S + s]) != LTPG_KV_READ) continue;
+                    const uint32_t *src = f.h_reads + (((uint64_t)s * B) + t) * fw;
+                    v.insert(v.end(), src, src + fw);
+                }
+            } else if (t < f.n_no) {
+                for (uint32_t k = 0; k < kNoReads; k++) v.push_back(f.h_reads[(uint64_t)k * f.n_no + t]);
+            } else {
+                const uint32_t p = t - f.n_no;
+                for (uint32_t k = 0; k < kPayReads; k++) v.push_back(f.h_reads2[(uint64_t)k * f.n_pay + p]);
+            }
+            sv->reads[r.id] = std::move(v);
+        }
+    }
+    f.busy = false;
+}
+
+void serve_loop(ltpg_server *sv)
+{
+    (void)hipSetDevice((sv->lane[0].ctx ? sv->lane[0].ctx : sv->lane[1].ctx)->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        bool progress = false, inflight = false;
+        for (int w = 0; w < 2; w++) {
+            Lane &L = sv->lane[w];
+            if (!L.ctx) continue;
+            // results first: re-queued aborts join the next epoch formed
+            for (auto &f : L.flights) {
+                if (!f.busy) continue;
+                const hipError_t q = hipEventQuery(f.done);
+                if (q == hipSuccess) {
+                    complete(sv, w, f);
+                    progress = true;
+                } else if (q != hipErrorNotReady) {
+                    sv->error = LTPG_ERR_DEVICE;
+                    sv->error_msg = std::string("serving epoch: ") + hipGetErrorString(q);
+                    f.busy = false;
+                }
+            }
+            // start epochs on free slots, in slot order so epochs stay ordered
+            for (auto &f : L.flights) {
+                if (f.busy || (L.fresh.empty() && L.requeue.empty()) || sv->error) continue;
+                take_members(sv, w, f);
+                f.epoch = L.epochs++;
+                L.max_epoch = std::max<uint64_t>(L.max_epoch, f.members.size());
+                if (sv->cfg.record_epochs) {
+                    std::vector<uint64_t> ids;
+                    for (uint64_t r : f.members) ids.push_back(sv->reqs[r].id);
+                    L.log.push_back(std::move(ids));
+                }
+                const int rc = launch(sv, w, f);
+                if (rc) {
+                    sv->error = rc;
+                    sv->error_msg = ltpg_last_error();
+                }
+                progress = true;
+            }
+            for (auto &f : L.flights) inflight |= f.busy;
+        }
+        if (progress) sv->cv.notify_all();
+        const bool pending = !sv->lane[0].fresh.empty() || !sv->lane[0].requeue.empty() ||
+                             !sv->lane[1].fresh.empty() || !sv->lane[1].requeue.empty();
+        if (sv->stop && !inflight && (!pending || sv->error)) break;
+        if (!progress) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(inflight ? 5 : 50));
+            lk.lock();
+        }
+    }
+    sv->cv.notify_all();
+}
+
+}  // namespace
+
+extern "C" {
+
+void ltpg_serve_config_init(ltpg_serve_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->size = sizeof(*c);
+    c->slots = 2;
+    c->kv_ops = 16;
+}
+
+int ltpg_serve_stop(ltpg_server *sv)
+{
+    if (!sv) return LTPG_OK;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+    }
+    if (sv->th.joinable()) sv->th.join();
+    const int err = sv->error;
+    const std::string msg = sv->error_msg;
+    for (int w = 0; w < 2; w++) {
+        Lane &L = sv->lane[w];
+        if (!L.ctx) continue;
+        (void)hipSetDevice(L.ctx->device);
+        (void)hipStreamSynchronize(L.ctx->stream);
+        if (L.copy) (void)hipStreamSynchronize(L.copy);
+        for (auto &f : L.flights) free_flight(f);
+        if (L.copy) (void)hipStreamDestroy(L.copy);
+    }
+    delete sv;
+    if (err) LTPG_FAIL(err, msg);
+    return LTPG_OK;
+}
+
+int ltpg_serve_start(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_serve_config *cfg, ltpg_server **out)
+{
+    if (!out || !cfg) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->size != sizeof(ltpg_serve_config)) LTPG_FAIL(LTPG_ERR_INVALID, "ltpg_serve_config.size mismatch");
+    if (!kv_ctx && !tpcc_ctx) LTPG_FAIL(LTPG_ERR_INVALID, "no context");
+    if (kv_ctx && !kv_ctx->kv.table) LTPG_FAIL(LTPG_ERR_INVALID, "kv_ctx is not a key-value context");
+    if (tpcc_ctx && !tpcc_ctx->tpcc.d.regs) LTPG_FAIL(LTPG_ERR_INVALID, "tpcc_ctx is not an order-entry context");
+    if (cfg->slots < 1 || cfg->slots > 4) LTPG_FAIL(LTPG_ERR_INVALID, "slots must be in [1, 4]");
+    if (kv_ctx && (cfg->kv_ops == 0 || cfg->kv_ops > kv_ctx->kv.max_slots))
+        LTPG_FAIL(LTPG_ERR_INVALID, "kv_ops must be in [1, kv_max_slots]");
+    ltpg_server *sv = new (std::nothrow) ltpg_server();
+    if (!sv) LTPG_FAIL(LTPG_ERR_NOMEM, "host allocation of server failed");
+    sv->cfg = *cfg;
+    sv->lane[0].ctx = kv_ctx;
+    sv->lane[1].ctx = tpcc_ctx;
+    int rc = LTPG_OK;
+    for (int w = 0; w < 2 && !rc; w++) {
+        Lane &L = sv->lane[w];
+        if (!L.ctx) continue;
+        if ((rc = check_ctx(L.ctx))) break;
+        // per-transaction bytes of the access-set buffers an epoch holds:
+        // its batch slot and its read values
+        const uint64_t per = w == 0 ? (uint64_t)cfg->kv_ops * (12 + 4ull * L.ctx->kv.field_words) + 4
+                                    : 4ull * (53 + kNoReads);
+        uint64_t cap = L.ctx->cfg.max_txns;
+        if (cfg->max_epoch_txns) cap = std::min<uint64_t>(cap, cfg->max_epoch_txns);
+        if (cfg->hbm_budget) cap = std::min<uint64_t>(cap, cfg->hbm_budget / per);
+        if (cap == 0) {
+            set_error("the HBM budget admits no transaction");
+            rc = LTPG_ERR_BUDGET;
+            break;
+        }
+        L.cap = (uint32_t)cap;
+        if (hipStreamCreateWithFlags(&L.copy, hipStreamNonBlocking) != hipSuccess) {
+            set_error("hipStreamCreate failed");
+            rc = LTPG_ERR_DEVICE;
+            break;
+        }
+        L.flights.resize(cfg->slots);
+        for (auto &f : L.flights)
+            if (!rc) rc = alloc_flight(sv, w, f);
+    }
+    if (rc) {
+        const std::string msg = ltpg_last_error();
+        ltpg_serve_stop(sv);
+        set_error(msg);
+        return rc;
+    }
+    sv->th = std::thread(serve_loop, sv);
+    *out = sv;
+    return LTPG_OK;
+}
+
+int ltpg_serve_submit_kv(ltpg_server *sv, uint32_t n, const uint64_t *keys, const uint32_t *meta, uint64_t *first_id)
+{
+    if (!sv || (n && (!keys || !meta))) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    if (!sv->lane[0].ctx) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server has no key-value context");
+    const uint64_t t = now_ns();
+    const uint32_t S = sv->cfg.kv_ops;
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (first_id) *first_id = sv->reqs.size();
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t data = sv->kv_keys.size() / S;
+        sv->kv_keys.insert(sv->kv_keys.end(), keys + (uint64_t)i * S, keys + (uint64_t)(i + 1) * S);
+        sv->kv_meta.insert(sv->kv_meta.end(), meta + (uint64_t)i * S, meta + (uint64_t)(i + 1) * S);
+        sv->lane[0].fresh.push_back(sv->reqs.size());
+        sv->reqs.push_back(Req{sv->reqs.size(), LTPG_TYPE_KV, sv->lane[0].next_tid++, 0, t, data});
+    }
+    sv->outstanding += n;
+    return LTPG_OK;
+}
+
+int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *no, const ltpg_payment_batch *pay,
+                           uint64_t *first_id)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    if (!sv->lane[1].ctx) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server has no order-entry context");
+    const uint32_t n_no = no ? no->n : 0, n_pay = pay ? pay->n : 0;
+    if (n_no && (no->line_stride < n_no || !no->W_ID || !no->D_ID || !no->C_ID || !no->O_ID || !no->N_O_ID ||
+                 !no->O_OL_CNT || !no->O_OL_ID || !no->OL_I_ID || !no->OL_SUPPLY_W_ID || !no->OL_QUANTITY))
+        LTPG_FAIL(LTPG_ERR_INVALID, "bad NewOrder arrays");
+    if (n_pay && (!pay->W_ID || !pay->D_ID || !pay->C_ID || !pay->C_LAST || !pay->isName || !pay->C_D_ID ||
+                  !pay->C_W_ID || !pay->H_AMOUNT || !pay->H_ID))
+        LTPG_FAIL(LTPG_ERR_INVALID, "bad Payment arrays");
+    const uint64_t t = now_ns();
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (first_id) *first_id = sv->reqs.size();
+    Lane &L = sv->lane[1];
+    for (uint32_t q = 0; q < n_no; q++) {
+        const uint64_t data = sv->no_data.size() / kNoFields;
+        const uint32_t *heads[7] = {no->W_ID, no->D_ID, no->C_ID, no->O_ID, no->N_O_ID, no->O_OL_CNT, no->O_OL_ID};
+        for (auto h : heads) sv->no_data.push_back(h[q]);
+        for (const uint32_t *a : {no->OL_I_ID, no->OL_SUPPLY_W_ID, no->OL_QUANTITY})
+            for (uint32_t l = 0; l < LTPG_TPCC_MAX_LINES; l++) sv->no_data.push_back(a[(uint64_t)l * no->line_stride + q]);
+        L.fresh.push_back(sv->reqs.size());
+        sv->reqs.push_back(Req{sv->reqs.size(), LTPG_TYPE_NEWORDER, L.next_tid++, 0, t, data});
+    }
+    for (uint32_t p = 0; p < n_pay; p++) {
+        const uint64_t data = sv->pay_data.size() / kPayFields;
+        const uint32_t *cols[9] = {pay->W_ID, pay->D_ID, pay->C_ID, pay->C_LAST, pay->isName,
+                                   pay->C_D_ID, pay->C_W_ID, pay->H_AMOUNT, pay->H_ID};
+        for (auto c : cols) sv->pay_data.push_back(c[p]);
+        L.fresh.push_back(sv->reqs.size());
+        sv->reqs.push_back(Req{sv->reqs.size(), LTPG_TYPE_PAYMENT, L.next_tid++, 0, t, data});
+    }
+    sv->outstanding += n_no + n_pay;
+    return LTPG_OK;
+}
+
+int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out)
+{
+    if (!sv || (!out && max)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    uint32_t n = 0;
+    while (n < max && !sv->done.empty()) {
+        out[n++] = sv->done.front();
+        sv->done.pop_front();
+    }
+    if (n_out) *n_out = n;
+    if (sv->error) LTPG_FAIL(sv->error, sv->error_msg);
+    return LTPG_OK;
+}
+
+int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (!sv->cfg.keep_reads) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server does not keep read values");
+    auto it = sv->reads.find(id);
+    if (it == sv->reads.end()) LTPG_FAIL(LTPG_ERR_STATE, "request has not committed");
+    const auto &v = it->second;
+    if (n_words) *n_words = (uint32_t)v.size();
+    if (out) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), max_words), out);
+    return LTPG_OK;
+}
+
+int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *ids, uint32_t max, uint32_t *n)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    if (workload != LTPG_WORKLOAD_KV && workload != LTPG_WORKLOAD_TPCC) LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (!sv->cfg.record_epochs) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server does not record epochs");
+    const Lane &L = sv->lane[workload == LTPG_WORKLOAD_KV ? 0 : 1];
+    if (e >= L.log.size()) LTPG_FAIL(LTPG_ERR_INVALID, "no such epoch");
+    const auto &v = L.log[e];
+    if (n) *n = (uint32_t)v.size();
+    if (ids) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), max), ids);
+    return LTPG_OK;
+}
+
+int ltpg_serve_drain(ltpg_server *sv)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv.wait(lk, [&] { return sv->outstanding == 0 || sv->error; });
+    if (sv->error) LTPG_FAIL(sv->error, sv->error_msg);
+    return LTPG_OK;
+}
+
+int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st)
+{
+    if (!sv || !st) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (int k = 0; k < 3; k++) {
+        st->committed[k] = sv->committed[k];
+        st->attempts[k] = sv->attempts[k];
+    }
+    for (int w = 0; w < 2; w++) {
+        st->epochs[w] = sv->lane[w].epochs;
+        st->max_epoch_txns[w] = sv->lane[w].max_epoch;
+    }
+    return LTPG_OK;
+}
+
+}  // extern "C"

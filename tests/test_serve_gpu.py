@@ -1,0 +1,203 @@
+"""Served form: continuous admission over resident tables.
+
+Epoch composition depends on arrival timing, so parity is checked on the
+epochs the server actually formed: its recorded epoch membership, replayed
+through the oracle, must reproduce every commit decision, every read value
+and the final tables.  Requests on rows nobody else touches must see the
+same results alone and in a contended mix.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import ltpg_amd as L
+from oracle import oracle
+from tests import kvcases as kc
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x4C545047
+
+
+def _kv_requests(seed, n, S, rows, fields, theta=0.9, lo=0):
+    keys, meta, _ = L.kv_generate(seed=seed, rows=rows, fields=fields, n_txns=n, n_slots=S, min_ops=1, max_ops=S,
+                                  theta=theta)
+    keys = keys.reshape(S, n).T.copy() + lo
+    return keys, meta.reshape(S, n).T.copy()
+
+
+def _kv_ctx(rows, fields, fw, S, max_txns):
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                    max_txns=max_txns, kv_max_slots=S)
+    ctx.kv_table_load(kc.initial_table(rows, fields, fw))
+    return ctx
+
+
+def _submit_in_waves(sv, keys, meta, waves=12, pause=0.002):
+    for part_k, part_m in zip(np.array_split(keys, waves), np.array_split(meta, waves)):
+        sv.submit_kv(part_k, part_m)
+        time.sleep(pause)
+
+
+def test_served_kv_epochs_replay_on_the_oracle(gpu):
+    rows, fields, fw, S, n = 3000, 3, 2, 8, 6000
+    ctx = _kv_ctx(rows, fields, fw, S, 1024)
+    keys, meta = _kv_requests(31, n, S, rows, fields)
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=1, record_epochs=1)
+    _submit_in_waves(sv, keys, meta)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_KV, e) for e in range(st["epochs"][0])]
+    reads = {rid: sv.read_values(rid) for rid in done}
+    sv.stop()
+    assert len(done) == n and st["committed"][0] == n and st["max_epoch_txns"][0] <= 700
+    table = kc.initial_table(rows, fields, fw)
+    for e, ids in enumerate(epochs):
+        ids = ids.astype(np.int64)
+        B = len(ids)
+        k = keys[ids].T.reshape(-1).copy()
+        m = meta[ids].T.reshape(-1).copy()
+        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True)
+        for t, rid in enumerate(ids):
+            committed = ep.committed(t)
+            assert committed == (done[rid]["epoch"] == e), f"request {rid} epoch {e}"
+            if committed:
+                want = [ep.read_values[(s * B + t) * fw:(s * B + t + 1) * fw] for s in range(S)
+                        if (m[s * B + t] & 3) == 1]
+                got = reads[rid]
+                np.testing.assert_array_equal(got, np.concatenate(want) if want else np.zeros(0, np.uint32))
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+    # every request committed after at least one attempt, latency is positive
+    assert all(d["attempts"] >= 1 and d["commit_ns"] >= d["arrival_ns"] for d in done.values())
+
+
+def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
+    rows, fields, fw, S = 20000, 3, 2, 8
+    n_probe = 400
+    # probes: request i reads and writes rows [4i, 4i + 4) only
+    pk = np.zeros((n_probe, S), np.uint64)
+    pm = np.zeros((n_probe, S), np.uint32)
+    for i in range(n_probe):
+        for s in range(4):
+            pk[i, s] = 4 * i + s
+            pm[i, s] = L.kv_meta(L.KV_READ if s % 2 == 0 else L.KV_WRITE, s % fields)
+    bk, bm = _kv_requests(77, 20000, S, rows - 4 * n_probe, fields, theta=0.99, lo=4 * n_probe)
+    results = []
+    for with_mix in (False, True):
+        ctx = _kv_ctx(rows, fields, fw, S, 4096)
+        tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=1, max_txns=4096)
+        tp.tpcc_populate(SEED)
+        sv = L.Server(ctx, tp, kv_ops=S, slots=2, keep_reads=1)
+        ids = []
+        if with_mix:
+            def background():
+                gen = L.TpccGenerator(warehouses=1, seed=3)
+                for part_k, part_m in zip(np.array_split(bk, 20), np.array_split(bm, 20)):
+                    sv.submit_kv(part_k, part_m)
+                    sv.submit_tpcc(gen.next(500))
+            th = threading.Thread(target=background)
+            th.start()
+        for part_k, part_m in zip(np.array_split(pk, 8), np.array_split(pm, 8)):
+            first = sv.submit_kv(part_k, part_m)
+            ids.extend(range(first, first + len(part_k)))
+            time.sleep(0.001)
+        if with_mix:
+            th.join()
+        sv.drain()
+        done = {d["id"]: d for d in sv.poll()}
+        probe_reads = [sv.read_values(i) for i in ids]
+        probe_attempts = [done[i]["attempts"] for i in ids]
+        sv.stop()
+        tab = ctx.kv_table_read().reshape(rows, fields, fw)[:4 * n_probe].copy()
+        # a write adds the writer's TID | 1 (its arrival rank, which the mix
+        # shifts): subtract it to get the part of the result that must agree
+        for j, rid in enumerate(ids):
+            for s in range(4):
+                if pm[j, s] & 3 == L.KV_WRITE:
+                    tab[pk[j, s], (pm[j, s] >> 8) & 0xFFFF] -= np.uint32(done[rid]["tid"] | 1)
+        results.append((probe_reads, probe_attempts, tab))
+    (ra, aa, ta), (rb, ab, tb) = results
+    for x, y in zip(ra, rb):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(ta, tb)
+    np.testing.assert_array_equal(ta, kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)[:4 * n_probe])
+    # private rows: every probe commits at its first attempt, alone or not
+    assert set(aa) == {1} and set(ab) == {1}
+
+
+def test_served_order_entry_epochs_replay_on_the_oracle(gpu):
+    W = 2
+    tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=2048)
+    tp.tpcc_populate(SEED)
+    gen = L.TpccGenerator(warehouses=W, seed=12)
+    batches = [gen.next(1500) for _ in range(4)]
+    sv = L.Server(None, tp, slots=2, max_epoch_txns=900, keep_reads=1, record_epochs=1)
+    index = {}  # request id -> (batch, type, position)
+    for bi, b in enumerate(batches):
+        first = sv.submit_tpcc(b)
+        for q in range(b.n_no):
+            index[first + q] = (bi, L.TYPE_NEWORDER, q)
+        for p in range(b.n_pay):
+            index[first + b.n_no + p] = (bi, L.TYPE_PAYMENT, p)
+        time.sleep(0.002)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_TPCC, e) for e in range(st["epochs"][1])]
+    reads = {rid: sv.read_values(rid) for rid in done}
+    sv.stop()
+    assert len(done) == len(index)
+    tables = oracle.tpcc_populate(W, SEED)
+    ora = oracle.TpccOracle(W)
+    for e, ids in enumerate(epochs):
+        nos = [i for i in ids if index[int(i)][1] == L.TYPE_NEWORDER]
+        pays = [i for i in ids if index[int(i)][1] == L.TYPE_PAYMENT]
+        eb = L.TpccBatch.empty(len(nos), len(pays))
+        for q, rid in enumerate(nos):
+            bi, _, pos = index[int(rid)]
+            src = batches[bi]
+            for f in ("W_ID", "D_ID", "C_ID", "O_ID", "N_O_ID", "O_OL_CNT", "O_OL_ID"):
+                eb.nf(f)[q] = src.nf(f)[pos]
+            for f in ("OL_I_ID", "OL_SUPPLY_W_ID", "OL_QUANTITY"):
+                for l in range(15):
+                    eb.nf(f, l)[q] = src.nf(f, l)[pos]
+            eb.nf("TID")[q] = done[int(rid)]["tid"]
+        for p, rid in enumerate(pays):
+            bi, _, pos = index[int(rid)]
+            src = batches[bi]
+            for f in ("W_ID", "D_ID", "C_ID", "C_LAST", "isName", "C_D_ID", "C_W_ID", "H_AMOUNT", "H_ID"):
+                eb.pf(f)[p] = src.pf(f)[pos]
+            eb.pf("TID")[p] = done[int(rid)]["tid"]
+        ep = ora.epoch(tables, eb)
+        for q, rid in enumerate(nos):
+            c = bool((int(ep.no_bitmap[q // 64]) >> (q % 64)) & 1)
+            assert c == (done[int(rid)]["epoch"] == e)
+            if c:
+                np.testing.assert_array_equal(reads[int(rid)], ep.no_reads.reshape(51, len(nos))[:, q])
+        for p, rid in enumerate(pays):
+            c = bool((int(ep.pay_bitmap[p // 64]) >> (p % 64)) & 1)
+            assert c == (done[int(rid)]["epoch"] == e)
+            if c:
+                np.testing.assert_array_equal(reads[int(rid)], ep.pay_reads.reshape(24, len(pays))[:, p])
+    for t in range(9):
+        np.testing.assert_array_equal(tp.tpcc_table_read(t), tables[t])
+
+
+def test_hbm_budget_bounds_epochs(gpu):
+    rows, fields, fw, S = 1000, 2, 1, 4
+    ctx = _kv_ctx(rows, fields, fw, S, 4096)
+    per = S * (12 + 4 * fw) + 4
+    with pytest.raises(L.LtpgError) as ei:
+        L.Server(ctx, None, kv_ops=S, hbm_budget=per - 1)
+    assert ei.value.code == L.ERR_BUDGET
+    sv = L.Server(ctx, None, kv_ops=S, hbm_budget=per * 50)
+    keys, meta = _kv_requests(5, 3000, S, rows, fields)
+    sv.submit_kv(keys, meta)
+    sv.drain()
+    st = sv.stats()
+    sv.stop()
+    assert st["committed"][0] == 3000 and 0 < st["max_epoch_txns"][0] <= 50
