@@ -461,6 +461,126 @@ def run_batched_c4(args):
     }
 
 
+def run_served_c5(args):
+    """Served form under Poisson arrivals: a 50/50 mix of key-value requests
+    (1-64 ops, Zipf 0.99, 100M rows x 10 x 100 B) and order-entry requests
+    (W = 1024, 50/50 NewOrder / Payment) offered at each rate in
+    --c5-rates.  Requests are submitted in 1 ms windows as their scheduled
+    arrival time comes; latency is commit time minus the scheduled arrival
+    (so a submitter that falls behind the schedule counts as queueing).
+    Aborted requests re-queue with their TID.  The drain is capped at
+    --c5-drain-s: requests still uncommitted then are dropped and counted
+    (the completed fraction), and latency covers the committed ones."""
+    import ltpg_amd as L
+
+    seed = 0x4C545047
+    rows, fields, fw, S, W = args.c5_rows, 10, 25, 64, 1024
+    desc = (f"served form: Poisson arrivals, 50/50 key-value (1-64 ops, Zipf 0.99, {rows // 1_000_000}M rows x 10 "
+            f"x 100 B) and order-entry (W={W}) requests, {args.c5_seconds} s of arrivals per rate")
+    log(f"C5: {desc}")
+    kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                   max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW)
+    kv.kv_fill_reference()
+    tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=1 << 20, values=L.VALUES_RMW)
+    tp.tpcc_populate(seed)
+    log("C5: tables resident")
+    rng = np.random.default_rng(seed)
+    gen = L.TpccGenerator(warehouses=W, seed=seed)
+    out = []
+    for rate in args.c5_rates:
+        n = max(1000, int(rate * 1e6 * args.c5_seconds))
+        n_kv = n // 2
+        keys, meta, _ = L.kv_generate(seed=seed + int(rate * 1000), rows=rows, fields=fields, n_txns=n_kv,
+                                      n_slots=S, min_ops=1, max_ops=S, theta=0.99)
+        keys = keys.reshape(S, n_kv).T.copy()
+        meta = meta.reshape(S, n_kv).T.copy()
+        tb = gen.next(n - n_kv)
+        oe_tid = np.r_[tb.nf("TID"), tb.pf("TID")]
+        oe_order = np.argsort(oe_tid, kind="stable")  # request k of the batch's own arrival order
+        at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
+        kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
+        arrival_of = {}  # request id -> arrival index
+        sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20)
+        window = 1e-3
+        edges = np.searchsorted(at, np.arange(window, at[-1] + window, window), side="right")
+        ki = oi = prev = 0
+        kv_ids, oe_ids = [], []
+        t0_ns = time.monotonic_ns()
+        for e in edges:
+            if e <= prev:
+                continue
+            while time.monotonic_ns() - t0_ns < at[e - 1] * 1e9:
+                pass
+            sel = np.arange(prev, e)
+            kv_arr = sel[kind[prev:e] == 0]
+            oe_arr = sel[kind[prev:e] == 1]
+            if len(kv_arr):
+                first = sv.submit_kv(keys[ki:ki + len(kv_arr)], meta[ki:ki + len(kv_arr)])
+                kv_ids.append((first, kv_arr))
+                ki += len(kv_arr)
+            if len(oe_arr):
+                idx = oe_order[oi:oi + len(oe_arr)]
+                part, perm = _tpcc_select(tb, idx)
+                first = sv.submit_tpcc(part)
+                oe_ids.append((first, oe_arr[perm]))
+                oi += len(oe_arr)
+            prev = e
+        submit_s = (time.monotonic_ns() - t0_ns) * 1e-9
+        done = []
+        deadline = time.monotonic() + args.c5_drain_s
+        while len(done) < n and time.monotonic() < deadline:
+            time.sleep(0.25)
+            done += sv.poll(1 << 26)
+            log(f"C5: {rate}M txn/s: {len(done)}/{n} committed")
+        st = sv.stats()
+        dropped = sv.cancel()
+        for first, arr in kv_ids + oe_ids:
+            for j, a_i in enumerate(arr):
+                arrival_of[first + j] = a_i
+        sched = np.array([t0_ns + at[arrival_of[d["id"]]] * 1e9 for d in done])
+        commit = np.array([d["commit_ns"] for d in done], dtype=np.float64)
+        lat = (commit - sched) * 1e-6
+        span_s = (commit.max() - t0_ns) * 1e-9
+        out.append({"offered_txn_per_s": rate * 1e6, "requests": n, "committed": len(done), "dropped_at_cap": dropped,
+                    "completed_fraction": len(done) / n, "drain_cap_s": args.c5_drain_s,
+                    "committed_txn_per_s": len(done) / span_s, "submit_s": submit_s, "span_s": span_s,
+                    "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                   "max": float(lat.max())},
+                    "epochs": {"kv": st["epochs"][0], "tpcc": st["epochs"][1]},
+                    "max_epoch_txns": {"kv": st["max_epoch_txns"][0], "tpcc": st["max_epoch_txns"][1]},
+                    "attempts_per_commit": sum(st["attempts"]) / max(sum(st["committed"]), 1)})
+        log(f"C5: offered {rate}M txn/s -> {out[-1]['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
+            f"p50 {out[-1]['latency_ms']['p50']:.2f} ms, p99 {out[-1]['latency_ms']['p99']:.2f} ms, "
+            f"submit {submit_s:.3f} s for {at[-1]:.3f} s of arrivals")
+    kv.close()
+    tp.close()
+    best = max(out, key=lambda r: r["committed_txn_per_s"])
+    return {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
+            "latency_ms": best["latency_ms"], "rates": out,
+            "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
+                       "slots": 2, "form": "served (continuous admission, requeue with original TID)"}}
+
+
+def _tpcc_select(b, idx):
+    """Order-entry requests idx (indices into NewOrders then Payments of b)
+    as a new TpccBatch, and the permutation of idx in the order the server
+    numbers them (the batch's NewOrders first, then its Payments)."""
+    import ltpg_amd as L
+
+    idx = np.asarray(idx)
+    is_no = idx < b.n_no
+    nos, pays = idx[is_no], idx[~is_no] - b.n_no
+    out = L.TpccBatch.empty(len(nos), len(pays))
+    for f in ("W_ID", "D_ID", "C_ID", "O_ID", "N_O_ID", "O_OL_CNT", "O_OL_ID", "TID"):
+        out.nf(f)[:] = b.nf(f)[nos]
+    for f in ("OL_I_ID", "OL_SUPPLY_W_ID", "OL_QUANTITY"):
+        out.nf(f)[:, :] = b.nf(f)[:, nos]
+    for f in ("W_ID", "D_ID", "C_ID", "C_LAST", "isName", "C_D_ID", "C_W_ID", "H_AMOUNT", "H_ID", "TID"):
+        out.pf(f)[:] = b.pf(f)[pays]
+    perm = np.r_[np.nonzero(is_no)[0], np.nonzero(~is_no)[0]]
+    return out, perm
+
+
 def measured_traffic(phase, family="kv"):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if
     one exists for this kernel (profiles/kv_c2_traffic.json,
@@ -498,7 +618,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3", "batched-c4"])
+    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3", "batched-c4", "served-c5"])
+    ap.add_argument("--c5-rates", type=float, nargs="+", default=[1, 5, 10, 25, 50],
+                    help="offered loads of the served form, M txn/s")
+    ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
+    ap.add_argument("--c5-drain-s", type=float, default=8.0, help="cap on the drain after the arrivals, per load")
+    ap.add_argument("--c5-rows", type=int, default=100_000_000, help="key-value rows of the served form")
+    ap.add_argument("--c5-budget-mb", type=int, default=4096, help="HBM budget per in-flight epoch, MiB")
     ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form variant of the default run")
     ap.add_argument("--theta", type=float, default=None,
@@ -522,6 +648,10 @@ def main():
     variants = []
     if args.workload == "tpcc-c3":
         result = run_tpcc(args)
+    elif args.workload == "served-c5":
+        v = run_served_c5(args)
+        result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True,
+                  "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic"} | v
     elif args.workload == "batched-c4":
         v = run_batched_c4(args)
         result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True,
@@ -537,6 +667,7 @@ def main():
             variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
             if not args.no_batched:
                 variants.append(run_batched_c4(args))
+                variants.append(run_served_c5(args))
     if variants:
         result["variants"] = variants
     print(json.dumps(result))

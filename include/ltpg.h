@@ -499,6 +499,10 @@ int ltpg_serve_drain(ltpg_server *sv);
 int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st);
 /* Drain, stop the thread and free the server (the contexts stay). */
 int ltpg_serve_stop(ltpg_server *sv);
+/* Drop every request not in an epoch yet (new and re-queued, including the
+ * aborts of the epochs still in flight), let those epochs finish, then stop
+ * and free the server.  *dropped = requests that never committed. */
+int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped);
 
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:

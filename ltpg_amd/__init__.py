@@ -580,6 +580,15 @@ class Server:
             h, self.handle = self.handle, None
             _check(lib.ltpg_serve_stop(h), "ltpg_serve_stop")
 
+    def cancel(self):
+        """Drop what has not been admitted yet, finish the epochs in flight,
+        stop; returns the number of requests that never committed."""
+        dropped = ctypes.c_uint64()
+        if self.handle:
+            h, self.handle = self.handle, None
+            _check(lib.ltpg_serve_cancel(h, ctypes.byref(dropped)), "ltpg_serve_cancel")
+        return dropped.value
+
     def __del__(self):
         try:
             self.stop()

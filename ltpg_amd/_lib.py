@@ -200,6 +200,7 @@ SIGNATURES = [
     ("ltpg_serve_drain", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_serve_stats_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeStats)]),
     ("ltpg_serve_stop", ctypes.c_int, [ctypes.c_void_p]),
+    ("ltpg_serve_cancel", ctypes.c_int, [ctypes.c_void_p, c_u64p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),
     ("ltpg_kv_generate", ctypes.c_int, [ctypes.POINTER(KvGen), c_u64p, c_u32p, c_u32p, ctypes.c_uint32]),
 ]

@@ -134,7 +134,9 @@ struct ltpg_server {
     std::unordered_map<uint64_t, std::vector<uint32_t>> reads;
     uint64_t committed[3] = {0, 0, 0}, attempts[3] = {0, 0, 0};
     uint64_t outstanding = 0;        // submitted, not yet committed
+    uint64_t dropped = 0;            // dropped by ltpg_serve_cancel
     bool stop = false;
+    bool cancel = false;
     int error = LTPG_OK;
     std::string error_msg;
     std::thread th;
@@ -358,6 +360,14 @@ void serve_loop(ltpg_server *sv)
     std::unique_lock<std::mutex> lk(sv->mu);
     while (true) {
         bool progress = false, inflight = false;
+        if (sv->cancel) {
+            for (auto &L : sv->lane) {
+                sv->dropped += L.fresh.size() + L.requeue.size();
+                sv->outstanding -= L.fresh.size() + L.requeue.size();
+                L.fresh.clear();
+                while (!L.requeue.empty()) L.requeue.pop();
+            }
+        }
         for (int w = 0; w < 2; w++) {
             Lane &L = sv->lane[w];
             if (!L.ctx) continue;
@@ -367,6 +377,11 @@ void serve_loop(ltpg_server *sv)
                 const hipError_t q = hipEventQuery(f.done);
                 if (q == hipSuccess) {
                     complete(sv, w, f);
+                    if (sv->cancel) {  // aborts of a cancelled run are dropped too
+                        sv->dropped += L.requeue.size();
+                        sv->outstanding -= L.requeue.size();
+                        while (!L.requeue.empty()) L.requeue.pop();
+                    }
                     progress = true;
                 } else if (q != hipErrorNotReady) {
                     sv->error = LTPG_ERR_DEVICE;
@@ -418,6 +433,19 @@ void ltpg_serve_config_init(ltpg_serve_config *c)
     c->size = sizeof(*c);
     c->slots = 2;
     c->kv_ops = 16;
+}
+
+int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->cancel = true;  // the loop drops what is pending, finishes its
+        sv->stop = true;    // epochs in flight and returns
+    }
+    if (sv->th.joinable()) sv->th.join();
+    if (dropped) *dropped = sv->dropped;
+    return ltpg_serve_stop(sv);
 }
 
 int ltpg_serve_stop(ltpg_server *sv)

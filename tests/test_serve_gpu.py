@@ -201,3 +201,20 @@ def test_hbm_budget_bounds_epochs(gpu):
     st = sv.stats()
     sv.stop()
     assert st["committed"][0] == 3000 and 0 < st["max_epoch_txns"][0] <= 50
+
+
+def test_cancel_drops_what_was_not_admitted(gpu):
+    rows, fields, fw, S = 500, 2, 1, 8
+    ctx = _kv_ctx(rows, fields, fw, S, 512)
+    keys, meta = _kv_requests(9, 20000, S, rows, fields, theta=0.99)
+    sv = L.Server(ctx, None, kv_ops=S, max_epoch_txns=256)
+    sv.submit_kv(keys, meta)
+    time.sleep(0.05)
+    committed = len(sv.poll())
+    dropped = sv.cancel()
+    assert dropped > 0 and committed + dropped <= 20000
+    # the context works epoch by epoch afterwards
+    k, m, t = L.kv_generate(seed=1, rows=rows, fields=fields, n_txns=100, n_slots=S)
+    ctx.kv_submit(k, m, t, S)
+    ctx.epoch_run()
+    ctx.sync()
