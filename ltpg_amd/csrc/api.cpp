@@ -36,6 +36,7 @@ void free_kv(KvState &kv)
     dev_free(kv.keys_buf);
     dev_free(kv.meta_buf);
     dev_free(kv.tids_buf);
+    dev_free(kv.hot_acc);
 }
 
 int init_kv(ltpg_ctx *ctx)
@@ -48,8 +49,9 @@ int init_kv(ltpg_ctx *ctx)
     if (c.kv_fields > 65536) LTPG_FAIL(LTPG_ERR_INVALID, "kv_fields must be <= 65536");
     if (c.kv_max_slots == 0 || c.kv_max_slots > 64)
         LTPG_FAIL(LTPG_ERR_INVALID, "kv_max_slots must be in [1, 64]");
-    if (c.kv_hot_row != LTPG_NO_HOT_ROW)
-        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "kv_hot_row is not supported by this build");
+    if (c.kv_hot_row != LTPG_NO_HOT_ROW && c.kv_hot_row >= c.kv_rows)
+        LTPG_FAIL(LTPG_ERR_INVALID, "kv_hot_row must be LTPG_NO_HOT_ROW or a row of the table");
+    kv.hot_row = c.kv_hot_row;
     kv.rows = c.kv_rows;
     kv.fields = c.kv_fields;
     kv.field_words = c.kv_field_bytes / 4;
@@ -68,6 +70,10 @@ int init_kv(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&kv.keys_buf, S * B, "kv keys"))) return rc;
     if ((rc = dev_alloc(&kv.meta_buf, S * B, "kv meta"))) return rc;
     if ((rc = dev_alloc(&kv.tids_buf, B, "kv tids"))) return rc;
+    if (kv.hot_row != LTPG_NO_HOT_ROW) {
+        if ((rc = dev_alloc(&kv.hot_acc, (uint64_t)kCounterStripes * kv.fields, "hot-row accumulators"))) return rc;
+        LTPG_HIP_TRY(hipMemsetAsync(kv.hot_acc, 0, (uint64_t)kCounterStripes * kv.fields * 4, ctx->stream));
+    }
     // Registers start as all-ones: tag ~0 belongs to epoch 0, which is never
     // current (epochs start at 1), so every row reads as "no registrant".
     LTPG_HIP_TRY(launch_fill_u64(kv.regs, kv.rows * 2, ~0ull, ctx->stream));

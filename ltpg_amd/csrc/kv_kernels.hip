@@ -248,8 +248,8 @@ __global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__r
 __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint8_t *__restrict__ doomed,
-    uint8_t epoch, DeviceCounters *__restrict__ ctr)
+    unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
+    uint8_t *__restrict__ doomed, uint8_t epoch, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     for (uint32_t s = 0; s < S; ++s) {
         const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
         bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
-        if (!o.valid) continue;
+        if (!o.valid || o.row == hot) continue;  // the hot row never registers
         if (o.kind == LTPG_KV_WRITE)
             waw |= register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch);
         else
@@ -278,8 +278,8 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
 __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint8_t *__restrict__ doomed,
-    uint8_t epoch, DeviceCounters *__restrict__ ctr)
+    unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
+    uint8_t *__restrict__ doomed, uint8_t epoch, DeviceCounters *__restrict__ ctr)
 {
     if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
         ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const uint32_t t = (uint32_t)(i - s * B);
     const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
     const bool bad = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid;
-    if (o.valid) {
+    if (o.valid && o.row != hot) {
         const unsigned long long me = priority(tag, tids[t], t);
         if (o.kind == LTPG_KV_WRITE) {
             if (register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch)) doomed[t] = epoch;
@@ -309,8 +309,8 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
 __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    const unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag, uint64_t *__restrict__ bitmap,
-    uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
+    const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
+    uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     uint32_t raw = 0, war = 0, waw = 0;
     for (uint32_t s = 0; s < S; ++s) {
         const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
-        if (!o.valid) continue;
+        if (!o.valid || o.row == hot) continue;  // the hot row never conflicts
         if (o.kind == LTPG_KV_READ) {
             raw += regs[rl.w(o.row)] < me ? 1u : 0u;
         } else {
@@ -437,8 +437,8 @@ template <int G, int CH>
 __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, RegLayout rl, uint32_t tag,
-    const uint8_t *__restrict__ doomed, uint8_t epoch, const uint32_t *__restrict__ table,
+    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot,
+    uint32_t tag, const uint8_t *__restrict__ doomed, uint8_t epoch, const uint32_t *__restrict__ table,
     uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
     DeviceCounters *__restrict__ ctr)
 {
@@ -468,9 +468,10 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
         // a read consults W (raw), a write consults R (war): one load
-        const unsigned long long rv = regs[o.valid ? (o.kind == LTPG_KV_READ ? rl.w(o.row) : rl.r(o.row)) : 0ull];
-        const bool raw = o.valid && o.kind == LTPG_KV_READ && rv < me;
-        const bool war = o.valid && o.kind == LTPG_KV_WRITE && rv < me;
+        const bool cold = o.valid && o.row != hot;  // the hot row never conflicts
+        const unsigned long long rv = regs[cold ? (o.kind == LTPG_KV_READ ? rl.w(o.row) : rl.r(o.row)) : 0ull];
+        const bool raw = cold && o.kind == LTPG_KV_READ && rv < me;
+        const bool war = cold && o.kind == LTPG_KV_WRITE && rv < me;
         const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
         const unsigned long long rd = __ballot(commit && o.valid && o.kind == LTPG_KV_READ);
         if (rd)
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     uint32_t fw, uint32_t fsw, uint32_t *__restrict__ table, const uint64_t *__restrict__ bitmap,
-    uint32_t values)
+    uint32_t values, uint64_t hot)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ uint32_t s_n;
@@ -530,7 +531,8 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
         const uint32_t t = has ? s_list[j] : 0u;
         const uint32_t step = (has && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
         const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
-        const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
+        // writes to the hot row go through kv_hot_sum_kernel
+        const bool wr = o.valid && o.kind == LTPG_KV_WRITE && o.row != hot;
         const unsigned long long wm = __ballot(wr);
         if (!wm) continue;
         const uint64_t off = (o.row * fields + o.field) * fsw;
@@ -548,6 +550,56 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
         const unsigned long long am = __ballot(wr && first);
         group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
     }
+}
+
+// Hot row: committed writes to it add their delta (TID | 1 per word) in any
+// order.  A block sums its transactions' deltas per field in LDS and adds
+// each non-zero sum to one of kCounterStripes accumulator rows; then
+// kv_hot_apply_kernel folds the stripes into the row and clears them.
+constexpr uint32_t kHotLdsFields = 1024;
+
+__global__ __launch_bounds__(kBlock) void kv_hot_sum_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields, uint64_t hot,
+    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ acc)
+{
+    __shared__ uint32_t bins[kHotLdsFields];
+    const bool lds = fields <= kHotLdsFields;
+    if (lds)
+        for (uint32_t f = threadIdx.x; f < fields; f += kBlock) bins[f] = 0;
+    __syncthreads();
+    uint32_t *stripe = acc + (uint64_t)(blockIdx.x % kCounterStripes) * fields;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const bool committed = t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull);
+    if (committed) {
+        const uint32_t delta = tids[t] | 1u;
+        for (uint32_t s = 0; s < S; ++s) {
+            const Op o = load_op(keys, meta, (uint64_t)s * B + t, true, rows, fields);
+            if (!o.valid || o.kind != LTPG_KV_WRITE || o.row != hot) continue;
+            if (lds) atomicAdd(&bins[o.field], delta);
+            else atomicAdd(&stripe[o.field], delta);
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (uint32_t f = threadIdx.x; f < fields; f += kBlock)
+            if (bins[f]) atomicAdd(&stripe[f], bins[f]);
+}
+
+__global__ __launch_bounds__(kBlock) void kv_hot_apply_kernel(uint32_t *__restrict__ table, uint64_t hot,
+                                                             uint32_t fields, uint32_t fw, uint32_t fsw,
+                                                             uint32_t *__restrict__ acc)
+{
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= fields) return;
+    uint32_t total = 0;
+    for (int k = 0; k < kCounterStripes; ++k) {
+        total += acc[(uint64_t)k * fields + f];
+        acc[(uint64_t)k * fields + f] = 0;
+    }
+    if (total == 0) return;
+    uint32_t *w = table + (hot * fields + f) * (uint64_t)fsw;
+    for (uint32_t k = 0; k < fw; ++k) w[k] += total;
 }
 
 int lanes_for(uint32_t fw)
@@ -666,11 +718,11 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
         const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
         hipLaunchKernelGGL(kv_execute_ops_kernel, dim3((unsigned)((ops + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
-                           reg_layout(kv), e.tag, kv.doomed, e.epoch, ctr);
+                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, ctr);
     } else {
         hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
-                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), e.tag, kv.doomed,
-                           e.epoch, ctr);
+                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
+                           kv.doomed, e.epoch, ctr);
     }
     return hipGetLastError();
 }
@@ -702,13 +754,13 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
     if (counts) {
         hipLaunchKernelGGL(kv_check_counting_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys,
                            b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv),
-                           e.tag, kv.bitmap, kv.conflicts, ctr);
+                           kv.hot_row, e.tag, kv.bitmap, kv.conflicts, ctr);
         return hipGetLastError();
     }
 #define CALL1(GV, CHV)                                                                               \
     hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,     \
-                       kv.field_stride, kv.regs, reg_layout(kv), e.tag, kv.doomed, e.epoch, kv.table, \
+                       kv.field_stride, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, kv.table, \
                        kv.read_values, kv.bitmap, ctr)
 #define CALL(GV)                    \
     if (kv.small_chunks)            \
@@ -741,7 +793,7 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 #define CALL1(GV, CHV)                                                                                      \
     hipLaunchKernelGGL((kv_writeback_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,             \
-                       kv.field_stride, kv.table, kv.bitmap, values)
+                       kv.field_stride, kv.table, kv.bitmap, values, kv.hot_row)
 #define CALL(GV)         \
     if (kv.small_chunks) \
         CALL1(GV, 64);   \
@@ -750,6 +802,12 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
     LTPG_G_SWITCH(group_for(b.n_slots), CALL)
 #undef CALL
 #undef CALL1
+    if (kv.hot_row != LTPG_NO_HOT_ROW && values == LTPG_VALUES_RMW) {
+        hipLaunchKernelGGL(kv_hot_sum_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
+                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.hot_row, kv.bitmap, kv.hot_acc);
+        hipLaunchKernelGGL(kv_hot_apply_kernel, grid_for(kv.fields), dim3(kBlock), 0, s, kv.table, kv.hot_row,
+                           kv.fields, kv.field_words, kv.field_stride, kv.hot_acc);
+    }
     return hipGetLastError();
 }
 

@@ -33,20 +33,21 @@ typedef struct oracle_kv_result {
  * u32), updated in place.  Batch is slot-major: slot s of txn t at
  * keys/meta[s*B + t].  values: 0 = copy-back, 1 = RMW (+ (TID|1) per word).
  * reg_scratch must hold 2*rows u64 (R and W registers of priority pairs
- * TID << 32 | batch position).
+ * TID << 32 | batch position).  hot_row: UINT64_MAX, or the row whose ops
+ * never register or conflict and whose committed writes add commutatively.
  */
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
-                    oracle_kv_result *res);
+                    oracle_kv_result *res, uint64_t hot_row);
 
 /* The four phases separately (used by tests that pin each one). */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                        uint64_t *reg_r, uint64_t *reg_w);
+                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row);
 void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                      const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts);
+                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row);
 uint32_t oracle_commit(uint32_t B, const uint32_t *conflicts, uint64_t *bitmap);
 void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                          uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,

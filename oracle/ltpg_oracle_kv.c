@@ -25,6 +25,15 @@
  * A slot whose meta kind is NONE does nothing.  A key >= rows is counted in
  * range_errors and the op does nothing.
  *
+ * Hot row (the reference's ZIPFIAN hot key, row 0: skipped by __REGISTER
+ * and __CHECK, Genericfunction.h:74-79,88-93,114-119,128-133, and by
+ * writeback_w, Execute.h:202; its writes go to the reduce, Execute.h:63-73,
+ * 207-224, Genericfunction.h:160-210, which never reaches the table).  Here
+ * ops on hot_row never register and never conflict; a committed read of it
+ * returns the snapshot; committed writes to it apply their RMW delta as a
+ * commutative add (the same value rule as every other write).  hot_row =
+ * UINT64_MAX: no hot row.
+ *
  * Priority.  The reference compares TIDs only and relies on them being
  * distinct (TID = (epoch & 0xfff) << 20 | seq, TPCC-allopt-final/Query.h:
  * 500).  Here priority is the pair (TID, batch position) in lexicographic
@@ -61,13 +70,13 @@ static uint64_t prio(const uint32_t *tids, uint32_t t) { return ((uint64_t)tids[
  * the priority pair. */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                        uint64_t *reg_r, uint64_t *reg_w)
+                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row)
 {
     for (uint32_t s = 0; s < S; s++) {
         for (uint32_t t = 0; t < B; t++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
-            if (!op_valid(meta[i], row, rows, fields)) continue;
+            if (!op_valid(meta[i], row, rows, fields) || row == hot_row) continue;
             uint64_t *reg = (KIND(meta[i]) == K_READ) ? reg_r : reg_w;
             if (prio(tids, t) < reg[row]) reg[row] = prio(tids, t);
         }
@@ -79,7 +88,7 @@ void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
  * W (waw). */
 void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                      const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts)
+                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row)
 {
     for (uint32_t t = 0; t < B; t++) {
         uint32_t raw = 0, war = 0, waw = 0;
@@ -87,7 +96,7 @@ void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
         for (uint32_t s = 0; s < S; s++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
-            if (!op_valid(meta[i], row, rows, fields)) continue;
+            if (!op_valid(meta[i], row, rows, fields) || row == hot_row) continue;
             if (KIND(meta[i]) == K_READ) {
                 if (me > reg_w[row]) raw++;
             } else {
@@ -141,7 +150,7 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
-                    oracle_kv_result *res)
+                    oracle_kv_result *res, uint64_t hot_row)
 {
     uint64_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
     uint32_t errors = 0;
@@ -150,13 +159,13 @@ int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32
     memset(reg_scratch, 0xff, (size_t)rows * 2 * sizeof(uint64_t));
 
     /* execute: register */
-    oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w);
+    oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, hot_row);
     for (uint64_t i = 0; i < (uint64_t)S * B; i++) {
         uint32_t k = KIND(meta[i]);
         if (!op_valid(meta[i], keys[i], rows, fields) && (k == K_READ || k == K_WRITE)) errors++;
     }
 
-    oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts);
+    oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts, hot_row);
     res->commit_count = oracle_commit(B, res->conflicts, res->commit_bitmap);
 
     /* committed reads see the snapshot: stage them before any write-back */

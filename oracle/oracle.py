@@ -34,7 +34,7 @@ def _load(path=None):
     lib.oracle_kv_epoch.restype = ctypes.c_int
     lib.oracle_kv_epoch.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                     ctypes.c_uint32, ctypes.c_uint32, u64p, u32p, u32p, ctypes.c_uint32,
-                                    u64p, ctypes.POINTER(KvResult)]
+                                    u64p, ctypes.POINTER(KvResult), ctypes.c_uint64]
     lib.KvResult = KvResult
 
     class TpccResult(ctypes.Structure):
@@ -95,8 +95,11 @@ class KvEpoch:
         return bool((int(self.bitmap[t // 64]) >> (t % 64)) & 1)
 
 
+NO_HOT_ROW = (1 << 64) - 1
+
+
 def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values=1, want_reads=True,
-             inplace=False):
+             inplace=False, hot_row=NO_HOT_ROW):
     """Run one epoch on a copy of `table` (row-major u32), or on `table`
     itself when inplace=True; return KvEpoch."""
     if inplace:
@@ -117,7 +120,7 @@ def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values
     regs = np.empty(rows * 2, dtype=np.uint64)
     res = _lib.KvResult(_u64p(bitmap), _u32p(conflicts), _u32p(reads) if want_reads else None, 0, 0)
     rc = _lib.oracle_kv_epoch(rows, fields, field_words, _u32p(table), B, S, _u64p(keys), _u32p(meta),
-                              _u32p(tids), values, _u64p(regs), ctypes.byref(res))
+                              _u32p(tids), values, _u64p(regs), ctypes.byref(res), hot_row)
     assert rc == 0
     return KvEpoch(bitmap[: (B + 63) // 64], conflicts[:B],
                    reads[: B * S * field_words] if want_reads else None,

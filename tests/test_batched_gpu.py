@@ -13,8 +13,8 @@ KV = dict(kv_rows=200_000, kv_fields=4, kv_field_bytes=16, kv_max_slots=24, max_
 W = 2
 
 
-def _contexts():
-    kv = L.Context(workload=L.WORKLOAD_KV, **KV)
+def _contexts(hot_row=L.NO_HOT_ROW):
+    kv = L.Context(workload=L.WORKLOAD_KV, kv_hot_row=hot_row, **KV)
     kv.kv_fill_reference()
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=6000)
     tp.tpcc_populate(SEED)
@@ -58,8 +58,8 @@ def _tables(kv, tp):
     return [kv.kv_table_read()] + [tp.tpcc_table_read(t) for t in range(9)]
 
 
-def _run(mode, n, runs, kinds=None):
-    kv, tp = _contexts()
+def _run(mode, n, runs, kinds=None, hot_row=L.NO_HOT_ROW):
+    kv, tp = _contexts(hot_row)
     descs, keep, meta = _batches(np.random.default_rng(42), n, kv, tp, kinds)
     r = L.Batched(kv, tp, descs)
     maps = []
@@ -93,6 +93,19 @@ def test_graph_replays_match_across_an_epoch_tag_wrap(gpu):
     np.testing.assert_array_equal(plain_tabs[0], graph_tabs[0])
 
 
+def test_graph_form_with_hot_row_matches_plain_loop(gpu):
+    """The hot-row reduce's stripes are cleared inside each epoch, so graph
+    replays give the plain loop's results."""
+    plain_maps, plain_tabs = _run(L.RUN_PLAIN, 24, 2, hot_row=0)
+    graph_maps, graph_tabs = _run(L.RUN_GRAPH, 24, 2, hot_row=0)
+    for rp, rg in zip(plain_maps, graph_maps):
+        for bp, bg in zip(rp, rg):
+            for x, y in zip(bp, bg):
+                np.testing.assert_array_equal(x, y)
+    for x, y in zip(plain_tabs, graph_tabs):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_plain_epochs_after_a_graph_run(gpu):
     """A context used by a graph run keeps working epoch by epoch."""
     kv, tp = _contexts()
@@ -111,14 +124,15 @@ def test_plain_epochs_after_a_graph_run(gpu):
     r.close()
 
 
+@pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
 @pytest.mark.parametrize("flags", range(8))
-def test_kv_launch_shapes_match_oracle(gpu, flags):
+def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row):
     from oracle import oracle
     from tests import kvcases as kc
 
     rows, fields, fw, B, S = 5000, 3, 2, 3000, 12
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw, max_txns=B,
-                    kv_max_slots=S)
+                    kv_max_slots=S, kv_hot_row=hot_row)
     table = kc.initial_table(rows, fields, fw)
     ctx.kv_table_load(table)
     ctx.kv_set_tuning(flags)
@@ -128,7 +142,7 @@ def test_kv_launch_shapes_match_oracle(gpu, flags):
         ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run()
         ctx.sync()
-        ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, inplace=True)
+        ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, inplace=True, hot_row=hot_row)
         bm = ctx.commit_bitmap()
         np.testing.assert_array_equal(bm, ora.bitmap)
         com = np.unpackbits(bm.view(np.uint8), bitorder="little")[:B].astype(bool)

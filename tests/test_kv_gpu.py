@@ -16,10 +16,10 @@ pytestmark = pytest.mark.gpu
 KAT = kc.load_kat()
 
 
-def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True):
+def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True, hot_row=L.NO_HOT_ROW):
     return L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                      max_txns=max_txns, kv_max_slots=max_slots, values=values,
-                     flags=L.FLAG_CONFLICT_COUNTS if counts else 0)
+                     flags=L.FLAG_CONFLICT_COUNTS if counts else 0, kv_hot_row=hot_row)
 
 
 def _gpu_epoch(ctx, keys, meta, tids, S, device=False):
@@ -64,7 +64,8 @@ def test_gpu_known_answers(gpu, case, counts):
     keys, meta, tids, S = kc.case_arrays(case)
     values = L.VALUES_RMW if case["values"] == "rmw" else L.VALUES_COPYBACK
     B = len(tids)
-    with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts) as ctx:
+    hot = case.get("hot_row", L.NO_HOT_ROW)
+    with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts, hot_row=hot) as ctx:
         ctx.kv_table_load(kc.initial_table(rows, fields, fw))
         ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run(L.PHASE_ALL)
@@ -116,6 +117,36 @@ def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct,
         ctx.kv_table_load(table0)
         g = _gpu_epoch(ctx, keys, meta, tids, S, device=device)
     _assert_epoch(g, ora, meta, tids, fw)
+
+
+@pytest.mark.parametrize("hot_row", [0, 17])
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts):
+    """Ops on the hot row never register or conflict; committed writes to it
+    add (TID|1) per word through the striped reduce.  Several epochs in place
+    at Zipf 0.99 (row 0 is the most drawn key), 100-byte fields and a field
+    count above the reduce's LDS bins per block pass."""
+    rows, fields, fw, B, S = 20_000, 10, 25, 8000, 16
+    rng = np.random.default_rng(hot_row + 3)
+    table = rng.integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64).astype(np.uint32)
+    with _ctx(rows, fields, fw, B, S, counts=counts, hot_row=hot_row) as ctx:
+        ctx.kv_table_load(table)
+        for ep in range(4):
+            n = B - ep * 1500
+            keys, meta, tids = L.kv_generate(seed=70 + ep, rows=rows, fields=fields, n_txns=n, n_slots=S,
+                                             min_ops=1, max_ops=S, theta=0.99, tid_base=ep * B)
+            g = _gpu_epoch(ctx, keys, meta, tids, S)
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1, hot_row=hot_row)
+            _assert_epoch(g, ora, meta, tids, fw)
+            hot_ops = (keys.reshape(S, n) == hot_row) & ((meta.reshape(S, n) & 3) != 0)
+            assert hot_ops.any()
+            table = ora.table
+
+
+def test_gpu_hot_row_beyond_table_rejected(gpu):
+    with pytest.raises(L.LtpgError) as ei:
+        _ctx(100, 2, 1, 10, 2, hot_row=100)
+    assert ei.value.code == L.ERR_INVALID
 
 
 def test_gpu_config1_reference_table(gpu):

@@ -23,7 +23,8 @@ def test_oracle_known_answers(case):
     rows, fields, fw = case["rows"], case["fields"], case["field_words"]
     keys, meta, tids, S = kc.case_arrays(case)
     values = 1 if case["values"] == "rmw" else 0
-    ep = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values)
+    ep = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values,
+                         hot_row=case.get("hot_row", oracle.NO_HOT_ROW))
     B = len(tids)
     exp = case["expect"]
     assert kc.bitmap_to_list(ep.bitmap, B) == exp["commit"]
@@ -165,7 +166,7 @@ def test_pins_catch_planted_mistakes(mistake):
     replay, so the pins above would catch it in the oracle."""
     caught = False
     for case in KAT["cases"]:
-        if case["values"] != "rmw" or case["expect"]["range_errors"]:
+        if case["values"] != "rmw" or case["expect"]["range_errors"] or "hot_row" in case:
             continue
         rows, fields, fw = case["rows"], case["fields"], case["field_words"]
         keys, meta, tids, S = kc.case_arrays(case)
