@@ -48,14 +48,20 @@ def kv_workload(name, theta):
     raise SystemExit(f"unknown workload {name}")
 
 
-KERNELS = ["execute", "check", "readout", "writeback"]
+KERNELS = ["execute", "check", "writeback"]  # the fast CHECK includes the read-out
 
 
 def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw):
-    """Algorithmic bytes each key-value kernel must move for one batch: every
-    byte the algorithm has to read or write, once ('Algorithmic
-    bytes').  The fast CHECK's early exit is simulated exactly: a surviving
-    transaction visits slots until it has both a raw and a war conflict."""
+    """Algorithmic bytes each key-value phase must move for one batch (fast
+    form): every byte the algorithm has to read or write, once.
+    EXECUTE (write registration + read marking): the batch (tids, keys and
+    meta of every slot), one 8-byte W probe per valid op, 8 bytes per W
+    register written, one flag byte per doomed transaction and per raw / war
+    flag set.  CHECK (commit + read-out): doomed, raw and war flags, the
+    bitmap, then each committed transaction's ops and its read fields
+    (snapshot gather + read-value store).  WRITEBACK: the bitmap, each
+    committed transaction's tid and ops, and each committed write's field
+    (load + store)."""
     F = 4 * fw
     kind = (meta & 3).reshape(S, B)
     rows_ = keys.reshape(S, B).astype(np.int64)
@@ -63,26 +69,43 @@ def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw):
     prio = (tids.astype(np.int64) << 24) | np.arange(B, dtype=np.int64)
     big = np.iinfo(np.int64).max
     nrows = int(rows_.max()) + 1 if rows_.size else 1
-    Rmin = np.full(nrows, big, dtype=np.int64)
     Wmin = np.full(nrows, big, dtype=np.int64)
     pr = np.broadcast_to(prio, (S, B))
-    np.minimum.at(Rmin, rows_[is_r], pr[is_r])
     np.minimum.at(Wmin, rows_[is_w], pr[is_w])
     waw = (is_w & (Wmin[rows_] < pr)).any(axis=0)
-    raw_op = is_r & (Wmin[rows_] < pr)
-    war_op = is_w & (Rmin[rows_] < pr)
-    done = np.logical_or.accumulate(raw_op, axis=0) & np.logical_or.accumulate(war_op, axis=0)
-    before_done = np.vstack([np.zeros((1, B), bool), done[:-1]])  # slot s visited iff not done before s
-    visited = (~before_done) & ~waw[None, :]
+    raw = (is_r & (Wmin[rows_] < pr)).any(axis=0)
+    owner = Wmin[rows_] & ((1 << 24) - 1)
+    war_owners = np.unique(owner[is_r & (Wmin[rows_] < big) & (Wmin[rows_] > pr)])
     bits = np.unpackbits(bitmap.view(np.uint8), bitorder="little")[:B].astype(bool)
     committed = int(bits.sum())
     valid = int((is_r | is_w).sum())
-    regs_written = int((Rmin < big).sum() + (Wmin < big).sum())
-    execute = 4 * B + 12 * S * B + 8 * valid + 8 * regs_written + int(waw.sum())
-    check = 5 * B + B // 8 + 20 * int((visited & (is_r | is_w)).sum())
-    readout = B // 8 + committed * 12 * S + int((is_r & bits[None, :]).sum()) * 2 * F
+    w_regs = int((Wmin < big).sum())
+    execute = 4 * B + 12 * S * B + 8 * valid + 8 * w_regs + int(waw.sum()) + int(raw.sum()) + war_owners.size
+    check = 3 * B + B // 8 + committed * 12 * S + int((is_r & bits[None, :]).sum()) * 2 * F
     writeback = B // 8 + committed * (4 + 12 * S) + int((is_w & bits[None, :]).sum()) * 2 * F
-    return {"execute": execute, "check": check, "readout": readout, "writeback": writeback}, committed
+    return {"execute": execute, "check": check, "writeback": writeback}, committed
+
+
+def choose_kv_shape(ctx, batch, reps=3):
+    """Time the 8 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
+    block-local row folding, split register arrays, 64-transaction chunks)
+    on one batch, one warm-up and `reps` epochs each, and keep the fastest
+    for the run.  The timed loop is the plain per-batch loop either way."""
+    times = []
+    for flags in range(8):
+        ctx.kv_set_tuning(flags)
+        ctx.kv_submit_device(batch)
+        ctx.epoch_run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.kv_submit_device(batch)
+            ctx.epoch_run()
+        ctx.sync()
+        times.append((time.perf_counter() - t0) * 1e3 / reps)
+    best = int(np.argmin(times))
+    ctx.kv_set_tuning(best)
+    return best, times
 
 
 def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
@@ -139,6 +162,9 @@ def run_kv(args, theta, cpu_baseline=True):
         ctx.sync()
         per_batch.append(kv_kernel_bytes(gen[i][0], gen[i][1], gen[i][2], ctx.commit_bitmap(), B, S, fw))
 
+    shape, shape_ms = choose_kv_shape(ctx, dev[0]) if not args.default_shape else (0, [])
+    log(f"launch shape flags {shape}")
+
     for k in range(args.warmup):
         ctx.kv_submit_device(dev[k % P])
         ctx.epoch_run()
@@ -189,7 +215,10 @@ def run_kv(args, theta, cpu_baseline=True):
         "data": "synthetic",
         "config": {"workload": w["desc"], "rows": rows, "fields": fields, "field_bytes": 4 * fw,
                    "batch_txns": B, "ops_per_txn": S, "zipf_theta": w["theta"], "form": "plain per-batch",
-                   "inputs": "resident in HBM", "distinct_batches": P},
+                   "inputs": "resident in HBM", "distinct_batches": P,
+                   "launch_shape": {"flags": shape, "epoch_ms_by_flags": shape_ms,
+                                    "chosen_by": "fastest of the 8 LTPG_TUNE_* shapes on the first batch, "
+                                                 "before the timed region"}},
         "abort_rate": 1.0 - commits / max(attempts, 1),
         "attempted_txns": attempts,
         "committed_txns": commits,
@@ -631,6 +660,8 @@ def main():
                     help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--default-shape", action="store_true",
+                    help="key-value plain runs keep the default launch shape instead of the fastest of the 8")
     ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
                     help="only run the known-size copy used to calibrate traffic counters")
     args = ap.parse_args()

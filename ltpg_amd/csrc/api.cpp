@@ -30,6 +30,8 @@ void free_kv(KvState &kv)
     dev_free(kv.table);
     dev_free(kv.regs);
     dev_free(kv.doomed);
+    dev_free(kv.raw_flag);
+    dev_free(kv.war_flag);
     dev_free(kv.read_values);
     dev_free(kv.bitmap);
     dev_free(kv.conflicts);
@@ -64,6 +66,8 @@ int init_kv(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&kv.table, kv.rows * kv.fields * kv.field_stride, "kv table"))) return rc;
     if ((rc = dev_alloc(&kv.regs, kv.rows * 2, "kv registers"))) return rc;
     if ((rc = dev_alloc(&kv.doomed, B, "kv doomed flags"))) return rc;
+    if ((rc = dev_alloc(&kv.raw_flag, B, "kv raw flags"))) return rc;
+    if ((rc = dev_alloc(&kv.war_flag, B, "kv war flags"))) return rc;
     if ((rc = dev_alloc(&kv.read_values, S * B * kv.field_words, "kv read values"))) return rc;
     if ((rc = dev_alloc(&kv.bitmap, (B + 63) / 64, "kv bitmap"))) return rc;
     if ((rc = dev_alloc(&kv.conflicts, B, "kv conflicts"))) return rc;
@@ -78,6 +82,8 @@ int init_kv(ltpg_ctx *ctx)
     // current (epochs start at 1), so every row reads as "no registrant".
     LTPG_HIP_TRY(launch_fill_u64(kv.regs, kv.rows * 2, ~0ull, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u8(kv.doomed, B, 0, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(kv.raw_flag, B, 0, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(kv.war_flag, B, 0, ctx->stream));
     LTPG_HIP_TRY(hipMemsetAsync(kv.table, 0, kv.rows * kv.fields * kv.field_stride * 4, ctx->stream));
     LTPG_HIP_TRY(hipMemsetAsync(kv.bitmap, 0, (B + 63) / 64 * 8, ctx->stream));
     LTPG_HIP_TRY(hipMemsetAsync(kv.conflicts, 0, B * 4, ctx->stream));
@@ -169,6 +175,8 @@ int refill_registers(ltpg_ctx *ctx)
     if (ctx->kv.regs) {
         LTPG_HIP_TRY(launch_fill_u64(ctx->kv.regs, ctx->kv.rows * 2, ~0ull, ctx->stream));
         LTPG_HIP_TRY(launch_fill_u8(ctx->kv.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u8(ctx->kv.raw_flag, ctx->cfg.max_txns, 0, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u8(ctx->kv.war_flag, ctx->cfg.max_txns, 0, ctx->stream));
     }
     if (ctx->tpcc.d.regs) {
         LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.regs, ctx->tpcc.d.n_regs, ~0ull, ctx->stream));
@@ -667,7 +675,9 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     }
     // CID has no key-value work: its bracket is empty.
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
-    if (phase_mask & LTPG_PHASE_EXECUTE) LTPG_HIP_TRY(launch_kv_execute(kv, kv.cur, e, ctx->counters, s));
+    if (phase_mask & LTPG_PHASE_EXECUTE)
+        LTPG_HIP_TRY(launch_kv_execute(kv, kv.cur, e, (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS) != 0,
+                                       ctx->counters, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
     if (phase_mask & LTPG_PHASE_CHECK) {
         if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));

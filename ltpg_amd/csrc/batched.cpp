@@ -11,6 +11,15 @@
 // of its commit bitmap(s) into a device archive.  The key-value and the
 // order-entry graphs replay concurrently on their contexts' streams (the two
 // workloads share no table), with one host synchronisation for the lot.
+//
+// Key-value epochs are pipelined inside their graph.  EXECUTE (write
+// registration + read marking) touches only the registers, the flags and the
+// batch, never the table; CHECK and WRITEBACK touch the table.  With two
+// register sets (and two sets of doomed / raw / war flags) used by alternate
+// batches, EXECUTE of batch j+1 runs on a side stream while CHECK and
+// WRITEBACK of batch j run on the main stream; EXECUTE of batch j+2 waits
+// for CHECK of batch j, the last reader of that set.  Each batch keeps its
+// own epoch tag, so the result is the plain loop's, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -34,6 +43,11 @@ struct ltpg_batched {
     std::vector<uint8_t> tune;              // key-value batch i: LTPG_TUNE_* flags
     unsigned long long *stamps = nullptr;   // graph runs: device clock after batch i; [n + k] start of graph k
     std::vector<double> plain_ms;           // plain runs: host ms from the run's start to batch i's readback
+    // key-value pipelining: side stream for EXECUTE, second register / flag set
+    hipStream_t side = nullptr;
+    unsigned long long *regs2 = nullptr;
+    uint8_t *doomed2 = nullptr, *raw2 = nullptr, *war2 = nullptr;
+    std::vector<hipEvent_t> events;         // capture dependencies (kept until destroy)
     uint32_t last_mode = LTPG_RUN_PLAIN;
     bool ran = false;
 };
@@ -71,6 +85,92 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp)
     return LTPG_OK;
 }
 
+hipError_t new_event(ltpg_batched *b, hipEvent_t *e)
+{
+    const hipError_t r = hipEventCreateWithFlags(e, hipEventDisableTiming);
+    if (r == hipSuccess) b->events.push_back(*e);
+    return r;
+}
+
+// the second register set and its flags back to their initial state
+int refill_second_set(ltpg_batched *b, ltpg_ctx *ctx)
+{
+    LTPG_HIP_TRY(launch_fill_u64(b->regs2, ctx->kv.rows * 2, ~0ull, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(b->doomed2, ctx->cfg.max_txns, 0, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(b->raw2, ctx->cfg.max_txns, 0, ctx->stream));
+    LTPG_HIP_TRY(launch_fill_u8(b->war2, ctx->cfg.max_txns, 0, ctx->stream));
+    return LTPG_OK;
+}
+
+// main waits for everything issued so far on side, and side for main
+int join_streams(ltpg_batched *b, hipStream_t main, hipStream_t side)
+{
+    hipEvent_t a, c;
+    LTPG_HIP_TRY(new_event(b, &a));
+    LTPG_HIP_TRY(hipEventRecord(a, side));
+    LTPG_HIP_TRY(hipStreamWaitEvent(main, a, 0));
+    LTPG_HIP_TRY(new_event(b, &c));
+    LTPG_HIP_TRY(hipEventRecord(c, main));
+    LTPG_HIP_TRY(hipStreamWaitEvent(side, c, 0));
+    return LTPG_OK;
+}
+
+// The key-value epochs of the runner, pipelined over two streams (captured).
+int enqueue_kv_pipelined(ltpg_batched *b)
+{
+    ltpg_ctx *ctx = b->ctx[0];
+    hipStream_t main = ctx->stream, side = b->side;
+    int rc;
+    if ((rc = refill_second_set(b, ctx))) return rc;
+    if ((rc = join_streams(b, main, side))) return rc;  // side joins the capture after the refills
+    std::vector<hipEvent_t> checked;                    // per key-value batch: its CHECK is done
+    uint32_t j = 0;
+    for (uint32_t i = 0; i < b->d.size(); i++) {
+        const ltpg_batch_desc &d = b->d[i];
+        if (d.workload != LTPG_WORKLOAD_KV) continue;
+        if ((rc = ltpg_kv_set_tuning(ctx, b->tune[i]))) return rc;
+        if (ctx->epoch >= 255) {
+            // the submit below wraps the epoch and refills the first set on
+            // main: nothing may be in flight on side, and the second set
+            // starts over too
+            if ((rc = join_streams(b, main, side))) return rc;
+            if ((rc = refill_second_set(b, ctx))) return rc;
+        }
+        const uint32_t before = ctx->epoch;
+        if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
+        if (ctx->epoch < before) {
+            if ((rc = join_streams(b, main, side))) return rc;  // side waits for the refills
+            checked.assign(checked.size(), nullptr);            // older checks precede the refill
+        }
+        EpochTag e;
+        e.epoch = (uint8_t)ctx->epoch;
+        e.tag = 255u - ctx->epoch;
+        KvState st = ctx->kv;
+        if (j & 1) {
+            st.regs = b->regs2;
+            st.doomed = b->doomed2;
+            st.raw_flag = b->raw2;
+            st.war_flag = b->war2;
+        }
+        if (j >= 2 && checked[j - 2]) LTPG_HIP_TRY(hipStreamWaitEvent(side, checked[j - 2], 0));
+        LTPG_HIP_TRY(launch_kv_execute(st, st.cur, e, false, ctx->counters, side, false));
+        hipEvent_t executed, done;
+        LTPG_HIP_TRY(new_event(b, &executed));
+        LTPG_HIP_TRY(hipEventRecord(executed, side));
+        LTPG_HIP_TRY(hipStreamWaitEvent(main, executed, 0));
+        LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, main));
+        LTPG_HIP_TRY(launch_kv_check(st, st.cur, e, false, ctx->counters, main));
+        LTPG_HIP_TRY(new_event(b, &done));
+        LTPG_HIP_TRY(hipEventRecord(done, main));
+        checked.push_back(done);
+        LTPG_HIP_TRY(launch_kv_writeback(st, st.cur, ctx->cfg.values, main));
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i], ctx->kv.bitmap, words(d.kv.n_txns), main));
+        LTPG_HIP_TRY(launch_stamp(b->stamps + i, main));
+        j++;
+    }
+    return join_streams(b, main, side);  // side rejoins main before the capture ends
+}
+
 // Capture the epochs of workload slot k into a graph.
 int capture(ltpg_batched *b, int k)
 {
@@ -84,8 +184,12 @@ int capture(ltpg_batched *b, int k)
     ctx->epoch = 0;
     int rc = refill_registers(ctx);
     if (!rc && launch_stamp(b->stamps + b->d.size() + k, ctx->stream) != hipSuccess) rc = LTPG_ERR_DEVICE;
-    for (uint32_t i = 0; i < b->d.size() && !rc; i++)
-        if (slot_of(b->d[i].workload) == k) rc = enqueue_batch(b, i, true);
+    if (k == 0 && b->side) {
+        if (!rc) rc = enqueue_kv_pipelined(b);
+    } else {
+        for (uint32_t i = 0; i < b->d.size() && !rc; i++)
+            if (slot_of(b->d[i].workload) == k) rc = enqueue_batch(b, i, true);
+    }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     b->final_epoch[k] = ctx->epoch;
@@ -113,6 +217,13 @@ int ltpg_batched_destroy(ltpg_batched *b)
         if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
         if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
     }
+    if (b->side) (void)hipStreamSynchronize(b->side);
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    dev_free(b->regs2);
+    dev_free(b->doomed2);
+    dev_free(b->raw2);
+    dev_free(b->war2);
     dev_free(b->archive);
     dev_free(b->stamps);
     delete b;
@@ -155,6 +266,21 @@ int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_d
     if (!rc) rc = dev_alloc(&b->archive, w ? w : 1, "bitmap archive");
     if (!rc) rc = dev_alloc(&b->stamps, (uint64_t)n + 2, "batch clock stamps");
     b->plain_ms.assign(n, 0.0);
+    bool any_kv = false;
+    for (const auto &d : b->d) any_kv |= d.workload == LTPG_WORKLOAD_KV;
+    if (!rc && any_kv && !(kv_ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)) {
+        // the counting form has no read marking to pipeline; it runs unpipelined
+        const uint64_t B = kv_ctx->cfg.max_txns;
+        if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess) {
+            b->side = nullptr;
+            set_error("hipStreamCreateWithFlags failed");
+            rc = LTPG_ERR_DEVICE;
+        }
+        if (!rc) rc = dev_alloc(&b->regs2, kv_ctx->kv.rows * 2, "second register set");
+        if (!rc) rc = dev_alloc(&b->doomed2, B, "second doomed flags");
+        if (!rc) rc = dev_alloc(&b->raw2, B, "second raw flags");
+        if (!rc) rc = dev_alloc(&b->war2, B, "second war flags");
+    }
     for (int k = 0; k < 2 && !rc; k++) {
         bool any = false;
         for (const auto &d : b->d) any |= slot_of(d.workload) == k;

@@ -245,16 +245,24 @@ __global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__r
 // only decreases, so even a stale cached value is >= its current value); on
 // hot rows this turns a serialized chain of same-address atomics into
 // cached loads.
+//
+// Reads register R only for the counting form.  The fast form needs war
+// only for transactions without waw, i.e. for the minimum writer of each row
+// they write, and a reader with a lower priority than that writer can flag
+// it directly from the W register (kv_mark_reads_kernel).  Random 64-bit
+// atomics run at about half the rate of random loads on this part, so
+// leaving read ops to a load-only pass halves the atomics of an epoch.
 __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ doomed, uint8_t epoch, DeviceCounters *__restrict__ ctr)
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts,
+    DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
     // the epoch's commit counts start at zero (CHECK adds to them)
-    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
         ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
     const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     bool waw = false;
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
         if (!o.valid || o.row == hot) continue;  // the hot row never registers
         if (o.kind == LTPG_KV_WRITE)
             waw |= register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch);
-        else
+        else if (reg_reads)
             register_read(&regs[rl.r(o.row)], me);
     }
     if (waw) doomed[t] = epoch;
@@ -279,9 +287,10 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ doomed, uint8_t epoch, DeviceCounters *__restrict__ ctr)
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts,
+    DeviceCounters *__restrict__ ctr)
 {
-    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
         ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < (uint64_t)S * B;
@@ -293,11 +302,134 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
         const unsigned long long me = priority(tag, tids[t], t);
         if (o.kind == LTPG_KV_WRITE) {
             if (register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch)) doomed[t] = epoch;
-        } else {
+        } else if (reg_reads) {
             register_read(&regs[rl.r(o.row)], me);
         }
     }
     if (bad) atomicAdd(&ctr->range_errors, 1u);
+}
+
+// EXECUTE, fast form, one lane per op with the block's duplicate rows
+// folded first.  Under skew thousands of in-flight writes hit the same hot
+// register at once, all see it stale, and their atomics serialize on one
+// address.  Here each block takes kDedupOps consecutive ops (kDedupPer per lane),
+// finds the minimum writer priority of every row among them in an LDS hash
+// table, and only that writer goes to the global register; the block's
+// other writers of the row already have waw (a lower writer exists).  Every
+// non-minimum writer of a row is still doomed exactly once.
+constexpr uint32_t kDedupPer = 2;
+constexpr uint32_t kDedupOps = kDedupPer * kBlock;
+constexpr uint32_t kDedupSlots = 2 * kDedupOps;  // load factor <= 1/2: probing always ends
+
+__global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool zero_counts, DeviceCounters *__restrict__ ctr)
+{
+    __shared__ unsigned long long s_key[kDedupSlots];  // row + 1, 0 = empty
+    __shared__ unsigned long long s_min[kDedupSlots];
+    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
+        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    for (uint32_t k = threadIdx.x; k < kDedupSlots; k += kBlock) {
+        s_key[k] = 0ull;
+        s_min[k] = ~0ull;
+    }
+    __syncthreads();
+    const uint64_t n_ops = (uint64_t)S * B;
+    const uint64_t base = (uint64_t)blockIdx.x * kDedupOps;
+    uint64_t row[kDedupPer];
+    unsigned long long me[kDedupPer];
+    uint32_t tt[kDedupPer], slot[kDedupPer];
+    bool wr[kDedupPer];
+    uint32_t bad = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        const bool live = i < n_ops;
+        const uint64_t s = live ? i / B : 0;
+        const uint32_t t = (uint32_t)(i - s * B);
+        const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
+        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
+        wr[k] = o.valid && o.kind == LTPG_KV_WRITE && o.row != hot;
+        row[k] = o.row;
+        tt[k] = t;
+        me[k] = wr[k] ? priority(tag, tids[t], t) : 0ull;
+        slot[k] = 0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k) {
+        if (!wr[k]) continue;
+        const unsigned long long key = row[k] + 1ull;
+        uint32_t h = (uint32_t)((row[k] * 0x9E3779B97F4A7C15ull) >> 40) & (kDedupSlots - 1);
+        while (true) {
+            const unsigned long long prev = atomicCAS(&s_key[h], 0ull, key);
+            if (prev == 0ull || prev == key) break;
+            h = (h + 1) & (kDedupSlots - 1);
+        }
+        slot[k] = h;
+        atomicMin(&s_min[h], me[k]);
+    }
+    __syncthreads();
+    // register_write for the block minima, split into phases so that the
+    // four probes, then the four atomics, are in flight together
+    bool lead[kDedupPer], waw[kDedupPer];
+    unsigned long long cur[kDedupPer], old[kDedupPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k) {
+        waw[k] = wr[k] && s_min[slot[k]] < me[k];
+        lead[k] = wr[k] && !waw[k];
+        cur[k] = lead[k] ? probe(&regs[rl.w(row[k])]) : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k) {
+        old[k] = ~0ull;
+        if (lead[k] && cur[k] > me[k]) old[k] = atomicMin(&regs[rl.w(row[k])], me[k]);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k) {
+        if (!lead[k]) continue;
+        if (cur[k] < me[k] || old[k] < me[k]) {
+            waw[k] = true;
+        } else if (cur[k] > me[k] && old[k] > me[k] && old[k] != ~0ull &&
+                   (uint32_t)(old[k] >> kTagShift) == tag) {
+            doomed[old[k] & kPosMask] = epoch;  // displaced: that writer has waw
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kDedupPer; ++k)
+        if (waw[k]) doomed[tt[k]] = epoch;
+    if (bad) atomicAdd(&ctr->range_errors, bad);
+}
+
+// Read marking (fast form), one lane per op after every write has
+// registered.  A read of row x by t against w = W[x] of this epoch:
+//   w < me: t reads a row a higher-priority transaction writes -> raw(t);
+//   w > me: the row's minimum writer (position bits of w) writes a row a
+//           higher-priority transaction reads -> war for that writer.
+// A writer with waw aborts anyway, so flagging only the minimum writer
+// gives every transaction that can still commit its exact war bit; raw(t)
+// is exact for every t.  Flags hold the epoch number, so equal stores race
+// harmlessly and nothing is cleared between epochs.
+__global__ __launch_bounds__(kBlock) void kv_mark_reads_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
+    const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
+    uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < (uint64_t)S * B;
+    const uint64_t s = live ? i / B : 0;
+    const uint32_t t = (uint32_t)(i - s * B);
+    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
+    if (!o.valid || o.kind != LTPG_KV_READ || o.row == hot) return;
+    const unsigned long long w = regs[rl.w(o.row)];
+    if ((uint32_t)(w >> kTagShift) != tag) return;  // no writer this epoch
+    const unsigned long long me = priority(tag, tids[t], t);
+    if (w < me)
+        raw_flag[t] = epoch;
+    else if (w > me)
+        war_flag[w & kPosMask] = epoch;
 }
 
 // CHECK + COMMIT, counting form (check_r / check_w_0 / check_w_1 and
@@ -427,69 +559,57 @@ __device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t sr
 }
 
 // CHECK + COMMIT + read-out, fast form (rules as kv_check_counting_kernel).
-// Transactions doomed by waw in EXECUTE abort without any further work;
-// the others need raw (reads vs W) and war (writes vs R) only, OR-reduced
-// over the transaction's lanes with ballots.  A committed transaction
-// gathers its read fields right away: no write-back has run yet, so the
-// table is the snapshot (read_1 / __READ, YCSB/0/Execute.h:27-37).  Commit
-// bits collect in LDS and leave as whole bitmap words.
+// EXECUTE and the read marking have decided everything: a transaction
+// commits iff it has no waw (doomed) and not both raw and war.  A block
+// compacts its committed transactions into an LDS list and gathers their
+// read fields right away, G lanes per transaction: no write-back has run
+// yet, so the table is the snapshot (read_1 / __READ, YCSB/0/Execute.h:
+// 27-37).  Commit bits collect in LDS and leave as whole bitmap words.
 template <int G, int CH>
 __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
-    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
-    const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
-    uint32_t fw, uint32_t fsw, const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot,
-    uint32_t tag, const uint8_t *__restrict__ doomed, uint8_t epoch, const uint32_t *__restrict__ table,
-    uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, uint32_t B, uint32_t S,
+    uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint8_t *__restrict__ doomed,
+    const uint8_t *__restrict__ raw_flag, const uint8_t *__restrict__ war_flag, uint8_t epoch,
+    const uint32_t *__restrict__ table, uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
     DeviceCounters *__restrict__ ctr)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ unsigned long long s_bits[CH / 64];
-    __shared__ uint32_t s_n, s_commits;
+    __shared__ uint32_t s_n;
     const int lane = lane_id();
     const uint32_t chunk0 = blockIdx.x * CH;
-    if (threadIdx.x == 0) s_n = s_commits = 0;
-    if (threadIdx.x < CH / 64) s_bits[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     {
         const uint32_t t = chunk0 + threadIdx.x;
-        list_append(threadIdx.x < CH && t < B && doomed[t] != epoch, t, s_list, &s_n);
+        const bool in = threadIdx.x < CH && t < B;
+        const bool commit =
+            in && doomed[t] != epoch && !(raw_flag[t] == epoch && war_flag[t] == epoch);
+        const unsigned long long m = __ballot(commit);
+        if (threadIdx.x < CH && lane == 0) s_bits[threadIdx.x / 64] = m;
+        list_append(commit, t, s_list, &s_n);
     }
     __syncthreads();
     const uint32_t n = s_n;
     constexpr uint32_t PER = 64 / G;  // transactions per wave per round
     const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
-    const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (grp * G);
-    uint32_t commits = 0;
     for (uint32_t j0 = (threadIdx.x / 64) * PER; j0 < n; j0 += (kBlock / 64) * PER) {
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
-        const unsigned long long me = priority(tag, has ? tids[t] : 0u, t);
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
-        // a read consults W (raw), a write consults R (war): one load
-        const bool cold = o.valid && o.row != hot;  // the hot row never conflicts
-        const unsigned long long rv = regs[cold ? (o.kind == LTPG_KV_READ ? rl.w(o.row) : rl.r(o.row)) : 0ull];
-        const bool raw = cold && o.kind == LTPG_KV_READ && rv < me;
-        const bool war = cold && o.kind == LTPG_KV_WRITE && rv < me;
-        const bool commit = has && !((__ballot(raw) & gmask) && (__ballot(war) & gmask));
-        const unsigned long long rd = __ballot(commit && o.valid && o.kind == LTPG_KV_READ);
+        const unsigned long long rd = __ballot(o.valid && o.kind == LTPG_KV_READ);
         if (rd)
             group_copy<G, 8>(rd, (o.row * fields + o.field) * fsw, i * fw, 0u, table, read_values, fw, S,
                              grp, gl);
-        if (commit && gl == 0) {
-            atomicOr(&s_bits[(t - chunk0) / 64], 1ull << ((t - chunk0) % 64));
-            ++commits;
-        }
     }
-    if (commits) atomicAdd(&s_commits, commits);
-    __syncthreads();
     const uint32_t nwords = (B + 63) / 64;
     if (threadIdx.x < CH / 64 && chunk0 / 64 + threadIdx.x < nwords)
         bitmap[chunk0 / 64 + threadIdx.x] = s_bits[threadIdx.x];
     if (threadIdx.x == 0) {
         const uint32_t live_n = (B - chunk0) < (uint32_t)CH ? (B - chunk0) : (uint32_t)CH;
-        add_counts(ctr, blockIdx.x % kCounterStripes, LTPG_TYPE_KV, s_commits, live_n);
+        add_counts(ctr, blockIdx.x % kCounterStripes, LTPG_TYPE_KV, n, live_n);
     }
 }
 
@@ -710,20 +830,29 @@ RegLayout reg_layout(const KvState &kv)
     return rl;
 }
 
-hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e,
-                             DeviceCounters *ctr, hipStream_t s)
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
+                             DeviceCounters *ctr, hipStream_t s, bool zero_counts)
 {
     if (b.n_txns == 0) return hipSuccess;
-    if (kv.exec_per_op) {
-        const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
-        hipLaunchKernelGGL(kv_execute_ops_kernel, dim3((unsigned)((ops + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                           b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
-                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, ctr);
+    const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
+    const dim3 op_grid((unsigned)((ops + kBlock - 1) / kBlock));
+    if (kv.exec_per_op && !counts) {
+        hipLaunchKernelGGL(kv_execute_dedup_kernel, dim3((unsigned)((ops + kDedupOps - 1) / kDedupOps)), dim3(kBlock),
+                           0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
+                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, ctr);
+    } else if (kv.exec_per_op) {
+        hipLaunchKernelGGL(kv_execute_ops_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
+                           b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed,
+                           e.epoch, counts, zero_counts, ctr);
     } else {
         hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
                            b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.doomed, e.epoch, ctr);
+                           kv.doomed, e.epoch, counts, zero_counts, ctr);
     }
+    if (!counts)
+        hipLaunchKernelGGL(kv_mark_reads_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
+                           b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
+                           kv.raw_flag, kv.war_flag, e.epoch);
     return hipGetLastError();
 }
 
@@ -759,9 +888,8 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
     }
 #define CALL1(GV, CHV)                                                                               \
     hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
-                       b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,     \
-                       kv.field_stride, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, kv.table, \
-                       kv.read_values, kv.bitmap, ctr)
+                       b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words, kv.field_stride,   \
+                       kv.doomed, kv.raw_flag, kv.war_flag, e.epoch, kv.table, kv.read_values, kv.bitmap, ctr)
 #define CALL(GV)                    \
     if (kv.small_chunks)            \
         CALL1(GV, 64);              \

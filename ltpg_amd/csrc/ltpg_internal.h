@@ -68,6 +68,8 @@ struct KvState {
     uint32_t *table = nullptr;          // rows * fields * field_stride, row-major
     unsigned long long *regs = nullptr; // 2 * rows: [R, W] per row, epoch-tagged
     uint8_t *doomed = nullptr;          // max_txns: epoch number if waw > 0
+    uint8_t *raw_flag = nullptr;        // max_txns: epoch number if a read found a lower W (fast form)
+    uint8_t *war_flag = nullptr;        // max_txns: epoch number if a lower reader hit its W (fast form)
     uint32_t *read_values = nullptr;    // max_slots * max_txns * field_words
     uint64_t *bitmap = nullptr;         // ceil(max_txns / 64)
     uint32_t *conflicts = nullptr;      // max_txns
@@ -219,8 +221,8 @@ hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, 
                          uint32_t fsw, int to_strided, hipStream_t s);
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                int to_row_major, hipStream_t s);
-hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e,
-                             DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
+                             DeviceCounters *ctr, hipStream_t s, bool zero_counts = true);
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s);

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -129,6 +130,7 @@ struct ltpg_server {
     std::vector<uint32_t> pay_data;  // kPayFields per Payment
     Lane lane[2];                    // [0] key-value, [1] order-entry
     std::mutex mu;
+    std::atomic<int> waiting{0};     // API calls waiting for mu (the loop yields to them)
     std::condition_variable cv;
     std::deque<ltpg_serve_done> done;
     std::unordered_map<uint64_t, std::vector<uint32_t>> reads;
@@ -144,6 +146,22 @@ struct ltpg_server {
 };
 
 namespace {
+
+// The serving thread holds mu while it forms, launches and completes
+// epochs, and re-takes it at once; a plain lock from an API call could wait
+// behind thousands of epochs.  API calls announce themselves in `waiting`
+// and the loop steps aside until they have the lock.
+struct ClientLock {
+    std::unique_lock<std::mutex> lk;
+    explicit ClientLock(ltpg_server *sv) : lk(lock_for(sv)) {}
+    static std::unique_lock<std::mutex> lock_for(ltpg_server *sv)
+    {
+        sv->waiting.fetch_add(1);
+        std::unique_lock<std::mutex> l(sv->mu);
+        sv->waiting.fetch_sub(1);
+        return l;
+    }
+};
 
 void free_flight(Flight &f)
 {
@@ -417,6 +435,10 @@ void serve_loop(ltpg_server *sv)
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(inflight ? 5 : 50));
             lk.lock();
+        } else if (sv->waiting.load() > 0) {
+            lk.unlock();
+            while (sv->waiting.load() > 0) std::this_thread::yield();
+            lk.lock();
         }
     }
     sv->cv.notify_all();
@@ -439,7 +461,7 @@ int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped)
 {
     if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
     {
-        std::lock_guard<std::mutex> g(sv->mu);
+        ClientLock g(sv);
         sv->cancel = true;  // the loop drops what is pending, finishes its
         sv->stop = true;    // epochs in flight and returns
     }
@@ -452,7 +474,7 @@ int ltpg_serve_stop(ltpg_server *sv)
 {
     if (!sv) return LTPG_OK;
     {
-        std::lock_guard<std::mutex> g(sv->mu);
+        ClientLock g(sv);
         sv->stop = true;
     }
     if (sv->th.joinable()) sv->th.join();
@@ -532,7 +554,7 @@ int ltpg_serve_submit_kv(ltpg_server *sv, uint32_t n, const uint64_t *keys, cons
     if (!sv->lane[0].ctx) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server has no key-value context");
     const uint64_t t = now_ns();
     const uint32_t S = sv->cfg.kv_ops;
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     if (first_id) *first_id = sv->reqs.size();
     for (uint32_t i = 0; i < n; i++) {
         const uint64_t data = sv->kv_keys.size() / S;
@@ -558,7 +580,7 @@ int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *no, const
                   !pay->C_W_ID || !pay->H_AMOUNT || !pay->H_ID))
         LTPG_FAIL(LTPG_ERR_INVALID, "bad Payment arrays");
     const uint64_t t = now_ns();
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     if (first_id) *first_id = sv->reqs.size();
     Lane &L = sv->lane[1];
     for (uint32_t q = 0; q < n_no; q++) {
@@ -585,7 +607,7 @@ int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *no, const
 int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out)
 {
     if (!sv || (!out && max)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     uint32_t n = 0;
     while (n < max && !sv->done.empty()) {
         out[n++] = sv->done.front();
@@ -599,7 +621,7 @@ int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_
 int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words)
 {
     if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     if (!sv->cfg.keep_reads) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server does not keep read values");
     auto it = sv->reads.find(id);
     if (it == sv->reads.end()) LTPG_FAIL(LTPG_ERR_STATE, "request has not committed");
@@ -613,7 +635,7 @@ int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *i
 {
     if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
     if (workload != LTPG_WORKLOAD_KV && workload != LTPG_WORKLOAD_TPCC) LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     if (!sv->cfg.record_epochs) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server does not record epochs");
     const Lane &L = sv->lane[workload == LTPG_WORKLOAD_KV ? 0 : 1];
     if (e >= L.log.size()) LTPG_FAIL(LTPG_ERR_INVALID, "no such epoch");
@@ -626,8 +648,8 @@ int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *i
 int ltpg_serve_drain(ltpg_server *sv)
 {
     if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    sv->cv.wait(lk, [&] { return sv->outstanding == 0 || sv->error; });
+    ClientLock g(sv);
+    sv->cv.wait(g.lk, [&] { return sv->outstanding == 0 || sv->error; });
     if (sv->error) LTPG_FAIL(sv->error, sv->error_msg);
     return LTPG_OK;
 }
@@ -635,7 +657,7 @@ int ltpg_serve_drain(ltpg_server *sv)
 int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st)
 {
     if (!sv || !st) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> g(sv->mu);
+    ClientLock g(sv);
     for (int k = 0; k < 3; k++) {
         st->committed[k] = sv->committed[k];
         st->attempts[k] = sv->attempts[k];
