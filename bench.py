@@ -199,6 +199,19 @@ def run_kv(args, theta, cpu_baseline=True):
     dom = max(KERNELS, key=lambda k: avg_ms[k])
     achieved = bytes_avg[dom] / (avg_ms[dom] * 1e-3) / 1e9
     kernel_gbs = {k: bytes_avg[k] / (avg_ms[k] * 1e-3) / 1e9 for k in KERNELS if avg_ms[k] > 0}
+    # EXECUTE is bound by random register accesses, not bytes: one probe +
+    # atomicMin per write op and one W load per read op, against the rates
+    # this device sustains for them over an array of the registers' size
+    load_gops, atomic_gops = L.random_access_rates(ctx, rows * 16, 1 << 24)
+    n_w = sum(int(((gen[j % P][1] & 3) == 2).sum()) for j in range(args.steps)) / args.steps
+    n_r = sum(int(((gen[j % P][1] & 3) == 1).sum()) for j in range(args.steps)) / args.steps
+    bound_ms = n_w / (atomic_gops * 1e6) + n_r / (load_gops * 1e6)
+    random_access = {"kernel": "kv_execute", "write_ops": n_w, "read_ops": n_r,
+                     "probe_atomic_gops": atomic_gops, "load_gops": load_gops,
+                     "bound_ms": bound_ms, "measured_ms": avg_ms["execute"],
+                     "frac": bound_ms / avg_ms["execute"] if avg_ms["execute"] > 0 else None,
+                     "what": "write ops / probe+atomicMin rate + read ops / load rate, rates measured live "
+                             "over an array the size of the registers (ltpg_random_access_rates)"}
 
     result = {
         "metric": METRIC,
@@ -229,7 +242,7 @@ def run_kv(args, theta, cpu_baseline=True):
         "kernel_algorithmic_bytes": bytes_avg,
         "roofline": {"kernel": f"kv_{dom}", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                     "algorithmic_bytes_per_launch": bytes_avg[dom]},
+                     "algorithmic_bytes_per_launch": bytes_avg[dom], "random_access": random_access},
     }
     for d in dev:
         d.free()
@@ -450,18 +463,36 @@ def run_batched_c4(args):
                 out.append(np.concatenate([runner.bitmap(i, L.TYPE_NEWORDER), runner.bitmap(i, L.TYPE_PAYMENT)]))
         return out
 
-    def timed(mode):
-        runner.run(mode)  # warm-up pass
+    phases = {}
+
+    def timed(mode, label=None):
+        # warm-up pass; for the plain loop it also yields the per-phase
+        # kernel times (HIP events), kept out of the wall-clock pass
+        if label:
+            for c in (kv, tp):
+                c.set_timing(True)
+                c.phase_times()
+        runner.run(mode)
+        if label:
+            phases[label] = {}
+            for name, c in (("key_value", kv), ("order_entry", tp)):
+                ms, _ = c.phase_times()
+                c.set_timing(False)
+                phases[label][name] = {L.TIMING_NAMES[i]: float(ms[i]) for i in range(L.TIMING_SLOTS)}
         t0 = time.perf_counter()
         runner.run(mode)
         return time.perf_counter() - t0, runner.times(), bitmaps()
 
     log("C4: plain per-batch loop (default launch shapes)")
-    plain_s, plain_t, plain_bm = timed(L.RUN_PLAIN)
+    plain_s, plain_t, plain_bm = timed(L.RUN_PLAIN, "plain_default_shape")
     log("C4: autotuning launch shapes")
     tuning = runner.autotune(reps=3)
     log("C4: plain per-batch loop with the tuned launch shapes")
-    tuned_s, _, _ = timed(L.RUN_PLAIN)
+    tuned_s, _, _ = timed(L.RUN_PLAIN, "plain_tuned_shapes")
+    if args.c4_plain_only:  # for kernel traces: the profiler cannot follow graph launches
+        runner.close()
+        return {"plain_ms": plain_s * 1e3, "plain_tuned_ms": tuned_s * 1e3, "phase_ms_per_pass": phases,
+                "autotuner": tuning}
     log("C4: batched form (graphs, two streams, tuned shapes)")
     graph_s, graph_t, graph_bm = timed(L.RUN_GRAPH)
     same = all(np.array_equal(a, b) for a, b in zip(plain_bm, graph_bm))
@@ -483,6 +514,7 @@ def run_batched_c4(args):
             "what": "completion time of each batch from the start of the run (all batches arrive at once)",
             "plain_p50": float(np.percentile(plain_t, 50)), "plain_p99": float(np.percentile(plain_t, 99)),
             "batched_p50": float(np.percentile(graph_t, 50)), "batched_p99": float(np.percentile(graph_t, 99))},
+        "phase_ms_per_pass": phases,
         "autotuner": tuning,
         "config": {"workload": desc, "batches": nb, "kv_batches": int(is_kv.sum()), "seed": seed,
                    "form": "batched (hipGraph per workload, concurrent streams, autotuned) vs plain loop",
@@ -656,6 +688,8 @@ def main():
     ap.add_argument("--c5-budget-mb", type=int, default=4096, help="HBM budget per in-flight epoch, MiB")
     ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form variant of the default run")
+    ap.add_argument("--c4-plain-only", action="store_true",
+                    help="C4: run the plain legs only (kernel traces; rocprofv3 cannot follow graph launches)")
     ap.add_argument("--theta", type=float, default=None,
                     help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")

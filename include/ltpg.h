@@ -192,6 +192,14 @@ int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
  * aligned).  Used to snapshot tables and as the known-size copy that
  * calibrates HBM traffic counters. */
 int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* Random-access ceilings of this device (no counterpart in the reference:
+ * the roofline the register phases are measured against).  Over a scratch
+ * array of `bytes` (rounded down to 8) it times n_ops random 8-byte loads,
+ * then n_ops random probe + 64-bit atomicMin pairs (the registration step
+ * of EXECUTE), each spread over the whole array, and returns the rates in
+ * G ops/s.  Synchronous; allocates and frees the array. */
+int ltpg_random_access_rates(ltpg_ctx *ctx, uint64_t bytes, uint64_t n_ops, double *load_gops,
+                             double *atomic_gops);
 
 /* ---- key-value tables (YCSB/0/Database.h:14-131) ------------------------ */
 /* Load rows x kv_fields x kv_field_bytes.  COLUMN_MAJOR requires

@@ -91,6 +91,15 @@ def default_config(**kw):
     return cfg
 
 
+def random_access_rates(ctx, nbytes, n_ops=1 << 24):
+    """(load, probe+atomicMin) random 8-byte access rates of the device in
+    G ops/s over an array of nbytes (ltpg_random_access_rates)."""
+    lo, at = ctypes.c_double(), ctypes.c_double()
+    _check(lib.ltpg_random_access_rates(ctx.handle, int(nbytes), int(n_ops), ctypes.byref(lo), ctypes.byref(at)),
+           "ltpg_random_access_rates")
+    return lo.value, at.value
+
+
 class DeviceBuffer:
     """A library-owned device allocation (for LTPG_MEM_DEVICE batches)."""
 

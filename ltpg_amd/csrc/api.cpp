@@ -483,6 +483,50 @@ int ltpg_copy_to_host(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     return LTPG_OK;
 }
 
+int ltpg_random_access_rates(ltpg_ctx *ctx, uint64_t bytes, uint64_t n_ops, double *load_gops,
+                             double *atomic_gops)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    const uint64_t len = bytes / 8;
+    if (len == 0 || n_ops == 0) LTPG_FAIL(LTPG_ERR_INVALID, "bytes and n_ops must be non-zero");
+    if (n_ops > (1ull << 36)) LTPG_FAIL(LTPG_ERR_INVALID, "n_ops too large");
+    unsigned long long *a = nullptr, *sink = nullptr;
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    float ms[2] = {0.f, 0.f};
+    if ((rc = dev_alloc(&a, len, "calibration array"))) return rc;
+    if ((rc = dev_alloc(&sink, 1, "calibration sink"))) {
+        dev_free(a);
+        return rc;
+    }
+    hipError_t h = launch_fill_u64(a, len, ~0ull, ctx->stream);
+    for (int k = 0; k < 3 && h == hipSuccess; k++) h = hipEventCreate(&e[k]);
+    // warm-up pass of each, then the timed pair (the atomic pass needs fresh
+    // all-ones contents so that every probe goes on to its atomic)
+    if (h == hipSuccess) h = launch_calibration(a, len, n_ops, 0, sink, ctx->stream);
+    if (h == hipSuccess) h = hipEventRecord(e[0], ctx->stream);
+    if (h == hipSuccess) h = launch_calibration(a, len, n_ops, 0, sink, ctx->stream);
+    if (h == hipSuccess) h = hipEventRecord(e[1], ctx->stream);
+    if (h == hipSuccess) h = launch_fill_u64(a, len, ~0ull, ctx->stream);
+    if (h == hipSuccess) h = hipEventRecord(e[2], ctx->stream);
+    if (h == hipSuccess) h = launch_calibration(a, len, n_ops, 1, sink, ctx->stream);
+    hipEvent_t e3 = nullptr;
+    if (h == hipSuccess) h = hipEventCreate(&e3);
+    if (h == hipSuccess) h = hipEventRecord(e3, ctx->stream);
+    if (h == hipSuccess) h = hipEventSynchronize(e3);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms[0], e[0], e[1]);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms[1], e[2], e3);
+    for (int k = 0; k < 3; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    if (e3) (void)hipEventDestroy(e3);
+    dev_free(a);
+    dev_free(sink);
+    if (h != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("calibration: ") + hipGetErrorString(h));
+    if (load_gops) *load_gops = ms[0] > 0 ? n_ops / (ms[0] * 1e6) : 0.0;
+    if (atomic_gops) *atomic_gops = ms[1] > 0 ? n_ops / (ms[1] * 1e6) : 0.0;
+    return LTPG_OK;
+}
+
 int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 {
     int rc = check_ctx(ctx);

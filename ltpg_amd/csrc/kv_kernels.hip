@@ -182,6 +182,43 @@ __global__ void copy_u64_kernel(uint64_t *__restrict__ dst, const uint64_t *__re
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Random-access calibration: n random 8-byte loads, or n random probe +
+// atomicMin pairs, over a[0..len).  The sink store is never taken for the
+// all-ones contents (it keeps the loads from being optimised away).
+__device__ __forceinline__ uint64_t calib_index(uint64_t i, uint64_t len)
+{
+    i ^= i >> 33;
+    i *= 0xff51afd7ed558ccdull;
+    i ^= i >> 33;
+    i *= 0xc4ceb9fe1a85ec53ull;
+    i ^= i >> 33;
+    return i % len;
+}
+
+__global__ void calib_load_kernel(const unsigned long long *a, uint64_t len, uint64_t n, uint32_t per,
+                                  unsigned long long *sink)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * per;
+    unsigned long long acc = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (base + k < n) acc += a[calib_index(base + k, len)];
+    if (acc == 1ull) sink[0] = acc;
+}
+
+__global__ void calib_atomic_kernel(unsigned long long *a, uint64_t len, uint64_t n, uint32_t per,
+                                    unsigned long long *sink)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * per;
+    unsigned long long acc = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        if (base + k >= n) break;
+        unsigned long long *p = &a[calib_index(base + k, len)];
+        const unsigned long long me = base + k;
+        if (probe(p) > me) acc += atomicMin(p, me);
+    }
+    if (acc == 1ull) sink[0] = acc;
+}
+
 // Device wall-clock stamp (constant-rate counter), one lane.
 __global__ void stamp_kernel(unsigned long long *out) { out[threadIdx.x] = wall_clock64(); }
 
@@ -783,6 +820,19 @@ hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipSt
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(copy_u64_kernel, grid_stride(n), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_calibration(unsigned long long *a, uint64_t len, uint64_t n, int atomics,
+                              unsigned long long *sink, hipStream_t s)
+{
+    constexpr uint32_t per = 16;
+    const uint64_t threads = (n + per - 1) / per;
+    const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
+    if (atomics)
+        hipLaunchKernelGGL(calib_atomic_kernel, grid, dim3(kBlock), 0, s, a, len, n, per, sink);
+    else
+        hipLaunchKernelGGL(calib_load_kernel, grid, dim3(kBlock), 0, s, a, len, n, per, sink);
     return hipGetLastError();
 }
 

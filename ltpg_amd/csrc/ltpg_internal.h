@@ -207,6 +207,8 @@ inline int check_ctx(ltpg_ctx *ctx)
 // Kernel launchers (kv_kernels.hip).  All asynchronous on `s`.
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
+hipError_t launch_calibration(unsigned long long *a, uint64_t len, uint64_t n, int atomics,
+                              unsigned long long *sink, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s);
 hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipStream_t s);
