@@ -542,6 +542,10 @@ def run_served_c5(args):
     kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                    max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW)
     kv.kv_fill_reference()
+    # served epochs are small (the server sizes them by their commits under
+    # contention): per-op registration with row folding and 64-transaction
+    # chunks, the shape the batched-form autotuner picks below 2^17 txns
+    kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS)
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=1 << 20, values=L.VALUES_RMW)
     tp.tpcc_populate(seed)
     log("C5: tables resident")
@@ -619,7 +623,8 @@ def run_served_c5(args):
     return {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
             "latency_ms": best["latency_ms"], "rates": out,
             "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
-                       "slots": 2, "form": "served (continuous admission, requeue with original TID)"}}
+                       "slots": 2, "epoch_growth": 8, "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS",
+                       "form": "served (continuous admission, requeue with original TID)"}}
 
 
 def _tpcc_select(b, idx):

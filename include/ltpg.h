@@ -458,6 +458,12 @@ typedef struct ltpg_serve_config {
                                  slots + read values) admitted per slot; 0 = no cap */
     uint32_t keep_reads;      /* 1: keep committed requests' read values        */
     uint32_t record_epochs;   /* 1: keep every epoch's member list (tests)       */
+    uint32_t epoch_growth;    /* > 0: an epoch admits at most epoch_growth x the
+                                 commits of the workload's last epoch (at least
+                                 1024) -- under contention epochs commit a short
+                                 chain whatever their size; 0 = always the cap.
+                                 Default 8. */
+    uint32_t reserved;
 } ltpg_serve_config;
 
 typedef struct ltpg_serve_done {

@@ -257,9 +257,11 @@ __global__ void repack_kernel(const uint32_t *__restrict__ src, uint32_t *__rest
     }
 }
 
-// Column-major <-> row-major for u32 tables (rows x cols).
-__global__ void transpose_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                 uint64_t rows, uint32_t cols, int to_row_major)
+// Column-major <-> row-major for u32 tables (rows x cols); with T = u64 also
+// the request-major -> slot-major transpose of served key-value epochs.
+template <typename T>
+__global__ void transpose_kernel(const T *__restrict__ src, T *__restrict__ dst, uint64_t rows, uint32_t cols,
+                                 int to_row_major)
 {
     const uint64_t n = rows * cols;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -867,8 +869,18 @@ hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, 
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                int to_row_major, hipStream_t s)
 {
-    hipLaunchKernelGGL(transpose_kernel, grid_stride(rows * cols), dim3(kBlock), 0, s, src, dst,
-                       rows, cols, to_row_major);
+    hipLaunchKernelGGL(transpose_kernel<uint32_t>, grid_stride(rows * cols), dim3(kBlock), 0, s, src, dst, rows,
+                       cols, to_row_major);
+    return hipGetLastError();
+}
+
+hipError_t launch_slot_major(const uint64_t *keys_rm, const uint32_t *meta_rm, uint64_t *keys, uint32_t *meta,
+                             uint32_t n_txns, uint32_t n_slots, hipStream_t s)
+{
+    if ((uint64_t)n_txns * n_slots == 0) return hipSuccess;
+    const dim3 g = grid_stride((uint64_t)n_txns * n_slots);
+    hipLaunchKernelGGL(transpose_kernel<uint64_t>, g, dim3(kBlock), 0, s, keys_rm, keys, (uint64_t)n_txns, n_slots, 0);
+    hipLaunchKernelGGL(transpose_kernel<uint32_t>, g, dim3(kBlock), 0, s, meta_rm, meta, (uint64_t)n_txns, n_slots, 0);
     return hipGetLastError();
 }
 

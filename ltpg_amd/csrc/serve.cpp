@@ -54,6 +54,7 @@ uint64_t words(uint64_t n) { return (n + 63) / 64; }
 
 constexpr uint32_t kNoFields = 52;  // NewOrder request: 7 head fields + 3 x 15 line fields
 constexpr uint32_t kPayFields = 9;
+constexpr uint64_t kAdaptiveFloor = 256;  // smallest epoch the commit-based cap allows
 
 struct Req {
     uint64_t id;
@@ -96,8 +97,8 @@ struct Flight {
     hipEvent_t copied = nullptr, done = nullptr;
     bool used = false;
     // key-value staging (pinned) and device slot
-    uint64_t *h_keys = nullptr, *d_keys = nullptr;
-    uint32_t *h_meta = nullptr, *d_meta = nullptr, *h_tids = nullptr, *d_tids = nullptr;
+    uint64_t *h_keys = nullptr, *d_keys = nullptr, *d_keys_rm = nullptr;
+    uint32_t *h_meta = nullptr, *d_meta = nullptr, *d_meta_rm = nullptr, *h_tids = nullptr, *d_tids = nullptr;
     // order-entry staging (pinned, packed query blocks) and device slot
     uint32_t *h_no = nullptr, *d_no = nullptr, *h_pay = nullptr, *d_pay = nullptr;
     // results (pinned)
@@ -108,7 +109,8 @@ struct Flight {
 struct Lane {                        // one workload
     ltpg_ctx *ctx = nullptr;
     hipStream_t copy = nullptr;
-    uint32_t cap = 0;                // transactions per epoch
+    uint32_t cap = 0;                // transactions per epoch (budget)
+    uint64_t last_commits = ~0ull;   // commits of the last completed epoch (none yet: ~0)
     std::deque<uint64_t> fresh;      // arrival order
     std::priority_queue<uint64_t, std::vector<uint64_t>, ByTid> requeue;
     std::vector<Flight> flights;
@@ -178,6 +180,8 @@ void free_flight(Flight &f)
     host_free(f.h_reads2);
     dev_free(f.d_keys);
     dev_free(f.d_meta);
+    dev_free(f.d_keys_rm);
+    dev_free(f.d_meta_rm);
     dev_free(f.d_tids);
     dev_free(f.d_no);
     dev_free(f.d_pay);
@@ -198,6 +202,8 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
         if (sv->cfg.keep_reads && (rc = host_alloc(&f.h_reads, S * C * fw))) return rc;
         if ((rc = dev_alloc(&f.d_keys, S * C, "serving key slot")) ||
             (rc = dev_alloc(&f.d_meta, S * C, "serving meta slot")) ||
+            (rc = dev_alloc(&f.d_keys_rm, S * C, "serving key staging")) ||
+            (rc = dev_alloc(&f.d_meta_rm, S * C, "serving meta staging")) ||
             (rc = dev_alloc(&f.d_tids, C, "serving tid slot")))
             return rc;
     } else {
@@ -220,11 +226,18 @@ void take_members(ltpg_server *sv, int w, Flight &f)
 {
     Lane &L = sv->lane[w];
     f.members.clear();
-    while (f.members.size() < L.cap && !L.requeue.empty()) {
+    // Under contention an epoch commits a short chain of the lowest
+    // priorities whatever its size, so a large epoch mostly re-runs aborts:
+    // cap it at epoch_growth times the last epoch's commits (not below
+    // kAdaptiveFloor), so that epochs turn over faster.
+    uint64_t cap = L.cap;
+    if (sv->cfg.epoch_growth && L.last_commits != ~0ull)
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(kAdaptiveFloor, sv->cfg.epoch_growth * L.last_commits));
+    while (f.members.size() < cap && !L.requeue.empty()) {
         f.members.push_back(L.requeue.top());
         L.requeue.pop();
     }
-    while (f.members.size() < L.cap && !L.fresh.empty()) {
+    while (f.members.size() < cap && !L.fresh.empty()) {
         f.members.push_back(L.fresh.front());
         L.fresh.pop_front();
     }
@@ -248,20 +261,22 @@ int launch(ltpg_server *sv, int w, Flight &f)
     if (f.used) LTPG_HIP_TRY(hipStreamWaitEvent(L.copy, f.done, 0));
     f.used = true;
     if (w == 0) {
+        // requests are stored request-major; the staging keeps that layout
+        // (one contiguous copy per request) and the device transposes it
+        // into the slot-major batch
         const uint32_t S = sv->cfg.kv_ops, B = (uint32_t)f.members.size();
         for (uint32_t t = 0; t < B; t++) {
             const Req &r = sv->reqs[f.members[t]];
             f.h_tids[t] = r.tid;
-            for (uint32_t s = 0; s < S; s++) {
-                f.h_keys[(uint64_t)s * B + t] = sv->kv_keys[r.data * S + s];
-                f.h_meta[(uint64_t)s * B + t] = sv->kv_meta[r.data * S + s];
-            }
+            std::memcpy(f.h_keys + (uint64_t)t * S, &sv->kv_keys[r.data * S], S * sizeof(uint64_t));
+            std::memcpy(f.h_meta + (uint64_t)t * S, &sv->kv_meta[r.data * S], S * sizeof(uint32_t));
         }
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_keys, f.h_keys, (uint64_t)S * B * 8, hipMemcpyHostToDevice, L.copy));
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_meta, f.h_meta, (uint64_t)S * B * 4, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_keys_rm, f.h_keys, (uint64_t)S * B * 8, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_meta_rm, f.h_meta, (uint64_t)S * B * 4, hipMemcpyHostToDevice, L.copy));
         LTPG_HIP_TRY(hipMemcpyAsync(f.d_tids, f.h_tids, (uint64_t)B * 4, hipMemcpyHostToDevice, L.copy));
         LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
         LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
+        LTPG_HIP_TRY(launch_slot_major(f.d_keys_rm, f.d_meta_rm, f.d_keys, f.d_meta, B, S, ctx->stream));
         ltpg_kv_batch b = {B, S, f.d_keys, f.d_meta, f.d_tids};
         if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE))) return rc;
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
@@ -326,6 +341,7 @@ void complete(ltpg_server *sv, int w, Flight &f)
     const uint64_t t_now = now_ns();
     const uint32_t B = (uint32_t)f.members.size();
     const uint32_t S = sv->cfg.kv_ops;
+    uint64_t commits = 0;
     for (uint32_t t = 0; t < B; t++) {
         const uint64_t ri = f.members[t];
         Req &r = sv->reqs[ri];
@@ -341,6 +357,7 @@ void complete(ltpg_server *sv, int w, Flight &f)
         }
         sv->committed[r.type]++;
         sv->outstanding--;
+        commits++;
         ltpg_serve_done d;
         d.id = r.id;
         d.type = r.type;
@@ -369,6 +386,7 @@ void complete(ltpg_server *sv, int w, Flight &f)
             sv->reads[r.id] = std::move(v);
         }
     }
+    L.last_commits = commits;
     f.busy = false;
 }
 
@@ -455,6 +473,7 @@ void ltpg_serve_config_init(ltpg_serve_config *c)
     c->size = sizeof(*c);
     c->slots = 2;
     c->kv_ops = 16;
+    c->epoch_growth = 8;
 }
 
 int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped)
