@@ -104,16 +104,26 @@ extern "C" {
 #define LTPG_PHASE_ALL 15u
 
 /* ---- key-value op encoding (meta word of one op slot) ------------------ */
-/* bits 0-1: kind; bits 8-23: field index.  A slot of kind NONE is empty
- * (ragged transactions).  A READ registers R and stages its field; a WRITE
- * is a read-modify-write of its field that registers W (YCSB/0/Execute.h:
- * 11-78). */
+/* bits 0-1: kind; bits 8-23: field index; bits 24-31: scan length - 1.  A
+ * slot of kind NONE is empty (ragged transactions).  A READ registers R and
+ * stages its field; a WRITE is a read-modify-write of its field that
+ * registers W (YCSB/0/Execute.h:11-78).  A SCAN reads the field of rows
+ * key .. key + len - 1, clipped at the end of the table: every row is a read
+ * for registration and conflicts, and the rows' fields are staged in order
+ * (the range reads of YCSB/2/Execute.h:21-32,126-134, whose check consults
+ * row key + 1 for every row and whose staging keeps only the last row; here
+ * each row is checked and delivered).  Scans need ltpg_config.kv_max_scan >=
+ * len; a longer scan is a range error. */
 #define LTPG_KV_NONE 0u
 #define LTPG_KV_READ 1u
 #define LTPG_KV_WRITE 2u
+#define LTPG_KV_SCAN 3u
 #define LTPG_KV_META(kind, field) ((uint32_t)(kind) | ((uint32_t)(field) << 8))
+#define LTPG_KV_SCAN_META(field, len) (LTPG_KV_SCAN | ((uint32_t)(field) << 8) | (((uint32_t)(len) - 1u) << 24))
 #define LTPG_KV_KIND(meta) ((meta) & 3u)
 #define LTPG_KV_FIELD(meta) (((meta) >> 8) & 0xffffu)
+#define LTPG_KV_SCAN_LEN(meta) (((meta) >> 24) + 1u)
+#define LTPG_KV_MAX_SCAN 256u
 
 #define LTPG_NO_HOT_ROW UINT64_MAX
 
@@ -155,7 +165,9 @@ typedef struct ltpg_config {
                              /* through the commutative reduce                 */
     /* order-entry (TPCC-allopt-final/Predefine.h:9) */
     uint32_t tpcc_warehouses;
-    uint32_t reserved1;
+    uint32_t kv_max_scan;    /* longest key-value scan (0: no scans; <= 256);
+                                each op slot's read values hold
+                                max(1, kv_max_scan) fields                    */
     uint64_t hbm_budget;     /* serving: bytes of access-set buffers admitted  */
 } ltpg_config;
 
@@ -363,10 +375,12 @@ int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out);
  * Needs LTPG_FLAG_CONFLICT_COUNTS (else LTPG_ERR_UNSUPPORTED). */
 int ltpg_conflict_counters(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint32_t n);
 /* Read values of the last key-value epoch (set_local_set,
- * YCSB/0/Genericfunction.h:31-45): field words of slot s of transaction t at
- * out[((s * B) + t) * field_words + w], taken from the snapshot (the table
- * before the epoch's write-back).  Only read slots of committed transactions
- * are written: an aborted transaction's reads are discarded. */
+ * YCSB/0/Genericfunction.h:31-45): with L = max(1, kv_max_scan), word w of
+ * row i (0 for a READ, 0 .. len - 1 for a SCAN) of slot s of transaction t
+ * at out[(((s * B) + t) * L + i) * field_words + w], taken from the snapshot
+ * (the table before the epoch's write-back).  Only read and scan slots of
+ * committed transactions are written, and of a scan only the rows inside
+ * the table: an aborted transaction's reads are discarded. */
 int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words);
 /* Running totals since the last reset: committed and attempted (checked)
  * transactions of `type`, kept on the device so a pipelined epoch loop needs
@@ -503,7 +517,8 @@ int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *neworder,
 /* Up to max completed (committed) requests, oldest first. */
 int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out);
 /* Read values of a committed request (keep_reads): key-value: the field
- * words of its read ops in op order; NewOrder 51 / Payment 24 values as
+ * words of its read ops, and of each scan's rows inside the table, in op
+ * order; NewOrder 51 / Payment 24 values as
  * ltpg_tpcc_read_values lays them out for one transaction. */
 int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words);
 /* Request ids of serving epoch e of a workload, in batch order (record_epochs). */
@@ -535,6 +550,8 @@ typedef struct ltpg_kv_gen {
     uint32_t tid_base;       /* tids[t] = tid_base + t                         */
     uint32_t distinct_keys;  /* 1: keys within one transaction are distinct    */
     uint64_t txn_base;       /* global index of transaction 0 (stream offset)  */
+    uint32_t scan_len;       /* > 0: reads are SCANs of this length (YCSB/2)   */
+    uint32_t reserved;
 } ltpg_kv_gen;
 
 void ltpg_kv_gen_init(ltpg_kv_gen *g);

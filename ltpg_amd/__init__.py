@@ -40,7 +40,7 @@ LAYOUT_ROW_MAJOR, LAYOUT_COLUMN_MAJOR = 0, 1
 WORKLOAD_KV, WORKLOAD_TPCC = 1, 2
 TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT = 0, 1, 2
 PHASE_CID, PHASE_EXECUTE, PHASE_CHECK, PHASE_WRITEBACK, PHASE_ALL = 1, 2, 4, 8, 15
-KV_NONE, KV_READ, KV_WRITE = 0, 1, 2
+KV_NONE, KV_READ, KV_WRITE, KV_SCAN = 0, 1, 2, 3
 NO_HOT_ROW = (1 << 64) - 1
 FLAG_CONFLICT_COUNTS = 1
 TIMING_SLOTS = 5
@@ -59,6 +59,11 @@ TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_ALL = 1, 2, 4, 7
 def kv_meta(kind, field):
     """LTPG_KV_META(kind, field)."""
     return (int(kind) | (int(field) << 8)) & 0xFFFFFFFF
+
+
+def kv_scan_meta(field, length):
+    """LTPG_KV_SCAN_META(field, length): a scan of `length` rows (1..256)."""
+    return (KV_SCAN | (int(field) << 8) | ((int(length) - 1) << 24)) & 0xFFFFFFFF
 
 
 class LtpgError(RuntimeError):
@@ -339,10 +344,11 @@ class Context:
         return self.conflict_counters(type_, self._n_no if type_ == TYPE_NEWORDER else self._n_pay)
 
     def kv_read_values(self):
-        """Read values of committed transactions' read slots (slot-major,
-        field_words per op); other slots are left as they were."""
+        """Read values of committed transactions' read and scan slots
+        (slot-major, max(1, kv_max_scan) fields of field_words per op);
+        other entries are left as they were."""
         _, _, fw = self.kv_shape
-        n = self._cur_txns * self._cur_slots * fw
+        n = self._cur_txns * self._cur_slots * fw * max(1, int(self.config.kv_max_scan))
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib.ltpg_kv_read_values(self.handle, out.ctypes.data_as(_lib.c_u32p), n), "ltpg_kv_read_values")
         return out[:n]
@@ -607,10 +613,12 @@ class Server:
 
 def kv_generate(seed=0x4C545047, rows=1_000_000, fields=10, n_txns=65536, n_slots=10, min_ops=None,
                 max_ops=None, read_pct=50, theta=0.8, tid_base=0, distinct_keys=True, txn_base=0,
-                threads=0):
-    """Seeded key-value batch (slot-major keys/meta, tids) from libltpg's generator."""
+                threads=0, scan_len=0):
+    """Seeded key-value batch (slot-major keys/meta, tids) from libltpg's
+    generator; with scan_len > 0 the reads are scans of that many rows."""
     g = KvGen()
     lib.ltpg_kv_gen_init(ctypes.byref(g))
+    g.scan_len = scan_len
     g.seed, g.rows, g.fields, g.n_txns, g.n_slots = seed, rows, fields, n_txns, n_slots
     g.min_ops = n_slots if min_ops is None else min_ops
     g.max_ops = n_slots if max_ops is None else max_ops

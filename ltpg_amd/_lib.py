@@ -32,7 +32,7 @@ class Config(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("kv_hot_row", ctypes.c_uint64),
         ("tpcc_warehouses", ctypes.c_uint32),
-        ("reserved1", ctypes.c_uint32),
+        ("kv_max_scan", ctypes.c_uint32),
         ("hbm_budget", ctypes.c_uint64),
     ]
 
@@ -65,6 +65,8 @@ class KvGen(ctypes.Structure):
         ("tid_base", ctypes.c_uint32),
         ("distinct_keys", ctypes.c_uint32),
         ("txn_base", ctypes.c_uint64),
+        ("scan_len", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -61,6 +61,9 @@ int init_kv(ltpg_ctx *ctx)
     // field read touches the fewest 64-byte memory segments.
     kv.field_stride = kv.field_words >= 16 ? (kv.field_words + 15) / 16 * 16 : kv.field_words;
     kv.max_slots = c.kv_max_slots;
+    if (c.kv_max_scan > LTPG_KV_MAX_SCAN) LTPG_FAIL(LTPG_ERR_INVALID, "kv_max_scan must be at most 256");
+    kv.max_scan = c.kv_max_scan;
+    kv.read_slot = c.kv_max_scan > 1 ? c.kv_max_scan : 1;
     const uint64_t B = c.max_txns, S = c.kv_max_slots;
     int rc;
     if ((rc = dev_alloc(&kv.table, kv.rows * kv.fields * kv.field_stride, "kv table"))) return rc;
@@ -68,7 +71,7 @@ int init_kv(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&kv.doomed, B, "kv doomed flags"))) return rc;
     if ((rc = dev_alloc(&kv.raw_flag, B, "kv raw flags"))) return rc;
     if ((rc = dev_alloc(&kv.war_flag, B, "kv war flags"))) return rc;
-    if ((rc = dev_alloc(&kv.read_values, S * B * kv.field_words, "kv read values"))) return rc;
+    if ((rc = dev_alloc(&kv.read_values, S * B * kv.read_slot * kv.field_words, "kv read values"))) return rc;
     if ((rc = dev_alloc(&kv.bitmap, (B + 63) / 64, "kv bitmap"))) return rc;
     if ((rc = dev_alloc(&kv.conflicts, B, "kv conflicts"))) return rc;
     if ((rc = dev_alloc(&kv.keys_buf, S * B, "kv keys"))) return rc;
@@ -824,7 +827,7 @@ int ltpg_kv_read_values(ltpg_ctx *ctx, uint32_t *out, uint64_t n_words)
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
     KvState &kv = ctx->kv;
     if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    const uint64_t have = (uint64_t)kv.cur.n_slots * kv.cur.n_txns * kv.field_words;
+    const uint64_t have = (uint64_t)kv.cur.n_slots * kv.cur.n_txns * kv.read_slot * kv.field_words;
     if (n_words > have) LTPG_FAIL(LTPG_ERR_INVALID, "n_words exceeds the staged read values");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n_words) LTPG_HIP_TRY(hipMemcpy(out, kv.read_values, n_words * 4, hipMemcpyDeviceToHost));

@@ -128,6 +128,8 @@ void ltpg_kv_gen_init(ltpg_kv_gen *g)
     g->tid_base = 0;
     g->distinct_keys = 1;
     g->txn_base = 0;
+    g->scan_len = 0;
+    g->reserved = 0;
 }
 
 int ltpg_kv_generate(const ltpg_kv_gen *g, uint64_t *keys, uint32_t *meta, uint32_t *tids,
@@ -140,7 +142,7 @@ int ltpg_kv_generate(const ltpg_kv_gen *g, uint64_t *keys, uint32_t *meta, uint3
     if (g->rows == 0 || g->fields == 0 || g->fields > 65536 || g->n_slots == 0 ||
         g->n_slots > 64 || g->min_ops > g->max_ops || g->max_ops > g->n_slots ||
         g->read_pct > 100 || g->theta < 0.0 || g->theta >= 1.0 ||
-        (g->theta > 0.0 && g->rows < 3)) {
+        (g->theta > 0.0 && g->rows < 3) || g->scan_len > LTPG_KV_MAX_SCAN) {
         ltpg::set_error("invalid generator parameters");
         return LTPG_ERR_INVALID;
     }
@@ -184,7 +186,8 @@ int ltpg_kv_generate(const ltpg_kv_gen *g, uint64_t *keys, uint32_t *meta, uint3
                 }
                 picked[s] = row;
                 keys[i] = row;
-                meta[i] = LTPG_KV_META(kind, field);
+                meta[i] = (kind == LTPG_KV_READ && g->scan_len) ? LTPG_KV_SCAN_META(field, g->scan_len)
+                                                                 : LTPG_KV_META(kind, field);
             }
             tids[t] = g->tid_base + (uint32_t)t;
         }

@@ -133,6 +133,7 @@ struct Op {
     uint64_t row;
     uint32_t kind;
     uint32_t field;
+    uint32_t len;  // rows read: 1 for a read, the clipped range for a scan, 0 otherwise
     bool valid;
 };
 
@@ -140,9 +141,11 @@ struct Op {
 // when the lane has no op) and the result is selected afterwards: a load
 // under a lane-dependent branch sits in its own basic block and the compiler
 // waits for it before the next one, which would serialize the misses.
+// A scan (kind 3) is valid only if its length is at most max_scan; kernels
+// that never look at reads pass 0.
 __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
                                       const uint32_t *__restrict__ meta, uint64_t i, bool live,
-                                      uint64_t rows, uint32_t fields)
+                                      uint64_t rows, uint32_t fields, uint32_t max_scan = 0)
 {
     Op o;
     const uint64_t j = live ? i : 0ull;
@@ -152,7 +155,13 @@ __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
     o.row = live ? r0 : 0ull;
     o.kind = m & 3u;
     o.field = (m >> 8) & 0xffffu;
-    o.valid = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && o.row < rows && o.field < fields;
+    const uint32_t sl = (m >> 24) + 1u;
+    o.valid = o.kind != LTPG_KV_NONE && o.row < rows && o.field < fields &&
+              (o.kind != LTPG_KV_SCAN || sl <= max_scan);
+    const uint64_t left = o.row < rows ? rows - o.row : 0ull;
+    o.len = !o.valid || o.kind == LTPG_KV_WRITE ? 0u
+            : o.kind == LTPG_KV_READ            ? 1u
+                                                : (uint32_t)(sl < left ? sl : left);
     return o;
 }
 
@@ -295,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts,
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts, uint32_t max_scan,
     DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -307,13 +316,16 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     bool waw = false;
     uint32_t bad = 0;
     for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
-        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
-        if (!o.valid || o.row == hot) continue;  // the hot row never registers
-        if (o.kind == LTPG_KV_WRITE)
-            waw |= register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch);
-        else if (reg_reads)
-            register_read(&regs[rl.r(o.row)], me);
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields, max_scan);
+        bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
+        if (!o.valid) continue;
+        if (o.kind == LTPG_KV_WRITE) {
+            if (o.row != hot)  // the hot row never registers
+                waw |= register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch);
+        } else if (reg_reads) {
+            for (uint32_t k = 0; k < o.len; ++k)
+                if (o.row + k != hot) register_read(&regs[rl.r(o.row + k)], me);
+        }
     }
     if (waw) doomed[t] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts,
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts, uint32_t max_scan,
     DeviceCounters *__restrict__ ctr)
 {
     if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
@@ -335,14 +347,15 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const bool live = i < (uint64_t)S * B;
     const uint64_t s = live ? i / B : 0;
     const uint32_t t = (uint32_t)(i - s * B);
-    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
-    const bool bad = (o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid;
-    if (o.valid && o.row != hot) {
+    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
+    const bool bad = o.kind != LTPG_KV_NONE && !o.valid;
+    if (o.valid) {
         const unsigned long long me = priority(tag, tids[t], t);
         if (o.kind == LTPG_KV_WRITE) {
-            if (register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch)) doomed[t] = epoch;
+            if (o.row != hot && register_write(&regs[rl.w(o.row)], me, tag, doomed, epoch)) doomed[t] = epoch;
         } else if (reg_reads) {
-            register_read(&regs[rl.r(o.row)], me);
+            for (uint32_t k = 0; k < o.len; ++k)
+                if (o.row + k != hot) register_read(&regs[rl.r(o.row + k)], me);
         }
     }
     if (bad) atomicAdd(&ctr->range_errors, 1u);
@@ -364,7 +377,8 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ doomed, uint8_t epoch, bool zero_counts, DeviceCounters *__restrict__ ctr)
+    uint8_t *__restrict__ doomed, uint8_t epoch, bool zero_counts, uint32_t max_scan,
+    DeviceCounters *__restrict__ ctr)
 {
     __shared__ unsigned long long s_key[kDedupSlots];  // row + 1, 0 = empty
     __shared__ unsigned long long s_min[kDedupSlots];
@@ -388,8 +402,8 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
         const bool live = i < n_ops;
         const uint64_t s = live ? i / B : 0;
         const uint32_t t = (uint32_t)(i - s * B);
-        const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
-        bad += ((o.kind == LTPG_KV_READ || o.kind == LTPG_KV_WRITE) && !o.valid) ? 1u : 0u;
+        const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
+        bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
         wr[k] = o.valid && o.kind == LTPG_KV_WRITE && o.row != hot;
         row[k] = o.row;
         tt[k] = t;
@@ -454,21 +468,26 @@ __global__ __launch_bounds__(kBlock) void kv_mark_reads_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch)
+    uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch, uint32_t max_scan)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < (uint64_t)S * B;
     const uint64_t s = live ? i / B : 0;
     const uint32_t t = (uint32_t)(i - s * B);
-    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields);
-    if (!o.valid || o.kind != LTPG_KV_READ || o.row == hot) return;
-    const unsigned long long w = regs[rl.w(o.row)];
-    if ((uint32_t)(w >> kTagShift) != tag) return;  // no writer this epoch
+    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
+    if (o.len == 0) return;  // writes, empty and invalid slots
     const unsigned long long me = priority(tag, tids[t], t);
-    if (w < me)
-        raw_flag[t] = epoch;
-    else if (w > me)
-        war_flag[w & kPosMask] = epoch;
+    bool raw = false;
+    for (uint32_t k = 0; k < o.len; ++k) {  // a scan: every row of its range
+        if (o.row + k == hot) continue;
+        const unsigned long long w = regs[rl.w(o.row + k)];
+        if ((uint32_t)(w >> kTagShift) != tag) continue;  // no writer this epoch
+        if (w < me)
+            raw = true;
+        else if (w > me)
+            war_flag[w & kPosMask] = epoch;
+    }
+    if (raw) raw_flag[t] = epoch;
 }
 
 // CHECK + COMMIT, counting form (check_r / check_w_0 / check_w_1 and
@@ -481,7 +500,8 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts, DeviceCounters *__restrict__ ctr)
+    uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts, uint32_t max_scan,
+    DeviceCounters *__restrict__ ctr)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -489,11 +509,12 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     uint32_t raw = 0, war = 0, waw = 0;
     for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields);
-        if (!o.valid || o.row == hot) continue;  // the hot row never conflicts
-        if (o.kind == LTPG_KV_READ) {
-            raw += regs[rl.w(o.row)] < me ? 1u : 0u;
-        } else {
+        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields, max_scan);
+        if (!o.valid) continue;
+        if (o.kind != LTPG_KV_WRITE) {
+            for (uint32_t k = 0; k < o.len; ++k)  // the hot row never conflicts
+                raw += (o.row + k != hot && regs[rl.w(o.row + k)] < me) ? 1u : 0u;
+        } else if (o.row != hot) {
             war += regs[rl.r(o.row)] < me ? 1u : 0u;
             waw += regs[rl.w(o.row)] < me ? 1u : 0u;
         }
@@ -518,7 +539,7 @@ template <int LP>
 __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, uint32_t B, uint32_t S,
     uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint32_t *__restrict__ table,
-    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ read_values)
+    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ read_values, uint32_t max_scan, uint32_t L)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -529,10 +550,13 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     const bool committed = t < B && ((word >> lane) & 1ull);
     for (uint32_t s = 0; s < S; ++s) {
         const uint64_t i = (uint64_t)s * B + t;
-        const Op o = load_op(keys, meta, i, committed, rows, fields);
-        const unsigned long long mask = __ballot(o.valid && o.kind == LTPG_KV_READ);
-        if (mask)
-            copy_fields<LP>(mask, (o.row * fields + o.field) * fsw, i * fw, 0u, table, read_values, fw);
+        const Op o = load_op(keys, meta, i, committed, rows, fields, max_scan);
+        for (uint32_t k = 0;; ++k) {  // row k of each read / scan
+            const unsigned long long mask = __ballot(k < o.len);
+            if (!mask) break;
+            copy_fields<LP>(mask, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                            read_values, fw);
+        }
     }
 }
 
@@ -610,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint8_t *__restrict__ doomed,
     const uint8_t *__restrict__ raw_flag, const uint8_t *__restrict__ war_flag, uint8_t epoch,
     const uint32_t *__restrict__ table, uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
-    DeviceCounters *__restrict__ ctr)
+    uint32_t max_scan, uint32_t L, DeviceCounters *__restrict__ ctr)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ unsigned long long s_bits[CH / 64];
@@ -637,11 +661,13 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
         const uint64_t i = (uint64_t)gl * B + t;
-        const Op o = load_op(keys, meta, i, has && gl < S, rows, fields);
-        const unsigned long long rd = __ballot(o.valid && o.kind == LTPG_KV_READ);
-        if (rd)
-            group_copy<G, 8>(rd, (o.row * fields + o.field) * fsw, i * fw, 0u, table, read_values, fw, S,
-                             grp, gl);
+        const Op o = load_op(keys, meta, i, has && gl < S, rows, fields, max_scan);
+        for (uint32_t k = 0;; ++k) {  // row k of each read / scan
+            const unsigned long long rd = __ballot(k < o.len);
+            if (!rd) break;
+            group_copy<G, 8>(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                             read_values, fw, S, grp, gl);
+        }
     }
     const uint32_t nwords = (B + 63) / 64;
     if (threadIdx.x < CH / 64 && chunk0 / 64 + threadIdx.x < nwords)
@@ -901,20 +927,20 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
     if (kv.exec_per_op && !counts) {
         hipLaunchKernelGGL(kv_execute_dedup_kernel, dim3((unsigned)((ops + kDedupOps - 1) / kDedupOps)), dim3(kBlock),
                            0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
-                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, ctr);
+                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, kv.max_scan, ctr);
     } else if (kv.exec_per_op) {
         hipLaunchKernelGGL(kv_execute_ops_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                            b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed,
-                           e.epoch, counts, zero_counts, ctr);
+                           e.epoch, counts, zero_counts, kv.max_scan, ctr);
     } else {
         hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
                            b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.doomed, e.epoch, counts, zero_counts, ctr);
+                           kv.doomed, e.epoch, counts, zero_counts, kv.max_scan, ctr);
     }
     if (!counts)
         hipLaunchKernelGGL(kv_mark_reads_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                            b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.raw_flag, kv.war_flag, e.epoch);
+                           kv.raw_flag, kv.war_flag, e.epoch, kv.max_scan);
     return hipGetLastError();
 }
 
@@ -945,13 +971,14 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
     if (counts) {
         hipLaunchKernelGGL(kv_check_counting_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys,
                            b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv),
-                           kv.hot_row, e.tag, kv.bitmap, kv.conflicts, ctr);
+                           kv.hot_row, e.tag, kv.bitmap, kv.conflicts, kv.max_scan, ctr);
         return hipGetLastError();
     }
 #define CALL1(GV, CHV)                                                                               \
     hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words, kv.field_stride,   \
-                       kv.doomed, kv.raw_flag, kv.war_flag, e.epoch, kv.table, kv.read_values, kv.bitmap, ctr)
+                       kv.doomed, kv.raw_flag, kv.war_flag, e.epoch, kv.table, kv.read_values, kv.bitmap,  \
+                       kv.max_scan, kv.read_slot, ctr)
 #define CALL(GV)                    \
     if (kv.small_chunks)            \
         CALL1(GV, 64);              \
@@ -970,7 +997,8 @@ hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t
 #define CALL(LPV)                                                                          \
     hipLaunchKernelGGL(kv_readout_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,    \
                        b.keys, b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields,           \
-                       kv.field_words, kv.field_stride, kv.table, kv.bitmap, kv.read_values)
+                       kv.field_words, kv.field_stride, kv.table, kv.bitmap, kv.read_values, kv.max_scan, \
+                       kv.read_slot)
     LTPG_LP_SWITCH(lp, CALL)
 #undef CALL
     return hipGetLastError();

@@ -65,6 +65,8 @@ struct KvState {
     uint32_t field_words = 0;
     uint32_t field_stride = 0;          // words between field starts (>= field_words)
     uint32_t max_slots = 0;
+    uint32_t max_scan = 0;              // longest scan accepted (0: none)
+    uint32_t read_slot = 1;             // fields of read values per op slot: max(1, max_scan)
     uint32_t *table = nullptr;          // rows * fields * field_stride, row-major
     unsigned long long *regs = nullptr; // 2 * rows: [R, W] per row, epoch-tagged
     uint8_t *doomed = nullptr;          // max_txns: epoch number if waw > 0

@@ -195,7 +195,7 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
     LTPG_HIP_TRY(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
     if (w == 0) {
         const uint64_t S = sv->cfg.kv_ops;
-        const uint64_t fw = sv->lane[0].ctx->kv.field_words;
+        const uint64_t fw = (uint64_t)sv->lane[0].ctx->kv.field_words * sv->lane[0].ctx->kv.read_slot;
         if ((rc = host_alloc(&f.h_keys, S * C)) || (rc = host_alloc(&f.h_meta, S * C)) ||
             (rc = host_alloc(&f.h_tids, C)) || (rc = host_alloc(&f.h_bits, words(C))))
             return rc;
@@ -282,7 +282,8 @@ int launch(ltpg_server *sv, int w, Flight &f)
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
         LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->kv.bitmap, words(B) * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (sv->cfg.keep_reads)
-            LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values, (uint64_t)S * B * ctx->kv.field_words * 4,
+            LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values,
+                                        (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words * 4,
                                         hipMemcpyDeviceToHost, ctx->stream));
     } else {
         const uint32_t n_no = f.n_no, n_pay = f.n_pay;
@@ -370,12 +371,21 @@ void complete(ltpg_server *sv, int w, Flight &f)
         sv->done.push_back(d);
         if (sv->cfg.keep_reads) {
             std::vector<uint32_t> v;
-            if (w == 0) {
-                const uint32_t fw = L.ctx->kv.field_words;
+            if (w == 0) {  // each read's field, each scan's rows inside the table, in slot order
+                const KvState &kv = L.ctx->kv;
+                const uint32_t fw = kv.field_words;
                 for (uint32_t s = 0; s < S; s++) {
-                    if (LTPG_KV_KIND(sv->kv_meta[r.data * S + s]) != LTPG_KV_READ) continue;
-                    const uint32_t *src = f.h_reads + (((uint64_t)s * B) + t) * fw;
-                    v.insert(v.end(), src, src + fw);
+                    const uint32_t m = sv->kv_meta[r.data * S + s];
+                    const uint64_t row = sv->kv_keys[r.data * S + s];
+                    const uint32_t kind = LTPG_KV_KIND(m);
+                    if (row >= kv.rows || LTPG_KV_FIELD(m) >= kv.fields) continue;
+                    uint64_t n = 0;
+                    if (kind == LTPG_KV_READ)
+                        n = 1;
+                    else if (kind == LTPG_KV_SCAN && LTPG_KV_SCAN_LEN(m) <= kv.max_scan)
+                        n = std::min<uint64_t>(LTPG_KV_SCAN_LEN(m), kv.rows - row);
+                    const uint32_t *src = f.h_reads + (((uint64_t)s * B) + t) * kv.read_slot * fw;
+                    v.insert(v.end(), src, src + n * fw);
                 }
             } else if (t < f.n_no) {
                 for (uint32_t k = 0; k < kNoReads; k++) v.push_back(f.h_reads[(uint64_t)k * f.n_no + t]);
@@ -536,7 +546,7 @@ int ltpg_serve_start(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_serve_conf
         if ((rc = check_ctx(L.ctx))) break;
         // per-transaction bytes of the access-set buffers an epoch holds:
         // its batch slot and its read values
-        const uint64_t per = w == 0 ? (uint64_t)cfg->kv_ops * (12 + 4ull * L.ctx->kv.field_words) + 4
+        const uint64_t per = w == 0 ? (uint64_t)cfg->kv_ops * (12 + 4ull * L.ctx->kv.field_words * L.ctx->kv.read_slot) + 4
                                     : 4ull * (53 + kNoReads);
         uint64_t cap = L.ctx->cfg.max_txns;
         if (cfg->max_epoch_txns) cap = std::min<uint64_t>(cap, cfg->max_epoch_txns);

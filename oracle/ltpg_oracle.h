@@ -22,8 +22,9 @@ extern "C" {
 typedef struct oracle_kv_result {
     uint64_t *commit_bitmap;   /* [ceil(B/64)]                                */
     uint32_t *conflicts;       /* [B] raw | war << 8 | waw << 16               */
-    uint32_t *read_values;     /* [S*B*field_words] or NULL; only read slots
-                                  of committed transactions are written    */
+    uint32_t *read_values;     /* [S*B*L*field_words], L = max(1, max_scan),
+                                  or NULL; only read and scan slots of
+                                  committed transactions are written       */
     uint32_t commit_count;
     uint32_t range_errors;     /* ops whose key is >= rows (skipped)          */
 } oracle_kv_result;
@@ -35,19 +36,22 @@ typedef struct oracle_kv_result {
  * reg_scratch must hold 2*rows u64 (R and W registers of priority pairs
  * TID << 32 | batch position).  hot_row: UINT64_MAX, or the row whose ops
  * never register or conflict and whose committed writes add commutatively.
+ * max_scan: the longest scan accepted; read_values holds max(1, max_scan)
+ * fields per op slot.
  */
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
-                    oracle_kv_result *res, uint64_t hot_row);
+                    oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan);
 
 /* The four phases separately (used by tests that pin each one). */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row);
+                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row, uint32_t max_scan);
 void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                      const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row);
+                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row,
+                     uint32_t max_scan);
 uint32_t oracle_commit(uint32_t B, const uint32_t *conflicts, uint64_t *bitmap);
 void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                          uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,

@@ -44,6 +44,17 @@
  * transaction's reads are discarded (it re-executes in a later epoch), so
  * the build does not produce them.  They come from the snapshot, i.e. the
  * table before this epoch's write-back.
+ *
+ * Scans (LTPG/YCSB/2: read_0 registers R on rows Loc .. Loc + SCAN_SIZE - 1
+ * below the table size, Execute.h:21-32; check_r checks them, Execute.h:
+ * 126-134; __READ stages them, Genericfunction.h:44-50).  Build-defined: a
+ * SCAN op (kind 3, length (meta >> 24) + 1 <= max_scan) is a read of every
+ * row key .. min(key + len, rows) - 1 of its field -- each row registers R
+ * and counts raw against W -- and stages each row's field (the reference
+ * checks row Loc + 1 for every i and keeps only the last staged row).  A
+ * scan whose first row or field is outside the table, or whose length
+ * exceeds max_scan, is a range error.  Read values of slot i hold L =
+ * max(1, max_scan) fields: row j of a scan at ((i * L) + j) * field_words.
  */
 #include "ltpg_oracle.h"
 
@@ -52,16 +63,29 @@
 
 #define KIND(m) ((m) & 3u)
 #define FIELD(m) (((m) >> 8) & 0xffffu)
+#define SCANLEN(m) (((m) >> 24) + 1u)
 #define K_READ 1u
 #define K_WRITE 2u
+#define K_SCAN 3u
 
-/* An op takes part in the epoch iff its kind is READ or WRITE and its row
- * and field exist; anything else is skipped by every phase (and counted as a
- * range error when the kind is READ or WRITE). */
-static int op_valid(uint32_t m, uint64_t row, uint64_t rows, uint32_t fields)
+/* An op takes part in the epoch iff its kind is READ, WRITE or SCAN, its
+ * (first) row and field exist and a scan is at most max_scan rows; anything
+ * else is skipped by every phase (and counted as a range error unless the
+ * slot is empty). */
+static int op_valid(uint32_t m, uint64_t row, uint64_t rows, uint32_t fields, uint32_t max_scan)
 {
     uint32_t k = KIND(m);
-    return (k == K_READ || k == K_WRITE) && row < rows && FIELD(m) < fields;
+    if (k == K_SCAN && SCANLEN(m) > max_scan) return 0;
+    return k != 0 && row < rows && FIELD(m) < fields;
+}
+
+/* Rows read by op m at row: 1 for a READ, the clipped range for a SCAN. */
+static uint64_t rows_read(uint32_t m, uint64_t row, uint64_t rows)
+{
+    if (KIND(m) == K_READ) return 1;
+    if (KIND(m) != K_SCAN) return 0;
+    uint64_t n = SCANLEN(m);
+    return row + n <= rows ? n : rows - row;
 }
 
 static uint64_t prio(const uint32_t *tids, uint32_t t) { return ((uint64_t)tids[t] << 32) | t; }
@@ -70,15 +94,20 @@ static uint64_t prio(const uint32_t *tids, uint32_t t) { return ((uint64_t)tids[
  * the priority pair. */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row)
+                        uint64_t *reg_r, uint64_t *reg_w, uint64_t hot_row, uint32_t max_scan)
 {
     for (uint32_t s = 0; s < S; s++) {
         for (uint32_t t = 0; t < B; t++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
-            if (!op_valid(meta[i], row, rows, fields) || row == hot_row) continue;
-            uint64_t *reg = (KIND(meta[i]) == K_READ) ? reg_r : reg_w;
-            if (prio(tids, t) < reg[row]) reg[row] = prio(tids, t);
+            if (!op_valid(meta[i], row, rows, fields, max_scan)) continue;
+            if (KIND(meta[i]) == K_WRITE) {
+                if (row != hot_row && prio(tids, t) < reg_w[row]) reg_w[row] = prio(tids, t);
+                continue;
+            }
+            uint64_t n = rows_read(meta[i], row, rows);
+            for (uint64_t j = 0; j < n; j++)
+                if (row + j != hot_row && prio(tids, t) < reg_r[row + j]) reg_r[row + j] = prio(tids, t);
         }
     }
 }
@@ -88,7 +117,8 @@ void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
  * W (waw). */
 void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                      const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,
-                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row)
+                     const uint64_t *reg_r, const uint64_t *reg_w, uint32_t *conflicts, uint64_t hot_row,
+                     uint32_t max_scan)
 {
     for (uint32_t t = 0; t < B; t++) {
         uint32_t raw = 0, war = 0, waw = 0;
@@ -96,10 +126,12 @@ void oracle_kv_check(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
         for (uint32_t s = 0; s < S; s++) {
             uint64_t i = (uint64_t)s * B + t;
             uint64_t row = keys[i];
-            if (!op_valid(meta[i], row, rows, fields) || row == hot_row) continue;
-            if (KIND(meta[i]) == K_READ) {
-                if (me > reg_w[row]) raw++;
-            } else {
+            if (!op_valid(meta[i], row, rows, fields, max_scan)) continue;
+            if (KIND(meta[i]) != K_WRITE) {
+                uint64_t n = rows_read(meta[i], row, rows);
+                for (uint64_t j = 0; j < n; j++)
+                    if (row + j != hot_row && me > reg_w[row + j]) raw++;
+            } else if (row != hot_row) {
                 if (me > reg_r[row]) war++;
                 if (me > reg_w[row]) waw++;
             }
@@ -140,7 +172,7 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
             uint64_t i = (uint64_t)s * B + t;
             uint32_t m = meta[i];
             uint64_t row = keys[i];
-            if (KIND(m) != K_WRITE || !op_valid(m, row, rows, fields)) continue;
+            if (KIND(m) != K_WRITE || !op_valid(m, row, rows, fields, 0)) continue;
             uint32_t *w = table + (row * fields + FIELD(m)) * (uint64_t)field_words;
             for (uint32_t j = 0; j < field_words; j++) w[j] += delta;
         }
@@ -150,8 +182,9 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
-                    oracle_kv_result *res, uint64_t hot_row)
+                    oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan)
 {
+    const uint64_t L = max_scan > 1 ? max_scan : 1;
     uint64_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
     uint32_t errors = 0;
     /* clear_LOG: every register back to "no registrant" (YCSB/0/Database.h:
@@ -159,13 +192,11 @@ int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32
     memset(reg_scratch, 0xff, (size_t)rows * 2 * sizeof(uint64_t));
 
     /* execute: register */
-    oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, hot_row);
-    for (uint64_t i = 0; i < (uint64_t)S * B; i++) {
-        uint32_t k = KIND(meta[i]);
-        if (!op_valid(meta[i], keys[i], rows, fields) && (k == K_READ || k == K_WRITE)) errors++;
-    }
+    oracle_kv_register(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, hot_row, max_scan);
+    for (uint64_t i = 0; i < (uint64_t)S * B; i++)
+        if (KIND(meta[i]) != 0 && !op_valid(meta[i], keys[i], rows, fields, max_scan)) errors++;
 
-    oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts, hot_row);
+    oracle_kv_check(rows, fields, B, S, keys, meta, tids, reg_r, reg_w, res->conflicts, hot_row, max_scan);
     res->commit_count = oracle_commit(B, res->conflicts, res->commit_bitmap);
 
     /* committed reads see the snapshot: stage them before any write-back */
@@ -174,9 +205,13 @@ int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32
             for (uint32_t t = 0; t < B; t++) {
                 uint64_t i = (uint64_t)s * B + t;
                 if (!((res->commit_bitmap[t / 64] >> (t % 64)) & 1ull)) continue;
-                if (KIND(meta[i]) != K_READ || !op_valid(meta[i], keys[i], rows, fields)) continue;
-                const uint32_t *src = table + (keys[i] * fields + FIELD(meta[i])) * (uint64_t)field_words;
-                memcpy(res->read_values + i * field_words, src, field_words * sizeof(uint32_t));
+                if (KIND(meta[i]) == K_WRITE || !op_valid(meta[i], keys[i], rows, fields, max_scan)) continue;
+                uint64_t n = rows_read(meta[i], keys[i], rows);
+                for (uint64_t j = 0; j < n; j++) {
+                    const uint32_t *src =
+                        table + ((keys[i] + j) * fields + FIELD(meta[i])) * (uint64_t)field_words;
+                    memcpy(res->read_values + (i * L + j) * field_words, src, field_words * sizeof(uint32_t));
+                }
             }
         }
     }

@@ -34,7 +34,7 @@ def _load(path=None):
     lib.oracle_kv_epoch.restype = ctypes.c_int
     lib.oracle_kv_epoch.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                     ctypes.c_uint32, ctypes.c_uint32, u64p, u32p, u32p, ctypes.c_uint32,
-                                    u64p, ctypes.POINTER(KvResult), ctypes.c_uint64]
+                                    u64p, ctypes.POINTER(KvResult), ctypes.c_uint64, ctypes.c_uint32]
     lib.KvResult = KvResult
 
     class TpccResult(ctypes.Structure):
@@ -99,9 +99,10 @@ NO_HOT_ROW = (1 << 64) - 1
 
 
 def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values=1, want_reads=True,
-             inplace=False, hot_row=NO_HOT_ROW):
+             inplace=False, hot_row=NO_HOT_ROW, max_scan=0):
     """Run one epoch on a copy of `table` (row-major u32), or on `table`
-    itself when inplace=True; return KvEpoch."""
+    itself when inplace=True; return KvEpoch.  Read values hold
+    max(1, max_scan) fields per op slot."""
     if inplace:
         assert table.dtype == np.uint32 and table.flags.c_contiguous
         table = table.reshape(-1)
@@ -116,14 +117,15 @@ def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values
     assert keys.size == B * S and meta.size == B * S
     bitmap = np.zeros(max(1, (B + 63) // 64), dtype=np.uint64)
     conflicts = np.zeros(max(B, 1), dtype=np.uint32)
-    reads = np.zeros(max(B * S * field_words, 1), dtype=np.uint32) if want_reads else None
+    L = max(1, int(max_scan))
+    reads = np.zeros(max(B * S * L * field_words, 1), dtype=np.uint32) if want_reads else None
     regs = np.empty(rows * 2, dtype=np.uint64)
     res = _lib.KvResult(_u64p(bitmap), _u32p(conflicts), _u32p(reads) if want_reads else None, 0, 0)
     rc = _lib.oracle_kv_epoch(rows, fields, field_words, _u32p(table), B, S, _u64p(keys), _u32p(meta),
-                              _u32p(tids), values, _u64p(regs), ctypes.byref(res), hot_row)
+                              _u32p(tids), values, _u64p(regs), ctypes.byref(res), hot_row, int(max_scan))
     assert rc == 0
     return KvEpoch(bitmap[: (B + 63) // 64], conflicts[:B],
-                   reads[: B * S * field_words] if want_reads else None,
+                   reads[: B * S * L * field_words] if want_reads else None,
                    res.commit_count, res.range_errors, table)
 
 

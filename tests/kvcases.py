@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-KIND = {"n": 0, "r": 1, "w": 2}
+KIND = {"n": 0, "r": 1, "w": 2, "s": 3}
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -40,9 +40,12 @@ def case_arrays(case):
     meta = np.zeros(S * B, dtype=np.uint32)
     tids = np.array([t["tid"] for t in txns], dtype=np.uint32)
     for t, txn in enumerate(txns):
-        for s, (kind, row, field) in enumerate(txn["ops"]):
+        for s, op in enumerate(txn["ops"]):
+            kind, row, field = op[:3]
             keys[s * B + t] = row
             meta[s * B + t] = KIND[kind] | (field << 8)
+            if kind == "s":  # ["s", row, field, length]
+                meta[s * B + t] |= (op[3] - 1) << 24
     return keys, meta, tids, S
 
 
@@ -63,8 +66,10 @@ def unpack_conflicts(c):
     return np.stack([c & 0xFF, (c >> 8) & 0xFF, (c >> 16) & 0xFF], axis=1)
 
 
-def ops_of(keys, meta, B, S, rows, fields):
-    """List per txn of (slot, kind, row, field) for valid ops."""
+def ops_of(keys, meta, B, S, rows, fields, max_scan=0):
+    """List per txn of (slot, kind, row, field) for valid ops; a scan of
+    length n <= max_scan is one read (kind 1) per row of key .. key + n - 1
+    inside the table."""
     out = [[] for _ in range(B)]
     for s in range(S):
         for t in range(B):
@@ -72,13 +77,16 @@ def ops_of(keys, meta, B, S, rows, fields):
             kind, field, row = m & 3, (m >> 8) & 0xFFFF, int(keys[s * B + t])
             if kind in (1, 2) and row < rows and field < fields:
                 out[t].append((s, kind, row, field))
+            elif kind == 3 and row < rows and field < fields and (m >> 24) + 1 <= max_scan:
+                for r in range(row, min(row + (m >> 24) + 1, rows)):
+                    out[t].append((s, 1, r, field))
     return out
 
 
-def pairwise_conflicts(keys, meta, tids, B, S, rows, fields):
+def pairwise_conflicts(keys, meta, tids, B, S, rows, fields, max_scan=0):
     """raw/war/waw by direct pairwise comparison (no registers).  Priority
     is (TID, batch position): lower wins."""
-    ops = ops_of(keys, meta, B, S, rows, fields)
+    ops = ops_of(keys, meta, B, S, rows, fields, max_scan)
     readers, writers = {}, {}
     for t in range(B):
         for _, kind, row, _ in ops[t]:
@@ -102,14 +110,14 @@ def aria_commit(conf):
     return ((waw == 0) & ((raw == 0) | (war == 0))).astype(int)
 
 
-def serial_check(table0, keys, meta, tids, B, S, rows, fields, fw, committed, values=1):
+def serial_check(table0, keys, meta, tids, B, S, rows, fields, fw, committed, values=1, max_scan=0):
     """Check the committed set is conflict-serializable and return the table
     obtained by running the committed transactions one by one.
 
     Every committed read saw the snapshot, so a committed reader of row x
     must come before any committed writer of x; two committed writers of one
     row are forbidden.  Returns (ok, message, final_table)."""
-    ops = ops_of(keys, meta, B, S, rows, fields)
+    ops = ops_of(keys, meta, B, S, rows, fields, max_scan)
     C = [t for t in range(B) if committed[t]]
     writers = {}
     for t in C:
@@ -157,12 +165,14 @@ def serial_check(table0, keys, meta, tids, B, S, rows, fields, fw, committed, va
     return True, "", tab.reshape(-1)
 
 
-def random_batch(rng, B, S, rows, fields, p_read=0.5, p_empty=0.1, distinct_tids=True):
+def random_batch(rng, B, S, rows, fields, p_read=0.5, p_empty=0.1, distinct_tids=True, p_scan=0.0, max_len=1):
     keys = rng.integers(0, rows, size=S * B, dtype=np.uint64)
     kinds = np.where(rng.random(S * B) < p_read, 1, 2)
+    kinds = np.where((kinds == 1) & (rng.random(S * B) < p_scan), 3, kinds)
     kinds = np.where(rng.random(S * B) < p_empty, 0, kinds)
     fld = rng.integers(0, fields, size=S * B)
-    meta = (kinds | (fld << 8)).astype(np.uint32)
+    lens = rng.integers(1, max_len + 1, size=S * B)
+    meta = (kinds | (fld << 8) | np.where(kinds == 3, (lens - 1) << 24, 0)).astype(np.uint32)
     if distinct_tids:
         tids = rng.permutation(np.arange(1, 4 * B + 1, dtype=np.uint32))[:B].astype(np.uint32)
     else:

@@ -106,6 +106,38 @@ def test_graph_form_with_hot_row_matches_plain_loop(gpu):
         np.testing.assert_array_equal(x, y)
 
 
+def test_graph_form_with_scans_matches_plain_loop_and_oracle(gpu):
+    """Key-value batches whose reads are 6-row scans: the pipelined graph
+    gives the plain loop's commit sets and table, and both the oracle's."""
+    from oracle import oracle
+    from tests import kvcases as kc
+
+    rows, fields, fw, S, max_scan = 20000, 3, 2, 16, 6
+    rng = np.random.default_rng(3)
+    sizes = [int(np.exp(rng.uniform(np.log(64), np.log(4000)))) for _ in range(24)]
+    gens = [L.kv_generate(seed=50 + i, rows=rows, fields=fields, n_txns=n, n_slots=S, min_ops=1, max_ops=S,
+                          read_pct=70, theta=0.9, scan_len=6, tid_base=i * 10000) for i, n in enumerate(sizes)]
+    out = {}
+    for mode in (L.RUN_PLAIN, L.RUN_GRAPH):
+        kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                       kv_max_slots=S, max_txns=4000, kv_max_scan=max_scan)
+        kv.kv_table_load(kc.initial_table(rows, fields, fw))
+        keep = [L.DeviceKvBatch(kv, k, m, t, S) for (k, m, t) in gens]
+        r = L.Batched(kv, None, [L.kv_desc(d) for d in keep])
+        r.run(mode)
+        out[mode] = ([r.bitmap(i, L.TYPE_KV) for i in range(len(gens))], kv.kv_table_read())
+        r.close()
+    for a, b in zip(out[L.RUN_PLAIN][0], out[L.RUN_GRAPH][0]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out[L.RUN_PLAIN][1], out[L.RUN_GRAPH][1])
+    table = kc.initial_table(rows, fields, fw)
+    for i, (k, m, t) in enumerate(gens):
+        ora = oracle.kv_epoch(table, rows, fields, fw, k, m, t, S, inplace=True, want_reads=False,
+                              max_scan=max_scan)
+        np.testing.assert_array_equal(out[L.RUN_GRAPH][0][i], ora.bitmap)
+    np.testing.assert_array_equal(out[L.RUN_GRAPH][1], table)
+
+
 def test_plain_epochs_after_a_graph_run(gpu):
     """A context used by a graph run keeps working epoch by epoch."""
     kv, tp = _contexts()

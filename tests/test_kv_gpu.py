@@ -16,10 +16,11 @@ pytestmark = pytest.mark.gpu
 KAT = kc.load_kat()
 
 
-def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True, hot_row=L.NO_HOT_ROW):
+def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True, hot_row=L.NO_HOT_ROW,
+         max_scan=0):
     return L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                      max_txns=max_txns, kv_max_slots=max_slots, values=values,
-                     flags=L.FLAG_CONFLICT_COUNTS if counts else 0, kv_hot_row=hot_row)
+                     flags=L.FLAG_CONFLICT_COUNTS if counts else 0, kv_hot_row=hot_row, kv_max_scan=max_scan)
 
 
 def _gpu_epoch(ctx, keys, meta, tids, S, device=False):
@@ -65,7 +66,9 @@ def test_gpu_known_answers(gpu, case, counts):
     values = L.VALUES_RMW if case["values"] == "rmw" else L.VALUES_COPYBACK
     B = len(tids)
     hot = case.get("hot_row", L.NO_HOT_ROW)
-    with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts, hot_row=hot) as ctx:
+    max_scan = case.get("max_scan", 0)
+    Ls = max(1, max_scan)
+    with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts, hot_row=hot, max_scan=max_scan) as ctx:
         ctx.kv_table_load(kc.initial_table(rows, fields, fw))
         ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run(L.PHASE_ALL)
@@ -87,7 +90,10 @@ def test_gpu_known_answers(gpu, case, counts):
         np.testing.assert_array_equal(ctx.kv_table_read(), kc.expected_table(case))
         reads = ctx.kv_read_values()
         for t, s, words in exp["reads"]:
-            i = s * B + t
+            i = (s * B + t) * Ls
+            assert reads[i * fw:(i + 1) * fw].tolist() == words
+        for t, s, j, words in exp.get("scan_reads", []):
+            i = (s * B + t) * Ls + j
             assert reads[i * fw:(i + 1) * fw].tolist() == words
 
 
@@ -141,6 +147,81 @@ def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts):
             hot_ops = (keys.reshape(S, n) == hot_row) & ((meta.reshape(S, n) & 3) != 0)
             assert hot_ops.any()
             table = ora.table
+
+
+def _assert_scan_reads(gpu_reads, ora, keys, meta, B, S, rows, fw, max_scan):
+    """Delivered rows of committed reads and scans match the oracle's."""
+    L_ = max(1, max_scan)
+    committed = np.unpackbits(ora.bitmap.view(np.uint8), bitorder="little")[:B].astype(bool)
+    kind = (meta.reshape(S, B) & 3).astype(np.int64)
+    ln = np.where(kind == 1, 1, np.where(kind == 3, (meta.reshape(S, B) >> 24).astype(np.int64) + 1, 0))
+    ln = np.where(ln > max(max_scan, 1), 0, ln)
+    ln = np.minimum(ln, rows - keys.reshape(S, B).astype(np.int64))
+    ln = np.where(committed[None, :], ln, 0)
+    mask = (np.arange(L_)[None, None, :] < ln[:, :, None]).reshape(-1)
+    mask = np.repeat(mask, fw)
+    np.testing.assert_array_equal(gpu_reads[mask], ora.read_values[mask])
+
+
+@pytest.mark.parametrize("flags", [0, 1, 4, 7])
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_gpu_scans_match_oracle(gpu, counts, flags):
+    """Reads as scans (YCSB/2: each read a range of rows): ragged 1..12 ops,
+    90 % scans of 10 rows, kv_max_scan 16 (so read-value slots are wider than
+    the scans), several epochs in place, every launch shape family."""
+    rows, fields, fw, B, S, max_scan = 5000, 3, 2, 3000, 12, 16
+    table = kc.initial_table(rows, fields, fw)
+    with _ctx(rows, fields, fw, B, S, counts=counts, max_scan=max_scan) as ctx:
+        ctx.kv_table_load(table)
+        ctx.kv_set_tuning(flags)
+        for ep_i in range(3):
+            keys, meta, tids = L.kv_generate(seed=300 + ep_i, rows=rows, fields=fields, n_txns=B, n_slots=S,
+                                             min_ops=1, max_ops=S, read_pct=90, theta=0.7, scan_len=10,
+                                             tid_base=ep_i * B)
+            g = _gpu_epoch(ctx, keys, meta, tids, S)
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1, max_scan=max_scan)
+            np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
+            if g["conflicts"] is not None:
+                np.testing.assert_array_equal(g["conflicts"], ora.conflicts)
+            np.testing.assert_array_equal(g["table"], ora.table)
+            _assert_scan_reads(g["reads"], ora, keys, meta, B, S, rows, fw, max_scan)
+            assert 0 < ora.commit_count < B
+            table = ora.table
+
+
+def test_gpu_scan_reference_config(gpu):
+    """The scan variant's configuration (YCSB/2/Predefine.h:6-13): 1M rows x
+    10 u32 fields, batch 10,000 txns of 9 scans of 10 rows + 1 write, uniform
+    keys; the reference's initial table."""
+    rows, fields, fw, B, S = 1_000_000, 10, 1, 10000, 10
+    keys, meta, tids = L.kv_generate(seed=0x4C545047, rows=rows, fields=fields, n_txns=B, n_slots=S,
+                                     read_pct=90, theta=0.0, scan_len=10)
+    for counts in (True, False):
+        with _ctx(rows, fields, fw, B, S, counts=counts, max_scan=10) as ctx:
+            ctx.kv_fill_reference()
+            table0 = ctx.kv_table_read()
+            g = _gpu_epoch(ctx, keys, meta, tids, S)
+        ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S, values=1, max_scan=10)
+        np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
+        np.testing.assert_array_equal(g["table"], ora.table)
+        _assert_scan_reads(g["reads"], ora, keys, meta, B, S, rows, fw, 10)
+
+
+def test_gpu_scans_without_max_scan_are_range_errors(gpu):
+    rows, fields, fw, B, S = 100, 2, 1, 50, 3
+    keys, meta, tids = L.kv_generate(seed=5, rows=rows, fields=fields, n_txns=B, n_slots=S, read_pct=50,
+                                     theta=0.0, scan_len=4)
+    n_scans = int(((meta & 3) == 3).sum())
+    assert n_scans > 0
+    with _ctx(rows, fields, fw, B, S, max_scan=0) as ctx:
+        ctx.kv_table_load(kc.initial_table(rows, fields, fw))
+        ctx.kv_submit(keys, meta, tids, S)
+        ctx.epoch_run()
+        with pytest.raises(L.LtpgError) as ei:
+            ctx.sync()
+        assert ei.value.code == L.ERR_RANGE
+    ora = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, max_scan=0)
+    assert ora.range_errors == n_scans
 
 
 def test_gpu_hot_row_beyond_table_rejected(gpu):

@@ -23,9 +23,11 @@ def test_oracle_known_answers(case):
     rows, fields, fw = case["rows"], case["fields"], case["field_words"]
     keys, meta, tids, S = kc.case_arrays(case)
     values = 1 if case["values"] == "rmw" else 0
+    max_scan = case.get("max_scan", 0)
     ep = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values,
-                         hot_row=case.get("hot_row", oracle.NO_HOT_ROW))
+                         hot_row=case.get("hot_row", oracle.NO_HOT_ROW), max_scan=max_scan)
     B = len(tids)
+    L = max(1, max_scan)
     exp = case["expect"]
     assert kc.bitmap_to_list(ep.bitmap, B) == exp["commit"]
     assert kc.unpack_conflicts(ep.conflicts).tolist() == exp["conflicts"]
@@ -33,7 +35,10 @@ def test_oracle_known_answers(case):
     assert ep.commit_count == sum(exp["commit"])
     np.testing.assert_array_equal(ep.table, kc.expected_table(case))
     for t, s, words in exp["reads"]:
-        i = s * B + t
+        i = (s * B + t) * L
+        assert ep.read_values[i * fw:(i + 1) * fw].tolist() == words
+    for t, s, j, words in exp.get("scan_reads", []):
+        i = (s * B + t) * L + j
         assert ep.read_values[i * fw:(i + 1) * fw].tolist() == words
 
 
@@ -121,6 +126,42 @@ def test_oracle_copyback_never_changes_table():
     np.testing.assert_array_equal(ep.bitmap, ep1.bitmap)
 
 
+@pytest.mark.parametrize("seed", range(30))
+def test_oracle_scans_match_pairwise_and_are_serializable(seed):
+    """Scans are reads of every row of their (clipped) range: conflicts match
+    the pairwise computation over the expanded rows, the committed set is
+    serializable, and every delivered row is the snapshot's."""
+    rng = np.random.default_rng(1000 + seed)
+    B, S = int(rng.integers(1, 30)), int(rng.integers(1, 5))
+    rows, fields, fw, max_scan = int(rng.integers(2, 14)), 2, int(rng.integers(1, 3)), int(rng.integers(1, 6))
+    keys, meta, tids = kc.random_batch(rng, B, S, rows, fields, p_scan=0.6, max_len=max_scan + 1)
+    ep = oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, 1,
+                         max_scan=max_scan)
+    conf = kc.unpack_conflicts(ep.conflicts)
+    np.testing.assert_array_equal(conf, kc.pairwise_conflicts(keys, meta, tids, B, S, rows, fields, max_scan))
+    committed = kc.bitmap_to_list(ep.bitmap, B)
+    assert committed == kc.aria_commit(conf).tolist()
+    ok, msg, final = kc.serial_check(kc.initial_table(rows, fields, fw), keys, meta, tids, B, S, rows, fields,
+                                     fw, committed, max_scan=max_scan)
+    assert ok, msg
+    np.testing.assert_array_equal(ep.table, final)
+    too_long = ((meta & 3) == 3) & ((meta >> 24) + 1 > max_scan)
+    assert ep.range_errors == int(too_long.sum())
+    snap = kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)
+    L = max(1, max_scan)
+    for s in range(S):
+        for t in range(B):
+            i = s * B + t
+            m = int(meta[i])
+            if not committed[t] or m & 3 not in (1, 3) or too_long[i]:
+                continue
+            n = 1 if m & 3 == 1 else min((m >> 24) + 1, rows - int(keys[i]))
+            for j in range(n):
+                k = i * L + j
+                np.testing.assert_array_equal(ep.read_values[k * fw:(k + 1) * fw],
+                                              snap[int(keys[i]) + j, (m >> 8) & 0xFFFF])
+
+
 def _rules_with_mistake(keys, meta, tids, B, S, rows, fields, fw, table0, mistake):
     """A CPU copy of the rules with one planted mistake (test-only)."""
     ops = kc.ops_of(keys, meta, B, S, rows, fields)
@@ -166,7 +207,7 @@ def test_pins_catch_planted_mistakes(mistake):
     replay, so the pins above would catch it in the oracle."""
     caught = False
     for case in KAT["cases"]:
-        if case["values"] != "rmw" or case["expect"]["range_errors"] or "hot_row" in case:
+        if case["values"] != "rmw" or case["expect"]["range_errors"] or "hot_row" in case or "max_scan" in case:
             continue
         rows, fields, fw = case["rows"], case["fields"], case["field_words"]
         keys, meta, tids, S = kc.case_arrays(case)

@@ -75,6 +75,49 @@ def test_served_kv_epochs_replay_on_the_oracle(gpu):
     assert all(d["attempts"] >= 1 and d["commit_ns"] >= d["arrival_ns"] for d in done.values())
 
 
+def test_served_scans_replay_on_the_oracle(gpu):
+    """Requests whose reads are scans: each committed request gets its
+    scanned rows (inside the table, in op order) as the oracle's replay of
+    the server's epochs delivers them."""
+    rows, fields, fw, S, n, max_scan = 2000, 2, 1, 6, 3000, 5
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                    max_txns=1024, kv_max_slots=S, kv_max_scan=max_scan)
+    ctx.kv_table_load(kc.initial_table(rows, fields, fw))
+    keys, meta, _ = L.kv_generate(seed=12, rows=rows, fields=fields, n_txns=n, n_slots=S, min_ops=1, max_ops=S,
+                                  read_pct=60, theta=0.8, scan_len=5)
+    keys = keys.reshape(S, n).T.copy()
+    meta = meta.reshape(S, n).T.copy()
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=500, keep_reads=1, record_epochs=1)
+    _submit_in_waves(sv, keys, meta)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_KV, e) for e in range(st["epochs"][0])]
+    reads = {rid: sv.read_values(rid) for rid in done}
+    sv.stop()
+    assert len(done) == n
+    table = kc.initial_table(rows, fields, fw)
+    for e, ids in enumerate(epochs):
+        ids = ids.astype(np.int64)
+        B = len(ids)
+        k = keys[ids].T.reshape(-1).copy()
+        m = meta[ids].T.reshape(-1).copy()
+        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, max_scan=max_scan)
+        for t, rid in enumerate(ids):
+            assert ep.committed(t) == (done[rid]["epoch"] == e)
+            if not ep.committed(t):
+                continue
+            want = []
+            for s in range(S):
+                mm, row = int(m[s * B + t]), int(k[s * B + t])
+                n_rows = 1 if mm & 3 == 1 else (min((mm >> 24) + 1, rows - row) if mm & 3 == 3 else 0)
+                base = (s * B + t) * max_scan
+                want.append(ep.read_values[base * fw:(base + n_rows) * fw])
+            np.testing.assert_array_equal(reads[rid], np.concatenate(want))
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+
+
 def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
     rows, fields, fw, S = 20000, 3, 2, 8
     n_probe = 400
