@@ -45,13 +45,17 @@ def kv_workload(name, theta):
         return dict(rows=1_000_000, fields=10, fw=1, B=65536, S=10, theta=0.8, read_pct=50,
                     desc="key-value plain form: 1M rows x 10 u32 fields, 65,536 txns x 10 ops, "
                          "50/50 read/RMW, Zipf 0.8")
+    if name == "kv-scan":
+        return dict(rows=1_000_000, fields=10, fw=1, B=10000, S=10, theta=0.0, read_pct=90, scan_len=10,
+                    desc="key-value scans (the YCSB/2 configuration): 1M rows x 10 u32 fields, batches of "
+                         "10,000 txns x 10 ops, 90 % scans of 10 rows / 10 % RMW, uniform keys")
     raise SystemExit(f"unknown workload {name}")
 
 
 KERNELS = ["execute", "check", "writeback"]  # the fast CHECK includes the read-out
 
 
-def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw):
+def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw, rows=None, max_scan=0):
     """Algorithmic bytes each key-value phase must move for one batch (fast
     form): every byte the algorithm has to read or write, once.
     EXECUTE (write registration + read marking): the batch (tids, keys and
@@ -65,23 +69,33 @@ def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw):
     F = 4 * fw
     kind = (meta & 3).reshape(S, B)
     rows_ = keys.reshape(S, B).astype(np.int64)
-    is_r, is_w = kind == 1, kind == 2
+    nrows = int(rows_.max()) + 1 if rows_.size else 1
+    rows = nrows if rows is None else rows
+    is_w = kind == 2
+    # rows read per slot: 1 for a read, the clipped range for a scan
+    slen = (meta.reshape(S, B) >> 24).astype(np.int64) + 1
+    nread = np.where(kind == 1, 1, np.where((kind == 3) & (slen <= max_scan), np.minimum(slen, rows - rows_), 0))
     prio = (tids.astype(np.int64) << 24) | np.arange(B, dtype=np.int64)
     big = np.iinfo(np.int64).max
-    nrows = int(rows_.max()) + 1 if rows_.size else 1
-    Wmin = np.full(nrows, big, dtype=np.int64)
+    Wmin = np.full(max(nrows, rows), big, dtype=np.int64)
     pr = np.broadcast_to(prio, (S, B))
     np.minimum.at(Wmin, rows_[is_w], pr[is_w])
     waw = (is_w & (Wmin[rows_] < pr)).any(axis=0)
-    raw = (is_r & (Wmin[rows_] < pr)).any(axis=0)
-    owner = Wmin[rows_] & ((1 << 24) - 1)
-    war_owners = np.unique(owner[is_r & (Wmin[rows_] < big) & (Wmin[rows_] > pr)])
+    # every row read, as (row, txn, priority)
+    cnt = nread.reshape(-1)
+    rr = np.repeat(rows_.reshape(-1), cnt) + (np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    rt = np.repeat(np.broadcast_to(np.arange(B), (S, B)).reshape(-1), cnt)
+    rp = prio[rt]
+    raw = np.zeros(B, dtype=bool)
+    raw[rt[Wmin[rr] < rp]] = True
+    owner = Wmin[rr] & ((1 << 24) - 1)
+    war_owners = np.unique(owner[(Wmin[rr] < big) & (Wmin[rr] > rp)])
     bits = np.unpackbits(bitmap.view(np.uint8), bitorder="little")[:B].astype(bool)
     committed = int(bits.sum())
-    valid = int((is_r | is_w).sum())
+    reg_ops = int(is_w.sum()) + int(cnt.sum())
     w_regs = int((Wmin < big).sum())
-    execute = 4 * B + 12 * S * B + 8 * valid + 8 * w_regs + int(waw.sum()) + int(raw.sum()) + war_owners.size
-    check = 3 * B + B // 8 + committed * 12 * S + int((is_r & bits[None, :]).sum()) * 2 * F
+    execute = 4 * B + 12 * S * B + 8 * reg_ops + 8 * w_regs + int(waw.sum()) + int(raw.sum()) + war_owners.size
+    check = 3 * B + B // 8 + committed * 12 * S + int((nread * bits[None, :]).sum()) * 2 * F
     writeback = B // 8 + committed * (4 + 12 * S) + int((is_w & bits[None, :]).sum()) * 2 * F
     return {"execute": execute, "check": check, "writeback": writeback}, committed
 
@@ -125,7 +139,8 @@ def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
         m = meta.reshape(S, B)[:, :n].copy()
         t = tids[:n].copy()
         t0 = time.perf_counter()
-        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, t, S, values=1, want_reads=True, inplace=True)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, t, S, values=1, want_reads=True, inplace=True,
+                             max_scan=w.get("scan_len", 0))
         dt = time.perf_counter() - t0
         total_s, total_commits, total_txns, epochs = (total_s + dt, total_commits + ep.commit_count,
                                                       total_txns + n, epochs + 1)
@@ -138,17 +153,18 @@ def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
                       f"{total_s:.1f} s"}
 
 
-def run_kv(args, theta, cpu_baseline=True):
+def run_kv(args, theta, cpu_baseline=True, name=None):
     import ltpg_amd as L
 
-    w = kv_workload(args.workload, theta)
+    w = kv_workload(name or args.workload, theta)
     rows, fields, fw, B, S = w["rows"], w["fields"], w["fw"], w["B"], w["S"]
     P = args.batches
     log(f"generating {P} batches: {w['desc']}")
+    scan = w.get("scan_len", 0)
     gen = [L.kv_generate(seed=0x4C545047 + i, rows=rows, fields=fields, n_txns=B, n_slots=S,
-                         read_pct=w["read_pct"], theta=w["theta"], tid_base=0) for i in range(P)]
+                         read_pct=w["read_pct"], theta=w["theta"], tid_base=0, scan_len=scan) for i in range(P)]
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
-                    max_txns=B, kv_max_slots=S, values=L.VALUES_RMW)
+                    max_txns=B, kv_max_slots=S, values=L.VALUES_RMW, kv_max_scan=scan)
     ctx.kv_fill_reference()
     dev = [L.DeviceKvBatch(ctx, k, m, t, S) for (k, m, t) in gen]
     log("batches resident in HBM")
@@ -160,7 +176,8 @@ def run_kv(args, theta, cpu_baseline=True):
         ctx.kv_submit_device(dev[i])
         ctx.epoch_run()
         ctx.sync()
-        per_batch.append(kv_kernel_bytes(gen[i][0], gen[i][1], gen[i][2], ctx.commit_bitmap(), B, S, fw))
+        per_batch.append(kv_kernel_bytes(gen[i][0], gen[i][1], gen[i][2], ctx.commit_bitmap(), B, S, fw, rows,
+                                         scan))
 
     shape, shape_ms = choose_kv_shape(ctx, dev[0]) if not args.default_shape else (0, [])
     log(f"launch shape flags {shape}")
@@ -203,8 +220,12 @@ def run_kv(args, theta, cpu_baseline=True):
     # atomicMin per write op and one W load per read op, against the rates
     # this device sustains for them over an array of the registers' size
     load_gops, atomic_gops = L.random_access_rates(ctx, rows * 16, 1 << 24)
+    def rows_read(meta_):
+        k_ = meta_ & 3
+        return int((k_ == 1).sum()) + int(((meta_ >> 24).astype(np.int64) + 1)[k_ == 3].sum())
+
     n_w = sum(int(((gen[j % P][1] & 3) == 2).sum()) for j in range(args.steps)) / args.steps
-    n_r = sum(int(((gen[j % P][1] & 3) == 1).sum()) for j in range(args.steps)) / args.steps
+    n_r = sum(rows_read(gen[j % P][1]) for j in range(args.steps)) / args.steps
     bound_ms = n_w / (atomic_gops * 1e6) + n_r / (load_gops * 1e6)
     random_access = {"kernel": "kv_execute", "write_ops": n_w, "read_ops": n_r,
                      "probe_atomic_gops": atomic_gops, "load_gops": load_gops,
@@ -684,7 +705,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kv-c2", choices=["kv-c2", "kv-c1", "tpcc-c3", "batched-c4", "served-c5"])
+    ap.add_argument("--workload", default="kv-c2",
+                    choices=["kv-c2", "kv-c1", "kv-scan", "tpcc-c3", "batched-c4", "served-c5"])
     ap.add_argument("--c5-rates", type=float, nargs="+", default=[1, 5, 10, 25, 50],
                     help="offered loads of the served form, M txn/s")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
@@ -735,6 +757,8 @@ def main():
         if args.workload == "kv-c2" and args.theta is None:
             v = run_tpcc(args)
             variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
+            v = run_kv(args, None, name="kv-scan")
+            variants.append({k: v[k] for k in keep + ("cpu_baseline",) if k in v})
             if not args.no_batched:
                 variants.append(run_batched_c4(args))
                 variants.append(run_served_c5(args))

@@ -662,11 +662,30 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint32_t t = has ? s_list[j] : 0u;
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields, max_scan);
-        for (uint32_t k = 0;; ++k) {  // row k of each read / scan
-            const unsigned long long rd = __ballot(k < o.len);
-            if (!rd) break;
-            group_copy<G, 8>(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
-                             read_values, fw, S, grp, gl);
+        if (fw <= 2 && L > 1) {
+            // narrow fields of scans: each lane copies its own rows, four
+            // loads in flight at a time
+            const uint64_t src0 = (o.row * fields + o.field) * fsw, step = (uint64_t)fields * fsw;
+            for (uint32_t k0 = 0; k0 < o.len; k0 += 4) {
+                uint32_t v[4][2];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+                    for (uint32_t w = 0; w < 2; ++w)
+                        v[u][w] = table[(k0 + u < o.len && w < fw) ? src0 + (k0 + u) * step + w : 0ull];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+                    for (uint32_t w = 0; w < 2; ++w)
+                        if (k0 + u < o.len && w < fw) read_values[(i * L + k0 + u) * fw + w] = v[u][w];
+            }
+        } else {
+            for (uint32_t k = 0;; ++k) {  // row k of each read / scan
+                const unsigned long long rd = __ballot(k < o.len);
+                if (!rd) break;
+                group_copy<G, 8>(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                                 read_values, fw, S, grp, gl);
+            }
         }
     }
     const uint32_t nwords = (B + 63) / 64;
