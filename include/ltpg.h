@@ -95,6 +95,8 @@ extern "C" {
 #define LTPG_TYPE_KV 0u
 #define LTPG_TYPE_NEWORDER 1u
 #define LTPG_TYPE_PAYMENT 2u
+#define LTPG_TYPE_ORDERSTATUS 3u /* read-only order-entry types, see       */
+#define LTPG_TYPE_STOCKLEVEL 4u  /* ltpg_tpcc_submit_readonly               */
 
 /* ---- phases ------------------------------------------------------------ */
 #define LTPG_PHASE_CID 1u
@@ -286,6 +288,38 @@ void ltpg_payment_view(const uint32_t *block, uint32_t cap, uint32_t n, ltpg_pay
  * NewOrder q has batch position q, Payment p has position n_neworder + p. */
 int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *neworder,
                      const ltpg_payment_batch *payment, uint32_t mem);
+/* Read-only order-entry transactions (the Order-Status and Stock-Level
+ * slots of TPCC-alltxn/Execute.h:1592-1770,2155-2231, whose query sets the
+ * reference declares but never fills, TPCC-alltxn/Query.h:390-392).
+ *   Order-Status o reads CUSTOMER row W_ID*30000 + D_ID*3000 + C_ID (column
+ *     2), ORDER row W_ID*30000 + O_ID (column 0) and ORDERLINE row
+ *     W_ID*450000 + OL_ID (column 0) (op_0 .. op_2);
+ *   Stock-Level k reads STOCK row W_ID*100000 + S_ID (column 2) (op_0).
+ * Each read registers R and counts raw against W; having no writes, the
+ * transactions always commit (Aria's rule with war = waw = 0), and their R
+ * registrations give war to NewOrders that write those rows.  Priorities:
+ * Order-Status o has batch position n_neworder + n_payment + o, Stock-Level
+ * k position n_neworder + n_payment + n_orderstatus + k.  The query's C_LAST
+ * and isName fields are not used by the reference's Order-Status slots and
+ * are not taken.  Not available with LTPG_FLAG_CONFLICT_COUNTS. */
+typedef struct ltpg_orderstatus_batch {
+    uint32_t n;
+    uint32_t reserved;
+    const uint32_t *W_ID, *D_ID, *C_ID, *O_ID, *OL_ID, *TID;  /* all [n]      */
+} ltpg_orderstatus_batch;
+
+typedef struct ltpg_stocklevel_batch {
+    uint32_t n;
+    uint32_t reserved;
+    const uint32_t *W_ID, *S_ID, *TID;                         /* all [n]      */
+} ltpg_stocklevel_batch;
+
+/* Add read-only transactions to the batch of the last ltpg_tpcc_submit
+ * (which starts every epoch with none).  Either may be NULL or empty; all
+ * four types together must fit max_txns. */
+int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *orderstatus,
+                              const ltpg_stocklevel_batch *stocklevel, uint32_t mem);
+
 /* C_ID used by each Payment of the current batch: the query's C_ID, or,
  * once the CID phase has run, the by-name resolution for isName Payments
  * (KERNEL_CID writes it back into PAYMENT_QUERY.C_ID, TPCC-allopt-final/
@@ -301,6 +335,8 @@ int ltpg_tpcc_payment_cid(ltpg_ctx *ctx, uint32_t *out, uint32_t n);
  *     then I_PRICE, I_DATA, I_NAME of lines 0..14 (0 beyond O_OL_CNT).
  *   PAYMENT (24): WAREHOUSE cols 3..8; DISTRICT cols 5..10; CUSTOMER cols
  *     3, 4, 11..20.
+ *   ORDERSTATUS (3): CUSTOMER col 2, ORDER col 0, ORDERLINE col 0.
+ *   STOCKLEVEL (1): STOCK col 2.
  * A slot whose row is outside its table reads 0; aborted transactions'
  * entries are not written. */
 int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t n_words);

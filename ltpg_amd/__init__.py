@@ -38,7 +38,7 @@ VALUES_COPYBACK, VALUES_RMW = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 LAYOUT_ROW_MAJOR, LAYOUT_COLUMN_MAJOR = 0, 1
 WORKLOAD_KV, WORKLOAD_TPCC = 1, 2
-TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT = 0, 1, 2
+TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT, TYPE_ORDERSTATUS, TYPE_STOCKLEVEL = 0, 1, 2, 3, 4
 PHASE_CID, PHASE_EXECUTE, PHASE_CHECK, PHASE_WRITEBACK, PHASE_ALL = 1, 2, 4, 8, 15
 KV_NONE, KV_READ, KV_WRITE, KV_SCAN = 0, 1, 2, 3
 NO_HOT_ROW = (1 << 64) - 1
@@ -49,7 +49,9 @@ TPCC_WAREHOUSE, TPCC_DISTRICT, TPCC_CUSTOMER, TPCC_HISTORY, TPCC_NEWORDER, TPCC_
     TPCC_STOCK, TPCC_ITEM = range(9)
 TPCC_TABLES = ["warehouse", "district", "customer", "history", "neworder", "order", "orderline", "stock", "item"]
 TPCC_MAX_LINES = 15
-NEWORDER_READS, PAYMENT_READS = 51, 24
+NEWORDER_READS, PAYMENT_READS, ORDERSTATUS_READS, STOCKLEVEL_READS = 51, 24, 3, 1
+ORDERSTATUS_FIELDS = ("W_ID", "D_ID", "C_ID", "O_ID", "OL_ID", "TID")
+STOCKLEVEL_FIELDS = ("W_ID", "S_ID", "TID")
 # u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
 NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
@@ -177,6 +179,8 @@ class Context:
         self._cur_slots = 0
         self._n_no = 0
         self._n_pay = 0
+        self._n_os = 0
+        self._n_sl = 0
 
     def close(self):
         if self.handle:
@@ -323,6 +327,23 @@ class Context:
             n_no, n_pay = nb.n, pb.n
         _check(lib.ltpg_tpcc_submit(self.handle, ctypes.byref(nb), ctypes.byref(pb), mem), "ltpg_tpcc_submit")
         self._n_no, self._n_pay = n_no, n_pay
+        self._n_os = self._n_sl = 0
+
+    def tpcc_submit_readonly(self, ro, mem=MEM_HOST):
+        """Add a ReadOnlyBatch (host arrays), or with mem=MEM_DEVICE a pair
+        of (OrderstatusBatch, StocklevelBatch) structs of device pointers, to
+        the batch of the last tpcc_submit."""
+        if mem == MEM_HOST:
+            ob, sb = ro.structs()
+        else:
+            ob, sb = ro
+        _check(lib.ltpg_tpcc_submit_readonly(self.handle, ctypes.byref(ob), ctypes.byref(sb), mem),
+               "ltpg_tpcc_submit_readonly")
+        self._n_os, self._n_sl = ob.n, sb.n
+
+    def _tpcc_n(self, type_):
+        return {TYPE_NEWORDER: self._n_no, TYPE_PAYMENT: self._n_pay, TYPE_ORDERSTATUS: self._n_os,
+                TYPE_STOCKLEVEL: self._n_sl}[type_]
 
     def tpcc_payment_cid(self):
         out = np.zeros(max(self._n_pay, 1), dtype=np.uint32)
@@ -331,14 +352,16 @@ class Context:
         return out[:self._n_pay]
 
     def tpcc_read_values(self, type_):
-        n = self._n_no * NEWORDER_READS if type_ == TYPE_NEWORDER else self._n_pay * PAYMENT_READS
+        per = {TYPE_NEWORDER: NEWORDER_READS, TYPE_PAYMENT: PAYMENT_READS, TYPE_ORDERSTATUS: ORDERSTATUS_READS,
+               TYPE_STOCKLEVEL: STOCKLEVEL_READS}[type_]
+        n = self._tpcc_n(type_) * per
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib.ltpg_tpcc_read_values(self.handle, type_, out.ctypes.data_as(_lib.c_u32p), n),
                "ltpg_tpcc_read_values")
         return out[:n]
 
     def tpcc_bitmap(self, type_):
-        return self.commit_bitmap(type_, self._n_no if type_ == TYPE_NEWORDER else self._n_pay)
+        return self.commit_bitmap(type_, self._tpcc_n(type_))
 
     def tpcc_conflicts(self, type_):
         return self.conflict_counters(type_, self._n_no if type_ == TYPE_NEWORDER else self._n_pay)
@@ -352,6 +375,26 @@ class Context:
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib.ltpg_kv_read_values(self.handle, out.ctypes.data_as(_lib.c_u32p), n), "ltpg_kv_read_values")
         return out[:n]
+
+
+class ReadOnlyBatch:
+    """Host arrays of read-only order-entry transactions: Order-Status
+    fields (W_ID, D_ID, C_ID, O_ID, OL_ID, TID) and Stock-Level fields
+    (W_ID, S_ID, TID), one u32 array each."""
+
+    def __init__(self, n_os, n_sl):
+        self.n_os, self.n_sl = int(n_os), int(n_sl)
+        self.os = {f: np.zeros(max(self.n_os, 1), dtype=np.uint32) for f in ORDERSTATUS_FIELDS}
+        self.sl = {f: np.zeros(max(self.n_sl, 1), dtype=np.uint32) for f in STOCKLEVEL_FIELDS}
+
+    def structs(self):
+        ob, sb = _lib.OrderstatusBatch(), _lib.StocklevelBatch()
+        ob.n, sb.n = self.n_os, self.n_sl
+        for f in ORDERSTATUS_FIELDS:
+            setattr(ob, f, self.os[f].ctypes.data)
+        for f in STOCKLEVEL_FIELDS:
+            setattr(sb, f, self.sl[f].ctypes.data)
+        return ob, sb
 
 
 def neworder_struct(block, cap, n):
@@ -439,6 +482,61 @@ class TpccGenerator:
             self.close()
         except Exception:
             pass
+
+
+def readonly_batch(b, warehouses, n_os, n_sl, seed, tid_lo=0, tid_hi=None, hit_pct=50):
+    """Seeded Order-Status / Stock-Level transactions beside a TpccBatch b:
+    hit_pct % of them read rows b's NewOrders write (the order, one of its
+    order lines, the stock row of one of its lines), the rest uniform rows
+    (Stock-Level items skewed as the NewOrder items: OR of two uniform
+    draws).  TIDs are drawn in [tid_lo, tid_hi), so their priorities
+    interleave with the batch's."""
+    rng = np.random.default_rng(seed)
+    W = warehouses
+    tid_hi = tid_lo + b.n_no + b.n_pay + n_os + n_sl if tid_hi is None else tid_hi
+    ro = ReadOnlyBatch(n_os, n_sl)
+    nos = b.n_no
+    hit_os = (rng.random(n_os) < hit_pct / 100.0) & (nos > 0)
+    q = rng.integers(0, max(nos, 1), n_os)
+    nl = np.maximum(b.nf("O_OL_CNT").astype(np.int64), 1) if nos else np.ones(1, np.int64)
+    line = (rng.random(n_os) * nl[q]).astype(np.int64)
+    uni = lambda lo, hi, n: rng.integers(lo, hi, n).astype(np.uint32)
+    if n_os:
+        ro.os["W_ID"][:n_os] = np.where(hit_os, b.nf("W_ID")[q] if nos else 0, uni(0, W, n_os))
+        ro.os["D_ID"][:n_os] = np.where(hit_os, b.nf("D_ID")[q] if nos else 0, uni(0, 10, n_os))
+        ro.os["C_ID"][:n_os] = np.where(hit_os, b.nf("C_ID")[q] if nos else 0, uni(0, 3000, n_os))
+        ro.os["O_ID"][:n_os] = np.where(hit_os, b.nf("O_ID")[q] if nos else 0, uni(0, 30000, n_os))
+        ro.os["OL_ID"][:n_os] = np.where(hit_os, (b.nf("O_OL_ID")[q] + line) if nos else 0, uni(0, 450000, n_os))
+        ro.os["TID"][:n_os] = uni(tid_lo, tid_hi, n_os)
+    if n_sl:
+        hit_sl = (rng.random(n_sl) < hit_pct / 100.0) & (nos > 0)
+        q2 = rng.integers(0, max(nos, 1), n_sl)
+        l2 = (rng.random(n_sl) * nl[q2]).astype(np.int64)
+        if nos:
+            sw = np.array([b.nf("OL_SUPPLY_W_ID", int(k))[int(t)] for t, k in zip(q2, l2)], dtype=np.uint32)
+            it = np.array([b.nf("OL_I_ID", int(k))[int(t)] for t, k in zip(q2, l2)], dtype=np.uint32)
+        else:
+            sw = it = np.zeros(n_sl, np.uint32)
+        skew = ((rng.integers(0, 8192, n_sl) | rng.integers(1, 100001, n_sl)) % 100000).astype(np.uint32)
+        ro.sl["W_ID"][:n_sl] = np.where(hit_sl, sw, uni(0, W, n_sl))
+        ro.sl["S_ID"][:n_sl] = np.where(hit_sl, it, skew)
+        ro.sl["TID"][:n_sl] = uni(tid_lo, tid_hi, n_sl)
+    return ro
+
+
+def upload_readonly_batch(ctx, ro):
+    """Copy a ReadOnlyBatch into device buffers of ctx; returns
+    ((OrderstatusBatch, StocklevelBatch) of device pointers, [buffers])."""
+    bufs = []
+    ob, sb = _lib.OrderstatusBatch(), _lib.StocklevelBatch()
+    ob.n, sb.n = ro.n_os, ro.n_sl
+    for st, arrays, fields in ((ob, ro.os, ORDERSTATUS_FIELDS), (sb, ro.sl, STOCKLEVEL_FIELDS)):
+        for f in fields:
+            buf = DeviceBuffer(ctx, max(arrays[f].nbytes, 16))
+            buf.upload(arrays[f])
+            bufs.append(buf)
+            setattr(st, f, buf.ptr)
+    return (ob, sb), bufs
 
 
 def upload_tpcc_batch(ctx, b):

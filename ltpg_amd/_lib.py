@@ -86,6 +86,20 @@ class PaymentBatch(ctypes.Structure):
                                              "H_AMOUNT", "H_ID", "TID")]
 
 
+class OrderstatusBatch(ctypes.Structure):
+    """struct ltpg_orderstatus_batch."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)] + [
+        (name, ctypes.c_void_p) for name in ("W_ID", "D_ID", "C_ID", "O_ID", "OL_ID", "TID")]
+
+
+class StocklevelBatch(ctypes.Structure):
+    """struct ltpg_stocklevel_batch."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)] + [
+        (name, ctypes.c_void_p) for name in ("W_ID", "S_ID", "TID")]
+
+
 class TpccGenConfig(ctypes.Structure):
     """struct ltpg_tpcc_gen_config."""
 
@@ -175,6 +189,8 @@ SIGNATURES = [
      [c_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(PaymentBatch)]),
     ("ltpg_tpcc_submit", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), ctypes.c_uint32]),
+    ("ltpg_tpcc_submit_readonly", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(OrderstatusBatch), ctypes.POINTER(StocklevelBatch), ctypes.c_uint32]),
     ("ltpg_tpcc_payment_cid", ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_uint32]),
     ("ltpg_tpcc_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint64]),
     ("ltpg_tpcc_gen_config_init", None, [ctypes.POINTER(TpccGenConfig)]),

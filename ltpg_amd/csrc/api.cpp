@@ -118,6 +118,12 @@ void free_tpcc(TpccState &tp)
     dev_free(tp.d.cid);
     dev_free(tp.no_buf);
     dev_free(tp.pay_buf);
+    dev_free(tp.ro_buf);
+    dev_free(tp.d.war);
+    dev_free(tp.d.bitmap_os);
+    dev_free(tp.d.bitmap_sl);
+    dev_free(tp.d.reads_os);
+    dev_free(tp.d.reads_sl);
 }
 
 int init_tpcc(ltpg_ctx *ctx)
@@ -159,6 +165,13 @@ int init_tpcc(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&d.cid, B, "Payment C_ID"))) return rc;
     if ((rc = dev_alloc(&tp.no_buf, 53 * B, "NewOrder batch"))) return rc;
     if ((rc = dev_alloc(&tp.pay_buf, 10 * B, "Payment batch"))) return rc;
+    if ((rc = dev_alloc(&tp.ro_buf, 9 * B, "read-only batches"))) return rc;
+    if ((rc = dev_alloc(&d.war, B, "read-only war flags"))) return rc;
+    if ((rc = dev_alloc(&d.bitmap_os, (B + 63) / 64, "Order-Status bitmap"))) return rc;
+    if ((rc = dev_alloc(&d.bitmap_sl, (B + 63) / 64, "Stock-Level bitmap"))) return rc;
+    if ((rc = dev_alloc(&d.reads_os, (uint64_t)kOsReads * B, "Order-Status read values"))) return rc;
+    if ((rc = dev_alloc(&d.reads_sl, (uint64_t)kSlReads * B, "Stock-Level read values"))) return rc;
+    LTPG_HIP_TRY(launch_fill_u8(d.war, B, 0, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u64(d.regs, d.n_regs, ~0ull, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u8(d.doomed, B, 0, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u64(d.new_last, d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
@@ -184,6 +197,7 @@ int refill_registers(ltpg_ctx *ctx)
     if (ctx->tpcc.d.regs) {
         LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.regs, ctx->tpcc.d.n_regs, ~0ull, ctx->stream));
         LTPG_HIP_TRY(launch_fill_u8(ctx->tpcc.d.doomed, ctx->cfg.max_txns, 0, ctx->stream));
+        LTPG_HIP_TRY(launch_fill_u8(ctx->tpcc.d.war, ctx->cfg.max_txns, 0, ctx->stream));
         LTPG_HIP_TRY(launch_fill_u64(ctx->tpcc.d.new_last, ctx->tpcc.d.t[LTPG_TPCC_NEWORDER].rows, 0, ctx->stream));
     }
     return LTPG_OK;
@@ -420,7 +434,7 @@ int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    if (type > LTPG_TYPE_STOCKLEVEL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     static_assert(sizeof(DeviceCounters::totals) == kCounterStripes * 32 * 8, "stripe layout");
     unsigned long long h[kCounterStripes][32];
@@ -428,7 +442,7 @@ int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t
     uint64_t c = 0, a = 0;
     for (int k = 0; k < kCounterStripes; k++) {
         c += h[k][type];
-        a += h[k][3 + type];
+        a += h[k][kAttempts + type];
     }
     if (commits) *commits = c;
     if (attempts) *attempts = a;
@@ -687,12 +701,16 @@ static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     }
     if (phase_mask & LTPG_PHASE_CID) LTPG_HIP_TRY(launch_tpcc_cid(tp.d, tp.pay, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
-    if (phase_mask & LTPG_PHASE_EXECUTE)
+    const bool ro = tp.ro.n_os + tp.ro.n_sl > 0;
+    if (phase_mask & LTPG_PHASE_EXECUTE) {
         LTPG_HIP_TRY(launch_tpcc_execute(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
+        // read-only transactions mark war on the registered writers
+        if (ro) LTPG_HIP_TRY(launch_tpcc_readonly(tp.d, tp.ro, tp.no.n + tp.pay.n, e, ctx->counters, s));
+    }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
     if (phase_mask & LTPG_PHASE_CHECK) {
         if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
-        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
+        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ro, ctx->counters, s));
     }
     if (ev) {
         LTPG_HIP_TRY(hipEventRecord(ev[3], s));
@@ -768,6 +786,12 @@ int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_w
     } else if (type == LTPG_TYPE_PAYMENT && ctx->tpcc.d.regs) {
         src = ctx->tpcc.d.bitmap_pay;
         n = ctx->tpcc.pay.n;
+    } else if (type == LTPG_TYPE_ORDERSTATUS && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.bitmap_os;
+        n = ctx->tpcc.ro.n_os;
+    } else if (type == LTPG_TYPE_STOCKLEVEL && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.bitmap_sl;
+        n = ctx->tpcc.ro.n_sl;
     } else {
         LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "transaction type not in this context's workload");
     }
@@ -783,7 +807,7 @@ int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
-    if (type > LTPG_TYPE_PAYMENT) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    if (type > LTPG_TYPE_STOCKLEVEL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     uint32_t h[kCounterStripes][64];
     LTPG_HIP_TRY(hipMemcpy(h, ctx->counters->commits, sizeof(h), hipMemcpyDeviceToHost));
@@ -982,7 +1006,57 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
         LTPG_HIP_TRY(hipMemcpyAsync(tp.d.cid, pay.C_ID, (uint64_t)n_pay * 4, hipMemcpyDeviceToDevice, ctx->stream));
     tp.no = no;
     tp.pay = pay;
+    tp.ro = ReadOnlyDev();
     tp.has_batch = true;
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *ob, const ltpg_stocklevel_batch *sb,
+                              uint32_t mem)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    TpccState &tp = ctx->tpcc;
+    if (!tp.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "ltpg_tpcc_submit first");
+    if (mem != LTPG_MEM_HOST && mem != LTPG_MEM_DEVICE) LTPG_FAIL(LTPG_ERR_INVALID, "unknown memory kind");
+    if (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "read-only transactions need the fast check (no LTPG_FLAG_CONFLICT_COUNTS)");
+    const uint32_t n_os = ob ? ob->n : 0, n_sl = sb ? sb->n : 0;
+    if ((uint64_t)tp.no.n + tp.pay.n + n_os + n_sl > ctx->cfg.max_txns)
+        LTPG_FAIL(LTPG_ERR_INVALID, "batch exceeds max_txns");
+    if (n_os && (!ob->W_ID || !ob->D_ID || !ob->C_ID || !ob->O_ID || !ob->OL_ID || !ob->TID))
+        LTPG_FAIL(LTPG_ERR_INVALID, "null Order-Status array");
+    if (n_sl && (!sb->W_ID || !sb->S_ID || !sb->TID)) LTPG_FAIL(LTPG_ERR_INVALID, "null Stock-Level array");
+    ReadOnlyDev ro;
+    ro.n_os = n_os;
+    ro.n_sl = n_sl;
+    const uint32_t *os_src[6] = {n_os ? ob->W_ID : nullptr, n_os ? ob->D_ID : nullptr, n_os ? ob->C_ID : nullptr,
+                                 n_os ? ob->O_ID : nullptr, n_os ? ob->OL_ID : nullptr, n_os ? ob->TID : nullptr};
+    const uint32_t *sl_src[3] = {n_sl ? sb->W_ID : nullptr, n_sl ? sb->S_ID : nullptr, n_sl ? sb->TID : nullptr};
+    const uint32_t *os_dev[6], *sl_dev[3];
+    if (mem == LTPG_MEM_DEVICE) {
+        for (int k = 0; k < 6; k++) os_dev[k] = os_src[k];
+        for (int k = 0; k < 3; k++) sl_dev[k] = sl_src[k];
+    } else {
+        const uint64_t C = ctx->cfg.max_txns;
+        for (int k = 0; k < 6; k++) {
+            os_dev[k] = tp.ro_buf + k * C;
+            if (n_os)
+                LTPG_HIP_TRY(hipMemcpyAsync(tp.ro_buf + k * C, os_src[k], (uint64_t)n_os * 4, hipMemcpyHostToDevice,
+                                            ctx->stream));
+        }
+        for (int k = 0; k < 3; k++) {
+            sl_dev[k] = tp.ro_buf + (6 + k) * C;
+            if (n_sl)
+                LTPG_HIP_TRY(hipMemcpyAsync(tp.ro_buf + (6 + k) * C, sl_src[k], (uint64_t)n_sl * 4,
+                                            hipMemcpyHostToDevice, ctx->stream));
+        }
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));  // pageable sources
+    }
+    ro.os_W_ID = os_dev[0]; ro.os_D_ID = os_dev[1]; ro.os_C_ID = os_dev[2];
+    ro.os_O_ID = os_dev[3]; ro.os_OL_ID = os_dev[4]; ro.os_TID = os_dev[5];
+    ro.sl_W_ID = sl_dev[0]; ro.sl_S_ID = sl_dev[1]; ro.sl_TID = sl_dev[2];
+    tp.ro = ro;
     return LTPG_OK;
 }
 
@@ -1010,8 +1084,14 @@ int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t 
     } else if (type == LTPG_TYPE_PAYMENT) {
         src = ctx->tpcc.d.reads_pay;
         have = (uint64_t)kPayReads * ctx->tpcc.pay.n;
+    } else if (type == LTPG_TYPE_ORDERSTATUS) {
+        src = ctx->tpcc.d.reads_os;
+        have = (uint64_t)kOsReads * ctx->tpcc.ro.n_os;
+    } else if (type == LTPG_TYPE_STOCKLEVEL) {
+        src = ctx->tpcc.d.reads_sl;
+        have = (uint64_t)kSlReads * ctx->tpcc.ro.n_sl;
     } else {
-        LTPG_FAIL(LTPG_ERR_INVALID, "type must be NEWORDER or PAYMENT");
+        LTPG_FAIL(LTPG_ERR_INVALID, "type must be an order-entry type");
     }
     if (n_words > have) LTPG_FAIL(LTPG_ERR_INVALID, "n_words exceeds the batch's read values");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));

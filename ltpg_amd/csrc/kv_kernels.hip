@@ -125,7 +125,7 @@ __device__ __forceinline__ void add_counts(DeviceCounters *ctr, uint32_t stripe,
         atomicAdd(&ctr->commits[stripe][type], commits);
         atomicAdd(&ctr->totals[stripe][type], (unsigned long long)commits);
     }
-    if (attempts) atomicAdd(&ctr->totals[stripe][3 + type], (unsigned long long)attempts);
+    if (attempts) atomicAdd(&ctr->totals[stripe][kAttempts + type], (unsigned long long)attempts);
 }
 
 // Per-op decode shared by every phase.
@@ -169,7 +169,8 @@ __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
 // epoch can be captured in a graph).
 __global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
 {
-    if (threadIdx.x < kCounterStripes * 3) ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    for (uint32_t i = threadIdx.x; i < kCounterStripes * kCountedTypes; i += blockDim.x)
+        ctr->commits[i / kCountedTypes][i % kCountedTypes] = 0;
 }
 
 // Streaming 16-byte copy, grid-stride.
@@ -310,8 +311,9 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
     // the epoch's commit counts start at zero (CHECK adds to them)
-    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
-        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    if (zero_counts && blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
+            ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     bool waw = false;
     uint32_t bad = 0;
@@ -341,8 +343,9 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts, uint32_t max_scan,
     DeviceCounters *__restrict__ ctr)
 {
-    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
-        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    if (zero_counts && blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
+            ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < (uint64_t)S * B;
     const uint64_t s = live ? i / B : 0;
@@ -382,8 +385,9 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
 {
     __shared__ unsigned long long s_key[kDedupSlots];  // row + 1, 0 = empty
     __shared__ unsigned long long s_min[kDedupSlots];
-    if (zero_counts && blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
-        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    if (zero_counts && blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
+            ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     for (uint32_t k = threadIdx.x; k < kDedupSlots; k += kBlock) {
         s_key[k] = 0ull;
         s_min[k] = ~0ull;
@@ -843,7 +847,7 @@ inline dim3 grid_stride(uint64_t n)
 
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s)
 {
-    hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(kCounterStripes * 3), 0, s, ctr);
+    hipLaunchKernelGGL(reset_epoch_counters_kernel, dim3(1), dim3(kBlock), 0, s, ctr);
     return hipGetLastError();
 }
 

@@ -35,12 +35,15 @@ void set_error(const std::string &msg);
 // the stripes, so no single address takes an atomic from every block.
 constexpr int kCounterStripes = 64;
 
+constexpr uint32_t kAttempts = 16;  // attempts of type t at totals[stripe][kAttempts + t]
+constexpr uint32_t kCountedTypes = 5;  // LTPG_TYPE_KV .. LTPG_TYPE_STOCKLEVEL
+
 struct DeviceCounters {
     uint32_t range_errors;                            // ops outside their table
     uint32_t pad0[63];
     uint32_t commits[kCounterStripes][64];            // [stripe][type]: last epoch
     unsigned long long totals[kCounterStripes][32];   // [stripe][type]: commits,
-                                                      // [stripe][3 + type]: attempts
+                                                      // [stripe][kAttempts + type]: attempts
 };
 
 // One key-value batch as the kernels see it (device pointers).
@@ -98,6 +101,8 @@ struct KvState {
 constexpr int kTpccTables = LTPG_TPCC_NTABLES;
 constexpr uint32_t kNoReads = 51;   // read values per NewOrder
 constexpr uint32_t kPayReads = 24;  // read values per Payment
+constexpr uint32_t kOsReads = 3;    // read values per Order-Status
+constexpr uint32_t kSlReads = 1;    // read values per Stock-Level
 
 // One table on the device: row-major, `stride` u32 words per row (cols
 // rounded up to a multiple of 4 so rows start 16-byte aligned).
@@ -122,6 +127,13 @@ struct PaymentDev {
     const uint32_t *W_ID = nullptr, *D_ID = nullptr, *C_ID = nullptr, *C_LAST = nullptr, *isName = nullptr,
                    *C_D_ID = nullptr, *C_W_ID = nullptr, *H_AMOUNT = nullptr, *H_ID = nullptr, *TID = nullptr;
 };
+// Read-only transactions (Order-Status, Stock-Level) of the current batch.
+struct ReadOnlyDev {
+    uint32_t n_os = 0, n_sl = 0;
+    const uint32_t *os_W_ID = nullptr, *os_D_ID = nullptr, *os_C_ID = nullptr, *os_O_ID = nullptr,
+                   *os_OL_ID = nullptr, *os_TID = nullptr;
+    const uint32_t *sl_W_ID = nullptr, *sl_S_ID = nullptr, *sl_TID = nullptr;
+};
 
 // Everything the order-entry kernels take, passed by value.
 struct TpccDev {
@@ -142,14 +154,21 @@ struct TpccDev {
     uint32_t *conf_no = nullptr, *conf_pay = nullptr;
     uint32_t *reads_no = nullptr, *reads_pay = nullptr;  // [51 * n_no], [24 * n_pay] value-major
     uint32_t *cid = nullptr;                             // [n_pay] C_ID each Payment uses
+    // read-only transactions: war flags they give NewOrders (epoch number),
+    // their bitmaps and read values (value-major)
+    uint8_t *war = nullptr;
+    uint64_t *bitmap_os = nullptr, *bitmap_sl = nullptr;
+    uint32_t *reads_os = nullptr, *reads_sl = nullptr;
 };
 
 struct TpccState {
     TpccDev d;
     uint32_t *no_buf = nullptr;   // packed NEWORDER_QUERY, capacity max_txns
     uint32_t *pay_buf = nullptr;  // packed PAYMENT_QUERY, capacity max_txns
+    uint32_t *ro_buf = nullptr;   // read-only batches: 6 + 3 arrays of capacity max_txns
     NeworderDev no;
     PaymentDev pay;
+    ReadOnlyDev ro;
     bool has_batch = false;
 };
 
@@ -244,8 +263,10 @@ hipError_t launch_tpcc_relayout(const uint32_t *src, uint32_t *dst, uint64_t row
 hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t s);
 hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
                                const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
-hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
-                             const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_t base, const EpochTag &e,
+                                DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                             bool counts, bool ro, DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
                                  const EpochTag &e, uint32_t values, hipStream_t s);
 

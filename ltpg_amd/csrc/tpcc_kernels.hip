@@ -269,8 +269,9 @@ __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev 
                                                         uint8_t epoch, uint32_t no_blocks, DeviceCounters *ctr)
 {
     // the epoch's commit counts start at zero (CHECK adds to them)
-    if (blockIdx.x == 0 && threadIdx.x < kCounterStripes * 3)
-        ctr->commits[threadIdx.x / 3][threadIdx.x % 3] = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
+            ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     uint32_t bad = 0;
     if (blockIdx.x < no_blocks) {
         const uint32_t q = blockIdx.x * kNoPerBlock + threadIdx.x / 16;
@@ -404,9 +405,77 @@ __device__ __forceinline__ void gather_payment(const TpccDev &d, const PaymentDe
     for (uint32_t k = 0; k < kPayReads; k++) out[(uint64_t)k * n] = v[k];
 }
 
+// Read-only transactions, one thread each, after every write has
+// registered (Order-Status op_0..op_2, TPCC-alltxn/Execute.h:1594-1672;
+// Stock-Level op_0, :2157-2181; their check slots consult W, :1710-1745,
+// 2203-2214).  They register no R: the only use of R in the commit rule is
+// war of a row's writer, and a writer without waw is the row's minimum, the
+// owner of W; so a read of row x against w = W[x] of this epoch with
+// w > me flags w's owner with war directly.  A read-only transaction always
+// commits (war = waw = 0); the thread gathers its read values (the table is
+// the snapshot until WRITEBACK) and sets its commit bit.
+__device__ __forceinline__ void ro_mark(const TpccDev &d, const unsigned long long *reg, unsigned long long me,
+                                        uint32_t tag, uint8_t epoch)
+{
+    const unsigned long long w = *reg;
+    if ((uint32_t)(w >> kTagShift) == tag && w > me) d.war[w & kPosMask] = epoch;
+}
+
+__global__ __launch_bounds__(kBlock) void readonly_kernel(TpccDev d, ReadOnlyDev ro, uint32_t base, uint32_t tag,
+                                                         uint8_t epoch, uint32_t os_waves, DeviceCounters *ctr)
+{
+    __shared__ uint32_t s_cnt[2];  // Order-Status, Stock-Level transactions (all commit)
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    const bool is_os = wave < os_waves;
+    const uint32_t i = (is_os ? wave : wave - os_waves) * 64 + lane;
+    const bool live = i < (is_os ? ro.n_os : ro.n_sl);
+    uint32_t bad = 0;
+    if (live && is_os) {
+        const uint64_t w = ro.os_W_ID[i];
+        const uint64_t cu = w * 30000 + (uint64_t)ro.os_D_ID[i] * 3000 + ro.os_C_ID[i];   // op_0_0
+        const uint64_t od = w * 30000 + ro.os_O_ID[i];                                     // op_1_0
+        const uint64_t ol = w * 450000 + ro.os_OL_ID[i];                                   // op_2_0
+        const unsigned long long me = priority(tag, ro.os_TID[i], base + i);
+        const bool okc = cu < d.t[CU].rows, oko = od < d.t[OR].rows, okl = ol < d.t[OL].rows;
+        uint32_t v[kOsReads];
+        v[0] = okc ? d.t[CU].data[cu * kStride[CU] + 2] : 0u;  // op_0_1
+        v[1] = oko ? d.t[OR].data[od * kStride[OR]] : 0u;      // op_1_1
+        v[2] = okl ? d.t[OL].data[ol * kStride[OL]] : 0u;      // op_2_1
+        if (okc) ro_mark(d, cust_w(d, cu), me, tag, epoch); else ++bad;
+        if (oko) ro_mark(d, d.regs + d.off_order + od, me, tag, epoch); else ++bad;
+        if (okl) ro_mark(d, d.regs + d.off_line + ol, me, tag, epoch); else ++bad;
+#pragma unroll
+        for (uint32_t k = 0; k < kOsReads; k++) d.reads_os[(uint64_t)k * ro.n_os + i] = v[k];
+    } else if (live) {
+        const uint64_t st = (uint64_t)ro.sl_W_ID[i] * 100000 + ro.sl_S_ID[i];             // op_0_0
+        const unsigned long long me = priority(tag, ro.sl_TID[i], base + ro.n_os + i);
+        const bool oks = st < d.t[ST].rows;
+        d.reads_sl[i] = oks ? d.t[ST].data[st * kStride[ST] + 2] : 0u;                     // op_0_1
+        if (oks) ro_mark(d, d.regs + d.off_stock + st, me, tag, epoch); else ++bad;
+    }
+    const unsigned long long bits = __ballot(live);
+    if (lane == 0 && bits) {
+        if (is_os) d.bitmap_os[wave] = bits;
+        else d.bitmap_sl[wave - os_waves] = bits;
+        atomicAdd(&s_cnt[is_os ? 0 : 1], (uint32_t)__popcll(bits));
+    }
+    if (bad) atomicAdd(&ctr->range_errors, bad);
+    __syncthreads();
+    if (threadIdx.x < 2 && s_cnt[threadIdx.x]) {
+        const uint32_t stripe = blockIdx.x % kCounterStripes, c = s_cnt[threadIdx.x];
+        const uint32_t type = threadIdx.x == 0 ? LTPG_TYPE_ORDERSTATUS : LTPG_TYPE_STOCKLEVEL;
+        atomicAdd(&ctr->commits[stripe][type], c);
+        atomicAdd(&ctr->totals[stripe][type], (unsigned long long)c);
+        atomicAdd(&ctr->totals[stripe][kAttempts + type], (unsigned long long)c);
+    }
+}
+
 template <bool COUNTS>
 __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
-                                                      uint8_t epoch, uint32_t no_waves, DeviceCounters *ctr)
+                                                      uint8_t epoch, uint32_t no_waves, bool ro, DeviceCounters *ctr)
 {
     __shared__ uint32_t s_cnt[4];  // NewOrder commits, attempts, Payment commits, attempts
     if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
@@ -441,6 +510,15 @@ __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no
                 commit = waw == 0;
             } else {
                 commit = d.doomed[q] != epoch;
+                // read-only readers of its ORDER / STOCK / ORDERLINE rows
+                // give a NewOrder war; it then aborts iff it also has raw,
+                // whose only source is a lower Payment of its customer
+                // (check_2_0: the WAREHOUSE, DISTRICT and ITEM W registers
+                // are never lowered)
+                if (ro && commit && d.war[q] == epoch) {
+                    const NoRows r = no_rows(no, q);
+                    commit = !(r.cu < d.t[CU].rows && *cust_w(d, r.cu) < priority(tag, no.TID[q], q));
+                }
             }
             if (commit) {
                 gather_neworder(d, no, q);
@@ -492,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no
                 atomicAdd(&ctr->commits[stripe][type], c);
                 atomicAdd(&ctr->totals[stripe][type], (unsigned long long)c);
             }
-            atomicAdd(&ctr->totals[stripe][3 + type], (unsigned long long)a);
+            atomicAdd(&ctr->totals[stripe][kAttempts + type], (unsigned long long)a);
         }
     }
 }
@@ -718,18 +796,28 @@ hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const Pa
     return hipGetLastError();
 }
 
+hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_t base, const EpochTag &e,
+                                DeviceCounters *ctr, hipStream_t s)
+{
+    const uint32_t os_waves = waves64(ro.n_os), waves = os_waves + waves64(ro.n_sl);
+    if (waves == 0) return hipSuccess;
+    hipLaunchKernelGGL(readonly_kernel, dim3((waves + kWaves - 1) / kWaves), dim3(kBlock), 0, s, d, ro, base, e.tag,
+                       e.epoch, os_waves, ctr);
+    return hipGetLastError();
+}
+
 hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                             bool counts, DeviceCounters *ctr, hipStream_t s)
+                             bool counts, bool ro, DeviceCounters *ctr, hipStream_t s)
 {
     const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
     if (waves == 0) return hipSuccess;
     const uint32_t blocks = (waves + kWaves - 1) / kWaves;
     if (counts)
         hipLaunchKernelGGL(check_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ctr);
+                           no_waves, ro, ctr);
     else
         hipLaunchKernelGGL(check_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ctr);
+                           no_waves, ro, ctr);
     return hipGetLastError();
 }
 

@@ -74,6 +74,17 @@ typedef struct oracle_tpcc_result {
     uint32_t range_errors;    /* op slots whose row is outside its table      */
 } oracle_tpcc_result;
 
+/* Results of an epoch's read-only transactions (Order-Status, Stock-Level):
+ * bitmaps, raw counters (war = waw = 0) and read values, value-major (3 per
+ * Order-Status: CUSTOMER col 2, ORDER col 0, ORDERLINE col 0; 1 per
+ * Stock-Level: STOCK col 2).  Read-value pointers may be NULL. */
+typedef struct oracle_tpcc_ro_result {
+    uint64_t *os_bitmap, *sl_bitmap;
+    uint32_t *os_conflicts, *sl_conflicts;
+    uint32_t *os_reads, *sl_reads;
+    uint32_t os_commits, sl_commits;
+} oracle_tpcc_ro_result;
+
 typedef struct oracle_tpcc oracle_tpcc;
 
 /* Rows and columns of table t (0..8: WAREHOUSE .. ITEM) at W warehouses. */
@@ -92,6 +103,15 @@ int oracle_tpcc_epoch(oracle_tpcc *o, uint32_t *const tables[9], uint32_t rules,
                       const uint32_t *no_block, uint32_t no_cap, uint32_t n_no,
                       const uint32_t *pay_block, uint32_t pay_cap, uint32_t n_pay,
                       oracle_tpcc_result *res);
+/* The same epoch with read-only transactions: Order-Status blocks of 6 x
+ * os_cap u32 (W_ID, D_ID, C_ID, O_ID, OL_ID, TID) and Stock-Level blocks of
+ * 3 x sl_cap u32 (W_ID, S_ID, TID). */
+int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tables[9], uint32_t rules, uint32_t values,
+                         const uint32_t *no_block, uint32_t no_cap, uint32_t n_no,
+                         const uint32_t *pay_block, uint32_t pay_cap, uint32_t n_pay,
+                         const uint32_t *os_block, uint32_t os_cap, uint32_t n_os,
+                         const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
+                         oracle_tpcc_result *res, oracle_tpcc_ro_result *ro);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,17 @@
  *   - Read values: only committed transactions' reads are delivered, from
  *     the table before the epoch's write-back; a slot whose row is outside
  *     its table, or a line beyond O_OL_CNT, reads 0.
+ *   - Read-only transactions (the Order-Status and Stock-Level slots of
+ *     TPCC-alltxn/Execute.h:1592-1770,2155-2231; the reference never fills
+ *     their query sets): Order-Status registers R on CUSTOMER row
+ *     W*30000 + D*3000 + C, ORDER row W*30000 + O_ID and ORDERLINE row
+ *     W*450000 + OL_ID and counts raw against W on each (check_0..check_2);
+ *     Stock-Level registers R on STOCK row W*100000 + S_ID and counts raw
+ *     against its W (check_0).  With no writes they commit (Aria's rule,
+ *     war = waw = 0) and read CUSTOMER col 2, ORDER col 0, ORDERLINE col 0 /
+ *     STOCK col 2 from the snapshot.  Batch positions follow the NewOrders
+ *     and Payments: Order-Status o at n_no + n_pay + o, Stock-Level k at
+ *     n_no + n_pay + n_os + k.
  */
 #include "ltpg_oracle.h"
 
@@ -289,8 +300,36 @@ int oracle_tpcc_epoch(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules, u
                       const uint32_t *no_block, uint32_t no_cap, uint32_t n_no, const uint32_t *pay_block,
                       uint32_t pay_cap, uint32_t n_pay, oracle_tpcc_result *res)
 {
+    return oracle_tpcc_epoch_ro(o, tbl, rules, values, no_block, no_cap, n_no, pay_block, pay_cap, n_pay, NULL, 0,
+                                0, NULL, 0, 0, res, NULL);
+}
+
+/* Read-only transactions' rows: Order-Status op_0_0 / op_1_0 / op_2_0
+ * (TPCC-alltxn/Execute.h:1594-1660), Stock-Level op_0_0 (:2157-2171). */
+static void os_rows(const uint32_t *b, uint32_t cap, uint32_t o, uint64_t r[3])
+{
+    const uint64_t C = cap, w = b[o];
+    r[0] = w * 30000 + (uint64_t)b[C + o] * 3000 + b[2 * C + o]; /* CUSTOMER  */
+    r[1] = w * 30000 + b[3 * C + o];                             /* ORDER     */
+    r[2] = w * 450000 + b[4 * C + o];                            /* ORDERLINE */
+}
+
+static uint64_t sl_row(const uint32_t *b, uint32_t cap, uint32_t k)
+{
+    return (uint64_t)b[k] * 100000 + b[(uint64_t)cap + k];
+}
+
+int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules, uint32_t values,
+                         const uint32_t *no_block, uint32_t no_cap, uint32_t n_no, const uint32_t *pay_block,
+                         uint32_t pay_cap, uint32_t n_pay, const uint32_t *os_block, uint32_t os_cap,
+                         uint32_t n_os, const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
+                         oracle_tpcc_result *res, oracle_tpcc_ro_result *ro)
+{
+    static const uint32_t os_tables[3] = {CU, OR, OL};
+    static const uint32_t os_cols[3] = {2, 0, 0};
     const nview nv = nview_of(no_block, no_cap, n_no);
     const pview pv = pview_of(pay_block, pay_cap, n_pay);
+    const uint64_t os_base = (uint64_t)n_no + n_pay, sl_base = os_base + n_os;
     uint32_t errors = 0;
     uint32_t *cid = res->pay_cid ? res->pay_cid : (uint32_t *)malloc(((size_t)n_pay + 1) * sizeof(uint32_t));
     if (!cid) return -1;
@@ -329,6 +368,19 @@ int oracle_tpcc_epoch(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules, u
         if (ok(o, CU, cu)) reg_min(o, CU, R, cu, me); else errors++;           /* op_4_0 */
         if (ok(o, CU, cu)) reg_min(o, CU, W, cu, me); else errors++;           /* op_5_0 */
         if (ok(o, HI, hi)) reg_min(o, HI, W, hi, me); else errors++;           /* op_6_0 */
+    }
+    for (uint32_t i = 0; i < n_os; i++) {  /* Order-Status op_0_0, op_1_0, op_2_0 */
+        const uint64_t me = ((uint64_t)os_block[5 * (uint64_t)os_cap + i] << 32) | (os_base + i);
+        uint64_t r[3];
+        os_rows(os_block, os_cap, i, r);
+        for (int k = 0; k < 3; k++) {
+            if (ok(o, os_tables[k], r[k])) reg_min(o, os_tables[k], R, r[k], me); else errors++;
+        }
+    }
+    for (uint32_t i = 0; i < n_sl; i++) {  /* Stock-Level op_0_0 */
+        const uint64_t me = ((uint64_t)sl_block[2 * (uint64_t)sl_cap + i] << 32) | (sl_base + i);
+        const uint64_t st = sl_row(sl_block, sl_cap, i);
+        if (ok(o, ST, st)) reg_min(o, ST, R, st, me); else errors++;
     }
 
     /* CHECK (check dispatch as it runs: NewOrder role 2 does nothing, role 3
@@ -393,6 +445,37 @@ int oracle_tpcc_epoch(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules, u
         if (waw == 0 && (raw == 0 || war == 0)) {
             res->pay_bitmap[p / 64] |= 1ull << (p % 64);
             res->pay_commits++;
+        }
+    }
+
+    /* Read-only transactions: raw against W on each row (check_0..check_2,
+     * TPCC-alltxn/Execute.h:1710-1745; check_0, :2203-2214); they commit */
+    if (ro) {
+        memset(ro->os_bitmap, 0, ((size_t)n_os + 63) / 64 * 8);
+        memset(ro->sl_bitmap, 0, ((size_t)n_sl + 63) / 64 * 8);
+        ro->os_commits = ro->sl_commits = 0;
+        for (uint32_t i = 0; i < n_os; i++) {
+            const uint64_t me = ((uint64_t)os_block[5 * (uint64_t)os_cap + i] << 32) | (os_base + i);
+            uint64_t r[3];
+            os_rows(os_block, os_cap, i, r);
+            uint32_t raw = 0;
+            for (int k = 0; k < 3; k++)
+                if (ok(o, os_tables[k], r[k])) raw += lower(o, os_tables[k], W, r[k], me);
+            ro->os_conflicts[i] = raw & 0xffu;
+            ro->os_bitmap[i / 64] |= 1ull << (i % 64); /* waw == 0 and war == 0 */
+            ro->os_commits++;
+            if (ro->os_reads)
+                for (int k = 0; k < 3; k++)
+                    ro->os_reads[(uint64_t)k * n_os + i] =
+                        ok(o, os_tables[k], r[k]) ? get(tbl, o, os_tables[k], os_cols[k], r[k]) : 0;
+        }
+        for (uint32_t i = 0; i < n_sl; i++) {
+            const uint64_t me = ((uint64_t)sl_block[2 * (uint64_t)sl_cap + i] << 32) | (sl_base + i);
+            const uint64_t st = sl_row(sl_block, sl_cap, i);
+            ro->sl_conflicts[i] = ok(o, ST, st) ? lower(o, ST, W, st, me) : 0;
+            ro->sl_bitmap[i / 64] |= 1ull << (i % 64);
+            ro->sl_commits++;
+            if (ro->sl_reads) ro->sl_reads[i] = ok(o, ST, st) ? get(tbl, o, ST, 2, st) : 0;
         }
     }
 

@@ -60,6 +60,20 @@ def _load(path=None):
     lib.oracle_tpcc_epoch.argtypes = [ctypes.c_void_p, tables_t, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                       ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.POINTER(TpccResult)]
+
+    class TpccRoResult(ctypes.Structure):
+        _fields_ = [("os_bitmap", u64p), ("sl_bitmap", u64p), ("os_conflicts", u32p), ("sl_conflicts", u32p),
+                    ("os_reads", u32p), ("sl_reads", u32p), ("os_commits", ctypes.c_uint32),
+                    ("sl_commits", ctypes.c_uint32)]
+
+    lib.TpccRoResult = TpccRoResult
+    if hasattr(lib, "oracle_tpcc_epoch_ro"):
+        lib.oracle_tpcc_epoch_ro.restype = ctypes.c_int
+        lib.oracle_tpcc_epoch_ro.argtypes = [ctypes.c_void_p, tables_t, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                             ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                             ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TpccResult),
+                                             ctypes.POINTER(TpccRoResult)]
     return lib
 
 
@@ -183,10 +197,13 @@ class TpccOracle:
     def __del__(self):
         self.close()
 
-    def epoch(self, tables, batch, rules=0, values=1):
+    def epoch(self, tables, batch, rules=0, values=1, ro=None):
         """One epoch over `tables` (list of 9 column-major u32 arrays, updated
         in place) for `batch` (an object with no/no_cap/n_no/pay/pay_cap/n_pay:
-        the packed NEWORDER_QUERY / PAYMENT_QUERY blocks)."""
+        the packed NEWORDER_QUERY / PAYMENT_QUERY blocks) and, optionally,
+        read-only transactions `ro` (an object with n_os / n_sl and field
+        arrays os[name] / sl[name]); their results come back in
+        TpccEpoch.ro."""
         for t in tables:
             assert t.dtype == np.uint32 and t.flags.c_contiguous
         n_no, n_pay = int(batch.n_no), int(batch.n_pay)
@@ -201,10 +218,31 @@ class TpccOracle:
         res = lib.TpccResult(_u64p(nb), _u64p(pb), _u32p(nc), _u32p(pc), _u32p(nr), _u32p(pr), _u32p(cid), 0, 0, 0)
         no = np.ascontiguousarray(batch.no, dtype=np.uint32)
         pay = np.ascontiguousarray(batch.pay, dtype=np.uint32)
-        rc = lib.oracle_tpcc_epoch(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values, _u32p(no),
-                                    batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay, ctypes.byref(res))
+        if ro is None:
+            rc = lib.oracle_tpcc_epoch(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
+                                        _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
+                                        ctypes.byref(res))
+            ro_out = None
+        else:
+            n_os, n_sl = int(ro.n_os), int(ro.n_sl)
+            osb = np.ascontiguousarray(np.stack([ro.os[f][:max(n_os, 1)] for f in
+                                                 ("W_ID", "D_ID", "C_ID", "O_ID", "OL_ID", "TID")]), np.uint32)
+            slb = np.ascontiguousarray(np.stack([ro.sl[f][:max(n_sl, 1)] for f in ("W_ID", "S_ID", "TID")]),
+                                       np.uint32)
+            ob = np.zeros(max(1, (n_os + 63) // 64), np.uint64)
+            sb = np.zeros(max(1, (n_sl + 63) // 64), np.uint64)
+            oc, sc = np.zeros(max(1, n_os), np.uint32), np.zeros(max(1, n_sl), np.uint32)
+            orr, srr = np.zeros(max(1, 3 * n_os), np.uint32), np.zeros(max(1, n_sl), np.uint32)
+            rres = lib.TpccRoResult(_u64p(ob), _u64p(sb), _u32p(oc), _u32p(sc), _u32p(orr), _u32p(srr), 0, 0)
+            rc = lib.oracle_tpcc_epoch_ro(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
+                                           _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
+                                           _u32p(osb), max(n_os, 1), n_os, _u32p(slb), max(n_sl, 1), n_sl,
+                                           ctypes.byref(res), ctypes.byref(rres))
+            ro_out = dict(os_bitmap=ob[:(n_os + 63) // 64], sl_bitmap=sb[:(n_sl + 63) // 64],
+                          os_conflicts=oc[:n_os], sl_conflicts=sc[:n_sl], os_reads=orr[:3 * n_os],
+                          sl_reads=srr[:n_sl], os_commits=rres.os_commits, sl_commits=rres.sl_commits)
         assert rc == 0
         return TpccEpoch(no_bitmap=nb[:(n_no + 63) // 64], pay_bitmap=pb[:(n_pay + 63) // 64],
                          no_conflicts=nc[:n_no], pay_conflicts=pc[:n_pay], no_reads=nr[:51 * n_no],
                          pay_reads=pr[:24 * n_pay], pay_cid=cid[:n_pay], no_commits=res.no_commits,
-                         pay_commits=res.pay_commits, range_errors=res.range_errors)
+                         pay_commits=res.pay_commits, range_errors=res.range_errors, ro=ro_out)

@@ -45,17 +45,33 @@ def _kat_batch(case):
     return tc.make_batch(nos, case["payments"])
 
 
+def _kat_readonly(case):
+    """ReadOnlyBatch of a case's Order-Status / Stock-Level lists, or None."""
+    if "orderstatus" not in case and "stocklevel" not in case:
+        return None
+    os_, sl = case.get("orderstatus", []), case.get("stocklevel", [])
+    ro = L.ReadOnlyBatch(len(os_), len(sl))
+    for i, t in enumerate(os_):
+        for f in L.ORDERSTATUS_FIELDS:
+            ro.os[f][i] = t.get(f, 0)
+    for i, t in enumerate(sl):
+        for f in L.STOCKLEVEL_FIELDS:
+            ro.sl[f][i] = t.get(f, 0)
+    return ro
+
+
 def _check_kat(case, run):
-    """run(tables, batch, rules, values) -> TpccEpoch; compares every
+    """run(tables, batch, rules, values, ro) -> TpccEpoch; compares every
     expectation of the case, return False on the first mismatch."""
     W = case["W"]
     tables = _tables(W)
     _apply(W, tables, case["set"])
     initial = [t.copy() for t in tables]
     b = _kat_batch(case)
+    ro = _kat_readonly(case)
     rules = 0 if case["rules"] == "reference" else 1
     values = 1 if case["values"] == "rmw" else 0
-    ep = run(tables, b, rules, values)
+    ep = run(tables, b, rules, values, ro)
     exp = case["expect"]
     ok = (tc.bits(ep.no_bitmap, b.n_no) == exp["no_commit"] and tc.bits(ep.pay_bitmap, b.n_pay) == exp["pay_commit"]
           and tc.unpack(ep.no_conflicts).tolist() == exp["no_conf"]
@@ -69,13 +85,24 @@ def _check_kat(case, run):
         ok = ok and int(ep.no_reads[v * b.n_no + q]) == val
     for p, v, val in exp["pay_reads"]:
         ok = ok and int(ep.pay_reads[v * b.n_pay + p]) == val
+    if ro is not None:
+        r = ep.ro
+        ok = ok and tc.bits(r["os_bitmap"], ro.n_os) == exp["os_commit"]
+        ok = ok and tc.bits(r["sl_bitmap"], ro.n_sl) == exp["sl_commit"]
+        if r.get("os_conflicts") is not None:
+            ok = ok and tc.unpack(r["os_conflicts"]).tolist() == exp["os_conf"]
+            ok = ok and tc.unpack(r["sl_conflicts"]).tolist() == exp["sl_conf"]
+        for i, v, val in exp["os_reads"]:
+            ok = ok and int(r["os_reads"][v * ro.n_os + i]) == val
+        for i, v, val in exp["sl_reads"]:
+            ok = ok and int(r["sl_reads"][v * ro.n_sl + i]) == val
     return ok
 
 
 @pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
 def test_oracle_known_answers(case):
     ora = oracle.TpccOracle(case["W"])
-    assert _check_kat(case, lambda tables, b, r, v: ora.epoch(tables, b, r, v))
+    assert _check_kat(case, lambda tables, b, r, v, ro: ora.epoch(tables, b, r, v, ro=ro))
 
 
 def test_loader_contents():
@@ -166,6 +193,106 @@ def test_oracle_pairwise_and_serializable(seed):
     if prios:
         _, typ, i = min(prios)
         assert tc.bits(ep.no_bitmap if typ == "no" else ep.pay_bitmap, i + 1)[i] == 1
+
+
+def _random_readonly(rng, W, n_os, n_sl, tid_pool):
+    """Order-Status / Stock-Level transactions whose rows collide with
+    _random_batch's (customers 0..3, orders 0..4, order lines 0..14, items
+    0..5)."""
+    ro = L.ReadOnlyBatch(n_os, n_sl)
+    tids = rng.permutation(tid_pool)[:n_os + n_sl]
+    for i in range(n_os):
+        ro.os["W_ID"][i], ro.os["D_ID"][i] = rng.integers(0, W), rng.integers(0, 2)
+        ro.os["C_ID"][i], ro.os["O_ID"][i] = rng.integers(0, 4), rng.integers(0, 5)
+        ro.os["OL_ID"][i], ro.os["TID"][i] = rng.integers(0, 15), tids[i]
+    for i in range(n_sl):
+        ro.sl["W_ID"][i], ro.sl["S_ID"][i], ro.sl["TID"][i] = rng.integers(0, W), rng.integers(0, 6), tids[n_os + i]
+    return ro
+
+
+def _pins_hold_ro(W, tables0, b, ro, ep, rules, values=1):
+    """_pins_hold with read-only transactions: their raw counters, Aria's
+    rule for every type (read-only ones always commit), and the serial
+    execution of all committed transactions with every read value."""
+    no_c, pay_c, bad, os_c, sl_c = tc.pairwise_conflicts(b, W, ep.pay_cid, rules, ro)
+    r = ep.ro
+    got = [tc.unpack(ep.no_conflicts), tc.unpack(ep.pay_conflicts), tc.unpack(r["os_conflicts"]),
+           tc.unpack(r["sl_conflicts"])]
+    for g, e, name in zip(got, (no_c, pay_c, os_c, sl_c), ("no", "pay", "os", "sl")):
+        if not np.array_equal(g, e):
+            return False, f"{name} conflict counters differ from the pairwise count"
+    if bad != ep.range_errors:
+        return False, "range errors"
+    commits = {"no": tc.bits(ep.no_bitmap, b.n_no), "pay": tc.bits(ep.pay_bitmap, b.n_pay),
+               "os": tc.bits(r["os_bitmap"], ro.n_os), "sl": tc.bits(r["sl_bitmap"], ro.n_sl)}
+    for name, conf in (("no", no_c), ("pay", pay_c), ("os", os_c), ("sl", sl_c)):
+        if commits[name] != tc.aria(conf).tolist():
+            return False, f"{name} commit is not waw == 0 && (raw == 0 || war == 0)"
+    ok, why, final, reads = tc.serial_check(W, tables0, b, ep.pay_cid, commits["no"], commits["pay"],
+                                            rmw=values == 1, ro=ro, ro_commit={"os": commits["os"],
+                                                                               "sl": commits["sl"]})
+    if not ok:
+        return False, why
+    flats = {"no": (ep.no_reads, b.n_no, 51), "pay": (ep.pay_reads, b.n_pay, 24),
+             "os": (r["os_reads"], ro.n_os, 3), "sl": (r["sl_reads"], ro.n_sl, 1)}
+    for (typ, i), vals in reads.items():
+        flat, n, nv = flats[typ]
+        if [int(flat[v * n + i]) for v in range(nv)] != vals:
+            return False, f"read values of {typ} {i}"
+    return True, final
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_oracle_readonly_pairwise_and_serializable(seed):
+    """Order-Status / Stock-Level mixed into colliding batches: counters
+    match the pairwise count from the dispatch tables (R registrations give
+    NewOrders war), every type follows Aria's rule, the committed set
+    executes serially to the oracle's tables and read values."""
+    rng = np.random.default_rng(500 + seed)
+    W = 1 + seed % 2
+    n_no, n_pay = int(rng.integers(0, 8)), int(rng.integers(0, 8))
+    b = _random_batch(rng, W, n_no, n_pay)
+    ro = _random_readonly(rng, W, int(rng.integers(0, 6)), int(rng.integers(0, 6)),
+                          np.arange(0, 2 * (n_no + n_pay) + 24, 2))
+    tables0 = _tables(W)
+    tables = [t.copy() for t in tables0]
+    ep = oracle.TpccOracle(W).epoch(tables, b, seed % 2, 1, ro=ro)
+    ok, res = _pins_hold_ro(W, tables0, b, ro, ep, seed % 2)
+    assert ok, res
+    for t in range(9):
+        np.testing.assert_array_equal(tables[t], res[t])
+    assert ep.ro["os_commits"] == ro.n_os and ep.ro["sl_commits"] == ro.n_sl
+
+
+def test_oracle_readonly_bruteforce_tid_orders():
+    """Every TID order of small batches with one Order-Status and one
+    Stock-Level on colliding rows."""
+    rng = np.random.default_rng(91)
+    W = 1
+    tables0 = _tables(W)
+    ora = oracle.TpccOracle(W)
+    checked = aborts = 0
+    for pattern in range(4):
+        base = _random_batch(rng, W, 2, 1)
+        ro0 = _random_readonly(rng, W, 1, 1, np.arange(100, 110))
+        # make the readers hit the NewOrders' rows
+        ro0.os["O_ID"][0] = base.nf("O_ID")[0]
+        ro0.os["W_ID"][0] = base.nf("W_ID")[0]
+        if base.nf("O_OL_CNT")[1]:
+            ro0.sl["W_ID"][0] = base.nf("OL_SUPPLY_W_ID", 0)[1]
+            ro0.sl["S_ID"][0] = base.nf("OL_I_ID", 0)[1]
+        for perm in itertools.permutations(range(5)):
+            b = L.TpccBatch(base.no.copy(), base.no_cap, base.n_no, base.pay.copy(), base.pay_cap, base.n_pay)
+            b.nf("TID")[:] = np.array(perm[:2], dtype=np.uint32) * 5 + 3
+            b.pf("TID")[:] = np.array(perm[2:3], dtype=np.uint32) * 5 + 3
+            ro0.os["TID"][0], ro0.sl["TID"][0] = perm[3] * 5 + 3, perm[4] * 5 + 3
+            tables = [t.copy() for t in tables0]
+            ep = ora.epoch(tables, b, pattern % 2, 1, ro=ro0)
+            ok, res = _pins_hold_ro(W, tables0, b, ro0, ep, pattern % 2)
+            assert ok, res
+            aborts += b.n_no - sum(tc.bits(ep.no_bitmap, b.n_no))
+            checked += 1
+    assert checked == 480 and aborts > 0
 
 
 def test_oracle_bruteforce_all_tid_orders():
@@ -282,7 +409,7 @@ def test_pins_catch_planted_mistakes(tmp_path, mistake):
     caught = False
     for case in KAT["cases"]:
         ora = oracle.TpccOracle(case["W"], lib=lib)
-        if not _check_kat(case, lambda tables, b, r, v: ora.epoch(tables, b, r, v)):
+        if not _check_kat(case, lambda tables, b, r, v, ro: ora.epoch(tables, b, r, v, ro=ro)):
             caught = True
             break
     rng = np.random.default_rng(99)

@@ -126,7 +126,32 @@ def payment_ops(b, p, W, cid):
     return regs, checks, bad
 
 
-def all_ops(b, W, cid, rules):
+def readonly_rows(ro, typ, i):
+    """Rows read by read-only transaction i: Order-Status (TPCC-alltxn/
+    Execute.h:1594-1660) CUSTOMER, ORDER, ORDERLINE; Stock-Level (:2157-2171)
+    STOCK, with the column each reads."""
+    if typ == "os":
+        w = int(ro.os["W_ID"][i])
+        return [(CU, w * 30000 + int(ro.os["D_ID"][i]) * 3000 + int(ro.os["C_ID"][i]), 2),
+                (OR, w * 30000 + int(ro.os["O_ID"][i]), 0),
+                (OL, w * 450000 + int(ro.os["OL_ID"][i]), 0)]
+    return [(ST, int(ro.sl["W_ID"][i]) * 100000 + int(ro.sl["S_ID"][i]), 2)]
+
+
+def readonly_ops(ro, typ, i, W):
+    """(registrations, checks, range errors): R on each row read, raw
+    against its W (check_0 .. check_2)."""
+    regs, checks, bad = [], [], 0
+    for t, row, _ in readonly_rows(ro, typ, i):
+        if row >= rows_of(W, t):
+            bad += 1
+            continue
+        regs.append((t, row, "R"))
+        checks.append((t, row, "W", "raw"))
+    return regs, checks, bad
+
+
+def all_ops(b, W, cid, rules, ro=None):
     """[(priority, type, index, regs, checks)] and the range-error count."""
     out, bad = [], 0
     for q in range(b.n_no):
@@ -137,25 +162,36 @@ def all_ops(b, W, cid, rules):
         r, c, e = payment_ops(b, p, W, cid[p])
         out.append(((int(b.pf("TID")[p]), b.n_no + p), "pay", p, r, c))
         bad += e
+    if ro is not None:
+        base = b.n_no + b.n_pay
+        for typ, n, tids, off in (("os", ro.n_os, ro.os["TID"], base), ("sl", ro.n_sl, ro.sl["TID"], base + ro.n_os)):
+            for i in range(n):
+                r, c, e = readonly_ops(ro, typ, i, W)
+                out.append(((int(tids[i]), off + i), typ, i, r, c))
+                bad += e
     return out, bad
 
 
-def pairwise_conflicts(b, W, cid, rules):
+def pairwise_conflicts(b, W, cid, rules, ro=None):
     """raw/war/waw per transaction by pairwise comparison: a check slot of t
     on (table, row, kind) counts iff some transaction of higher priority
-    registered `kind` on that row."""
-    txns, bad = all_ops(b, W, cid, rules)
+    registered `kind` on that row.  With read-only transactions `ro` also
+    returns their counters: (no, pay, bad, os, sl)."""
+    txns, bad = all_ops(b, W, cid, rules, ro)
     holders = {}
     for prio, _, _, regs, _ in txns:
         for t, row, kind in regs:
             holders.setdefault((t, row, kind), []).append(prio)
-    conf = {"no": np.zeros((b.n_no, 3), np.uint32), "pay": np.zeros((b.n_pay, 3), np.uint32)}
+    conf = {"no": np.zeros((b.n_no, 3), np.uint32), "pay": np.zeros((b.n_pay, 3), np.uint32),
+            "os": np.zeros((ro.n_os if ro else 0, 3), np.uint32), "sl": np.zeros((ro.n_sl if ro else 0, 3), np.uint32)}
     col = {"raw": 0, "war": 1, "waw": 2}
     for prio, typ, i, _, checks in txns:
         for t, row, kind, ctr in checks:
             if any(h < prio for h in holders.get((t, row, kind), [])):
                 conf[typ][i, col[ctr]] += 1
-    return conf["no"], conf["pay"], bad
+    if ro is None:
+        return conf["no"], conf["pay"], bad
+    return conf["no"], conf["pay"], bad, conf["os"], conf["sl"]
 
 
 def aria(conf):
@@ -265,10 +301,15 @@ def _payment_effects(tb, b, p, cid, rmw):
     return reads
 
 
-def _rw_sets(b, W, cid):
+def _rw_sets(b, W, cid, ro=None):
     """Row-level read and write sets of every transaction (the YTD columns
     are commutative adds and do not order transactions)."""
     out = []
+    if ro is not None:
+        base = b.n_no + b.n_pay
+        for typ, n, tids, off in (("os", ro.n_os, ro.os["TID"], base), ("sl", ro.n_sl, ro.sl["TID"], base + ro.n_os)):
+            for i in range(n):
+                out.append(((int(tids[i]), off + i), typ, i, {(t, r) for t, r, _ in readonly_rows(ro, typ, i)}, set()))
     for q in range(b.n_no):
         w, d, c = int(b.nf("W_ID")[q]), int(b.nf("D_ID")[q]), int(b.nf("C_ID")[q])
         rd = {(WH, w), (DI, w * 10 + d), (CU, w * 30000 + d * 3000 + c)}
@@ -288,11 +329,16 @@ def _rw_sets(b, W, cid):
     return out
 
 
-def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True):
+def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True, ro=None, ro_commit=None):
     """(ok, message, final tables, reads) of a serial execution of the
     committed transactions in an order that honours every precedence
-    constraint; reads are {('no', q): [..51], ('pay', p): [..24]}."""
-    txns = [t for t in _rw_sets(b, W, cid) if (no_commit if t[1] == "no" else pay_commit)[t[2]]]
+    constraint; reads are {('no', q): [..51], ('pay', p): [..24],
+    ('os', o): [..3], ('sl', k): [..1]}.  ro_commit: {'os': [...], 'sl':
+    [...]} commit lists of the read-only transactions `ro`."""
+    commits = {"no": no_commit, "pay": pay_commit}
+    if ro is not None:
+        commits.update(ro_commit)
+    txns = [t for t in _rw_sets(b, W, cid, ro) if commits[t[1]][t[2]]]
     n = len(txns)
     succ = [set() for _ in range(n)]
     indeg = [0] * n
@@ -328,8 +374,10 @@ def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True):
         _, typ, i, _, _ = txns[k]
         if typ == "no":
             reads[(typ, i)] = _neworder_effects(tb, b, i, rmw)
-        else:
+        elif typ == "pay":
             reads[(typ, i)] = _payment_effects(tb, b, i, cid[i], rmw)
+        else:
+            reads[(typ, i)] = [tb.get(t, r, c) if tb.ok(t, r) else 0 for t, r, c in readonly_rows(ro, typ, i)]
     return True, "", tb.a, reads
 
 
