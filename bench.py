@@ -321,7 +321,7 @@ def device_tpcc_batch(ctx, b):
     return (nb, pb), (nob, pab)
 
 
-def cpu_baseline_tpcc(W, batches, seed, budget_s=10.0):
+def cpu_baseline_tpcc(W, batches, seed, budget_s=10.0, ros=None):
     """The oracle (single-threaded C replay of the reference's rules, every
     register of every slot, read staging and RMW write-back) on whole batches
     of the same workload over the full-size tables, until about budget_s."""
@@ -332,11 +332,12 @@ def cpu_baseline_tpcc(W, batches, seed, budget_s=10.0):
     total_s, commits, txns, epochs = 0.0, 0, 0, 0
     while total_s < budget_s:
         b = batches[epochs % len(batches)]
+        ro = ros[epochs % len(ros)] if ros else None
         t0 = time.perf_counter()
-        ep = ora.epoch(tables, b, 0, 1)
+        ep = ora.epoch(tables, b, 0, 1, ro=ro)
         total_s += time.perf_counter() - t0
-        commits += ep.no_commits + ep.pay_commits
-        txns += b.n_no + b.n_pay
+        commits += ep.no_commits + ep.pay_commits + (ep.ro["os_commits"] + ep.ro["sl_commits"] if ro else 0)
+        txns += b.n_no + b.n_pay + (ro.n_os + ro.n_sl if ro else 0)
         epochs += 1
     return {"value": commits / total_s, "unit": "txn/s", "cores": 1, "kind": "port",
             "attempted_txn_per_s": txns / total_s,
@@ -344,24 +345,42 @@ def cpu_baseline_tpcc(W, batches, seed, budget_s=10.0):
                       f"batches, {txns} txns in {total_s:.1f} s"}
 
 
-def run_tpcc(args, cpu_baseline=True):
+def run_tpcc(args, cpu_baseline=True, mix=False):
+    """C3 (mix=False), or with mix=True the same tables and batch size with
+    45 % NewOrder, 43 % Payment, 6 % Order-Status and 6 % Stock-Level (half
+    of the read-only transactions on rows the batch's NewOrders write)."""
     import ltpg_amd as L
 
     W, n, P = 64, 1 << 18, args.batches
     seed = 0x4C545047
-    desc = (f"order-entry plain form: W={W}, 2^18 txns, 50/50 NewOrder/Payment, NURand items, 1% remote "
-            f"supply, 15% remote customer, 60% by name")
+    n_ro = n * 6 // 100 if mix else 0
+    if mix:
+        desc = (f"order-entry mix, plain form: W={W}, 2^18 txns: 45% NewOrder, 43% Payment, 6% Order-Status, "
+                f"6% Stock-Level (half on rows the NewOrders write), NURand items, 1% remote supply, "
+                f"15% remote customer, 60% by name")
+    else:
+        desc = (f"order-entry plain form: W={W}, 2^18 txns, 50/50 NewOrder/Payment, NURand items, 1% remote "
+                f"supply, 15% remote customer, 60% by name")
     log(f"generating {P} batches: {desc}")
-    gen = L.TpccGenerator(warehouses=W, seed=seed)
-    batches = [gen.next(n, tid_base=i * n) for i in range(P)]
+    gen = L.TpccGenerator(warehouses=W, seed=seed, neworder_pct=51 if mix else 50)
+    n_rw = n - 2 * n_ro
+    batches = [gen.next(n_rw, tid_base=i * n) for i in range(P)]
+    ros = [L.readonly_batch(batches[i], W, n_ro, n_ro, seed=seed + i, tid_lo=i * n, tid_hi=i * n + n_rw)
+           for i in range(P)] if mix else None
     ctx = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=n, values=L.VALUES_RMW)
     ctx.tpcc_populate(seed)
     dev = [device_tpcc_batch(ctx, b) for b in batches]
+    rdev = [L.upload_readonly_batch(ctx, r) for r in ros] if mix else []
+
+    def submit(i):
+        ctx.tpcc_submit(dev[i][0], mem=L.MEM_DEVICE)
+        if mix:
+            ctx.tpcc_submit_readonly(rdev[i][0], mem=L.MEM_DEVICE)
     log("order-entry batches resident in HBM")
 
     per_batch = []
     for i in range(P):
-        ctx.tpcc_submit(dev[i][0], mem=L.MEM_DEVICE)
+        submit(i)
         ctx.epoch_run()
         ctx.sync()
         nc = np.unpackbits(ctx.commit_bitmap(L.TYPE_NEWORDER, n_txns=batches[i].n_no).view(np.uint8),
@@ -369,30 +388,40 @@ def run_tpcc(args, cpu_baseline=True):
         pc = np.unpackbits(ctx.commit_bitmap(L.TYPE_PAYMENT, n_txns=batches[i].n_pay).view(np.uint8),
                            bitorder="little")[:batches[i].n_pay]
         per_batch.append(tpcc_kernel_bytes(batches[i], nc, pc, 0))
+        if mix:
+            # read-only transactions (in EXECUTE): their fields, one W
+            # register per row read, the table word and its read value,
+            # the commit bit
+            per_batch[-1]["execute"] += n_ro * (6 * 4 + 3 * 8 + 3 * 8 + 1 / 8) + n_ro * (3 * 4 + 8 + 8 + 1 / 8)
 
     for k in range(args.warmup):
-        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        submit(k % P)
         ctx.epoch_run()
     ctx.sync()
     log(f"warmup done; timing {args.steps} steps")
-    for t in (L.TYPE_NEWORDER, L.TYPE_PAYMENT):
+    types = (L.TYPE_NEWORDER, L.TYPE_PAYMENT) + ((L.TYPE_ORDERSTATUS, L.TYPE_STOCKLEVEL) if mix else ())
+    for t in types:
         ctx.commit_totals(t, reset=True)
     ctx.set_timing(True)
     ctx.phase_times()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        submit(k % P)
         ctx.epoch_run()
     ctx.sync()
     elapsed = time.perf_counter() - t0
     phase_ms, n_timed = ctx.phase_times()
     ctx.set_timing(False)
     (cn, an), (cp, ap_) = ctx.commit_totals(L.TYPE_NEWORDER), ctx.commit_totals(L.TYPE_PAYMENT)
+    c_ro = a_ro = 0
+    for t in types[2:]:
+        c_, a_ = ctx.commit_totals(t)
+        c_ro, a_ro = c_ro + c_, a_ro + a_
 
     lat = []
     for k in range(max(5, args.steps // 2)):
         t1 = time.perf_counter()
-        ctx.tpcc_submit(dev[k % P][0], mem=L.MEM_DEVICE)
+        submit(k % P)
         ctx.epoch_run()
         ctx.commit_count(L.TYPE_PAYMENT)
         lat.append((time.perf_counter() - t1) * 1e3)
@@ -402,14 +431,16 @@ def run_tpcc(args, cpu_baseline=True):
     dom = max(TPCC_KERNELS, key=lambda k: avg_ms[k])
     achieved = bytes_avg[dom] / (avg_ms[dom] * 1e-3) / 1e9
     result = {
-        "metric": METRIC, "value": (cn + cp) / elapsed, "unit": "txn/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": (cn + cp + c_ro) / elapsed, "unit": "txn/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": desc, "warehouses": W, "batch_txns": n, "neworder_pct": 50,
+        "config": {"workload": desc, "warehouses": W, "batch_txns": n, "neworder_pct": 45 if mix else 50,
+                   "payment_pct": 43 if mix else 50, "orderstatus_pct": 6 if mix else 0,
+                   "stocklevel_pct": 6 if mix else 0,
                    "form": "plain per-batch", "inputs": "resident in HBM", "distinct_batches": P},
-        "abort_rate": 1.0 - (cn + cp) / max(an + ap_, 1),
+        "abort_rate": 1.0 - (cn + cp + c_ro) / max(an + ap_ + a_ro, 1),
         "abort_rate_by_type": {"neworder": 1.0 - cn / max(an, 1), "payment": 1.0 - cp / max(ap_, 1)},
-        "committed_txns": cn + cp, "attempted_txns": an + ap_,
+        "committed_txns": cn + cp + c_ro, "attempted_txns": an + ap_ + a_ro,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "what": "submit -> commit count on host, one synchronous epoch at a time"},
         "kernel_ms": avg_ms,
@@ -420,13 +451,13 @@ def run_tpcc(args, cpu_baseline=True):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "tpcc"),
                      "algorithmic_bytes_per_launch": bytes_avg[dom]},
     }
-    for (_, bufs) in dev:
+    for (_, bufs) in dev + rdev:
         for x in bufs:
             x.free()
     ctx.close()
     if cpu_baseline and not args.no_cpu_baseline:
         log("cpu baseline (order-entry oracle)")
-        result["cpu_baseline"] = cpu_baseline_tpcc(W, batches, seed)
+        result["cpu_baseline"] = cpu_baseline_tpcc(W, batches, seed, ros=ros)
     return result
 
 
@@ -706,7 +737,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="kv-c2",
-                    choices=["kv-c2", "kv-c1", "kv-scan", "tpcc-c3", "batched-c4", "served-c5"])
+                    choices=["kv-c2", "kv-c1", "kv-scan", "tpcc-c3", "tpcc-mix", "batched-c4", "served-c5"])
     ap.add_argument("--c5-rates", type=float, nargs="+", default=[1, 5, 10, 25, 50],
                     help="offered loads of the served form, M txn/s")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
@@ -740,6 +771,8 @@ def main():
     variants = []
     if args.workload == "tpcc-c3":
         result = run_tpcc(args)
+    elif args.workload == "tpcc-mix":
+        result = run_tpcc(args, mix=True)
     elif args.workload == "served-c5":
         v = run_served_c5(args)
         result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True,
@@ -756,6 +789,8 @@ def main():
             variants.append({k: v[k] for k in keep})
         if args.workload == "kv-c2" and args.theta is None:
             v = run_tpcc(args)
+            variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
+            v = run_tpcc(args, mix=True)
             variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
             v = run_kv(args, None, name="kv-scan")
             variants.append({k: v[k] for k in keep + ("cpu_baseline",) if k in v})
