@@ -101,12 +101,13 @@ def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw, rows=None, max_scan=0):
 
 
 def choose_kv_shape(ctx, batch, reps=3):
-    """Time the 8 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
-    block-local row folding, split register arrays, 64-transaction chunks)
-    on one batch, one warm-up and `reps` epochs each, and keep the fastest
-    for the run.  The timed loop is the plain per-batch loop either way."""
+    """Time the 16 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
+    block-local row folding, split register arrays, 64-transaction chunks,
+    per-lane field copies) on one batch, one warm-up and `reps` epochs each,
+    and keep the fastest for the run.  The timed loop is the plain per-batch
+    loop either way."""
     times = []
-    for flags in range(8):
+    for flags in range(16):
         ctx.kv_set_tuning(flags)
         ctx.kv_submit_device(batch)
         ctx.epoch_run()

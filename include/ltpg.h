@@ -379,12 +379,17 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   one 16-byte pair.
  * SMALL_CHUNKS: CHECK and WRITEBACK blocks own 64 transactions instead of
  *   256 (more blocks, fewer dependent rounds each).
+ * LANE_COPY: the read-out and write-back copy a field per lane with 16-byte
+ *   loads all in flight (fields of 16-byte aligned stride, at most 32 words)
+ *   instead of spreading each field over a group of lanes, one dependent
+ *   load round per 8 op slots; for epochs too small to hide that latency.
  * Results are identical for every combination; may change between any two
  * epochs. */
 #define LTPG_TUNE_EXEC_PER_OP 1u
 #define LTPG_TUNE_SPLIT_REGISTERS 2u
 #define LTPG_TUNE_SMALL_CHUNKS 4u
-#define LTPG_TUNE_ALL 7u
+#define LTPG_TUNE_LANE_COPY 8u
+#define LTPG_TUNE_ALL 15u
 int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags);
 
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
@@ -490,7 +495,7 @@ typedef struct ltpg_tuning_entry {
     uint32_t log2_txns;        /* the size class                                    */
     uint32_t flags;            /* chosen LTPG_TUNE_* combination                    */
     uint32_t reserved;
-    float ms[8];               /* epoch time of every combination, by flags         */
+    float ms[16];              /* epoch time of every combination, by flags         */
 } ltpg_tuning_entry;
 int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
                           uint32_t *n_out);

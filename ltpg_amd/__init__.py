@@ -55,7 +55,7 @@ STOCKLEVEL_FIELDS = ("W_ID", "S_ID", "TID")
 # u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
 NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
-TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_ALL = 1, 2, 4, 7
+TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_LANE_COPY, TUNE_ALL = 1, 2, 4, 8, 15
 
 
 def kv_meta(kind, field):

@@ -120,7 +120,7 @@ class TuningEntry(ctypes.Structure):
     """struct ltpg_tuning_entry."""
 
     _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 8)]
+                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 16)]
 
 
 class ServeConfig(ctypes.Structure):

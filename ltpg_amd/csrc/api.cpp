@@ -623,6 +623,7 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
     ctx->kv.exec_per_op = (flags & LTPG_TUNE_EXEC_PER_OP) ? 1 : 0;
     ctx->kv.split_regs = (flags & LTPG_TUNE_SPLIT_REGISTERS) ? 1 : 0;
     ctx->kv.small_chunks = (flags & LTPG_TUNE_SMALL_CHUNKS) ? 1 : 0;
+    ctx->kv.lane_copy = (flags & LTPG_TUNE_LANE_COPY) ? 1 : 0;
     return LTPG_OK;
 }
 

@@ -366,7 +366,7 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     for (int c = 0; c < 33 && !rc; c++) {
         if (rep[c] < 0) continue;
         const ltpg_batch_desc &d = b->d[rep[c]];
-        float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        float ms[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t cfg = 0; cfg <= LTPG_TUNE_ALL && !rc; cfg++) {
             rc = ltpg_kv_set_tuning(ctx, cfg);
             for (uint32_t k = 0; k <= reps && !rc; k++) {
@@ -395,7 +395,7 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
             o.log2_txns = (uint32_t)c;
             o.flags = best;
             o.reserved = 0;
-            for (int k = 0; k < 8; k++) o.ms[k] = ms[k];
+            for (int k = 0; k < 16; k++) o.ms[k] = ms[k];
         }
         n++;
     }

@@ -564,24 +564,38 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     }
 }
 
-// Block-local compaction.  A block owns CH consecutive transactions (CH =
-// 256, or 64 for batches too small to fill the chip: a block walks its
-// list in rounds of 4 * 64 / G transactions, so fewer per block means
-// more blocks and fewer dependent rounds each);
-// the transactions that still have work (alive ones for CHECK, committed
-// ones for WRITEBACK) are gathered into an LDS list, then processed G lanes
-// per transaction (one lane per slot), so lanes stay busy whatever the
-// abort rate and every op load of a transaction is issued at once.
+// Block-local compaction.  A chunk is CH consecutive transactions (CH =
+// 256, or 64 as a launch shape); the transactions that still have work
+// (committed ones, for the read-out of CHECK and for WRITEBACK) are gathered
+// into an LDS list, then processed G lanes per transaction (one lane per
+// slot), so lanes stay busy whatever the abort rate and every op load of a
+// transaction is issued at once.  A wave walks the list in rounds of 64 / G
+// transactions; for small batches gridDim.y blocks share one chunk's list
+// (each builds the same list and takes every gridDim.y-th round), so that a
+// small epoch runs as many short waves rather than a few long ones.
 constexpr uint32_t kChunk = kBlock;
 
-__device__ __forceinline__ void list_append(bool pick, uint32_t t, uint32_t *s_list, uint32_t *s_n)
+// Gathers the picked transactions (one per thread) into s_list in thread
+// order, so that every block of a chunk builds the same list and can take
+// its own share of it by index.  Every thread of the block calls it (it
+// holds two barriers); s_cnt has one word per wave.  Returns the length.
+__device__ __forceinline__ uint32_t list_build(bool pick, uint32_t t, uint32_t *s_list, uint32_t *s_cnt)
 {
     const int lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
     const unsigned long long m = __ballot(pick);
-    uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(s_n, (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
+    if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = 0, n = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        const uint32_t c = s_cnt[w];
+        base += w < wave ? c : 0u;
+        n += c;
+    }
     if (pick) s_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+    __syncthreads();
+    return n;
 }
 
 // Field copies of a group's flagged slots.  The op of slot k lives in lane
@@ -589,6 +603,36 @@ __device__ __forceinline__ void list_append(bool pick, uint32_t t, uint32_t *s_l
 // w < fw, and lane gl of the group handles words gl, gl + G, ...  Loads of
 // KB slots are issued before their stores, which requires that no two
 // flagged slots of one call write the same word (callers guarantee it).
+// One lane copies one field of fw <= 32 words (src_off a multiple of 4
+// words): every 16-byte load of the field is issued before the first store,
+// so a wave has all its fields in flight in one round.  Stores are 16 bytes
+// wide where dst_off is a multiple of 4 words and the vector lies inside the
+// field, else word by word, so nothing past word fw is written.
+__device__ __forceinline__ void lane_copy(bool active, uint64_t src_off, uint64_t dst_off,
+                                          uint32_t delta, const uint32_t *src, uint32_t *dst,
+                                          uint32_t fw)
+{
+    if (!active) return;
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src + src_off);
+    uint4 v[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u)
+        if (4 * u < fw) v[u] = s4[u];
+    const bool vec = (dst_off & 3ull) == 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+        if (4 * u >= fw) continue;
+        const uint32_t w[4] = {v[u].x + delta, v[u].y + delta, v[u].z + delta, v[u].w + delta};
+        if (vec && 4 * u + 4 <= fw) {
+            *reinterpret_cast<uint4 *>(dst + dst_off + 4 * u) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r)
+                if (4 * u + r < fw) dst[dst_off + 4 * u + r] = w[r];
+        }
+    }
+}
+
 template <int G, int KB>
 __device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t src_off,
                                            uint64_t dst_off, uint32_t delta, const uint32_t *src,
@@ -638,35 +682,36 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint8_t *__restrict__ doomed,
     const uint8_t *__restrict__ raw_flag, const uint8_t *__restrict__ war_flag, uint8_t epoch,
     const uint32_t *__restrict__ table, uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
-    uint32_t max_scan, uint32_t L, DeviceCounters *__restrict__ ctr)
+    uint32_t max_scan, uint32_t L, uint32_t per_lane, DeviceCounters *__restrict__ ctr)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ unsigned long long s_bits[CH / 64];
-    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_cnt[kBlock / 64];
     const int lane = lane_id();
     const uint32_t chunk0 = blockIdx.x * CH;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
+    const uint32_t t0 = chunk0 + threadIdx.x;
+    const bool commit = threadIdx.x < CH && t0 < B && doomed[t0] != epoch &&
+                        !(raw_flag[t0] == epoch && war_flag[t0] == epoch);
     {
-        const uint32_t t = chunk0 + threadIdx.x;
-        const bool in = threadIdx.x < CH && t < B;
-        const bool commit =
-            in && doomed[t] != epoch && !(raw_flag[t] == epoch && war_flag[t] == epoch);
         const unsigned long long m = __ballot(commit);
         if (threadIdx.x < CH && lane == 0) s_bits[threadIdx.x / 64] = m;
-        list_append(commit, t, s_list, &s_n);
     }
-    __syncthreads();
-    const uint32_t n = s_n;
+    const uint32_t n = list_build(commit, t0, s_list, s_cnt);
     constexpr uint32_t PER = 64 / G;  // transactions per wave per round
     const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
-    for (uint32_t j0 = (threadIdx.x / 64) * PER; j0 < n; j0 += (kBlock / 64) * PER) {
+    for (uint32_t j0 = (blockIdx.y * (kBlock / 64) + threadIdx.x / 64) * PER; j0 < n;
+         j0 += gridDim.y * (kBlock / 64) * PER) {
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
         const uint64_t i = (uint64_t)gl * B + t;
         const Op o = load_op(keys, meta, i, has && gl < S, rows, fields, max_scan);
-        if (fw <= 2 && L > 1) {
+        if (per_lane) {
+            // one lane per read op: all fields of a round in flight at once
+            for (uint32_t k = 0; k < o.len; ++k)
+                lane_copy(true, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                          read_values, fw);
+        } else if (fw <= 2 && L > 1) {
             // narrow fields of scans: each lane copies its own rows, four
             // loads in flight at a time
             const uint64_t src0 = (o.row * fields + o.field) * fsw, step = (uint64_t)fields * fsw;
@@ -692,6 +737,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
             }
         }
     }
+    if (blockIdx.y != 0) return;  // the chunk's other blocks only read out
     const uint32_t nwords = (B + 63) / 64;
     if (threadIdx.x < CH / 64 && chunk0 / 64 + threadIdx.x < nwords)
         bitmap[chunk0 / 64 + threadIdx.x] = s_bits[threadIdx.x];
@@ -715,25 +761,21 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     uint32_t fw, uint32_t fsw, uint32_t *__restrict__ table, const uint64_t *__restrict__ bitmap,
-    uint32_t values, uint64_t hot)
+    uint32_t values, uint64_t hot, uint32_t per_lane)
 {
     __shared__ uint32_t s_list[CH];
-    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_cnt[kBlock / 64];
     const int lane = lane_id();
     const uint32_t chunk0 = blockIdx.x * CH;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    {
-        const uint32_t t = chunk0 + threadIdx.x;
-        // t < B also guards a bitmap word left by an earlier, larger batch
-        // when this phase runs without a fresh CHECK
-        list_append(threadIdx.x < CH && t < B && ((bitmap[t / 64] >> (t % 64)) & 1ull), t, s_list, &s_n);
-    }
-    __syncthreads();
-    const uint32_t n = s_n;
+    const uint32_t t0 = chunk0 + threadIdx.x;
+    // t0 < B also guards a bitmap word left by an earlier, larger batch
+    // when this phase runs without a fresh CHECK
+    const uint32_t n = list_build(threadIdx.x < CH && t0 < B && ((bitmap[t0 / 64] >> (t0 % 64)) & 1ull), t0,
+                                  s_list, s_cnt);
     constexpr uint32_t PER = 64 / G;
     const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
-    for (uint32_t j0 = (threadIdx.x / 64) * PER; j0 < n; j0 += (kBlock / 64) * PER) {
+    for (uint32_t j0 = (blockIdx.y * (kBlock / 64) + threadIdx.x / 64) * PER; j0 < n;
+         j0 += gridDim.y * (kBlock / 64) * PER) {
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
@@ -754,6 +796,10 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
                 ++count;
                 first &= k >= gl;
             }
+        }
+        if (per_lane) {
+            lane_copy(wr && first, off, off, step * count, table, table, fw);
+            continue;
         }
         const unsigned long long am = __ballot(wr && first);
         group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
@@ -985,7 +1031,27 @@ int group_for(uint32_t S)
     default: CALL(64); break;   \
     }
 
-inline dim3 grid_chunks(uint32_t B, uint32_t ch) { return dim3((B + ch - 1) / ch); }
+// A chunk's committed transactions are shared out over gridDim.y blocks,
+// each of which rebuilds the chunk's list (CH flag reads), until the grid
+// has about 2048 blocks or every wave of the chunk copies a single round of
+// transactions: a small epoch's read-out and write-back then run as many
+// short waves instead of a few long ones.
+inline dim3 grid_chunks(uint32_t B, uint32_t ch, uint32_t g)
+{
+    const uint32_t chunks = (B + ch - 1) / ch;
+    const uint32_t per_block = (kBlock / 64) * (64 / g);  // transactions per block and round
+    const uint32_t max_split = (ch + per_block - 1) / per_block;
+    uint32_t split = (2048 + chunks - 1) / chunks;
+    if (split > max_split) split = max_split;
+    return dim3(chunks, split < 1 ? 1 : split);
+}
+
+// The per-lane copy needs fields that start on 16-byte boundaries and fit
+// eight 16-byte loads.
+static uint32_t lane_copy_ok(const KvState &kv)
+{
+    return kv.lane_copy && kv.field_stride % 4 == 0 && kv.field_words <= 32 ? 1u : 0u;
+}
 
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                            DeviceCounters *ctr, hipStream_t s)
@@ -998,10 +1064,10 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
         return hipGetLastError();
     }
 #define CALL1(GV, CHV)                                                                               \
-    hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
+    hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV, GV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words, kv.field_stride,   \
                        kv.doomed, kv.raw_flag, kv.war_flag, e.epoch, kv.table, kv.read_values, kv.bitmap,  \
-                       kv.max_scan, kv.read_slot, ctr)
+                       kv.max_scan, kv.read_slot, lane_copy_ok(kv), ctr)
 #define CALL(GV)                    \
     if (kv.small_chunks)            \
         CALL1(GV, 64);              \
@@ -1032,9 +1098,9 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 {
     if (b.n_txns == 0) return hipSuccess;
 #define CALL1(GV, CHV)                                                                                      \
-    hipLaunchKernelGGL((kv_writeback_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV), dim3(kBlock), 0, s, b.keys, \
+    hipLaunchKernelGGL((kv_writeback_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV, GV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,             \
-                       kv.field_stride, kv.table, kv.bitmap, values, kv.hot_row)
+                       kv.field_stride, kv.table, kv.bitmap, values, kv.hot_row, lane_copy_ok(kv))
 #define CALL(GV)         \
     if (kv.small_chunks) \
         CALL1(GV, 64);   \

@@ -90,6 +90,7 @@ struct KvState {
     uint32_t exec_per_op = 0;
     uint32_t split_regs = 0;
     uint32_t small_chunks = 0;
+    uint32_t lane_copy = 0;
     // hot row (LTPG_NO_HOT_ROW or a row): its ops never register or
     // conflict; committed writes add through hot_acc ([kCounterStripes][fields])
     uint64_t hot_row = LTPG_NO_HOT_ROW;

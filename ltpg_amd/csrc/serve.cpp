@@ -400,6 +400,16 @@ void complete(ltpg_server *sv, int w, Flight &f)
     f.busy = false;
 }
 
+bool any_flight_done(ltpg_server *sv)
+{
+    for (auto &L : sv->lane) {
+        if (!L.ctx) continue;
+        for (auto &f : L.flights)
+            if (f.busy && hipEventQuery(f.done) != hipErrorNotReady) return true;
+    }
+    return false;
+}
+
 void serve_loop(ltpg_server *sv)
 {
     (void)hipSetDevice((sv->lane[0].ctx ? sv->lane[0].ctx : sv->lane[1].ctx)->device);
@@ -461,7 +471,18 @@ void serve_loop(ltpg_server *sv)
         if (sv->stop && !inflight && (!pending || sv->error)) break;
         if (!progress) {
             lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(inflight ? 5 : 50));
+            if (inflight) {
+                // Poll the epochs in flight rather than sleep: a sleep lasts
+                // at least a scheduler tick (~50 us), as long as a whole
+                // small epoch on the device.  Only this thread touches the
+                // flights, so their events are read without the lock; a
+                // client asking for the lock ends the poll.
+                const uint64_t until = now_ns() + 200000;
+                while (sv->waiting.load() == 0 && !any_flight_done(sv) && now_ns() < until)
+                    std::this_thread::yield();
+            } else {
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
             lk.lock();
         } else if (sv->waiting.load() > 0) {
             lk.unlock();

@@ -156,13 +156,17 @@ def test_plain_epochs_after_a_graph_run(gpu):
     r.close()
 
 
+@pytest.mark.parametrize("fw", [2, 25], ids=["8B", "100B"])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
-@pytest.mark.parametrize("flags", range(8))
-def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row):
+@pytest.mark.parametrize("flags", range(16))
+def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw):
+    """Every launch shape against the oracle; 100-byte fields (stride 32
+    words) take the per-lane copy under TUNE_LANE_COPY, 8-byte ones keep the
+    group copy."""
     from oracle import oracle
     from tests import kvcases as kc
 
-    rows, fields, fw, B, S = 5000, 3, 2, 3000, 12
+    rows, fields, B, S = 5000, 3, 3000, 12
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw, max_txns=B,
                     kv_max_slots=S, kv_hot_row=hot_row)
     table = kc.initial_table(rows, fields, fw)
@@ -193,7 +197,7 @@ def test_autotuned_graph_matches_untuned_plain(gpu):
     descs1, keep1, _ = _batches(np.random.default_rng(11), 30, kv1, tp1)
     tuned = L.Batched(kv1, tp1, descs1)
     table = tuned.autotune(reps=2)
-    assert table and all(len(e["ms"]) == 8 and min(e["ms"]) > 0 for e in table)
+    assert table and all(len(e["ms"]) == 16 and min(e["ms"]) > 0 for e in table)
     tuned.run(L.RUN_GRAPH)
     got = _bitmaps(tuned, meta)
     for a, b in zip(want, got):

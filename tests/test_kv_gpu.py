@@ -163,13 +163,15 @@ def _assert_scan_reads(gpu_reads, ora, keys, meta, B, S, rows, fw, max_scan):
     np.testing.assert_array_equal(gpu_reads[mask], ora.read_values[mask])
 
 
-@pytest.mark.parametrize("flags", [0, 1, 4, 7])
+@pytest.mark.parametrize("fw", [2, 4], ids=["8B", "16B"])
+@pytest.mark.parametrize("flags", [0, 1, 4, 7, 8, 15])
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
-def test_gpu_scans_match_oracle(gpu, counts, flags):
+def test_gpu_scans_match_oracle(gpu, counts, flags, fw):
     """Reads as scans (YCSB/2: each read a range of rows): ragged 1..12 ops,
     90 % scans of 10 rows, kv_max_scan 16 (so read-value slots are wider than
-    the scans), several epochs in place, every launch shape family."""
-    rows, fields, fw, B, S, max_scan = 5000, 3, 2, 3000, 12, 16
+    the scans), several epochs in place, every launch shape family; 16-byte
+    fields take the per-lane copy under TUNE_LANE_COPY."""
+    rows, fields, B, S, max_scan = 5000, 3, 3000, 12, 16
     table = kc.initial_table(rows, fields, fw)
     with _ctx(rows, fields, fw, B, S, counts=counts, max_scan=max_scan) as ctx:
         ctx.kv_table_load(table)
