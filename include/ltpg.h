@@ -79,6 +79,12 @@ extern "C" {
 /* ---- memory kinds ------------------------------------------------------ */
 #define LTPG_MEM_HOST 0u
 #define LTPG_MEM_DEVICE 1u
+/* OR'd into the memory kind of ltpg_kv_submit: the batch is request-major,
+ * slot s of transaction t at [t * n_slots + s] (each transaction's slots one
+ * contiguous run, as a request queue stages them).  The phases read it in
+ * place; committed reads keep their slot-major positions.  No counterpart
+ * in the reference, whose batches are slot-major only. */
+#define LTPG_BATCH_REQUEST_MAJOR 0x10u
 
 /* ---- table layouts ----------------------------------------------------- */
 /* COLUMN_MAJOR is the reference's SNAPSHOT layout, tbl[col * rows + row]
@@ -237,7 +243,8 @@ typedef struct ltpg_kv_batch {
 
 /* Make `batch` the context's current epoch (copy_to_device,
  * YCSB/0/Query.h:197-216).  Starts a new epoch: registers of the previous
- * epoch become stale without a memset. */
+ * epoch become stale without a memset.  `mem`: LTPG_MEM_HOST or
+ * LTPG_MEM_DEVICE, optionally | LTPG_BATCH_REQUEST_MAJOR. */
 int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *batch, uint32_t mem);
 
 /* ---- order-entry tables (SNAPSHOT, TPCC-allopt-final/Datastructure.h:119-130;

@@ -36,6 +36,7 @@ ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_RANGE, ERR_BUDGET, ERR_UNSUPP
 RULES_REFERENCE, RULES_INTENDED = 0, 1
 VALUES_COPYBACK, VALUES_RMW = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
+BATCH_REQUEST_MAJOR = 0x10
 LAYOUT_ROW_MAJOR, LAYOUT_COLUMN_MAJOR = 0, 1
 WORKLOAD_KV, WORKLOAD_TPCC = 1, 2
 TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT, TYPE_ORDERSTATUS, TYPE_STOCKLEVEL = 0, 1, 2, 3, 4
@@ -145,7 +146,8 @@ class DeviceBuffer:
 class DeviceKvBatch:
     """A key-value batch resident in device memory (inputs already in HBM)."""
 
-    def __init__(self, ctx, keys, meta, tids, n_slots):
+    def __init__(self, ctx, keys, meta, tids, n_slots, request_major=False):
+        self.request_major = bool(request_major)
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         meta = np.ascontiguousarray(meta, dtype=np.uint32)
         tids = np.ascontiguousarray(tids, dtype=np.uint32)
@@ -226,7 +228,9 @@ class Context:
         _check(lib.ltpg_kv_table_fill_reference(self.handle), "ltpg_kv_table_fill_reference")
 
     # ---- batches and phases --------------------------------------------
-    def kv_submit(self, keys, meta, tids, n_slots):
+    def kv_submit(self, keys, meta, tids, n_slots, request_major=False):
+        """Submit a host batch: slot-major keys/meta ([s * B + t]), or with
+        request_major ([t * n_slots + s])."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         meta = np.ascontiguousarray(meta, dtype=np.uint32)
         tids = np.ascontiguousarray(tids, dtype=np.uint32)
@@ -235,12 +239,14 @@ class Context:
             raise ValueError("keys/meta must hold n_slots * n_txns entries")
         b = KvBatch(n, int(n_slots), keys.ctypes.data if keys.size else None,
                     meta.ctypes.data if meta.size else None, tids.ctypes.data if n else None)
-        _check(lib.ltpg_kv_submit(self.handle, ctypes.byref(b), MEM_HOST), "ltpg_kv_submit")
+        _check(lib.ltpg_kv_submit(self.handle, ctypes.byref(b), MEM_HOST | (BATCH_REQUEST_MAJOR if request_major else 0)),
+               "ltpg_kv_submit")
         self._cur_txns, self._cur_slots = n, int(n_slots)
 
     def kv_submit_device(self, batch):
         b = batch.struct()
-        _check(lib.ltpg_kv_submit(self.handle, ctypes.byref(b), MEM_DEVICE), "ltpg_kv_submit")
+        mem = MEM_DEVICE | (BATCH_REQUEST_MAJOR if batch.request_major else 0)
+        _check(lib.ltpg_kv_submit(self.handle, ctypes.byref(b), mem), "ltpg_kv_submit")
         self._cur_txns, self._cur_slots = batch.n_txns, batch.n_slots
 
     def epoch_run(self, phases=PHASE_ALL):

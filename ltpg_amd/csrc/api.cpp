@@ -639,11 +639,14 @@ int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
     if (b->n_txns > ctx->cfg.max_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n_txns exceeds max_txns");
     if (b->n_slots > kv.max_slots) LTPG_FAIL(LTPG_ERR_INVALID, "n_slots exceeds kv_max_slots");
     if (b->n_txns > 0 && (!b->keys || !b->meta || !b->tids)) LTPG_FAIL(LTPG_ERR_INVALID, "null batch array");
+    const bool request_major = (mem & LTPG_BATCH_REQUEST_MAJOR) != 0;
+    mem &= ~LTPG_BATCH_REQUEST_MAJOR;
     if (mem != LTPG_MEM_HOST && mem != LTPG_MEM_DEVICE) LTPG_FAIL(LTPG_ERR_INVALID, "unknown memory kind");
     if ((rc = next_epoch(ctx))) return rc;
     KvBatchDev d;
     d.n_txns = b->n_txns;
     d.n_slots = b->n_slots;
+    d.request_major = request_major;
     if (mem == LTPG_MEM_DEVICE) {
         d.keys = b->keys;
         d.meta = b->meta;

@@ -165,6 +165,20 @@ __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
     return o;
 }
 
+// Batch layouts.  Slot s of transaction t sits at s * B + t (slot-major,
+// the reference's LOC[B*op + NO]) or, with rm, at t * S + s (request-major:
+// the served form's staging, where a transaction's slots are one contiguous
+// run).  Committed reads always go to the slot-major position.
+__device__ __forceinline__ uint64_t op_at(uint32_t s, uint32_t t, uint32_t B, uint32_t S, bool rm)
+{
+    return rm ? (uint64_t)t * S + s : (uint64_t)s * B + t;
+}
+
+__device__ __forceinline__ uint32_t txn_of(uint64_t i, uint32_t B, uint32_t S, bool rm)
+{
+    return rm ? (uint32_t)(i / S) : (uint32_t)(i % B);
+}
+
 // Zero the per-epoch commit counts (a kernel rather than a memset so the
 // epoch can be captured in a graph).
 __global__ void reset_epoch_counters_kernel(DeviceCounters *ctr)
@@ -306,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
     uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts, uint32_t max_scan,
-    DeviceCounters *__restrict__ ctr)
+    DeviceCounters *__restrict__ ctr, bool rm)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const bool live = t < B;
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     bool waw = false;
     uint32_t bad = 0;
     for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields, max_scan);
+        const Op o = load_op(keys, meta, op_at(s, t, B, S, rm), live, rows, fields, max_scan);
         bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
         if (!o.valid) continue;
         if (o.kind == LTPG_KV_WRITE) {
@@ -333,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_kernel(
     if (bad) atomicAdd(&ctr->range_errors, bad);
 }
 
-// EXECUTE, one lane per op (slot-major op index s * B + t) instead of one
+// EXECUTE, one lane per op (batch index i) instead of one
 // lane per transaction: S times the lanes, each with a single registration,
 // for small or ragged batches that would leave the chip idle.
 __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
@@ -341,15 +355,14 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
     uint8_t *__restrict__ doomed, uint8_t epoch, bool reg_reads, bool zero_counts, uint32_t max_scan,
-    DeviceCounters *__restrict__ ctr)
+    DeviceCounters *__restrict__ ctr, bool rm)
 {
     if (zero_counts && blockIdx.x == 0)
         for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
             ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < (uint64_t)S * B;
-    const uint64_t s = live ? i / B : 0;
-    const uint32_t t = (uint32_t)(i - s * B);
+    const uint32_t t = live ? txn_of(i, B, S, rm) : 0u;
     const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
     const bool bad = o.kind != LTPG_KV_NONE && !o.valid;
     if (o.valid) {
@@ -381,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
     uint8_t *__restrict__ doomed, uint8_t epoch, bool zero_counts, uint32_t max_scan,
-    DeviceCounters *__restrict__ ctr)
+    DeviceCounters *__restrict__ ctr, bool rm)
 {
     __shared__ unsigned long long s_key[kDedupSlots];  // row + 1, 0 = empty
     __shared__ unsigned long long s_min[kDedupSlots];
@@ -404,8 +417,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     for (uint32_t k = 0; k < kDedupPer; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         const bool live = i < n_ops;
-        const uint64_t s = live ? i / B : 0;
-        const uint32_t t = (uint32_t)(i - s * B);
+        const uint32_t t = live ? txn_of(i, B, S, rm) : 0u;
         const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
         bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
         wr[k] = o.valid && o.kind == LTPG_KV_WRITE && o.row != hot;
@@ -472,12 +484,11 @@ __global__ __launch_bounds__(kBlock) void kv_mark_reads_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
-    uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch, uint32_t max_scan)
+    uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch, uint32_t max_scan, bool rm)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < (uint64_t)S * B;
-    const uint64_t s = live ? i / B : 0;
-    const uint32_t t = (uint32_t)(i - s * B);
+    const uint32_t t = live ? txn_of(i, B, S, rm) : 0u;
     const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
     if (o.len == 0) return;  // writes, empty and invalid slots
     const unsigned long long me = priority(tag, tids[t], t);
@@ -505,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
     uint64_t *__restrict__ bitmap, uint32_t *__restrict__ conflicts, uint32_t max_scan,
-    DeviceCounters *__restrict__ ctr)
+    DeviceCounters *__restrict__ ctr, bool rm)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -513,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_counting_kernel(
     const unsigned long long me = live ? priority(tag, tids[t], t) : 0ull;
     uint32_t raw = 0, war = 0, waw = 0;
     for (uint32_t s = 0; s < S; ++s) {
-        const Op o = load_op(keys, meta, (uint64_t)s * B + t, live, rows, fields, max_scan);
+        const Op o = load_op(keys, meta, op_at(s, t, B, S, rm), live, rows, fields, max_scan);
         if (!o.valid) continue;
         if (o.kind != LTPG_KV_WRITE) {
             for (uint32_t k = 0; k < o.len; ++k)  // the hot row never conflicts
@@ -543,7 +554,8 @@ template <int LP>
 __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, uint32_t B, uint32_t S,
     uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint32_t *__restrict__ table,
-    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ read_values, uint32_t max_scan, uint32_t L)
+    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ read_values, uint32_t max_scan, uint32_t L,
+    bool rm)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const int lane = lane_id();
@@ -553,8 +565,8 @@ __global__ __launch_bounds__(kBlock) void kv_readout_kernel(
     if (word == 0) return;
     const bool committed = t < B && ((word >> lane) & 1ull);
     for (uint32_t s = 0; s < S; ++s) {
-        const uint64_t i = (uint64_t)s * B + t;
-        const Op o = load_op(keys, meta, i, committed, rows, fields, max_scan);
+        const uint64_t i = (uint64_t)s * B + t;  // slot-major: where its reads go
+        const Op o = load_op(keys, meta, op_at(s, t, B, S, rm), committed, rows, fields, max_scan);
         for (uint32_t k = 0;; ++k) {  // row k of each read / scan
             const unsigned long long mask = __ballot(k < o.len);
             if (!mask) break;
@@ -682,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, const uint8_t *__restrict__ doomed,
     const uint8_t *__restrict__ raw_flag, const uint8_t *__restrict__ war_flag, uint8_t epoch,
     const uint32_t *__restrict__ table, uint32_t *__restrict__ read_values, uint64_t *__restrict__ bitmap,
-    uint32_t max_scan, uint32_t L, uint32_t per_lane, DeviceCounters *__restrict__ ctr)
+    uint32_t max_scan, uint32_t L, uint32_t per_lane, DeviceCounters *__restrict__ ctr, bool rm)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ unsigned long long s_bits[CH / 64];
@@ -704,8 +716,8 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint32_t j = j0 + grp;
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
-        const uint64_t i = (uint64_t)gl * B + t;
-        const Op o = load_op(keys, meta, i, has && gl < S, rows, fields, max_scan);
+        const uint64_t i = (uint64_t)gl * B + t;  // slot-major: where its reads go
+        const Op o = load_op(keys, meta, op_at(gl, t, B, S, rm), has && gl < S, rows, fields, max_scan);
         if (per_lane) {
             // one lane per read op: all fields of a round in flight at once
             for (uint32_t k = 0; k < o.len; ++k)
@@ -761,7 +773,7 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     uint32_t fw, uint32_t fsw, uint32_t *__restrict__ table, const uint64_t *__restrict__ bitmap,
-    uint32_t values, uint64_t hot, uint32_t per_lane)
+    uint32_t values, uint64_t hot, uint32_t per_lane, bool rm)
 {
     __shared__ uint32_t s_list[CH];
     __shared__ uint32_t s_cnt[kBlock / 64];
@@ -780,7 +792,7 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
         const bool has = j < n;
         const uint32_t t = has ? s_list[j] : 0u;
         const uint32_t step = (has && values == LTPG_VALUES_RMW) ? (tids[t] | 1u) : 0u;
-        const Op o = load_op(keys, meta, (uint64_t)gl * B + t, has && gl < S, rows, fields);
+        const Op o = load_op(keys, meta, op_at(gl, t, B, S, rm), has && gl < S, rows, fields);
         // writes to the hot row go through kv_hot_sum_kernel
         const bool wr = o.valid && o.kind == LTPG_KV_WRITE && o.row != hot;
         const unsigned long long wm = __ballot(wr);
@@ -815,7 +827,7 @@ constexpr uint32_t kHotLdsFields = 1024;
 __global__ __launch_bounds__(kBlock) void kv_hot_sum_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields, uint64_t hot,
-    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ acc)
+    const uint64_t *__restrict__ bitmap, uint32_t *__restrict__ acc, bool rm)
 {
     __shared__ uint32_t bins[kHotLdsFields];
     const bool lds = fields <= kHotLdsFields;
@@ -828,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void kv_hot_sum_kernel(
     if (committed) {
         const uint32_t delta = tids[t] | 1u;
         for (uint32_t s = 0; s < S; ++s) {
-            const Op o = load_op(keys, meta, (uint64_t)s * B + t, true, rows, fields);
+            const Op o = load_op(keys, meta, op_at(s, t, B, S, rm), true, rows, fields);
             if (!o.valid || o.kind != LTPG_KV_WRITE || o.row != hot) continue;
             if (lds) atomicAdd(&bins[o.field], delta);
             else atomicAdd(&stripe[o.field], delta);
@@ -969,16 +981,6 @@ hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows
     return hipGetLastError();
 }
 
-hipError_t launch_slot_major(const uint64_t *keys_rm, const uint32_t *meta_rm, uint64_t *keys, uint32_t *meta,
-                             uint32_t n_txns, uint32_t n_slots, hipStream_t s)
-{
-    if ((uint64_t)n_txns * n_slots == 0) return hipSuccess;
-    const dim3 g = grid_stride((uint64_t)n_txns * n_slots);
-    hipLaunchKernelGGL(transpose_kernel<uint64_t>, g, dim3(kBlock), 0, s, keys_rm, keys, (uint64_t)n_txns, n_slots, 0);
-    hipLaunchKernelGGL(transpose_kernel<uint32_t>, g, dim3(kBlock), 0, s, meta_rm, meta, (uint64_t)n_txns, n_slots, 0);
-    return hipGetLastError();
-}
-
 RegLayout reg_layout(const KvState &kv)
 {
     RegLayout rl;
@@ -996,20 +998,21 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
     if (kv.exec_per_op && !counts) {
         hipLaunchKernelGGL(kv_execute_dedup_kernel, dim3((unsigned)((ops + kDedupOps - 1) / kDedupOps)), dim3(kBlock),
                            0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
-                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, kv.max_scan, ctr);
+                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, kv.max_scan, ctr,
+                           b.request_major);
     } else if (kv.exec_per_op) {
         hipLaunchKernelGGL(kv_execute_ops_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                            b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed,
-                           e.epoch, counts, zero_counts, kv.max_scan, ctr);
+                           e.epoch, counts, zero_counts, kv.max_scan, ctr, b.request_major);
     } else {
         hipLaunchKernelGGL(kv_execute_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
                            b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.doomed, e.epoch, counts, zero_counts, kv.max_scan, ctr);
+                           kv.doomed, e.epoch, counts, zero_counts, kv.max_scan, ctr, b.request_major);
     }
     if (!counts)
         hipLaunchKernelGGL(kv_mark_reads_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                            b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.raw_flag, kv.war_flag, e.epoch, kv.max_scan);
+                           kv.raw_flag, kv.war_flag, e.epoch, kv.max_scan, b.request_major);
     return hipGetLastError();
 }
 
@@ -1060,14 +1063,14 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
     if (counts) {
         hipLaunchKernelGGL(kv_check_counting_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys,
                            b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv),
-                           kv.hot_row, e.tag, kv.bitmap, kv.conflicts, kv.max_scan, ctr);
+                           kv.hot_row, e.tag, kv.bitmap, kv.conflicts, kv.max_scan, ctr, b.request_major);
         return hipGetLastError();
     }
 #define CALL1(GV, CHV)                                                                               \
     hipLaunchKernelGGL((kv_check_fast_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV, GV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words, kv.field_stride,   \
                        kv.doomed, kv.raw_flag, kv.war_flag, e.epoch, kv.table, kv.read_values, kv.bitmap,  \
-                       kv.max_scan, kv.read_slot, lane_copy_ok(kv), ctr)
+                       kv.max_scan, kv.read_slot, lane_copy_ok(kv), ctr, b.request_major)
 #define CALL(GV)                    \
     if (kv.small_chunks)            \
         CALL1(GV, 64);              \
@@ -1087,7 +1090,7 @@ hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t
     hipLaunchKernelGGL(kv_readout_kernel<LPV>, grid_for(b.n_txns), dim3(kBlock), 0, s,    \
                        b.keys, b.meta, b.n_txns, b.n_slots, kv.rows, kv.fields,           \
                        kv.field_words, kv.field_stride, kv.table, kv.bitmap, kv.read_values, kv.max_scan, \
-                       kv.read_slot)
+                       kv.read_slot, b.request_major)
     LTPG_LP_SWITCH(lp, CALL)
 #undef CALL
     return hipGetLastError();
@@ -1100,7 +1103,7 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 #define CALL1(GV, CHV)                                                                                      \
     hipLaunchKernelGGL((kv_writeback_kernel<GV, CHV>), grid_chunks(b.n_txns, CHV, GV), dim3(kBlock), 0, s, b.keys, \
                        b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.field_words,             \
-                       kv.field_stride, kv.table, kv.bitmap, values, kv.hot_row, lane_copy_ok(kv))
+                       kv.field_stride, kv.table, kv.bitmap, values, kv.hot_row, lane_copy_ok(kv), b.request_major)
 #define CALL(GV)         \
     if (kv.small_chunks) \
         CALL1(GV, 64);   \
@@ -1111,7 +1114,8 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 #undef CALL1
     if (kv.hot_row != LTPG_NO_HOT_ROW && values == LTPG_VALUES_RMW) {
         hipLaunchKernelGGL(kv_hot_sum_kernel, grid_for(b.n_txns), dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
-                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.hot_row, kv.bitmap, kv.hot_acc);
+                           b.n_txns, b.n_slots, kv.rows, kv.fields, kv.hot_row, kv.bitmap, kv.hot_acc,
+                           b.request_major);
         hipLaunchKernelGGL(kv_hot_apply_kernel, grid_for(kv.fields), dim3(kBlock), 0, s, kv.table, kv.hot_row,
                            kv.fields, kv.field_words, kv.field_stride, kv.hot_acc);
     }

@@ -53,6 +53,9 @@ struct KvBatchDev {
     const uint64_t *keys = nullptr;
     const uint32_t *meta = nullptr;
     const uint32_t *tids = nullptr;
+    // slot s of transaction t at t * n_slots + s instead of s * n_txns + t
+    // (the served form's staging); committed reads stay slot-major
+    bool request_major = false;
 };
 
 // Register tag of the current epoch: registers hold tag:8 | TID:32 | pos:24
@@ -245,9 +248,6 @@ hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, 
                          uint32_t fsw, int to_strided, hipStream_t s);
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                int to_row_major, hipStream_t s);
-// Request-major [n_txns][n_slots] keys / meta -> the slot-major batch layout.
-hipError_t launch_slot_major(const uint64_t *keys_rm, const uint32_t *meta_rm, uint64_t *keys, uint32_t *meta,
-                             uint32_t n_txns, uint32_t n_slots, hipStream_t s);
 hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                              DeviceCounters *ctr, hipStream_t s, bool zero_counts = true);
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,

@@ -96,9 +96,10 @@ struct Flight {
     uint32_t n_no = 0, n_pay = 0;
     hipEvent_t copied = nullptr, done = nullptr;
     bool used = false;
-    // key-value staging (pinned) and device slot
-    uint64_t *h_keys = nullptr, *d_keys = nullptr, *d_keys_rm = nullptr;
-    uint32_t *h_meta = nullptr, *d_meta = nullptr, *d_meta_rm = nullptr, *h_tids = nullptr, *d_tids = nullptr;
+    // key-value staging (pinned) and device slot, one block each: an epoch
+    // of B transactions packs its keys (8 S B bytes), metas (4 S B) and TIDs
+    // (4 B) back to back, request-major, and goes over in one copy
+    uint8_t *h_stage = nullptr, *d_stage = nullptr;
     // order-entry staging (pinned, packed query blocks) and device slot
     uint32_t *h_no = nullptr, *d_no = nullptr, *h_pay = nullptr, *d_pay = nullptr;
     // results (pinned)
@@ -169,20 +170,14 @@ void free_flight(Flight &f)
 {
     if (f.copied) (void)hipEventDestroy(f.copied);
     if (f.done) (void)hipEventDestroy(f.done);
-    host_free(f.h_keys);
-    host_free(f.h_meta);
-    host_free(f.h_tids);
+    host_free(f.h_stage);
     host_free(f.h_no);
     host_free(f.h_pay);
     host_free(f.h_bits);
     host_free(f.h_bits2);
     host_free(f.h_reads);
     host_free(f.h_reads2);
-    dev_free(f.d_keys);
-    dev_free(f.d_meta);
-    dev_free(f.d_keys_rm);
-    dev_free(f.d_meta_rm);
-    dev_free(f.d_tids);
+    dev_free(f.d_stage);
     dev_free(f.d_no);
     dev_free(f.d_pay);
 }
@@ -196,16 +191,10 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
     if (w == 0) {
         const uint64_t S = sv->cfg.kv_ops;
         const uint64_t fw = (uint64_t)sv->lane[0].ctx->kv.field_words * sv->lane[0].ctx->kv.read_slot;
-        if ((rc = host_alloc(&f.h_keys, S * C)) || (rc = host_alloc(&f.h_meta, S * C)) ||
-            (rc = host_alloc(&f.h_tids, C)) || (rc = host_alloc(&f.h_bits, words(C))))
-            return rc;
+        const uint64_t stage = (12 * S + 4) * C;
+        if ((rc = host_alloc(&f.h_stage, stage)) || (rc = host_alloc(&f.h_bits, words(C)))) return rc;
         if (sv->cfg.keep_reads && (rc = host_alloc(&f.h_reads, S * C * fw))) return rc;
-        if ((rc = dev_alloc(&f.d_keys, S * C, "serving key slot")) ||
-            (rc = dev_alloc(&f.d_meta, S * C, "serving meta slot")) ||
-            (rc = dev_alloc(&f.d_keys_rm, S * C, "serving key staging")) ||
-            (rc = dev_alloc(&f.d_meta_rm, S * C, "serving meta staging")) ||
-            (rc = dev_alloc(&f.d_tids, C, "serving tid slot")))
-            return rc;
+        if ((rc = dev_alloc(&f.d_stage, stage, "serving batch slot"))) return rc;
     } else {
         if ((rc = host_alloc(&f.h_no, 53 * C)) || (rc = host_alloc(&f.h_pay, 10 * C)) ||
             (rc = host_alloc(&f.h_bits, words(C))) || (rc = host_alloc(&f.h_bits2, words(C))))
@@ -261,24 +250,27 @@ int launch(ltpg_server *sv, int w, Flight &f)
     if (f.used) LTPG_HIP_TRY(hipStreamWaitEvent(L.copy, f.done, 0));
     f.used = true;
     if (w == 0) {
-        // requests are stored request-major; the staging keeps that layout
-        // (one contiguous copy per request) and the device transposes it
-        // into the slot-major batch
+        // requests are stored request-major and the epoch runs on that
+        // layout: one contiguous copy per request into the staging, one
+        // copy of the staging, no transpose
         const uint32_t S = sv->cfg.kv_ops, B = (uint32_t)f.members.size();
+        const uint64_t keys_at = 0, meta_at = 8ull * S * B, tids_at = 12ull * S * B;
+        uint64_t *h_keys = reinterpret_cast<uint64_t *>(f.h_stage + keys_at);
+        uint32_t *h_meta = reinterpret_cast<uint32_t *>(f.h_stage + meta_at);
+        uint32_t *h_tids = reinterpret_cast<uint32_t *>(f.h_stage + tids_at);
         for (uint32_t t = 0; t < B; t++) {
             const Req &r = sv->reqs[f.members[t]];
-            f.h_tids[t] = r.tid;
-            std::memcpy(f.h_keys + (uint64_t)t * S, &sv->kv_keys[r.data * S], S * sizeof(uint64_t));
-            std::memcpy(f.h_meta + (uint64_t)t * S, &sv->kv_meta[r.data * S], S * sizeof(uint32_t));
+            h_tids[t] = r.tid;
+            std::memcpy(h_keys + (uint64_t)t * S, &sv->kv_keys[r.data * S], S * sizeof(uint64_t));
+            std::memcpy(h_meta + (uint64_t)t * S, &sv->kv_meta[r.data * S], S * sizeof(uint32_t));
         }
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_keys_rm, f.h_keys, (uint64_t)S * B * 8, hipMemcpyHostToDevice, L.copy));
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_meta_rm, f.h_meta, (uint64_t)S * B * 4, hipMemcpyHostToDevice, L.copy));
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_tids, f.h_tids, (uint64_t)B * 4, hipMemcpyHostToDevice, L.copy));
+        LTPG_HIP_TRY(hipMemcpyAsync(f.d_stage, f.h_stage, tids_at + 4ull * B, hipMemcpyHostToDevice, L.copy));
         LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
         LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
-        LTPG_HIP_TRY(launch_slot_major(f.d_keys_rm, f.d_meta_rm, f.d_keys, f.d_meta, B, S, ctx->stream));
-        ltpg_kv_batch b = {B, S, f.d_keys, f.d_meta, f.d_tids};
-        if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE))) return rc;
+        ltpg_kv_batch b = {B, S, reinterpret_cast<const uint64_t *>(f.d_stage + keys_at),
+                           reinterpret_cast<const uint32_t *>(f.d_stage + meta_at),
+                           reinterpret_cast<const uint32_t *>(f.d_stage + tids_at)};
+        if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE | LTPG_BATCH_REQUEST_MAJOR))) return rc;
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
         LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->kv.bitmap, words(B) * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (sv->cfg.keep_reads)

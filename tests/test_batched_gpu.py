@@ -156,13 +156,14 @@ def test_plain_epochs_after_a_graph_run(gpu):
     r.close()
 
 
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("fw", [2, 25], ids=["8B", "100B"])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
 @pytest.mark.parametrize("flags", range(16))
-def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw):
-    """Every launch shape against the oracle; 100-byte fields (stride 32
-    words) take the per-lane copy under TUNE_LANE_COPY, 8-byte ones keep the
-    group copy."""
+def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw, rm):
+    """Every launch shape against the oracle, batches in either layout;
+    100-byte fields (stride 32 words) take the per-lane copy under
+    TUNE_LANE_COPY, 8-byte ones keep the group copy."""
     from oracle import oracle
     from tests import kvcases as kc
 
@@ -175,7 +176,11 @@ def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw):
     for ep in range(3):
         keys, meta, tids = L.kv_generate(seed=900 + ep, rows=rows, fields=fields, n_txns=B, n_slots=S, min_ops=1,
                                          max_ops=S, theta=0.9, tid_base=ep * B)
-        ctx.kv_submit(keys, meta, tids, S)
+        if rm:
+            ctx.kv_submit(keys.reshape(S, B).T.copy().reshape(-1), meta.reshape(S, B).T.copy().reshape(-1), tids, S,
+                          request_major=True)
+        else:
+            ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run()
         ctx.sync()
         ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, inplace=True, hot_row=hot_row)

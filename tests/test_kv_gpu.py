@@ -23,12 +23,20 @@ def _ctx(rows, fields, fw, max_txns, max_slots, values=L.VALUES_RMW, counts=True
                      flags=L.FLAG_CONFLICT_COUNTS if counts else 0, kv_hot_row=hot_row, kv_max_scan=max_scan)
 
 
-def _gpu_epoch(ctx, keys, meta, tids, S, device=False):
+def _request_major(a, S):
+    """A slot-major [S * B] array as request-major [B * S]."""
+    return np.ascontiguousarray(a.reshape(S, -1).T).reshape(-1)
+
+
+def _gpu_epoch(ctx, keys, meta, tids, S, device=False, rm=False):
+    """One epoch of a slot-major batch; with rm it goes over request-major."""
+    if rm:
+        keys, meta = _request_major(keys, S), _request_major(meta, S)
     if device:
-        b = L.DeviceKvBatch(ctx, keys, meta, tids, S)
+        b = L.DeviceKvBatch(ctx, keys, meta, tids, S, request_major=rm)
         ctx.kv_submit_device(b)
     else:
-        ctx.kv_submit(keys, meta, tids, S)
+        ctx.kv_submit(keys, meta, tids, S, request_major=rm)
     ctx.epoch_run(L.PHASE_ALL)
     ctx.sync()
     counts = bool(ctx.config.flags & L.FLAG_CONFLICT_COUNTS)
@@ -110,9 +118,10 @@ CASES = [
 
 
 @pytest.mark.parametrize("rows,fields,fw,B,S,theta,read_pct,min_ops,seed", CASES)
-@pytest.mark.parametrize("device", [False, True], ids=["host-batch", "device-batch"])
+@pytest.mark.parametrize("device,rm", [(False, False), (True, False), (True, True)],
+                         ids=["host-batch", "device-batch", "device-request-major"])
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
-def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct, min_ops, seed, device,
+def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct, min_ops, seed, device, rm,
                                    counts):
     keys, meta, tids = L.kv_generate(seed=seed, rows=rows, fields=fields, n_txns=B, n_slots=S,
                                      min_ops=min_ops, max_ops=S, read_pct=read_pct, theta=theta)
@@ -121,13 +130,14 @@ def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct,
     ora = oracle.kv_epoch(table0, rows, fields, fw, keys, meta, tids, S, values=1)
     with _ctx(rows, fields, fw, B, S, counts=counts) as ctx:
         ctx.kv_table_load(table0)
-        g = _gpu_epoch(ctx, keys, meta, tids, S, device=device)
+        g = _gpu_epoch(ctx, keys, meta, tids, S, device=device, rm=rm)
     _assert_epoch(g, ora, meta, tids, fw)
 
 
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("hot_row", [0, 17])
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
-def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts):
+def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts, rm):
     """Ops on the hot row never register or conflict; committed writes to it
     add (TID|1) per word through the striped reduce.  Several epochs in place
     at Zipf 0.99 (row 0 is the most drawn key), 100-byte fields and a field
@@ -141,7 +151,7 @@ def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts):
             n = B - ep * 1500
             keys, meta, tids = L.kv_generate(seed=70 + ep, rows=rows, fields=fields, n_txns=n, n_slots=S,
                                              min_ops=1, max_ops=S, theta=0.99, tid_base=ep * B)
-            g = _gpu_epoch(ctx, keys, meta, tids, S)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
             ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1, hot_row=hot_row)
             _assert_epoch(g, ora, meta, tids, fw)
             hot_ops = (keys.reshape(S, n) == hot_row) & ((meta.reshape(S, n) & 3) != 0)
@@ -163,14 +173,16 @@ def _assert_scan_reads(gpu_reads, ora, keys, meta, B, S, rows, fw, max_scan):
     np.testing.assert_array_equal(gpu_reads[mask], ora.read_values[mask])
 
 
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("fw", [2, 4], ids=["8B", "16B"])
 @pytest.mark.parametrize("flags", [0, 1, 4, 7, 8, 15])
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
-def test_gpu_scans_match_oracle(gpu, counts, flags, fw):
+def test_gpu_scans_match_oracle(gpu, counts, flags, fw, rm):
     """Reads as scans (YCSB/2: each read a range of rows): ragged 1..12 ops,
     90 % scans of 10 rows, kv_max_scan 16 (so read-value slots are wider than
-    the scans), several epochs in place, every launch shape family; 16-byte
-    fields take the per-lane copy under TUNE_LANE_COPY."""
+    the scans), several epochs in place, every launch shape family, both
+    batch layouts; 16-byte fields take the per-lane copy under
+    TUNE_LANE_COPY."""
     rows, fields, B, S, max_scan = 5000, 3, 3000, 12, 16
     table = kc.initial_table(rows, fields, fw)
     with _ctx(rows, fields, fw, B, S, counts=counts, max_scan=max_scan) as ctx:
@@ -180,7 +192,7 @@ def test_gpu_scans_match_oracle(gpu, counts, flags, fw):
             keys, meta, tids = L.kv_generate(seed=300 + ep_i, rows=rows, fields=fields, n_txns=B, n_slots=S,
                                              min_ops=1, max_ops=S, read_pct=90, theta=0.7, scan_len=10,
                                              tid_base=ep_i * B)
-            g = _gpu_epoch(ctx, keys, meta, tids, S)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
             ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1, max_scan=max_scan)
             np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
             if g["conflicts"] is not None:
