@@ -252,7 +252,7 @@ def run_kv(args, theta, cpu_baseline=True, name=None):
                    "batch_txns": B, "ops_per_txn": S, "zipf_theta": w["theta"], "form": "plain per-batch",
                    "inputs": "resident in HBM", "distinct_batches": P,
                    "launch_shape": {"flags": shape, "epoch_ms_by_flags": shape_ms,
-                                    "chosen_by": "fastest of the 8 LTPG_TUNE_* shapes on the first batch, "
+                                    "chosen_by": "fastest of the 16 LTPG_TUNE_* shapes (by flags) on the first batch, "
                                                  "before the timed region"}},
         "abort_rate": 1.0 - commits / max(attempts, 1),
         "attempted_txns": attempts,
@@ -596,9 +596,10 @@ def run_served_c5(args):
                    max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW)
     kv.kv_fill_reference()
     # served epochs are small (the server sizes them by their commits under
-    # contention): per-op registration with row folding and 64-transaction
-    # chunks, the shape the batched-form autotuner picks below 2^17 txns
-    kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS)
+    # contention): per-op registration with row folding, 64-transaction
+    # chunks and per-lane field copies, the fastest shape for epochs of
+    # 64-1024 transactions of this table
+    kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY)
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=1 << 20, values=L.VALUES_RMW)
     tp.tpcc_populate(seed)
     log("C5: tables resident")
@@ -676,7 +677,7 @@ def run_served_c5(args):
     return {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
             "latency_ms": best["latency_ms"], "rates": out,
             "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
-                       "slots": 2, "epoch_growth": 8, "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS",
+                       "slots": 2, "epoch_growth": 8, "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
                        "form": "served (continuous admission, requeue with original TID)"}}
 
 
@@ -754,7 +755,7 @@ def main():
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--default-shape", action="store_true",
-                    help="key-value plain runs keep the default launch shape instead of the fastest of the 8")
+                    help="key-value plain runs keep the default launch shape instead of the fastest of the 16")
     ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
                     help="only run the known-size copy used to calibrate traffic counters")
     args = ap.parse_args()

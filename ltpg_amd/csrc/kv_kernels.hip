@@ -200,6 +200,29 @@ __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// u32 copy, grid-stride; the source may be pinned host memory (the served
+// form's staging: a kernel on the epoch's own stream costs one launch, where
+// a copy-engine transfer costs a copy call plus a cross-stream event).
+__global__ void copy_u32_kernel(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Publish an epoch's commit bitmaps to pinned host memory, then its sequence
+// number with a system-scope release: a host that reads the sequence number
+// with an acquire load also reads the bitmaps.  One block.
+__global__ void publish_kernel(const uint64_t *__restrict__ a, uint64_t *__restrict__ ha, uint32_t na,
+                               const uint64_t *__restrict__ b, uint64_t *__restrict__ hb, uint32_t nb,
+                               uint64_t *seq_out, uint64_t seq)
+{
+    for (uint32_t w = threadIdx.x; w < na; w += blockDim.x) ha[w] = a[w];
+    for (uint32_t w = threadIdx.x; w < nb; w += blockDim.x) hb[w] = b[w];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void copy_u64_kernel(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -922,6 +945,20 @@ hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_u64_kernel, grid_stride(n), dim3(kBlock), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_u32_kernel, grid_stride(n), dim3(kBlock), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_publish(const uint64_t *a, uint64_t *ha, uint32_t na, const uint64_t *b, uint64_t *hb,
+                          uint32_t nb, uint64_t *seq_out, uint64_t seq, hipStream_t s)
+{
+    hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(kBlock), 0, s, a, ha, na, b, hb, nb, seq_out, seq);
     return hipGetLastError();
 }
 

@@ -12,8 +12,11 @@
 //              read values per transaction);
 //   overlap    each workload has `slots` epochs in flight: while one runs on
 //              the context's stream, the next is packed into pinned staging
-//              and copied on a copy stream; key-value and order-entry epochs
-//              run on their own contexts' streams at the same time;
+//              and queued behind it; key-value and order-entry epochs run on
+//              their own contexts' streams at the same time;
+//   results    the epoch's last kernel writes its commit bitmaps and then a
+//              sequence number into pinned host memory, which the loop polls
+//              (no event or copy call per epoch);
 //   requeue    an aborted transaction keeps the TID given at its arrival and
 //              joins the next epoch formed after its result is read, ahead of
 //              every later arrival, so it commits after finitely many epochs;
@@ -70,11 +73,14 @@ struct ByTid {
     bool operator()(uint64_t a, uint64_t b) const { return (*reqs)[a].tid > (*reqs)[b].tid; }
 };
 
+// Pinned host memory; `coherent` (fine-grained) for buffers that kernels read
+// or write directly while the host polls them.
 template <typename T>
-int host_alloc(T **p, uint64_t count)
+int host_alloc(T **p, uint64_t count, bool coherent = false)
 {
     *p = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void **>(p), std::max<uint64_t>(count, 1) * sizeof(T), 0) != hipSuccess) {
+    const unsigned flags = coherent ? (hipHostMallocCoherent | hipHostMallocMapped) : 0u;
+    if (hipHostMalloc(reinterpret_cast<void **>(p), std::max<uint64_t>(count, 1) * sizeof(T), flags) != hipSuccess) {
         *p = nullptr;
         LTPG_FAIL(LTPG_ERR_NOMEM, "pinned host allocation failed");
     }
@@ -94,8 +100,10 @@ struct Flight {
     uint64_t epoch = 0;
     std::vector<uint64_t> members;  // request indices, batch order (NewOrders, then Payments)
     uint32_t n_no = 0, n_pay = 0;
-    hipEvent_t copied = nullptr, done = nullptr;
-    bool used = false;
+    hipEvent_t copied = nullptr;     // order-entry staging copied (copy stream)
+    uint64_t *h_seq = nullptr;       // pinned: sequence number of the last published epoch
+    uint64_t seq = 0;                // sequence number of the epoch in flight
+    uint64_t launched_ns = 0;
     // key-value staging (pinned) and device slot, one block each: an epoch
     // of B transactions packs its keys (8 S B bytes), metas (4 S B) and TIDs
     // (4 B) back to back, request-major, and goes over in one copy
@@ -169,7 +177,7 @@ struct ClientLock {
 void free_flight(Flight &f)
 {
     if (f.copied) (void)hipEventDestroy(f.copied);
-    if (f.done) (void)hipEventDestroy(f.done);
+    host_free(f.h_seq);
     host_free(f.h_stage);
     host_free(f.h_no);
     host_free(f.h_pay);
@@ -187,17 +195,18 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
     const uint64_t C = sv->lane[w].cap;
     int rc;
     LTPG_HIP_TRY(hipEventCreateWithFlags(&f.copied, hipEventDisableTiming));
-    LTPG_HIP_TRY(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+    if ((rc = host_alloc(&f.h_seq, 1, true))) return rc;
+    *f.h_seq = 0;
     if (w == 0) {
         const uint64_t S = sv->cfg.kv_ops;
         const uint64_t fw = (uint64_t)sv->lane[0].ctx->kv.field_words * sv->lane[0].ctx->kv.read_slot;
         const uint64_t stage = (12 * S + 4) * C;
-        if ((rc = host_alloc(&f.h_stage, stage)) || (rc = host_alloc(&f.h_bits, words(C)))) return rc;
+        if ((rc = host_alloc(&f.h_stage, stage, true)) || (rc = host_alloc(&f.h_bits, words(C), true))) return rc;
         if (sv->cfg.keep_reads && (rc = host_alloc(&f.h_reads, S * C * fw))) return rc;
         if ((rc = dev_alloc(&f.d_stage, stage, "serving batch slot"))) return rc;
     } else {
         if ((rc = host_alloc(&f.h_no, 53 * C)) || (rc = host_alloc(&f.h_pay, 10 * C)) ||
-            (rc = host_alloc(&f.h_bits, words(C))) || (rc = host_alloc(&f.h_bits2, words(C))))
+            (rc = host_alloc(&f.h_bits, words(C), true)) || (rc = host_alloc(&f.h_bits2, words(C), true)))
             return rc;
         if (sv->cfg.keep_reads &&
             ((rc = host_alloc(&f.h_reads, (uint64_t)kNoReads * C)) || (rc = host_alloc(&f.h_reads2, (uint64_t)kPayReads * C))))
@@ -247,12 +256,12 @@ int launch(ltpg_server *sv, int w, Flight &f)
     ltpg_ctx *ctx = L.ctx;
     const uint64_t C = L.cap;
     int rc;
-    if (f.used) LTPG_HIP_TRY(hipStreamWaitEvent(L.copy, f.done, 0));
-    f.used = true;
+    // a flight is restaged only after its last epoch was published, so
+    // nothing on the device still reads its buffers
     if (w == 0) {
         // requests are stored request-major and the epoch runs on that
-        // layout: one contiguous copy per request into the staging, one
-        // copy of the staging, no transpose
+        // layout: one contiguous copy per request into the staging, which a
+        // kernel on the epoch's stream reads over the bus; no transpose
         const uint32_t S = sv->cfg.kv_ops, B = (uint32_t)f.members.size();
         const uint64_t keys_at = 0, meta_at = 8ull * S * B, tids_at = 12ull * S * B;
         uint64_t *h_keys = reinterpret_cast<uint64_t *>(f.h_stage + keys_at);
@@ -264,19 +273,20 @@ int launch(ltpg_server *sv, int w, Flight &f)
             std::memcpy(h_keys + (uint64_t)t * S, &sv->kv_keys[r.data * S], S * sizeof(uint64_t));
             std::memcpy(h_meta + (uint64_t)t * S, &sv->kv_meta[r.data * S], S * sizeof(uint32_t));
         }
-        LTPG_HIP_TRY(hipMemcpyAsync(f.d_stage, f.h_stage, tids_at + 4ull * B, hipMemcpyHostToDevice, L.copy));
-        LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
-        LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
+        LTPG_HIP_TRY(launch_copy_u32(reinterpret_cast<uint32_t *>(f.d_stage),
+                                     reinterpret_cast<const uint32_t *>(f.h_stage), (tids_at + 4ull * B) / 4,
+                                     ctx->stream));
         ltpg_kv_batch b = {B, S, reinterpret_cast<const uint64_t *>(f.d_stage + keys_at),
                            reinterpret_cast<const uint32_t *>(f.d_stage + meta_at),
                            reinterpret_cast<const uint32_t *>(f.d_stage + tids_at)};
         if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE | LTPG_BATCH_REQUEST_MAJOR))) return rc;
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
-        LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->kv.bitmap, words(B) * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (sv->cfg.keep_reads)
             LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values,
                                         (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words * 4,
                                         hipMemcpyDeviceToHost, ctx->stream));
+        LTPG_HIP_TRY(launch_publish(ctx->kv.bitmap, f.h_bits, (uint32_t)words(B), nullptr, nullptr, 0, f.h_seq,
+                                    ++f.seq, ctx->stream));
     } else {
         const uint32_t n_no = f.n_no, n_pay = f.n_pay;
         for (uint32_t q = 0; q < n_no; q++) {
@@ -305,12 +315,6 @@ int launch(ltpg_server *sv, int w, Flight &f)
         ltpg_payment_view(f.d_pay, (uint32_t)C, n_pay, &pb);
         if ((rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_DEVICE))) return rc;
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
-        if (n_no)
-            LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits, ctx->tpcc.d.bitmap_no, words(n_no) * 8, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-        if (n_pay)
-            LTPG_HIP_TRY(hipMemcpyAsync(f.h_bits2, ctx->tpcc.d.bitmap_pay, words(n_pay) * 8, hipMemcpyDeviceToHost,
-                                        ctx->stream));
         if (sv->cfg.keep_reads) {
             if (n_no)
                 LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->tpcc.d.reads_no, (uint64_t)kNoReads * n_no * 4,
@@ -319,8 +323,10 @@ int launch(ltpg_server *sv, int w, Flight &f)
                 LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads2, ctx->tpcc.d.reads_pay, (uint64_t)kPayReads * n_pay * 4,
                                             hipMemcpyDeviceToHost, ctx->stream));
         }
+        LTPG_HIP_TRY(launch_publish(ctx->tpcc.d.bitmap_no, f.h_bits, (uint32_t)words(n_no), ctx->tpcc.d.bitmap_pay,
+                                    f.h_bits2, (uint32_t)words(n_pay), f.h_seq, ++f.seq, ctx->stream));
     }
-    LTPG_HIP_TRY(hipEventRecord(f.done, ctx->stream));
+    f.launched_ns = now_ns();
     f.busy = true;
     return LTPG_OK;
 }
@@ -392,12 +398,25 @@ void complete(ltpg_server *sv, int w, Flight &f)
     f.busy = false;
 }
 
+// hipErrorNotReady unless the stream has failed, or has been idle for
+// 100 ms since the launch without the epoch's results becoming visible
+// (reported as an error rather than waited for forever).
+hipError_t stream_error(hipStream_t s, uint64_t launched_ns)
+{
+    const hipError_t q = hipStreamQuery(s);
+    if (q != hipSuccess) return q;
+    return now_ns() - launched_ns > 100000000ull ? hipErrorUnknown : hipErrorNotReady;
+}
+
+// The flight's epoch has been published (its results are in host memory).
+bool published(const Flight &f) { return __atomic_load_n(f.h_seq, __ATOMIC_ACQUIRE) == f.seq; }
+
 bool any_flight_done(ltpg_server *sv)
 {
     for (auto &L : sv->lane) {
         if (!L.ctx) continue;
         for (auto &f : L.flights)
-            if (f.busy && hipEventQuery(f.done) != hipErrorNotReady) return true;
+            if (f.busy && published(f)) return true;
     }
     return false;
 }
@@ -422,7 +441,10 @@ void serve_loop(ltpg_server *sv)
             // results first: re-queued aborts join the next epoch formed
             for (auto &f : L.flights) {
                 if (!f.busy) continue;
-                const hipError_t q = hipEventQuery(f.done);
+                // a device error never publishes: after 2 ms ask the stream
+                const hipError_t q = published(f)                              ? hipSuccess
+                                     : now_ns() - f.launched_ns < 2000000ull ? hipErrorNotReady
+                                                                             : stream_error(L.ctx->stream, f.launched_ns);
                 if (q == hipSuccess) {
                     complete(sv, w, f);
                     if (sv->cancel) {  // aborts of a cancelled run are dropped too
