@@ -638,6 +638,59 @@ __device__ __forceinline__ uint32_t list_build(bool pick, uint32_t t, uint32_t *
 // w < fw, and lane gl of the group handles words gl, gl + G, ...  Loads of
 // KB slots are issued before their stores, which requires that no two
 // flagged slots of one call write the same word (callers guarantee it).
+// Position of the j-th set bit of m (j < popcount(m)), by halving.
+__device__ __forceinline__ uint32_t nth_set_bit(unsigned long long m, uint32_t j)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t width = 32; width >= 1; width >>= 1) {
+        const unsigned long long lo = m & ((1ull << width) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(lo);
+        if (j >= c) {
+            j -= c;
+            m >>= width;
+            pos += width;
+        } else {
+            m = lo;
+        }
+    }
+    return pos;
+}
+
+// Field copies of the wave's flagged lanes, fields of at most 32 words, two
+// fields per instruction: lanes 0-31 copy the field of one flagged lane and
+// lanes 32-63 the next, one word per lane, so a 100-byte field is a single
+// coalesced access.  Flagged lane k: src[src_off + w] (+ delta) ->
+// dst[dst_off + w], w < fw.  Up to 32 fields' loads are issued before their
+// stores, which requires that no two flagged fields of one call write the
+// same word (callers guarantee it).
+__device__ __forceinline__ void wave_copy_pairs(unsigned long long mask, uint64_t src_off, uint64_t dst_off,
+                                                uint32_t delta, const uint32_t *src, uint32_t *dst, uint32_t fw)
+{
+    const uint32_t n = (uint32_t)__popcll(mask);
+    const uint32_t half = (uint32_t)lane_id() >> 5, wl = (uint32_t)lane_id() & 31u;
+    for (uint32_t b = 0; b < n; b += 32) {
+        uint32_t v[16], dl[16];
+        uint64_t d[16];
+        bool ok[16];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            const uint32_t j = b + 2 * u + half;
+            const bool has = j < n;
+            const int owner = has ? (int)nth_set_bit(mask, j) : 0;
+            const uint64_t so = __shfl(src_off, owner);
+            d[u] = __shfl(dst_off, owner);
+            dl[u] = __shfl(delta, owner);
+            ok[u] = has && wl < fw;
+            // unconditional load (word 0 when idle), so all 16 are in flight
+            v[u] = src[ok[u] ? so + wl : 0ull];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u)
+            if (ok[u]) dst[d[u] + wl] = v[u] + dl[u];
+    }
+}
+
 // One lane copies one field of fw <= 32 words (src_off a multiple of 4
 // words): every 16-byte load of the field is issued before the first store,
 // so a wave has all its fields in flight in one round.  Stores are 16 bytes
@@ -767,8 +820,12 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
             for (uint32_t k = 0;; ++k) {  // row k of each read / scan
                 const unsigned long long rd = __ballot(k < o.len);
                 if (!rd) break;
-                group_copy<G, 8>(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
-                                 read_values, fw, S, grp, gl);
+                if (fw <= 32)
+                    wave_copy_pairs(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                                    read_values, fw);
+                else
+                    group_copy<G, 8>(rd, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
+                                     read_values, fw, S, grp, gl);
             }
         }
     }
@@ -837,7 +894,10 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
             continue;
         }
         const unsigned long long am = __ballot(wr && first);
-        group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
+        if (fw <= 32)
+            wave_copy_pairs(am, off, off, step * count, table, table, fw);
+        else
+            group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
     }
 }
 
