@@ -575,6 +575,66 @@ def run_batched_c4(args):
     }
 
 
+def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain):
+    """One offered rate of the served-form workload through a fresh server
+    (plain = 1: the same admission run as the plain per-batch form).
+    Requests are submitted in 1 ms windows as their scheduled arrival time
+    comes; latency is commit time minus the scheduled arrival."""
+    oe_tid = np.r_[tb.nf("TID"), tb.pf("TID")]
+    oe_order = np.argsort(oe_tid, kind="stable")  # request k of the batch's own arrival order
+    arrival_of = {}  # request id -> arrival index
+    sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20, plain=plain)
+    window = 1e-3
+    edges = np.searchsorted(at, np.arange(window, at[-1] + window, window), side="right")
+    ki = oi = prev = 0
+    kv_ids, oe_ids = [], []
+    t0_ns = time.monotonic_ns()
+    for e in edges:
+        if e <= prev:
+            continue
+        while time.monotonic_ns() - t0_ns < at[e - 1] * 1e9:
+            pass
+        sel = np.arange(prev, e)
+        kv_arr = sel[kind[prev:e] == 0]
+        oe_arr = sel[kind[prev:e] == 1]
+        if len(kv_arr):
+            first = sv.submit_kv(keys[ki:ki + len(kv_arr)], meta[ki:ki + len(kv_arr)])
+            kv_ids.append((first, kv_arr))
+            ki += len(kv_arr)
+        if len(oe_arr):
+            idx = oe_order[oi:oi + len(oe_arr)]
+            part, perm = _tpcc_select(tb, idx)
+            first = sv.submit_tpcc(part)
+            oe_ids.append((first, oe_arr[perm]))
+            oi += len(oe_arr)
+        prev = e
+    submit_s = (time.monotonic_ns() - t0_ns) * 1e-9
+    done = []
+    deadline = time.monotonic() + args.c5_drain_s
+    form = "plain" if plain else "served"
+    while len(done) < n and time.monotonic() < deadline:
+        time.sleep(0.25)
+        done += sv.poll(1 << 26)
+        log(f"C5 {form}: {rate}M txn/s: {len(done)}/{n} committed")
+    st = sv.stats()
+    dropped = sv.cancel()
+    for first, arr in kv_ids + oe_ids:
+        for j, a_i in enumerate(arr):
+            arrival_of[first + j] = a_i
+    sched = np.array([t0_ns + at[arrival_of[d["id"]]] * 1e9 for d in done])
+    commit = np.array([d["commit_ns"] for d in done], dtype=np.float64)
+    lat = (commit - sched) * 1e-6
+    span_s = (commit.max() - t0_ns) * 1e-9
+    return {"offered_txn_per_s": rate * 1e6, "requests": n, "committed": len(done), "dropped_at_cap": dropped,
+            "completed_fraction": len(done) / n, "drain_cap_s": args.c5_drain_s,
+            "committed_txn_per_s": len(done) / span_s, "submit_s": submit_s, "span_s": span_s,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "epochs": {"kv": st["epochs"][0], "tpcc": st["epochs"][1]},
+            "max_epoch_txns": {"kv": st["max_epoch_txns"][0], "tpcc": st["max_epoch_txns"][1]},
+            "attempts_per_commit": sum(st["attempts"]) / max(sum(st["committed"]), 1)}
+
+
 def run_served_c5(args):
     """Served form under Poisson arrivals: a 50/50 mix of key-value requests
     (1-64 ops, Zipf 0.99, 100M rows x 10 x 100 B) and order-entry requests
@@ -605,7 +665,7 @@ def run_served_c5(args):
     log("C5: tables resident")
     rng = np.random.default_rng(seed)
     gen = L.TpccGenerator(warehouses=W, seed=seed)
-    out = []
+    out, plain_out = [], []
     for rate in args.c5_rates:
         n = max(1000, int(rate * 1e6 * args.c5_seconds))
         n_kv = n // 2
@@ -614,71 +674,38 @@ def run_served_c5(args):
         keys = keys.reshape(S, n_kv).T.copy()
         meta = meta.reshape(S, n_kv).T.copy()
         tb = gen.next(n - n_kv)
-        oe_tid = np.r_[tb.nf("TID"), tb.pf("TID")]
-        oe_order = np.argsort(oe_tid, kind="stable")  # request k of the batch's own arrival order
         at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
         kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
-        arrival_of = {}  # request id -> arrival index
-        sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20)
-        window = 1e-3
-        edges = np.searchsorted(at, np.arange(window, at[-1] + window, window), side="right")
-        ki = oi = prev = 0
-        kv_ids, oe_ids = [], []
-        t0_ns = time.monotonic_ns()
-        for e in edges:
-            if e <= prev:
-                continue
-            while time.monotonic_ns() - t0_ns < at[e - 1] * 1e9:
-                pass
-            sel = np.arange(prev, e)
-            kv_arr = sel[kind[prev:e] == 0]
-            oe_arr = sel[kind[prev:e] == 1]
-            if len(kv_arr):
-                first = sv.submit_kv(keys[ki:ki + len(kv_arr)], meta[ki:ki + len(kv_arr)])
-                kv_ids.append((first, kv_arr))
-                ki += len(kv_arr)
-            if len(oe_arr):
-                idx = oe_order[oi:oi + len(oe_arr)]
-                part, perm = _tpcc_select(tb, idx)
-                first = sv.submit_tpcc(part)
-                oe_ids.append((first, oe_arr[perm]))
-                oi += len(oe_arr)
-            prev = e
-        submit_s = (time.monotonic_ns() - t0_ns) * 1e-9
-        done = []
-        deadline = time.monotonic() + args.c5_drain_s
-        while len(done) < n and time.monotonic() < deadline:
-            time.sleep(0.25)
-            done += sv.poll(1 << 26)
-            log(f"C5: {rate}M txn/s: {len(done)}/{n} committed")
-        st = sv.stats()
-        dropped = sv.cancel()
-        for first, arr in kv_ids + oe_ids:
-            for j, a_i in enumerate(arr):
-                arrival_of[first + j] = a_i
-        sched = np.array([t0_ns + at[arrival_of[d["id"]]] * 1e9 for d in done])
-        commit = np.array([d["commit_ns"] for d in done], dtype=np.float64)
-        lat = (commit - sched) * 1e-6
-        span_s = (commit.max() - t0_ns) * 1e-9
-        out.append({"offered_txn_per_s": rate * 1e6, "requests": n, "committed": len(done), "dropped_at_cap": dropped,
-                    "completed_fraction": len(done) / n, "drain_cap_s": args.c5_drain_s,
-                    "committed_txn_per_s": len(done) / span_s, "submit_s": submit_s, "span_s": span_s,
-                    "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                                   "max": float(lat.max())},
-                    "epochs": {"kv": st["epochs"][0], "tpcc": st["epochs"][1]},
-                    "max_epoch_txns": {"kv": st["max_epoch_txns"][0], "tpcc": st["max_epoch_txns"][1]},
-                    "attempts_per_commit": sum(st["attempts"]) / max(sum(st["committed"]), 1)})
-        log(f"C5: offered {rate}M txn/s -> {out[-1]['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
-            f"p50 {out[-1]['latency_ms']['p50']:.2f} ms, p99 {out[-1]['latency_ms']['p99']:.2f} ms, "
-            f"submit {submit_s:.3f} s for {at[-1]:.3f} s of arrivals")
+        forms = [("served", out, 0)] + ([("plain", plain_out, 1)] if not args.c5_no_plain else [])
+        for form, dest, plain in forms:
+            dest.append(_c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain))
+            r = dest[-1]
+            log(f"C5 {form}: offered {rate}M txn/s -> {r['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
+                f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms, "
+                f"submit {r['submit_s']:.3f} s for {at[-1]:.3f} s of arrivals")
     kv.close()
     tp.close()
     best = max(out, key=lambda r: r["committed_txn_per_s"])
-    return {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
-            "latency_ms": best["latency_ms"], "rates": out,
-            "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
-                       "slots": 2, "epoch_growth": 8, "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
-                       "form": "served (continuous admission, requeue with original TID)"}}
+    res = {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
+           "latency_ms": best["latency_ms"], "rates": out,
+           "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
+                      "slots": 2, "epoch_growth": 8,
+                      "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
+                      "form": "served (continuous admission, requeue with original TID)"}}
+    if plain_out:
+        # the plain per-batch form on the same arrivals and admission: one
+        # epoch at a time, host-copied slot-major batch, phases, sync,
+        # bitmap read back (ltpg_serve_config.plain)
+        pbest = max(plain_out, key=lambda r: r["committed_txn_per_s"])
+        res["plain_value"] = pbest["committed_txn_per_s"]
+        res["speedup_vs_plain"] = res["value"] / max(pbest["committed_txn_per_s"], 1e-9)
+        res["plain_rates"] = plain_out
+        res["vs_plain_by_rate"] = [
+            {"offered_txn_per_s": a["offered_txn_per_s"],
+             "committed_ratio": a["committed_txn_per_s"] / max(b["committed_txn_per_s"], 1e-9),
+             "p99_ms": a["latency_ms"]["p99"], "plain_p99_ms": b["latency_ms"]["p99"]}
+            for a, b in zip(out, plain_out)]
+    return res
 
 
 def _tpcc_select(b, idx):
@@ -744,6 +771,8 @@ def main():
                     help="offered loads of the served form, M txn/s")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
     ap.add_argument("--c5-drain-s", type=float, default=8.0, help="cap on the drain after the arrivals, per load")
+    ap.add_argument("--c5-no-plain", action="store_true",
+                    help="skip the plain per-batch form run at each C5 rate")
     ap.add_argument("--c5-rows", type=int, default=100_000_000, help="key-value rows of the served form")
     ap.add_argument("--c5-budget-mb", type=int, default=4096, help="HBM budget per in-flight epoch, MiB")
     ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")

@@ -522,10 +522,17 @@ typedef struct ltpg_serve_config {
     uint32_t record_epochs;   /* 1: keep every epoch's member list (tests)       */
     uint32_t epoch_growth;    /* > 0: an epoch admits at most epoch_growth x the
                                  commits of the workload's last epoch (at least
-                                 1024) -- under contention epochs commit a short
+                                 256) -- under contention epochs commit a short
                                  chain whatever their size; 0 = always the cap.
                                  Default 8. */
-    uint32_t reserved;
+    uint32_t plain;           /* 1: run the same admission as the plain
+                                 per-batch form instead, for comparison: one
+                                 epoch at a time, the batch copied from host
+                                 memory in the reference's slot-major layout,
+                                 phases, a synchronisation and the commit
+                                 bitmap read back before the next epoch is
+                                 formed (the driver loop of tp.cu:136-261);
+                                 `slots` is ignored.  Default 0. */
 } ltpg_serve_config;
 
 typedef struct ltpg_serve_done {

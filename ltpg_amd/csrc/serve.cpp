@@ -331,6 +331,78 @@ int launch(ltpg_server *sv, int w, Flight &f)
     return LTPG_OK;
 }
 
+// The plain per-batch form of one epoch, for comparison (ltpg_serve_config.
+// plain): the batch is packed on the host in the reference's layout
+// (slot-major key-value ops, value-major order-entry blocks), copied by the
+// submit, run, synchronised, and its bitmaps (and reads) copied back, one
+// step after the other.  Packing happens under the lock (the request store
+// may grow); the device work runs without it.  On return the flight holds
+// a published epoch, as the served path leaves it.
+int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex> &lk)
+{
+    Lane &L = sv->lane[w];
+    ltpg_ctx *ctx = L.ctx;
+    const uint64_t C = L.cap;
+    const uint32_t B = (uint32_t)f.members.size();
+    int rc = LTPG_OK;
+    if (w == 0) {
+        const uint32_t S = sv->cfg.kv_ops;
+        uint64_t *h_keys = reinterpret_cast<uint64_t *>(f.h_stage);
+        uint32_t *h_meta = reinterpret_cast<uint32_t *>(f.h_stage + 8ull * S * B);
+        uint32_t *h_tids = reinterpret_cast<uint32_t *>(f.h_stage + 12ull * S * B);
+        for (uint32_t t = 0; t < B; t++) {
+            const Req &r = sv->reqs[f.members[t]];
+            h_tids[t] = r.tid;
+            for (uint32_t s = 0; s < S; s++) {
+                h_keys[(uint64_t)s * B + t] = sv->kv_keys[r.data * S + s];
+                h_meta[(uint64_t)s * B + t] = sv->kv_meta[r.data * S + s];
+            }
+        }
+        lk.unlock();
+        const ltpg_kv_batch b = {B, S, h_keys, h_meta, h_tids};
+        rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_HOST);
+        if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
+        if (!rc) rc = ltpg_sync(ctx);
+        if (!rc) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_KV, f.h_bits, words(B));
+        if (!rc && sv->cfg.keep_reads)
+            rc = ltpg_kv_read_values(ctx, f.h_reads, (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words);
+    } else {
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay;
+        for (uint32_t q = 0; q < n_no; q++) {
+            const Req &r = sv->reqs[f.members[q]];
+            const uint32_t *src = &sv->no_data[r.data * kNoFields];
+            for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
+            f.h_no[52 * C + q] = r.tid;
+        }
+        for (uint32_t p = 0; p < n_pay; p++) {
+            const Req &r = sv->reqs[f.members[n_no + p]];
+            const uint32_t *src = &sv->pay_data[r.data * kPayFields];
+            for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
+            f.h_pay[9 * C + p] = r.tid;
+        }
+        lk.unlock();
+        ltpg_neworder_batch nb;
+        ltpg_payment_batch pb;
+        ltpg_neworder_view(f.h_no, (uint32_t)C, n_no, &nb);
+        ltpg_payment_view(f.h_pay, (uint32_t)C, n_pay, &pb);
+        rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_HOST);
+        if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
+        if (!rc) rc = ltpg_sync(ctx);
+        if (!rc && n_no) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_NEWORDER, f.h_bits, words(n_no));
+        if (!rc && n_pay) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_PAYMENT, f.h_bits2, words(n_pay));
+        if (!rc && sv->cfg.keep_reads && n_no)
+            rc = ltpg_tpcc_read_values(ctx, LTPG_TYPE_NEWORDER, f.h_reads, (uint64_t)kNoReads * n_no);
+        if (!rc && sv->cfg.keep_reads && n_pay)
+            rc = ltpg_tpcc_read_values(ctx, LTPG_TYPE_PAYMENT, f.h_reads2, (uint64_t)kPayReads * n_pay);
+    }
+    lk.lock();
+    if (rc) return rc;
+    *f.h_seq = ++f.seq;
+    f.launched_ns = now_ns();
+    f.busy = true;
+    return LTPG_OK;
+}
+
 bool bit(const uint64_t *b, uint32_t i) { return (b[i / 64] >> (i % 64)) & 1ull; }
 
 // Read a finished epoch's results (lock held).
@@ -470,7 +542,7 @@ void serve_loop(ltpg_server *sv)
                     for (uint64_t r : f.members) ids.push_back(sv->reqs[r].id);
                     L.log.push_back(std::move(ids));
                 }
-                const int rc = launch(sv, w, f);
+                const int rc = sv->cfg.plain ? launch_plain(sv, w, f, lk) : launch(sv, w, f);
                 if (rc) {
                     sv->error = rc;
                     sv->error_msg = ltpg_last_error();
@@ -597,7 +669,7 @@ int ltpg_serve_start(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_serve_conf
             rc = LTPG_ERR_DEVICE;
             break;
         }
-        L.flights.resize(cfg->slots);
+        L.flights.resize(cfg->plain ? 1 : cfg->slots);  // the plain form: one epoch at a time
         for (auto &f : L.flights)
             if (!rc) rc = alloc_flight(sv, w, f);
     }

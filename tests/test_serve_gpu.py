@@ -41,11 +41,13 @@ def _submit_in_waves(sv, keys, meta, waves=12, pause=0.002):
         time.sleep(pause)
 
 
-def test_served_kv_epochs_replay_on_the_oracle(gpu):
+@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
+def test_served_kv_epochs_replay_on_the_oracle(gpu, plain):
+    """Both the served path and the plain per-batch form (same admission)."""
     rows, fields, fw, S, n = 3000, 3, 2, 8, 6000
     ctx = _kv_ctx(rows, fields, fw, S, 1024)
     keys, meta = _kv_requests(31, n, S, rows, fields)
-    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=1, record_epochs=1)
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=1, record_epochs=1, plain=plain)
     _submit_in_waves(sv, keys, meta)
     sv.drain()
     done = {d["id"]: d for d in sv.poll()}
@@ -172,13 +174,14 @@ def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
     assert set(aa) == {1} and set(ab) == {1}
 
 
-def test_served_order_entry_epochs_replay_on_the_oracle(gpu):
+@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
+def test_served_order_entry_epochs_replay_on_the_oracle(gpu, plain):
     W = 2
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=2048)
     tp.tpcc_populate(SEED)
     gen = L.TpccGenerator(warehouses=W, seed=12)
     batches = [gen.next(1500) for _ in range(4)]
-    sv = L.Server(None, tp, slots=2, max_epoch_txns=900, keep_reads=1, record_epochs=1)
+    sv = L.Server(None, tp, slots=2, max_epoch_txns=900, keep_reads=1, record_epochs=1, plain=plain)
     index = {}  # request id -> (batch, type, position)
     for bi, b in enumerate(batches):
         first = sv.submit_tpcc(b)
