@@ -459,8 +459,9 @@ typedef struct ltpg_batch_desc {
     ltpg_payment_batch payment;
 } ltpg_batch_desc;
 
-/* PLAIN: one epoch per batch, in list order, each followed by a host
- *   synchronisation and commit-count readback (the reference's loop).
+/* PLAIN: one epoch per batch, in list order, as the reference's driver loop
+ *   runs it (TPF/tp.cu:136-261, Y0/ycsb.cu:78-152): each phase kernel
+ *   followed by a stream synchronisation, then the commit-count readback.
  * GRAPH: the batches of each workload, in list order, captured once into a
  *   HIP graph (register refill, then per batch: submit, CID / EXECUTE /
  *   CHECK / WRITEBACK, bitmap archive copy); the key-value and order-entry

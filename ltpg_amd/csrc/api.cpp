@@ -765,6 +765,30 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     return LTPG_OK;
 }
 
+}  // extern "C"
+
+namespace ltpg {
+
+// The reference's launch pattern for one epoch (the driver loops of
+// TPF/tp.cu:136-261 and Y0/ycsb.cu:78-152): one phase kernel at a time, each
+// followed by a stream synchronisation, then the error check.  The plain
+// per-batch form that the batched and served forms are compared with.
+int run_epoch_stepwise(ltpg_ctx *ctx)
+{
+    static const uint32_t kPhases[4] = {LTPG_PHASE_CID, LTPG_PHASE_EXECUTE, LTPG_PHASE_CHECK, LTPG_PHASE_WRITEBACK};
+    for (uint32_t p : kPhases) {
+        if (p == LTPG_PHASE_CID && ctx->cfg.workload != LTPG_WORKLOAD_TPCC) continue;  // no key-value CID
+        int rc = ltpg_epoch_run(ctx, p);
+        if (rc) return rc;
+        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return ltpg_sync(ctx);
+}
+
+}  // namespace ltpg
+
+extern "C" {
+
 int ltpg_epoch_reset(ltpg_ctx *ctx)
 {
     int rc = check_ctx(ctx);

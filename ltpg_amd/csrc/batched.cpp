@@ -60,8 +60,10 @@ int slot_of(uint32_t workload) { return workload == LTPG_WORKLOAD_KV ? 0 : 1; }
 
 // One epoch of batch i on its context: submit (device-resident), all
 // phases, archive copy of the commit bitmap(s), and (captured runs) a device
-// clock stamp.  No synchronisation, so it can be captured.
-int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp)
+// clock stamp.  No synchronisation, so it can be captured -- unless
+// `stepwise`: then the phases run one at a time with a synchronisation after
+// each, as the reference's driver loop does (the plain loop).
+int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false)
 {
     const ltpg_batch_desc &d = b->d[i];
     ltpg_ctx *ctx = b->ctx[slot_of(d.workload)];
@@ -72,7 +74,7 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp)
     } else {
         if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
     }
-    if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+    if ((rc = stepwise ? run_epoch_stepwise(ctx) : ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
     if (d.workload == LTPG_WORKLOAD_KV) {
         LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i], ctx->kv.bitmap, words(d.kv.n_txns), ctx->stream));
     } else {
@@ -302,13 +304,12 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
     int rc = check_ctx(b->ctx[0] ? b->ctx[0] : b->ctx[1]);
     if (rc) return rc;
     if (mode == LTPG_RUN_PLAIN) {
-        // the reference's loop: per batch its phases, a synchronisation, the
-        // commit count back on the host
+        // the reference's loop: per batch its phases, each launch followed
+        // by a synchronisation, then the commit count back on the host
         const auto t0 = std::chrono::steady_clock::now();
         for (uint32_t i = 0; i < b->d.size(); i++) {
             ltpg_ctx *ctx = b->ctx[slot_of(b->d[i].workload)];
-            if ((rc = enqueue_batch(b, i, false))) return rc;
-            if ((rc = ltpg_sync(ctx))) return rc;
+            if ((rc = enqueue_batch(b, i, false, true))) return rc;
             uint32_t c = 0;
             if ((rc = ltpg_commit_count(ctx, b->d[i].workload == LTPG_WORKLOAD_KV ? LTPG_TYPE_KV
                                                                                   : LTPG_TYPE_PAYMENT, &c)))

@@ -261,6 +261,9 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
 hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s);
 hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
                                hipStream_t s);
+// The current batch's epoch phase by phase, a stream synchronisation after
+// each (the reference's driver loop), then ltpg_sync.
+int run_epoch_stepwise(ltpg_ctx *ctx);
 
 // Order-entry launchers (tpcc_kernels.hip).
 hipError_t launch_tpcc_populate(const TpccDev &d, uint64_t seed, hipStream_t s);

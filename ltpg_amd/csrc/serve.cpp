@@ -334,8 +334,8 @@ int launch(ltpg_server *sv, int w, Flight &f)
 // The plain per-batch form of one epoch, for comparison (ltpg_serve_config.
 // plain): the batch is packed on the host in the reference's layout
 // (slot-major key-value ops, value-major order-entry blocks), copied by the
-// submit, run, synchronised, and its bitmaps (and reads) copied back, one
-// step after the other.  Packing happens under the lock (the request store
+// submit, run phase by phase with a synchronisation after each, and its
+// bitmaps (and reads) copied back, one step after the other.  Packing happens under the lock (the request store
 // may grow); the device work runs without it.  On return the flight holds
 // a published epoch, as the served path leaves it.
 int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex> &lk)
@@ -361,8 +361,7 @@ int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex>
         lk.unlock();
         const ltpg_kv_batch b = {B, S, h_keys, h_meta, h_tids};
         rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_HOST);
-        if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
-        if (!rc) rc = ltpg_sync(ctx);
+        if (!rc) rc = run_epoch_stepwise(ctx);
         if (!rc) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_KV, f.h_bits, words(B));
         if (!rc && sv->cfg.keep_reads)
             rc = ltpg_kv_read_values(ctx, f.h_reads, (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words);
@@ -386,8 +385,7 @@ int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex>
         ltpg_neworder_view(f.h_no, (uint32_t)C, n_no, &nb);
         ltpg_payment_view(f.h_pay, (uint32_t)C, n_pay, &pb);
         rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_HOST);
-        if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
-        if (!rc) rc = ltpg_sync(ctx);
+        if (!rc) rc = run_epoch_stepwise(ctx);
         if (!rc && n_no) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_NEWORDER, f.h_bits, words(n_no));
         if (!rc && n_pay) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_PAYMENT, f.h_bits2, words(n_pay));
         if (!rc && sv->cfg.keep_reads && n_no)
