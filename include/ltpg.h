@@ -457,6 +457,10 @@ typedef struct ltpg_batch_desc {
     ltpg_kv_batch kv;               /* device pointers (workload KV)              */
     ltpg_neworder_batch neworder;   /* device pointers (workload TPCC)            */
     ltpg_payment_batch payment;
+    ltpg_orderstatus_batch orderstatus;  /* device pointers, n may be 0 (workload
+                                            TPCC: read-only transactions added to
+                                            the epoch, ltpg_tpcc_submit_readonly) */
+    ltpg_stocklevel_batch stocklevel;
 } ltpg_batch_desc;
 
 /* PLAIN: one epoch per batch, in list order, as the reference's driver loop

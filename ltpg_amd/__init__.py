@@ -593,7 +593,8 @@ class Batched:
 
     def bitmap(self, i, type_):
         d = self.descs[i]
-        n = {TYPE_KV: d.kv.n_txns, TYPE_NEWORDER: d.neworder.n, TYPE_PAYMENT: d.payment.n}[type_]
+        n = {TYPE_KV: d.kv.n_txns, TYPE_NEWORDER: d.neworder.n, TYPE_PAYMENT: d.payment.n,
+             TYPE_ORDERSTATUS: d.orderstatus.n, TYPE_STOCKLEVEL: d.stocklevel.n}[type_]
         out = np.zeros(max(1, (n + 63) // 64), dtype=np.uint64)
         _check(lib.ltpg_batched_bitmap(self.handle, i, type_, out.ctypes.data_as(_lib.c_u64p), out.size),
                "ltpg_batched_bitmap")
@@ -619,11 +620,15 @@ def kv_desc(batch):
     return d
 
 
-def tpcc_desc(dev):
-    """BatchDesc of an uploaded order-entry batch ((nb, pb) of upload_tpcc_batch)."""
+def tpcc_desc(dev, ro_dev=None):
+    """BatchDesc of an uploaded order-entry batch ((nb, pb) of
+    upload_tpcc_batch), with read-only transactions ((ob, sb) of
+    upload_readonly_batch) if given."""
     d = BatchDesc()
     d.workload = WORKLOAD_TPCC
     d.neworder, d.payment = dev
+    if ro_dev is not None:
+        d.orderstatus, d.stocklevel = ro_dev
     return d
 
 

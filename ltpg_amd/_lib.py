@@ -113,7 +113,8 @@ class BatchDesc(ctypes.Structure):
     """struct ltpg_batch_desc."""
 
     _fields_ = [("workload", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("kv", KvBatch),
-                ("neworder", NeworderBatch), ("payment", PaymentBatch)]
+                ("neworder", NeworderBatch), ("payment", PaymentBatch), ("orderstatus", OrderstatusBatch),
+                ("stocklevel", StocklevelBatch)]
 
 
 class TuningEntry(ctypes.Structure):

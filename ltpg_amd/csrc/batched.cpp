@@ -73,6 +73,9 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false
         if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
     } else {
         if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
+        if ((d.orderstatus.n || d.stocklevel.n) &&
+            (rc = ltpg_tpcc_submit_readonly(ctx, &d.orderstatus, &d.stocklevel, LTPG_MEM_DEVICE)))
+            return rc;
     }
     if ((rc = stepwise ? run_epoch_stepwise(ctx) : ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
     if (d.workload == LTPG_WORKLOAD_KV) {
@@ -80,8 +83,12 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false
     } else {
         LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i], ctx->tpcc.d.bitmap_no, words(d.neworder.n),
                                      ctx->stream));
-        LTPG_HIP_TRY(launch_copy_u64(b->archive + b->off[i] + words(d.neworder.n), ctx->tpcc.d.bitmap_pay,
-                                     words(d.payment.n), ctx->stream));
+        uint64_t at = b->off[i] + words(d.neworder.n);
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_pay, words(d.payment.n), ctx->stream));
+        at += words(d.payment.n);
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_os, words(d.orderstatus.n), ctx->stream));
+        at += words(d.orderstatus.n);
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_sl, words(d.stocklevel.n), ctx->stream));
     }
     if (stamp) LTPG_HIP_TRY(launch_stamp(b->stamps + i, ctx->stream));
     return LTPG_OK;
@@ -257,7 +264,7 @@ int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_d
             w += words(d.kv.n_txns);
         } else if (d.workload == LTPG_WORKLOAD_TPCC) {
             if (!tpcc_ctx) { ltpg_batched_destroy(b); LTPG_FAIL(LTPG_ERR_INVALID, "order-entry batch without tpcc_ctx"); }
-            w += words(d.neworder.n) + words(d.payment.n);
+            w += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n) + words(d.stocklevel.n);
         } else {
             ltpg_batched_destroy(b);
             LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload in batch descriptor");
@@ -451,6 +458,12 @@ int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t
     } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_PAYMENT) {
         off += words(d.neworder.n);
         n = words(d.payment.n);
+    } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_ORDERSTATUS) {
+        off += words(d.neworder.n) + words(d.payment.n);
+        n = words(d.orderstatus.n);
+    } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_STOCKLEVEL) {
+        off += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n);
+        n = words(d.stocklevel.n);
     } else {
         LTPG_FAIL(LTPG_ERR_INVALID, "type does not match the batch's workload");
     }

@@ -138,6 +138,59 @@ def test_graph_form_with_scans_matches_plain_loop_and_oracle(gpu):
     np.testing.assert_array_equal(out[L.RUN_GRAPH][1], table)
 
 
+def test_graph_form_with_readonly_matches_plain_loop_and_oracle(gpu):
+    """Order-entry batches carrying Order-Status / Stock-Level transactions
+    beside their NewOrders and Payments (priorities interleaved, half of the
+    readers on rows the NewOrders write): the graph gives the plain loop's
+    commit sets of all four types and its tables, and both the oracle's."""
+    from oracle import oracle
+
+    W = 2
+    rng = np.random.default_rng(17)
+    gen = L.TpccGenerator(warehouses=W, seed=23)
+    batches = []
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(1500))))
+        b = gen.next(n, tid_base=i * 10000)
+        ro = L.readonly_batch(b, W, n // 6 + 1, n // 6 + 1, seed=80 + i, tid_lo=i * 10000)
+        batches.append((b, ro))
+    types = (L.TYPE_NEWORDER, L.TYPE_PAYMENT, L.TYPE_ORDERSTATUS, L.TYPE_STOCKLEVEL)
+    out = {}
+    for mode in (L.RUN_PLAIN, L.RUN_GRAPH):
+        tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=4096)
+        tp.tpcc_populate(SEED)
+        keep, descs = [], []
+        for b, ro in batches:
+            dev, bufs = L.upload_tpcc_batch(tp, b)
+            rdev, rbufs = L.upload_readonly_batch(tp, ro)
+            keep += bufs + rbufs
+            descs.append(L.tpcc_desc(dev, rdev))
+        r = L.Batched(None, tp, descs)
+        r.run(mode)
+        out[mode] = ([[r.bitmap(i, t) for t in types] for i in range(len(batches))],
+                     [tp.tpcc_table_read(t) for t in range(9)])
+        r.close()
+    for a, b in zip(out[L.RUN_PLAIN][0], out[L.RUN_GRAPH][0]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    for x, y in zip(out[L.RUN_PLAIN][1], out[L.RUN_GRAPH][1]):
+        np.testing.assert_array_equal(x, y)
+    tables = oracle.tpcc_populate(W, SEED)
+    ora = oracle.TpccOracle(W)
+    n_ro_commits = 0
+    for i, (b, ro) in enumerate(batches):
+        ep = ora.epoch(tables, b, ro=ro)
+        got = out[L.RUN_GRAPH][0][i]
+        np.testing.assert_array_equal(got[0], ep.no_bitmap)
+        np.testing.assert_array_equal(got[1], ep.pay_bitmap)
+        np.testing.assert_array_equal(got[2], ep.ro["os_bitmap"])
+        np.testing.assert_array_equal(got[3], ep.ro["sl_bitmap"])
+        n_ro_commits += int(sum(bin(int(w)).count("1") for w in got[2]))
+    assert n_ro_commits > 0
+    for t in range(9):
+        np.testing.assert_array_equal(out[L.RUN_GRAPH][1][t], tables[t])
+
+
 def test_plain_epochs_after_a_graph_run(gpu):
     """A context used by a graph run keeps working epoch by epoch."""
     kv, tp = _contexts()
