@@ -552,8 +552,8 @@ typedef struct ltpg_serve_done {
 } ltpg_serve_done;
 
 typedef struct ltpg_serve_stats {
-    uint64_t committed[3];    /* by type                                        */
-    uint64_t attempts[3];     /* transaction-epochs by type                     */
+    uint64_t committed[5];    /* by type (LTPG_TYPE_KV .. LTPG_TYPE_STOCKLEVEL)  */
+    uint64_t attempts[5];     /* transaction-epochs by type                     */
     uint64_t epochs[2];       /* key-value, order-entry                         */
     uint64_t max_epoch_txns[2];
 } ltpg_serve_stats;
@@ -574,12 +574,21 @@ int ltpg_serve_submit_kv(ltpg_server *sv, uint32_t n, const uint64_t *keys, cons
  * first_id .. first_id + n_no - 1, then the Payments. */
 int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *neworder,
                            const ltpg_payment_batch *payment, uint64_t *first_id);
+/* Read-only order-entry requests (Order-Status, Stock-Level; host arrays,
+ * TID fields ignored), which join the order-entry epochs after their
+ * NewOrders and Payments as ltpg_tpcc_submit_readonly adds them.  Ids
+ * first_id .. for the Order-Status requests, then the Stock-Level ones.
+ * LTPG_ERR_UNSUPPORTED for a context with LTPG_FLAG_CONFLICT_COUNTS.  No
+ * counterpart in the reference, whose read-only transactions are unfed
+ * stubs (TPCC-alltxn/Execute.h:1592-1770,2155-2231). */
+int ltpg_serve_submit_readonly(ltpg_server *sv, const ltpg_orderstatus_batch *orderstatus,
+                               const ltpg_stocklevel_batch *stocklevel, uint64_t *first_id);
 /* Up to max completed (committed) requests, oldest first. */
 int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out);
 /* Read values of a committed request (keep_reads): key-value: the field
  * words of its read ops, and of each scan's rows inside the table, in op
- * order; NewOrder 51 / Payment 24 values as
- * ltpg_tpcc_read_values lays them out for one transaction. */
+ * order; NewOrder 51 / Payment 24 / Order-Status 3 / Stock-Level 1 values
+ * as ltpg_tpcc_read_values lays them out for one transaction. */
 int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words);
 /* Request ids of serving epoch e of a workload, in batch order (record_epochs). */
 int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *ids, uint32_t max, uint32_t *n);

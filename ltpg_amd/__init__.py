@@ -668,6 +668,14 @@ class Server:
                "ltpg_serve_submit_tpcc")
         return first.value
 
+    def submit_readonly(self, ro):
+        """Requests from a ReadOnlyBatch (Order-Status get the first ids, then Stock-Level)."""
+        ob, sb = ro.structs()
+        first = ctypes.c_uint64()
+        _check(lib.ltpg_serve_submit_readonly(self.handle, ctypes.byref(ob), ctypes.byref(sb), ctypes.byref(first)),
+               "ltpg_serve_submit_readonly")
+        return first.value
+
     def poll(self, max_n=1 << 16):
         out = (ServeDone * max_n)()
         n = ctypes.c_uint32()

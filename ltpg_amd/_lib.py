@@ -143,7 +143,7 @@ class ServeDone(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     """struct ltpg_serve_stats."""
 
-    _fields_ = [("committed", ctypes.c_uint64 * 3), ("attempts", ctypes.c_uint64 * 3), ("epochs", ctypes.c_uint64 * 2),
+    _fields_ = [("committed", ctypes.c_uint64 * 5), ("attempts", ctypes.c_uint64 * 5), ("epochs", ctypes.c_uint64 * 2),
                 ("max_epoch_txns", ctypes.c_uint64 * 2)]
 
 
@@ -214,6 +214,8 @@ SIGNATURES = [
     ("ltpg_serve_submit_kv", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u64p, c_u32p, c_u64p]),
     ("ltpg_serve_submit_tpcc", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), c_u64p]),
+    ("ltpg_serve_submit_readonly", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(OrderstatusBatch), ctypes.POINTER(StocklevelBatch), c_u64p]),
     ("ltpg_serve_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeDone), ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32p, ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_epoch", ctypes.c_int,

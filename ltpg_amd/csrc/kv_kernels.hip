@@ -212,12 +212,10 @@ __global__ void copy_u32_kernel(uint32_t *__restrict__ dst, const uint32_t *__re
 // Publish an epoch's commit bitmaps to pinned host memory, then its sequence
 // number with a system-scope release: a host that reads the sequence number
 // with an acquire load also reads the bitmaps.  One block.
-__global__ void publish_kernel(const uint64_t *__restrict__ a, uint64_t *__restrict__ ha, uint32_t na,
-                               const uint64_t *__restrict__ b, uint64_t *__restrict__ hb, uint32_t nb,
-                               uint64_t *seq_out, uint64_t seq)
+__global__ void publish_kernel(PublishSet p, uint64_t *seq_out, uint64_t seq)
 {
-    for (uint32_t w = threadIdx.x; w < na; w += blockDim.x) ha[w] = a[w];
-    for (uint32_t w = threadIdx.x; w < nb; w += blockDim.x) hb[w] = b[w];
+    for (int k = 0; k < 4; k++)
+        for (uint32_t w = threadIdx.x; w < p.n[k]; w += blockDim.x) p.dst[k][w] = p.src[k][w];
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1015,10 +1013,9 @@ hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_publish(const uint64_t *a, uint64_t *ha, uint32_t na, const uint64_t *b, uint64_t *hb,
-                          uint32_t nb, uint64_t *seq_out, uint64_t seq, hipStream_t s)
+hipError_t launch_publish(const PublishSet &p, uint64_t *seq_out, uint64_t seq, hipStream_t s)
 {
-    hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(kBlock), 0, s, a, ha, na, b, hb, nb, seq_out, seq);
+    hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(kBlock), 0, s, p, seq_out, seq);
     return hipGetLastError();
 }
 

@@ -239,10 +239,16 @@ hipError_t launch_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t s);
 hipError_t launch_copy_u64(uint64_t *dst, const uint64_t *src, uint64_t n, hipStream_t s);
 // n u32 words; src may be pinned host memory
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipStream_t s);
-// copy na (nb) bitmap words to pinned host memory, then store seq to
-// *seq_out (pinned host memory) with a system-scope release
-hipError_t launch_publish(const uint64_t *a, uint64_t *ha, uint32_t na, const uint64_t *b, uint64_t *hb,
-                          uint32_t nb, uint64_t *seq_out, uint64_t seq, hipStream_t s);
+// Up to four bitmaps to copy to pinned host memory (n[k] words from src[k]
+// to dst[k]; n[k] = 0 for unused entries).
+struct PublishSet {
+    const uint64_t *src[4];
+    uint64_t *dst[4];
+    uint32_t n[4];
+};
+// copy the bitmaps of p, then store seq to *seq_out (pinned host memory)
+// with a system-scope release
+hipError_t launch_publish(const PublishSet &p, uint64_t *seq_out, uint64_t seq, hipStream_t s);
 hipError_t launch_stamp(unsigned long long *out, hipStream_t s);
 
 // Registers back to "no registrant" (all-ones), doomed flags and NEWORDER

@@ -57,6 +57,8 @@ uint64_t words(uint64_t n) { return (n + 63) / 64; }
 
 constexpr uint32_t kNoFields = 52;  // NewOrder request: 7 head fields + 3 x 15 line fields
 constexpr uint32_t kPayFields = 9;
+constexpr uint32_t kOsFields = 5;    // Order-Status: W_ID, D_ID, C_ID, O_ID, OL_ID
+constexpr uint32_t kSlFields = 2;    // Stock-Level: W_ID, S_ID
 constexpr uint64_t kAdaptiveFloor = 256;  // smallest epoch the commit-based cap allows
 
 struct Req {
@@ -99,7 +101,7 @@ struct Flight {
     bool busy = false;
     uint64_t epoch = 0;
     std::vector<uint64_t> members;  // request indices, batch order (NewOrders, then Payments)
-    uint32_t n_no = 0, n_pay = 0;
+    uint32_t n_no = 0, n_pay = 0, n_os = 0, n_sl = 0;  // order-entry members by type, in this order
     hipEvent_t copied = nullptr;     // order-entry staging copied (copy stream)
     uint64_t *h_seq = nullptr;       // pinned: sequence number of the last published epoch
     uint64_t seq = 0;                // sequence number of the epoch in flight
@@ -110,9 +112,10 @@ struct Flight {
     uint8_t *h_stage = nullptr, *d_stage = nullptr;
     // order-entry staging (pinned, packed query blocks) and device slot
     uint32_t *h_no = nullptr, *d_no = nullptr, *h_pay = nullptr, *d_pay = nullptr;
+    uint32_t *h_os = nullptr, *d_os = nullptr, *h_sl = nullptr, *d_sl = nullptr;  // read-only, value-major + TID
     // results (pinned)
-    uint64_t *h_bits = nullptr, *h_bits2 = nullptr;
-    uint32_t *h_reads = nullptr, *h_reads2 = nullptr;
+    uint64_t *h_bits = nullptr, *h_bits2 = nullptr, *h_bits3 = nullptr, *h_bits4 = nullptr;
+    uint32_t *h_reads = nullptr, *h_reads2 = nullptr, *h_reads3 = nullptr, *h_reads4 = nullptr;
 };
 
 struct Lane {                        // one workload
@@ -139,13 +142,15 @@ struct ltpg_server {
     std::vector<uint32_t> kv_meta;
     std::vector<uint32_t> no_data;   // kNoFields per NewOrder
     std::vector<uint32_t> pay_data;  // kPayFields per Payment
+    std::vector<uint32_t> os_data;   // kOsFields per Order-Status
+    std::vector<uint32_t> sl_data;   // kSlFields per Stock-Level
     Lane lane[2];                    // [0] key-value, [1] order-entry
     std::mutex mu;
     std::atomic<int> waiting{0};     // API calls waiting for mu (the loop yields to them)
     std::condition_variable cv;
     std::deque<ltpg_serve_done> done;
     std::unordered_map<uint64_t, std::vector<uint32_t>> reads;
-    uint64_t committed[3] = {0, 0, 0}, attempts[3] = {0, 0, 0};
+    uint64_t committed[kCountedTypes] = {}, attempts[kCountedTypes] = {};
     uint64_t outstanding = 0;        // submitted, not yet committed
     uint64_t dropped = 0;            // dropped by ltpg_serve_cancel
     bool stop = false;
@@ -181,6 +186,14 @@ void free_flight(Flight &f)
     host_free(f.h_stage);
     host_free(f.h_no);
     host_free(f.h_pay);
+    host_free(f.h_os);
+    host_free(f.h_sl);
+    host_free(f.h_bits3);
+    host_free(f.h_bits4);
+    host_free(f.h_reads3);
+    host_free(f.h_reads4);
+    dev_free(f.d_os);
+    dev_free(f.d_sl);
     host_free(f.h_bits);
     host_free(f.h_bits2);
     host_free(f.h_reads);
@@ -206,16 +219,66 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
         if ((rc = dev_alloc(&f.d_stage, stage, "serving batch slot"))) return rc;
     } else {
         if ((rc = host_alloc(&f.h_no, 53 * C)) || (rc = host_alloc(&f.h_pay, 10 * C)) ||
-            (rc = host_alloc(&f.h_bits, words(C), true)) || (rc = host_alloc(&f.h_bits2, words(C), true)))
+            (rc = host_alloc(&f.h_os, (kOsFields + 1) * C)) || (rc = host_alloc(&f.h_sl, (kSlFields + 1) * C)) ||
+            (rc = host_alloc(&f.h_bits, words(C), true)) || (rc = host_alloc(&f.h_bits2, words(C), true)) ||
+            (rc = host_alloc(&f.h_bits3, words(C), true)) || (rc = host_alloc(&f.h_bits4, words(C), true)))
             return rc;
         if (sv->cfg.keep_reads &&
-            ((rc = host_alloc(&f.h_reads, (uint64_t)kNoReads * C)) || (rc = host_alloc(&f.h_reads2, (uint64_t)kPayReads * C))))
+            ((rc = host_alloc(&f.h_reads, (uint64_t)kNoReads * C)) || (rc = host_alloc(&f.h_reads2, (uint64_t)kPayReads * C)) ||
+             (rc = host_alloc(&f.h_reads3, (uint64_t)kOsReads * C)) || (rc = host_alloc(&f.h_reads4, (uint64_t)kSlReads * C))))
             return rc;
         if ((rc = dev_alloc(&f.d_no, 53 * C, "serving NewOrder slot")) ||
-            (rc = dev_alloc(&f.d_pay, 10 * C, "serving Payment slot")))
+            (rc = dev_alloc(&f.d_pay, 10 * C, "serving Payment slot")) ||
+            (rc = dev_alloc(&f.d_os, (kOsFields + 1) * C, "serving Order-Status slot")) ||
+            (rc = dev_alloc(&f.d_sl, (kSlFields + 1) * C, "serving Stock-Level slot")))
             return rc;
     }
     return LTPG_OK;
+}
+
+// Position of an order-entry type in an epoch's batch.
+uint32_t type_rank(uint32_t type)
+{
+    return type == LTPG_TYPE_NEWORDER ? 0 : type == LTPG_TYPE_PAYMENT ? 1 : type == LTPG_TYPE_ORDERSTATUS ? 2 : 3;
+}
+
+// Pack an order-entry epoch's members into the flight's pinned blocks
+// (lock held): value-major columns of stride C, TIDs in the last column.
+void pack_tpcc(ltpg_server *sv, Flight &f, uint64_t C)
+{
+    uint32_t at = 0;
+    for (uint32_t q = 0; q < f.n_no; q++, at++) {
+        const Req &r = sv->reqs[f.members[at]];
+        const uint32_t *src = &sv->no_data[r.data * kNoFields];
+        for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
+        f.h_no[52 * C + q] = r.tid;
+    }
+    for (uint32_t p = 0; p < f.n_pay; p++, at++) {
+        const Req &r = sv->reqs[f.members[at]];
+        const uint32_t *src = &sv->pay_data[r.data * kPayFields];
+        for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
+        f.h_pay[9 * C + p] = r.tid;
+    }
+    for (uint32_t o = 0; o < f.n_os; o++, at++) {
+        const Req &r = sv->reqs[f.members[at]];
+        const uint32_t *src = &sv->os_data[r.data * kOsFields];
+        for (uint32_t k = 0; k < kOsFields; k++) f.h_os[(uint64_t)k * C + o] = src[k];
+        f.h_os[kOsFields * C + o] = r.tid;
+    }
+    for (uint32_t l = 0; l < f.n_sl; l++, at++) {
+        const Req &r = sv->reqs[f.members[at]];
+        const uint32_t *src = &sv->sl_data[r.data * kSlFields];
+        for (uint32_t k = 0; k < kSlFields; k++) f.h_sl[(uint64_t)k * C + l] = src[k];
+        f.h_sl[kSlFields * C + l] = r.tid;
+    }
+}
+
+// Read-only batches over value-major columns of stride C (host or device).
+void readonly_views(uint32_t *os, uint32_t *sl, uint64_t C, uint32_t n_os, uint32_t n_sl,
+                    ltpg_orderstatus_batch *ob, ltpg_stocklevel_batch *sb)
+{
+    *ob = ltpg_orderstatus_batch{n_os, 0, os, os + C, os + 2 * C, os + 3 * C, os + 4 * C, os + kOsFields * C};
+    *sb = ltpg_stocklevel_batch{n_sl, 0, sl, sl + C, sl + kSlFields * C};
 }
 
 // Take the next epoch's members (lock held): re-queued first, by TID, then
@@ -239,12 +302,17 @@ void take_members(ltpg_server *sv, int w, Flight &f)
         f.members.push_back(L.fresh.front());
         L.fresh.pop_front();
     }
-    if (w == 1) {  // NewOrders first, then Payments, each in member order
-        std::stable_partition(f.members.begin(), f.members.end(),
-                              [&](uint64_t r) { return sv->reqs[r].type == LTPG_TYPE_NEWORDER; });
-        f.n_no = 0;
-        for (uint64_t r : f.members) f.n_no += sv->reqs[r].type == LTPG_TYPE_NEWORDER;
-        f.n_pay = (uint32_t)f.members.size() - f.n_no;
+    if (w == 1) {  // NewOrders, Payments, Order-Status, Stock-Level, each in member order
+        std::stable_sort(f.members.begin(), f.members.end(),
+                         [&](uint64_t a, uint64_t b) { return type_rank(sv->reqs[a].type) < type_rank(sv->reqs[b].type); });
+        f.n_no = f.n_pay = f.n_os = f.n_sl = 0;
+        for (uint64_t r : f.members) {
+            const uint32_t ty = sv->reqs[r].type;
+            f.n_no += ty == LTPG_TYPE_NEWORDER;
+            f.n_pay += ty == LTPG_TYPE_PAYMENT;
+            f.n_os += ty == LTPG_TYPE_ORDERSTATUS;
+            f.n_sl += ty == LTPG_TYPE_STOCKLEVEL;
+        }
     }
 }
 
@@ -285,27 +353,23 @@ int launch(ltpg_server *sv, int w, Flight &f)
             LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values,
                                         (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words * 4,
                                         hipMemcpyDeviceToHost, ctx->stream));
-        LTPG_HIP_TRY(launch_publish(ctx->kv.bitmap, f.h_bits, (uint32_t)words(B), nullptr, nullptr, 0, f.h_seq,
-                                    ++f.seq, ctx->stream));
+        PublishSet ps{{ctx->kv.bitmap, nullptr, nullptr, nullptr}, {f.h_bits, nullptr, nullptr, nullptr},
+                      {(uint32_t)words(B), 0, 0, 0}};
+        LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
     } else {
-        const uint32_t n_no = f.n_no, n_pay = f.n_pay;
-        for (uint32_t q = 0; q < n_no; q++) {
-            const Req &r = sv->reqs[f.members[q]];
-            const uint32_t *src = &sv->no_data[r.data * kNoFields];
-            for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
-            f.h_no[52 * C + q] = r.tid;
-        }
-        for (uint32_t p = 0; p < n_pay; p++) {
-            const Req &r = sv->reqs[f.members[n_no + p]];
-            const uint32_t *src = &sv->pay_data[r.data * kPayFields];
-            for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
-            f.h_pay[9 * C + p] = r.tid;
-        }
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl;
+        pack_tpcc(sv, f, C);
         if (n_no)
             LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_no, C * 4, f.h_no, C * 4, (uint64_t)n_no * 4, 53,
                                           hipMemcpyHostToDevice, L.copy));
         if (n_pay)
             LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_pay, C * 4, f.h_pay, C * 4, (uint64_t)n_pay * 4, 10,
+                                          hipMemcpyHostToDevice, L.copy));
+        if (n_os)
+            LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_os, C * 4, f.h_os, C * 4, (uint64_t)n_os * 4, kOsFields + 1,
+                                          hipMemcpyHostToDevice, L.copy));
+        if (n_sl)
+            LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_sl, C * 4, f.h_sl, C * 4, (uint64_t)n_sl * 4, kSlFields + 1,
                                           hipMemcpyHostToDevice, L.copy));
         LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
         LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
@@ -314,17 +378,28 @@ int launch(ltpg_server *sv, int w, Flight &f)
         ltpg_neworder_view(f.d_no, (uint32_t)C, n_no, &nb);
         ltpg_payment_view(f.d_pay, (uint32_t)C, n_pay, &pb);
         if ((rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_DEVICE))) return rc;
+        if (n_os || n_sl) {
+            ltpg_orderstatus_batch ob;
+            ltpg_stocklevel_batch sb;
+            readonly_views(f.d_os, f.d_sl, C, n_os, n_sl, &ob, &sb);
+            if ((rc = ltpg_tpcc_submit_readonly(ctx, &ob, &sb, LTPG_MEM_DEVICE))) return rc;
+        }
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
         if (sv->cfg.keep_reads) {
-            if (n_no)
-                LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->tpcc.d.reads_no, (uint64_t)kNoReads * n_no * 4,
-                                            hipMemcpyDeviceToHost, ctx->stream));
-            if (n_pay)
-                LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads2, ctx->tpcc.d.reads_pay, (uint64_t)kPayReads * n_pay * 4,
-                                            hipMemcpyDeviceToHost, ctx->stream));
+            const std::pair<uint32_t *, const uint32_t *> reads[4] = {
+                {f.h_reads, ctx->tpcc.d.reads_no}, {f.h_reads2, ctx->tpcc.d.reads_pay},
+                {f.h_reads3, ctx->tpcc.d.reads_os}, {f.h_reads4, ctx->tpcc.d.reads_sl}};
+            const uint64_t n_words[4] = {(uint64_t)kNoReads * n_no, (uint64_t)kPayReads * n_pay,
+                                         (uint64_t)kOsReads * n_os, (uint64_t)kSlReads * n_sl};
+            for (int k = 0; k < 4; k++)
+                if (n_words[k])
+                    LTPG_HIP_TRY(hipMemcpyAsync(reads[k].first, reads[k].second, n_words[k] * 4,
+                                                hipMemcpyDeviceToHost, ctx->stream));
         }
-        LTPG_HIP_TRY(launch_publish(ctx->tpcc.d.bitmap_no, f.h_bits, (uint32_t)words(n_no), ctx->tpcc.d.bitmap_pay,
-                                    f.h_bits2, (uint32_t)words(n_pay), f.h_seq, ++f.seq, ctx->stream));
+        PublishSet ps{{ctx->tpcc.d.bitmap_no, ctx->tpcc.d.bitmap_pay, ctx->tpcc.d.bitmap_os, ctx->tpcc.d.bitmap_sl},
+                      {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4},
+                      {(uint32_t)words(n_no), (uint32_t)words(n_pay), (uint32_t)words(n_os), (uint32_t)words(n_sl)}};
+        LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
     }
     f.launched_ns = now_ns();
     f.busy = true;
@@ -366,32 +441,32 @@ int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex>
         if (!rc && sv->cfg.keep_reads)
             rc = ltpg_kv_read_values(ctx, f.h_reads, (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words);
     } else {
-        const uint32_t n_no = f.n_no, n_pay = f.n_pay;
-        for (uint32_t q = 0; q < n_no; q++) {
-            const Req &r = sv->reqs[f.members[q]];
-            const uint32_t *src = &sv->no_data[r.data * kNoFields];
-            for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
-            f.h_no[52 * C + q] = r.tid;
-        }
-        for (uint32_t p = 0; p < n_pay; p++) {
-            const Req &r = sv->reqs[f.members[n_no + p]];
-            const uint32_t *src = &sv->pay_data[r.data * kPayFields];
-            for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
-            f.h_pay[9 * C + p] = r.tid;
-        }
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl;
+        pack_tpcc(sv, f, C);
         lk.unlock();
         ltpg_neworder_batch nb;
         ltpg_payment_batch pb;
         ltpg_neworder_view(f.h_no, (uint32_t)C, n_no, &nb);
         ltpg_payment_view(f.h_pay, (uint32_t)C, n_pay, &pb);
         rc = ltpg_tpcc_submit(ctx, &nb, &pb, LTPG_MEM_HOST);
+        if (!rc && (n_os || n_sl)) {
+            ltpg_orderstatus_batch ob;
+            ltpg_stocklevel_batch sb;
+            readonly_views(f.h_os, f.h_sl, C, n_os, n_sl, &ob, &sb);
+            rc = ltpg_tpcc_submit_readonly(ctx, &ob, &sb, LTPG_MEM_HOST);
+        }
         if (!rc) rc = run_epoch_stepwise(ctx);
-        if (!rc && n_no) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_NEWORDER, f.h_bits, words(n_no));
-        if (!rc && n_pay) rc = ltpg_commit_bitmap(ctx, LTPG_TYPE_PAYMENT, f.h_bits2, words(n_pay));
-        if (!rc && sv->cfg.keep_reads && n_no)
-            rc = ltpg_tpcc_read_values(ctx, LTPG_TYPE_NEWORDER, f.h_reads, (uint64_t)kNoReads * n_no);
-        if (!rc && sv->cfg.keep_reads && n_pay)
-            rc = ltpg_tpcc_read_values(ctx, LTPG_TYPE_PAYMENT, f.h_reads2, (uint64_t)kPayReads * n_pay);
+        const uint32_t types[4] = {LTPG_TYPE_NEWORDER, LTPG_TYPE_PAYMENT, LTPG_TYPE_ORDERSTATUS, LTPG_TYPE_STOCKLEVEL};
+        const uint32_t counts[4] = {n_no, n_pay, n_os, n_sl};
+        const uint32_t per[4] = {kNoReads, kPayReads, kOsReads, kSlReads};
+        uint64_t *bits[4] = {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4};
+        uint32_t *reads[4] = {f.h_reads, f.h_reads2, f.h_reads3, f.h_reads4};
+        for (int k = 0; k < 4 && !rc; k++) {
+            if (!counts[k]) continue;
+            rc = ltpg_commit_bitmap(ctx, types[k], bits[k], words(counts[k]));
+            if (!rc && sv->cfg.keep_reads)
+                rc = ltpg_tpcc_read_values(ctx, types[k], reads[k], (uint64_t)per[k] * counts[k]);
+        }
     }
     lk.lock();
     if (rc) return rc;
@@ -417,9 +492,13 @@ void complete(ltpg_server *sv, int w, Flight &f)
         r.attempts++;
         sv->attempts[r.type]++;
         bool ok;
+        // order-entry members: NewOrders, Payments, Order-Status, Stock-Level
+        const uint32_t c1 = f.n_no, c2 = c1 + f.n_pay, c3 = c2 + f.n_os;
         if (w == 0) ok = bit(f.h_bits, t);
-        else if (t < f.n_no) ok = bit(f.h_bits, t);
-        else ok = bit(f.h_bits2, t - f.n_no);
+        else if (t < c1) ok = bit(f.h_bits, t);
+        else if (t < c2) ok = bit(f.h_bits2, t - c1);
+        else if (t < c3) ok = bit(f.h_bits3, t - c2);
+        else ok = bit(f.h_bits4, t - c3);
         if (!ok) {
             L.requeue.push(ri);
             continue;
@@ -455,11 +534,17 @@ void complete(ltpg_server *sv, int w, Flight &f)
                     const uint32_t *src = f.h_reads + (((uint64_t)s * B) + t) * kv.read_slot * fw;
                     v.insert(v.end(), src, src + n * fw);
                 }
-            } else if (t < f.n_no) {
+            } else if (t < c1) {
                 for (uint32_t k = 0; k < kNoReads; k++) v.push_back(f.h_reads[(uint64_t)k * f.n_no + t]);
-            } else {
-                const uint32_t p = t - f.n_no;
+            } else if (t < c2) {
+                const uint32_t p = t - c1;
                 for (uint32_t k = 0; k < kPayReads; k++) v.push_back(f.h_reads2[(uint64_t)k * f.n_pay + p]);
+            } else if (t < c3) {
+                const uint32_t o = t - c2;
+                for (uint32_t k = 0; k < kOsReads; k++) v.push_back(f.h_reads3[(uint64_t)k * f.n_os + o]);
+            } else {
+                const uint32_t l = t - c3;
+                for (uint32_t k = 0; k < kSlReads; k++) v.push_back(f.h_reads4[(uint64_t)k * f.n_sl + l]);
             }
             sv->reads[r.id] = std::move(v);
         }
@@ -738,6 +823,37 @@ int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *no, const
     return LTPG_OK;
 }
 
+int ltpg_serve_submit_readonly(ltpg_server *sv, const ltpg_orderstatus_batch *os, const ltpg_stocklevel_batch *sl,
+                               uint64_t *first_id)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    if (!sv->lane[1].ctx) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server has no order-entry context");
+    if (sv->lane[1].ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "read-only transactions need the fast check (no conflict counts)");
+    const uint32_t n_os = os ? os->n : 0, n_sl = sl ? sl->n : 0;
+    if (n_os && (!os->W_ID || !os->D_ID || !os->C_ID || !os->O_ID || !os->OL_ID))
+        LTPG_FAIL(LTPG_ERR_INVALID, "bad Order-Status arrays");
+    if (n_sl && (!sl->W_ID || !sl->S_ID)) LTPG_FAIL(LTPG_ERR_INVALID, "bad Stock-Level arrays");
+    const uint64_t t = now_ns();
+    ClientLock g(sv);
+    if (first_id) *first_id = sv->reqs.size();
+    Lane &L = sv->lane[1];
+    for (uint32_t o = 0; o < n_os; o++) {
+        const uint64_t data = sv->os_data.size() / kOsFields;
+        for (const uint32_t *c : {os->W_ID, os->D_ID, os->C_ID, os->O_ID, os->OL_ID}) sv->os_data.push_back(c[o]);
+        L.fresh.push_back(sv->reqs.size());
+        sv->reqs.push_back(Req{sv->reqs.size(), LTPG_TYPE_ORDERSTATUS, L.next_tid++, 0, t, data});
+    }
+    for (uint32_t l = 0; l < n_sl; l++) {
+        const uint64_t data = sv->sl_data.size() / kSlFields;
+        for (const uint32_t *c : {sl->W_ID, sl->S_ID}) sv->sl_data.push_back(c[l]);
+        L.fresh.push_back(sv->reqs.size());
+        sv->reqs.push_back(Req{sv->reqs.size(), LTPG_TYPE_STOCKLEVEL, L.next_tid++, 0, t, data});
+    }
+    sv->outstanding += n_os + n_sl;
+    return LTPG_OK;
+}
+
 int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out)
 {
     if (!sv || (!out && max)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
@@ -792,7 +908,7 @@ int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st)
 {
     if (!sv || !st) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
     ClientLock g(sv);
-    for (int k = 0; k < 3; k++) {
+    for (uint32_t k = 0; k < kCountedTypes; k++) {
         st->committed[k] = sv->committed[k];
         st->attempts[k] = sv->attempts[k];
     }

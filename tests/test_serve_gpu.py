@@ -233,6 +233,88 @@ def test_served_order_entry_epochs_replay_on_the_oracle(gpu, plain):
         np.testing.assert_array_equal(tp.tpcc_table_read(t), tables[t])
 
 
+@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
+def test_served_readonly_requests_replay_on_the_oracle(gpu, plain):
+    """Order-Status and Stock-Level requests beside NewOrders and Payments
+    (half of them on rows the NewOrders write): the server's epochs, each
+    NewOrders, Payments, Order-Status, Stock-Level in member order, replayed
+    through the oracle reproduce every decision, read value and table."""
+    W = 2
+    tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=2048)
+    tp.tpcc_populate(SEED)
+    gen = L.TpccGenerator(warehouses=W, seed=31)
+    sv = L.Server(None, tp, slots=2, max_epoch_txns=800, keep_reads=1, record_epochs=1, plain=plain)
+    index = {}  # request id -> (kind, source, position)
+    srcs = []
+    for bi in range(3):
+        b = gen.next(900)
+        ro = L.readonly_batch(b, W, 150, 150, seed=60 + bi)
+        srcs.append((b, ro))
+        first = sv.submit_tpcc(b)
+        for q in range(b.n_no):
+            index[first + q] = ("no", bi, q)
+        for p in range(b.n_pay):
+            index[first + b.n_no + p] = ("pay", bi, p)
+        first = sv.submit_readonly(ro)
+        for o in range(ro.n_os):
+            index[first + o] = ("os", bi, o)
+        for l in range(ro.n_sl):
+            index[first + ro.n_os + l] = ("sl", bi, l)
+        time.sleep(0.002)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_TPCC, e) for e in range(st["epochs"][1])]
+    reads = {rid: sv.read_values(rid) for rid in done}
+    sv.stop()
+    assert len(done) == len(index)
+    assert st["committed"][L.TYPE_ORDERSTATUS] == 450 and st["committed"][L.TYPE_STOCKLEVEL] == 450
+    tables = oracle.tpcc_populate(W, SEED)
+    ora = oracle.TpccOracle(W)
+    for e, ids in enumerate(epochs):
+        groups = {k: [int(i) for i in ids if index[int(i)][0] == k] for k in ("no", "pay", "os", "sl")}
+        # the server's batch order: NewOrders, Payments, Order-Status, Stock-Level
+        assert [int(i) for i in ids] == groups["no"] + groups["pay"] + groups["os"] + groups["sl"]
+        eb = L.TpccBatch.empty(len(groups["no"]), len(groups["pay"]))
+        for q, rid in enumerate(groups["no"]):
+            _, bi, pos = index[rid]
+            src = srcs[bi][0]
+            for f in ("W_ID", "D_ID", "C_ID", "O_ID", "N_O_ID", "O_OL_CNT", "O_OL_ID"):
+                eb.nf(f)[q] = src.nf(f)[pos]
+            for f in ("OL_I_ID", "OL_SUPPLY_W_ID", "OL_QUANTITY"):
+                for l in range(15):
+                    eb.nf(f, l)[q] = src.nf(f, l)[pos]
+            eb.nf("TID")[q] = done[rid]["tid"]
+        for p, rid in enumerate(groups["pay"]):
+            _, bi, pos = index[rid]
+            src = srcs[bi][0]
+            for f in ("W_ID", "D_ID", "C_ID", "C_LAST", "isName", "C_D_ID", "C_W_ID", "H_AMOUNT", "H_ID"):
+                eb.pf(f)[p] = src.pf(f)[pos]
+            eb.pf("TID")[p] = done[rid]["tid"]
+        er = L.ReadOnlyBatch(len(groups["os"]), len(groups["sl"]))
+        for o, rid in enumerate(groups["os"]):
+            _, bi, pos = index[rid]
+            for f in L.ORDERSTATUS_FIELDS:
+                er.os[f][o] = done[rid]["tid"] if f == "TID" else srcs[bi][1].os[f][pos]
+        for l, rid in enumerate(groups["sl"]):
+            _, bi, pos = index[rid]
+            for f in L.STOCKLEVEL_FIELDS:
+                er.sl[f][l] = done[rid]["tid"] if f == "TID" else srcs[bi][1].sl[f][pos]
+        ep = ora.epoch(tables, eb, ro=er)
+        decided = [("no", ep.no_bitmap, ep.no_reads, 51), ("pay", ep.pay_bitmap, ep.pay_reads, 24),
+                   ("os", ep.ro["os_bitmap"], ep.ro["os_reads"], L.ORDERSTATUS_READS),
+                   ("sl", ep.ro["sl_bitmap"], ep.ro["sl_reads"], L.STOCKLEVEL_READS)]
+        for kind, bitmap, ora_reads, per in decided:
+            n = len(groups[kind])
+            for j, rid in enumerate(groups[kind]):
+                c = bool((int(bitmap[j // 64]) >> (j % 64)) & 1)
+                assert c == (done[rid]["epoch"] == e), f"{kind} request {rid} epoch {e}"
+                if c:
+                    np.testing.assert_array_equal(reads[rid], np.asarray(ora_reads).reshape(per, n)[:, j])
+    for t in range(9):
+        np.testing.assert_array_equal(tp.tpcc_table_read(t), tables[t])
+
+
 def test_hbm_budget_bounds_epochs(gpu):
     rows, fields, fw, S = 1000, 2, 1, 4
     ctx = _kv_ctx(rows, fields, fw, S, 4096)
