@@ -386,10 +386,10 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   one 16-byte pair.
  * SMALL_CHUNKS: CHECK and WRITEBACK blocks own 64 transactions instead of
  *   256 (more blocks, fewer dependent rounds each).
- * LANE_COPY: the read-out and write-back copy a field per lane with 16-byte
- *   loads all in flight (fields of 16-byte aligned stride, at most 32 words)
- *   instead of spreading each field over a group of lanes, one dependent
- *   load round per 8 op slots; for epochs too small to hide that latency.
+ * LANE_COPY: the read-out and write-back copy each field by one lane with
+ *   16-byte loads (fields of 16-byte aligned stride, at most 32 words)
+ *   instead of by 32 lanes one word each; faster for small epochs, slower
+ *   for large ones.
  * Results are identical for every combination; may change between any two
  * epochs. */
 #define LTPG_TUNE_EXEC_PER_OP 1u

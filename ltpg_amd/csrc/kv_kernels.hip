@@ -35,9 +35,13 @@
 // whose fields start at multiples of field_stride words (field_stride =
 // field_words rounded up to 16 words = 64 bytes for fields of 64 bytes or
 // more, so a 100-byte field touches exactly two 64-byte segments).  A wave
-// moves the fields of its ops cooperatively: LP lanes per op (LP = next
-// power of two >= field_words, at most 64), the op's addresses fetched from
-// its owner lane by __shfl.
+// moves the fields of its ops cooperatively, each op's addresses fetched from
+// its owner lane by __shfl: in CHECK's read-out and in WRITEBACK, fields of
+// up to 32 words two per instruction (32 lanes per field, wave_copy_pairs)
+// and wider ones G lanes per field (group_copy), or, with the LANE_COPY
+// shape, one lane per field with 16-byte loads (lane_copy); the counting
+// form's read-out LP lanes per op (LP = next power of two >= field_words,
+// at most 64).
 #include "device_common.h"
 #include "ltpg_internal.h"
 
@@ -631,11 +635,6 @@ __device__ __forceinline__ uint32_t list_build(bool pick, uint32_t t, uint32_t *
     return n;
 }
 
-// Field copies of a group's flagged slots.  The op of slot k lives in lane
-// grp * G + k; its field is src[src_off + w] -> dst[dst_off + w] (+ delta),
-// w < fw, and lane gl of the group handles words gl, gl + G, ...  Loads of
-// KB slots are issued before their stores, which requires that no two
-// flagged slots of one call write the same word (callers guarantee it).
 // Position of the j-th set bit of m (j < popcount(m)), by halving.
 __device__ __forceinline__ uint32_t nth_set_bit(unsigned long long m, uint32_t j)
 {
@@ -719,6 +718,11 @@ __device__ __forceinline__ void lane_copy(bool active, uint64_t src_off, uint64_
     }
 }
 
+// Field copies of a group's flagged slots.  The op of slot k lives in lane
+// grp * G + k; its field is src[src_off + w] -> dst[dst_off + w] (+ delta),
+// w < fw, and lane gl of the group handles words gl, gl + G, ...  Loads of
+// KB slots are issued before their stores, which requires that no two
+// flagged slots of one call write the same word (callers guarantee it).
 template <int G, int KB>
 __device__ __forceinline__ void group_copy(unsigned long long flags, uint64_t src_off,
                                            uint64_t dst_off, uint32_t delta, const uint32_t *src,
