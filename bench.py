@@ -699,6 +699,35 @@ def run_served_c5(args):
                 f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms, "
                 f"submit {r['submit_s']:.3f} s for {at[-1]:.3f} s of arrivals")
     kv.close()
+    # the reference's ZIPFIAN mode makes row 0 a hot key (Y0/Predefine.h:12,
+    # Y0/Execute.h:63-73): its RMW writes go to the commutative reduce and
+    # never register or conflict.  The same arrivals again, on a table
+    # with row 0 as the hot row (a fresh context: the hot row is fixed at
+    # creation, and two 128 GB tables do not fit beside each other).
+    hot_out = []
+    if args.c5_hot_rates:
+        kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                       max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW, kv_hot_row=0)
+        kv.kv_fill_reference()
+        kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY)
+        for rate in args.c5_hot_rates:
+            n = max(1000, int(rate * 1e6 * args.c5_seconds))
+            n_kv = n // 2
+            keys, meta, _ = L.kv_generate(seed=seed + int(rate * 1000), rows=rows, fields=fields, n_txns=n_kv,
+                                          n_slots=S, min_ops=1, max_ops=S, theta=0.99)
+            keys = keys.reshape(S, n_kv).T.copy()
+            meta = meta.reshape(S, n_kv).T.copy()
+            tb = gen.next(n - n_kv)
+            at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
+            kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
+            entry = {}
+            for form, plain in (("served", 0), ("plain", 1)):
+                r = _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain)
+                entry[form] = r
+                log(f"C5 hot row 0, {form}: offered {rate}M txn/s -> {r['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
+                    f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms")
+            hot_out.append(entry)
+        kv.close()
     tp.close()
     best = max(out, key=lambda r: r["committed_txn_per_s"])
     res = {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
@@ -720,6 +749,11 @@ def run_served_c5(args):
              "committed_ratio": a["committed_txn_per_s"] / max(b["committed_txn_per_s"], 1e-9),
              "p99_ms": a["latency_ms"]["p99"], "plain_p99_ms": b["latency_ms"]["p99"]}
             for a, b in zip(out, plain_out)]
+    if hot_out:
+        res["reference_zipfian_hot_key"] = {
+            "what": "the same workload with row 0 as the hot key, as the reference's ZIPFIAN mode runs it (its RMW "
+                    "writes reduced commutatively, never registered); served and plain forms at each rate",
+            "rates": hot_out}
     return res
 
 
@@ -786,6 +820,8 @@ def main():
                     help="offered loads of the served form, M txn/s")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
     ap.add_argument("--c5-drain-s", type=float, default=8.0, help="cap on the drain after the arrivals, per load")
+    ap.add_argument("--c5-hot-rates", type=float, nargs="*", default=[5],
+                    help="offered rates (M txn/s) also run with row 0 as the hot key (the reference's ZIPFIAN mode)")
     ap.add_argument("--c5-no-plain", action="store_true",
                     help="skip the plain per-batch form run at each C5 rate")
     ap.add_argument("--c5-rows", type=int, default=100_000_000, help="key-value rows of the served form")

@@ -77,6 +77,38 @@ def test_served_kv_epochs_replay_on_the_oracle(gpu, plain):
     assert all(d["attempts"] >= 1 and d["commit_ns"] >= d["arrival_ns"] for d in done.values())
 
 
+@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
+def test_served_hot_row_epochs_replay_on_the_oracle(gpu, plain):
+    """Row 0 as the hot key (the reference's ZIPFIAN mode): its ops never
+    conflict and committed writes to it add through the reduce; the served
+    epochs replayed through the oracle with the same hot row reproduce every
+    decision and the final table."""
+    rows, fields, fw, S, n = 3000, 3, 2, 8, 4000
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
+                    max_txns=1024, kv_max_slots=S, kv_hot_row=0)
+    ctx.kv_table_load(kc.initial_table(rows, fields, fw))
+    keys, meta = _kv_requests(41, n, S, rows, fields, theta=0.99)
+    assert (keys == 0).any()
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=600, record_epochs=1, plain=plain)
+    _submit_in_waves(sv, keys, meta)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_KV, e) for e in range(st["epochs"][0])]
+    sv.stop()
+    assert len(done) == n
+    table = kc.initial_table(rows, fields, fw)
+    for e, ids in enumerate(epochs):
+        ids = ids.astype(np.int64)
+        k = keys[ids].T.reshape(-1).copy()
+        m = meta[ids].T.reshape(-1).copy()
+        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, hot_row=0)
+        for t, rid in enumerate(ids):
+            assert ep.committed(t) == (done[rid]["epoch"] == e), f"request {rid} epoch {e}"
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+
+
 def test_served_scans_replay_on_the_oracle(gpu):
     """Requests whose reads are scans: each committed request gets its
     scanned rows (inside the table, in op order) as the oracle's replay of
