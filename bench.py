@@ -154,10 +154,14 @@ def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
                       f"{total_s:.1f} s"}
 
 
-def run_kv(args, theta, cpu_baseline=True, name=None):
+def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
+    """hot_row = 0: row 0 is the hot key, as in the reference's ZIPFIAN mode
+    (its byte accounting still counts row-0 writes as registrations)."""
     import ltpg_amd as L
 
     w = kv_workload(name or args.workload, theta)
+    if hot_row is not None:
+        w["desc"] += f", row {hot_row} the hot key (the reference's ZIPFIAN mode)"
     rows, fields, fw, B, S = w["rows"], w["fields"], w["fw"], w["B"], w["S"]
     P = args.batches
     log(f"generating {P} batches: {w['desc']}")
@@ -165,7 +169,8 @@ def run_kv(args, theta, cpu_baseline=True, name=None):
     gen = [L.kv_generate(seed=0x4C545047 + i, rows=rows, fields=fields, n_txns=B, n_slots=S,
                          read_pct=w["read_pct"], theta=w["theta"], tid_base=0, scan_len=scan) for i in range(P)]
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
-                    max_txns=B, kv_max_slots=S, values=L.VALUES_RMW, kv_max_scan=scan)
+                    max_txns=B, kv_max_slots=S, values=L.VALUES_RMW, kv_max_scan=scan,
+                    kv_hot_row=L.NO_HOT_ROW if hot_row is None else hot_row)
     ctx.kv_fill_reference()
     dev = [L.DeviceKvBatch(ctx, k, m, t, S) for (k, m, t) in gen]
     log("batches resident in HBM")
@@ -870,6 +875,8 @@ def main():
             v = run_kv(args, th, cpu_baseline=False)
             variants.append({k: v[k] for k in keep})
         if args.workload == "kv-c2" and args.theta is None:
+            v = run_kv(args, 0.99, cpu_baseline=False, hot_row=0)
+            variants.append({k: v[k] for k in keep})
             v = run_tpcc(args)
             variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
             v = run_tpcc(args, mix=True)
