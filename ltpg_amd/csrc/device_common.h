@@ -45,10 +45,20 @@ __device__ __forceinline__ void register_read(unsigned long long *r, unsigned lo
 // position is in v now has waw: its doomed byte is set.  Every writer but the
 // minimum one is caught by exactly one of the two: the first registration
 // that goes below a writer sees exactly that writer's value.
+__device__ __forceinline__ bool finish_write(unsigned long long *r, unsigned long long cur, unsigned long long me,
+                                             uint32_t tag, uint8_t *doomed, uint8_t epoch);
+
 __device__ __forceinline__ bool register_write(unsigned long long *r, unsigned long long me,
                                                uint32_t tag, uint8_t *doomed, uint8_t epoch)
 {
-    const unsigned long long cur = probe(r);
+    return finish_write(r, probe(r), me, tag, doomed, epoch);
+}
+
+// register_write in two steps, so that a thread with several registers
+// issues all their probes before the first atomic: cur is probe(r).
+__device__ __forceinline__ bool finish_write(unsigned long long *r, unsigned long long cur, unsigned long long me,
+                                             uint32_t tag, uint8_t *doomed, uint8_t epoch)
+{
     if (cur < me) return true;
     if (cur > me) {
         const unsigned long long old = atomicMin(r, me);

@@ -288,10 +288,14 @@ __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev 
                     const uint64_t sr = (uint64_t)no.OL_SUPPLY_W_ID[at] * 100000 + item;
                     const uint64_t lr = (uint64_t)no.W_ID[q] * 450000 + no.O_OL_ID[q] + j;
                     bad += item >= d.t[IT].rows;
-                    if (sr < d.t[ST].rows) waw |= register_write(d.regs + d.off_stock + sr, me, tag, d.doomed, epoch);
-                    else bad++;
-                    if (lr < d.t[OL].rows) waw |= register_write(d.regs + d.off_line + lr, me, tag, d.doomed, epoch);
-                    else bad++;
+                    const bool sok = sr < d.t[ST].rows, lok = lr < d.t[OL].rows;
+                    bad += !sok + !lok;
+                    // both probes in flight before either atomic
+                    unsigned long long *rs = d.regs + d.off_stock + (sok ? sr : 0);
+                    unsigned long long *rl = d.regs + d.off_line + (lok ? lr : 0);
+                    const unsigned long long cs = probe(rs), cl = probe(rl);
+                    if (sok) waw |= finish_write(rs, cs, me, tag, d.doomed, epoch);
+                    if (lok) waw |= finish_write(rl, cl, me, tag, d.doomed, epoch);
                 }
             } else {
                 // header role: slots 0..4
@@ -320,12 +324,15 @@ __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev 
             bad += r.wh >= d.t[WH].rows ? 2 : 0;  // slots 0, 1
             bad += r.di >= d.t[DI].rows ? 2 : 0;  // slots 2, 3
             bool waw = false;
-            if (r.cu < d.t[CU].rows) {            // slots 4, 5
-                if (COUNTS) register_read(cust_r(d, r.cu), me);
-                waw |= register_write(cust_w(d, r.cu), me, tag, d.doomed, epoch);
-            } else bad += 2;
-            if (r.hi < d.t[HI].rows) waw |= register_write(d.regs + d.off_hist + r.hi, me, tag, d.doomed, epoch);
-            else bad++;
+            const bool cok = r.cu < d.t[CU].rows, hok = r.hi < d.t[HI].rows;
+            bad += (cok ? 0 : 2) + !hok;          // slots 4, 5 and the HISTORY insert
+            if (COUNTS && cok) register_read(cust_r(d, r.cu), me);
+            // both probes in flight before either atomic
+            unsigned long long *rc = cust_w(d, cok ? r.cu : 0);
+            unsigned long long *rh = d.regs + d.off_hist + (hok ? r.hi : 0);
+            const unsigned long long cc = probe(rc), ch = probe(rh);
+            if (cok) waw |= finish_write(rc, cc, me, tag, d.doomed, epoch);
+            if (hok) waw |= finish_write(rh, ch, me, tag, d.doomed, epoch);
             if (waw) d.doomed[no.n + p] = epoch;
         }
     }
