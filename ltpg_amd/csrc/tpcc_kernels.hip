@@ -711,17 +711,21 @@ __global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDe
 // per block instead of one per Payment.  More bins than LDS holds: direct
 // global atomics.
 constexpr uint32_t kYtdBins = 12288;  // 48 KB of LDS
+// 1024 threads per block: the block count is kept small (each block ends
+// with one global atomic per touched bin), so each block is made wide
+// instead, which keeps every thread's chain of dependent loads short
+constexpr uint32_t kYtdBlock = 1024;
 
-__global__ __launch_bounds__(kBlock) void ytd_kernel(TpccDev d, PaymentDev pay, uint32_t chunk)
+__global__ __launch_bounds__(kYtdBlock) void ytd_kernel(TpccDev d, PaymentDev pay, uint32_t chunk)
 {
     __shared__ uint32_t bins[kYtdBins];
     const uint64_t W = d.t[WH].rows, nd = d.t[DI].rows;
     const bool lds = W + nd <= kYtdBins;
     if (lds)
-        for (uint32_t i = threadIdx.x; i < W + nd; i += kBlock) bins[i] = 0;
+        for (uint32_t i = threadIdx.x; i < W + nd; i += kYtdBlock) bins[i] = 0;
     __syncthreads();
     const uint64_t p0 = (uint64_t)blockIdx.x * chunk, p1 = min<uint64_t>(pay.n, p0 + chunk);
-    for (uint64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    for (uint64_t p = p0 + threadIdx.x; p < p1; p += kYtdBlock) {
         if (!((d.bitmap_pay[p / 64] >> (p % 64)) & 1ull)) continue;
         const uint64_t w = pay.W_ID[p], di = w * 10 + pay.D_ID[p];
         const uint32_t amt = pay.H_AMOUNT[p];
@@ -736,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void ytd_kernel(TpccDev d, PaymentDev pay, 
     }
     __syncthreads();
     if (lds) {
-        for (uint32_t i = threadIdx.x; i < W + nd; i += kBlock) {
+        for (uint32_t i = threadIdx.x; i < W + nd; i += kYtdBlock) {
             const uint32_t v = bins[i];
             if (v == 0) continue;
             if (i < W) atomicAdd(d.t[WH].data + (uint64_t)i * kStride[WH] + 2, v);
@@ -838,7 +842,7 @@ hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const 
     hipLaunchKernelGGL(writeback_kernel, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_blocks, rmw);
     if (rmw && pay.n) {
         const uint32_t chunk = std::max<uint32_t>(4096, (pay.n + 255) / 256);
-        hipLaunchKernelGGL(ytd_kernel, dim3((pay.n + chunk - 1) / chunk), dim3(kBlock), 0, s, d, pay, chunk);
+        hipLaunchKernelGGL(ytd_kernel, dim3((pay.n + chunk - 1) / chunk), dim3(kYtdBlock), 0, s, d, pay, chunk);
     }
     return hipGetLastError();
 }
