@@ -234,8 +234,8 @@ int ltpg_kv_table_fill_reference(ltpg_ctx *ctx);
 /* Slot-major, as the reference kernel indexes LOC[B * op + NO]
  * (YCSB/0/Execute.h:20,47): slot s of transaction t is at [s * n_txns + t]. */
 typedef struct ltpg_kv_batch {
-    uint32_t n_txns;         /* B                                              */
-    uint32_t n_slots;        /* S (<= kv_max_slots)                            */
+    uint32_t n_txns;         /* B (0: an empty epoch; the arrays may be NULL)  */
+    uint32_t n_slots;        /* S (1..kv_max_slots when B > 0)                 */
     const uint64_t *keys;    /* [S * B] row ids                                */
     const uint32_t *meta;    /* [S * B] LTPG_KV_META(kind, field)              */
     const uint32_t *tids;    /* [B]                                            */

@@ -638,6 +638,7 @@ int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
     if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null batch");
     if (b->n_txns > ctx->cfg.max_txns) LTPG_FAIL(LTPG_ERR_INVALID, "n_txns exceeds max_txns");
     if (b->n_slots > kv.max_slots) LTPG_FAIL(LTPG_ERR_INVALID, "n_slots exceeds kv_max_slots");
+    if (b->n_txns > 0 && b->n_slots == 0) LTPG_FAIL(LTPG_ERR_INVALID, "n_slots must be > 0 for a non-empty batch");
     if (b->n_txns > 0 && (!b->keys || !b->meta || !b->tids)) LTPG_FAIL(LTPG_ERR_INVALID, "null batch array");
     const bool request_major = (mem & LTPG_BATCH_REQUEST_MAJOR) != 0;
     mem &= ~LTPG_BATCH_REQUEST_MAJOR;

@@ -1090,7 +1090,8 @@ RegLayout reg_layout(const KvState &kv)
 hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                              DeviceCounters *ctr, hipStream_t s, bool zero_counts)
 {
-    if (b.n_txns == 0) return hipSuccess;
+    // an empty epoch still starts its commit counts at zero
+    if (b.n_txns == 0) return zero_counts ? launch_reset_epoch_counters(ctr, s) : hipSuccess;
     const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
     const dim3 op_grid((unsigned)((ops + kBlock - 1) / kBlock));
     if (kv.exec_per_op && !counts) {

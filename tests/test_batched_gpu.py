@@ -209,6 +209,60 @@ def test_plain_epochs_after_a_graph_run(gpu):
     r.close()
 
 
+KINDS = ["kv", "tpcc", "kv", "kv", "tpcc", "tpcc", "kv", "tpcc"]
+EMPTY_AT = {0: "kv", 3: "tpcc", 4: "kv", 11: "tpcc"}  # positions among the 12 batches
+
+
+def _empty_desc(kind):
+    d = L.BatchDesc()
+    if kind == "kv":
+        d.workload = L.WORKLOAD_KV
+        d.kv = L.KvBatch(0, KV["kv_max_slots"], None, None, None)
+    else:
+        d.workload = L.WORKLOAD_TPCC  # every transaction count zero
+    return d
+
+
+@pytest.mark.parametrize("mode", [L.RUN_PLAIN, L.RUN_GRAPH], ids=["plain", "graph"])
+def test_empty_batches_change_nothing(gpu, mode):
+    """Batches with no transactions commit nothing and change no row: a run
+    with them interleaved gives the same commit sets and tables as the run
+    without them."""
+    def run(empties):
+        kv, tp = _contexts()
+        descs, keep, meta = _batches(np.random.default_rng(7), len(KINDS), kv, tp, KINDS)
+        real = list(range(len(descs)))
+        if empties:
+            src = iter(zip(descs, meta))
+            descs, meta, real = [], [], []
+            for pos in range(len(KINDS) + len(EMPTY_AT)):
+                if pos in EMPTY_AT:
+                    descs.append(_empty_desc(EMPTY_AT[pos]))
+                    meta.append(EMPTY_AT[pos])
+                else:
+                    d, m = next(src)
+                    descs.append(d)
+                    meta.append(m)
+                    real.append(pos)
+        r = L.Batched(kv, tp, descs)
+        r.run(mode)
+        r.run(mode)
+        maps = _bitmaps(r, meta)
+        tabs = _tables(kv, tp)
+        r.close()
+        return maps, real, tabs
+
+    base_maps, _, base_tabs = run(False)
+    maps, real, tabs = run(True)
+    for i, pos in enumerate(real):
+        for x, y in zip(base_maps[i], maps[pos]):
+            np.testing.assert_array_equal(x, y)
+    for pos in EMPTY_AT:
+        assert all(m.size == 0 for m in maps[pos])
+    for x, y in zip(base_tabs, tabs):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("fw", [2, 25], ids=["8B", "100B"])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])

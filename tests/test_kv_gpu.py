@@ -135,6 +135,32 @@ def test_gpu_matches_oracle_seeded(gpu, rows, fields, fw, B, S, theta, read_pct,
 
 
 @pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_gpu_empty_epochs_between_batches(gpu, counts, rm):
+    """An epoch of no transactions commits nothing, changes no row and starts
+    its commit count at zero; the epochs around it match the oracle."""
+    rows, fields, fw, S = 5000, 4, 2, 8
+    table = np.random.default_rng(11).integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64).astype(np.uint32)
+    empty = (np.zeros(0, np.uint64), np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    with _ctx(rows, fields, fw, 4000, S, counts=counts) as ctx:
+        ctx.kv_table_load(table)
+        for ep, B in enumerate([4000, 0, 0, 3000, 0]):
+            if B:
+                keys, meta, tids = L.kv_generate(seed=20 + ep, rows=rows, fields=fields, n_txns=B, n_slots=S,
+                                                 min_ops=1, max_ops=S, read_pct=50, theta=0.8, tid_base=ep * 4000)
+            else:
+                keys, meta, tids = empty
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
+            _assert_epoch(g, ora, meta, tids, fw)
+            if B == 0:
+                assert g["count"] == 0 and g["bitmap"].size == 0
+            else:
+                assert g["count"] > 0
+            table = ora.table
+
+
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("hot_row", [0, 17])
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
 def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts, rm):
@@ -378,6 +404,9 @@ def test_gpu_rejects_bad_shapes(gpu):
         keys, meta, tids = L.kv_generate(seed=1, rows=10, fields=2, n_txns=65, n_slots=4, theta=0.0)
         with pytest.raises(L.LtpgError):
             ctx.kv_submit(keys, meta, tids, 4)  # 65 > max_txns
+        with pytest.raises(L.LtpgError) as ei:   # transactions without slots
+            ctx.kv_submit(np.zeros(0, np.uint64), np.zeros(0, np.uint32), np.arange(3, dtype=np.uint32), 0)
+        assert ei.value.code == L.ERR_INVALID
         with pytest.raises(L.LtpgError) as ei:
             ctx.epoch_run()
         assert ei.value.code == L.ERR_STATE

@@ -112,6 +112,31 @@ def test_seeded_epochs_match_oracle(gpu, seed, rules, values, counts):
     _assert_tables(ctx, tables)
 
 
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_empty_and_one_sided_epochs_match_oracle(gpu, counts):
+    """Epochs with no transactions, only NewOrders or only Payments between
+    mixed ones: each matches the oracle, an empty one commits nothing and
+    starts its counts at zero, and the tables carry over."""
+    W = 2
+    ctx = _ctx(W, 4096, counts=counts)
+    ctx.tpcc_populate(SEED)
+    tables = oracle.tpcc_populate(W, SEED)
+    ora = oracle.TpccOracle(W)
+    gens = {pct: L.TpccGenerator(warehouses=W, seed=60 + pct, neworder_pct=pct) for pct in (50, 100, 0)}
+    tid = 0
+    for pct, n in [(50, 2000), (None, 0), (100, 1500), (0, 1500), (None, 0), (50, 2000), (None, 0)]:
+        b = L.TpccBatch.empty(0, 0) if pct is None else gens[pct].next(n, tid_base=tid)
+        tid += n
+        g = _gpu_epoch(ctx, b)
+        o = ora.epoch(tables, b, L.RULES_REFERENCE, L.VALUES_RMW)
+        assert o.range_errors == 0
+        _assert_epoch(g, o, b)
+        if pct is None:
+            assert g["no_count"] == 0 and g["pay_count"] == 0
+            assert g["no_bitmap"].size == 0 and g["pay_bitmap"].size == 0
+    _assert_tables(ctx, tables)
+
+
 def test_fast_and_counting_checks_agree_at_scale(gpu):
     W = 8
     gen_a = L.TpccGenerator(warehouses=W, seed=9)
