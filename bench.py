@@ -603,7 +603,8 @@ def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain):
     oe_tid = np.r_[tb.nf("TID"), tb.pf("TID")]
     oe_order = np.argsort(oe_tid, kind="stable")  # request k of the batch's own arrival order
     arrival_of = {}  # request id -> arrival index
-    sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20, plain=plain)
+    sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20, plain=plain,
+                  epoch_floor=args.c5_epoch_floor)
     window = 1e-3
     edges = np.searchsorted(at, np.arange(window, at[-1] + window, window), side="right")
     ki = oi = prev = 0
@@ -738,7 +739,7 @@ def run_served_c5(args):
     res = {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
            "latency_ms": best["latency_ms"], "rates": out,
            "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
-                      "slots": 2, "epoch_growth": 8,
+                      "slots": 2, "epoch_growth": 8, "epoch_floor": args.c5_epoch_floor,
                       "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
                       "form": "served (continuous admission, requeue with original TID)"}}
     if plain_out:
@@ -831,6 +832,8 @@ def main():
                     help="skip the plain per-batch form run at each C5 rate")
     ap.add_argument("--c5-rows", type=int, default=100_000_000, help="key-value rows of the served form")
     ap.add_argument("--c5-budget-mb", type=int, default=4096, help="HBM budget per in-flight epoch, MiB")
+    ap.add_argument("--c5-epoch-floor", type=int, default=32,
+                    help="smallest epoch the commit-based epoch cap allows (ltpg_serve_config.epoch_floor)")
     ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form variant of the default run")
     ap.add_argument("--c4-plain-only", action="store_true",

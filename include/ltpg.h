@@ -390,6 +390,14 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   16-byte loads (fields of 16-byte aligned stride, at most 32 words)
  *   instead of by 32 lanes one word each; faster for small epochs, slower
  *   for large ones.
+ * SMALL_EPOCH: an epoch of at most LTPG_SMALL_EPOCH_OPS op slots (n_txns *
+ *   n_slots) run with LTPG_PHASE_ALL, in the fast form (no
+ *   LTPG_FLAG_CONFLICT_COUNTS), without a hot row and without timing, runs
+ *   as one single-workgroup kernel: ops, flags and the W registrations stay
+ *   in LDS and the per-row registers in HBM are not touched.  One launch
+ *   instead of four or five; other epochs run as before.  Not one of the
+ *   autotuner's combinations (LTPG_TUNE_ALL).  The served form runs its
+ *   small key-value epochs this way whatever the context's flags.
  * Results are identical for every combination; may change between any two
  * epochs. */
 #define LTPG_TUNE_EXEC_PER_OP 1u
@@ -397,6 +405,8 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
 #define LTPG_TUNE_SMALL_CHUNKS 4u
 #define LTPG_TUNE_LANE_COPY 8u
 #define LTPG_TUNE_ALL 15u
+#define LTPG_TUNE_SMALL_EPOCH 16u
+#define LTPG_SMALL_EPOCH_OPS 2048u
 int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags);
 
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
@@ -527,7 +537,7 @@ typedef struct ltpg_serve_config {
     uint32_t record_epochs;   /* 1: keep every epoch's member list (tests)       */
     uint32_t epoch_growth;    /* > 0: an epoch admits at most epoch_growth x the
                                  commits of the workload's last epoch (at least
-                                 256) -- under contention epochs commit a short
+                                 epoch_floor) -- under contention epochs commit a short
                                  chain whatever their size; 0 = always the cap.
                                  Default 8. */
     uint32_t plain;           /* 1: run the same admission as the plain
@@ -538,6 +548,10 @@ typedef struct ltpg_serve_config {
                                  bitmap read back before the next epoch is
                                  formed (the driver loop of tp.cu:136-261);
                                  `slots` is ignored.  Default 0. */
+    uint32_t epoch_floor;     /* smallest epoch the epoch_growth cap allows
+                                 (0 = 32).  Default 32: with 64-op requests an
+                                 epoch of 32 fits one single-workgroup kernel
+                                 (LTPG_SMALL_EPOCH_OPS) */
 } ltpg_serve_config;
 
 typedef struct ltpg_serve_done {

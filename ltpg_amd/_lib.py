@@ -129,7 +129,8 @@ class ServeConfig(ctypes.Structure):
 
     _fields_ = [("size", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("kv_ops", ctypes.c_uint32),
                 ("max_epoch_txns", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64), ("keep_reads", ctypes.c_uint32),
-                ("record_epochs", ctypes.c_uint32), ("epoch_growth", ctypes.c_uint32), ("plain", ctypes.c_uint32)]
+                ("record_epochs", ctypes.c_uint32), ("epoch_growth", ctypes.c_uint32), ("plain", ctypes.c_uint32),
+                ("epoch_floor", ctypes.c_uint32)]
 
 
 class ServeDone(ctypes.Structure):

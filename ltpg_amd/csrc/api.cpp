@@ -619,11 +619,12 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    if (flags & ~LTPG_TUNE_ALL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_SMALL_EPOCH)) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
     ctx->kv.exec_per_op = (flags & LTPG_TUNE_EXEC_PER_OP) ? 1 : 0;
     ctx->kv.split_regs = (flags & LTPG_TUNE_SPLIT_REGISTERS) ? 1 : 0;
     ctx->kv.small_chunks = (flags & LTPG_TUNE_SMALL_CHUNKS) ? 1 : 0;
     ctx->kv.lane_copy = (flags & LTPG_TUNE_LANE_COPY) ? 1 : 0;
+    ctx->kv.small_epoch = (flags & LTPG_TUNE_SMALL_EPOCH) ? 1 : 0;
     return LTPG_OK;
 }
 
@@ -738,6 +739,11 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     if (!kv.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "no batch submitted");
     const EpochTag e = epoch_tag(ctx);
     hipStream_t s = ctx->stream;
+    if (kv.small_epoch && phase_mask == LTPG_PHASE_ALL && !ctx->timing &&
+        !(ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS) && kv_small_epoch_ok(kv, kv.cur)) {
+        LTPG_HIP_TRY(launch_kv_small_epoch(kv, kv.cur, ctx->cfg.values, ctx->counters, nullptr, nullptr, 0, s));
+        return LTPG_OK;
+    }
     hipEvent_t *ev = nullptr;
     if (ctx->timing) {
         if ((rc = take_events(ctx, &ev))) return rc;

@@ -141,22 +141,13 @@ struct Op {
     bool valid;
 };
 
-// Loads are unconditional on a clamped index (element 0 of a non-empty batch
-// when the lane has no op) and the result is selected afterwards: a load
-// under a lane-dependent branch sits in its own basic block and the compiler
-// waits for it before the next one, which would serialize the misses.
-// A scan (kind 3) is valid only if its length is at most max_scan; kernels
-// that never look at reads pass 0.
-__device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
-                                      const uint32_t *__restrict__ meta, uint64_t i, bool live,
-                                      uint64_t rows, uint32_t fields, uint32_t max_scan = 0)
+// A slot's op from its row and meta word.  A scan (kind 3) is valid only if
+// its length is at most max_scan; kernels that never look at reads pass 0.
+__device__ __forceinline__ Op decode_op(uint64_t row, uint32_t m, uint64_t rows, uint32_t fields,
+                                        uint32_t max_scan)
 {
     Op o;
-    const uint64_t j = live ? i : 0ull;
-    const uint32_t m0 = meta[j];
-    const uint64_t r0 = keys[j];
-    const uint32_t m = live ? m0 : 0u;
-    o.row = live ? r0 : 0ull;
+    o.row = row;
     o.kind = m & 3u;
     o.field = (m >> 8) & 0xffffu;
     const uint32_t sl = (m >> 24) + 1u;
@@ -167,6 +158,20 @@ __device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
             : o.kind == LTPG_KV_READ            ? 1u
                                                 : (uint32_t)(sl < left ? sl : left);
     return o;
+}
+
+// Loads are unconditional on a clamped index (element 0 of a non-empty batch
+// when the lane has no op) and the result is selected afterwards: a load
+// under a lane-dependent branch sits in its own basic block and the compiler
+// waits for it before the next one, which would serialize the misses.
+__device__ __forceinline__ Op load_op(const uint64_t *__restrict__ keys,
+                                      const uint32_t *__restrict__ meta, uint64_t i, bool live,
+                                      uint64_t rows, uint32_t fields, uint32_t max_scan = 0)
+{
+    const uint64_t j = live ? i : 0ull;
+    const uint32_t m0 = meta[j];
+    const uint64_t r0 = keys[j];
+    return decode_op(live ? r0 : 0ull, live ? m0 : 0u, rows, fields, max_scan);
 }
 
 // Batch layouts.  Slot s of transaction t sits at s * B + t (slot-major,
@@ -903,6 +908,288 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
     }
 }
 
+// One small epoch in one workgroup (LTPG_TUNE_SMALL_EPOCH, and the served
+// form's epochs): EXECUTE, read marking, CHECK with the read-out, and
+// WRITEBACK back to back, separated by workgroup barriers instead of kernel
+// boundaries.  The rules are those of the fast form (kv_execute_dedup_kernel,
+// kv_mark_reads_kernel, kv_check_fast_kernel, kv_writeback_kernel) with the
+// whole epoch in one workgroup: the ops, TIDs and conflict flags sit in LDS,
+// and the W registrations go to an LDS hash of the written rows (row + 1 ->
+// minimum TID << 24 | position) instead of the per-row registers in HBM,
+// which this path never touches.  The batch may sit in pinned host memory
+// (read once).  With seq_out the commit bitmap is also written to host_bits
+// and then seq is stored there with a system-scope release, as
+// publish_kernel does after a multi-kernel epoch.  No hot row; rows < 2^32 - 1.
+constexpr uint32_t kSmallBlock = 1024;
+constexpr uint32_t kSmallOps = LTPG_SMALL_EPOCH_OPS;
+constexpr uint32_t kSmallPer = kSmallOps / kSmallBlock;  // op slots per thread
+constexpr uint32_t kSmallSlots = 2 * kSmallOps;          // hash load <= 1/2: probing always ends
+constexpr uint32_t kSmallUnroll = 4;                     // field words in flight per thread
+constexpr uint32_t kSmallMerge = 8;                      // items per thread of the one-round copy
+
+__device__ __forceinline__ uint32_t small_hash(uint64_t row)
+{
+    return (uint32_t)((row * 0x9E3779B97F4A7C15ull) >> 40) & (kSmallSlots - 1);
+}
+
+// Slot of a written row in the hash, or -1 if no op of the epoch writes it.
+__device__ __forceinline__ int small_find(const uint32_t *s_key, uint64_t row)
+{
+    const uint32_t key = (uint32_t)row + 1u;
+    for (uint32_t h = small_hash(row);; h = (h + 1) & (kSmallSlots - 1)) {
+        const uint32_t k = s_key[h];
+        if (k == key) return (int)h;
+        if (k == 0u) return -1;
+    }
+}
+
+__global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, const uint32_t *__restrict__ tids,
+    uint32_t B, uint32_t S, bool rm, uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, uint32_t max_scan,
+    uint32_t L, uint32_t values, uint32_t *__restrict__ table, uint32_t *__restrict__ read_values,
+    uint64_t *__restrict__ bitmap, uint64_t *__restrict__ host_bits, uint64_t *seq_out, uint64_t seq,
+    DeviceCounters *__restrict__ ctr)
+{
+    __shared__ uint64_t s_row[kSmallOps];
+    __shared__ uint32_t s_meta[kSmallOps];
+    __shared__ uint32_t s_tid[kSmallOps];
+    __shared__ uint32_t s_key[kSmallSlots];            // row + 1 of a written row, 0 = empty
+    __shared__ unsigned long long s_min[kSmallSlots];  // its writers' minimum TID << 24 | position
+    __shared__ uint8_t s_waw[kSmallOps], s_raw[kSmallOps], s_war[kSmallOps];
+    __shared__ uint32_t s_list[kSmallOps];             // committed transactions
+    __shared__ uint32_t s_fold[kSmallOps];             // per committed (txn, slot): folded write count
+    __shared__ unsigned long long s_bits[kSmallOps / 64];  // the commit bitmap
+    __shared__ uint32_t s_n, s_bad;
+    const uint32_t n = B * S;
+    // the batch: every load issued before the first use
+    uint64_t r[kSmallPer];
+    uint32_t m[kSmallPer], tv[kSmallPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) {
+        const uint32_t i = k * kSmallBlock + threadIdx.x;
+        r[k] = keys[i < n ? i : 0u];
+        m[k] = meta[i < n ? i : 0u];
+        tv[k] = tids[i < B ? i : 0u];
+    }
+    for (uint32_t k = threadIdx.x; k < kSmallSlots; k += kSmallBlock) {
+        s_key[k] = 0u;
+        s_min[k] = ~0ull;
+    }
+    // the epoch's commit counts start at zero (thread 0 adds to them last)
+    for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kSmallBlock)
+        ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_bad = 0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) {
+        const uint32_t i = k * kSmallBlock + threadIdx.x;
+        if (i < n) {
+            s_row[i] = r[k];
+            s_meta[i] = m[k];
+        }
+        if (i < B) {
+            s_tid[i] = tv[k];
+            s_waw[i] = 0;
+            s_raw[i] = 0;
+            s_war[i] = 0;
+        }
+    }
+    // (the counter stores and thread 0's later atomics meet in one L2; the
+    // barriers order them)
+    __syncthreads();
+    // EXECUTE: W registrations into the hash
+    uint32_t bad = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) {
+        const uint32_t i = k * kSmallBlock + threadIdx.x;
+        if (i >= n) continue;
+        const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+        bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
+        if (!o.valid || o.kind != LTPG_KV_WRITE) continue;
+        const uint32_t t = txn_of(i, B, S, rm);
+        const uint32_t key = (uint32_t)o.row + 1u;
+        uint32_t h = small_hash(o.row);
+        while (true) {
+            const uint32_t prev = atomicCAS(&s_key[h], 0u, key);
+            if (prev == 0u || prev == key) break;
+            h = (h + 1) & (kSmallSlots - 1);
+        }
+        atomicMin(&s_min[h], ((unsigned long long)s_tid[t] << kTidShift) | t);
+    }
+    if (bad) atomicAdd(&s_bad, bad);
+    __syncthreads();
+    // waw of every write; raw of every read, war of the row's minimum writer
+    // when a higher-priority transaction reads it
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) {
+        const uint32_t i = k * kSmallBlock + threadIdx.x;
+        if (i >= n) continue;
+        const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+        if (!o.valid) continue;
+        const uint32_t t = txn_of(i, B, S, rm);
+        const unsigned long long me = ((unsigned long long)s_tid[t] << kTidShift) | t;
+        if (o.kind == LTPG_KV_WRITE) {
+            if (s_min[small_find(s_key, o.row)] < me) s_waw[t] = 1;
+            continue;
+        }
+        bool raw = false;
+        for (uint32_t q = 0; q < o.len; ++q) {  // a scan: every row of its range
+            const int h = small_find(s_key, o.row + q);
+            if (h < 0) continue;
+            const unsigned long long w = s_min[h];
+            if (w < me)
+                raw = true;
+            else if (w > me)
+                s_war[w & kPosMask] = 1;
+        }
+        if (raw) s_raw[t] = 1;
+    }
+    __syncthreads();
+    // CHECK: commit bits, the bitmap and the list of committed transactions
+    const uint32_t nw = (B + 63) / 64;
+    for (uint32_t t0 = 0; t0 < B; t0 += kSmallBlock) {
+        const uint32_t t = t0 + threadIdx.x;
+        const bool c = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
+        const unsigned long long bits = __ballot(c);
+        if (c) s_list[atomicAdd(&s_n, 1u)] = t;
+        if (lane_id() == 0 && t / 64 < nw) {
+            bitmap[t / 64] = bits;
+            s_bits[t / 64] = bits;
+        }
+    }
+    __syncthreads();
+    const uint32_t nc = s_n;
+    // write folding, one wave per committed transaction (lane = slot): the
+    // first write slot of a transaction on a field writes word + count *
+    // delta, the others nothing
+    for (uint32_t j = threadIdx.x / 64; j < nc; j += kSmallBlock / 64) {
+        const uint32_t t = s_list[j], sl = lane_id();
+        const uint32_t i = sl < S ? (rm ? t * S + sl : sl * B + t) : 0u;
+        const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+        const bool wr = sl < S && o.valid && o.kind == LTPG_KV_WRITE;
+        const unsigned long long wm = __ballot(wr);
+        const uint64_t key = o.row * fields + o.field;
+        uint32_t count = 0;
+        bool first = true;
+        for (uint32_t k = 0; k < S; ++k) {
+            const uint64_t kk = __shfl(key, (int)k);
+            if (((wm >> k) & 1ull) && kk == key) {
+                ++count;
+                first &= k >= sl;
+            }
+        }
+        if (sl < S) s_fold[j * S + sl] = wr && first ? count : 0u;
+    }
+    __syncthreads();
+    // read-out of the committed reads and write-back of the committed
+    // writes.  Reads see the snapshot: when all items fit one round, every
+    // load is issued before a barrier and every store after it; otherwise
+    // the read-out runs in rounds, then a barrier, then the write-back.
+    const uint32_t per = L * fw, total_r = nc * S * per;
+    const uint32_t total_w = values == LTPG_VALUES_RMW ? nc * S * fw : 0u;
+    const auto read_item = [&](uint32_t x, uint64_t &src, uint64_t &dst) -> bool {
+        const uint32_t j = x / per, q = (x % per) / fw, w = x % fw;
+        const uint32_t t = s_list[j / S], sl = j % S;
+        const uint32_t i = rm ? t * S + sl : sl * B + t;
+        const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+        const bool ok = q < o.len;
+        src = ok ? ((o.row + q) * fields + o.field) * fsw + w : 0ull;
+        dst = ((((uint64_t)sl * B + t) * L) + q) * fw + w;
+        return ok;
+    };
+    const auto write_item = [&](uint32_t x, uint64_t &off, uint32_t &add) -> bool {
+        const uint32_t j = x / fw, w = x % fw;
+        const uint32_t count = s_fold[j];
+        const uint32_t t = s_list[j / S], sl = j % S;
+        const uint32_t i = rm ? t * S + sl : sl * B + t;
+        const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+        add = count * (s_tid[t] | 1u);
+        off = count ? (o.row * fields + o.field) * fsw + w : 0ull;
+        return count != 0u;
+    };
+    if (total_r + total_w <= kSmallMerge * kSmallBlock) {
+        uint64_t src[kSmallMerge], dst[kSmallMerge];
+        uint32_t v[kSmallMerge], add[kSmallMerge];
+        bool ok[kSmallMerge], rd[kSmallMerge];
+#pragma unroll
+        for (uint32_t u = 0; u < kSmallMerge; ++u) {
+            const uint32_t x = u * kSmallBlock + threadIdx.x;
+            rd[u] = x < total_r;
+            add[u] = 0u;
+            if (rd[u])
+                ok[u] = read_item(x, src[u], dst[u]);
+            else if (x < total_r + total_w)
+                ok[u] = write_item(x - total_r, src[u], add[u]);
+            else
+                ok[u] = false;
+            if (!ok[u]) src[u] = 0ull;
+            if (!rd[u]) dst[u] = src[u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kSmallMerge; ++u) v[u] = table[src[u]];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < kSmallMerge; ++u) {
+            if (!ok[u]) continue;
+            if (rd[u])
+                read_values[dst[u]] = v[u];
+            else
+                table[dst[u]] = v[u] + add[u];
+        }
+    } else {
+        for (uint32_t b0 = 0; b0 < total_r; b0 += kSmallUnroll * kSmallBlock) {
+            uint64_t src[kSmallUnroll], dst[kSmallUnroll];
+            bool ok[kSmallUnroll];
+            uint32_t v[kSmallUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u) {
+                const uint32_t x = b0 + u * kSmallBlock + threadIdx.x;
+                ok[u] = x < total_r && read_item(x, src[u], dst[u]);
+                if (!ok[u]) src[u] = 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u) v[u] = table[src[u]];
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u)
+                if (ok[u]) read_values[dst[u]] = v[u];
+        }
+        __syncthreads();
+        for (uint32_t b0 = 0; b0 < total_w; b0 += kSmallUnroll * kSmallBlock) {
+            uint64_t off[kSmallUnroll];
+            uint32_t add[kSmallUnroll], v[kSmallUnroll];
+            bool ok[kSmallUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u) {
+                const uint32_t x = b0 + u * kSmallBlock + threadIdx.x;
+                ok[u] = x < total_w && write_item(x, off[u], add[u]);
+                if (!ok[u]) off[u] = 0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u) v[u] = table[off[u]];
+#pragma unroll
+            for (uint32_t u = 0; u < kSmallUnroll; ++u)
+                if (ok[u]) table[off[u]] = v[u] + add[u];
+        }
+    }
+    if (threadIdx.x == 0) {
+        add_counts(ctr, 0, LTPG_TYPE_KV, nc, B);
+        if (s_bad) atomicAdd(&ctr->range_errors, s_bad);
+    }
+    if (seq_out) {
+        // one wave copies the bitmap to host memory, then releases seq at
+        // system scope: a host that reads seq with an acquire load reads the
+        // bitmap too
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            for (uint32_t w = threadIdx.x; w < nw; w += 64) host_bits[w] = s_bits[w];
+            __threadfence_system();
+            if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
 // Hot row: committed writes to it add their delta (TID | 1 per word) in any
 // order.  A block sums its transactions' deltas per field in LDS and adds
 // each non-zero sum to one of kCounterStripes accumulator rows; then
@@ -1218,6 +1505,22 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
         hipLaunchKernelGGL(kv_hot_apply_kernel, grid_for(kv.fields), dim3(kBlock), 0, s, kv.table, kv.hot_row,
                            kv.fields, kv.field_words, kv.field_stride, kv.hot_acc);
     }
+    return hipGetLastError();
+}
+
+bool kv_small_epoch_ok(const KvState &kv, const KvBatchDev &b)
+{
+    return b.n_txns > 0 && b.n_slots > 0 && (uint64_t)b.n_txns * b.n_slots <= kSmallOps &&
+           kv.hot_row == LTPG_NO_HOT_ROW && kv.rows < 0xFFFFFFFFull;
+}
+
+hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_t values, DeviceCounters *ctr,
+                                 uint64_t *host_bits, uint64_t *seq_out, uint64_t seq, hipStream_t s)
+{
+    if (!kv_small_epoch_ok(kv, b)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kv_small_epoch_kernel, dim3(1), dim3(kSmallBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
+                       b.n_slots, b.request_major, kv.rows, kv.fields, kv.field_words, kv.field_stride, kv.max_scan,
+                       kv.read_slot, values, kv.table, kv.read_values, kv.bitmap, host_bits, seq_out, seq, ctr);
     return hipGetLastError();
 }
 

@@ -94,6 +94,7 @@ struct KvState {
     uint32_t split_regs = 0;
     uint32_t small_chunks = 0;
     uint32_t lane_copy = 0;
+    uint32_t small_epoch = 0;           // epochs of <= LTPG_SMALL_EPOCH_OPS op slots in one kernel
     // hot row (LTPG_NO_HOT_ROW or a row): its ops never register or
     // conflict; committed writes add through hot_acc ([kCounterStripes][fields])
     uint64_t hot_row = LTPG_NO_HOT_ROW;
@@ -267,6 +268,12 @@ hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTa
 hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t s);
 hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
                                hipStream_t s);
+// A whole fast-form epoch of at most LTPG_SMALL_EPOCH_OPS op slots in one
+// single-workgroup kernel (no hot row, rows < 2^32 - 1).  With seq_out it
+// also publishes: the bitmap to host_bits, then seq with a release.
+bool kv_small_epoch_ok(const KvState &kv, const KvBatchDev &b);
+hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_t values, DeviceCounters *ctr,
+                                 uint64_t *host_bits, uint64_t *seq_out, uint64_t seq, hipStream_t s);
 // The current batch's epoch phase by phase, a stream synchronisation after
 // each (the reference's driver loop), then ltpg_sync.
 int run_epoch_stepwise(ltpg_ctx *ctx);

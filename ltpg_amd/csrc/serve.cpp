@@ -59,7 +59,7 @@ constexpr uint32_t kNoFields = 52;  // NewOrder request: 7 head fields + 3 x 15 
 constexpr uint32_t kPayFields = 9;
 constexpr uint32_t kOsFields = 5;    // Order-Status: W_ID, D_ID, C_ID, O_ID, OL_ID
 constexpr uint32_t kSlFields = 2;    // Stock-Level: W_ID, S_ID
-constexpr uint64_t kAdaptiveFloor = 256;  // smallest epoch the commit-based cap allows
+constexpr uint64_t kAdaptiveFloor = 32;  // default smallest epoch the commit-based cap allows
 
 struct Req {
     uint64_t id;
@@ -290,10 +290,11 @@ void take_members(ltpg_server *sv, int w, Flight &f)
     // Under contention an epoch commits a short chain of the lowest
     // priorities whatever its size, so a large epoch mostly re-runs aborts:
     // cap it at epoch_growth times the last epoch's commits (not below
-    // kAdaptiveFloor), so that epochs turn over faster.
+    // epoch_floor), so that epochs turn over faster.
     uint64_t cap = L.cap;
+    const uint64_t floor = sv->cfg.epoch_floor ? sv->cfg.epoch_floor : kAdaptiveFloor;
     if (sv->cfg.epoch_growth && L.last_commits != ~0ull)
-        cap = std::min<uint64_t>(cap, std::max<uint64_t>(kAdaptiveFloor, sv->cfg.epoch_growth * L.last_commits));
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(floor, sv->cfg.epoch_growth * L.last_commits));
     while (f.members.size() < cap && !L.requeue.empty()) {
         f.members.push_back(L.requeue.top());
         L.requeue.pop();
@@ -341,21 +342,40 @@ int launch(ltpg_server *sv, int w, Flight &f)
             std::memcpy(h_keys + (uint64_t)t * S, &sv->kv_keys[r.data * S], S * sizeof(uint64_t));
             std::memcpy(h_meta + (uint64_t)t * S, &sv->kv_meta[r.data * S], S * sizeof(uint32_t));
         }
-        LTPG_HIP_TRY(launch_copy_u32(reinterpret_cast<uint32_t *>(f.d_stage),
-                                     reinterpret_cast<const uint32_t *>(f.h_stage), (tids_at + 4ull * B) / 4,
-                                     ctx->stream));
-        ltpg_kv_batch b = {B, S, reinterpret_cast<const uint64_t *>(f.d_stage + keys_at),
-                           reinterpret_cast<const uint32_t *>(f.d_stage + meta_at),
-                           reinterpret_cast<const uint32_t *>(f.d_stage + tids_at)};
-        if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE | LTPG_BATCH_REQUEST_MAJOR))) return rc;
-        if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+        KvBatchDev shape;
+        shape.n_txns = B;
+        shape.n_slots = S;
+        const bool small = !(ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS) && kv_small_epoch_ok(ctx->kv, shape);
+        bool published = false;
+        if (small) {
+            // one single-workgroup kernel reads the staging in place and,
+            // unless read values follow, publishes the epoch itself
+            ltpg_kv_batch b = {B, S, h_keys, h_meta, h_tids};
+            if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE | LTPG_BATCH_REQUEST_MAJOR))) return rc;
+            published = !sv->cfg.keep_reads;
+            const uint64_t seq = published ? ++f.seq : 0;
+            LTPG_HIP_TRY(launch_kv_small_epoch(ctx->kv, ctx->kv.cur, ctx->cfg.values, ctx->counters,
+                                               published ? f.h_bits : nullptr, published ? f.h_seq : nullptr, seq,
+                                               ctx->stream));
+        } else {
+            LTPG_HIP_TRY(launch_copy_u32(reinterpret_cast<uint32_t *>(f.d_stage),
+                                         reinterpret_cast<const uint32_t *>(f.h_stage), (tids_at + 4ull * B) / 4,
+                                         ctx->stream));
+            ltpg_kv_batch b = {B, S, reinterpret_cast<const uint64_t *>(f.d_stage + keys_at),
+                               reinterpret_cast<const uint32_t *>(f.d_stage + meta_at),
+                               reinterpret_cast<const uint32_t *>(f.d_stage + tids_at)};
+            if ((rc = ltpg_kv_submit(ctx, &b, LTPG_MEM_DEVICE | LTPG_BATCH_REQUEST_MAJOR))) return rc;
+            if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
+        }
         if (sv->cfg.keep_reads)
             LTPG_HIP_TRY(hipMemcpyAsync(f.h_reads, ctx->kv.read_values,
                                         (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words * 4,
                                         hipMemcpyDeviceToHost, ctx->stream));
-        PublishSet ps{{ctx->kv.bitmap, nullptr, nullptr, nullptr}, {f.h_bits, nullptr, nullptr, nullptr},
-                      {(uint32_t)words(B), 0, 0, 0}};
-        LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
+        if (!published) {
+            PublishSet ps{{ctx->kv.bitmap, nullptr, nullptr, nullptr}, {f.h_bits, nullptr, nullptr, nullptr},
+                          {(uint32_t)words(B), 0, 0, 0}};
+            LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
+        }
     } else {
         const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl;
         pack_tpcc(sv, f, C);
@@ -674,6 +694,7 @@ void ltpg_serve_config_init(ltpg_serve_config *c)
     c->slots = 2;
     c->kv_ops = 16;
     c->epoch_growth = 8;
+    c->epoch_floor = (uint32_t)kAdaptiveFloor;
 }
 
 int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped)

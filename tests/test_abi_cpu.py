@@ -57,6 +57,13 @@ def test_config_struct_layout_matches_c():
     assert ctypes.sizeof(L.KvBatch) == 32
 
 
+def test_serve_config_defaults_match_c():
+    cfg = L.ServeConfig()
+    L.lib.ltpg_serve_config_init(ctypes.byref(cfg))
+    assert cfg.size == ctypes.sizeof(L.ServeConfig) == 48
+    assert (cfg.slots, cfg.kv_ops, cfg.epoch_growth, cfg.epoch_floor, cfg.plain) == (2, 16, 8, 32, 0)
+
+
 @pytest.mark.skipif(L.device_count() > 0, reason="only meaningful without a GPU")
 def test_create_without_gpu_reports_device_error():
     with pytest.raises(L.LtpgError) as ei:

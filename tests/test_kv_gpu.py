@@ -67,8 +67,9 @@ def _assert_epoch(g, ora, meta, tids, fw):
 
 
 @pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
-@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
-def test_gpu_known_answers(gpu, case, counts):
+@pytest.mark.parametrize("form", ["counting", "fast", "small-epoch"])
+def test_gpu_known_answers(gpu, case, form):
+    counts = form == "counting"
     rows, fields, fw = case["rows"], case["fields"], case["field_words"]
     keys, meta, tids, S = kc.case_arrays(case)
     values = L.VALUES_RMW if case["values"] == "rmw" else L.VALUES_COPYBACK
@@ -78,6 +79,8 @@ def test_gpu_known_answers(gpu, case, counts):
     Ls = max(1, max_scan)
     with _ctx(rows, fields, fw, max(B, 1), S, values, counts=counts, hot_row=hot, max_scan=max_scan) as ctx:
         ctx.kv_table_load(kc.initial_table(rows, fields, fw))
+        if form == "small-epoch":
+            ctx.kv_set_tuning(L.TUNE_SMALL_EPOCH)
         ctx.kv_submit(keys, meta, tids, S)
         ctx.epoch_run(L.PHASE_ALL)
         exp = case["expect"]
@@ -226,6 +229,47 @@ def test_gpu_scans_match_oracle(gpu, counts, flags, fw, rm):
             np.testing.assert_array_equal(g["table"], ora.table)
             _assert_scan_reads(g["reads"], ora, keys, meta, B, S, rows, fw, max_scan)
             assert 0 < ora.commit_count < B
+            table = ora.table
+
+
+SMALL_CASES = [
+    # rows, fields, fw, B, S, theta, read_pct, min_ops, seed, scan_len, max_scan
+    (1000, 4, 1, 256, 8, 0.9, 50, 8, 1, 0, 0),          # exactly LTPG_SMALL_EPOCH_OPS slots
+    (50_000, 10, 25, 32, 64, 0.99, 50, 1, 2, 0, 0),     # 1-64 ops, 100-byte fields
+    (4096, 3, 7, 100, 20, 0.8, 30, 1, 4, 0, 0),         # odd field width
+    (100_000, 2, 80, 20, 5, 0.5, 70, 2, 5, 0, 0),       # fields wider than 64 words
+    (100, 2, 1, 65, 3, 0.0, 50, 3, 6, 0, 0),            # uniform keys
+    (5000, 3, 2, 150, 12, 0.7, 90, 1, 7, 10, 16),       # scans of 10 rows, read slots of 16
+    (200, 2, 1, 2048, 1, 0.9, 50, 1, 8, 0, 0),          # one op per transaction
+]
+
+
+@pytest.mark.parametrize("rows,fields,fw,B,S,theta,read_pct,min_ops,seed,scan_len,max_scan", SMALL_CASES)
+@pytest.mark.parametrize("values", [L.VALUES_RMW, L.VALUES_COPYBACK], ids=["rmw", "copyback"])
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
+def test_gpu_small_epochs_match_oracle(gpu, rows, fields, fw, B, S, theta, read_pct, min_ops, seed, scan_len,
+                                       max_scan, values, rm):
+    """TUNE_SMALL_EPOCH: epochs of at most LTPG_SMALL_EPOCH_OPS slots run as
+    one single-workgroup kernel; an epoch one transaction over the limit
+    between them runs the multi-kernel path on the same context.  Every
+    epoch bit-exact against the oracle (bitmap, count, committed reads,
+    table)."""
+    big = L.SMALL_EPOCH_OPS // S + 1
+    table = np.random.default_rng(seed).integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64)
+    table = table.astype(np.uint32)
+    with _ctx(rows, fields, fw, max(B, big), S, values, counts=False, max_scan=max_scan) as ctx:
+        ctx.kv_table_load(table)
+        ctx.kv_set_tuning(L.TUNE_SMALL_EPOCH)
+        for ep, n in enumerate([B, big, B, B]):
+            keys, meta, tids = L.kv_generate(seed=seed * 10 + ep, rows=rows, fields=fields, n_txns=n, n_slots=S,
+                                             min_ops=min_ops, max_ops=S, read_pct=read_pct, theta=theta,
+                                             scan_len=scan_len, tid_base=ep * 5000)
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=values, max_scan=max_scan)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
+            np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
+            assert g["count"] == ora.commit_count
+            np.testing.assert_array_equal(g["table"], ora.table)
+            _assert_scan_reads(g["reads"], ora, keys, meta, n, S, rows, fw, max_scan)
             table = ora.table
 
 
