@@ -392,7 +392,8 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   for large ones.
  * SMALL_EPOCH: an epoch of at most LTPG_SMALL_EPOCH_OPS op slots (n_txns *
  *   n_slots) run with LTPG_PHASE_ALL, in the fast form (no
- *   LTPG_FLAG_CONFLICT_COUNTS), without a hot row and without timing, runs
+ *   LTPG_FLAG_CONFLICT_COUNTS), without timing, and with a hot row only if
+ *   the table has at most 1024 fields, runs
  *   as one single-workgroup kernel: ops, flags and the W registrations stay
  *   in LDS and the per-row registers in HBM are not touched.  One launch
  *   instead of four or five; other epochs run as before.  Not one of the

@@ -919,13 +919,17 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
 // which this path never touches.  The batch may sit in pinned host memory
 // (read once).  With seq_out the commit bitmap is also written to host_bits
 // and then seq is stored there with a system-scope release, as
-// publish_kernel does after a multi-kernel epoch.  No hot row; rows < 2^32 - 1.
+// publish_kernel does after a multi-kernel epoch.  A hot row (at most
+// kSmallHotFields fields) never registers or conflicts, and its committed
+// deltas are summed per field in LDS and added to the row with the other
+// writes; rows < 2^32 - 1.
 constexpr uint32_t kSmallBlock = 1024;
 constexpr uint32_t kSmallOps = LTPG_SMALL_EPOCH_OPS;
 constexpr uint32_t kSmallPer = kSmallOps / kSmallBlock;  // op slots per thread
 constexpr uint32_t kSmallSlots = 2 * kSmallOps;          // hash load <= 1/2: probing always ends
 constexpr uint32_t kSmallUnroll = 4;                     // field words in flight per thread
 constexpr uint32_t kSmallMerge = 8;                      // items per thread of the one-round copy
+constexpr uint32_t kSmallHotFields = 1024;               // fields of a table with a hot row
 
 __device__ __forceinline__ uint32_t small_hash(uint64_t row)
 {
@@ -946,7 +950,7 @@ __device__ __forceinline__ int small_find(const uint32_t *s_key, uint64_t row)
 __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta, const uint32_t *__restrict__ tids,
     uint32_t B, uint32_t S, bool rm, uint64_t rows, uint32_t fields, uint32_t fw, uint32_t fsw, uint32_t max_scan,
-    uint32_t L, uint32_t values, uint32_t *__restrict__ table, uint32_t *__restrict__ read_values,
+    uint32_t L, uint32_t values, uint64_t hot, uint32_t *__restrict__ table, uint32_t *__restrict__ read_values,
     uint64_t *__restrict__ bitmap, uint64_t *__restrict__ host_bits, uint64_t *seq_out, uint64_t seq,
     DeviceCounters *__restrict__ ctr)
 {
@@ -959,6 +963,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
     __shared__ uint32_t s_list[kSmallOps];             // committed transactions
     __shared__ uint32_t s_fold[kSmallOps];             // per committed (txn, slot): folded write count
     __shared__ unsigned long long s_bits[kSmallOps / 64];  // the commit bitmap
+    __shared__ uint32_t s_hot[kSmallHotFields];        // committed hot-row deltas per field
     __shared__ uint32_t s_n, s_bad;
     const uint32_t n = B * S;
     // the batch: every load issued before the first use
@@ -975,6 +980,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
         s_key[k] = 0u;
         s_min[k] = ~0ull;
     }
+    const bool has_hot = hot < rows;
+    if (has_hot)
+        for (uint32_t f = threadIdx.x; f < fields; f += kSmallBlock) s_hot[f] = 0u;
     // the epoch's commit counts start at zero (thread 0 adds to them last)
     for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kSmallBlock)
         ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
@@ -1007,7 +1015,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
         if (i >= n) continue;
         const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
         bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
-        if (!o.valid || o.kind != LTPG_KV_WRITE) continue;
+        if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot) continue;  // hot-row writes never register
         const uint32_t t = txn_of(i, B, S, rm);
         const uint32_t key = (uint32_t)o.row + 1u;
         uint32_t h = small_hash(o.row);
@@ -1031,11 +1039,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
         const uint32_t t = txn_of(i, B, S, rm);
         const unsigned long long me = ((unsigned long long)s_tid[t] << kTidShift) | t;
         if (o.kind == LTPG_KV_WRITE) {
-            if (s_min[small_find(s_key, o.row)] < me) s_waw[t] = 1;
+            if (o.row != hot && s_min[small_find(s_key, o.row)] < me) s_waw[t] = 1;
             continue;
         }
         bool raw = false;
         for (uint32_t q = 0; q < o.len; ++q) {  // a scan: every row of its range
+            if (o.row + q == hot) continue;  // the hot row never conflicts
             const int h = small_find(s_key, o.row + q);
             if (h < 0) continue;
             const unsigned long long w = s_min[h];
@@ -1063,12 +1072,15 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
     const uint32_t nc = s_n;
     // write folding, one wave per committed transaction (lane = slot): the
     // first write slot of a transaction on a field writes word + count *
-    // delta, the others nothing
+    // delta, the others nothing.  Hot-row writes add their delta to the
+    // field's sum instead (kv_hot_sum_kernel's reduce).
     for (uint32_t j = threadIdx.x / 64; j < nc; j += kSmallBlock / 64) {
         const uint32_t t = s_list[j], sl = lane_id();
         const uint32_t i = sl < S ? (rm ? t * S + sl : sl * B + t) : 0u;
         const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
-        const bool wr = sl < S && o.valid && o.kind == LTPG_KV_WRITE;
+        const bool any_wr = sl < S && o.valid && o.kind == LTPG_KV_WRITE;
+        if (any_wr && o.row == hot && values == LTPG_VALUES_RMW) atomicAdd(&s_hot[o.field], s_tid[t] | 1u);
+        const bool wr = any_wr && o.row != hot;
         const unsigned long long wm = __ballot(wr);
         const uint64_t key = o.row * fields + o.field;
         uint32_t count = 0;
@@ -1089,6 +1101,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
     // the read-out runs in rounds, then a barrier, then the write-back.
     const uint32_t per = L * fw, total_r = nc * S * per;
     const uint32_t total_w = values == LTPG_VALUES_RMW ? nc * S * fw : 0u;
+    const uint32_t total_h = values == LTPG_VALUES_RMW && has_hot ? fields * fw : 0u;
     const auto read_item = [&](uint32_t x, uint64_t &src, uint64_t &dst) -> bool {
         const uint32_t j = x / per, q = (x % per) / fw, w = x % fw;
         const uint32_t t = s_list[j / S], sl = j % S;
@@ -1100,6 +1113,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
         return ok;
     };
     const auto write_item = [&](uint32_t x, uint64_t &off, uint32_t &add) -> bool {
+        if (x >= total_w) {  // a word of the hot row: its field's sum
+            const uint32_t f = (x - total_w) / fw, w = (x - total_w) % fw;
+            add = s_hot[f];
+            off = add ? (hot * fields + f) * fsw + w : 0ull;
+            return add != 0u;
+        }
         const uint32_t j = x / fw, w = x % fw;
         const uint32_t count = s_fold[j];
         const uint32_t t = s_list[j / S], sl = j % S;
@@ -1109,7 +1128,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
         off = count ? (o.row * fields + o.field) * fsw + w : 0ull;
         return count != 0u;
     };
-    if (total_r + total_w <= kSmallMerge * kSmallBlock) {
+    if (total_r + total_w + total_h <= kSmallMerge * kSmallBlock) {
         uint64_t src[kSmallMerge], dst[kSmallMerge];
         uint32_t v[kSmallMerge], add[kSmallMerge];
         bool ok[kSmallMerge], rd[kSmallMerge];
@@ -1120,7 +1139,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
             add[u] = 0u;
             if (rd[u])
                 ok[u] = read_item(x, src[u], dst[u]);
-            else if (x < total_r + total_w)
+            else if (x < total_r + total_w + total_h)
                 ok[u] = write_item(x - total_r, src[u], add[u]);
             else
                 ok[u] = false;
@@ -1156,14 +1175,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_small_epoch_kernel(
                 if (ok[u]) read_values[dst[u]] = v[u];
         }
         __syncthreads();
-        for (uint32_t b0 = 0; b0 < total_w; b0 += kSmallUnroll * kSmallBlock) {
+        for (uint32_t b0 = 0; b0 < total_w + total_h; b0 += kSmallUnroll * kSmallBlock) {
             uint64_t off[kSmallUnroll];
             uint32_t add[kSmallUnroll], v[kSmallUnroll];
             bool ok[kSmallUnroll];
 #pragma unroll
             for (uint32_t u = 0; u < kSmallUnroll; ++u) {
                 const uint32_t x = b0 + u * kSmallBlock + threadIdx.x;
-                ok[u] = x < total_w && write_item(x, off[u], add[u]);
+                ok[u] = x < total_w + total_h && write_item(x, off[u], add[u]);
                 if (!ok[u]) off[u] = 0ull;
             }
 #pragma unroll
@@ -1511,7 +1530,7 @@ hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t 
 bool kv_small_epoch_ok(const KvState &kv, const KvBatchDev &b)
 {
     return b.n_txns > 0 && b.n_slots > 0 && (uint64_t)b.n_txns * b.n_slots <= kSmallOps &&
-           kv.hot_row == LTPG_NO_HOT_ROW && kv.rows < 0xFFFFFFFFull;
+           (kv.hot_row == LTPG_NO_HOT_ROW || kv.fields <= kSmallHotFields) && kv.rows < 0xFFFFFFFFull;
 }
 
 hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_t values, DeviceCounters *ctr,
@@ -1520,7 +1539,8 @@ hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_
     if (!kv_small_epoch_ok(kv, b)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kv_small_epoch_kernel, dim3(1), dim3(kSmallBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                        b.n_slots, b.request_major, kv.rows, kv.fields, kv.field_words, kv.field_stride, kv.max_scan,
-                       kv.read_slot, values, kv.table, kv.read_values, kv.bitmap, host_bits, seq_out, seq, ctr);
+                       kv.read_slot, values, kv.hot_row, kv.table, kv.read_values, kv.bitmap, host_bits, seq_out, seq,
+                       ctr);
     return hipGetLastError();
 }
 

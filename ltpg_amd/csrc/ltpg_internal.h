@@ -269,7 +269,8 @@ hipError_t launch_kv_readout(const KvState &kv, const KvBatchDev &b, hipStream_t
 hipError_t launch_kv_writeback(const KvState &kv, const KvBatchDev &b, uint32_t values,
                                hipStream_t s);
 // A whole fast-form epoch of at most LTPG_SMALL_EPOCH_OPS op slots in one
-// single-workgroup kernel (no hot row, rows < 2^32 - 1).  With seq_out it
+// single-workgroup kernel (a hot row only with at most 1024 fields; rows <
+// 2^32 - 1).  With seq_out it
 // also publishes: the bitmap to host_bits, then seq with a release.
 bool kv_small_epoch_ok(const KvState &kv, const KvBatchDev &b);
 hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_t values, DeviceCounters *ctr,

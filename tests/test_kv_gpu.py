@@ -188,6 +188,38 @@ def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts, rm):
             table = ora.table
 
 
+@pytest.mark.parametrize("values", [L.VALUES_RMW, L.VALUES_COPYBACK], ids=["rmw", "copyback"])
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
+@pytest.mark.parametrize("hot_row", [0, 17])
+def test_gpu_small_epochs_with_hot_row_match_oracle(gpu, hot_row, rm, values):
+    """The single-workgroup epoch with a hot row: its ops never register or
+    conflict, committed writes to it add (TID|1) per word; 100-byte fields,
+    Zipf 0.99 (row 0 in most transactions), scans over the hot row, epochs
+    of 32 x 64 slots in place with a larger one in between."""
+    rows, fields, fw, S, max_scan = 20_000, 10, 25, 64, 4
+    rng = np.random.default_rng(hot_row + 11)
+    table = rng.integers(0, 2**32, size=rows * fields * fw, dtype=np.uint64).astype(np.uint32)
+    with _ctx(rows, fields, fw, 64, S, values, counts=False, hot_row=hot_row, max_scan=max_scan) as ctx:
+        ctx.kv_table_load(table)
+        ctx.kv_set_tuning(L.TUNE_SMALL_EPOCH)
+        for ep, n in enumerate([32, 32, 64, 20]):
+            keys, meta, tids = L.kv_generate(seed=90 + ep, rows=rows, fields=fields, n_txns=n, n_slots=S,
+                                             min_ops=1, max_ops=S, theta=0.99, tid_base=ep * 100,
+                                             scan_len=3 if ep == 3 else 0)
+            if hot_row:  # put the hot row into some slots
+                keys = np.where((np.arange(keys.size) % 7) == 0, hot_row, keys).astype(np.uint64)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=values, hot_row=hot_row,
+                                  max_scan=max_scan)
+            np.testing.assert_array_equal(g["bitmap"], ora.bitmap)
+            assert g["count"] == ora.commit_count
+            np.testing.assert_array_equal(g["table"], ora.table)
+            _assert_scan_reads(g["reads"], ora, keys, meta, n, S, rows, fw, max_scan)
+            hot_ops = (keys.reshape(S, n) == hot_row) & ((meta.reshape(S, n) & 3) != 0)
+            assert hot_ops.any()
+            table = ora.table
+
+
 def _assert_scan_reads(gpu_reads, ora, keys, meta, B, S, rows, fw, max_scan):
     """Delivered rows of committed reads and scans match the oracle's."""
     L_ = max(1, max_scan)
