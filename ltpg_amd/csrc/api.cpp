@@ -206,13 +206,24 @@ int refill_registers(ltpg_ctx *ctx)
 namespace {
 
 // Advance the epoch (1..255).  At wrap-around every register goes back to
-// all-ones (the tag of epoch 0, never current) and the doomed flags to 0.
+// all-ones (the tag of epoch 0, never current) and the doomed flags to 0 --
+// unless no key-value kernel has written them since the last refill ran
+// (only single-workgroup epochs, which keep their registrations in LDS).
+// A refill recorded into a graph capture has not run yet, so it leaves the
+// registers marked as written.
 int next_epoch(ltpg_ctx *ctx)
 {
     ctx->epoch += 1;
     if (ctx->epoch > 255) {
-        int rc = refill_registers(ctx);
-        if (rc) return rc;
+        KvState &kv = ctx->kv;
+        const bool skip = kv.regs && !ctx->tpcc.d.regs && kv.regs_clean;
+        if (!skip) {
+            int rc = refill_registers(ctx);
+            if (rc) return rc;
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            LTPG_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cs));
+            kv.regs_clean = cs == hipStreamCaptureStatusNone;
+        }
         ctx->epoch = 1;
     }
     return LTPG_OK;
@@ -744,6 +755,7 @@ int ltpg_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
         LTPG_HIP_TRY(launch_kv_small_epoch(kv, kv.cur, ctx->cfg.values, ctx->counters, nullptr, nullptr, 0, s));
         return LTPG_OK;
     }
+    kv.regs_clean = false;  // the phase kernels below write registers and flags
     hipEvent_t *ev = nullptr;
     if (ctx->timing) {
         if ((rc = take_events(ctx, &ev))) return rc;

@@ -184,6 +184,7 @@ int enqueue_kv_pipelined(ltpg_batched *b)
 int capture(ltpg_batched *b, int k)
 {
     ltpg_ctx *ctx = b->ctx[k];
+    ctx->kv.regs_clean = false;  // replays write registers outside ltpg_epoch_run
     const bool timing = ctx->timing;
     const uint32_t epoch = ctx->epoch;  // capture runs nothing: the device state keeps this epoch
     ctx->timing = false;
@@ -334,6 +335,7 @@ int ltpg_batched_run(ltpg_batched *b, uint32_t mode)
         LTPG_HIP_TRY(hipGraphLaunch(b->exec[k], b->ctx[k]->stream));
         // the registers now carry the captured sequence's epoch tags
         b->ctx[k]->epoch = b->final_epoch[k];
+        b->ctx[k]->kv.regs_clean = false;
     }
     int first = LTPG_OK;
     for (int k = 0; k < 2; k++) {
@@ -355,6 +357,7 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     if (!ctx) return LTPG_OK;  // no key-value batches: nothing to tune
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    ctx->kv.regs_clean = false;
     if (reps == 0) reps = 1;
     // size classes and their first batch
     std::vector<int> rep(33, -1);

@@ -95,6 +95,9 @@ struct KvState {
     uint32_t small_chunks = 0;
     uint32_t lane_copy = 0;
     uint32_t small_epoch = 0;           // epochs of <= LTPG_SMALL_EPOCH_OPS op slots in one kernel
+    // no kernel has written regs / doomed / raw_flag / war_flag since the
+    // last refill ran (the next epoch-tag wrap may skip the refill)
+    bool regs_clean = false;
     // hot row (LTPG_NO_HOT_ROW or a row): its ops never register or
     // conflict; committed writes add through hot_acc ([kCounterStripes][fields])
     uint64_t hot_row = LTPG_NO_HOT_ROW;

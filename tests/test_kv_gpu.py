@@ -188,6 +188,31 @@ def test_gpu_hot_row_reduce_matches_oracle(gpu, hot_row, counts, rm):
             table = ora.table
 
 
+@pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
+def test_gpu_small_epochs_across_tag_wraps(gpu, rm):
+    """810 epochs through three epoch-tag wraps: single-workgroup epochs
+    throughout, and separate-kernel epochs at the same tag in three cycles
+    (submits 45, 300 and 810) with one cycle that has none.  The wrap
+    refill, skipped when only single-workgroup epochs ran since the last
+    one, never leaves an older epoch's register looking current.  Every
+    epoch against the oracle."""
+    rows, fields, fw, S = 300, 2, 1, 8
+    table = kc.initial_table(rows, fields, fw)
+    big_at = {45, 300, 810}
+    with _ctx(rows, fields, fw, 400, S, counts=False) as ctx:
+        ctx.kv_table_load(table)
+        ctx.kv_set_tuning(L.TUNE_SMALL_EPOCH)
+        for k in range(1, 811):
+            n = 400 if k in big_at else 24
+            keys, meta, tids = L.kv_generate(seed=1000 + k, rows=rows, fields=fields, n_txns=n, n_slots=S,
+                                             min_ops=1, max_ops=S, theta=0.8, tid_base=k * 1000)
+            g = _gpu_epoch(ctx, keys, meta, tids, S, rm=rm)
+            ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, values=1)
+            np.testing.assert_array_equal(g["bitmap"], ora.bitmap, err_msg=f"submit {k}")
+            np.testing.assert_array_equal(g["table"], ora.table, err_msg=f"submit {k}")
+            table = ora.table
+
+
 @pytest.mark.parametrize("values", [L.VALUES_RMW, L.VALUES_COPYBACK], ids=["rmw", "copyback"])
 @pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("hot_row", [0, 17])
