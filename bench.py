@@ -677,9 +677,10 @@ def run_served_c5(args):
                    max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW)
     kv.kv_fill_reference()
     # served epochs are small (the server sizes them by their commits under
-    # contention): per-op registration with row folding, 64-transaction
-    # chunks and per-lane field copies, the fastest shape for epochs of
-    # 64-1024 transactions of this table
+    # contention): those of at most 2048 op slots run as one single-workgroup
+    # kernel; for larger ones per-op registration with row folding,
+    # 64-transaction chunks and per-lane field copies, the fastest shape for
+    # epochs of 64-1024 transactions of this table
     kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY)
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=1 << 20, values=L.VALUES_RMW)
     tp.tpcc_populate(seed)
@@ -740,7 +741,8 @@ def run_served_c5(args):
            "latency_ms": best["latency_ms"], "rates": out,
            "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
                       "slots": 2, "epoch_growth": 8, "epoch_floor": args.c5_epoch_floor,
-                      "kv_launch_shape": "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
+                      "kv_launch_shape": "epochs of <= 2048 op slots: one single-workgroup kernel; larger: "
+                                         "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
                       "form": "served (continuous admission, requeue with original TID)"}}
     if plain_out:
         # the plain per-batch form on the same arrivals and admission: one
