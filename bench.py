@@ -656,6 +656,54 @@ def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain):
             "attempts_per_commit": sum(st["attempts"]) / max(sum(st["committed"]), 1)}
 
 
+def run_kv_small(args):
+    """Small epochs in the plain form: the reference's loop (submit, every
+    phase, synchronise, read the commit count) over 2,000 epochs of 32
+    transactions x 1-64 ops (Zipf 0.99) on the 10M-row x 10 x 100 B table,
+    once with the separate phase kernels (the served form's shape for larger
+    epochs) and once with TUNE_SMALL_EPOCH (one single-workgroup kernel per
+    epoch).  Both runs start from the same table and commit the same
+    transactions."""
+    import ltpg_amd as L
+
+    rows, fields, fw, B, S, n_ep = 10_000_000, 10, 25, 32, 64, 2000
+    keys, meta, tids = L.kv_generate(seed=0x4C545047, rows=rows, fields=fields, n_txns=B * n_ep, n_slots=S,
+                                     min_ops=1, max_ops=S, theta=0.99)
+    k3, m3, t2 = keys.reshape(S, n_ep, B), meta.reshape(S, n_ep, B), tids.reshape(n_ep, B)
+    desc = (f"key-value plain form, small epochs: {n_ep} epochs of {B} txns x 1-{S} ops, Zipf 0.99, "
+            f"{rows // 1_000_000}M rows x {fields} x 100 B; a synchronisation and the commit count after each")
+    log(f"small epochs: {desc}")
+    res = {}
+    for name, flags in (("separate_kernels", L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY),
+                        ("single_workgroup", L.TUNE_SMALL_EPOCH)):
+        ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw, max_txns=B,
+                        kv_max_slots=S, values=L.VALUES_RMW)
+        ctx.kv_fill_reference()
+        ctx.kv_set_tuning(flags)
+        batches = [L.DeviceKvBatch(ctx, np.ascontiguousarray(k3[:, e, :]).reshape(-1),
+                                   np.ascontiguousarray(m3[:, e, :]).reshape(-1), t2[e], S) for e in range(n_ep)]
+        ctx.sync()
+        committed, t0 = 0, time.perf_counter()
+        for b in batches:
+            ctx.kv_submit_device(b)
+            ctx.epoch_run(L.PHASE_ALL)
+            ctx.sync()
+            committed += ctx.commit_count()
+        dt = time.perf_counter() - t0
+        res[name] = {"ms_per_epoch": dt * 1e3 / n_ep, "committed_txn_per_s": committed / dt, "committed": committed}
+        log(f"small epochs, {name}: {dt * 1e6 / n_ep:.1f} us per epoch, {committed / dt / 1e3:.1f}K committed/s")
+        for b in batches:
+            b.free()
+        ctx.close()
+    assert res["separate_kernels"]["committed"] == res["single_workgroup"]["committed"]
+    v = res["single_workgroup"]
+    return {"metric": METRIC, "value": v["committed_txn_per_s"], "unit": "txn/s", "n_gpus": 1, "steps": n_ep,
+            "warmup": 0, "ms_per_step": v["ms_per_epoch"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32", "data": "synthetic", "forms": res,
+            "speedup_vs_separate_kernels": v["committed_txn_per_s"] / res["separate_kernels"]["committed_txn_per_s"],
+            "config": {"workload": desc}}
+
+
 def run_served_c5(args):
     """Served form under Poisson arrivals: a 50/50 mix of key-value requests
     (1-64 ops, Zipf 0.99, 100M rows x 10 x 100 B) and order-entry requests
@@ -823,7 +871,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="kv-c2",
-                    choices=["kv-c2", "kv-c1", "kv-scan", "tpcc-c3", "tpcc-mix", "batched-c4", "served-c5"])
+                    choices=["kv-c2", "kv-c1", "kv-scan", "kv-small", "tpcc-c3", "tpcc-mix", "batched-c4",
+                             "served-c5"])
     ap.add_argument("--c5-rates", type=float, nargs="+", default=[1, 5, 10, 25, 50],
                     help="offered loads of the served form, M txn/s")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
@@ -865,6 +914,8 @@ def main():
         result = run_tpcc(args)
     elif args.workload == "tpcc-mix":
         result = run_tpcc(args, mix=True)
+    elif args.workload == "kv-small":
+        result = run_kv_small(args)
     elif args.workload == "served-c5":
         v = run_served_c5(args)
         result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True,
