@@ -292,9 +292,9 @@ void take_members(ltpg_server *sv, int w, Flight &f)
     // cap it at epoch_growth times the last epoch's commits (not below
     // epoch_floor), so that epochs turn over faster.
     uint64_t cap = L.cap;
-    const uint64_t floor = sv->cfg.epoch_floor ? sv->cfg.epoch_floor : kAdaptiveFloor;
+    const uint64_t min_epoch = sv->cfg.epoch_floor ? sv->cfg.epoch_floor : kAdaptiveFloor;
     if (sv->cfg.epoch_growth && L.last_commits != ~0ull)
-        cap = std::min<uint64_t>(cap, std::max<uint64_t>(floor, sv->cfg.epoch_growth * L.last_commits));
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(min_epoch, sv->cfg.epoch_growth * L.last_commits));
     while (f.members.size() < cap && !L.requeue.empty()) {
         f.members.push_back(L.requeue.top());
         L.requeue.pop();
