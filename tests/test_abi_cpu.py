@@ -57,6 +57,20 @@ def test_config_struct_layout_matches_c():
     assert ctypes.sizeof(L.KvBatch) == 32
 
 
+def test_python_constants_match_the_header():
+    """Every integer #define LTPG_X of include/ltpg.h that the Python binding
+    mirrors as X has the header's value."""
+    text = open(os.path.join(ROOT, "include", "ltpg.h")).read()
+    defines = dict(re.findall(r"#define LTPG_([A-Z0-9_]+) (0x[0-9a-fA-F]+|\d+)u?\b", text))
+    checked = set()
+    for name, value in defines.items():
+        if hasattr(L, name):
+            assert getattr(L, name) == int(value, 0), name
+            checked.add(name)
+    assert {"TUNE_ALL", "TUNE_SMALL_EPOCH", "SMALL_EPOCH_OPS", "BATCH_REQUEST_MAJOR"} <= checked
+    assert len(checked) >= 20
+
+
 def test_serve_config_defaults_match_c():
     cfg = L.ServeConfig()
     L.lib.ltpg_serve_config_init(ctypes.byref(cfg))
