@@ -553,14 +553,32 @@ typedef struct ltpg_serve_config {
                                  (0 = 32).  Default 32: with 64-op requests an
                                  epoch of 32 fits one single-workgroup kernel
                                  (LTPG_SMALL_EPOCH_OPS) */
+    uint32_t kv_loop;         /* 1: key-value epochs form and run on the device
+                                 in one persistent single-workgroup kernel
+                                 (LTPG_SMALL_EPOCH_OPS op slots per epoch, fast
+                                 form); requests reach it through a ring in
+                                 pinned host memory and aborted ones stay on
+                                 the device.  0: the host forms each epoch.
+                                 Ignored with `plain` or conflict counts.
+                                 Default 1. */
+    uint32_t ring_requests;   /* key-value request ring of the device loop, in
+                                 requests (rounded up to a power of two; 0 =
+                                 65536); a submit waits while it is full */
+    uint64_t first_seq;       /* arrival sequence number of each workload's
+                                 first request (default 0) */
 } ltpg_serve_config;
 
+/* Priorities of the served form.  Each workload numbers its requests in
+ * arrival order with a 64-bit sequence number that never wraps; an epoch's
+ * priorities are its members' ranks in that order, so a re-queued request
+ * always precedes every later arrival.  A key-value write adds
+ * (low 32 bits of the sequence number) | 1 to each word of its field. */
 typedef struct ltpg_serve_done {
     uint64_t id;              /* request id (submission order, both workloads)   */
     uint32_t type;            /* LTPG_TYPE_KV / NEWORDER / PAYMENT               */
-    uint32_t tid;             /* priority, assigned at arrival, kept on requeue  */
+    uint32_t tid;             /* low 32 bits of the arrival sequence number      */
     uint32_t attempts;        /* epochs the request took part in                */
-    uint32_t reserved;
+    uint32_t tid_hi;          /* high 32 bits of the arrival sequence number     */
     uint64_t epoch;           /* serving epoch (per workload) it committed in    */
     uint64_t arrival_ns;      /* steady clock at submission                     */
     uint64_t commit_ns;       /* steady clock when its epoch's result was read  */

@@ -681,8 +681,8 @@ class Server:
         out = (ServeDone * max_n)()
         n = ctypes.c_uint32()
         _check(lib.ltpg_serve_poll(self.handle, out, max_n, ctypes.byref(n)), "ltpg_serve_poll")
-        return [{"id": d.id, "type": d.type, "tid": d.tid, "attempts": d.attempts, "epoch": d.epoch,
-                 "arrival_ns": d.arrival_ns, "commit_ns": d.commit_ns} for d in out[:n.value]]
+        return [{"id": d.id, "type": d.type, "tid": d.tid, "seq": (d.tid_hi << 32) | d.tid, "attempts": d.attempts,
+                 "epoch": d.epoch, "arrival_ns": d.arrival_ns, "commit_ns": d.commit_ns} for d in out[:n.value]]
 
     def read_values(self, rid, max_words=1 << 16):
         out = np.zeros(max_words, dtype=np.uint32)

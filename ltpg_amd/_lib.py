@@ -130,14 +130,15 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("kv_ops", ctypes.c_uint32),
                 ("max_epoch_txns", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64), ("keep_reads", ctypes.c_uint32),
                 ("record_epochs", ctypes.c_uint32), ("epoch_growth", ctypes.c_uint32), ("plain", ctypes.c_uint32),
-                ("epoch_floor", ctypes.c_uint32)]
+                ("epoch_floor", ctypes.c_uint32), ("kv_loop", ctypes.c_uint32), ("ring_requests", ctypes.c_uint32),
+                ("first_seq", ctypes.c_uint64)]
 
 
 class ServeDone(ctypes.Structure):
     """struct ltpg_serve_done."""
 
     _fields_ = [("id", ctypes.c_uint64), ("type", ctypes.c_uint32), ("tid", ctypes.c_uint32),
-                ("attempts", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("epoch", ctypes.c_uint64),
+                ("attempts", ctypes.c_uint32), ("tid_hi", ctypes.c_uint32), ("epoch", ctypes.c_uint64),
                 ("arrival_ns", ctypes.c_uint64), ("commit_ns", ctypes.c_uint64)]
 
 

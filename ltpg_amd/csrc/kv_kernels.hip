@@ -1259,6 +1259,522 @@ __global__ __launch_bounds__(kBlock) void kv_hot_apply_kernel(uint32_t *__restri
     for (uint32_t k = 0; k < fw; ++k) w[k] += total;
 }
 
+// ---- served key-value epochs in one persistent workgroup ------------------
+//
+// kv_serve_loop_kernel runs the served form's key-value epochs back to back
+// in one 1024-thread workgroup, without a kernel launch or a host round trip
+// between them:
+//
+//   admission  the requests aborted by the last epoch (kept in LDS, in
+//              priority order) head the next epoch, followed by arrivals
+//              taken from the request ring in arrival order, up to the epoch
+//              cap.  Priority inside an epoch is batch position, which is
+//              arrival order: every carried request arrived before every new
+//              one, and carried requests keep their relative order;
+//   phases     those of kv_small_epoch_kernel: W registrations in an LDS
+//              hash of the written rows, read marking, Aria's commit rule,
+//              read-out of committed reads (when the host keeps them) and
+//              write-back of committed writes, the RMW delta being the low
+//              32 bits of the arrival sequence number | 1;
+//   results    one record per epoch in the done ring (pinned host memory),
+//              written by wave 0 and published by a system-scope release of
+//              ctl->done_tail;
+//   requeue    the aborted requests are compacted to the front of the LDS
+//              op arrays, in position order, and head the next epoch.
+//
+// Every wait is bounded: the loop ends when the host sets ctl->stop, after
+// idle_ticks without work, after stall_ticks with the done ring full and
+// after slice_ticks in all; the kernel then saves the carried requests to
+// device memory and clears ctl->running.  Control decisions are taken from
+// values thread 0 broadcasts through LDS, so every wave leaves the loop at
+// the same iteration.
+constexpr uint32_t kLoopMaxT = kSmallOps;  // transactions per epoch at one op slot each
+
+__device__ __forceinline__ uint64_t sys_load(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void sys_store(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Workgroup barrier for LDS only: it orders LDS accesses like __syncthreads
+// but does not wait for the wave's global loads, which stay in flight across
+// it (the served loop keeps PCIe reads in flight over several phases).
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Exclusive prefix count of the flags of items k * kSmallBlock + threadIdx.x
+// (k < kSmallPer, in item order): per-wave ballots, then one pass over the
+// wave totals in LDS.  Returns the total; s_w holds 16 * kSmallPer + 1 words.
+__device__ __forceinline__ uint32_t block_count_before(const bool (&f)[kSmallPer], uint32_t (&before)[kSmallPer],
+                                                       uint32_t *s_w)
+{
+    constexpr uint32_t kWaves = kSmallBlock / 64;
+    const uint32_t lane = (uint32_t)lane_id(), wave = threadIdx.x / 64;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t pre[kSmallPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) {
+        const unsigned long long m = __ballot(f[k]);
+        pre[k] = (uint32_t)__popcll(m & lt);
+        if (lane == 0) s_w[k * kWaves + wave] = (uint32_t)__popcll(m);
+    }
+    lds_sync();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t j = 0; j < kSmallPer * kWaves; ++j) {
+            const uint32_t c = s_w[j];
+            s_w[j] = run;
+            run += c;
+        }
+        s_w[kSmallPer * kWaves] = run;
+    }
+    lds_sync();
+#pragma unroll
+    for (uint32_t k = 0; k < kSmallPer; ++k) before[k] = s_w[k * kWaves + wave] + pre[k];
+    const uint32_t total = s_w[kSmallPer * kWaves];
+    lds_sync();
+    return total;
+}
+
+__global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArgs a)
+{
+    __shared__ uint64_t s_row[kSmallOps];
+    __shared__ uint32_t s_meta[kSmallOps];
+    __shared__ uint64_t s_seq[kLoopMaxT];            // arrival sequence number by position
+    __shared__ uint32_t s_key[kSmallSlots];          // row + 1 of a written row, 0 = empty
+    __shared__ uint32_t s_min[kSmallSlots];          // its writers' minimum position
+    __shared__ uint8_t s_waw[kLoopMaxT], s_raw[kLoopMaxT], s_war[kLoopMaxT];
+    __shared__ uint32_t s_list[kLoopMaxT];           // committed transactions, in position order
+    __shared__ uint32_t s_cbefore[kLoopMaxT];        // committed transactions before each position
+    __shared__ uint32_t s_w[16 * kSmallPer + 1];
+    // arrivals prefetched during the last epoch: requests st_head ..
+    // st_head + st_n - 1 of the ring, request-major
+    __shared__ uint64_t st_row[kSmallOps];
+    __shared__ uint32_t st_meta[kSmallOps];
+    __shared__ uint64_t s_st_head;
+    __shared__ uint32_t s_st_n;
+    __shared__ uint64_t s_ctl[4];
+    __shared__ uint32_t s_bad;
+
+    // kernel arguments as locals: the lambdas below capture by reference,
+    // and a reference to the argument struct would force it into scratch
+    const uint32_t S = a.S, fields = a.fields, fw = a.fw, fsw = a.fsw, L = a.L, max_scan = a.max_scan;
+    const uint32_t values = a.values;
+    const uint64_t rows = a.rows, hot = a.hot, done_mask = a.done_mask;
+    ServeCtl *ctl = a.ctl;
+    uint32_t *table = a.table;
+    uint32_t *reads = a.reads;
+    uint64_t *done = a.done;
+    // where the previous launch stopped (fields only the kernel writes)
+    // (the start time too comes from thread 0: every exit test below must
+    // give the same answer in every wave)
+    __shared__ uint64_t s_start;
+    if (threadIdx.x == 0) {
+        s_ctl[0] = sys_load(&ctl->head);
+        s_ctl[1] = sys_load(&ctl->done_tail);
+        s_ctl[2] = sys_load(&ctl->epochs);
+        s_ctl[3] = (sys_load(&ctl->n_carry) & 0xffffffffull) | (sys_load(&ctl->cap) << 32);
+        s_start = wall_clock64();
+    }
+    __syncthreads();
+    uint64_t head = s_ctl[0], done_tail = s_ctl[1], epoch = s_ctl[2];
+    uint32_t n_carry = (uint32_t)(s_ctl[3] & 0xffffffffull), cap = (uint32_t)(s_ctl[3] >> 32);
+    const uint64_t t_start = s_start;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
+        s_row[i] = a.carry_keys[i];
+        s_meta[i] = a.carry_meta[i];
+    }
+    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seq[t] = a.carry_seq[t];
+    uint64_t t_work = t_start, code = 0;
+    // profile: thread 0 stamps the end of each part (after its barrier)
+    // (sums in LDS: a register array would be spilled)
+    const bool prof = a.profile != 0;
+    __shared__ uint64_t s_prof[kServeProfParts + 2];  // sums, then the last stamp and the epoch count
+    if (prof && threadIdx.x == 0) {
+        for (int k = 0; k < kServeProfParts; ++k) s_prof[k] = 0;
+        s_prof[kServeProfParts] = t_start;
+        s_prof[kServeProfParts + 1] = 0;
+    }
+    const auto stamp = [&](int part) {
+        if (prof && threadIdx.x == 0) {
+            const uint64_t t = wall_clock64();
+            s_prof[part] += t - s_prof[kServeProfParts];
+            s_prof[kServeProfParts] = t;
+        }
+    };
+    // The host's control words (tail, stop, done_head) cross PCIe: during an
+    // epoch thread 0 issues their loads after admission and stores the
+    // values at the end of the epoch, so the next admission decides on
+    // values at most one epoch old without waiting for them; only an idle or
+    // stalled loop waits for a fresh read.  (Ring entries are read only below
+    // the tail so read: the host writes them before it advances the tail.)
+    const auto poll = [&]() {
+        if (threadIdx.x == 0) {
+            s_ctl[0] = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            s_ctl[1] = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            s_ctl[2] = __hip_atomic_load(&ctl->done_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            s_ctl[3] = wall_clock64();
+        }
+    };
+    poll();
+    if (threadIdx.x == 0) {
+        s_st_head = head;
+        s_st_n = 0;
+    }
+    __syncthreads();
+    while (true) {
+        const uint64_t tail = s_ctl[0], stop = s_ctl[1], done_head = s_ctl[2], now = s_ctl[3];
+        lds_sync();
+        if (stop) {
+            code = kServeExitStop;
+            break;
+        }
+        if (now - t_start > a.slice_ticks) {
+            code = kServeExitSlice;
+            break;
+        }
+        const uint64_t avail = tail > head ? tail - head : 0ull;
+        const uint32_t room = cap > n_carry ? cap - n_carry : 0u;
+        const uint32_t n_new = (uint32_t)(avail < room ? avail : (uint64_t)room);
+        const uint32_t B = n_carry + n_new;
+        if (B == 0) {
+            if (now - t_work > a.idle_ticks) {
+                code = kServeExitIdle;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(32);
+            poll();
+            lds_sync();
+            stamp(9);
+            continue;
+        }
+        if (done_tail + 1 + B - done_head > done_mask + 1) {
+            if (now - t_work > a.stall_ticks) {
+                code = kServeExitStall;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(32);
+            poll();
+            lds_sync();
+            stamp(9);
+            continue;
+        }
+        stamp(0);
+        // admission: arrivals head .. head + n_new - 1 behind the carried
+        // requests, from the prefetched staging when it holds them, else from
+        // the ring (every ring load issued before the first LDS store)
+        const uint32_t n = B * S, c0 = n_carry * S;
+        {
+            const uint32_t staged = s_st_head == head ? s_st_n : 0u;
+            const bool any_ring = n_new > staged;  // uniform: skip the PCIe reads when all are staged
+            uint64_t r[kSmallPer] = {};
+            uint32_t m[kSmallPer] = {};
+            bool from_ring[kSmallPer];
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallPer; ++k) {
+                const uint32_t i = k * kSmallBlock + threadIdx.x;
+                const bool live = i >= c0 && i < n;
+                const uint32_t t = live ? i / S : 0u;
+                const uint32_t jn = live ? t - n_carry : 0u;
+                from_ring[k] = live && jn >= staged;
+                const uint64_t at = from_ring[k] ? ((head + jn) & a.ring_mask) * S + (i - t * S) : 0ull;
+                if (any_ring) {
+                    r[k] = a.ring_keys[at];
+                    m[k] = a.ring_meta[at];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kSmallPer; ++k) {
+                const uint32_t i = k * kSmallBlock + threadIdx.x;
+                if (i >= c0 && i < n) {
+                    const uint32_t st = i - c0;  // staging index: request jn, slot i % S
+                    s_row[i] = from_ring[k] ? r[k] : st_row[st];
+                    s_meta[i] = from_ring[k] ? m[k] : st_meta[st];
+                }
+            }
+        }
+        for (uint32_t t = n_carry + threadIdx.x; t < B; t += kSmallBlock) s_seq[t] = head + (t - n_carry);
+        for (uint32_t k = threadIdx.x; k < kSmallSlots; k += kSmallBlock) {
+            s_key[k] = 0u;
+            s_min[k] = ~0u;
+        }
+        for (uint32_t t = threadIdx.x; t < B; t += kSmallBlock) {
+            s_waw[t] = 0;
+            s_raw[t] = 0;
+            s_war[t] = 0;
+        }
+        if (threadIdx.x == 0) s_bad = 0;
+        lds_sync();
+        stamp(1);
+        // the ring entries are in LDS: their slots may be reused by the host
+        head += n_new;
+        // (relaxed: the entries' loads completed before the barrier)
+        if (threadIdx.x == 0 && n_new) __hip_atomic_store(&ctl->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        // the next admission's control words: loads in flight during the epoch
+        uint64_t c_tail = 0, c_stop = 0, c_dh = 0;
+        if (threadIdx.x == 0) {
+            c_tail = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            c_stop = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            c_dh = __hip_atomic_load(&ctl->done_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        // and its arrivals: the published requests after this admission, up
+        // to a full epoch, loaded now and staged in LDS at the end
+        const uint32_t pf_n = (uint32_t)(tail - head < a.cap_max ? tail - head : (uint64_t)a.cap_max);
+        uint64_t pf_r[kSmallPer] = {};
+        uint32_t pf_m[kSmallPer] = {};
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            const uint32_t q = i / S;
+            if (i < pf_n * S) {
+                const uint64_t at = ((head + q) & a.ring_mask) * S + (i - q * S);
+                pf_r[k] = a.ring_keys[at];
+                pf_m[k] = a.ring_meta[at];
+            }
+        }
+        uint64_t r[kSmallPer];
+        uint32_t m[kSmallPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            r[k] = i < n ? s_row[i] : 0ull;
+            m[k] = i < n ? s_meta[i] : 0u;
+        }
+        // EXECUTE: W registrations into the hash (priority = position)
+        uint32_t bad = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            if (i >= n) continue;
+            const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+            bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
+            if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot) continue;
+            const uint32_t key = (uint32_t)o.row + 1u;
+            uint32_t h = small_hash(o.row);
+            while (true) {
+                const uint32_t prev = atomicCAS(&s_key[h], 0u, key);
+                if (prev == 0u || prev == key) break;
+                h = (h + 1) & (kSmallSlots - 1);
+            }
+            atomicMin(&s_min[h], i / S);
+        }
+        if (bad) atomicAdd(&s_bad, bad);
+        lds_sync();
+        stamp(2);
+        // waw of every write; raw of every read, war of the row's minimum
+        // writer when a higher-priority transaction reads it
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            if (i >= n) continue;
+            const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+            if (!o.valid) continue;
+            const uint32_t t = i / S;
+            if (o.kind == LTPG_KV_WRITE) {
+                if (o.row != hot && s_min[small_find(s_key, o.row)] < t) s_waw[t] = 1;
+                continue;
+            }
+            bool raw = false;
+            for (uint32_t q = 0; q < o.len; ++q) {
+                if (o.row + q == hot) continue;
+                const int h = small_find(s_key, o.row + q);
+                if (h < 0) continue;
+                const uint32_t w = s_min[h];
+                if (w < t)
+                    raw = true;
+                else if (w > t)
+                    s_war[w] = 1;
+            }
+            if (raw) s_raw[t] = 1;
+        }
+        lds_sync();
+        stamp(3);
+        // CHECK: commit bits and the committed list in position order
+        bool c[kSmallPer];
+        uint32_t before[kSmallPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t t = k * kSmallBlock + threadIdx.x;
+            c[k] = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
+        }
+        const uint32_t nc = block_count_before(c, before, s_w);
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t t = k * kSmallBlock + threadIdx.x;
+            if (t < B) s_cbefore[t] = before[k];
+            if (c[k]) s_list[before[k]] = t;
+        }
+        lds_sync();
+        stamp(4);
+        // read-out (when kept) and write-back; reads see the snapshot, so
+        // every read completes before the first write-back store
+        const uint32_t per = L * fw;
+        const uint32_t total_r = reads ? nc * S * per : 0u;
+        const uint32_t total_w = values == LTPG_VALUES_RMW ? nc * S * fw : 0u;
+        const uint64_t entry_words = (uint64_t)S * per;
+        const auto read_item = [&](uint32_t x, uint64_t &src, uint64_t &dst) -> bool {
+            const uint32_t j = x / per, q = (x % per) / fw, w = x % fw;
+            const uint32_t t = s_list[j / S], sl = j % S;
+            const uint32_t i = t * S + sl;
+            const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+            const bool ok = q < o.len;
+            src = ok ? ((o.row + q) * fields + o.field) * fsw + w : 0ull;
+            dst = ((done_tail + 1 + t) & done_mask) * entry_words + ((uint64_t)sl * L + q) * fw + w;
+            return ok;
+        };
+        // a committed write adds its delta to every word of its field with
+        // an atomic add: several writes of one transaction to one field (and
+        // the writes to the hot row, which never conflict) then need no
+        // folding, and the sum is the sequential one (modular addition)
+        const auto write_item = [&](uint32_t x, uint64_t &off, uint32_t &add) -> bool {
+            const uint32_t j = x / fw, w = x % fw;
+            const uint32_t t = s_list[j / S], sl = j % S;
+            const uint32_t i = t * S + sl;
+            const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+            const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
+            add = (uint32_t)s_seq[t] | 1u;
+            off = wr ? (o.row * fields + o.field) * fsw + w : 0ull;
+            return wr;
+        };
+        // the read-out (kept reads only) in rounds, then a barrier, then the
+        // write-back; without kept reads an epoch's writes are one round
+        {
+            for (uint32_t b0 = 0; b0 < total_r; b0 += kSmallUnroll * kSmallBlock) {
+                uint64_t src[kSmallUnroll], dst[kSmallUnroll];
+                bool ok[kSmallUnroll];
+                uint32_t v[kSmallUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kSmallUnroll; ++u) {
+                    const uint32_t x = b0 + u * kSmallBlock + threadIdx.x;
+                    ok[u] = x < total_r && read_item(x, src[u], dst[u]);
+                    if (!ok[u]) src[u] = 0ull;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kSmallUnroll; ++u)
+                    v[u] = __hip_atomic_load(&table[src[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (uint32_t u = 0; u < kSmallUnroll; ++u)
+                    if (ok[u]) reads[dst[u]] = v[u];
+            }
+            // every read of the snapshot is done before the first write
+            if (total_r) __syncthreads();
+            for (uint32_t x = threadIdx.x; x < total_w; x += kSmallBlock) {
+                uint64_t off;
+                uint32_t add;
+                if (write_item(x, off, add)) atomicAdd(&table[off], add);
+            }
+        }
+        // committed reads went to host memory from every wave
+        if (reads && total_r) __threadfence_system();
+        if (threadIdx.x == 0) {
+            add_counts(a.ctr, (uint32_t)(epoch % kCounterStripes), LTPG_TYPE_KV, nc, B);
+            if (s_bad) atomicAdd(&a.ctr->range_errors, s_bad);
+        }
+        // a full barrier: the epoch's atomics are complete before the next
+        // epoch's coherent reads (by then the prefetch has long landed)
+        __syncthreads();
+        stamp(6);
+        // the epoch's record: one wave writes it, then releases done_tail
+        if (threadIdx.x < 64) {
+            if (threadIdx.x == 0)
+                done[done_tail & done_mask] = (uint64_t)B | ((uint64_t)nc << 16) | (epoch << 32);
+            for (uint32_t t = threadIdx.x; t < B; t += 64) {
+                const bool ct = !s_waw[t] && !(s_raw[t] && s_war[t]);
+                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (ct ? kDoneCommitted : 0ull);
+            }
+            __threadfence_system();
+            if (threadIdx.x == 0) sys_store(&ctl->done_tail, done_tail + 1 + B);
+        }
+        done_tail += 1 + B;
+        lds_sync();
+        stamp(7);
+        // requeue: the aborted requests move to the front, in position order
+        // (everything is re-read from LDS: nothing of the phases above stays
+        // in registers across the read-out and write-back)
+        uint64_t sq[kSmallPer], ro[kSmallPer];
+        uint32_t mo[kSmallPer], dst_t[kSmallPer], dst_i[kSmallPer];
+        bool ab[kSmallPer], abop[kSmallPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t t = k * kSmallBlock + threadIdx.x;
+            ab[k] = t < B && (s_waw[t] || (s_raw[t] && s_war[t]));
+            dst_t[k] = ab[k] ? t - s_cbefore[t] : 0u;
+            sq[k] = ab[k] ? s_seq[t] : 0ull;
+            const uint32_t i = t;
+            const uint32_t ti = i < n ? i / S : 0u;
+            abop[k] = i < n && (s_waw[ti] || (s_raw[ti] && s_war[ti]));
+            dst_i[k] = (ti - s_cbefore[ti]) * S + (i - ti * S);
+            ro[k] = abop[k] ? s_row[i] : 0ull;
+            mo[k] = abop[k] ? s_meta[i] : 0u;
+        }
+        lds_sync();
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            if (ab[k]) s_seq[dst_t[k]] = sq[k];
+            if (abop[k]) {
+                s_row[dst_i[k]] = ro[k];
+                s_meta[dst_i[k]] = mo[k];
+            }
+        }
+        n_carry = B - nc;
+        if (a.growth) {
+            const uint64_t g = (uint64_t)a.growth * nc;
+            cap = (uint32_t)(g < a.floor ? a.floor : (g > a.cap_max ? a.cap_max : g));
+        } else {
+            cap = a.cap_max;
+        }
+        ++epoch;
+        t_work = now;
+        if (threadIdx.x == 0) {
+            s_ctl[0] = c_tail;
+            s_ctl[1] = c_stop;
+            s_ctl[2] = c_dh;
+            s_ctl[3] = wall_clock64();
+            s_st_head = head;
+            s_st_n = pf_n;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            if (i < pf_n * S) {
+                st_row[i] = pf_r[k];
+                st_meta[i] = pf_m[k];
+            }
+        }
+        lds_sync();
+        stamp(8);
+        if (prof && threadIdx.x == 0) s_prof[kServeProfParts + 1] += 1;
+    }
+    // save the carried requests for the next launch (same stream: ordered)
+    for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
+        a.carry_keys[i] = s_row[i];
+        a.carry_meta[i] = s_meta[i];
+    }
+    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) a.carry_seq[t] = s_seq[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (prof) {
+            for (int k = 0; k < kServeProfParts; ++k) ctl->prof[k] = s_prof[k];
+            ctl->prof_epochs = s_prof[kServeProfParts + 1];
+        }
+        sys_store(&ctl->n_carry, n_carry);
+        sys_store(&ctl->cap, cap);
+        sys_store(&ctl->epochs, epoch);
+        sys_store(&ctl->head, head);
+        sys_store(&ctl->exit_code, code);
+        sys_store(&ctl->running, 0ull);
+    }
+}
+
 int lanes_for(uint32_t fw)
 {
     if (fw > 64) return 0;
@@ -1541,6 +2057,20 @@ hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_
                        b.n_slots, b.request_major, kv.rows, kv.fields, kv.field_words, kv.field_stride, kv.max_scan,
                        kv.read_slot, values, kv.hot_row, kv.table, kv.read_values, kv.bitmap, host_bits, seq_out, seq,
                        ctr);
+    return hipGetLastError();
+}
+
+bool kv_serve_loop_ok(const KvState &kv, uint32_t S)
+{
+    return S >= 1 && S <= 64 && S <= kSmallOps && (kv.hot_row == LTPG_NO_HOT_ROW || kv.fields <= kSmallHotFields) &&
+           kv.rows < 0xFFFFFFFFull;
+}
+
+hipError_t launch_kv_serve_loop(const ServeLoopArgs &a, hipStream_t s)
+{
+    if (a.S < 1 || a.S > 64 || (uint64_t)a.cap_max * a.S > kSmallOps || a.cap_max == 0 || a.floor == 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kv_serve_loop_kernel, dim3(1), dim3(kSmallBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -282,6 +282,81 @@ hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_
 // each (the reference's driver loop), then ltpg_sync.
 int run_epoch_stepwise(ltpg_ctx *ctx);
 
+// ---- served key-value epochs in one persistent workgroup -----------------
+//
+// The served form's key-value lane can run as one single-workgroup kernel
+// that forms its own epochs: it takes arrivals from a request ring in
+// pinned host memory, keeps the aborted requests of each epoch in LDS as the
+// head of the next epoch (their priority order is their arrival order), and
+// publishes one record per epoch into a done ring in pinned host memory.
+// The kernel exits when asked, when idle, when the done ring stays full, or
+// after a time slice; the host relaunches it when work is pending, and the
+// aborted requests it carried wait in device memory in between.
+
+// Control block in pinned, fine-grained host memory.  The host writes the
+// first cache line, the kernel the second.
+struct ServeCtl {
+    uint64_t tail;       // host: requests published in the ring (release)
+    uint64_t stop;       // host: nonzero = exit at the next epoch boundary
+    uint64_t done_head;  // host: done-ring words consumed (release)
+    uint64_t pad0[5];
+    uint64_t head;       // kernel: requests taken from the ring (release)
+    uint64_t done_tail;  // kernel: done-ring words published (release)
+    uint64_t running;    // host sets 1 before a launch, the kernel clears it last
+    uint64_t exit_code;  // why the last launch returned (kServeExit*)
+    uint64_t epochs;     // epochs run, all launches
+    uint64_t n_carry;    // requests carried to the next epoch
+    uint64_t cap;        // the next epoch's transaction cap
+    uint64_t pad1;
+    // profile (ServeLoopArgs.profile): wall-clock ticks spent in each part
+    // of the epochs of the last launch (kServeProf* order), and their count
+    uint64_t prof[10];
+    uint64_t prof_epochs;
+};
+// Parts of an epoch timed by the profile: control read, admission, EXECUTE,
+// read marking, CHECK, write folding, read-out and write-back, publication,
+// requeue, idle or waiting.
+constexpr int kServeProfParts = 10;
+constexpr uint64_t kServeExitStop = 1;   // stop requested
+constexpr uint64_t kServeExitIdle = 2;   // nothing to do for idle_ticks
+constexpr uint64_t kServeExitStall = 3;  // the done ring stayed full for stall_ticks
+constexpr uint64_t kServeExitSlice = 4;  // ran for slice_ticks
+
+// Done-ring record of one epoch: a header word (bits 0-15 transactions,
+// 16-31 commits, 32-63 epoch number), then one word per member in batch
+// order: its arrival sequence number, bit 63 set if it committed.
+constexpr uint64_t kDoneCommitted = 1ull << 63;
+
+struct ServeLoopArgs {
+    // table
+    uint32_t *table;
+    uint64_t rows;
+    uint32_t fields, fw, fsw, max_scan, L, values;
+    uint64_t hot;
+    // requests: kv_ops slots each, request-major; ring slot of arrival q is
+    // q & ring_mask
+    uint32_t S;
+    const uint64_t *ring_keys;
+    const uint32_t *ring_meta;
+    uint64_t ring_mask;
+    ServeCtl *ctl;
+    uint64_t *done;       // done ring (pinned)
+    uint64_t done_mask;
+    uint32_t *reads;      // or null: committed reads, S * L * fw words per done-ring word
+    // carried requests between launches (device memory)
+    uint64_t *carry_keys;
+    uint32_t *carry_meta;
+    uint64_t *carry_seq;
+    // epoch sizing: cap in [floor, cap_max], growth x the last epoch's commits
+    uint32_t cap_max, floor, growth;
+    // bounded waits, in wall-clock ticks
+    uint64_t idle_ticks, stall_ticks, slice_ticks;
+    DeviceCounters *ctr;
+    uint32_t profile;     // nonzero: time the parts of each epoch into ctl->prof
+};
+bool kv_serve_loop_ok(const KvState &kv, uint32_t S);
+hipError_t launch_kv_serve_loop(const ServeLoopArgs &a, hipStream_t s);
+
 // Order-entry launchers (tpcc_kernels.hip).
 hipError_t launch_tpcc_populate(const TpccDev &d, uint64_t seed, hipStream_t s);
 hipError_t launch_tpcc_name_index(const TpccDev &d, hipStream_t s);

@@ -44,6 +44,15 @@ int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
                     oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan);
 
+/* The same epoch with the RMW delta of transaction t given as deltas[t]
+ * instead of tids[t] | 1 (the served form's key-value requests: priorities
+ * are ranks inside the epoch, deltas come from the 64-bit arrival number). */
+int oracle_kv_epoch_deltas(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                           uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                           const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
+                           oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan,
+                           const uint32_t *deltas);
+
 /* The four phases separately (used by tests that pin each one). */
 void oracle_kv_register(uint64_t rows, uint32_t fields, uint32_t B, uint32_t S,
                         const uint64_t *keys, const uint32_t *meta, const uint32_t *tids,

@@ -160,14 +160,15 @@ uint32_t oracle_commit(uint32_t B, const uint32_t *conflicts, uint64_t *bitmap)
 
 /* writeback_w / __WRITEBACK (Execute.h:193-206, Genericfunction.h:145-158):
  * committed transactions write their write slots, in slot order. */
-void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
-                         uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
-                         const uint32_t *tids, const uint64_t *bitmap, uint32_t values)
+static void writeback_deltas(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                             uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                             const uint32_t *tids, const uint64_t *bitmap, uint32_t values,
+                             const uint32_t *deltas)
 {
     if (values == 0) return; /* copy-back stores the unchanged snapshot value */
     for (uint32_t t = 0; t < B; t++) {
         if (!((bitmap[t / 64] >> (t % 64)) & 1ull)) continue;
-        uint32_t delta = tids[t] | 1u;
+        uint32_t delta = deltas ? deltas[t] : (tids[t] | 1u);
         for (uint32_t s = 0; s < S; s++) {
             uint64_t i = (uint64_t)s * B + t;
             uint32_t m = meta[i];
@@ -179,10 +180,41 @@ void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, u
     }
 }
 
+void oracle_kv_writeback(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                         uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                         const uint32_t *tids, const uint64_t *bitmap, uint32_t values)
+{
+    writeback_deltas(rows, fields, field_words, table, B, S, keys, meta, tids, bitmap, values, NULL);
+}
+
+static int kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                    uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                    const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
+                    oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan, const uint32_t *deltas);
+
 int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
                     uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
                     const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
                     oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan)
+{
+    return kv_epoch(rows, fields, field_words, table, B, S, keys, meta, tids, values, reg_scratch, res,
+                    hot_row, max_scan, NULL);
+}
+
+int oracle_kv_epoch_deltas(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                           uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                           const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
+                           oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan,
+                           const uint32_t *deltas)
+{
+    return kv_epoch(rows, fields, field_words, table, B, S, keys, meta, tids, values, reg_scratch, res,
+                    hot_row, max_scan, deltas);
+}
+
+static int kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32_t *table,
+                    uint32_t B, uint32_t S, const uint64_t *keys, const uint32_t *meta,
+                    const uint32_t *tids, uint32_t values, uint64_t *reg_scratch,
+                    oracle_kv_result *res, uint64_t hot_row, uint32_t max_scan, const uint32_t *deltas)
 {
     const uint64_t L = max_scan > 1 ? max_scan : 1;
     uint64_t *reg_r = reg_scratch, *reg_w = reg_scratch + rows;
@@ -215,8 +247,8 @@ int oracle_kv_epoch(uint64_t rows, uint32_t fields, uint32_t field_words, uint32
             }
         }
     }
-    oracle_kv_writeback(rows, fields, field_words, table, B, S, keys, meta, tids,
-                        res->commit_bitmap, values);
+    writeback_deltas(rows, fields, field_words, table, B, S, keys, meta, tids, res->commit_bitmap, values,
+                     deltas);
     res->range_errors = errors;
     return 0;
 }

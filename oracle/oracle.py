@@ -36,6 +36,8 @@ def _load(path=None):
                                     ctypes.c_uint32, ctypes.c_uint32, u64p, u32p, u32p, ctypes.c_uint32,
                                     u64p, ctypes.POINTER(KvResult), ctypes.c_uint64, ctypes.c_uint32]
     lib.KvResult = KvResult
+    lib.oracle_kv_epoch_deltas.restype = ctypes.c_int
+    lib.oracle_kv_epoch_deltas.argtypes = lib.oracle_kv_epoch.argtypes + [u32p]
 
     class TpccResult(ctypes.Structure):
         _fields_ = [("no_bitmap", u64p), ("pay_bitmap", u64p), ("no_conflicts", u32p), ("pay_conflicts", u32p),
@@ -113,10 +115,11 @@ NO_HOT_ROW = (1 << 64) - 1
 
 
 def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values=1, want_reads=True,
-             inplace=False, hot_row=NO_HOT_ROW, max_scan=0):
+             inplace=False, hot_row=NO_HOT_ROW, max_scan=0, deltas=None):
     """Run one epoch on a copy of `table` (row-major u32), or on `table`
     itself when inplace=True; return KvEpoch.  Read values hold
-    max(1, max_scan) fields per op slot."""
+    max(1, max_scan) fields per op slot.  deltas: the RMW delta of each
+    transaction (default tids | 1)."""
     if inplace:
         assert table.dtype == np.uint32 and table.flags.c_contiguous
         table = table.reshape(-1)
@@ -135,8 +138,14 @@ def kv_epoch(table, rows, fields, field_words, keys, meta, tids, n_slots, values
     reads = np.zeros(max(B * S * L * field_words, 1), dtype=np.uint32) if want_reads else None
     regs = np.empty(rows * 2, dtype=np.uint64)
     res = _lib.KvResult(_u64p(bitmap), _u32p(conflicts), _u32p(reads) if want_reads else None, 0, 0)
-    rc = _lib.oracle_kv_epoch(rows, fields, field_words, _u32p(table), B, S, _u64p(keys), _u32p(meta),
-                              _u32p(tids), values, _u64p(regs), ctypes.byref(res), hot_row, int(max_scan))
+    args = (rows, fields, field_words, _u32p(table), B, S, _u64p(keys), _u32p(meta), _u32p(tids), values,
+            _u64p(regs), ctypes.byref(res), hot_row, int(max_scan))
+    if deltas is None:
+        rc = _lib.oracle_kv_epoch(*args)
+    else:
+        deltas = np.ascontiguousarray(deltas, dtype=np.uint32)
+        assert deltas.shape == (B,)
+        rc = _lib.oracle_kv_epoch_deltas(*args, _u32p(deltas))
     assert rc == 0
     return KvEpoch(bitmap[: (B + 63) // 64], conflicts[:B],
                    reads[: B * S * L * field_words] if want_reads else None,

@@ -48,13 +48,37 @@ def test_abi_version_and_config_defaults():
     assert cfg.values == L.VALUES_RMW and cfg.rules == L.RULES_REFERENCE
 
 
-def test_config_struct_layout_matches_c():
-    # offsets of the C struct (include/ltpg.h) on x86-64
-    assert ctypes.sizeof(L.Config) == 72
-    assert L.Config.kv_rows.offset == 24
-    assert L.Config.kv_hot_row.offset == 48
-    assert L.Config.hbm_budget.offset == 64
-    assert ctypes.sizeof(L.KvBatch) == 32
+# ctypes mirror -> C typedef of include/ltpg.h
+C_STRUCTS = {"Config": "ltpg_config", "KvBatch": "ltpg_kv_batch", "KvGen": "ltpg_kv_gen",
+             "NeworderBatch": "ltpg_neworder_batch", "PaymentBatch": "ltpg_payment_batch",
+             "OrderstatusBatch": "ltpg_orderstatus_batch", "StocklevelBatch": "ltpg_stocklevel_batch",
+             "TpccGenConfig": "ltpg_tpcc_gen_config", "BatchDesc": "ltpg_batch_desc",
+             "TuningEntry": "ltpg_tuning_entry", "ServeConfig": "ltpg_serve_config", "ServeDone": "ltpg_serve_done",
+             "ServeStats": "ltpg_serve_stats"}
+
+
+def test_struct_layouts_match_a_compiled_c_caller(tmp_path):
+    """sizeof and offsetof of every field of every ABI struct, as gcc lays
+    them out for a C caller that includes include/ltpg.h, equal those of the
+    Python binding's ctypes mirrors (a field missing in C fails to compile)."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "ltpg.h"', 'int main(void) {']
+    for py, c in C_STRUCTS.items():
+        lines.append(f'printf("{py} sizeof() %zu\\n", sizeof({c}));')
+        for name, *_ in getattr(_lib, py)._fields_:
+            lines.append(f'printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(v) for a, b, v in (ln.split() for ln in out if ln)}
+    for py in C_STRUCTS:
+        cls = getattr(_lib, py)
+        assert got[(py, "sizeof()")] == ctypes.sizeof(cls), py
+        for name, *_ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, f"{py}.{name}"
 
 
 def test_python_constants_match_the_header():
@@ -74,8 +98,9 @@ def test_python_constants_match_the_header():
 def test_serve_config_defaults_match_c():
     cfg = L.ServeConfig()
     L.lib.ltpg_serve_config_init(ctypes.byref(cfg))
-    assert cfg.size == ctypes.sizeof(L.ServeConfig) == 48
-    assert (cfg.slots, cfg.kv_ops, cfg.epoch_growth, cfg.epoch_floor, cfg.plain) == (2, 16, 8, 32, 0)
+    assert cfg.size == ctypes.sizeof(L.ServeConfig)
+    assert (cfg.slots, cfg.kv_ops, cfg.epoch_growth, cfg.epoch_floor, cfg.plain, cfg.kv_loop, cfg.ring_requests,
+            cfg.first_seq) == (2, 16, 8, 32, 0, 1, 0, 0)
 
 
 @pytest.mark.skipif(L.device_count() > 0, reason="only meaningful without a GPU")

@@ -46,6 +46,34 @@ def _run(rows, fields, fw, keys, meta, tids, S, values=1):
     return oracle.kv_epoch(kc.initial_table(rows, fields, fw), rows, fields, fw, keys, meta, tids, S, values)
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_oracle_explicit_deltas(seed):
+    """The deltas variant (served-form RMW deltas from 64-bit arrival
+    numbers): deltas = tids | 1 reproduces the plain epoch exactly; other
+    deltas leave every decision and read unchanged and add, per committed
+    write op, its transaction's delta to each word of its field (a closed
+    form summed here with numpy, independent of the oracle)."""
+    rng = np.random.default_rng(100 + seed)
+    B, S, rows, fields, fw = int(rng.integers(1, 40)), int(rng.integers(1, 6)), int(rng.integers(1, 12)), 3, 2
+    keys, meta, tids = kc.random_batch(rng, B, S, rows, fields)
+    t0 = kc.initial_table(rows, fields, fw)
+    a = oracle.kv_epoch(t0, rows, fields, fw, keys, meta, tids, S)
+    b = oracle.kv_epoch(t0, rows, fields, fw, keys, meta, tids, S, deltas=tids | 1)
+    np.testing.assert_array_equal(a.table, b.table)
+    deltas = rng.integers(0, 1 << 32, B, dtype=np.uint64).astype(np.uint32)
+    c = oracle.kv_epoch(t0, rows, fields, fw, keys, meta, tids, S, deltas=deltas)
+    np.testing.assert_array_equal(a.bitmap, c.bitmap)
+    np.testing.assert_array_equal(a.read_values, c.read_values)
+    want = t0.reshape(rows, fields, fw).astype(np.uint64)
+    committed = kc.bitmap_to_list(a.bitmap, B)
+    for s_ in range(S):
+        for t in range(B):
+            m = int(meta[s_ * B + t])
+            if m & 3 == 2 and committed[t] and int(keys[s_ * B + t]) < rows:
+                want[int(keys[s_ * B + t]), (m >> 8) & 0xFFFF] += int(deltas[t])
+    np.testing.assert_array_equal(c.table, (want & 0xFFFFFFFF).astype(np.uint32).reshape(-1))
+
+
 @pytest.mark.parametrize("seed", range(40))
 def test_oracle_matches_pairwise_and_is_serializable(seed):
     rng = np.random.default_rng(seed)

@@ -6,6 +6,7 @@ through the oracle, must reproduce every commit decision, every read value
 and the final tables.  Requests on rows nobody else touches must see the
 same results alone and in a contended mix.
 """
+import gc
 import threading
 import time
 
@@ -35,19 +36,34 @@ def _kv_ctx(rows, fields, fw, S, max_txns):
     return ctx
 
 
+# key-value serving forms: epochs formed and run by the persistent device
+# loop, epochs formed by the host thread, and the plain per-batch form
+KV_FORMS = {"device-loop": dict(kv_loop=1), "host-formed": dict(kv_loop=0), "plain-form": dict(plain=1)}
+
+
+def _replay_args(done, ids):
+    """Priorities and RMW deltas of a served key-value epoch for the
+    oracle: batch rank (arrival order) and low 32 bits of the arrival
+    number | 1."""
+    seqs = np.array([done[int(i)]["seq"] for i in ids], dtype=np.uint64)
+    assert (np.diff(seqs.astype(np.int64)) > 0).all(), "an epoch's members are in arrival order"
+    return np.arange(len(ids), dtype=np.uint32), ((seqs & 0xFFFFFFFF) | 1).astype(np.uint32)
+
+
 def _submit_in_waves(sv, keys, meta, waves=12, pause=0.002):
     for part_k, part_m in zip(np.array_split(keys, waves), np.array_split(meta, waves)):
         sv.submit_kv(part_k, part_m)
         time.sleep(pause)
 
 
-@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
-def test_served_kv_epochs_replay_on_the_oracle(gpu, plain):
-    """Both the served path and the plain per-batch form (same admission)."""
+@pytest.mark.parametrize("form", list(KV_FORMS))
+def test_served_kv_epochs_replay_on_the_oracle(gpu, form):
+    """The device loop, host-formed epochs and the plain per-batch form
+    (same admission): every recorded epoch replayed by the oracle."""
     rows, fields, fw, S, n = 3000, 3, 2, 8, 6000
     ctx = _kv_ctx(rows, fields, fw, S, 1024)
     keys, meta = _kv_requests(31, n, S, rows, fields)
-    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=1, record_epochs=1, plain=plain)
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=1, record_epochs=1, **KV_FORMS[form])
     _submit_in_waves(sv, keys, meta)
     sv.drain()
     done = {d["id"]: d for d in sv.poll()}
@@ -62,8 +78,8 @@ def test_served_kv_epochs_replay_on_the_oracle(gpu, plain):
         B = len(ids)
         k = keys[ids].T.reshape(-1).copy()
         m = meta[ids].T.reshape(-1).copy()
-        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
-        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True)
+        tids, deltas = _replay_args(done, ids)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, deltas=deltas)
         for t, rid in enumerate(ids):
             committed = ep.committed(t)
             assert committed == (done[rid]["epoch"] == e), f"request {rid} epoch {e}"
@@ -77,8 +93,8 @@ def test_served_kv_epochs_replay_on_the_oracle(gpu, plain):
     assert all(d["attempts"] >= 1 and d["commit_ns"] >= d["arrival_ns"] for d in done.values())
 
 
-@pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
-def test_served_hot_row_epochs_replay_on_the_oracle(gpu, plain):
+@pytest.mark.parametrize("form", list(KV_FORMS))
+def test_served_hot_row_epochs_replay_on_the_oracle(gpu, form):
     """Row 0 as the hot key (the reference's ZIPFIAN mode): its ops never
     conflict and committed writes to it add through the reduce; the served
     epochs replayed through the oracle with the same hot row reproduce every
@@ -89,7 +105,7 @@ def test_served_hot_row_epochs_replay_on_the_oracle(gpu, plain):
     ctx.kv_table_load(kc.initial_table(rows, fields, fw))
     keys, meta = _kv_requests(41, n, S, rows, fields, theta=0.99)
     assert (keys == 0).any()
-    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=600, record_epochs=1, plain=plain)
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=600, record_epochs=1, **KV_FORMS[form])
     _submit_in_waves(sv, keys, meta)
     sv.drain()
     done = {d["id"]: d for d in sv.poll()}
@@ -102,14 +118,15 @@ def test_served_hot_row_epochs_replay_on_the_oracle(gpu, plain):
         ids = ids.astype(np.int64)
         k = keys[ids].T.reshape(-1).copy()
         m = meta[ids].T.reshape(-1).copy()
-        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
-        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, hot_row=0)
+        tids, deltas = _replay_args(done, ids)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, hot_row=0, deltas=deltas)
         for t, rid in enumerate(ids):
             assert ep.committed(t) == (done[rid]["epoch"] == e), f"request {rid} epoch {e}"
     np.testing.assert_array_equal(ctx.kv_table_read(), table)
 
 
-def test_served_scans_replay_on_the_oracle(gpu):
+@pytest.mark.parametrize("form", ["device-loop", "host-formed"])
+def test_served_scans_replay_on_the_oracle(gpu, form):
     """Requests whose reads are scans: each committed request gets its
     scanned rows (inside the table, in op order) as the oracle's replay of
     the server's epochs delivers them."""
@@ -121,7 +138,7 @@ def test_served_scans_replay_on_the_oracle(gpu):
                                   read_pct=60, theta=0.8, scan_len=5)
     keys = keys.reshape(S, n).T.copy()
     meta = meta.reshape(S, n).T.copy()
-    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=500, keep_reads=1, record_epochs=1)
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=500, keep_reads=1, record_epochs=1, **KV_FORMS[form])
     _submit_in_waves(sv, keys, meta)
     sv.drain()
     done = {d["id"]: d for d in sv.poll()}
@@ -136,8 +153,8 @@ def test_served_scans_replay_on_the_oracle(gpu):
         B = len(ids)
         k = keys[ids].T.reshape(-1).copy()
         m = meta[ids].T.reshape(-1).copy()
-        tids = np.array([done[i]["tid"] for i in ids], dtype=np.uint32)
-        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, max_scan=max_scan)
+        tids, deltas = _replay_args(done, ids)
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, max_scan=max_scan, deltas=deltas)
         for t, rid in enumerate(ids):
             assert ep.committed(t) == (done[rid]["epoch"] == e)
             if not ep.committed(t):
@@ -152,7 +169,8 @@ def test_served_scans_replay_on_the_oracle(gpu):
     np.testing.assert_array_equal(ctx.kv_table_read(), table)
 
 
-def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
+@pytest.mark.parametrize("form", ["device-loop", "host-formed"])
+def test_served_results_are_the_same_alone_and_in_the_mix(gpu, form):
     rows, fields, fw, S = 20000, 3, 2, 8
     n_probe = 400
     # probes: request i reads and writes rows [4i, 4i + 4) only
@@ -168,7 +186,7 @@ def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
         ctx = _kv_ctx(rows, fields, fw, S, 4096)
         tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=1, max_txns=4096)
         tp.tpcc_populate(SEED)
-        sv = L.Server(ctx, tp, kv_ops=S, slots=2, keep_reads=1)
+        sv = L.Server(ctx, tp, kv_ops=S, slots=2, keep_reads=1, **KV_FORMS[form])
         ids = []
         if with_mix:
             def background():
@@ -204,6 +222,109 @@ def test_served_results_are_the_same_alone_and_in_the_mix(gpu):
     np.testing.assert_array_equal(ta, kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)[:4 * n_probe])
     # private rows: every probe commits at its first attempt, alone or not
     assert set(aa) == {1} and set(ab) == {1}
+
+
+@pytest.mark.parametrize("form", ["device-loop", "host-formed"])
+def test_served_kv_crosses_the_u32_sequence_wrap(gpu, form):
+    """Arrival numbers start 700 below 2^32: requests on both sides of the
+    wrap of their low 32 bits commit in priority order (every epoch's members
+    in arrival order, a re-queued request ahead of later arrivals), and the
+    recorded epochs replay on the oracle with rank priorities and the
+    arrival-number deltas."""
+    rows, fields, fw, S, n = 400, 2, 1, 8, 2000
+    ctx = _kv_ctx(rows, fields, fw, S, 512)
+    keys, meta = _kv_requests(17, n, S, rows, fields, theta=0.99)
+    first = (1 << 32) - 700
+    sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=256, record_epochs=1, first_seq=first,
+                  **KV_FORMS[form])
+    _submit_in_waves(sv, keys, meta, waves=8)
+    sv.drain()
+    done = {d["id"]: d for d in sv.poll()}
+    st = sv.stats()
+    epochs = [sv.epoch(L.WORKLOAD_KV, e) for e in range(st["epochs"][0])]
+    sv.stop()
+    assert len(done) == n
+    seqs = sorted(d["seq"] for d in done.values())
+    assert seqs == list(range(first, first + n))
+    assert any(d["attempts"] > 1 and d["seq"] < (1 << 32) for d in done.values())
+    table = kc.initial_table(rows, fields, fw)
+    for e, ids in enumerate(epochs):
+        ids = ids.astype(np.int64)
+        tids, deltas = _replay_args(done, ids)
+        k = keys[ids].T.reshape(-1).copy()
+        m = meta[ids].T.reshape(-1).copy()
+        ep = oracle.kv_epoch(table, rows, fields, fw, k, m, tids, S, inplace=True, deltas=deltas, want_reads=False)
+        for t, rid in enumerate(ids):
+            assert ep.committed(t) == (done[rid]["epoch"] == e), f"request {rid} epoch {e}"
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+
+
+@pytest.mark.parametrize("form", ["device-loop", "host-formed"])
+def test_served_contended_results_alone_and_in_the_mix(gpu, form):
+    """Probes that contend with each other (300 requests over 48 rows), run
+    alone and again inside a contended mix of other key-value and
+    order-entry requests on other rows.  In both runs every probe commits
+    once; every read of a probe returns the initial field plus the deltas of
+    the probe writes to it committed in earlier epochs (reads see the
+    epoch's snapshot); and the probe rows end as the initial table plus
+    every probe's deltas.  So a probe's result differs between the runs
+    exactly by the writes the serial order puts before it, and by nothing
+    else."""
+    rows, fields, fw, S = 20000, 2, 2, 8
+    n_probe, hot = 300, 48
+    rng = np.random.default_rng(5)
+    pk = np.zeros((n_probe, S), np.uint64)
+    pm = np.zeros((n_probe, S), np.uint32)
+    for i in range(n_probe):
+        r = rng.choice(hot, 4, replace=False)
+        for s in range(4):
+            pk[i, s] = r[s]
+            pm[i, s] = L.kv_meta(L.KV_READ if rng.random() < 0.5 else L.KV_WRITE, int(rng.integers(0, fields)))
+    bk, bm = _kv_requests(78, 12000, S, rows - hot, fields, theta=0.99, lo=hot)
+    init = kc.initial_table(rows, fields, fw).reshape(rows, fields, fw)
+    for with_mix in (False, True):
+        ctx = _kv_ctx(rows, fields, fw, S, 4096)
+        tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=1, max_txns=4096)
+        tp.tpcc_populate(SEED)
+        sv = L.Server(ctx, tp, kv_ops=S, slots=2, keep_reads=1, **KV_FORMS[form])
+        ids = []
+        if with_mix:
+            def background():
+                gen = L.TpccGenerator(warehouses=1, seed=4)
+                for part_k, part_m in zip(np.array_split(bk, 20), np.array_split(bm, 20)):
+                    sv.submit_kv(part_k, part_m)
+                    sv.submit_tpcc(gen.next(300))
+            th = threading.Thread(target=background)
+            th.start()
+        for part_k, part_m in zip(np.array_split(pk, 10), np.array_split(pm, 10)):
+            first = sv.submit_kv(part_k, part_m)
+            ids.extend(range(first, first + len(part_k)))
+            time.sleep(0.001)
+        if with_mix:
+            th.join()
+        sv.drain()
+        done = {d["id"]: d for d in sv.poll()}
+        reads = [sv.read_values(i) for i in ids]
+        sv.stop()
+        tab = ctx.kv_table_read().reshape(rows, fields, fw)
+        assert all(i in done for i in ids)
+        # the probe writes, as (row, field, delta, commit epoch)
+        writes = [(int(pk[j, s]), int(pm[j, s] >> 8) & 0xFFFF, (done[rid]["tid"] | 1), done[rid]["epoch"])
+                  for j, rid in enumerate(ids) for s in range(4) if pm[j, s] & 3 == L.KV_WRITE]
+        final = init[:hot].astype(np.uint64).copy()
+        for r, f, d, _ in writes:
+            final[r, f] += d
+        np.testing.assert_array_equal(tab[:hot], (final & 0xFFFFFFFF).astype(np.uint32))
+        for j, rid in enumerate(ids):
+            e = done[rid]["epoch"]
+            want = []
+            for s in range(4):
+                if pm[j, s] & 3 != L.KV_READ:
+                    continue
+                r, f = int(pk[j, s]), int(pm[j, s] >> 8) & 0xFFFF
+                add = sum(d for (rr, ff, d, ee) in writes if rr == r and ff == f and ee < e)
+                want += [(int(x) + add) & 0xFFFFFFFF for x in init[r, f]]
+            np.testing.assert_array_equal(reads[j], np.array(want, dtype=np.uint32), err_msg=f"probe {j}")
 
 
 @pytest.mark.parametrize("plain", [0, 1], ids=["served", "plain-form"])
@@ -347,6 +468,34 @@ def test_served_readonly_requests_replay_on_the_oracle(gpu, plain):
         np.testing.assert_array_equal(tp.tpcc_table_read(t), tables[t])
 
 
+@pytest.mark.parametrize("form", ["device-loop", "host-formed", "order-entry"])
+def test_drain_returns_a_serving_error_instead_of_waiting(gpu, monkeypatch, form):
+    """A failed epoch (forced from the host: the server's test hook reports
+    the 4th completed epoch as a failed stream while every kernel runs
+    normally) ends the drain with LTPG_ERR_DEVICE instead of leaving it
+    waiting for commits that will not come; stop reports it too."""
+    monkeypatch.setenv("LTPG_TEST_FAIL_EPOCH", "3")
+    if form == "order-entry":
+        tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=1, max_txns=512)
+        tp.tpcc_populate(SEED)
+        sv = L.Server(None, tp, slots=2, max_epoch_txns=100)
+        sv.submit_tpcc(L.TpccGenerator(warehouses=1, seed=8).next(2000))
+    else:
+        rows, fields, fw, S = 300, 2, 1, 8
+        ctx = _kv_ctx(rows, fields, fw, S, 512)
+        keys, meta = _kv_requests(6, 3000, S, rows, fields, theta=0.99)
+        sv = L.Server(ctx, None, kv_ops=S, max_epoch_txns=64, **KV_FORMS[form])
+        sv.submit_kv(keys, meta)
+    t0 = time.monotonic()
+    with pytest.raises(L.LtpgError) as ei:
+        sv.drain()
+    assert ei.value.code == L.ERR_DEVICE and "injected" in str(ei.value)
+    assert time.monotonic() - t0 < 30
+    with pytest.raises(L.LtpgError) as ei:
+        sv.stop()
+    assert ei.value.code == L.ERR_DEVICE
+
+
 def test_hbm_budget_bounds_epochs(gpu):
     rows, fields, fw, S = 1000, 2, 1, 4
     ctx = _kv_ctx(rows, fields, fw, S, 4096)
@@ -363,17 +512,26 @@ def test_hbm_budget_bounds_epochs(gpu):
     assert st["committed"][0] == 3000 and 0 < st["max_epoch_txns"][0] <= 50
 
 
-def test_cancel_drops_what_was_not_admitted(gpu):
-    rows, fields, fw, S = 500, 2, 1, 8
+@pytest.mark.parametrize("form", ["device-loop", "host-formed"])
+def test_cancel_drops_what_was_not_admitted(gpu, form):
+    # 100,000 requests over 40 rows commit a few per epoch: a cancel right
+    # after the submit finds most of them pending and drops them
+    rows, fields, fw, S, n = 40, 2, 1, 8, 100_000
+    gc.collect()  # contexts of earlier tests freed now, not in the middle of the timing
     ctx = _kv_ctx(rows, fields, fw, S, 512)
-    keys, meta = _kv_requests(9, 20000, S, rows, fields, theta=0.99)
-    sv = L.Server(ctx, None, kv_ops=S, max_epoch_txns=256)
+    keys, meta = _kv_requests(9, n, S, rows, fields, theta=0.99)
+    sv = L.Server(ctx, None, kv_ops=S, max_epoch_txns=256, ring_requests=1 << 17, **KV_FORMS[form])
+    t0 = time.perf_counter()
     sv.submit_kv(keys, meta)
-    # cancelled right away: at most the epochs already formed (256
-    # transactions each, 2 in flight) can finish, the rest is dropped
-    committed = len(sv.poll())
+    ts = time.perf_counter()
+    committed = len(sv.poll(1 << 20))
+    st = sv.stats()
+    t1 = time.perf_counter()
     dropped = sv.cancel()
-    assert dropped > 0 and committed + dropped <= 20000
+    print(f"{form}: submit {(ts - t0) * 1e3:.1f} ms, {committed} committed before the cancel "
+          f"({(t1 - t0) * 1e3:.1f} ms after the submit began; stats {st}), {dropped} dropped; cancel took "
+          f"{(time.perf_counter() - t1) * 1e3:.1f} ms")
+    assert dropped > 0 and committed + dropped <= n
     # the context works epoch by epoch afterwards
     k, m, t = L.kv_generate(seed=1, rows=rows, fields=fields, n_txns=100, n_slots=S)
     ctx.kv_submit(k, m, t, S)
