@@ -1,19 +1,23 @@
 """Benchmark of the deterministic concurrency-control hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kv-c2|kv-c1|tpcc-c3] [--theta T]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|served-c5|kv-c2|tpcc-c3|batched-c4|...]
 
-A step is one epoch of the plain per-batch form (CID, EXECUTE,
-CHECK+COMMIT, WRITEBACK) over one batch whose inputs are already resident
-in HBM.  The default workload is the key-value configuration the metric is
-quoted on: 10M rows x 10 fields x 100 B (row-major 1,000 B records), batches
-of 2^20 transactions x 16 ops, 50/50 read / read-modify-write, Zipf theta
-0.6, with the Zipf 0.99 run and the order-entry configuration (W = 64,
-2^18 transactions, 50/50 NewOrder / Payment) as variants on the same line.
+The headline is the served form on the served-form configuration: Poisson
+arrivals of a 50/50 mix of key-value requests (1-64 ops, Zipf 0.99, 100M
+rows x 10 fields x 100 B) and order-entry requests (W = 1024, 50/50
+NewOrder / Payment) at --c5-rate M txn/s.  A step is one 1 ms window of
+arrivals; the timed span runs from the first arrival to the last commit,
+and ms_per_step = span / steps.  The line carries the plain per-batch form
+on the same arrivals, p50/p99 commit latency of both (at the headline rate
+and below capacity), abort rate and attempts per commit, the roofline of
+the dominant kernel (the device loop, per epoch) and the CPU baseline (the
+oracle replaying the served key-value lane on one host core).
 
-Prints one JSON line: committed txns/s (whole job), abort rate, p50/p99
-commit latency of the reference-style synchronous loop, per-phase device
-time, the roofline of the dominant kernel and the CPU baseline (the oracle
-timed on this host's cores on a bounded sample of the same workload).
+Compact variants on the same line: the key-value plain form (10M rows x 10
+x 100 B, 2^20 txns x 16 ops, Zipf 0.6 and 0.99), the order-entry plain form
+(W = 64, 2^18 txns) and the batched form (256 mixed batches of 1e3-1e6
+txns) against the plain loop, each with its roofline.  Every figure behind
+the line goes to --detail (gpurun_out/bench_detail.json).
 """
 import argparse
 import json
@@ -154,6 +158,30 @@ def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
                       f"{total_s:.1f} s"}
 
 
+def reference_convention(ctx, submit, n_epochs, types, phases):
+    """The reference's own measurement (TPCC-allopt-final/tp.cu:141-291,
+    Predefine.h:15): epochs run one at a time, each phase followed by a
+    synchronisation, then the commit counts read back; the first and last
+    10 % of epochs are dropped; TPS = mean commits per epoch / mean epoch
+    time (host clock).  Batches are already resident, so there is no H2D."""
+    t_ep, c_ep = [], []
+    for k in range(n_epochs):
+        t1 = time.perf_counter()
+        submit(k)
+        for ph in phases:
+            ctx.epoch_run(ph)
+            ctx.sync()
+        c = sum(ctx.commit_count(t) for t in types)
+        t_ep.append(time.perf_counter() - t1)
+        c_ep.append(c)
+    lo, hi = n_epochs // 10, n_epochs - n_epochs // 10
+    t_ep, c_ep = np.array(t_ep[lo:hi]), np.array(c_ep[lo:hi])
+    return {"value": float(c_ep.mean() / t_ep.mean()), "unit": "txn/s", "epochs_kept": int(hi - lo),
+            "ms_per_epoch": float(t_ep.mean() * 1e3),
+            "what": "the reference's convention: per epoch submit, each phase + sync, commit counts read back; "
+                    "first and last 10% of epochs dropped; mean commits / mean epoch time"}
+
+
 def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
     """hot_row = 0: row 0 is the hot key, as in the reference's ZIPFIAN mode
     (its byte accounting still counts row-0 writes as registrations)."""
@@ -217,6 +245,9 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
         ctx.commit_count()
         lat.append((time.perf_counter() - t1) * 1e3)
 
+    ref_conv = reference_convention(ctx, lambda k: ctx.kv_submit_device(dev[k % P]), max(10, args.steps),
+                                    [L.TYPE_KV], [L.PHASE_EXECUTE, L.PHASE_CHECK, L.PHASE_WRITEBACK])
+
     # the same epochs with the batch handed over in host memory: the submit
     # copies keys / meta / TIDs over PCIe (pageable numpy arrays) each epoch
     pcie_ms, pcie_commits = [], 0
@@ -279,6 +310,7 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "what": "submit -> commit count on host, one synchronous epoch at a time"},
         "pcie_inclusive": pcie,
+        "reference_convention": ref_conv,
         "kernel_ms": avg_ms,
         "kernel_algorithmic_gbs": kernel_gbs,
         "kernel_algorithmic_bytes": bytes_avg,
@@ -447,6 +479,8 @@ def run_tpcc(args, cpu_baseline=True, mix=False):
         ctx.commit_count(L.TYPE_PAYMENT)
         lat.append((time.perf_counter() - t1) * 1e3)
 
+    ref_conv = reference_convention(ctx, lambda k: submit(k % P), max(10, args.steps), list(types),
+                                    [L.PHASE_CID, L.PHASE_EXECUTE, L.PHASE_CHECK, L.PHASE_WRITEBACK])
     avg_ms = {L.TIMING_NAMES[i]: phase_ms[i] / max(n_timed, 1) for i in range(L.TIMING_SLOTS)}
     bytes_avg = {k: sum(per_batch[j % P][k] for j in range(args.steps)) / args.steps for k in TPCC_KERNELS}
     dom = max(TPCC_KERNELS, key=lambda k: avg_ms[k])
@@ -464,6 +498,7 @@ def run_tpcc(args, cpu_baseline=True, mix=False):
         "committed_txns": cn + cp + c_ro, "attempted_txns": an + ap_ + a_ro,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "what": "submit -> commit count on host, one synchronous epoch at a time"},
+        "reference_convention": ref_conv,
         "kernel_ms": avg_ms,
         "kernel_algorithmic_gbs": {k: bytes_avg[k] / (avg_ms[k] * 1e-3) / 1e9 for k in TPCC_KERNELS
                                    if avg_ms.get(k, 0) > 0},
@@ -506,12 +541,18 @@ def run_batched_c4(args):
     tp.tpcc_populate(seed)
     gen = L.TpccGenerator(warehouses=256, seed=seed)
     descs, keep, kinds = [], [], []
+    kv_ops = {"batch_bytes": 0, "valid": 0, "writes": 0, "reads": 0}
     tid = 0
     for i in range(nb):
         n = int(sizes[i])
         if is_kv[i]:
             k, m, t = L.kv_generate(seed=seed + i, rows=rows, fields=fields, n_txns=n, n_slots=S, min_ops=1,
                                     max_ops=S, theta=0.9, read_pct=50, tid_base=tid)
+            kinds_ = m & 3
+            kv_ops["batch_bytes"] += 4 * n + 12 * S * n
+            kv_ops["valid"] += int((kinds_ != 0).sum())
+            kv_ops["writes"] += int((kinds_ == 2).sum())
+            kv_ops["reads"] += int((kinds_ == 1).sum())
             db = L.DeviceKvBatch(kv, k, m, t, S)
             keep.append(db)
             descs.append(L.kv_desc(db))
@@ -571,6 +612,23 @@ def run_batched_c4(args):
     same = all(np.array_equal(a, b) for a, b in zip(plain_bm, graph_bm))
     commits = int(sum(int(np.unpackbits(b.view(np.uint8)).sum()) for b in graph_bm))
     total = int(sizes.sum())
+    # roofline of the dominant kernel, key-value EXECUTE, per pass over the
+    # key-value batches (tuned plain leg, HIP events): algorithmic bytes are
+    # the batches (tids, keys, metas) and one 8-byte register probe per valid
+    # op (a lower bound: register writes and flag bytes left out); its
+    # random-access bound is write ops / probe+atomic rate + read ops / load
+    # rate over an array the size of the registers
+    ex_ms = phases["plain_tuned_shapes"]["key_value"]["execute"]
+    ex_bytes = kv_ops["batch_bytes"] + 8 * kv_ops["valid"]
+    load_gops, atomic_gops = L.random_access_rates(kv, rows * 16, 1 << 24)
+    bound_ms = kv_ops["writes"] / (atomic_gops * 1e6) + kv_ops["reads"] / (load_gops * 1e6)
+    ex_gbs = ex_bytes / (ex_ms * 1e-3) / 1e9 if ex_ms > 0 else None
+    roof = {"kernel": "kv_execute (all key-value batches of a pass)", "bound": "hbm", "achieved": ex_gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ex_gbs / HBM_PEAK_GBS if ex_gbs else None, "traffic": None,
+            "algorithmic_bytes_per_pass": ex_bytes, "ms_per_pass": ex_ms,
+            "random_access": {"write_ops": kv_ops["writes"], "read_ops": kv_ops["reads"],
+                              "probe_atomic_gops": atomic_gops, "load_gops": load_gops, "bound_ms": bound_ms,
+                              "frac": bound_ms / ex_ms if ex_ms > 0 else None}}
     runner.close()
     for x in keep:
         for buf in (x if isinstance(x, list) else [x]):
@@ -588,6 +646,7 @@ def run_batched_c4(args):
             "plain_p50": float(np.percentile(plain_t, 50)), "plain_p99": float(np.percentile(plain_t, 99)),
             "batched_p50": float(np.percentile(graph_t, 50)), "batched_p99": float(np.percentile(graph_t, 99))},
         "phase_ms_per_pass": phases,
+        "roofline": roof,
         "autotuner": tuning,
         "config": {"workload": desc, "batches": nb, "kv_batches": int(is_kv.sum()), "seed": seed,
                    "form": "batched (hipGraph per workload, concurrent streams, autotuned) vs plain loop",
@@ -595,7 +654,7 @@ def run_batched_c4(args):
     }
 
 
-def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain):
+def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain, profile=False):
     """One offered rate of the served-form workload through a fresh server
     (plain = 1: the same admission run as the plain per-batch form).
     Requests are submitted in 1 ms windows as their scheduled arrival time
@@ -603,57 +662,77 @@ def _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain):
     oe_tid = np.r_[tb.nf("TID"), tb.pf("TID")]
     oe_order = np.argsort(oe_tid, kind="stable")  # request k of the batch's own arrival order
     arrival_of = {}  # request id -> arrival index
+    if profile:
+        os.environ["LTPG_SERVE_PROFILE"] = "1"
     sv = L.Server(kv, tp, kv_ops=S, slots=2, hbm_budget=args.c5_budget_mb << 20, plain=plain,
-                  epoch_floor=args.c5_epoch_floor)
+                  epoch_floor=args.c5_epoch_floor, ring_requests=1 << 20)
+    os.environ.pop("LTPG_SERVE_PROFILE", None)
+    # every window's submissions prepared before the clock starts
     window = 1e-3
     edges = np.searchsorted(at, np.arange(window, at[-1] + window, window), side="right")
     ki = oi = prev = 0
-    kv_ids, oe_ids = [], []
-    t0_ns = time.monotonic_ns()
+    plan = []  # (due time, key-value slice, its arrivals, order-entry batch, its arrivals)
     for e in edges:
         if e <= prev:
             continue
-        while time.monotonic_ns() - t0_ns < at[e - 1] * 1e9:
-            pass
         sel = np.arange(prev, e)
         kv_arr = sel[kind[prev:e] == 0]
         oe_arr = sel[kind[prev:e] == 1]
-        if len(kv_arr):
-            first = sv.submit_kv(keys[ki:ki + len(kv_arr)], meta[ki:ki + len(kv_arr)])
-            kv_ids.append((first, kv_arr))
-            ki += len(kv_arr)
+        kv_part = (keys[ki:ki + len(kv_arr)], meta[ki:ki + len(kv_arr)]) if len(kv_arr) else None
+        ki += len(kv_arr)
+        oe_part = None
         if len(oe_arr):
-            idx = oe_order[oi:oi + len(oe_arr)]
-            part, perm = _tpcc_select(tb, idx)
-            first = sv.submit_tpcc(part)
-            oe_ids.append((first, oe_arr[perm]))
+            part, perm = _tpcc_select(tb, oe_order[oi:oi + len(oe_arr)])
+            oe_part, oe_arr = part, oe_arr[perm]
             oi += len(oe_arr)
+        plan.append((at[e - 1] * 1e9, kv_part, kv_arr, oe_part, oe_arr))
         prev = e
+    kv_ids, oe_ids = [], []
+    t0_ns = time.monotonic_ns()
+    for due, kv_part, kv_arr, oe_part, oe_arr in plan:
+        while time.monotonic_ns() - t0_ns < due:
+            pass
+        if kv_part is not None:
+            kv_ids.append((sv.submit_kv(*kv_part), kv_arr))
+        if oe_part is not None:
+            oe_ids.append((sv.submit_tpcc(oe_part), oe_arr))
     submit_s = (time.monotonic_ns() - t0_ns) * 1e-9
     done = []
     deadline = time.monotonic() + args.c5_drain_s
     form = "plain" if plain else "served"
     while len(done) < n and time.monotonic() < deadline:
-        time.sleep(0.25)
-        done += sv.poll(1 << 26)
-        log(f"C5 {form}: {rate}M txn/s: {len(done)}/{n} committed")
+        time.sleep(0.02)
+        done += sv.poll(1 << 22)
+    log(f"C5 {form}: {rate}M txn/s: {len(done)}/{n} committed")
     st = sv.stats()
+    dev = sv.device_stats()
     dropped = sv.cancel()
     for first, arr in kv_ids + oe_ids:
         for j, a_i in enumerate(arr):
             arrival_of[first + j] = a_i
     sched = np.array([t0_ns + at[arrival_of[d["id"]]] * 1e9 for d in done])
     commit = np.array([d["commit_ns"] for d in done], dtype=np.float64)
+    kinds = np.array([d["type"] for d in done])
     lat = (commit - sched) * 1e-6
     span_s = (commit.max() - t0_ns) * 1e-9
-    return {"offered_txn_per_s": rate * 1e6, "requests": n, "committed": len(done), "dropped_at_cap": dropped,
-            "completed_fraction": len(done) / n, "drain_cap_s": args.c5_drain_s,
-            "committed_txn_per_s": len(done) / span_s, "submit_s": submit_s, "span_s": span_s,
-            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                           "max": float(lat.max())},
-            "epochs": {"kv": st["epochs"][0], "tpcc": st["epochs"][1]},
-            "max_epoch_txns": {"kv": st["max_epoch_txns"][0], "tpcc": st["max_epoch_txns"][1]},
-            "attempts_per_commit": sum(st["attempts"]) / max(sum(st["committed"]), 1)}
+    att = sum(st["attempts"])
+    out = {"offered_txn_per_s": rate * 1e6, "requests": n, "arrival_window_s": float(at[-1]),
+           "committed": len(done), "dropped_at_cap": dropped, "completed_fraction": len(done) / n,
+           "drain_cap_s": args.c5_drain_s, "committed_txn_per_s": len(done) / span_s,
+           "submit_s": submit_s, "offered_effective_txn_per_s": n / max(submit_s, at[-1]), "span_s": span_s,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+           "latency_ms_kv": {"p50": float(np.percentile(lat[kinds == 0], 50)),
+                             "p99": float(np.percentile(lat[kinds == 0], 99))} if (kinds == 0).any() else None,
+           "epochs": {"kv": st["epochs"][0], "tpcc": st["epochs"][1]},
+           "max_epoch_txns": {"kv": st["max_epoch_txns"][0], "tpcc": st["max_epoch_txns"][1]},
+           "attempts_per_commit": att / max(sum(st["committed"]), 1), "attempts_kv": st["attempts"][0],
+           "abort_rate": 1.0 - sum(st["committed"]) / max(att, 1),
+           "abort_rate_by_type": {name: 1.0 - st["committed"][i] / st["attempts"][i]
+                                  for i, name in enumerate(["kv", "neworder", "payment"]) if st["attempts"][i]}}
+    if dev["kv_loop"]:
+        out["device_loop"] = dev
+    return out
 
 
 def run_kv_small(args):
@@ -704,112 +783,205 @@ def run_kv_small(args):
             "config": {"workload": desc}}
 
 
+def cpu_baseline_c5(keys, meta, S, fields, fw, floor, growth, cap_max, budget_s=10.0, max_requests=200_000):
+    """The oracle replaying the served form's key-value lane on one host
+    core: the C5 key-value requests, formed into epochs by the device loop's
+    rule (the aborts of the last epoch head the next one, arrivals fill it up
+    to the cap max(floor, growth x last commits), at most cap_max), each
+    epoch one oracle epoch with rank priorities and the arrival deltas.  The
+    rows an epoch touches are renumbered into a small table for the call
+    (the rules see only row equality; values move in and out of a compact
+    copy of the touched rows, which starts as the reference fill); only the
+    oracle calls are timed.  Runs until budget_s of oracle time or
+    max_requests have committed."""
+    from oracle import oracle
+
+    n = min(len(keys), max_requests)
+    uniq, inv = np.unique(keys[:n].reshape(-1), return_inverse=True)
+    inv = inv.reshape(n, S)
+    g = np.empty((len(uniq), fields, fw), dtype=np.uint32)
+    g[:] = np.arange(fields, dtype=np.uint32)[None, :, None]
+    carry, nxt, last, commits, epochs, t_or = [], 0, None, 0, 0, 0.0
+    while (carry or nxt < n) and t_or < budget_s:
+        if growth == 0:
+            cap = cap_max
+        elif last is None:
+            cap = min(cap_max, floor)
+        else:
+            cap = min(cap_max, max(floor, growth * last))
+        take = max(0, cap - len(carry))
+        members = carry + list(range(nxt, min(n, nxt + take)))
+        nxt = min(n, nxt + take)
+        B = len(members)
+        rows_e, loc = np.unique(inv[members].reshape(-1), return_inverse=True)
+        table = np.ascontiguousarray(g[rows_e]).reshape(-1)
+        k = loc.reshape(B, S).T.reshape(-1).astype(np.uint64)
+        m = meta[members].T.reshape(-1).copy()
+        tids = np.arange(B, dtype=np.uint32)
+        deltas = ((np.asarray(members, dtype=np.uint64) & 0xFFFFFFFF) | 1).astype(np.uint32)
+        t1 = time.perf_counter()
+        ep = oracle.kv_epoch(table, len(rows_e), fields, fw, k, m, tids, S, values=1, want_reads=False,
+                             inplace=True, deltas=deltas)
+        t_or += time.perf_counter() - t1
+        g[rows_e] = table.reshape(len(rows_e), fields, fw)
+        ok = [ep.committed(t) for t in range(B)]
+        carry = [r for r, c in zip(members, ok) if not c]
+        last = sum(ok)
+        commits += last
+        epochs += 1
+    return {"value": commits / t_or, "unit": "txn/s", "cores": 1, "kind": "port",
+            "sample": f"oracle replay of the served key-value lane (device-loop epoch rule, cap {cap_max}, floor "
+                      f"{floor}, growth {growth}) over the first {n} C5 key-value requests: {commits} committed in "
+                      f"{epochs} epochs, {t_or:.1f} s of oracle time; order-entry requests not in the sample"}
+
+
+def host_cpu():
+    """Model and core count of the host this runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "usable_cores": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def c5_inputs(L, rate, seconds, rows, fields, S, gen, rng, seed, min_requests=20000):
+    """Arrivals of one offered rate: at least min_requests (a longer window
+    at low rates), half key-value (1-64 ops, Zipf 0.99), half order-entry."""
+    n = max(min_requests, int(rate * 1e6 * seconds))
+    n_kv = n // 2
+    keys, meta, _ = L.kv_generate(seed=seed + int(rate * 1000), rows=rows, fields=fields, n_txns=n_kv,
+                                  n_slots=S, min_ops=1, max_ops=S, theta=0.99)
+    keys = keys.reshape(S, n_kv).T.copy()
+    meta = meta.reshape(S, n_kv).T.copy()
+    tb = gen.next(n - n_kv)
+    at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
+    kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
+    return n, keys, meta, tb, at, kind
+
+
 def run_served_c5(args):
     """Served form under Poisson arrivals: a 50/50 mix of key-value requests
     (1-64 ops, Zipf 0.99, 100M rows x 10 x 100 B) and order-entry requests
-    (W = 1024, 50/50 NewOrder / Payment) offered at each rate in
-    --c5-rates.  Requests are submitted in 1 ms windows as their scheduled
-    arrival time comes; latency is commit time minus the scheduled arrival
-    (so a submitter that falls behind the schedule counts as queueing).
-    Aborted requests re-queue with their TID.  The drain is capped at
-    --c5-drain-s: requests still uncommitted then are dropped and counted
-    (the completed fraction), and latency covers the committed ones."""
+    (W = 1024, 50/50 NewOrder / Payment) offered at each rate in --c5-rates,
+    the plain per-batch form on the same arrivals at the rates in
+    --c5-plain-rates.  Requests are submitted in 1 ms windows as their
+    scheduled arrival time comes; latency is commit time minus the scheduled
+    arrival (so a submitter that falls behind the schedule counts as
+    queueing).  Aborted requests re-queue with their priority.  The drain is
+    capped at --c5-drain-s: requests still uncommitted then are dropped and
+    counted (the completed fraction), and latency covers the committed ones.
+
+    The headline is the served form at --c5-rate: its arrival window is
+    --steps windows of 1 ms (after --warmup windows of arrivals served and
+    drained untimed on the same tables), and ms_per_step is the span from
+    the first arrival to the last commit divided by steps."""
     import ltpg_amd as L
 
     seed = 0x4C545047
     rows, fields, fw, S, W = args.c5_rows, 10, 25, 64, 1024
-    desc = (f"served form: Poisson arrivals, 50/50 key-value (1-64 ops, Zipf 0.99, {rows // 1_000_000}M rows x 10 "
-            f"x 100 B) and order-entry (W={W}) requests, {args.c5_seconds} s of arrivals per rate")
+    desc = (f"served form (C5): Poisson arrivals, 50/50 key-value (1-64 ops, Zipf 0.99, {rows // 1_000_000}M rows x "
+            f"10 x 100 B) and order-entry (W={W}, 50/50 NewOrder/Payment) requests")
     log(f"C5: {desc}")
     kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                    max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW)
     kv.kv_fill_reference()
-    # served epochs are small (the server sizes them by their commits under
-    # contention): those of at most 2048 op slots run as one single-workgroup
-    # kernel; for larger ones per-op registration with row folding,
-    # 64-transaction chunks and per-lane field copies, the fastest shape for
-    # epochs of 64-1024 transactions of this table
+    # host-formed epochs (the plain form's) of up to 2048 op slots run as one
+    # single-workgroup kernel; larger ones per-op registration with row
+    # folding, 64-transaction chunks and per-lane field copies
     kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY)
     tp = L.Context(workload=L.WORKLOAD_TPCC, tpcc_warehouses=W, max_txns=1 << 20, values=L.VALUES_RMW)
     tp.tpcc_populate(seed)
     log("C5: tables resident")
     rng = np.random.default_rng(seed)
     gen = L.TpccGenerator(warehouses=W, seed=seed)
-    out, plain_out = [], []
-    for rate in args.c5_rates:
-        n = max(1000, int(rate * 1e6 * args.c5_seconds))
-        n_kv = n // 2
-        keys, meta, _ = L.kv_generate(seed=seed + int(rate * 1000), rows=rows, fields=fields, n_txns=n_kv,
-                                      n_slots=S, min_ops=1, max_ops=S, theta=0.99)
-        keys = keys.reshape(S, n_kv).T.copy()
-        meta = meta.reshape(S, n_kv).T.copy()
-        tb = gen.next(n - n_kv)
-        at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
-        kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
-        forms = [("served", out, 0)] + ([("plain", plain_out, 1)] if not args.c5_no_plain else [])
-        for form, dest, plain in forms:
-            dest.append(_c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain))
-            r = dest[-1]
-            log(f"C5 {form}: offered {rate}M txn/s -> {r['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
+    head_rate = args.c5_rate
+    if args.warmup > 0:
+        n, keys, meta, tb, at, kind = c5_inputs(L, head_rate, args.warmup * 1e-3, rows, fields, S, gen, rng,
+                                                seed + 17, min_requests=1000)
+        _c5_rate(args, L, kv, tp, S, head_rate, n, keys, meta, tb, at, kind, 0)
+        log("C5: warm-up served and drained")
+    rates = sorted(set(args.c5_rates) | {head_rate})
+    served, plain = {}, {}
+    inputs = {}
+    for rate in rates:
+        secs = args.steps * 1e-3 if rate == head_rate else args.c5_seconds
+        inputs[rate] = c5_inputs(L, rate, secs, rows, fields, S, gen, rng, seed,
+                                 min_requests=1000 if rate == head_rate else 20000)
+        n, keys, meta, tb, at, kind = inputs[rate]
+        forms = [("served", served, 0)] + ([("plain", plain, 1)] if rate in args.c5_plain_rates else [])
+        for form, dest, pl in forms:
+            r = _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, pl)
+            dest[rate] = r
+            log(f"C5 {form}: offered {rate}M txn/s -> {r['committed_txn_per_s'] / 1e6:.3f}M committed/s, "
                 f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms, "
-                f"submit {r['submit_s']:.3f} s for {at[-1]:.3f} s of arrivals")
+                f"{r['completed_fraction'] * 100:.0f}% in the drain cap, submit {r['submit_s']:.3f} s for "
+                f"{r['arrival_window_s']:.3f} s of arrivals")
+    # the per-part chain of the device loop's epochs: one profiled pass at the
+    # headline rate (profiling adds ~1.5 us per epoch, so it is not timed)
+    n, keys, meta, tb, at, kind = inputs[head_rate]
+    prof = _c5_rate(args, L, kv, tp, S, head_rate, n, keys, meta, tb, at, kind, 0, profile=True)
+    # the reference's ZIPFIAN mode: row 0 as the hot key (its writes reduce
+    # commutatively and never conflict), on a fresh table
+    hot = {}
     kv.close()
-    # the reference's ZIPFIAN mode makes row 0 a hot key (Y0/Predefine.h:12,
-    # Y0/Execute.h:63-73): its RMW writes go to the commutative reduce and
-    # never register or conflict.  The same arrivals again, on a table
-    # with row 0 as the hot row (a fresh context: the hot row is fixed at
-    # creation, and two 128 GB tables do not fit beside each other).
-    hot_out = []
     if args.c5_hot_rates:
         kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                        max_txns=1 << 20, kv_max_slots=S, values=L.VALUES_RMW, kv_hot_row=0)
         kv.kv_fill_reference()
         kv.kv_set_tuning(L.TUNE_EXEC_PER_OP | L.TUNE_SMALL_CHUNKS | L.TUNE_LANE_COPY)
         for rate in args.c5_hot_rates:
-            n = max(1000, int(rate * 1e6 * args.c5_seconds))
-            n_kv = n // 2
-            keys, meta, _ = L.kv_generate(seed=seed + int(rate * 1000), rows=rows, fields=fields, n_txns=n_kv,
-                                          n_slots=S, min_ops=1, max_ops=S, theta=0.99)
-            keys = keys.reshape(S, n_kv).T.copy()
-            meta = meta.reshape(S, n_kv).T.copy()
-            tb = gen.next(n - n_kv)
-            at = np.cumsum(rng.exponential(1.0 / (rate * 1e6), n))
-            kind = rng.permutation(np.r_[np.zeros(n_kv, np.int8), np.ones(n - n_kv, np.int8)])
-            entry = {}
-            for form, plain in (("served", 0), ("plain", 1)):
-                r = _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, plain)
-                entry[form] = r
-                log(f"C5 hot row 0, {form}: offered {rate}M txn/s -> {r['committed_txn_per_s'] / 1e6:.2f}M committed/s, "
-                    f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms")
-            hot_out.append(entry)
+            n, keys, meta, tb, at, kind = inputs.get(rate) or c5_inputs(L, rate, args.c5_seconds, rows, fields, S,
+                                                                        gen, rng, seed)
+            hot[rate] = {form: _c5_rate(args, L, kv, tp, S, rate, n, keys, meta, tb, at, kind, pl)
+                         for form, pl in (("served", 0), ("plain", 1))}
+            log(f"C5 hot row 0 at {rate}M/s: served {hot[rate]['served']['committed_txn_per_s'] / 1e6:.3f}M, "
+                f"plain {hot[rate]['plain']['committed_txn_per_s'] / 1e6:.3f}M committed/s")
         kv.close()
     tp.close()
-    best = max(out, key=lambda r: r["committed_txn_per_s"])
-    res = {"value": best["committed_txn_per_s"], "unit": "txn/s", "ms_per_step": None,
-           "latency_ms": best["latency_ms"], "rates": out,
-           "config": {"workload": desc, "rates_mtxn_s": list(args.c5_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
-                      "slots": 2, "epoch_growth": 8, "epoch_floor": args.c5_epoch_floor,
-                      "kv_launch_shape": "epochs of <= 2048 op slots: one single-workgroup kernel; larger: "
-                                         "TUNE_EXEC_PER_OP | TUNE_SMALL_CHUNKS | TUNE_LANE_COPY",
-                      "form": "served (continuous admission, requeue with original TID)"}}
-    if plain_out:
-        # the plain per-batch form on the same arrivals and admission: one
-        # epoch at a time, host-copied slot-major batch, phases, sync,
-        # bitmap read back (ltpg_serve_config.plain)
-        pbest = max(plain_out, key=lambda r: r["committed_txn_per_s"])
-        res["plain_value"] = pbest["committed_txn_per_s"]
-        res["speedup_vs_plain"] = res["value"] / max(pbest["committed_txn_per_s"], 1e-9)
-        res["plain_rates"] = plain_out
-        res["vs_plain_by_rate"] = [
-            {"offered_txn_per_s": a["offered_txn_per_s"],
-             "committed_ratio": a["committed_txn_per_s"] / max(b["committed_txn_per_s"], 1e-9),
-             "p99_ms": a["latency_ms"]["p99"], "plain_p99_ms": b["latency_ms"]["p99"]}
-            for a, b in zip(out, plain_out)]
-    if hot_out:
-        res["reference_zipfian_hot_key"] = {
-            "what": "the same workload with row 0 as the hot key, as the reference's ZIPFIAN mode runs it (its RMW "
-                    "writes reduced commutatively, never registered); served and plain forms at each rate",
-            "rates": hot_out}
+    h = served[head_rate]
+    # roofline of the dominant kernel, the device loop, per epoch: HBM bytes
+    # are the committed writes' read-modify-writes of their fields (every
+    # request commits once; a request's write ops are its WRITE slots)
+    n, keys, meta, tb, at, kind = inputs[head_rate]
+    writes = int(((meta & 3) == 2).sum())
+    dl = h.get("device_loop", {})
+    epochs = max(dl.get("epochs", 0), 1)
+    epoch_us = dl.get("busy_ns", 0) / epochs / 1e3
+    hbm_per_epoch = writes * 2 * 4 * fw / epochs
+    achieved = hbm_per_epoch / (epoch_us * 1e-6) / 1e9 if epoch_us > 0 else None
+    pdl = prof.get("device_loop", {})
+    parts = {k: v / max(pdl.get("profile_epochs", 0), 1) / 1e3 for k, v in pdl.get("profile_ns", {}).items()
+             if k != "unused"}
+    roof = {"kernel": "kv_serve_loop_kernel", "unit_of_work": "one epoch", "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+            "algorithmic_bytes_per_epoch": hbm_per_epoch,
+            # request ring reads (12 B per padded op slot) and done-ring records
+            "pcie_bytes_per_epoch": (12 * S * (n // 2) + 8 * (epochs + h["attempts_kv"])) / epochs,
+            "epoch_us": epoch_us,
+            "latency_chain": {"what": "the epoch is a dependent chain of LDS phases separated by workgroup barriers "
+                                      "plus one PCIe-published record; per-part device time of a profiled pass "
+                                      "(profiling adds ~1.5 us per epoch)", "parts_us": parts,
+                              "profiled_epoch_us": sum(parts.values()) if parts else None}}
+    res = {"value": h["committed_txn_per_s"], "unit": "txn/s", "span_s": h["span_s"], "served": served,
+           "plain": plain, "profiled_pass": prof, "hot_row_0": hot, "roofline": roof,
+           "config": {"workload": desc, "offered_txn_per_s": head_rate * 1e6, "rates_mtxn_s": rates,
+                      "plain_rates_mtxn_s": sorted(args.c5_plain_rates), "hbm_budget_mb_per_slot": args.c5_budget_mb,
+                      "kv_form": "device loop: one persistent single-workgroup kernel forms and runs the key-value "
+                                 "epochs (requests through a pinned ring, aborts kept on the device)",
+                      "tpcc_form": "host-formed epochs, 2 in flight", "epoch_growth": 8,
+                      "epoch_floor": args.c5_epoch_floor, "inputs": "arrive over time; tables resident in HBM"}}
+    if not args.no_cpu_baseline:
+        log("C5: cpu baseline (oracle replay of the served key-value lane)")
+        floor = args.c5_epoch_floor
+        res["cpu_baseline"] = cpu_baseline_c5(keys, meta, S, fields, fw, floor, 8, 2048 // S)
+        res["cpu_baseline"]["host"] = host_cpu()
     return res
 
 
@@ -865,86 +1037,156 @@ def copy_calibration(gib):
     ctx.close()
 
 
+ROOF_KEYS = ("kernel", "bound", "achieved", "peak", "unit", "frac", "traffic")
+
+
+def compact(v, name):
+    """The few figures of a variant that go on the bench line."""
+    out = {"workload": name, "value": v.get("value"), "unit": v.get("unit", "txn/s")}
+    for k in ("ms_per_step", "abort_rate", "speedup_vs_plain", "plain_value", "commit_sets_identical"):
+        if k in v:
+            out[k] = v[k]
+    if "latency_ms" in v:
+        out["latency_ms"] = {k: v["latency_ms"][k] for k in v["latency_ms"] if k != "what"}
+    if "roofline" in v:
+        out["roofline"] = {k: v["roofline"].get(k) for k in ROOF_KEYS}
+        ra = v["roofline"].get("random_access")
+        if ra:
+            out["roofline"]["random_access_frac"] = ra.get("frac")
+    if "reference_convention" in v:
+        out["reference_convention_txn_per_s"] = v["reference_convention"]["value"]
+    if "cpu_baseline" in v:
+        out["cpu_baseline"] = {k: v["cpu_baseline"][k] for k in ("value", "unit", "cores", "kind")}
+    return out
+
+
+def write_detail(path, detail):
+    if not path:
+        return None
+    try:
+        os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(detail, f, indent=1, default=float)
+        return path
+    except OSError as e:
+        log(f"detail file not written: {e}")
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kv-c2",
-                    choices=["kv-c2", "kv-c1", "kv-scan", "kv-small", "tpcc-c3", "tpcc-mix", "batched-c4",
-                             "served-c5"])
-    ap.add_argument("--c5-rates", type=float, nargs="+", default=[1, 5, 10, 25, 50],
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "served-c5", "kv-c2", "kv-c1", "kv-scan", "kv-small", "tpcc-c3", "tpcc-mix",
+                             "batched-c4"])
+    ap.add_argument("--c5-rate", type=float, default=5.0,
+                    help="offered load of the headline served run, M txn/s (its window is --steps ms)")
+    ap.add_argument("--c5-rates", type=float, nargs="+", default=[0.02, 0.1, 1, 5, 25, 50],
                     help="offered loads of the served form, M txn/s")
+    ap.add_argument("--c5-plain-rates", type=float, nargs="*", default=[0.02, 0.1, 1, 5],
+                    help="offered loads (M txn/s) also run in the plain per-batch form")
     ap.add_argument("--c5-seconds", type=float, default=0.02, help="arrival window per offered load")
     ap.add_argument("--c5-drain-s", type=float, default=8.0, help="cap on the drain after the arrivals, per load")
     ap.add_argument("--c5-hot-rates", type=float, nargs="*", default=[5],
                     help="offered rates (M txn/s) also run with row 0 as the hot key (the reference's ZIPFIAN mode)")
-    ap.add_argument("--c5-no-plain", action="store_true",
-                    help="skip the plain per-batch form run at each C5 rate")
     ap.add_argument("--c5-rows", type=int, default=100_000_000, help="key-value rows of the served form")
     ap.add_argument("--c5-budget-mb", type=int, default=4096, help="HBM budget per in-flight epoch, MiB")
-    ap.add_argument("--c5-epoch-floor", type=int, default=32,
+    ap.add_argument("--c5-epoch-floor", type=int, default=16,
                     help="smallest epoch the commit-based epoch cap allows (ltpg_serve_config.epoch_floor)")
     ap.add_argument("--c4-batches", type=int, default=256, help="batches of the batched-form comparison")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form variant of the default run")
     ap.add_argument("--c4-plain-only", action="store_true",
-                    help="C4: run the plain legs only (kernel traces; rocprofv3 cannot follow graph launches)")
-    ap.add_argument("--theta", type=float, default=None,
-                    help="one Zipf theta (default for kv-c2: 0.6 headline plus 0.99 variant)")
+                    help="C4: run the plain legs only (kernel traces of the plain loop)")
+    ap.add_argument("--theta", type=float, default=None, help="Zipf theta of kv-c2 (default 0.6)")
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--default-shape", action="store_true",
                     help="key-value plain runs keep the default launch shape instead of the fastest of the 16")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file for every figure behind the line (empty: none)")
     ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
                     help="only run the known-size copy used to calibrate traffic counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is written up "
+              "beside the kernels' description).", file=sys.stderr)
         sys.exit(2)
     if args.steps < 1:
         raise SystemExit("--steps must be >= 1")
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    keep = ("value", "unit", "ms_per_step", "abort_rate", "latency_ms", "kernel_ms", "kernel_algorithmic_gbs",
-            "roofline", "config")
-    variants = []
-    if args.workload == "tpcc-c3":
-        result = run_tpcc(args)
-    elif args.workload == "tpcc-mix":
-        result = run_tpcc(args, mix=True)
-    elif args.workload == "kv-small":
-        result = run_kv_small(args)
-    elif args.workload == "served-c5":
-        v = run_served_c5(args)
-        result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True,
-                  "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic"} | v
-    elif args.workload == "batched-c4":
-        v = run_batched_c4(args)
-        result = {"metric": METRIC, "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True,
-                  "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic"} | v
-    else:
-        thetas = [args.theta] if args.theta is not None else ([0.6, 0.99] if args.workload == "kv-c2" else [None])
-        result = run_kv(args, thetas[0])
-        for th in thetas[1:]:
-            v = run_kv(args, th, cpu_baseline=False)
-            variants.append({k: v[k] for k in keep})
-        if args.workload == "kv-c2" and args.theta is None:
-            v = run_kv(args, 0.99, cpu_baseline=False, hot_row=0)
-            variants.append({k: v[k] for k in keep})
+    base = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32", "data": "synthetic"}
+    detail = {}
+    if args.workload in ("all", "served-c5"):
+        c5 = run_served_c5(args)
+        detail["served_c5"] = c5
+        h = c5["served"][args.c5_rate]
+        p = c5["plain"].get(args.c5_rate)
+        low = min(c5["served"])
+        result = dict(base, value=c5["value"], unit="txn/s", steps=args.steps, warmup=args.warmup,
+                      ms_per_step=c5["span_s"] * 1e3 / args.steps)
+        result["config"] = dict(c5["config"], timed="first arrival to last commit of --steps 1 ms arrival windows")
+        result["abort_rate"] = h["abort_rate"]
+        result["abort_rate_by_type"] = h["abort_rate_by_type"]
+        result["attempts_per_commit"] = h["attempts_per_commit"]
+        result["latency_ms"] = {k: h["latency_ms"][k] for k in ("p50", "p99")}
+        if p:
+            result["plain_form"] = {"value": p["committed_txn_per_s"],
+                                    "latency_ms": {k: p["latency_ms"][k] for k in ("p50", "p99")},
+                                    "completed_fraction": p["completed_fraction"]}
+            result["speedup_vs_plain"] = c5["value"] / max(p["committed_txn_per_s"], 1e-9)
+        result["below_capacity"] = {
+            "offered_txn_per_s": low * 1e6, "served_txn_per_s": c5["served"][low]["committed_txn_per_s"],
+            "served_latency_ms": {k: c5["served"][low]["latency_ms"][k] for k in ("p50", "p99")},
+            **({"plain_txn_per_s": c5["plain"][low]["committed_txn_per_s"],
+                "plain_latency_ms": {k: c5["plain"][low]["latency_ms"][k] for k in ("p50", "p99")}}
+               if low in c5["plain"] else {})}
+        result["by_rate"] = [{"offered": r, "served_txn_per_s": round(c5["served"][r]["committed_txn_per_s"]),
+                              "served_p99_ms": round(c5["served"][r]["latency_ms"]["p99"], 3),
+                              **({"plain_txn_per_s": round(c5["plain"][r]["committed_txn_per_s"]),
+                                  "plain_p99_ms": round(c5["plain"][r]["latency_ms"]["p99"], 3),
+                                  "plain_completed": round(c5["plain"][r]["completed_fraction"], 3)}
+                                 if r in c5["plain"] else {})} for r in sorted(c5["served"])]
+        result["roofline"] = {k: c5["roofline"].get(k) for k in ROOF_KEYS}
+        result["roofline"].update({"unit_of_work": "epoch", "epoch_us": c5["roofline"]["epoch_us"],
+                                   "chain_parts_us": c5["roofline"]["latency_chain"]["parts_us"]})
+        if "cpu_baseline" in c5:
+            result["cpu_baseline"] = c5["cpu_baseline"]
+        variants = []
+        if args.workload == "all":
+            for th in (0.6, 0.99):
+                v = run_kv(args, th, cpu_baseline=th == 0.6, name="kv-c2")
+                detail[f"kv_c2_theta{th}"] = v
+                variants.append(compact(v, f"C2 key-value plain form, Zipf {th}"))
             v = run_tpcc(args)
-            variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
+            detail["tpcc_c3"] = v
+            variants.append(compact(v, "C3 order-entry plain form"))
+            v = run_batched_c4(args)
+            detail["batched_c4"] = v
+            variants.append(compact(v, "C4 batched form vs plain loop"))
+            result["variants"] = variants
+    else:
+        if args.workload == "tpcc-c3":
+            v = run_tpcc(args)
+        elif args.workload == "tpcc-mix":
             v = run_tpcc(args, mix=True)
-            variants.append({k: v[k] for k in keep + ("abort_rate_by_type", "cpu_baseline") if k in v})
-            v = run_kv(args, None, name="kv-scan")
-            variants.append({k: v[k] for k in keep + ("cpu_baseline",) if k in v})
-            if not args.no_batched:
-                variants.append(run_batched_c4(args))
-                variants.append(run_served_c5(args))
-    if variants:
-        result["variants"] = variants
-    print(json.dumps(result))
+        elif args.workload == "kv-small":
+            v = run_kv_small(args)
+        elif args.workload == "batched-c4":
+            v = dict(base, steps=1, warmup=1, **run_batched_c4(args))
+        else:
+            v = run_kv(args, args.theta if args.workload == "kv-c2" else None)
+        detail[args.workload] = v
+        result = dict(base)
+        result.update({k: v[k] for k in v if k in ("value", "unit", "steps", "warmup", "ms_per_step", "config",
+                                                   "abort_rate", "latency_ms", "roofline", "cpu_baseline",
+                                                   "speedup_vs_plain", "plain_value", "commit_sets_identical")})
+    where = write_detail(args.detail, detail)
+    result["detail"] = where
+    print(json.dumps(result, default=float))
 
 
 if __name__ == "__main__":

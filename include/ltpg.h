@@ -591,6 +591,24 @@ typedef struct ltpg_serve_stats {
     uint64_t max_epoch_txns[2];
 } ltpg_serve_stats;
 
+/* The key-value device loop (ltpg_serve_config.kv_loop) as the device
+ * measured it.  No counterpart in the reference. */
+typedef struct ltpg_serve_device_stats {
+    uint32_t size;            /* sizeof(ltpg_serve_device_stats), set by the caller */
+    uint32_t kv_loop;         /* 1 when the key-value lane runs the device loop   */
+    uint64_t launches;        /* kernel launches (it returns when idle, etc.)      */
+    uint64_t epochs;          /* epochs it ran                                    */
+    uint64_t busy_ns;         /* device wall-clock time inside those epochs       */
+    uint64_t exits[5];        /* launches ended by: 0 other, 1 stop, 2 idle,
+                                 3 full done ring, 4 time slice                   */
+    uint64_t prof_ns[10];     /* environment LTPG_SERVE_PROFILE set at start:
+                                 time in each part of the epochs (control read,
+                                 admission, EXECUTE, read marking, CHECK, unused,
+                                 read-out and write-back, publication, requeue,
+                                 idle) -- a profiled epoch takes ~1.5 us longer */
+    uint64_t prof_epochs;     /* epochs profiled                                  */
+} ltpg_serve_device_stats;
+
 typedef struct ltpg_server ltpg_server;
 
 void ltpg_serve_config_init(ltpg_serve_config *c);
@@ -628,6 +646,7 @@ int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *i
 /* Block until every submitted request has committed. */
 int ltpg_serve_drain(ltpg_server *sv);
 int ltpg_serve_stats_get(ltpg_server *sv, ltpg_serve_stats *st);
+int ltpg_serve_device_stats_get(ltpg_server *sv, ltpg_serve_device_stats *st);
 /* Drain, stop the thread and free the server (the contexts stay). */
 int ltpg_serve_stop(ltpg_server *sv);
 /* Drop every request not in an epoch yet (new and re-queued, including the

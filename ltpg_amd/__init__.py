@@ -708,6 +708,17 @@ class Server:
         return {"committed": list(st.committed), "attempts": list(st.attempts), "epochs": list(st.epochs),
                 "max_epoch_txns": list(st.max_epoch_txns)}
 
+    def device_stats(self):
+        """The key-value device loop as the device measured it (empty
+        without one)."""
+        st = _lib.ServeDeviceStats()
+        st.size = ctypes.sizeof(st)
+        _check(lib.ltpg_serve_device_stats_get(self.handle, ctypes.byref(st)), "ltpg_serve_device_stats_get")
+        parts = ["control", "admit", "execute", "mark", "check", "unused", "write", "publish", "requeue", "idle"]
+        return {"kv_loop": bool(st.kv_loop), "launches": st.launches, "epochs": st.epochs, "busy_ns": st.busy_ns,
+                "exits": dict(zip(["other", "stop", "idle", "stall", "slice"], list(st.exits))),
+                "profile_ns": dict(zip(parts, list(st.prof_ns))), "profile_epochs": st.prof_epochs}
+
     def stop(self):
         if self.handle:
             h, self.handle = self.handle, None

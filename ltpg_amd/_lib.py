@@ -149,6 +149,14 @@ class ServeStats(ctypes.Structure):
                 ("max_epoch_txns", ctypes.c_uint64 * 2)]
 
 
+class ServeDeviceStats(ctypes.Structure):
+    """struct ltpg_serve_device_stats."""
+
+    _fields_ = [("size", ctypes.c_uint32), ("kv_loop", ctypes.c_uint32), ("launches", ctypes.c_uint64),
+                ("epochs", ctypes.c_uint64), ("busy_ns", ctypes.c_uint64), ("exits", ctypes.c_uint64 * 5),
+                ("prof_ns", ctypes.c_uint64 * 10), ("prof_epochs", ctypes.c_uint64)]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("ltpg_config_init", None, [ctypes.POINTER(Config)]),
@@ -224,6 +232,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, c_u64p, ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_drain", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_serve_stats_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeStats)]),
+    ("ltpg_serve_device_stats_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeDeviceStats)]),
     ("ltpg_serve_stop", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_serve_cancel", ctypes.c_int, [ctypes.c_void_p, c_u64p]),
     ("ltpg_kv_gen_init", None, [ctypes.POINTER(KvGen)]),

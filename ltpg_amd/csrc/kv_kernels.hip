@@ -1361,6 +1361,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __shared__ uint32_t st_meta[kSmallOps];
     __shared__ uint64_t s_st_head;
     __shared__ uint32_t s_st_n;
+    __shared__ uint64_t s_busy;  // wall-clock ticks inside epochs
     __shared__ uint64_t s_ctl[4];
     __shared__ uint32_t s_bad;
 
@@ -1429,6 +1430,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     if (threadIdx.x == 0) {
         s_st_head = head;
         s_st_n = 0;
+        s_busy = 0;
     }
     __syncthreads();
     while (true) {
@@ -1739,6 +1741,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             s_ctl[1] = c_stop;
             s_ctl[2] = c_dh;
             s_ctl[3] = wall_clock64();
+            s_busy += s_ctl[3] - now;
             s_st_head = head;
             s_st_n = pf_n;
         }
@@ -1763,9 +1766,10 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __syncthreads();
     if (threadIdx.x == 0) {
         if (prof) {
-            for (int k = 0; k < kServeProfParts; ++k) ctl->prof[k] = s_prof[k];
-            ctl->prof_epochs = s_prof[kServeProfParts + 1];
+            for (int k = 0; k < kServeProfParts; ++k) ctl->prof[k] += s_prof[k];
+            ctl->prof_epochs += s_prof[kServeProfParts + 1];
         }
+        ctl->busy_ticks += s_busy;
         sys_store(&ctl->n_carry, n_carry);
         sys_store(&ctl->cap, cap);
         sys_store(&ctl->epochs, epoch);

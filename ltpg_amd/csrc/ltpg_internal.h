@@ -308,8 +308,10 @@ struct ServeCtl {
     uint64_t n_carry;    // requests carried to the next epoch
     uint64_t cap;        // the next epoch's transaction cap
     uint64_t pad1;
+    // wall-clock ticks inside epochs (all launches)
+    uint64_t busy_ticks;
     // profile (ServeLoopArgs.profile): wall-clock ticks spent in each part
-    // of the epochs of the last launch (kServeProf* order), and their count
+    // of the epochs, all launches, and the number of epochs profiled
     uint64_t prof[10];
     uint64_t prof_epochs;
 };

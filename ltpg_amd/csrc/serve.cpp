@@ -212,6 +212,7 @@ struct Loop {
     bool launched = false;
     uint64_t launched_ns = 0;
     uint64_t launches = 0;
+    uint64_t exits[5] = {};
     uint64_t slice_ns = 0;
     double ticks_per_us = 100.0;
 };
@@ -926,6 +927,8 @@ bool tend_loop(ltpg_server *sv)
     if (lp.launched) {
         if (ld_acquire(&lp.ctl->running) == 0) {
             lp.launched = false;
+            const uint64_t code = lp.ctl->exit_code;
+            lp.exits[code < 5 ? code : 0]++;
             if (sv->trace)
                 std::fprintf(stderr, "[serve] device loop exit %llu after %.3f ms: head %llu of %llu, carry %llu, "
                                      "%llu epochs\n",
@@ -1419,6 +1422,29 @@ int ltpg_serve_drain(ltpg_server *sv)
     ClientLock g(sv);
     sv->cv.wait(g.lk, [&] { return sv->outstanding == 0 || sv->error; });
     if (sv->error) LTPG_FAIL(sv->error, sv->error_msg);
+    return LTPG_OK;
+}
+
+int ltpg_serve_device_stats_get(ltpg_server *sv, ltpg_serve_device_stats *st)
+{
+    if (!sv || !st) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    if (st->size != sizeof(ltpg_serve_device_stats)) LTPG_FAIL(LTPG_ERR_INVALID, "ltpg_serve_device_stats.size mismatch");
+    ClientLock g(sv);
+    const uint32_t size = st->size;
+    std::memset(st, 0, sizeof(*st));
+    st->size = size;
+    const Loop *lp = sv->lane[0].loop;
+    if (!lp) return LTPG_OK;
+    st->kv_loop = 1;
+    st->launches = lp->launches;
+    // the kernel writes these at each exit; while it runs they hold the
+    // previous launches' values
+    st->epochs = ld_acquire(&lp->ctl->epochs);
+    const double ns_per_tick = 1000.0 / lp->ticks_per_us;
+    st->busy_ns = (uint64_t)(lp->ctl->busy_ticks * ns_per_tick);
+    for (int k = 0; k < 5; k++) st->exits[k] = lp->exits[k];
+    for (int k = 0; k < kServeProfParts; k++) st->prof_ns[k] = (uint64_t)(lp->ctl->prof[k] * ns_per_tick);
+    st->prof_epochs = lp->ctl->prof_epochs;
     return LTPG_OK;
 }
 

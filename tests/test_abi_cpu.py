@@ -54,7 +54,7 @@ C_STRUCTS = {"Config": "ltpg_config", "KvBatch": "ltpg_kv_batch", "KvGen": "ltpg
              "OrderstatusBatch": "ltpg_orderstatus_batch", "StocklevelBatch": "ltpg_stocklevel_batch",
              "TpccGenConfig": "ltpg_tpcc_gen_config", "BatchDesc": "ltpg_batch_desc",
              "TuningEntry": "ltpg_tuning_entry", "ServeConfig": "ltpg_serve_config", "ServeDone": "ltpg_serve_done",
-             "ServeStats": "ltpg_serve_stats"}
+             "ServeStats": "ltpg_serve_stats", "ServeDeviceStats": "ltpg_serve_device_stats"}
 
 
 def test_struct_layouts_match_a_compiled_c_caller(tmp_path):
