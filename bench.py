@@ -964,7 +964,7 @@ def run_served_c5(args):
             "algorithmic_bytes_per_epoch": hbm_per_epoch,
             # request ring reads (12 B per padded op slot) and done-ring records
             "pcie_bytes_per_epoch": (12 * S * (n // 2) + 8 * (epochs + h["attempts_kv"])) / epochs,
-            "epoch_us": epoch_us,
+            "epoch_us": epoch_us, "epochs": dl.get("epochs", 0), "kernel_busy_ms": dl.get("busy_ns", 0) / 1e6,
             "latency_chain": {"what": "the epoch is a dependent chain of LDS phases separated by workgroup barriers "
                                       "plus one PCIe-published record; per-part device time of a profiled pass "
                                       "(profiling adds ~1.5 us per epoch)", "parts_us": parts,
@@ -1152,6 +1152,7 @@ def main():
                                  if r in c5["plain"] else {})} for r in sorted(c5["served"])]
         result["roofline"] = {k: c5["roofline"].get(k) for k in ROOF_KEYS}
         result["roofline"].update({"unit_of_work": "epoch", "epoch_us": c5["roofline"]["epoch_us"],
+                                   "epochs": c5["roofline"]["epochs"],
                                    "chain_parts_us": c5["roofline"]["latency_chain"]["parts_us"]})
         if "cpu_baseline" in c5:
             result["cpu_baseline"] = c5["cpu_baseline"]

@@ -1048,9 +1048,10 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
         // pageable sources: the copies finish before the caller may reuse them
         LTPG_HIP_TRY(hipStreamSynchronize(s));
     }
-    // Until a CID phase runs, each Payment uses its query's C_ID.
-    if (n_pay)
-        LTPG_HIP_TRY(hipMemcpyAsync(tp.d.cid, pay.C_ID, (uint64_t)n_pay * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    // Until a CID phase runs, each Payment uses its query's C_ID (a copy
+    // kernel rather than a memcpy: a batched run captures this submit into a
+    // graph, which then holds kernel nodes only)
+    if (n_pay) LTPG_HIP_TRY(launch_copy_u32(tp.d.cid, pay.C_ID, n_pay, ctx->stream));
     tp.no = no;
     tp.pay = pay;
     tp.ro = ReadOnlyDev();

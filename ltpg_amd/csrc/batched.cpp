@@ -111,16 +111,26 @@ int refill_second_set(ltpg_batched *b, ltpg_ctx *ctx)
     return LTPG_OK;
 }
 
-// main waits for everything issued so far on side, and side for main
-int join_streams(ltpg_batched *b, hipStream_t main, hipStream_t side)
+// Capture topology: side joins the capture by a fork from main (an event
+// recorded on the capturing main stream, which side then waits for), and
+// rejoins main by a join (an event recorded on side, which main waits for).
+// Every event is recorded inside the capture before it is waited for, and
+// the capture ends on main after a join, with nothing left on side.
+int fork_side(ltpg_batched *b, hipStream_t main, hipStream_t side)
 {
-    hipEvent_t a, c;
-    LTPG_HIP_TRY(new_event(b, &a));
-    LTPG_HIP_TRY(hipEventRecord(a, side));
-    LTPG_HIP_TRY(hipStreamWaitEvent(main, a, 0));
-    LTPG_HIP_TRY(new_event(b, &c));
-    LTPG_HIP_TRY(hipEventRecord(c, main));
-    LTPG_HIP_TRY(hipStreamWaitEvent(side, c, 0));
+    hipEvent_t e;
+    LTPG_HIP_TRY(new_event(b, &e));
+    LTPG_HIP_TRY(hipEventRecord(e, main));
+    LTPG_HIP_TRY(hipStreamWaitEvent(side, e, 0));
+    return LTPG_OK;
+}
+
+int join_side(ltpg_batched *b, hipStream_t main, hipStream_t side)
+{
+    hipEvent_t e;
+    LTPG_HIP_TRY(new_event(b, &e));
+    LTPG_HIP_TRY(hipEventRecord(e, side));
+    LTPG_HIP_TRY(hipStreamWaitEvent(main, e, 0));
     return LTPG_OK;
 }
 
@@ -131,8 +141,8 @@ int enqueue_kv_pipelined(ltpg_batched *b)
     hipStream_t main = ctx->stream, side = b->side;
     int rc;
     if ((rc = refill_second_set(b, ctx))) return rc;
-    if ((rc = join_streams(b, main, side))) return rc;  // side joins the capture after the refills
-    std::vector<hipEvent_t> checked;                    // per key-value batch: its CHECK is done
+    if ((rc = fork_side(b, main, side))) return rc;  // side joins the capture after the refills
+    std::vector<hipEvent_t> checked;                 // per key-value batch: its CHECK is done
     uint32_t j = 0;
     for (uint32_t i = 0; i < b->d.size(); i++) {
         const ltpg_batch_desc &d = b->d[i];
@@ -142,14 +152,14 @@ int enqueue_kv_pipelined(ltpg_batched *b)
             // the submit below wraps the epoch and refills the first set on
             // main: nothing may be in flight on side, and the second set
             // starts over too
-            if ((rc = join_streams(b, main, side))) return rc;
+            if ((rc = join_side(b, main, side))) return rc;
             if ((rc = refill_second_set(b, ctx))) return rc;
         }
         const uint32_t before = ctx->epoch;
         if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
         if (ctx->epoch < before) {
-            if ((rc = join_streams(b, main, side))) return rc;  // side waits for the refills
-            checked.assign(checked.size(), nullptr);            // older checks precede the refill
+            if ((rc = fork_side(b, main, side))) return rc;  // side waits for the refills
+            checked.assign(checked.size(), nullptr);        // older checks precede the refill
         }
         EpochTag e;
         e.epoch = (uint8_t)ctx->epoch;
@@ -177,7 +187,7 @@ int enqueue_kv_pipelined(ltpg_batched *b)
         LTPG_HIP_TRY(launch_stamp(b->stamps + i, main));
         j++;
     }
-    return join_streams(b, main, side);  // side rejoins main before the capture ends
+    return join_side(b, main, side);  // side rejoins main before the capture ends
 }
 
 // Capture the epochs of workload slot k into a graph.
