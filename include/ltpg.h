@@ -229,6 +229,12 @@ int ltpg_kv_table_read(ltpg_ctx *ctx, void *host, uint64_t rows, uint32_t layout
 /* Device-side fill with the reference's initial contents: every u32 word of
  * field c holds c (YCSB/0/Database.h:72-99). */
 int ltpg_kv_table_fill_reference(ltpg_ctx *ctx);
+/* Rows rows[0 .. n-1] of the table, all fields, packed into host
+ * (n * kv_fields * kv_field_bytes bytes; the reference copies its whole
+ * snapshot back, YCSB/0/Database.h:131; here a resident table of up to
+ * hundreds of GB is read by row).  Synchronous; LTPG_ERR_RANGE for a row
+ * outside the table. */
+int ltpg_kv_table_read_rows(ltpg_ctx *ctx, const uint64_t *rows, uint64_t n, uint32_t *host);
 
 /* ---- key-value batch (YCSB_A_QUERY, YCSB/0/Datastructure.h:36-41) ------- */
 /* Slot-major, as the reference kernel indexes LOC[B * op + NO]

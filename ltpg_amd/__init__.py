@@ -221,6 +221,15 @@ class Context:
         _check(lib.ltpg_kv_table_read(self.handle, _ptr(out), rows, layout), "ltpg_kv_table_read")
         return out
 
+    def kv_table_read_rows(self, rows):
+        """Rows `rows` of the table, shape (n, fields, field_words)."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        _, fields, fw = self.kv_shape
+        out = np.empty((rows.size, fields, fw), dtype=np.uint32)
+        _check(lib.ltpg_kv_table_read_rows(self.handle, rows.ctypes.data_as(_lib.c_u64p), rows.size,
+                                           out.ctypes.data_as(_lib.c_u32p)), "ltpg_kv_table_read_rows")
+        return out
+
     def kv_set_tuning(self, flags=0):
         """LTPG_TUNE_* launch shape of the key-value kernels (results never change)."""
         _check(lib.ltpg_kv_set_tuning(self.handle, int(flags)), "ltpg_kv_set_tuning")

@@ -614,6 +614,33 @@ int ltpg_kv_table_read(ltpg_ctx *ctx, void *host, uint64_t rows, uint32_t layout
     return LTPG_OK;
 }
 
+int ltpg_kv_table_read_rows(ltpg_ctx *ctx, const uint64_t *rows, uint64_t n, uint32_t *host)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    KvState &kv = ctx->kv;
+    if (!kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
+    if (n && (!rows || !host)) LTPG_FAIL(LTPG_ERR_INVALID, "null argument");
+    for (uint64_t i = 0; i < n; i++)
+        if (rows[i] >= kv.rows) LTPG_FAIL(LTPG_ERR_RANGE, "row index outside the table");
+    if (n == 0) return LTPG_OK;
+    const uint64_t per = (uint64_t)kv.fields * kv.field_words;
+    uint64_t *d_idx = nullptr;
+    uint32_t *d_out = nullptr;
+    if ((rc = dev_alloc(&d_idx, n, "row indices")) || (rc = dev_alloc(&d_out, n * per, "gathered rows"))) {
+        dev_free(d_idx);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_idx, rows, n * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_kv_gather_rows(kv, d_idx, n, d_out, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d_out, n * per * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(d_idx);
+    dev_free(d_out);
+    if (e != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("row read: ") + hipGetErrorString(e));
+    return LTPG_OK;
+}
+
 int ltpg_kv_table_fill_reference(ltpg_ctx *ctx)
 {
     int rc = check_ctx(ctx);

@@ -1259,6 +1259,21 @@ __global__ __launch_bounds__(kBlock) void kv_hot_apply_kernel(uint32_t *__restri
     for (uint32_t k = 0; k < fw; ++k) w[k] += total;
 }
 
+// Rows `idx` of the table, every field, packed (row i of the output holds
+// fields * fw words); one thread per output word.  The caller has checked
+// every index against the table.
+__global__ __launch_bounds__(kBlock) void kv_gather_rows_kernel(const uint32_t *__restrict__ table,
+                                                                const uint64_t *__restrict__ idx, uint64_t n,
+                                                                uint32_t fields, uint32_t fw, uint32_t fsw,
+                                                                uint32_t *__restrict__ out)
+{
+    const uint64_t per = (uint64_t)fields * fw;
+    for (uint64_t x = (uint64_t)blockIdx.x * kBlock + threadIdx.x; x < n * per; x += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t i = x / per, r = x % per, f = r / fw, w = r % fw;
+        out[x] = table[(idx[i] * fields + f) * fsw + w];
+    }
+}
+
 // ---- served key-value epochs in one persistent workgroup ------------------
 //
 // kv_serve_loop_kernel runs the served form's key-value epochs back to back
@@ -2061,6 +2076,14 @@ hipError_t launch_kv_small_epoch(const KvState &kv, const KvBatchDev &b, uint32_
                        b.n_slots, b.request_major, kv.rows, kv.fields, kv.field_words, kv.field_stride, kv.max_scan,
                        kv.read_slot, values, kv.hot_row, kv.table, kv.read_values, kv.bitmap, host_bits, seq_out, seq,
                        ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv_gather_rows(const KvState &kv, const uint64_t *idx, uint64_t n, uint32_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kv_gather_rows_kernel, grid_stride(n * kv.fields * kv.field_words), dim3(kBlock), 0, s,
+                       kv.table, idx, n, kv.fields, kv.field_words, kv.field_stride, out);
     return hipGetLastError();
 }
 

@@ -260,6 +260,8 @@ hipError_t launch_stamp(unsigned long long *out, hipStream_t s);
 // start of a captured batch sequence use it.
 int refill_registers(ltpg_ctx *ctx);
 hipError_t launch_kv_fill_reference(const KvState &kv, hipStream_t s);
+// rows idx[0 .. n-1] (device, each < kv.rows) of every field, packed, to out (device)
+hipError_t launch_kv_gather_rows(const KvState &kv, const uint64_t *idx, uint64_t n, uint32_t *out, hipStream_t s);
 hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, uint32_t fw,
                          uint32_t fsw, int to_strided, hipStream_t s);
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
