@@ -660,6 +660,33 @@ int ltpg_serve_stop(ltpg_server *sv);
  * and free the server.  *dropped = requests that never committed. */
 int ltpg_serve_cancel(ltpg_server *sv, uint64_t *dropped);
 
+/* ---- a key-value epoch that allocates nothing --------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: one key-value epoch (EXECUTE, CHECK with the read-out of
+ * committed reads, WRITEBACK; the fast form, no hot row, no scans) over a
+ * caller-owned device table, with the same kernels and results as
+ * ltpg_epoch_run on a context.  It calls no allocator: the per-row registers,
+ * per-transaction flags and counters live in a fixed device array of the
+ * library (about 19 MB, held by each GPU it runs on until the process ends).
+ *   table        device, rows x fields x field_bytes, row-major, packed;
+ *                updated in place (RMW: a write adds TID | 1 to each word)
+ *   batch        device arrays, slot-major (slot s of txn t at s*B + t)
+ *   bitmap       device, ceil(n_txns / 64) words: the commit bitmap
+ *   read_values  device, n_slots * n_txns * field_bytes / 4 words, laid out
+ *                as ltpg_kv_read_values (max_scan 0)
+ *   result       device, 2 words, or NULL: commit count, ops out of range
+ *   hip_stream   the stream the work is queued on (NULL: the null stream)
+ * Asynchronous.  Sizes beyond LTPG_NOALLOC_MAX_ROWS / LTPG_NOALLOC_MAX_TXNS
+ * are refused with LTPG_ERR_INVALID.  Every call on one GPU uses that GPU's
+ * one array: calls that overlap on a GPU (two streams, two threads) race;
+ * serialise them. */
+#define LTPG_NOALLOC_MAX_ROWS (1u << 20)
+#define LTPG_NOALLOC_MAX_TXNS (1u << 20)
+int ltpg_kv_epoch_noalloc(uint32_t *table, uint64_t rows, uint32_t fields, uint32_t field_bytes,
+                          const ltpg_kv_batch *batch, uint32_t values, uint64_t *bitmap, uint32_t *read_values,
+                          uint32_t *result, void *hip_stream);
+
 /* ---- seeded workload generators (replace MakeYCSB / Random.h, whose
  *      rand() after srand(time(0)) cannot be reproduced: YCSB/0/Query.h:
  *      17-121, YCSB/0/Random.h:12,52) ----------------------------------- */

@@ -58,6 +58,7 @@ NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
 TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_LANE_COPY, TUNE_ALL = 1, 2, 4, 8, 15
 TUNE_SMALL_EPOCH, SMALL_EPOCH_OPS = 16, 2048
+NOALLOC_MAX_ROWS = NOALLOC_MAX_TXNS = 1 << 20
 
 
 def kv_meta(kind, field):

@@ -178,6 +178,9 @@ SIGNATURES = [
     ("ltpg_kv_table_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     ("ltpg_kv_table_fill_reference", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_kv_table_read_rows", ctypes.c_int, [ctypes.c_void_p, c_u64p, ctypes.c_uint64, c_u32p]),
+    ("ltpg_kv_epoch_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(KvBatch), ctypes.c_uint32,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("ltpg_kv_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_kv_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KvBatch), ctypes.c_uint32]),
     ("ltpg_epoch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
