@@ -27,9 +27,12 @@
  *   - A context is single-threaded: the caller serialises calls on one
  *     context.  Separate contexts are independent.
  *   - The library owns every device allocation.  Batches given with
- *     LTPG_MEM_HOST are copied to the device before the call returns;
- *     batches given with LTPG_MEM_DEVICE are read in place and must stay
- *     valid until ltpg_sync() returns.
+ *     LTPG_MEM_HOST are copied to the device before the call returns (the
+ *     caller may reuse its buffers then), into one of two device slots on a
+ *     copy stream, so the copy overlaps the previous epoch still running on
+ *     the device; pinned host memory copies at full PCIe speed.  Batches
+ *     given with LTPG_MEM_DEVICE are read in place and must stay valid until
+ *     ltpg_sync() returns.
  *   - TIDs are u32 priorities; a lower TID wins.  The reference relies on
  *     TIDs being distinct within a batch (TID = (epoch & 0xfff) << 20 | seq,
  *     TPCC-allopt-final/Query.h:500); here equal TIDs are ordered by batch

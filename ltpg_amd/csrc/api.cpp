@@ -35,9 +35,11 @@ void free_kv(KvState &kv)
     dev_free(kv.read_values);
     dev_free(kv.bitmap);
     dev_free(kv.conflicts);
-    dev_free(kv.keys_buf);
-    dev_free(kv.meta_buf);
-    dev_free(kv.tids_buf);
+    for (int k = 0; k < 2; k++) {
+        dev_free(kv.keys_buf[k]);
+        dev_free(kv.meta_buf[k]);
+        dev_free(kv.tids_buf[k]);
+    }
     dev_free(kv.hot_acc);
 }
 
@@ -74,9 +76,9 @@ int init_kv(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&kv.read_values, S * B * kv.read_slot * kv.field_words, "kv read values"))) return rc;
     if ((rc = dev_alloc(&kv.bitmap, (B + 63) / 64, "kv bitmap"))) return rc;
     if ((rc = dev_alloc(&kv.conflicts, B, "kv conflicts"))) return rc;
-    if ((rc = dev_alloc(&kv.keys_buf, S * B, "kv keys"))) return rc;
-    if ((rc = dev_alloc(&kv.meta_buf, S * B, "kv meta"))) return rc;
-    if ((rc = dev_alloc(&kv.tids_buf, B, "kv tids"))) return rc;
+    if ((rc = dev_alloc(&kv.keys_buf[0], S * B, "kv keys"))) return rc;
+    if ((rc = dev_alloc(&kv.meta_buf[0], S * B, "kv meta"))) return rc;
+    if ((rc = dev_alloc(&kv.tids_buf[0], B, "kv tids"))) return rc;
     if (kv.hot_row != LTPG_NO_HOT_ROW) {
         if ((rc = dev_alloc(&kv.hot_acc, (uint64_t)kCounterStripes * kv.fields, "hot-row accumulators"))) return rc;
         LTPG_HIP_TRY(hipMemsetAsync(kv.hot_acc, 0, (uint64_t)kCounterStripes * kv.fields * 4, ctx->stream));
@@ -116,9 +118,11 @@ void free_tpcc(TpccState &tp)
     dev_free(tp.d.reads_no);
     dev_free(tp.d.reads_pay);
     dev_free(tp.d.cid);
-    dev_free(tp.no_buf);
-    dev_free(tp.pay_buf);
-    dev_free(tp.ro_buf);
+    for (int k = 0; k < 2; k++) {
+        dev_free(tp.no_buf[k]);
+        dev_free(tp.pay_buf[k]);
+        dev_free(tp.ro_buf[k]);
+    }
     dev_free(tp.d.war);
     dev_free(tp.d.bitmap_os);
     dev_free(tp.d.bitmap_sl);
@@ -163,9 +167,9 @@ int init_tpcc(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&d.reads_no, (uint64_t)kNoReads * B, "NewOrder read values"))) return rc;
     if ((rc = dev_alloc(&d.reads_pay, (uint64_t)kPayReads * B, "Payment read values"))) return rc;
     if ((rc = dev_alloc(&d.cid, B, "Payment C_ID"))) return rc;
-    if ((rc = dev_alloc(&tp.no_buf, 53 * B, "NewOrder batch"))) return rc;
-    if ((rc = dev_alloc(&tp.pay_buf, 10 * B, "Payment batch"))) return rc;
-    if ((rc = dev_alloc(&tp.ro_buf, 9 * B, "read-only batches"))) return rc;
+    if ((rc = dev_alloc(&tp.no_buf[0], 53 * B, "NewOrder batch"))) return rc;
+    if ((rc = dev_alloc(&tp.pay_buf[0], 10 * B, "Payment batch"))) return rc;
+    if ((rc = dev_alloc(&tp.ro_buf[0], 9 * B, "read-only batches"))) return rc;
     if ((rc = dev_alloc(&d.war, B, "read-only war flags"))) return rc;
     if ((rc = dev_alloc(&d.bitmap_os, (B + 63) / 64, "Order-Status bitmap"))) return rc;
     if ((rc = dev_alloc(&d.bitmap_sl, (B + 63) / 64, "Stock-Level bitmap"))) return rc;
@@ -380,11 +384,17 @@ int ltpg_destroy(ltpg_ctx *ctx)
     if (!ctx) return LTPG_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);  // copies into the batch slots
     free_kv(ctx->kv);
     free_tpcc(ctx->tpcc);
     dev_free(ctx->counters);
     for (auto &ev : ctx->ev_pool)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+    for (int k = 0; k < 2; k++) {
+        if (ctx->xfer_copied[k]) (void)hipEventDestroy(ctx->xfer_copied[k]);
+        if (ctx->xfer_released[k]) (void)hipEventDestroy(ctx->xfer_released[k]);
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return LTPG_OK;
@@ -668,6 +678,61 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
 
 // ---- key-value batches --------------------------------------------------
 
+}  // extern "C"
+
+namespace {
+
+// Transport of host-resident batches.  The reference keeps two device query
+// slots and copies the batch of epoch e+1 into one while epoch e runs on the
+// other (TPCC-allopt-final/Query.h:532-605, Predefine.h:18-19).  Here a host
+// submit takes the slot the epoch before last used, copies into it on the
+// context's copy stream once that epoch is done with it, and makes its own
+// epoch's kernels (on the context stream) wait for the copy: the copy of a
+// batch overlaps the epoch still running on the other slot.  The submit
+// returns once its copy is done, so the caller may reuse its buffers at once
+// (pageable memory is staged by the runtime; pinned memory goes at full
+// PCIe speed), without waiting for the device to go idle.
+int xfer_begin(ltpg_ctx *ctx, int *slot)
+{
+    if (!ctx->copy_stream) {
+        LTPG_HIP_TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        for (int k = 0; k < 2; k++) {
+            LTPG_HIP_TRY(hipEventCreateWithFlags(&ctx->xfer_copied[k], hipEventDisableTiming));
+            LTPG_HIP_TRY(hipEventCreateWithFlags(&ctx->xfer_released[k], hipEventDisableTiming));
+        }
+    }
+    const int k = ctx->xfer_slot ^ 1;
+    // everything queued so far -- the epoch on the other slot and all before
+    // it -- is what must finish before the other slot is free again
+    LTPG_HIP_TRY(hipEventRecord(ctx->xfer_released[k ^ 1], ctx->stream));
+    ctx->xfer_released_set[k ^ 1] = true;
+    if (ctx->xfer_released_set[k]) LTPG_HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->xfer_released[k], 0));
+    ctx->xfer_slot = k;
+    *slot = k;
+    return LTPG_OK;
+}
+
+// The slot's copies are queued on the copy stream: the epoch waits for them,
+// and the caller's buffers are free once they are done.
+int xfer_end(ltpg_ctx *ctx)
+{
+    const int k = ctx->xfer_slot;
+    LTPG_HIP_TRY(hipEventRecord(ctx->xfer_copied[k], ctx->copy_stream));
+    LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->xfer_copied[k], 0));
+    LTPG_HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    return LTPG_OK;
+}
+
+template <typename T>
+int slot_buffer(T **buf, uint64_t count, const char *what)
+{
+    return *buf ? LTPG_OK : dev_alloc(buf, count, what);
+}
+
+}  // namespace
+
+extern "C" {
+
 int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
 {
     int rc = check_ctx(ctx);
@@ -692,18 +757,24 @@ int ltpg_kv_submit(ltpg_ctx *ctx, const ltpg_kv_batch *b, uint32_t mem)
         d.meta = b->meta;
         d.tids = b->tids;
     } else {
-        const uint64_t n = (uint64_t)b->n_txns * b->n_slots;
+        int k = 0;
+        const uint64_t cap = (uint64_t)ctx->cfg.max_txns, n = (uint64_t)b->n_txns * b->n_slots;
+        if ((rc = xfer_begin(ctx, &k))) return rc;
+        if ((rc = slot_buffer(&kv.keys_buf[k], kv.max_slots * cap, "kv keys")) ||
+            (rc = slot_buffer(&kv.meta_buf[k], kv.max_slots * cap, "kv meta")) ||
+            (rc = slot_buffer(&kv.tids_buf[k], cap, "kv tids")))
+            return rc;
+        hipStream_t cs = ctx->copy_stream;
         if (n) {
-            LTPG_HIP_TRY(hipMemcpyAsync(kv.keys_buf, b->keys, n * 8, hipMemcpyHostToDevice, ctx->stream));
-            LTPG_HIP_TRY(hipMemcpyAsync(kv.meta_buf, b->meta, n * 4, hipMemcpyHostToDevice, ctx->stream));
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.keys_buf[k], b->keys, n * 8, hipMemcpyHostToDevice, cs));
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.meta_buf[k], b->meta, n * 4, hipMemcpyHostToDevice, cs));
         }
         if (b->n_txns)
-            LTPG_HIP_TRY(hipMemcpyAsync(kv.tids_buf, b->tids, (uint64_t)b->n_txns * 4, hipMemcpyHostToDevice, ctx->stream));
-        // pageable source: the copy must finish before the caller may reuse it
-        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        d.keys = kv.keys_buf;
-        d.meta = kv.meta_buf;
-        d.tids = kv.tids_buf;
+            LTPG_HIP_TRY(hipMemcpyAsync(kv.tids_buf[k], b->tids, (uint64_t)b->n_txns * 4, hipMemcpyHostToDevice, cs));
+        if ((rc = xfer_end(ctx))) return rc;
+        d.keys = kv.keys_buf[k];
+        d.meta = kv.meta_buf[k];
+        d.tids = kv.tids_buf[k];
     }
     kv.cur = d;
     kv.has_batch = true;
@@ -1044,11 +1115,18 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
             pay.H_ID = pb->H_ID; pay.TID = pb->TID;
         }
     } else {
-        // pack into the library's NEWORDER_QUERY / PAYMENT_QUERY blocks
+        // pack into the library's NEWORDER_QUERY / PAYMENT_QUERY blocks of
+        // the next host slot
         const uint64_t C = ctx->cfg.max_txns;
-        hipStream_t s = ctx->stream;
+        int slot = 0;
+        if ((rc = xfer_begin(ctx, &slot))) return rc;
+        if ((rc = slot_buffer(&tp.no_buf[slot], 53 * C, "NewOrder batch")) ||
+            (rc = slot_buffer(&tp.pay_buf[slot], 10 * C, "Payment batch")) ||
+            (rc = slot_buffer(&tp.ro_buf[slot], 9 * C, "read-only batches")))
+            return rc;
+        hipStream_t s = ctx->copy_stream;
         if (n_no) {
-            uint32_t *b = tp.no_buf;
+            uint32_t *b = tp.no_buf[slot];
             const uint32_t *heads[7] = {nb->W_ID, nb->D_ID, nb->C_ID, nb->O_ID, nb->N_O_ID, nb->O_OL_CNT, nb->O_OL_ID};
             for (int k = 0; k < 7; k++)
                 LTPG_HIP_TRY(hipMemcpyAsync(b + k * C, heads[k], (uint64_t)n_no * 4, hipMemcpyHostToDevice, s));
@@ -1063,7 +1141,7 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
             no.OL_QUANTITY = b + 37 * C; no.TID = b + 52 * C;
         }
         if (n_pay) {
-            uint32_t *b = tp.pay_buf;
+            uint32_t *b = tp.pay_buf[slot];
             const uint32_t *cols[10] = {pb->W_ID, pb->D_ID, pb->C_ID, pb->C_LAST, pb->isName,
                                         pb->C_D_ID, pb->C_W_ID, pb->H_AMOUNT, pb->H_ID, pb->TID};
             for (int k = 0; k < 10; k++)
@@ -1072,9 +1150,9 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
             pay.C_D_ID = b + 5 * C; pay.C_W_ID = b + 6 * C; pay.H_AMOUNT = b + 7 * C; pay.H_ID = b + 8 * C;
             pay.TID = b + 9 * C;
         }
-        // pageable sources: the copies finish before the caller may reuse them
-        LTPG_HIP_TRY(hipStreamSynchronize(s));
+        if ((rc = xfer_end(ctx))) return rc;
     }
+    tp.host_slot = mem == LTPG_MEM_HOST;
     // Until a CID phase runs, each Payment uses its query's C_ID (a copy
     // kernel rather than a memcpy: a batched run captures this submit into a
     // graph, which then holds kernel nodes only)
@@ -1113,20 +1191,27 @@ int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *ob, c
         for (int k = 0; k < 6; k++) os_dev[k] = os_src[k];
         for (int k = 0; k < 3; k++) sl_dev[k] = sl_src[k];
     } else {
+        // the epoch's host slot (its NewOrders and Payments took it), or, if
+        // they were device-resident, the next one
         const uint64_t C = ctx->cfg.max_txns;
+        int slot = ctx->xfer_slot;
+        if (!tp.host_slot && (rc = xfer_begin(ctx, &slot))) return rc;
+        tp.host_slot = true;
+        if ((rc = slot_buffer(&tp.ro_buf[slot], 9 * C, "read-only batches"))) return rc;
+        uint32_t *rb = tp.ro_buf[slot];
         for (int k = 0; k < 6; k++) {
-            os_dev[k] = tp.ro_buf + k * C;
+            os_dev[k] = rb + k * C;
             if (n_os)
-                LTPG_HIP_TRY(hipMemcpyAsync(tp.ro_buf + k * C, os_src[k], (uint64_t)n_os * 4, hipMemcpyHostToDevice,
-                                            ctx->stream));
+                LTPG_HIP_TRY(hipMemcpyAsync(rb + k * C, os_src[k], (uint64_t)n_os * 4, hipMemcpyHostToDevice,
+                                            ctx->copy_stream));
         }
         for (int k = 0; k < 3; k++) {
-            sl_dev[k] = tp.ro_buf + (6 + k) * C;
+            sl_dev[k] = rb + (6 + k) * C;
             if (n_sl)
-                LTPG_HIP_TRY(hipMemcpyAsync(tp.ro_buf + (6 + k) * C, sl_src[k], (uint64_t)n_sl * 4,
-                                            hipMemcpyHostToDevice, ctx->stream));
+                LTPG_HIP_TRY(hipMemcpyAsync(rb + (6 + k) * C, sl_src[k], (uint64_t)n_sl * 4, hipMemcpyHostToDevice,
+                                            ctx->copy_stream));
         }
-        LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));  // pageable sources
+        if ((rc = xfer_end(ctx))) return rc;
     }
     ro.os_W_ID = os_dev[0]; ro.os_D_ID = os_dev[1]; ro.os_C_ID = os_dev[2];
     ro.os_O_ID = os_dev[3]; ro.os_OL_ID = os_dev[4]; ro.os_TID = os_dev[5];

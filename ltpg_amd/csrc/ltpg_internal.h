@@ -81,10 +81,12 @@ struct KvState {
     uint32_t *read_values = nullptr;    // max_slots * max_txns * field_words
     uint64_t *bitmap = nullptr;         // ceil(max_txns / 64)
     uint32_t *conflicts = nullptr;      // max_txns
-    // library-owned copy of host-submitted batches
-    uint64_t *keys_buf = nullptr;
-    uint32_t *meta_buf = nullptr;
-    uint32_t *tids_buf = nullptr;
+    // library-owned copies of host-submitted batches: two slots, so that
+    // the next batch's copy overlaps the current epoch (slot 1 allocated at
+    // its first use)
+    uint64_t *keys_buf[2] = {nullptr, nullptr};
+    uint32_t *meta_buf[2] = {nullptr, nullptr};
+    uint32_t *tids_buf[2] = {nullptr, nullptr};
     KvBatchDev cur;
     bool has_batch = false;
     // tuning (ltpg_kv_set_tuning): EXECUTE one lane per op instead of per
@@ -171,9 +173,11 @@ struct TpccDev {
 
 struct TpccState {
     TpccDev d;
-    uint32_t *no_buf = nullptr;   // packed NEWORDER_QUERY, capacity max_txns
-    uint32_t *pay_buf = nullptr;  // packed PAYMENT_QUERY, capacity max_txns
-    uint32_t *ro_buf = nullptr;   // read-only batches: 6 + 3 arrays of capacity max_txns
+    // host-submitted batches, two slots as for key-value batches
+    uint32_t *no_buf[2] = {nullptr, nullptr};   // packed NEWORDER_QUERY, capacity max_txns
+    uint32_t *pay_buf[2] = {nullptr, nullptr};  // packed PAYMENT_QUERY, capacity max_txns
+    uint32_t *ro_buf[2] = {nullptr, nullptr};   // read-only batches: 6 + 3 arrays of capacity max_txns
+    bool host_slot = false;                     // the current epoch's batch went through a host slot
     NeworderDev no;
     PaymentDev pay;
     ReadOnlyDev ro;
@@ -197,6 +201,12 @@ struct ltpg_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     uint32_t ev_used = 0;
+    // transport of host-resident batches (two device slots, a copy stream)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t xfer_copied[2] = {nullptr, nullptr};
+    hipEvent_t xfer_released[2] = {nullptr, nullptr};  // the epochs that used the slot are done
+    bool xfer_released_set[2] = {false, false};
+    int xfer_slot = 1;  // slot of the current host batch (the next one takes the other)
 };
 
 namespace ltpg {
