@@ -60,6 +60,11 @@ def kv_workload(name, theta):
         return dict(rows=10_000_000, fields=10, fw=25, B=1 << 20, S=16, theta=theta, read_pct=50,
                     desc=f"key-value plain form: 10M rows x 10 fields x 100 B, 2^20 txns x 16 ops, "
                          f"50/50 read/RMW, Zipf {theta}")
+    if name == "kv-y1":
+        return dict(rows=10_000_000, fields=10, fw=25, B=1 << 20, S=10, theta=0.0, read_pct=90, kinds=[1] * 9 + [2],
+                    desc="key-value plain form, the YCSB/1 op mix (9 reads + 1 RMW per txn, uniform keys) at "
+                         "10M rows x 10 fields x 100 B and 2^20 txns: low contention, so most transactions "
+                         "commit and CHECK / WRITEBACK move their full volume")
     if name == "kv-c1":
         return dict(rows=1_000_000, fields=10, fw=1, B=65536, S=10, theta=0.8, read_pct=50,
                     desc="key-value plain form: 1M rows x 10 u32 fields, 65,536 txns x 10 ops, "
@@ -211,6 +216,10 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
     scan = w.get("scan_len", 0)
     gen = [L.kv_generate(seed=0x4C545047 + i, rows=rows, fields=fields, n_txns=B, n_slots=S,
                          read_pct=w["read_pct"], theta=w["theta"], tid_base=0, scan_len=scan) for i in range(P)]
+    if "kinds" in w:
+        # a fixed op kind per slot (the YCSB/1 mix: slots 0-8 read, slot 9 writes)
+        kinds = np.repeat(np.asarray(w["kinds"], dtype=np.uint32), B)
+        gen = [(k_, (m_ & ~np.uint32(3)) | kinds, t_) for (k_, m_, t_) in gen]
     ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw,
                     max_txns=B, kv_max_slots=S, values=L.VALUES_RMW, kv_max_scan=scan,
                     kv_hot_row=L.NO_HOT_ROW if hot_row is None else hot_row)
@@ -313,6 +322,10 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
     dom = max(KERNELS, key=lambda k: avg_ms[k])
     achieved = bytes_avg[dom] / (avg_ms[dom] * 1e-3) / 1e9
     kernel_gbs = {k: bytes_avg[k] / (avg_ms[k] * 1e-3) / 1e9 for k in KERNELS if avg_ms[k] > 0}
+    # counted HBM bytes exist for the workloads profiled with --pmc
+    traffic_src = {("kv-c2", 0.6): "kv_c2_traffic.json", ("kv-y1", 0.0): "kv_y1_traffic.json"}.get(
+        (name or args.workload, w["theta"]))
+    counted = {k: measured_traffic(k, source=traffic_src) for k in KERNELS}
     # EXECUTE is bound by random register accesses, not bytes: one probe +
     # atomicMin per write op and one W load per read op, against the rates
     # this device sustains for them over an array of the registers' size
@@ -359,9 +372,12 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
         "reference_convention": ref_conv,
         "kernel_ms": avg_ms,
         "kernel_algorithmic_gbs": kernel_gbs,
+        "kernel_counted_bytes": counted,
+        "kernel_counted_gbs": {k: counted[k] / (avg_ms[k] * 1e-3) / 1e9 for k in KERNELS
+                               if counted[k] and avg_ms[k] > 0},
         "kernel_algorithmic_bytes": bytes_avg,
         "roofline": {"kernel": f"kv_{dom}", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": counted[dom],
                      "algorithmic_bytes_per_launch": bytes_avg[dom], "random_access": random_access},
     }
     for d in dev:
@@ -1051,13 +1067,16 @@ def _tpcc_select(b, idx):
     return out, perm
 
 
-def measured_traffic(phase, family="kv"):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if
-    one exists for this kernel (profiles/kv_c2_traffic.json,
-    profiles/tpcc_c3_traffic.json)."""
-    p = os.path.join(ROOT, "profiles", "kv_c2_traffic.json" if family == "kv" else "tpcc_c3_traffic.json")
+def measured_traffic(phase, family="kv", source=None):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc
+    summary of the same workload (profiles/<source>), or None when none
+    exists for it."""
+    if source is None:
+        source = "tpcc_c3_traffic.json" if family == "tpcc" else None
+    if source is None:
+        return None
     try:
-        with open(p) as f:
+        with open(os.path.join(ROOT, "profiles", source)) as f:
             return json.load(f).get(f"{family}_{phase}")
     except (OSError, ValueError):
         return None
@@ -1099,6 +1118,10 @@ def compact(v, name):
         ra = v["roofline"].get("random_access")
         if ra:
             out["roofline"]["random_access_frac"] = ra.get("frac")
+    if "kernel_algorithmic_gbs" in v:
+        out["phase_hbm_frac"] = {k: round(g / HBM_PEAK_GBS, 4) for k, g in v["kernel_algorithmic_gbs"].items()}
+    if v.get("kernel_counted_gbs"):
+        out["phase_counted_hbm_frac"] = {k: round(g / HBM_PEAK_GBS, 4) for k, g in v["kernel_counted_gbs"].items()}
     if "reference_convention" in v:
         out["reference_convention_txn_per_s"] = v["reference_convention"]["value"]
     if "cpu_baseline" in v:
@@ -1125,7 +1148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "served-c5", "kv-c2", "kv-c1", "kv-scan", "kv-small", "tpcc-c3", "tpcc-mix",
+                    choices=["all", "served-c5", "kv-c2", "kv-y1", "kv-c1", "kv-scan", "kv-small", "tpcc-c3", "tpcc-mix",
                              "batched-c4"])
     ap.add_argument("--c5-rate", type=float, default=5.0,
                     help="offered load of the headline served run, M txn/s (its window is --steps ms)")
@@ -1208,6 +1231,9 @@ def main():
                 v = run_kv(args, th, cpu_baseline=th == 0.6, name="kv-c2")
                 detail[f"kv_c2_theta{th}"] = v
                 variants.append(compact(v, f"C2 key-value plain form, Zipf {th}"))
+            v = run_kv(args, None, cpu_baseline=False, name="kv-y1")
+            detail["kv_y1_mix"] = v
+            variants.append(compact(v, "key-value plain form, YCSB/1 mix (9 reads + 1 RMW, uniform), low contention"))
             v = run_tpcc(args)
             detail["tpcc_c3"] = v
             variants.append(compact(v, "C3 order-entry plain form"))
