@@ -337,6 +337,21 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
     n_w = sum(int(((gen[j % P][1] & 3) == 2).sum()) for j in range(args.steps)) / args.steps
     n_r = sum(rows_read(gen[j % P][1]) for j in range(args.steps)) / args.steps
     bound_ms = n_w / (atomic_gops * 1e6) + n_r / (load_gops * 1e6)
+    # the read-out and write-back copy 100-byte fields between random rows
+    # and dense arrays: their ceiling is this device's random-gather rate
+    # over an array of the table's size, measured live for as many fields
+    # as CHECK reads per epoch
+    gather = None
+    if 4 * fw <= 128:
+        fsw = (fw + 15) // 16 * 16 if fw >= 16 else fw  # the table's field stride (words)
+        n_fields = max(1, int(bytes_avg["check"] // (2 * 4 * fw)))
+        g = L.random_gather_rate(ctx, rows * fields * 4 * fsw, 4 * fw, 4 * fsw, n_fields)
+        gather = {"gbs": g, "fields": n_fields,
+                  "check_frac": kernel_gbs.get("check", 0.0) / g if g else None,
+                  "writeback_frac": kernel_gbs.get("writeback", 0.0) / g if g else None,
+                  "what": "random field copies (table-sized array, field stride of the table) into a dense array, "
+                          "read + written bytes per second (ltpg_random_gather_rate); fracs are the phases' "
+                          "algorithmic GB/s over it"}
     random_access = {"kernel": "kv_execute", "write_ops": n_w, "read_ops": n_r,
                      "probe_atomic_gops": atomic_gops, "load_gops": load_gops,
                      "bound_ms": bound_ms, "measured_ms": avg_ms["execute"],
@@ -373,6 +388,7 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
         "kernel_ms": avg_ms,
         "kernel_algorithmic_gbs": kernel_gbs,
         "kernel_counted_bytes": counted,
+        "random_gather_ceiling": gather,
         "kernel_counted_gbs": {k: counted[k] / (avg_ms[k] * 1e-3) / 1e9 for k in KERNELS
                                if counted[k] and avg_ms[k] > 0},
         "kernel_algorithmic_bytes": bytes_avg,
@@ -691,6 +707,10 @@ def run_batched_c4(args):
             "random_access": {"write_ops": kv_ops["writes"], "read_ops": kv_ops["reads"],
                               "probe_atomic_gops": atomic_gops, "load_gops": load_gops, "bound_ms": bound_ms,
                               "frac": bound_ms / ex_ms if ex_ms > 0 else None}}
+    # a persistent kernel would replace each kernel boundary inside an
+    # epoch's graph by a grid-wide barrier: both measured live, at the grid
+    # sizes of the small size classes (2^10-2^14 transactions) and of the chip
+    sync = {str(b): L.sync_costs(kv, b, 1000) for b in (64, 256, 1024)}
     runner.close()
     for x in keep:
         for buf in (x if isinstance(x, list) else [x]):
@@ -698,6 +718,7 @@ def run_batched_c4(args):
     kv.close()
     tp.close()
     return {
+        "sync_costs_us": sync,
         "value": commits / graph_s, "unit": "txn/s", "ms_per_step": graph_s * 1e3,
         "plain_value": commits / plain_s, "plain_ms": plain_s * 1e3, "speedup_vs_plain": plain_s / graph_s,
         "plain_tuned_value": commits / tuned_s, "plain_tuned_ms": tuned_s * 1e3,
@@ -1120,8 +1141,12 @@ def compact(v, name):
             out["roofline"]["random_access_frac"] = ra.get("frac")
     if "kernel_algorithmic_gbs" in v:
         out["phase_hbm_frac"] = {k: round(g / HBM_PEAK_GBS, 4) for k, g in v["kernel_algorithmic_gbs"].items()}
+    if v.get("random_gather_ceiling"):
+        out["random_gather_ceiling_gbs"] = round(v["random_gather_ceiling"]["gbs"], 1)
     if v.get("kernel_counted_gbs"):
         out["phase_counted_hbm_frac"] = {k: round(g / HBM_PEAK_GBS, 4) for k, g in v["kernel_counted_gbs"].items()}
+    if "sync_costs_us" in v:
+        out["sync_costs_us"] = {b: {k: round(x, 2) for k, x in c.items()} for b, c in v["sync_costs_us"].items()}
     if "reference_convention" in v:
         out["reference_convention_txn_per_s"] = v["reference_convention"]["value"]
     if "cpu_baseline" in v:

@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define LTPG_ABI_VERSION 1
+#define LTPG_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define LTPG_OK 0
@@ -106,6 +106,7 @@ extern "C" {
 #define LTPG_TYPE_PAYMENT 2u
 #define LTPG_TYPE_ORDERSTATUS 3u /* read-only order-entry types, see       */
 #define LTPG_TYPE_STOCKLEVEL 4u  /* ltpg_tpcc_submit_readonly               */
+#define LTPG_TYPE_DELIVERY 5u    /* ltpg_tpcc_submit_delivery               */
 
 /* ---- phases ------------------------------------------------------------ */
 #define LTPG_PHASE_CID 1u
@@ -224,6 +225,25 @@ int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int ltpg_random_access_rates(ltpg_ctx *ctx, uint64_t bytes, uint64_t n_ops, double *load_gops,
                              double *atomic_gops);
 
+/* Random-gather ceiling (no counterpart in the reference: the roofline of
+ * the read-out and write-back copies).  Over a scratch array of `bytes`
+ * split into slots of stride_bytes it times n_ops copies of field_bytes
+ * from random slots into a dense destination, 64 fields per wave step with
+ * up to 32 in flight, and returns the rate in GB/s of field bytes read plus
+ * written.  Synchronous; allocates and frees both arrays. */
+int ltpg_random_gather_rate(ltpg_ctx *ctx, uint64_t bytes, uint32_t field_bytes, uint32_t stride_bytes,
+                            uint64_t n_ops, double *gbs);
+
+/* Costs of the ways a multi-phase epoch can be sequenced on this device (no
+ * counterpart in the reference; the evidence behind the batched form's use of
+ * graphs rather than one persistent kernel): out[0] = microseconds per
+ * grid-wide barrier of `blocks` co-resident 256-thread workgroups (a
+ * cooperative launch passing `reps` barriers), out[1] = microseconds per
+ * kernel boundary inside a HIP graph of `reps` back-to-back `blocks`-wide
+ * kernels, out[2] = the same launched one by one on the stream.  Synchronous;
+ * LTPG_ERR_DEVICE if the workgroups cannot all be resident. */
+int ltpg_sync_costs(ltpg_ctx *ctx, uint32_t blocks, uint32_t reps, double out[3]);
+
 /* ---- key-value tables (YCSB/0/Database.h:14-131) ------------------------ */
 /* Load rows x kv_fields x kv_field_bytes.  COLUMN_MAJOR requires
  * kv_field_bytes == 4 (the reference's u32 snapshot). */
@@ -336,6 +356,43 @@ typedef struct ltpg_stocklevel_batch {
 int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *orderstatus,
                               const ltpg_stocklevel_batch *stocklevel, uint32_t mem);
 
+/* Delivery transactions (DeliveryQuery {O_ID, NO_W_ID, NO_O_ID} plus TID,
+ * TPCC-alltxn/Datastructure.h:201-206,278-288; the slots of TPCC-alltxn/
+ * Execute.h:1772-2153, whose query set the reference declares but never
+ * fills).  Rows as the reference computes them, without a warehouse offset
+ * except for NEWORDER:
+ *   op_0: NEWORDER row NO_W_ID * 30000 + NO_O_ID, writes (the delete);
+ *   op_1 / op_2: ORDER row O_ID, reads column 3, writes;
+ *   op_3 / op_4: ORDERLINE row O_ID, writes, reads column 3;
+ *   op_5: CUSTOMER row O_ID, writes.
+ * Checks as the reference dispatches them (its check_4_0 and check_5_0 sit
+ * behind repeated conditions and never run): raw from the NEWORDER row's R
+ * and W and from the ORDER and ORDERLINE rows' R, war from the ORDER row's
+ * W, waw from the ORDER, ORDERLINE and CUSTOMER rows' W.  War implies waw
+ * (the same register), so a Delivery commits iff no higher-priority
+ * transaction writes its ORDER, ORDERLINE or CUSTOMER row.  Its NEWORDER
+ * write conflicts only with NewOrders under LTPG_RULES_INTENDED (their
+ * check_3_1); under the reference rules a NewOrder inserting the row and a
+ * Delivery deleting it both commit, and the row ends as the last of them in
+ * priority order leaves it.  Committed reads: ORDER column 3, ORDERLINE
+ * column 3 (value-major, 2 per Delivery).  LTPG_VALUES_RMW (no write-back
+ * in the reference): NEWORDER row <- (0, 0, 0); ORDER O_CARRIER_ID (column
+ * 5) <- TID % 10 + 1; ORDERLINE OL_DELIVERY_D (column 6) <- TID; CUSTOMER
+ * C_BALANCE (column 5) += OL_AMOUNT (ORDERLINE column 8) and
+ * C_DELIVERY_CNT (column 8) += 1.  Batch positions follow every other type:
+ * Delivery d at n_neworder + n_payment + n_orderstatus + n_stocklevel + d.
+ * Not available with LTPG_FLAG_CONFLICT_COUNTS. */
+typedef struct ltpg_delivery_batch {
+    uint32_t n;
+    uint32_t reserved;
+    const uint32_t *O_ID, *NO_W_ID, *NO_O_ID, *TID;          /* all [n]      */
+} ltpg_delivery_batch;
+
+/* Add Deliveries to the batch of the last ltpg_tpcc_submit (which starts
+ * every epoch with none), before or after its read-only transactions; all
+ * five types together must fit max_txns. */
+int ltpg_tpcc_submit_delivery(ltpg_ctx *ctx, const ltpg_delivery_batch *delivery, uint32_t mem);
+
 /* C_ID used by each Payment of the current batch: the query's C_ID, or,
  * once the CID phase has run, the by-name resolution for isName Payments
  * (KERNEL_CID writes it back into PAYMENT_QUERY.C_ID, TPCC-allopt-final/
@@ -353,6 +410,7 @@ int ltpg_tpcc_payment_cid(ltpg_ctx *ctx, uint32_t *out, uint32_t n);
  *     3, 4, 11..20.
  *   ORDERSTATUS (3): CUSTOMER col 2, ORDER col 0, ORDERLINE col 0.
  *   STOCKLEVEL (1): STOCK col 2.
+ *   DELIVERY (2): ORDER col 3, ORDERLINE col 3.
  * A slot whose row is outside its table reads 0; aborted transactions'
  * entries are not written. */
 int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t n_words);
@@ -481,6 +539,8 @@ typedef struct ltpg_batch_desc {
                                             TPCC: read-only transactions added to
                                             the epoch, ltpg_tpcc_submit_readonly) */
     ltpg_stocklevel_batch stocklevel;
+    ltpg_delivery_batch delivery;    /* device pointers, n may be 0 (workload TPCC,
+                                        ltpg_tpcc_submit_delivery)                */
 } ltpg_batch_desc;
 
 /* PLAIN: one epoch per batch, in list order, as the reference's driver loop
@@ -578,13 +638,14 @@ typedef struct ltpg_serve_config {
 } ltpg_serve_config;
 
 /* Priorities of the served form.  Each workload numbers its requests in
- * arrival order with a 64-bit sequence number that never wraps; an epoch's
- * priorities are its members' ranks in that order, so a re-queued request
- * always precedes every later arrival.  A key-value write adds
- * (low 32 bits of the sequence number) | 1 to each word of its field. */
+ * arrival order with a 64-bit sequence number that never wraps; a request's
+ * batch TID is the low 32 bits of it, and no epoch spans a 2^32 boundary of
+ * those bits, so within every epoch TID order is arrival order and a
+ * re-queued request always precedes every later arrival.  A key-value write
+ * adds TID | 1 to each word of its field; a Delivery's values use its TID. */
 typedef struct ltpg_serve_done {
     uint64_t id;              /* request id (submission order, both workloads)   */
-    uint32_t type;            /* LTPG_TYPE_KV / NEWORDER / PAYMENT               */
+    uint32_t type;            /* LTPG_TYPE_*                                     */
     uint32_t tid;             /* low 32 bits of the arrival sequence number      */
     uint32_t attempts;        /* epochs the request took part in                */
     uint32_t tid_hi;          /* high 32 bits of the arrival sequence number     */
@@ -594,8 +655,8 @@ typedef struct ltpg_serve_done {
 } ltpg_serve_done;
 
 typedef struct ltpg_serve_stats {
-    uint64_t committed[5];    /* by type (LTPG_TYPE_KV .. LTPG_TYPE_STOCKLEVEL)  */
-    uint64_t attempts[5];     /* transaction-epochs by type                     */
+    uint64_t committed[6];    /* by type (LTPG_TYPE_KV .. LTPG_TYPE_DELIVERY)    */
+    uint64_t attempts[6];     /* transaction-epochs by type                     */
     uint64_t epochs[2];       /* key-value, order-entry                         */
     uint64_t max_epoch_txns[2];
 } ltpg_serve_stats;
@@ -643,12 +704,19 @@ int ltpg_serve_submit_tpcc(ltpg_server *sv, const ltpg_neworder_batch *neworder,
  * stubs (TPCC-alltxn/Execute.h:1592-1770,2155-2231). */
 int ltpg_serve_submit_readonly(ltpg_server *sv, const ltpg_orderstatus_batch *orderstatus,
                                const ltpg_stocklevel_batch *stocklevel, uint64_t *first_id);
+/* Delivery requests (host arrays, TID field ignored), which join the
+ * order-entry epochs after every other type as ltpg_tpcc_submit_delivery
+ * adds them; ids first_id ..  LTPG_ERR_UNSUPPORTED with
+ * LTPG_FLAG_CONFLICT_COUNTS.  No counterpart in the reference (an unfed
+ * stub there, TPCC-alltxn/Execute.h:1772-2153). */
+int ltpg_serve_submit_delivery(ltpg_server *sv, const ltpg_delivery_batch *delivery, uint64_t *first_id);
 /* Up to max completed (committed) requests, oldest first. */
 int ltpg_serve_poll(ltpg_server *sv, ltpg_serve_done *out, uint32_t max, uint32_t *n_out);
 /* Read values of a committed request (keep_reads): key-value: the field
  * words of its read ops, and of each scan's rows inside the table, in op
- * order; NewOrder 51 / Payment 24 / Order-Status 3 / Stock-Level 1 values
- * as ltpg_tpcc_read_values lays them out for one transaction. */
+ * order; NewOrder 51 / Payment 24 / Order-Status 3 / Stock-Level 1 /
+ * Delivery 2 values as ltpg_tpcc_read_values lays them out for one
+ * transaction. */
 int ltpg_serve_read_values(ltpg_server *sv, uint64_t id, uint32_t *out, uint32_t max_words, uint32_t *n_words);
 /* Request ids of serving epoch e of a workload, in batch order (record_epochs). */
 int ltpg_serve_epoch(ltpg_server *sv, uint32_t workload, uint64_t e, uint64_t *ids, uint32_t max, uint32_t *n);

@@ -39,7 +39,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 BATCH_REQUEST_MAJOR = 0x10
 LAYOUT_ROW_MAJOR, LAYOUT_COLUMN_MAJOR = 0, 1
 WORKLOAD_KV, WORKLOAD_TPCC = 1, 2
-TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT, TYPE_ORDERSTATUS, TYPE_STOCKLEVEL = 0, 1, 2, 3, 4
+TYPE_KV, TYPE_NEWORDER, TYPE_PAYMENT, TYPE_ORDERSTATUS, TYPE_STOCKLEVEL, TYPE_DELIVERY = 0, 1, 2, 3, 4, 5
 PHASE_CID, PHASE_EXECUTE, PHASE_CHECK, PHASE_WRITEBACK, PHASE_ALL = 1, 2, 4, 8, 15
 KV_NONE, KV_READ, KV_WRITE, KV_SCAN = 0, 1, 2, 3
 NO_HOT_ROW = (1 << 64) - 1
@@ -50,9 +50,10 @@ TPCC_WAREHOUSE, TPCC_DISTRICT, TPCC_CUSTOMER, TPCC_HISTORY, TPCC_NEWORDER, TPCC_
     TPCC_STOCK, TPCC_ITEM = range(9)
 TPCC_TABLES = ["warehouse", "district", "customer", "history", "neworder", "order", "orderline", "stock", "item"]
 TPCC_MAX_LINES = 15
-NEWORDER_READS, PAYMENT_READS, ORDERSTATUS_READS, STOCKLEVEL_READS = 51, 24, 3, 1
+NEWORDER_READS, PAYMENT_READS, ORDERSTATUS_READS, STOCKLEVEL_READS, DELIVERY_READS = 51, 24, 3, 1, 2
 ORDERSTATUS_FIELDS = ("W_ID", "D_ID", "C_ID", "O_ID", "OL_ID", "TID")
 STOCKLEVEL_FIELDS = ("W_ID", "S_ID", "TID")
+DELIVERY_FIELDS = ("O_ID", "NO_W_ID", "NO_O_ID", "TID")
 # u32 words per transaction of the packed NEWORDER_QUERY / PAYMENT_QUERY blocks
 NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
@@ -108,6 +109,23 @@ def random_access_rates(ctx, nbytes, n_ops=1 << 24):
     _check(lib.ltpg_random_access_rates(ctx.handle, int(nbytes), int(n_ops), ctypes.byref(lo), ctypes.byref(at)),
            "ltpg_random_access_rates")
     return lo.value, at.value
+
+
+def random_gather_rate(ctx, nbytes, field_bytes, stride_bytes, n_ops):
+    """GB/s (read + written) of n_ops field copies from random slots of an
+    nbytes array into a dense one (ltpg_random_gather_rate)."""
+    g = ctypes.c_double()
+    _check(lib.ltpg_random_gather_rate(ctx.handle, int(nbytes), int(field_bytes), int(stride_bytes), int(n_ops),
+                                       ctypes.byref(g)), "ltpg_random_gather_rate")
+    return g.value
+
+
+def sync_costs(ctx, blocks, reps=1000):
+    """{grid_barrier_us, graph_boundary_us, stream_boundary_us} for
+    `blocks` 256-thread workgroups (ltpg_sync_costs)."""
+    out = (ctypes.c_double * 3)()
+    _check(lib.ltpg_sync_costs(ctx.handle, int(blocks), int(reps), out), "ltpg_sync_costs")
+    return {"grid_barrier_us": out[0], "graph_boundary_us": out[1], "stream_boundary_us": out[2]}
 
 
 class DeviceBuffer:
@@ -185,6 +203,7 @@ class Context:
         self._n_pay = 0
         self._n_os = 0
         self._n_sl = 0
+        self._n_dl = 0
 
     def close(self):
         if self.handle:
@@ -344,7 +363,7 @@ class Context:
             n_no, n_pay = nb.n, pb.n
         _check(lib.ltpg_tpcc_submit(self.handle, ctypes.byref(nb), ctypes.byref(pb), mem), "ltpg_tpcc_submit")
         self._n_no, self._n_pay = n_no, n_pay
-        self._n_os = self._n_sl = 0
+        self._n_os = self._n_sl = self._n_dl = 0
 
     def tpcc_submit_readonly(self, ro, mem=MEM_HOST):
         """Add a ReadOnlyBatch (host arrays), or with mem=MEM_DEVICE a pair
@@ -358,9 +377,17 @@ class Context:
                "ltpg_tpcc_submit_readonly")
         self._n_os, self._n_sl = ob.n, sb.n
 
+    def tpcc_submit_delivery(self, dl, mem=MEM_HOST):
+        """Add a DeliveryBatch (host arrays), or with mem=MEM_DEVICE a
+        DeliveryBatch struct of device pointers (upload_delivery_batch), to
+        the batch of the last tpcc_submit."""
+        st = dl.struct() if mem == MEM_HOST else dl
+        _check(lib.ltpg_tpcc_submit_delivery(self.handle, ctypes.byref(st), mem), "ltpg_tpcc_submit_delivery")
+        self._n_dl = st.n
+
     def _tpcc_n(self, type_):
         return {TYPE_NEWORDER: self._n_no, TYPE_PAYMENT: self._n_pay, TYPE_ORDERSTATUS: self._n_os,
-                TYPE_STOCKLEVEL: self._n_sl}[type_]
+                TYPE_STOCKLEVEL: self._n_sl, TYPE_DELIVERY: self._n_dl}[type_]
 
     def tpcc_payment_cid(self):
         out = np.zeros(max(self._n_pay, 1), dtype=np.uint32)
@@ -370,7 +397,7 @@ class Context:
 
     def tpcc_read_values(self, type_):
         per = {TYPE_NEWORDER: NEWORDER_READS, TYPE_PAYMENT: PAYMENT_READS, TYPE_ORDERSTATUS: ORDERSTATUS_READS,
-               TYPE_STOCKLEVEL: STOCKLEVEL_READS}[type_]
+               TYPE_STOCKLEVEL: STOCKLEVEL_READS, TYPE_DELIVERY: DELIVERY_READS}[type_]
         n = self._tpcc_n(type_) * per
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib.ltpg_tpcc_read_values(self.handle, type_, out.ctypes.data_as(_lib.c_u32p), n),
@@ -412,6 +439,23 @@ class ReadOnlyBatch:
         for f in STOCKLEVEL_FIELDS:
             setattr(sb, f, self.sl[f].ctypes.data)
         return ob, sb
+
+
+class DeliveryBatch:
+    """Host arrays of Deliveries: DeliveryQuery (O_ID, NO_W_ID, NO_O_ID)
+    and TID, one u32 array each (attributes of those names)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        for f in DELIVERY_FIELDS:
+            setattr(self, f, np.zeros(max(self.n, 1), dtype=np.uint32))
+
+    def struct(self):
+        st = _lib.DeliveryBatch()
+        st.n = self.n
+        for f in DELIVERY_FIELDS:
+            setattr(st, f, getattr(self, f).ctypes.data)
+        return st
 
 
 def neworder_struct(block, cap, n):
@@ -556,6 +600,58 @@ def upload_readonly_batch(ctx, ro):
     return (ob, sb), bufs
 
 
+def delivery_batch(b, warehouses, n, seed, tid_lo=0, tid_hi=None, hit_pct=50):
+    """Seeded Deliveries beside a TpccBatch b.  hit_pct % of them take an
+    O_ID equal to a row b's NewOrders write (an ORDER row, or an ORDERLINE
+    row of one of its lines) or b's Payments write (a CUSTOMER row below
+    the ORDER table's size), and half of those also delete the NEWORDER row
+    a NewOrder inserts; the rest draw O_ID uniformly below W * 30000,
+    NO_W_ID below W and NO_O_ID below 30000.  TIDs are drawn in
+    [tid_lo, tid_hi), so priorities interleave with the batch's."""
+    rng = np.random.default_rng(seed)
+    W = warehouses
+    tid_hi = tid_lo + b.n_no + b.n_pay + n if tid_hi is None else tid_hi
+    dl = DeliveryBatch(n)
+    if not n:
+        return dl
+    rows = W * 30000
+    cand = []
+    if b.n_no:
+        w = b.nf("W_ID").astype(np.int64)
+        cand.append(w * 30000 + b.nf("O_ID"))
+        nl = np.maximum(np.minimum(b.nf("O_OL_CNT").astype(np.int64), 15), 1)
+        cand.append(w * 450000 + b.nf("O_OL_ID") + (rng.random(b.n_no) * nl).astype(np.int64))
+    if b.n_pay:
+        cand.append(b.pf("C_W_ID").astype(np.int64) * 30000 + b.pf("C_D_ID").astype(np.int64) * 3000 +
+                    b.pf("C_ID"))
+    pool = np.concatenate(cand) if cand else np.zeros(0, np.int64)
+    pool = pool[pool < rows]
+    hit = (rng.random(n) < hit_pct / 100.0) & (pool.size > 0)
+    pick = pool[rng.integers(0, max(pool.size, 1), n)] if pool.size else np.zeros(n, np.int64)
+    dl.O_ID[:n] = np.where(hit, pick, rng.integers(0, rows, n)).astype(np.uint32)
+    del_hit = hit & (rng.random(n) < 0.5) & (b.n_no > 0)
+    q = rng.integers(0, max(b.n_no, 1), n)
+    dl.NO_W_ID[:n] = np.where(del_hit, b.nf("W_ID")[q] if b.n_no else 0, rng.integers(0, W, n)).astype(np.uint32)
+    dl.NO_O_ID[:n] = np.where(del_hit, b.nf("N_O_ID")[q] if b.n_no else 0,
+                              rng.integers(0, 30000, n)).astype(np.uint32)
+    dl.TID[:n] = rng.integers(tid_lo, tid_hi, n).astype(np.uint32)
+    return dl
+
+
+def upload_delivery_batch(ctx, dl):
+    """Copy a DeliveryBatch into device buffers of ctx; returns
+    (DeliveryBatch struct of device pointers, [buffers])."""
+    st, bufs = _lib.DeliveryBatch(), []
+    st.n = dl.n
+    for f in DELIVERY_FIELDS:
+        a = getattr(dl, f)
+        buf = DeviceBuffer(ctx, max(a.nbytes, 16))
+        buf.upload(a)
+        bufs.append(buf)
+        setattr(st, f, buf.ptr)
+    return st, bufs
+
+
 def upload_tpcc_batch(ctx, b):
     """Copy a TpccBatch into device buffers of ctx; returns
     ((NeworderBatch, PaymentBatch) of device pointers, [buffers])."""
@@ -605,7 +701,7 @@ class Batched:
     def bitmap(self, i, type_):
         d = self.descs[i]
         n = {TYPE_KV: d.kv.n_txns, TYPE_NEWORDER: d.neworder.n, TYPE_PAYMENT: d.payment.n,
-             TYPE_ORDERSTATUS: d.orderstatus.n, TYPE_STOCKLEVEL: d.stocklevel.n}[type_]
+             TYPE_ORDERSTATUS: d.orderstatus.n, TYPE_STOCKLEVEL: d.stocklevel.n, TYPE_DELIVERY: d.delivery.n}[type_]
         out = np.zeros(max(1, (n + 63) // 64), dtype=np.uint64)
         _check(lib.ltpg_batched_bitmap(self.handle, i, type_, out.ctypes.data_as(_lib.c_u64p), out.size),
                "ltpg_batched_bitmap")
@@ -631,15 +727,18 @@ def kv_desc(batch):
     return d
 
 
-def tpcc_desc(dev, ro_dev=None):
+def tpcc_desc(dev, ro_dev=None, dl_dev=None):
     """BatchDesc of an uploaded order-entry batch ((nb, pb) of
     upload_tpcc_batch), with read-only transactions ((ob, sb) of
-    upload_readonly_batch) if given."""
+    upload_readonly_batch) and Deliveries (a struct of
+    upload_delivery_batch) if given."""
     d = BatchDesc()
     d.workload = WORKLOAD_TPCC
     d.neworder, d.payment = dev
     if ro_dev is not None:
         d.orderstatus, d.stocklevel = ro_dev
+    if dl_dev is not None:
+        d.delivery = dl_dev
     return d
 
 
@@ -685,6 +784,14 @@ class Server:
         first = ctypes.c_uint64()
         _check(lib.ltpg_serve_submit_readonly(self.handle, ctypes.byref(ob), ctypes.byref(sb), ctypes.byref(first)),
                "ltpg_serve_submit_readonly")
+        return first.value
+
+    def submit_delivery(self, dl):
+        """Requests from a DeliveryBatch (their TIDs are not taken)."""
+        st = dl.struct()
+        first = ctypes.c_uint64()
+        _check(lib.ltpg_serve_submit_delivery(self.handle, ctypes.byref(st), ctypes.byref(first)),
+               "ltpg_serve_submit_delivery")
         return first.value
 
     def poll(self, max_n=1 << 16):

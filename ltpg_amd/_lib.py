@@ -100,6 +100,13 @@ class StocklevelBatch(ctypes.Structure):
         (name, ctypes.c_void_p) for name in ("W_ID", "S_ID", "TID")]
 
 
+class DeliveryBatch(ctypes.Structure):
+    """struct ltpg_delivery_batch."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)] + [
+        (name, ctypes.c_void_p) for name in ("O_ID", "NO_W_ID", "NO_O_ID", "TID")]
+
+
 class TpccGenConfig(ctypes.Structure):
     """struct ltpg_tpcc_gen_config."""
 
@@ -114,7 +121,7 @@ class BatchDesc(ctypes.Structure):
 
     _fields_ = [("workload", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("kv", KvBatch),
                 ("neworder", NeworderBatch), ("payment", PaymentBatch), ("orderstatus", OrderstatusBatch),
-                ("stocklevel", StocklevelBatch)]
+                ("stocklevel", StocklevelBatch), ("delivery", DeliveryBatch)]
 
 
 class TuningEntry(ctypes.Structure):
@@ -145,7 +152,7 @@ class ServeDone(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     """struct ltpg_serve_stats."""
 
-    _fields_ = [("committed", ctypes.c_uint64 * 5), ("attempts", ctypes.c_uint64 * 5), ("epochs", ctypes.c_uint64 * 2),
+    _fields_ = [("committed", ctypes.c_uint64 * 6), ("attempts", ctypes.c_uint64 * 6), ("epochs", ctypes.c_uint64 * 2),
                 ("max_epoch_txns", ctypes.c_uint64 * 2)]
 
 
@@ -206,6 +213,12 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), ctypes.c_uint32]),
     ("ltpg_tpcc_submit_readonly", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(OrderstatusBatch), ctypes.POINTER(StocklevelBatch), ctypes.c_uint32]),
+    ("ltpg_sync_costs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    ("ltpg_random_gather_rate", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+      ctypes.POINTER(ctypes.c_double)]),
+    ("ltpg_tpcc_submit_delivery", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DeliveryBatch), ctypes.c_uint32]),
     ("ltpg_tpcc_payment_cid", ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_uint32]),
     ("ltpg_tpcc_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, ctypes.c_uint64]),
     ("ltpg_tpcc_gen_config_init", None, [ctypes.POINTER(TpccGenConfig)]),
@@ -230,6 +243,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), c_u64p]),
     ("ltpg_serve_submit_readonly", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(OrderstatusBatch), ctypes.POINTER(StocklevelBatch), c_u64p]),
+    ("ltpg_serve_submit_delivery", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DeliveryBatch), c_u64p]),
     ("ltpg_serve_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeDone), ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_read_values", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_u32p, ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_epoch", ctypes.c_int,

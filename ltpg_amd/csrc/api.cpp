@@ -122,12 +122,15 @@ void free_tpcc(TpccState &tp)
         dev_free(tp.no_buf[k]);
         dev_free(tp.pay_buf[k]);
         dev_free(tp.ro_buf[k]);
+        dev_free(tp.dl_buf[k]);
     }
     dev_free(tp.d.war);
     dev_free(tp.d.bitmap_os);
     dev_free(tp.d.bitmap_sl);
     dev_free(tp.d.reads_os);
     dev_free(tp.d.reads_sl);
+    dev_free(tp.d.bitmap_dl);
+    dev_free(tp.d.reads_dl);
 }
 
 int init_tpcc(ltpg_ctx *ctx)
@@ -175,6 +178,8 @@ int init_tpcc(ltpg_ctx *ctx)
     if ((rc = dev_alloc(&d.bitmap_sl, (B + 63) / 64, "Stock-Level bitmap"))) return rc;
     if ((rc = dev_alloc(&d.reads_os, (uint64_t)kOsReads * B, "Order-Status read values"))) return rc;
     if ((rc = dev_alloc(&d.reads_sl, (uint64_t)kSlReads * B, "Stock-Level read values"))) return rc;
+    if ((rc = dev_alloc(&d.bitmap_dl, (B + 63) / 64, "Delivery bitmap"))) return rc;
+    if ((rc = dev_alloc(&d.reads_dl, (uint64_t)kDlReads * B, "Delivery read values"))) return rc;
     LTPG_HIP_TRY(launch_fill_u8(d.war, B, 0, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u64(d.regs, d.n_regs, ~0ull, ctx->stream));
     LTPG_HIP_TRY(launch_fill_u8(d.doomed, B, 0, ctx->stream));
@@ -455,7 +460,7 @@ int ltpg_commit_totals(ltpg_ctx *ctx, uint32_t type, uint64_t *commits, uint64_t
 {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if (type > LTPG_TYPE_STOCKLEVEL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    if (type > LTPG_TYPE_DELIVERY) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     static_assert(sizeof(DeviceCounters::totals) == kCounterStripes * 32 * 8, "stripe layout");
     unsigned long long h[kCounterStripes][32];
@@ -562,6 +567,101 @@ int ltpg_random_access_rates(ltpg_ctx *ctx, uint64_t bytes, uint64_t n_ops, doub
     if (h != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("calibration: ") + hipGetErrorString(h));
     if (load_gops) *load_gops = ms[0] > 0 ? n_ops / (ms[0] * 1e6) : 0.0;
     if (atomic_gops) *atomic_gops = ms[1] > 0 ? n_ops / (ms[1] * 1e6) : 0.0;
+    return LTPG_OK;
+}
+
+int ltpg_random_gather_rate(ltpg_ctx *ctx, uint64_t bytes, uint32_t field_bytes, uint32_t stride_bytes,
+                            uint64_t n_ops, double *gbs)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!gbs) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    if (field_bytes == 0 || field_bytes % 4 || field_bytes > 128 || stride_bytes < field_bytes || stride_bytes % 4)
+        LTPG_FAIL(LTPG_ERR_INVALID, "field_bytes a multiple of 4 in [4, 128], stride_bytes >= field_bytes, a multiple of 4");
+    const uint64_t slots = bytes / stride_bytes;
+    if (slots == 0 || n_ops == 0 || n_ops > (1ull << 32)) LTPG_FAIL(LTPG_ERR_INVALID, "bad bytes or n_ops");
+    uint32_t *src = nullptr, *dst = nullptr;
+    if ((rc = dev_alloc(&src, slots * (stride_bytes / 4), "gather source"))) return rc;
+    if ((rc = dev_alloc(&dst, n_ops * (field_bytes / 4), "gather destination"))) {
+        dev_free(src);
+        return rc;
+    }
+    hipEvent_t e[2] = {nullptr, nullptr};
+    float ms = 0.f;
+    hipError_t h = hipMemsetAsync(src, 0, slots * stride_bytes, ctx->stream);
+    for (int k = 0; k < 2 && h == hipSuccess; k++) h = hipEventCreate(&e[k]);
+    if (h == hipSuccess)
+        h = launch_gather_calibration(src, slots, stride_bytes / 4, field_bytes / 4, dst, n_ops, ctx->stream);
+    if (h == hipSuccess) h = hipEventRecord(e[0], ctx->stream);
+    if (h == hipSuccess)
+        h = launch_gather_calibration(src, slots, stride_bytes / 4, field_bytes / 4, dst, n_ops, ctx->stream);
+    if (h == hipSuccess) h = hipEventRecord(e[1], ctx->stream);
+    if (h == hipSuccess) h = hipEventSynchronize(e[1]);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms, e[0], e[1]);
+    for (int k = 0; k < 2; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    dev_free(src);
+    dev_free(dst);
+    if (h != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("gather calibration: ") + hipGetErrorString(h));
+    *gbs = ms > 0 ? 2.0 * (double)n_ops * field_bytes / (ms * 1e6) : 0.0;
+    return LTPG_OK;
+}
+
+int ltpg_sync_costs(ltpg_ctx *ctx, uint32_t blocks, uint32_t reps, double out[3])
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out || blocks == 0 || reps == 0 || reps > 100000) LTPG_FAIL(LTPG_ERR_INVALID, "bad blocks, reps or output");
+    unsigned int *w = nullptr;  // [0] count, [1] generation, [2] error, then one word per block
+    if ((rc = dev_alloc(&w, 3 + (uint64_t)blocks, "sync cost words"))) return rc;
+    hipStream_t s = ctx->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e[6] = {};
+    float ms[3] = {0.f, 0.f, 0.f};
+    unsigned int err = 0;
+    hipError_t h = hipMemsetAsync(w, 0, 12, s);
+    for (int k = 0; k < 6 && h == hipSuccess; k++) h = hipEventCreate(&e[k]);
+    // grid barriers (a warm-up pass first)
+    constexpr uint32_t kSpinLimit = 1u << 20;
+    if (h == hipSuccess) h = launch_grid_barrier(blocks, w, w + 1, 8, kSpinLimit, w + 2, s);
+    if (h == hipSuccess) h = hipEventRecord(e[0], s);
+    if (h == hipSuccess) h = launch_grid_barrier(blocks, w, w + 1, reps, kSpinLimit, w + 2, s);
+    if (h == hipSuccess) h = hipEventRecord(e[1], s);
+    // kernel boundaries on the stream
+    if (h == hipSuccess) h = launch_boundary(blocks, w + 3, s);
+    if (h == hipSuccess) h = hipEventRecord(e[2], s);
+    for (uint32_t r = 0; r < reps && h == hipSuccess; r++) h = launch_boundary(blocks, w + 3, s);
+    if (h == hipSuccess) h = hipEventRecord(e[3], s);
+    // kernel boundaries inside a graph
+    if (h == hipSuccess) h = hipStreamSynchronize(s);
+    if (h == hipSuccess) h = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (h == hipSuccess) {
+        hipError_t hc = hipSuccess;
+        for (uint32_t r = 0; r < reps && hc == hipSuccess; r++) hc = launch_boundary(blocks, w + 3, s);
+        h = hipStreamEndCapture(s, &graph);
+        if (h == hipSuccess) h = hc;
+    }
+    if (h == hipSuccess) h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (h == hipSuccess) h = hipGraphLaunch(exec, s);
+    if (h == hipSuccess) h = hipEventRecord(e[4], s);
+    if (h == hipSuccess) h = hipGraphLaunch(exec, s);
+    if (h == hipSuccess) h = hipEventRecord(e[5], s);
+    if (h == hipSuccess) h = hipEventSynchronize(e[5]);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms[0], e[0], e[1]);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms[1], e[2], e[3]);
+    if (h == hipSuccess) h = hipEventElapsedTime(&ms[2], e[4], e[5]);
+    if (h == hipSuccess) h = hipMemcpy(&err, w + 2, 4, hipMemcpyDeviceToHost);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int k = 0; k < 6; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    dev_free(w);
+    if (h != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("sync costs: ") + hipGetErrorString(h));
+    if (err) LTPG_FAIL(LTPG_ERR_DEVICE, "grid barrier did not complete (poll limit reached)");
+    out[0] = ms[0] * 1e3 / reps;
+    out[1] = ms[2] * 1e3 / reps;
+    out[2] = ms[1] * 1e3 / reps;
     return LTPG_OK;
 }
 
@@ -816,23 +916,29 @@ static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     }
     if (phase_mask & LTPG_PHASE_CID) LTPG_HIP_TRY(launch_tpcc_cid(tp.d, tp.pay, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
-    const bool ro = tp.ro.n_os + tp.ro.n_sl > 0;
+    const bool ro = tp.ro.n_os + tp.ro.n_sl > 0, dl = tp.dl.n > 0;
+    const uint32_t dl_base = tp.no.n + tp.pay.n + tp.ro.n_os + tp.ro.n_sl;
     if (phase_mask & LTPG_PHASE_EXECUTE) {
         LTPG_HIP_TRY(launch_tpcc_execute(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
-        // read-only transactions mark war on the registered writers
+        // Deliveries register after NewOrders and Payments, then read-only
+        // transactions mark war on the registered writers
+        if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_execute(tp.d, tp.dl, dl_base, e, ctx->counters, s));
         if (ro) LTPG_HIP_TRY(launch_tpcc_readonly(tp.d, tp.ro, tp.no.n + tp.pay.n, e, ctx->counters, s));
     }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
     if (phase_mask & LTPG_PHASE_CHECK) {
         if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
-        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ro, ctx->counters, s));
+        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ro, dl, ctx->counters, s));
+        if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_check(tp.d, tp.dl, dl_base, e, ctx->counters, s));
     }
     if (ev) {
         LTPG_HIP_TRY(hipEventRecord(ev[3], s));
         LTPG_HIP_TRY(hipEventRecord(ev[4], s));
     }
-    if (phase_mask & LTPG_PHASE_WRITEBACK)
+    if (phase_mask & LTPG_PHASE_WRITEBACK) {
         LTPG_HIP_TRY(launch_tpcc_writeback(tp.d, tp.no, tp.pay, e, ctx->cfg.values, s));
+        if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_writeback(tp.d, tp.dl, dl_base, e, ctx->cfg.values, s));
+    }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[5], s));
     return LTPG_OK;
 }
@@ -937,6 +1043,9 @@ int ltpg_commit_bitmap(ltpg_ctx *ctx, uint32_t type, uint64_t *out, uint64_t n_w
     } else if (type == LTPG_TYPE_STOCKLEVEL && ctx->tpcc.d.regs) {
         src = ctx->tpcc.d.bitmap_sl;
         n = ctx->tpcc.ro.n_sl;
+    } else if (type == LTPG_TYPE_DELIVERY && ctx->tpcc.d.regs) {
+        src = ctx->tpcc.d.bitmap_dl;
+        n = ctx->tpcc.dl.n;
     } else {
         LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "transaction type not in this context's workload");
     }
@@ -952,7 +1061,7 @@ int ltpg_commit_count(ltpg_ctx *ctx, uint32_t type, uint32_t *out)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!out) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
-    if (type > LTPG_TYPE_STOCKLEVEL) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
+    if (type > LTPG_TYPE_DELIVERY) LTPG_FAIL(LTPG_ERR_INVALID, "unknown transaction type");
     LTPG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     uint32_t h[kCounterStripes][64];
     LTPG_HIP_TRY(hipMemcpy(h, ctx->counters->commits, sizeof(h), hipMemcpyDeviceToHost));
@@ -1160,6 +1269,7 @@ int ltpg_tpcc_submit(ltpg_ctx *ctx, const ltpg_neworder_batch *nb, const ltpg_pa
     tp.no = no;
     tp.pay = pay;
     tp.ro = ReadOnlyDev();
+    tp.dl = DeliveryDev();
     tp.has_batch = true;
     return LTPG_OK;
 }
@@ -1175,7 +1285,7 @@ int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *ob, c
     if (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)
         LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "read-only transactions need the fast check (no LTPG_FLAG_CONFLICT_COUNTS)");
     const uint32_t n_os = ob ? ob->n : 0, n_sl = sb ? sb->n : 0;
-    if ((uint64_t)tp.no.n + tp.pay.n + n_os + n_sl > ctx->cfg.max_txns)
+    if ((uint64_t)tp.no.n + tp.pay.n + n_os + n_sl + tp.dl.n > ctx->cfg.max_txns)
         LTPG_FAIL(LTPG_ERR_INVALID, "batch exceeds max_txns");
     if (n_os && (!ob->W_ID || !ob->D_ID || !ob->C_ID || !ob->O_ID || !ob->OL_ID || !ob->TID))
         LTPG_FAIL(LTPG_ERR_INVALID, "null Order-Status array");
@@ -1220,6 +1330,48 @@ int ltpg_tpcc_submit_readonly(ltpg_ctx *ctx, const ltpg_orderstatus_batch *ob, c
     return LTPG_OK;
 }
 
+int ltpg_tpcc_submit_delivery(ltpg_ctx *ctx, const ltpg_delivery_batch *db, uint32_t mem)
+{
+    int rc = tpcc_ctx(ctx);
+    if (rc) return rc;
+    TpccState &tp = ctx->tpcc;
+    if (!tp.has_batch) LTPG_FAIL(LTPG_ERR_STATE, "ltpg_tpcc_submit first");
+    if (mem != LTPG_MEM_HOST && mem != LTPG_MEM_DEVICE) LTPG_FAIL(LTPG_ERR_INVALID, "unknown memory kind");
+    if (ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "Deliveries need the fast check (no LTPG_FLAG_CONFLICT_COUNTS)");
+    const uint32_t n = db ? db->n : 0;
+    if ((uint64_t)tp.no.n + tp.pay.n + tp.ro.n_os + tp.ro.n_sl + n > ctx->cfg.max_txns)
+        LTPG_FAIL(LTPG_ERR_INVALID, "batch exceeds max_txns");
+    if (n && (!db->O_ID || !db->NO_W_ID || !db->NO_O_ID || !db->TID)) LTPG_FAIL(LTPG_ERR_INVALID, "null Delivery array");
+    const uint32_t *src[4] = {n ? db->O_ID : nullptr, n ? db->NO_W_ID : nullptr, n ? db->NO_O_ID : nullptr,
+                              n ? db->TID : nullptr};
+    const uint32_t *dev[4];
+    if (mem == LTPG_MEM_DEVICE || n == 0) {
+        for (int k = 0; k < 4; k++) dev[k] = src[k];
+    } else {
+        // the epoch's host slot, or, if its batch was device-resident, the next one
+        const uint64_t C = ctx->cfg.max_txns;
+        int slot = ctx->xfer_slot;
+        if (!tp.host_slot && (rc = xfer_begin(ctx, &slot))) return rc;
+        tp.host_slot = true;
+        if ((rc = slot_buffer(&tp.dl_buf[slot], 4 * C, "Delivery batches"))) return rc;
+        uint32_t *b = tp.dl_buf[slot];
+        for (int k = 0; k < 4; k++) {
+            dev[k] = b + k * C;
+            LTPG_HIP_TRY(hipMemcpyAsync(b + k * C, src[k], (uint64_t)n * 4, hipMemcpyHostToDevice, ctx->copy_stream));
+        }
+        if ((rc = xfer_end(ctx))) return rc;
+    }
+    DeliveryDev dl;
+    dl.n = n;
+    dl.O_ID = dev[0];
+    dl.NO_W_ID = dev[1];
+    dl.NO_O_ID = dev[2];
+    dl.TID = dev[3];
+    tp.dl = dl;
+    return LTPG_OK;
+}
+
 int ltpg_tpcc_payment_cid(ltpg_ctx *ctx, uint32_t *out, uint32_t n)
 {
     int rc = tpcc_ctx(ctx);
@@ -1250,6 +1402,9 @@ int ltpg_tpcc_read_values(ltpg_ctx *ctx, uint32_t type, uint32_t *out, uint64_t 
     } else if (type == LTPG_TYPE_STOCKLEVEL) {
         src = ctx->tpcc.d.reads_sl;
         have = (uint64_t)kSlReads * ctx->tpcc.ro.n_sl;
+    } else if (type == LTPG_TYPE_DELIVERY) {
+        src = ctx->tpcc.d.reads_dl;
+        have = (uint64_t)kDlReads * ctx->tpcc.dl.n;
     } else {
         LTPG_FAIL(LTPG_ERR_INVALID, "type must be an order-entry type");
     }

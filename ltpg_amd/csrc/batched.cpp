@@ -76,6 +76,7 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false
         if ((d.orderstatus.n || d.stocklevel.n) &&
             (rc = ltpg_tpcc_submit_readonly(ctx, &d.orderstatus, &d.stocklevel, LTPG_MEM_DEVICE)))
             return rc;
+        if (d.delivery.n && (rc = ltpg_tpcc_submit_delivery(ctx, &d.delivery, LTPG_MEM_DEVICE))) return rc;
     }
     if ((rc = stepwise ? run_epoch_stepwise(ctx) : ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
     if (d.workload == LTPG_WORKLOAD_KV) {
@@ -89,6 +90,8 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false
         LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_os, words(d.orderstatus.n), ctx->stream));
         at += words(d.orderstatus.n);
         LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_sl, words(d.stocklevel.n), ctx->stream));
+        at += words(d.stocklevel.n);
+        LTPG_HIP_TRY(launch_copy_u64(b->archive + at, ctx->tpcc.d.bitmap_dl, words(d.delivery.n), ctx->stream));
     }
     if (stamp) LTPG_HIP_TRY(launch_stamp(b->stamps + i, ctx->stream));
     return LTPG_OK;
@@ -275,7 +278,8 @@ int ltpg_batched_create(ltpg_ctx *kv_ctx, ltpg_ctx *tpcc_ctx, const ltpg_batch_d
             w += words(d.kv.n_txns);
         } else if (d.workload == LTPG_WORKLOAD_TPCC) {
             if (!tpcc_ctx) { ltpg_batched_destroy(b); LTPG_FAIL(LTPG_ERR_INVALID, "order-entry batch without tpcc_ctx"); }
-            w += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n) + words(d.stocklevel.n);
+            w += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n) + words(d.stocklevel.n) +
+                 words(d.delivery.n);
         } else {
             ltpg_batched_destroy(b);
             LTPG_FAIL(LTPG_ERR_INVALID, "unknown workload in batch descriptor");
@@ -477,6 +481,9 @@ int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t
     } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_STOCKLEVEL) {
         off += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n);
         n = words(d.stocklevel.n);
+    } else if (d.workload == LTPG_WORKLOAD_TPCC && type == LTPG_TYPE_DELIVERY) {
+        off += words(d.neworder.n) + words(d.payment.n) + words(d.orderstatus.n) + words(d.stocklevel.n);
+        n = words(d.delivery.n);
     } else {
         LTPG_FAIL(LTPG_ERR_INVALID, "type does not match the batch's workload");
     }

@@ -223,7 +223,7 @@ __global__ void copy_u32_kernel(uint32_t *__restrict__ dst, const uint32_t *__re
 // with an acquire load also reads the bitmaps.  One block.
 __global__ void publish_kernel(PublishSet p, uint64_t *seq_out, uint64_t seq)
 {
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
         for (uint32_t w = threadIdx.x; w < p.n[k]; w += blockDim.x) p.dst[k][w] = p.src[k][w];
     __threadfence_system();
     __syncthreads();
@@ -272,6 +272,50 @@ __global__ void calib_atomic_kernel(unsigned long long *a, uint64_t len, uint64_
     }
     if (acc == 1ull) sink[0] = acc;
 }
+
+// Cost of a grid-wide barrier, the step a persistent batched kernel would
+// take between phases: gridDim.x workgroups (resident together: the kernel
+// is launched cooperatively) pass `reps` barriers.  Lane 0 of each
+// workgroup reads the generation, arrives on the counter, and the last to
+// arrive resets the counter and advances the generation; the others poll
+// the generation.  Every poll is bounded: past spin_limit polls the
+// workgroup records an error and leaves, so the kernel always ends.
+__global__ void grid_barrier_kernel(unsigned int *count, unsigned int *gen, uint32_t reps, uint32_t spin_limit,
+                                    unsigned int *err)
+{
+    __shared__ uint32_t s_stop;
+    for (uint32_t r = 0; r < reps; ++r) {
+        if (threadIdx.x == 0) {
+            uint32_t stop = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+            const unsigned g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (!stop) {
+                const unsigned arrived =
+                    __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                if (arrived == gridDim.x - 1) {
+                    __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                        if (++spins > spin_limit) {
+                            __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            stop = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+            }
+            s_stop = stop;
+        }
+        __syncthreads();
+        if (s_stop) return;
+    }
+}
+
+// One store per workgroup: the smallest kernel, whose back-to-back launches
+// time a kernel boundary.
+__global__ void boundary_kernel(unsigned int *p) { if (threadIdx.x == 0) p[blockIdx.x] = blockIdx.x; }
 
 // Device wall-clock stamp (constant-rate counter), one lane.
 __global__ void stamp_kernel(unsigned long long *out) { out[threadIdx.x] = wall_clock64(); }
@@ -693,6 +737,78 @@ __device__ __forceinline__ void wave_copy_pairs(unsigned long long mask, uint64_
     }
 }
 
+// Random-gather ceiling: n fields of fw words, each from a random slot
+// (stride_w words apart) of src, copied to dst[i * fw ..] -- the access
+// pattern of CHECK's read-out -- with wave_copy_pairs, 64 fields per wave
+// step.
+__global__ __launch_bounds__(kBlock) void calib_gather_kernel(const uint32_t *__restrict__ src, uint64_t slots,
+                                                             uint32_t stride_w, uint32_t fw,
+                                                             uint32_t *__restrict__ dst, uint64_t n, uint32_t per)
+{
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint64_t i = (wave * per + k) * 64 + (uint64_t)lane_id();
+        const bool has = i < n;
+        const uint64_t so = has ? calib_index(i, slots) * stride_w : 0ull;
+        wave_copy_pairs(__ballot(has), so, has ? i * fw : 0ull, 0u, src, dst, fw);
+    }
+}
+
+// Per-wave staging of field copies.  A wave walks its transactions in
+// rounds of 64 / G; a round of a low-contention batch flags only a few
+// fields, and copying them right away leaves a wave with one round's fields
+// in flight.  Instead each round appends its flagged fields (source,
+// destination, delta) to the wave's own LDS list, and the list is copied
+// once it holds 64 or more (or at the end): wave_copy_pairs then has up to 32
+// fields in flight per step, and rounds of op loads run back to back.  The
+// list holds a round's worth more than the flush threshold.  Only the wave
+// that owns a list touches it, so a wavefront-scope fence orders its lanes'
+// LDS accesses.
+constexpr uint32_t kStage = 128;
+
+struct StageList {
+    unsigned long long src[kStage];
+    unsigned long long dst[kStage];
+    uint32_t delta[kStage];
+};
+
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Append the flagged lanes' copies; returns the new (wave-uniform) length.
+__device__ __forceinline__ uint32_t stage_push(StageList &l, uint32_t n, bool flag, uint64_t src_off,
+                                               uint64_t dst_off, uint32_t delta)
+{
+    const unsigned long long m = __ballot(flag);
+    if (flag) {
+        const uint32_t at = n + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+        l.src[at] = src_off;
+        l.dst[at] = dst_off;
+        l.delta[at] = delta;
+    }
+    return n + (uint32_t)__popcll(m);
+}
+
+// Copy the wave's n staged fields (fw <= 32) and empty the list.  The
+// staged fields never overlap: callers stage distinct destinations.
+__device__ __forceinline__ void stage_flush(StageList &l, uint32_t n, const uint32_t *src, uint32_t *dst,
+                                            uint32_t fw)
+{
+    wave_lds_fence();
+    for (uint32_t b = 0; b < n; b += 64) {
+        const uint32_t e = b + (uint32_t)lane_id();
+        const bool has = e < n;
+        const uint64_t so = has ? l.src[e] : 0ull, d = has ? l.dst[e] : 0ull;
+        const uint32_t dl = has ? l.delta[e] : 0u;
+        wave_copy_pairs(__ballot(has), so, d, dl, src, dst, fw);
+    }
+    wave_lds_fence();
+}
+
 // One lane copies one field of fw <= 32 words (src_off a multiple of 4
 // words): every 16-byte load of the field is issued before the first store,
 // so a wave has all its fields in flight in one round.  Stores are 16 bytes
@@ -782,6 +898,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     __shared__ uint32_t s_list[CH];
     __shared__ unsigned long long s_bits[CH / 64];
     __shared__ uint32_t s_cnt[kBlock / 64];
+    __shared__ StageList s_stage[kBlock / 64];
     const int lane = lane_id();
     const uint32_t chunk0 = blockIdx.x * CH;
     const uint32_t t0 = chunk0 + threadIdx.x;
@@ -794,6 +911,8 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
     const uint32_t n = list_build(commit, t0, s_list, s_cnt);
     constexpr uint32_t PER = 64 / G;  // transactions per wave per round
     const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
+    const bool staged = !per_lane && fw <= 32 && L == 1;
+    uint32_t pending = 0;
     for (uint32_t j0 = (blockIdx.y * (kBlock / 64) + threadIdx.x / 64) * PER; j0 < n;
          j0 += gridDim.y * (kBlock / 64) * PER) {
         const uint32_t j = j0 + grp;
@@ -801,7 +920,15 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
         const uint32_t t = has ? s_list[j] : 0u;
         const uint64_t i = (uint64_t)gl * B + t;  // slot-major: where its reads go
         const Op o = load_op(keys, meta, op_at(gl, t, B, S, rm), has && gl < S, rows, fields, max_scan);
-        if (per_lane) {
+        if (staged) {
+            // every read slot has its own destination
+            pending = stage_push(s_stage[threadIdx.x / 64], pending, o.len != 0, (o.row * fields + o.field) * fsw,
+                                 i * fw, 0u);
+            if (pending >= 64) {
+                stage_flush(s_stage[threadIdx.x / 64], pending, table, read_values, fw);
+                pending = 0;
+            }
+        } else if (per_lane) {
             // one lane per read op: all fields of a round in flight at once
             for (uint32_t k = 0; k < o.len; ++k)
                 lane_copy(true, ((o.row + k) * fields + o.field) * fsw, (i * L + k) * fw, 0u, table,
@@ -836,6 +963,7 @@ __global__ __launch_bounds__(kBlock) void kv_check_fast_kernel(
             }
         }
     }
+    if (pending) stage_flush(s_stage[threadIdx.x / 64], pending, table, read_values, fw);
     if (blockIdx.y != 0) return;  // the chunk's other blocks only read out
     const uint32_t nwords = (B + 63) / 64;
     if (threadIdx.x < CH / 64 && chunk0 / 64 + threadIdx.x < nwords)
@@ -864,6 +992,7 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
 {
     __shared__ uint32_t s_list[CH];
     __shared__ uint32_t s_cnt[kBlock / 64];
+    __shared__ StageList s_stage[kBlock / 64];
     const int lane = lane_id();
     const uint32_t chunk0 = blockIdx.x * CH;
     const uint32_t t0 = chunk0 + threadIdx.x;
@@ -873,6 +1002,7 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
                                   s_list, s_cnt);
     constexpr uint32_t PER = 64 / G;
     const uint32_t grp = (uint32_t)lane / G, gl = (uint32_t)lane % G;
+    uint32_t pending = 0;
     for (uint32_t j0 = (blockIdx.y * (kBlock / 64) + threadIdx.x / 64) * PER; j0 < n;
          j0 += gridDim.y * (kBlock / 64) * PER) {
         const uint32_t j = j0 + grp;
@@ -900,12 +1030,19 @@ __global__ __launch_bounds__(kBlock) void kv_writeback_kernel(
             lane_copy(wr && first, off, off, step * count, table, table, fw);
             continue;
         }
-        const unsigned long long am = __ballot(wr && first);
-        if (fw <= 32)
-            wave_copy_pairs(am, off, off, step * count, table, table, fw);
-        else
-            group_copy<G, 8>(am, off, off, step * count, table, table, fw, S, grp, gl);
+        if (fw <= 32) {
+            // a row's committed writer is unique and its duplicate writes
+            // are folded: staged fields never overlap
+            pending = stage_push(s_stage[threadIdx.x / 64], pending, wr && first, off, off, step * count);
+            if (pending >= 64) {
+                stage_flush(s_stage[threadIdx.x / 64], pending, table, table, fw);
+                pending = 0;
+            }
+            continue;
+        }
+        group_copy<G, 8>(__ballot(wr && first), off, off, step * count, table, table, fw, S, grp, gl);
     }
+    if (pending) stage_flush(s_stage[threadIdx.x / 64], pending, table, table, fw);
 }
 
 // One small epoch in one workgroup (LTPG_TUNE_SMALL_EPOCH, and the served
@@ -1881,6 +2018,30 @@ hipError_t launch_calibration(unsigned long long *a, uint64_t len, uint64_t n, i
         hipLaunchKernelGGL(calib_atomic_kernel, grid, dim3(kBlock), 0, s, a, len, n, per, sink);
     else
         hipLaunchKernelGGL(calib_load_kernel, grid, dim3(kBlock), 0, s, a, len, n, per, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_calibration(const uint32_t *src, uint64_t slots, uint32_t stride_w, uint32_t fw,
+                                     uint32_t *dst, uint64_t n, hipStream_t s)
+{
+    constexpr uint32_t per = 4;
+    const uint64_t waves = (n + 64ull * per - 1) / (64ull * per);
+    const dim3 grid((unsigned)((waves * 64 + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(calib_gather_kernel, grid, dim3(kBlock), 0, s, src, slots, stride_w, fw, dst, n, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_barrier(uint32_t blocks, unsigned int *count, unsigned int *gen, uint32_t reps,
+                               uint32_t spin_limit, unsigned int *err, hipStream_t s)
+{
+    void *args[] = {&count, &gen, &reps, &spin_limit, &err};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(grid_barrier_kernel), dim3(blocks), dim3(kBlock),
+                                      args, 0, s);
+}
+
+hipError_t launch_boundary(uint32_t blocks, unsigned int *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(boundary_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -36,7 +36,7 @@ void set_error(const std::string &msg);
 constexpr int kCounterStripes = 64;
 
 constexpr uint32_t kAttempts = 16;  // attempts of type t at totals[stripe][kAttempts + t]
-constexpr uint32_t kCountedTypes = 5;  // LTPG_TYPE_KV .. LTPG_TYPE_STOCKLEVEL
+constexpr uint32_t kCountedTypes = 6;  // LTPG_TYPE_KV .. LTPG_TYPE_DELIVERY
 
 struct DeviceCounters {
     uint32_t range_errors;                            // ops outside their table
@@ -113,6 +113,7 @@ constexpr uint32_t kNoReads = 51;   // read values per NewOrder
 constexpr uint32_t kPayReads = 24;  // read values per Payment
 constexpr uint32_t kOsReads = 3;    // read values per Order-Status
 constexpr uint32_t kSlReads = 1;    // read values per Stock-Level
+constexpr uint32_t kDlReads = 2;    // read values per Delivery
 
 // One table on the device: row-major, `stride` u32 words per row (cols
 // rounded up to a multiple of 4 so rows start 16-byte aligned).
@@ -144,6 +145,11 @@ struct ReadOnlyDev {
                    *os_OL_ID = nullptr, *os_TID = nullptr;
     const uint32_t *sl_W_ID = nullptr, *sl_S_ID = nullptr, *sl_TID = nullptr;
 };
+// Deliveries of the current batch (the fields of ltpg_delivery_batch).
+struct DeliveryDev {
+    uint32_t n = 0;
+    const uint32_t *O_ID = nullptr, *NO_W_ID = nullptr, *NO_O_ID = nullptr, *TID = nullptr;
+};
 
 // Everything the order-entry kernels take, passed by value.
 struct TpccDev {
@@ -169,6 +175,9 @@ struct TpccDev {
     uint8_t *war = nullptr;
     uint64_t *bitmap_os = nullptr, *bitmap_sl = nullptr;
     uint32_t *reads_os = nullptr, *reads_sl = nullptr;
+    // Deliveries: bitmap and read values (value-major)
+    uint64_t *bitmap_dl = nullptr;
+    uint32_t *reads_dl = nullptr;
 };
 
 struct TpccState {
@@ -177,10 +186,12 @@ struct TpccState {
     uint32_t *no_buf[2] = {nullptr, nullptr};   // packed NEWORDER_QUERY, capacity max_txns
     uint32_t *pay_buf[2] = {nullptr, nullptr};  // packed PAYMENT_QUERY, capacity max_txns
     uint32_t *ro_buf[2] = {nullptr, nullptr};   // read-only batches: 6 + 3 arrays of capacity max_txns
+    uint32_t *dl_buf[2] = {nullptr, nullptr};   // Deliveries: 4 arrays of capacity max_txns
     bool host_slot = false;                     // the current epoch's batch went through a host slot
     NeworderDev no;
     PaymentDev pay;
     ReadOnlyDev ro;
+    DeliveryDev dl;
     bool has_batch = false;
 };
 
@@ -246,6 +257,11 @@ inline int check_ctx(ltpg_ctx *ctx)
 // Kernel launchers (kv_kernels.hip).  All asynchronous on `s`.
 hipError_t launch_reset_epoch_counters(DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_t s);
+hipError_t launch_grid_barrier(uint32_t blocks, unsigned int *count, unsigned int *gen, uint32_t reps,
+                               uint32_t spin_limit, unsigned int *err, hipStream_t s);
+hipError_t launch_boundary(uint32_t blocks, unsigned int *p, hipStream_t s);
+hipError_t launch_gather_calibration(const uint32_t *src, uint64_t slots, uint32_t stride_w, uint32_t fw,
+                                     uint32_t *dst, uint64_t n, hipStream_t s);
 hipError_t launch_calibration(unsigned long long *a, uint64_t len, uint64_t n, int atomics,
                               unsigned long long *sink, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v, hipStream_t s);
@@ -256,9 +272,9 @@ hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t n, hipSt
 // Up to four bitmaps to copy to pinned host memory (n[k] words from src[k]
 // to dst[k]; n[k] = 0 for unused entries).
 struct PublishSet {
-    const uint64_t *src[4];
-    uint64_t *dst[4];
-    uint32_t n[4];
+    const uint64_t *src[5];
+    uint64_t *dst[5];
+    uint32_t n[5];
 };
 // copy the bitmaps of p, then store seq to *seq_out (pinned host memory)
 // with a system-scope release
@@ -381,8 +397,16 @@ hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const Pa
                                const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_t base, const EpochTag &e,
                                 DeviceCounters *ctr, hipStream_t s);
+// Deliveries (base = their first batch position): EXECUTE registrations,
+// CHECK (commit, bitmap, read values, counts) and WRITEBACK.
+hipError_t launch_tpcc_delivery_execute(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                        DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_delivery_check(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                      DeviceCounters *ctr, hipStream_t s);
+hipError_t launch_tpcc_delivery_writeback(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                          uint32_t values, hipStream_t s);
 hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                             bool counts, bool ro, DeviceCounters *ctr, hipStream_t s);
+                             bool counts, bool ro, bool dl, DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
                                  const EpochTag &e, uint32_t values, hipStream_t s);
 

@@ -6,8 +6,10 @@
 // (submission = arrival) and epochs are formed continuously:
 //
 //   priority   each workload numbers its arrivals with a 64-bit sequence
-//              number; an epoch's priorities are its members' ranks in that
-//              order, so the u32 priorities of the kernels never wrap;
+//              number; a batch TID is its low 32 bits (the key-value RMW
+//              delta and the Delivery's values use it), and an epoch never
+//              spans a 2^32 boundary of the low bits, so within an epoch the
+//              TIDs are in arrival order;
 //   admission  pending requests of a workload -- re-queued aborts first, in
 //              sequence order, then new arrivals -- fill the next epoch up to
 //              the per-epoch transaction budget, derived from the HBM budget
@@ -78,6 +80,7 @@ constexpr uint32_t kNoFields = 52;  // NewOrder request: 7 head fields + 3 x 15 
 constexpr uint32_t kPayFields = 9;
 constexpr uint32_t kOsFields = 5;    // Order-Status: W_ID, D_ID, C_ID, O_ID, OL_ID
 constexpr uint32_t kSlFields = 2;    // Stock-Level: W_ID, S_ID
+constexpr uint32_t kDlFields = 3;    // Delivery: O_ID, NO_W_ID, NO_O_ID
 constexpr uint64_t kAdaptiveFloor = 32;  // default smallest epoch the commit-based cap allows
 constexpr uint64_t kDefaultRing = 1ull << 16;
 constexpr uint64_t kDoneWords = 1ull << 20;          // done ring of the device loop
@@ -176,8 +179,7 @@ struct Flight {
     bool busy = false;
     uint64_t epoch = 0;
     std::vector<uint64_t> members;  // sequence numbers, batch order (order-entry: NewOrders, Payments, ...)
-    std::vector<uint32_t> ranks;    // each member's rank in sequence order (its in-epoch priority)
-    uint32_t n_no = 0, n_pay = 0, n_os = 0, n_sl = 0;  // order-entry members by type, in this order
+    uint32_t n_no = 0, n_pay = 0, n_os = 0, n_sl = 0, n_dl = 0;  // order-entry members by type, in this order
     hipEvent_t copied = nullptr;     // order-entry staging copied (copy stream)
     uint64_t *h_seq = nullptr;       // pinned: sequence number of the last published epoch
     uint64_t seq = 0;                // sequence number of the epoch in flight
@@ -190,9 +192,10 @@ struct Flight {
     // order-entry staging (pinned, packed query blocks) and device slot
     uint32_t *h_no = nullptr, *d_no = nullptr, *h_pay = nullptr, *d_pay = nullptr;
     uint32_t *h_os = nullptr, *d_os = nullptr, *h_sl = nullptr, *d_sl = nullptr;  // read-only, value-major + TID
+    uint32_t *h_dl = nullptr, *d_dl = nullptr;                                    // Deliveries, likewise
     // results (pinned)
-    uint64_t *h_bits = nullptr, *h_bits2 = nullptr, *h_bits3 = nullptr, *h_bits4 = nullptr;
-    uint32_t *h_reads = nullptr, *h_reads2 = nullptr, *h_reads3 = nullptr, *h_reads4 = nullptr;
+    uint64_t *h_bits = nullptr, *h_bits2 = nullptr, *h_bits3 = nullptr, *h_bits4 = nullptr, *h_bits5 = nullptr;
+    uint32_t *h_reads = nullptr, *h_reads2 = nullptr, *h_reads3 = nullptr, *h_reads4 = nullptr, *h_reads5 = nullptr;
 };
 
 // The device loop of the key-value lane (kv_loop).
@@ -307,8 +310,12 @@ void free_flight(Flight &f)
     host_free(f.h_bits4);
     host_free(f.h_reads3);
     host_free(f.h_reads4);
+    host_free(f.h_dl);
+    host_free(f.h_bits5);
+    host_free(f.h_reads5);
     dev_free(f.d_os);
     dev_free(f.d_sl);
+    dev_free(f.d_dl);
     host_free(f.h_bits);
     host_free(f.h_bits2);
     host_free(f.h_reads);
@@ -351,16 +358,19 @@ int alloc_flight(ltpg_server *sv, int w, Flight &f)
         if ((rc = host_alloc(&f.h_no, 53 * C)) || (rc = host_alloc(&f.h_pay, 10 * C)) ||
             (rc = host_alloc(&f.h_os, (kOsFields + 1) * C)) || (rc = host_alloc(&f.h_sl, (kSlFields + 1) * C)) ||
             (rc = host_alloc(&f.h_bits, words(C), true)) || (rc = host_alloc(&f.h_bits2, words(C), true)) ||
-            (rc = host_alloc(&f.h_bits3, words(C), true)) || (rc = host_alloc(&f.h_bits4, words(C), true)))
+            (rc = host_alloc(&f.h_bits3, words(C), true)) || (rc = host_alloc(&f.h_bits4, words(C), true)) ||
+            (rc = host_alloc(&f.h_dl, (kDlFields + 1) * C)) || (rc = host_alloc(&f.h_bits5, words(C), true)))
             return rc;
         if (sv->cfg.keep_reads &&
             ((rc = host_alloc(&f.h_reads, (uint64_t)kNoReads * C)) || (rc = host_alloc(&f.h_reads2, (uint64_t)kPayReads * C)) ||
-             (rc = host_alloc(&f.h_reads3, (uint64_t)kOsReads * C)) || (rc = host_alloc(&f.h_reads4, (uint64_t)kSlReads * C))))
+             (rc = host_alloc(&f.h_reads3, (uint64_t)kOsReads * C)) || (rc = host_alloc(&f.h_reads4, (uint64_t)kSlReads * C)) ||
+             (rc = host_alloc(&f.h_reads5, (uint64_t)kDlReads * C))))
             return rc;
         if ((rc = dev_alloc(&f.d_no, 53 * C, "serving NewOrder slot")) ||
             (rc = dev_alloc(&f.d_pay, 10 * C, "serving Payment slot")) ||
             (rc = dev_alloc(&f.d_os, (kOsFields + 1) * C, "serving Order-Status slot")) ||
-            (rc = dev_alloc(&f.d_sl, (kSlFields + 1) * C, "serving Stock-Level slot")))
+            (rc = dev_alloc(&f.d_sl, (kSlFields + 1) * C, "serving Stock-Level slot")) ||
+            (rc = dev_alloc(&f.d_dl, (kDlFields + 1) * C, "serving Delivery slot")))
             return rc;
     }
     return LTPG_OK;
@@ -439,13 +449,18 @@ int alloc_loop(ltpg_server *sv)
 // Position of an order-entry type in an epoch's batch.
 uint32_t type_rank(uint32_t type)
 {
-    return type == LTPG_TYPE_NEWORDER ? 0 : type == LTPG_TYPE_PAYMENT ? 1 : type == LTPG_TYPE_ORDERSTATUS ? 2 : 3;
+    return type == LTPG_TYPE_NEWORDER      ? 0
+           : type == LTPG_TYPE_PAYMENT     ? 1
+           : type == LTPG_TYPE_ORDERSTATUS ? 2
+           : type == LTPG_TYPE_STOCKLEVEL  ? 3
+                                           : 4;
 }
 
 const uint32_t *req_data(Lane &L, uint64_t seq) { return reinterpret_cast<const uint32_t *>(L.store.data(seq)); }
 
 // Pack an order-entry epoch's members into the flight's pinned blocks
-// (lock held): value-major columns of stride C, priorities (ranks) in the
+// (lock held): value-major columns of stride C, TIDs (low 32 bits of the
+// sequence numbers) in the
 // last column.
 void pack_tpcc(Lane &L, Flight &f, uint64_t C)
 {
@@ -453,23 +468,34 @@ void pack_tpcc(Lane &L, Flight &f, uint64_t C)
     for (uint32_t q = 0; q < f.n_no; q++, at++) {
         const uint32_t *src = req_data(L, f.members[at]);
         for (uint32_t k = 0; k < kNoFields; k++) f.h_no[(uint64_t)k * C + q] = src[k];
-        f.h_no[52 * C + q] = f.ranks[at];
+        f.h_no[52 * C + q] = (uint32_t)f.members[at];
     }
     for (uint32_t p = 0; p < f.n_pay; p++, at++) {
         const uint32_t *src = req_data(L, f.members[at]);
         for (uint32_t k = 0; k < kPayFields; k++) f.h_pay[(uint64_t)k * C + p] = src[k];
-        f.h_pay[9 * C + p] = f.ranks[at];
+        f.h_pay[9 * C + p] = (uint32_t)f.members[at];
     }
     for (uint32_t o = 0; o < f.n_os; o++, at++) {
         const uint32_t *src = req_data(L, f.members[at]);
         for (uint32_t k = 0; k < kOsFields; k++) f.h_os[(uint64_t)k * C + o] = src[k];
-        f.h_os[kOsFields * C + o] = f.ranks[at];
+        f.h_os[kOsFields * C + o] = (uint32_t)f.members[at];
     }
     for (uint32_t l = 0; l < f.n_sl; l++, at++) {
         const uint32_t *src = req_data(L, f.members[at]);
         for (uint32_t k = 0; k < kSlFields; k++) f.h_sl[(uint64_t)k * C + l] = src[k];
-        f.h_sl[kSlFields * C + l] = f.ranks[at];
+        f.h_sl[kSlFields * C + l] = (uint32_t)f.members[at];
     }
+    for (uint32_t d = 0; d < f.n_dl; d++, at++) {
+        const uint32_t *src = req_data(L, f.members[at]);
+        for (uint32_t k = 0; k < kDlFields; k++) f.h_dl[(uint64_t)k * C + d] = src[k];
+        f.h_dl[kDlFields * C + d] = (uint32_t)f.members[at];
+    }
+}
+
+// A Delivery batch over value-major columns of stride C (host or device).
+ltpg_delivery_batch delivery_view(uint32_t *dl, uint64_t C, uint32_t n)
+{
+    return ltpg_delivery_batch{n, 0, dl, dl + C, dl + 2 * C, dl + kDlFields * C};
 }
 
 // Read-only batches over value-major columns of stride C (host or device).
@@ -482,14 +508,13 @@ void readonly_views(uint32_t *os, uint32_t *sl, uint64_t C, uint32_t n_os, uint3
 
 // Take the next host-formed epoch's members (lock held): re-queued first,
 // then new arrivals, both in sequence order (every re-queued request arrived
-// before every request still new).  A key-value epoch, whose batch TIDs are
-// the low 32 bits of the sequence numbers (its RMW deltas), ends before the
-// 2^32 boundary of the low bits, so its TIDs stay in priority order.
+// before every request still new).  An epoch, whose batch TIDs are the low
+// 32 bits of the sequence numbers, ends before the 2^32 boundary of the low
+// bits, so its TIDs stay in priority order.
 void take_members(ltpg_server *sv, int w, Flight &f)
 {
     Lane &L = sv->lane[w];
     f.members.clear();
-    f.ranks.clear();
     // Under contention an epoch commits a short chain of the lowest
     // priorities whatever its size, so a large epoch mostly re-runs aborts:
     // cap it at epoch_growth times the last epoch's commits (not below
@@ -499,7 +524,7 @@ void take_members(ltpg_server *sv, int w, Flight &f)
     if (sv->cfg.epoch_growth && L.last_commits != ~0ull)
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(min_epoch, sv->cfg.epoch_growth * L.last_commits));
     const auto fits = [&](uint64_t seq) {
-        return f.members.empty() || w != 0 || (seq >> 32) == (f.members.front() >> 32);
+        return f.members.empty() || (seq >> 32) == (f.members.front() >> 32);
     };
     while (f.members.size() < cap && !L.requeue.empty() && fits(L.requeue.top())) {
         f.members.push_back(L.requeue.top());
@@ -509,27 +534,23 @@ void take_members(ltpg_server *sv, int w, Flight &f)
         f.members.push_back(L.fresh.front());
         L.fresh.pop_front();
     }
-    f.ranks.resize(f.members.size());
-    for (uint32_t j = 0; j < f.ranks.size(); ++j) f.ranks[j] = j;
-    if (w == 1) {  // NewOrders, Payments, Order-Status, Stock-Level, each in sequence order
+    if (w == 1) {  // NewOrders, Payments, Order-Status, Stock-Level, Deliveries, each in sequence order
         std::vector<uint32_t> perm(f.members.size());
         for (uint32_t j = 0; j < perm.size(); ++j) perm[j] = j;
         std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
             return type_rank(L.store.at(f.members[a]).type) < type_rank(L.store.at(f.members[b]).type);
         });
         std::vector<uint64_t> m(perm.size());
-        for (uint32_t j = 0; j < perm.size(); ++j) {
-            m[j] = f.members[perm[j]];
-            f.ranks[j] = perm[j];
-        }
+        for (uint32_t j = 0; j < perm.size(); ++j) m[j] = f.members[perm[j]];
         f.members.swap(m);
-        f.n_no = f.n_pay = f.n_os = f.n_sl = 0;
+        f.n_no = f.n_pay = f.n_os = f.n_sl = f.n_dl = 0;
         for (uint64_t s : f.members) {
             const uint32_t ty = L.store.at(s).type;
             f.n_no += ty == LTPG_TYPE_NEWORDER;
             f.n_pay += ty == LTPG_TYPE_PAYMENT;
             f.n_os += ty == LTPG_TYPE_ORDERSTATUS;
             f.n_sl += ty == LTPG_TYPE_STOCKLEVEL;
+            f.n_dl += ty == LTPG_TYPE_DELIVERY;
         }
     }
 }
@@ -595,7 +616,7 @@ int launch(ltpg_server *sv, int w, Flight &f)
             LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
         }
     } else {
-        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl;
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl, n_dl = f.n_dl;
         pack_tpcc(L, f, C);
         if (n_no)
             LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_no, C * 4, f.h_no, C * 4, (uint64_t)n_no * 4, 53,
@@ -608,6 +629,9 @@ int launch(ltpg_server *sv, int w, Flight &f)
                                           hipMemcpyHostToDevice, L.copy));
         if (n_sl)
             LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_sl, C * 4, f.h_sl, C * 4, (uint64_t)n_sl * 4, kSlFields + 1,
+                                          hipMemcpyHostToDevice, L.copy));
+        if (n_dl)
+            LTPG_HIP_TRY(hipMemcpy2DAsync(f.d_dl, C * 4, f.h_dl, C * 4, (uint64_t)n_dl * 4, kDlFields + 1,
                                           hipMemcpyHostToDevice, L.copy));
         LTPG_HIP_TRY(hipEventRecord(f.copied, L.copy));
         LTPG_HIP_TRY(hipStreamWaitEvent(ctx->stream, f.copied, 0));
@@ -622,21 +646,29 @@ int launch(ltpg_server *sv, int w, Flight &f)
             readonly_views(f.d_os, f.d_sl, C, n_os, n_sl, &ob, &sb);
             if ((rc = ltpg_tpcc_submit_readonly(ctx, &ob, &sb, LTPG_MEM_DEVICE))) return rc;
         }
+        if (n_dl) {
+            const ltpg_delivery_batch db = delivery_view(f.d_dl, C, n_dl);
+            if ((rc = ltpg_tpcc_submit_delivery(ctx, &db, LTPG_MEM_DEVICE))) return rc;
+        }
         if ((rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL))) return rc;
         if (sv->cfg.keep_reads) {
-            const std::pair<uint32_t *, const uint32_t *> reads[4] = {
+            const std::pair<uint32_t *, const uint32_t *> reads[5] = {
                 {f.h_reads, ctx->tpcc.d.reads_no}, {f.h_reads2, ctx->tpcc.d.reads_pay},
-                {f.h_reads3, ctx->tpcc.d.reads_os}, {f.h_reads4, ctx->tpcc.d.reads_sl}};
-            const uint64_t n_words[4] = {(uint64_t)kNoReads * n_no, (uint64_t)kPayReads * n_pay,
-                                         (uint64_t)kOsReads * n_os, (uint64_t)kSlReads * n_sl};
-            for (int k = 0; k < 4; k++)
+                {f.h_reads3, ctx->tpcc.d.reads_os}, {f.h_reads4, ctx->tpcc.d.reads_sl},
+                {f.h_reads5, ctx->tpcc.d.reads_dl}};
+            const uint64_t n_words[5] = {(uint64_t)kNoReads * n_no, (uint64_t)kPayReads * n_pay,
+                                         (uint64_t)kOsReads * n_os, (uint64_t)kSlReads * n_sl,
+                                         (uint64_t)kDlReads * n_dl};
+            for (int k = 0; k < 5; k++)
                 if (n_words[k])
                     LTPG_HIP_TRY(hipMemcpyAsync(reads[k].first, reads[k].second, n_words[k] * 4,
                                                 hipMemcpyDeviceToHost, ctx->stream));
         }
-        PublishSet ps{{ctx->tpcc.d.bitmap_no, ctx->tpcc.d.bitmap_pay, ctx->tpcc.d.bitmap_os, ctx->tpcc.d.bitmap_sl},
-                      {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4},
-                      {(uint32_t)words(n_no), (uint32_t)words(n_pay), (uint32_t)words(n_os), (uint32_t)words(n_sl)}};
+        PublishSet ps{{ctx->tpcc.d.bitmap_no, ctx->tpcc.d.bitmap_pay, ctx->tpcc.d.bitmap_os, ctx->tpcc.d.bitmap_sl,
+                       ctx->tpcc.d.bitmap_dl},
+                      {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4, f.h_bits5},
+                      {(uint32_t)words(n_no), (uint32_t)words(n_pay), (uint32_t)words(n_os), (uint32_t)words(n_sl),
+                       (uint32_t)words(n_dl)}};
         LTPG_HIP_TRY(launch_publish(ps, f.h_seq, ++f.seq, ctx->stream));
     }
     f.launched_ns = now_ns();
@@ -684,7 +716,7 @@ int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex>
         if (!rc && sv->cfg.keep_reads)
             rc = ltpg_kv_read_values(ctx, f.h_reads, (uint64_t)S * B * ctx->kv.read_slot * ctx->kv.field_words);
     } else {
-        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl;
+        const uint32_t n_no = f.n_no, n_pay = f.n_pay, n_os = f.n_os, n_sl = f.n_sl, n_dl = f.n_dl;
         pack_tpcc(L, f, C);
         lk.unlock();
         ltpg_neworder_batch nb;
@@ -698,13 +730,18 @@ int launch_plain(ltpg_server *sv, int w, Flight &f, std::unique_lock<std::mutex>
             readonly_views(f.h_os, f.h_sl, C, n_os, n_sl, &ob, &sb);
             rc = ltpg_tpcc_submit_readonly(ctx, &ob, &sb, LTPG_MEM_HOST);
         }
+        if (!rc && n_dl) {
+            const ltpg_delivery_batch db = delivery_view(f.h_dl, C, n_dl);
+            rc = ltpg_tpcc_submit_delivery(ctx, &db, LTPG_MEM_HOST);
+        }
         if (!rc) rc = run_epoch_stepwise(ctx);
-        const uint32_t types[4] = {LTPG_TYPE_NEWORDER, LTPG_TYPE_PAYMENT, LTPG_TYPE_ORDERSTATUS, LTPG_TYPE_STOCKLEVEL};
-        const uint32_t counts[4] = {n_no, n_pay, n_os, n_sl};
-        const uint32_t per[4] = {kNoReads, kPayReads, kOsReads, kSlReads};
-        uint64_t *bits[4] = {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4};
-        uint32_t *reads[4] = {f.h_reads, f.h_reads2, f.h_reads3, f.h_reads4};
-        for (int k = 0; k < 4 && !rc; k++) {
+        const uint32_t types[5] = {LTPG_TYPE_NEWORDER, LTPG_TYPE_PAYMENT, LTPG_TYPE_ORDERSTATUS, LTPG_TYPE_STOCKLEVEL,
+                                   LTPG_TYPE_DELIVERY};
+        const uint32_t counts[5] = {n_no, n_pay, n_os, n_sl, n_dl};
+        const uint32_t per[5] = {kNoReads, kPayReads, kOsReads, kSlReads, kDlReads};
+        uint64_t *bits[5] = {f.h_bits, f.h_bits2, f.h_bits3, f.h_bits4, f.h_bits5};
+        uint32_t *reads[5] = {f.h_reads, f.h_reads2, f.h_reads3, f.h_reads4, f.h_reads5};
+        for (int k = 0; k < 5 && !rc; k++) {
             if (!counts[k]) continue;
             rc = ltpg_commit_bitmap(ctx, types[k], bits[k], words(counts[k]));
             if (!rc && sv->cfg.keep_reads)
@@ -781,13 +818,14 @@ void complete(ltpg_server *sv, int w, Flight &f)
         r.attempts++;
         sv->attempts[r.type]++;
         bool ok;
-        // order-entry members: NewOrders, Payments, Order-Status, Stock-Level
-        const uint32_t c1 = f.n_no, c2 = c1 + f.n_pay, c3 = c2 + f.n_os;
+        // order-entry members: NewOrders, Payments, Order-Status, Stock-Level, Deliveries
+        const uint32_t c1 = f.n_no, c2 = c1 + f.n_pay, c3 = c2 + f.n_os, c4 = c3 + f.n_sl;
         if (w == 0) ok = bit(f.h_bits, t);
         else if (t < c1) ok = bit(f.h_bits, t);
         else if (t < c2) ok = bit(f.h_bits2, t - c1);
         else if (t < c3) ok = bit(f.h_bits3, t - c2);
-        else ok = bit(f.h_bits4, t - c3);
+        else if (t < c4) ok = bit(f.h_bits4, t - c3);
+        else ok = bit(f.h_bits5, t - c4);
         if (!ok) {
             L.requeue.push(seq);
             continue;
@@ -810,9 +848,12 @@ void complete(ltpg_server *sv, int w, Flight &f)
             } else if (t < c3) {
                 const uint32_t o = t - c2;
                 for (uint32_t k = 0; k < kOsReads; k++) v.push_back(f.h_reads3[(uint64_t)k * f.n_os + o]);
-            } else {
+            } else if (t < c4) {
                 const uint32_t l = t - c3;
                 for (uint32_t k = 0; k < kSlReads; k++) v.push_back(f.h_reads4[(uint64_t)k * f.n_sl + l]);
+            } else {
+                const uint32_t d = t - c4;
+                for (uint32_t k = 0; k < kDlReads; k++) v.push_back(f.h_reads5[(uint64_t)k * f.n_dl + d]);
             }
         }
         commit_request(sv, L, seq, f.epoch, t_now, sv->cfg.keep_reads ? &v : nullptr);
@@ -1369,6 +1410,31 @@ int ltpg_serve_submit_readonly(ltpg_server *sv, const ltpg_orderstatus_batch *os
         L.fresh.push_back(seq);
     }
     sv->outstanding += n_os + n_sl;
+    return LTPG_OK;
+}
+
+int ltpg_serve_submit_delivery(ltpg_server *sv, const ltpg_delivery_batch *dl, uint64_t *first_id)
+{
+    if (!sv) LTPG_FAIL(LTPG_ERR_INVALID, "null server");
+    if (!sv->lane[1].ctx) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "server has no order-entry context");
+    if (sv->lane[1].ctx->cfg.flags & LTPG_FLAG_CONFLICT_COUNTS)
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "Deliveries need the fast check (no conflict counts)");
+    const uint32_t n = dl ? dl->n : 0;
+    if (n && (!dl->O_ID || !dl->NO_W_ID || !dl->NO_O_ID)) LTPG_FAIL(LTPG_ERR_INVALID, "bad Delivery arrays");
+    const uint64_t t = now_ns();
+    ClientLock g(sv);
+    if (sv->stop) LTPG_FAIL(LTPG_ERR_STATE, "server is stopping");
+    if (first_id) *first_id = sv->next_id;
+    Lane &L = sv->lane[1];
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t seq = new_request(sv, L, LTPG_TYPE_DELIVERY, t);
+        uint32_t *d = reinterpret_cast<uint32_t *>(L.store.data(seq));
+        d[0] = dl->O_ID[i];
+        d[1] = dl->NO_W_ID[i];
+        d[2] = dl->NO_O_ID[i];
+        L.fresh.push_back(seq);
+    }
+    sv->outstanding += n;
     return LTPG_OK;
 }
 

@@ -482,7 +482,8 @@ __global__ __launch_bounds__(kBlock) void readonly_kernel(TpccDev d, ReadOnlyDev
 
 template <bool COUNTS>
 __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
-                                                      uint8_t epoch, uint32_t no_waves, bool ro, DeviceCounters *ctr)
+                                                      uint8_t epoch, uint32_t no_waves, bool ro, bool dl,
+                                                      DeviceCounters *ctr)
 {
     __shared__ uint32_t s_cnt[4];  // NewOrder commits, attempts, Payment commits, attempts
     if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
@@ -525,6 +526,12 @@ __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no
                 if (ro && commit && d.war[q] == epoch) {
                     const NoRows r = no_rows(no, q);
                     commit = !(r.cu < d.t[CU].rows && *cust_w(d, r.cu) < priority(tag, no.TID[q], q));
+                }
+                // check_3_1 (waw) reads NEWORDER W, which Deliveries lower
+                // without dooming anyone: decided here, against the register
+                if (dl && commit) {
+                    const NoRows r = no_rows(no, q);
+                    commit = !(r.no < d.t[NO].rows && d.regs[d.off_new + r.no] < priority(tag, no.TID[q], q));
                 }
             }
             if (commit) {
@@ -579,6 +586,132 @@ __global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no
             }
             atomicAdd(&ctr->totals[stripe][kAttempts + type], (unsigned long long)a);
         }
+    }
+}
+
+// ---- Deliveries ----------------------------------------------------------------
+
+// DELIVERYQUERY_NAMESPACE (TPCC-alltxn/Execute.h:1772-2153), one thread per
+// Delivery.  Its rows: NEWORDER NO_W_ID * 30000 + NO_O_ID (op_0_0) and
+// O_ID, without a warehouse offset, in ORDER, ORDERLINE and CUSTOMER
+// (op_1_0 .. op_5_0).
+struct DlRows {
+    uint64_t no, o;
+};
+
+__device__ __forceinline__ DlRows dl_rows(const DeliveryDev &dl, uint32_t i)
+{
+    return DlRows{(uint64_t)dl.NO_W_ID[i] * 30000 + dl.NO_O_ID[i], (uint64_t)dl.O_ID[i]};
+}
+
+// EXECUTE.  W on ORDER, ORDERLINE and CUSTOMER with early waw: the rows its
+// check_2_1, check_3_1 and check_5_1 consult, and the others' waw / raw on
+// them.  W on NEWORDER as a plain atomicMin: a NewOrder's check_3_1 (waw)
+// compares against it in CHECK, and the Delivery's own check_0_1 counts raw,
+// which never decides a Delivery (its only war, check_1_0, reads the ORDER W
+// register its check_2_1 waw reads).  The R registrations of op_1_0 and
+// op_4_0 are not kept: R of ORDER and ORDERLINE is consulted by a NewOrder's
+// war (check_4_0, check_7_0), but a Delivery writes both rows it reads, so a
+// NewOrder behind it on such a row has waw already; and by a Delivery's raw,
+// which decides nothing.  Runs after the NewOrders and Payments registered,
+// so no NewOrder ever displaces a Delivery's NEWORDER registration.
+__global__ __launch_bounds__(kBlock) void delivery_execute_kernel(TpccDev d, DeliveryDev dl, uint32_t base,
+                                                                 uint32_t tag, uint8_t epoch, DeviceCounters *ctr)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t bad = 0;
+    if (i < dl.n) {
+        const DlRows r = dl_rows(dl, i);
+        const unsigned long long me = priority(tag, dl.TID[i], base + i);
+        const bool nok = r.no < d.t[NO].rows, ook = r.o < d.t[OR].rows, lok = r.o < d.t[OL].rows,
+                   cok = r.o < d.t[CU].rows;
+        bad += !nok + 2 * !ook + 2 * !lok + !cok;  // slots 0, 1-2, 3-4, 5
+        unsigned long long *rn = d.regs + d.off_new + (nok ? r.no : 0);
+        unsigned long long *ro = d.regs + d.off_order + (ook ? r.o : 0);
+        unsigned long long *rl = d.regs + d.off_line + (lok ? r.o : 0);
+        unsigned long long *rc = cust_w(d, cok ? r.o : 0);
+        // every probe in flight before the first atomic
+        const unsigned long long cn = probe(rn), co = probe(ro), cl = probe(rl), cc = probe(rc);
+        if (nok && cn > me) atomicMin(rn, me);
+        bool waw = false;
+        if (ook) waw |= finish_write(ro, co, me, tag, d.doomed, epoch);
+        if (lok) waw |= finish_write(rl, cl, me, tag, d.doomed, epoch);
+        if (cok) waw |= finish_write(rc, cc, me, tag, d.doomed, epoch);
+        if (waw) d.doomed[base + i] = epoch;
+    }
+    if (bad) atomicAdd(&ctr->range_errors, bad);
+}
+
+// CHECK: a Delivery commits iff it has no waw (its war implies waw, see
+// above); committed ones read ORDER col 3 (op_1_1) and ORDERLINE col 3
+// (op_4_1) from the snapshot and enter the NEWORDER row's last-writer race.
+__global__ __launch_bounds__(kBlock) void delivery_check_kernel(TpccDev d, DeliveryDev dl, uint32_t base,
+                                                               uint8_t epoch, DeviceCounters *ctr)
+{
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < dl.n;
+    const bool commit = live && d.doomed[base + i] != epoch;
+    if (commit) {
+        const DlRows r = dl_rows(dl, i);
+        d.reads_dl[i] = r.o < d.t[OR].rows ? d.t[OR].data[r.o * kStride[OR] + 3] : 0u;
+        d.reads_dl[(uint64_t)dl.n + i] = r.o < d.t[OL].rows ? d.t[OL].data[r.o * kStride[OL] + 3] : 0u;
+        if (r.no < d.t[NO].rows)
+            atomicMax(d.new_last + r.no, ((unsigned long long)epoch << kTagShift) |
+                                             ((unsigned long long)dl.TID[i] << kTidShift) | (base + i));
+    }
+    const unsigned long long bits = __ballot(commit);
+    if (threadIdx.x % 64 == 0 && __ballot(live)) {
+        d.bitmap_dl[i / 64] = bits;
+        atomicAdd(&s_cnt, (uint32_t)__popcll(bits));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t first = blockIdx.x * kBlock;
+        const uint32_t a = dl.n - first < (uint32_t)kBlock ? dl.n - first : (uint32_t)kBlock;
+        const uint32_t stripe = blockIdx.x % kCounterStripes, c = s_cnt;
+        if (c) {
+            atomicAdd(&ctr->commits[stripe][LTPG_TYPE_DELIVERY], c);
+            atomicAdd(&ctr->totals[stripe][LTPG_TYPE_DELIVERY], (unsigned long long)c);
+        }
+        atomicAdd(&ctr->totals[stripe][kAttempts + LTPG_TYPE_DELIVERY], (unsigned long long)a);
+    }
+}
+
+// WRITEBACK (RMW values; the reference writes nothing back for Deliveries):
+// the NEWORDER row's last committed writer in priority order deletes it
+// (0, 0, 0); ORDER O_CARRIER_ID <- TID % 10 + 1; ORDERLINE OL_DELIVERY_D <-
+// TID; CUSTOMER C_BALANCE += OL_AMOUNT, C_DELIVERY_CNT += 1.  A committed
+// Delivery is the only committed writer of its ORDER, ORDERLINE and
+// CUSTOMER rows, so the OL_AMOUNT it adds is the snapshot's.
+__global__ __launch_bounds__(kBlock) void delivery_writeback_kernel(TpccDev d, DeliveryDev dl, uint32_t base,
+                                                                   uint8_t epoch)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= dl.n || !((d.bitmap_dl[i / 64] >> (i % 64)) & 1ull)) return;
+    const DlRows r = dl_rows(dl, i);
+    const uint32_t tid = dl.TID[i];
+    const unsigned long long last =
+        ((unsigned long long)epoch << kTagShift) | ((unsigned long long)tid << kTidShift) | (base + i);
+    if (r.no < d.t[NO].rows && d.new_last[r.no] == last) {
+        uint32_t *row = d.t[NO].data + r.no * kStride[NO];
+        row[0] = 0u;
+        row[1] = 0u;
+        row[2] = 0u;
+    }
+    if (r.o < d.t[OR].rows) d.t[OR].data[r.o * kStride[OR] + 5] = tid % 10u + 1u;
+    uint32_t amount = 0;
+    if (r.o < d.t[OL].rows) {
+        uint32_t *row = d.t[OL].data + r.o * kStride[OL];
+        amount = row[8];
+        row[6] = tid;
+    }
+    if (r.o < d.t[CU].rows) {
+        uint32_t *c = d.t[CU].data + r.o * kStride[CU];
+        c[5] += amount;
+        c[8] += 1u;
     }
 }
 
@@ -818,17 +951,44 @@ hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_
 }
 
 hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                             bool counts, bool ro, DeviceCounters *ctr, hipStream_t s)
+                             bool counts, bool ro, bool dl, DeviceCounters *ctr, hipStream_t s)
 {
     const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
     if (waves == 0) return hipSuccess;
     const uint32_t blocks = (waves + kWaves - 1) / kWaves;
     if (counts)
         hipLaunchKernelGGL(check_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ro, ctr);
+                           no_waves, ro, dl, ctr);
     else
         hipLaunchKernelGGL(check_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
-                           no_waves, ro, ctr);
+                           no_waves, ro, dl, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_delivery_execute(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                        DeviceCounters *ctr, hipStream_t s)
+{
+    if (dl.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(delivery_execute_kernel, dim3((dl.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, dl, base,
+                       e.tag, e.epoch, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_delivery_check(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                      DeviceCounters *ctr, hipStream_t s)
+{
+    if (dl.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(delivery_check_kernel, dim3((dl.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, dl, base,
+                       e.epoch, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_tpcc_delivery_writeback(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
+                                          uint32_t values, hipStream_t s)
+{
+    if (dl.n == 0 || values != LTPG_VALUES_RMW) return hipSuccess;  // copy-back changes nothing
+    hipLaunchKernelGGL(delivery_writeback_kernel, dim3((dl.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, dl, base,
+                       e.epoch);
     return hipGetLastError();
 }
 

@@ -94,6 +94,16 @@ typedef struct oracle_tpcc_ro_result {
     uint32_t os_commits, sl_commits;
 } oracle_tpcc_ro_result;
 
+/* Results of an epoch's Deliveries: bitmap, raw | war << 8 | waw << 16
+ * counters and read values, value-major (2 per Delivery: ORDER col 3,
+ * ORDERLINE col 3).  reads may be NULL. */
+typedef struct oracle_tpcc_dl_result {
+    uint64_t *bitmap;
+    uint32_t *conflicts;
+    uint32_t *reads;
+    uint32_t commits;
+} oracle_tpcc_dl_result;
+
 typedef struct oracle_tpcc oracle_tpcc;
 
 /* Rows and columns of table t (0..8: WAREHOUSE .. ITEM) at W warehouses. */
@@ -121,6 +131,16 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tables[9], uint32_t rul
                          const uint32_t *os_block, uint32_t os_cap, uint32_t n_os,
                          const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
                          oracle_tpcc_result *res, oracle_tpcc_ro_result *ro);
+/* The same epoch with Deliveries as well: blocks of 4 x dl_cap u32 (O_ID,
+ * NO_W_ID, NO_O_ID, TID).  ro may be NULL when n_os = n_sl = 0, dl when
+ * n_dl = 0. */
+int oracle_tpcc_epoch_all(oracle_tpcc *o, uint32_t *const tables[9], uint32_t rules, uint32_t values,
+                          const uint32_t *no_block, uint32_t no_cap, uint32_t n_no,
+                          const uint32_t *pay_block, uint32_t pay_cap, uint32_t n_pay,
+                          const uint32_t *os_block, uint32_t os_cap, uint32_t n_os,
+                          const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
+                          const uint32_t *dl_block, uint32_t dl_cap, uint32_t n_dl,
+                          oracle_tpcc_result *res, oracle_tpcc_ro_result *ro, oracle_tpcc_dl_result *dl);
 
 #ifdef __cplusplus
 }

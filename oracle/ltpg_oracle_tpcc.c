@@ -59,6 +59,30 @@
  *     STOCK col 2 from the snapshot.  Batch positions follow the NewOrders
  *     and Payments: Order-Status o at n_no + n_pay + o, Stock-Level k at
  *     n_no + n_pay + n_os + k.
+ *   - Deliveries (DELIVERYQUERY_NAMESPACE, TPCC-alltxn/Execute.h:1772-2153;
+ *     the reference declares DELIVERY_SET but never fills it).  Each
+ *     Delivery runs every op slot of its execute() chain (:1929-1977) and
+ *     every check slot its check() chain reaches (:2108-2149): the chain
+ *     tests wID % CHECK_WARP == 5 and == 6 twice, so check_4_0 and
+ *     check_5_0 never run.  (The reference also derives the transaction from
+ *     wID / 12 and wID / 8 but the slot from wID % EXECUTE_WARP and
+ *     wID % CHECK_WARP, so on its grid most Deliveries would reach only some
+ *     slots; that is launch geometry of an unfed stub, not a rule, and is
+ *     not reproduced -- as for Order-Status and Stock-Level.)  Rows keep the
+ *     reference's formulas, which omit the warehouse offset except for
+ *     NEWORDER: op_0_0 W on NEWORDER row NO_W_ID * 30000 + NO_O_ID; op_1_0
+ *     R and op_2_0 W on ORDER row O_ID; op_3_0 W and op_4_0 R on ORDERLINE
+ *     row O_ID; op_5_0 W on CUSTOMER row O_ID.  Checks: check_0_0 raw
+ *     (NEWORDER R), check_0_1 raw (NEWORDER W), check_1_0 war (ORDER W),
+ *     check_2_0 raw (ORDER R), check_2_1 waw (ORDER W), check_3_0 raw
+ *     (ORDERLINE R), check_3_1 waw (ORDERLINE W), check_5_1 waw (CUSTOMER
+ *     W).  The reference has no commit rule or write-back for them: Aria's
+ *     rule as for the other types; committed reads ORDER col 3 (op_1_1),
+ *     ORDERLINE col 3 (op_4_1); with values == 1, NEWORDER row <- (0, 0, 0)
+ *     when the Delivery is the row's last committed writer in priority order
+ *     (NewOrder inserters included), ORDER col 5 <- TID % 10 + 1, ORDERLINE
+ *     col 6 <- TID, CUSTOMER col 5 += ORDERLINE col 8 and col 8 += 1.
+ *     Positions: Delivery d at n_no + n_pay + n_os + n_sl + d.
  */
 #include "ltpg_oracle.h"
 
@@ -325,11 +349,30 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules
                          uint32_t n_os, const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
                          oracle_tpcc_result *res, oracle_tpcc_ro_result *ro)
 {
+    return oracle_tpcc_epoch_all(o, tbl, rules, values, no_block, no_cap, n_no, pay_block, pay_cap, n_pay, os_block,
+                                 os_cap, n_os, sl_block, sl_cap, n_sl, NULL, 0, 0, res, ro, NULL);
+}
+
+/* Delivery d's rows: NEWORDER (op_0_0, TPCC-alltxn/Execute.h:1780-1784) and
+ * the O_ID row of ORDER / ORDERLINE / CUSTOMER (op_1_0 .. op_5_0). */
+static uint64_t dl_new_row(const uint32_t *b, uint32_t cap, uint32_t d)
+{
+    return (uint64_t)b[(uint64_t)cap + d] * 30000 + b[2 * (uint64_t)cap + d];
+}
+
+int oracle_tpcc_epoch_all(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules, uint32_t values,
+                          const uint32_t *no_block, uint32_t no_cap, uint32_t n_no, const uint32_t *pay_block,
+                          uint32_t pay_cap, uint32_t n_pay, const uint32_t *os_block, uint32_t os_cap,
+                          uint32_t n_os, const uint32_t *sl_block, uint32_t sl_cap, uint32_t n_sl,
+                          const uint32_t *dl_block, uint32_t dl_cap, uint32_t n_dl, oracle_tpcc_result *res,
+                          oracle_tpcc_ro_result *ro, oracle_tpcc_dl_result *dl)
+{
     static const uint32_t os_tables[3] = {CU, OR, OL};
     static const uint32_t os_cols[3] = {2, 0, 0};
     const nview nv = nview_of(no_block, no_cap, n_no);
     const pview pv = pview_of(pay_block, pay_cap, n_pay);
-    const uint64_t os_base = (uint64_t)n_no + n_pay, sl_base = os_base + n_os;
+    const uint64_t os_base = (uint64_t)n_no + n_pay, sl_base = os_base + n_os, dl_base = sl_base + n_sl;
+    if (n_dl && !dl) return -1;
     uint32_t errors = 0;
     uint32_t *cid = res->pay_cid ? res->pay_cid : (uint32_t *)malloc(((size_t)n_pay + 1) * sizeof(uint32_t));
     if (!cid) return -1;
@@ -381,6 +424,17 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules
         const uint64_t me = ((uint64_t)sl_block[2 * (uint64_t)sl_cap + i] << 32) | (sl_base + i);
         const uint64_t st = sl_row(sl_block, sl_cap, i);
         if (ok(o, ST, st)) reg_min(o, ST, R, st, me); else errors++;
+    }
+
+    for (uint32_t d = 0; d < n_dl; d++) {  /* Delivery op_0_0 .. op_5_0 (:1774-1915) */
+        const uint64_t me = ((uint64_t)dl_block[3 * (uint64_t)dl_cap + d] << 32) | (dl_base + d);
+        const uint64_t nr = dl_new_row(dl_block, dl_cap, d), od = dl_block[d];
+        if (ok(o, NO, nr)) reg_min(o, NO, W, nr, me); else errors++;           /* op_0_0 */
+        if (ok(o, OR, od)) reg_min(o, OR, R, od, me); else errors++;           /* op_1_0 */
+        if (ok(o, OR, od)) reg_min(o, OR, W, od, me); else errors++;           /* op_2_0 */
+        if (ok(o, OL, od)) reg_min(o, OL, W, od, me); else errors++;           /* op_3_0 */
+        if (ok(o, OL, od)) reg_min(o, OL, R, od, me); else errors++;           /* op_4_0 */
+        if (ok(o, CU, od)) reg_min(o, CU, W, od, me); else errors++;           /* op_5_0 */
     }
 
     /* CHECK (check dispatch as it runs: NewOrder role 2 does nothing, role 3
@@ -476,6 +530,43 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules
             ro->sl_bitmap[i / 64] |= 1ull << (i % 64);
             ro->sl_commits++;
             if (ro->sl_reads) ro->sl_reads[i] = ok(o, ST, st) ? get(tbl, o, ST, 2, st) : 0;
+        }
+    }
+
+    /* Deliveries' check slots as the chain reaches them (:2108-2149) */
+    if (dl) {
+        memset(dl->bitmap, 0, ((size_t)n_dl + 63) / 64 * 8);
+        dl->commits = 0;
+        for (uint32_t d = 0; d < n_dl; d++) {
+            const uint64_t me = ((uint64_t)dl_block[3 * (uint64_t)dl_cap + d] << 32) | (dl_base + d);
+            const uint64_t nr = dl_new_row(dl_block, dl_cap, d), od = dl_block[d];
+            uint32_t raw = 0, war = 0, waw = 0;
+            if (ok(o, NO, nr)) {
+                raw += lower(o, NO, R, nr, me);                                 /* check_0_0 */
+                raw += lower(o, NO, W, nr, me);                                 /* check_0_1 */
+            }
+            if (ok(o, OR, od)) {
+                war += lower(o, OR, W, od, me);                                 /* check_1_0 */
+                raw += lower(o, OR, R, od, me);                                 /* check_2_0 */
+                waw += lower(o, OR, W, od, me);                                 /* check_2_1 */
+            }
+            if (ok(o, OL, od)) {
+                raw += lower(o, OL, R, od, me);                                 /* check_3_0 */
+                waw += lower(o, OL, W, od, me);                                 /* check_3_1 */
+            }
+            /* check_4_0 (war, ORDERLINE W) and check_5_0 (raw, CUSTOMER R)
+             * sit behind the repeated == 5 / == 6 tests and never run */
+            if (ok(o, CU, od)) waw += lower(o, CU, W, od, me);                  /* check_5_1 */
+            dl->conflicts[d] = (raw & 0xffu) | ((war & 0xffu) << 8) | ((waw & 0xffu) << 16);
+            if (waw == 0 && (raw == 0 || war == 0)) {
+                dl->bitmap[d / 64] |= 1ull << (d % 64);
+                dl->commits++;
+                if (ok(o, NO, nr) && me + 1 > o->nwin[nr]) o->nwin[nr] = me + 1;
+                if (dl->reads) {
+                    dl->reads[d] = ok(o, OR, od) ? get(tbl, o, OR, 3, od) : 0;                  /* op_1_1 */
+                    dl->reads[(uint64_t)n_dl + d] = ok(o, OL, od) ? get(tbl, o, OL, 3, od) : 0; /* op_4_1 */
+                }
+            }
         }
     }
 
@@ -575,6 +666,23 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules
             if (ok(o, WH, wh)) *at(tbl, o, WH, 2, wh) += amt;
             if (ok(o, DI, di)) *at(tbl, o, DI, 3, di) += amt;
         }
+        /* Deliveries: a committed Delivery is the only committed writer of
+         * its ORDER, ORDERLINE and CUSTOMER rows (waw), so the ORDERLINE
+         * amount it adds is the snapshot's */
+        for (uint32_t d = 0; d < n_dl; d++) {
+            if (!((dl->bitmap[d / 64] >> (d % 64)) & 1ull)) continue;
+            const uint32_t tid = dl_block[3 * (uint64_t)dl_cap + d];
+            const uint64_t me = ((uint64_t)tid << 32) | (dl_base + d);
+            const uint64_t nr = dl_new_row(dl_block, dl_cap, d), od = dl_block[d];
+            if (ok(o, NO, nr) && o->nwin[nr] == me + 1)
+                for (uint32_t c = 0; c < 3; c++) *at(tbl, o, NO, c, nr) = 0;
+            if (ok(o, OR, od)) *at(tbl, o, OR, 5, od) = tid % 10 + 1;
+            if (ok(o, OL, od)) *at(tbl, o, OL, 6, od) = tid;
+            if (ok(o, CU, od)) {
+                *at(tbl, o, CU, 5, od) += ok(o, OL, od) ? get(tbl, o, OL, 8, od) : 0;
+                *at(tbl, o, CU, 8, od) += 1;
+            }
+        }
     }
 
     /* clear_LOG (Database.h:78-102) for the registers this epoch touched */
@@ -587,6 +695,10 @@ int oracle_tpcc_epoch_ro(oracle_tpcc *o, uint32_t *const tbl[NT], uint32_t rules
     for (uint32_t q = 0; q < n_no; q++) {
         const nhead h = nrows(&nv, q);
         if (ok(o, NO, h.no)) o->nwin[h.no] = 0;
+    }
+    for (uint32_t d = 0; d < n_dl; d++) {
+        const uint64_t nr = dl_new_row(dl_block, dl_cap, d);
+        if (ok(o, NO, nr)) o->nwin[nr] = 0;
     }
     res->range_errors = errors;
     if (!res->pay_cid) free(cid);

@@ -76,6 +76,15 @@ def _load(path=None):
                                              ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                              ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TpccResult),
                                              ctypes.POINTER(TpccRoResult)]
+    class TpccDlResult(ctypes.Structure):
+        _fields_ = [("bitmap", u64p), ("conflicts", u32p), ("reads", u32p), ("commits", ctypes.c_uint32)]
+
+    lib.TpccDlResult = TpccDlResult
+    if hasattr(lib, "oracle_tpcc_epoch_all"):
+        lib.oracle_tpcc_epoch_all.restype = ctypes.c_int
+        lib.oracle_tpcc_epoch_all.argtypes = (lib.oracle_tpcc_epoch_ro.argtypes[:-2] +
+                                              [u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TpccResult),
+                                               ctypes.POINTER(TpccRoResult), ctypes.POINTER(TpccDlResult)])
     return lib
 
 
@@ -206,13 +215,14 @@ class TpccOracle:
     def __del__(self):
         self.close()
 
-    def epoch(self, tables, batch, rules=0, values=1, ro=None):
+    def epoch(self, tables, batch, rules=0, values=1, ro=None, dl=None):
         """One epoch over `tables` (list of 9 column-major u32 arrays, updated
         in place) for `batch` (an object with no/no_cap/n_no/pay/pay_cap/n_pay:
         the packed NEWORDER_QUERY / PAYMENT_QUERY blocks) and, optionally,
         read-only transactions `ro` (an object with n_os / n_sl and field
-        arrays os[name] / sl[name]); their results come back in
-        TpccEpoch.ro."""
+        arrays os[name] / sl[name]) and Deliveries `dl` (an object with n and
+        arrays O_ID, NO_W_ID, NO_O_ID, TID); their results come back in
+        TpccEpoch.ro and TpccEpoch.dl."""
         for t in tables:
             assert t.dtype == np.uint32 and t.flags.c_contiguous
         n_no, n_pay = int(batch.n_no), int(batch.n_pay)
@@ -227,10 +237,27 @@ class TpccOracle:
         res = lib.TpccResult(_u64p(nb), _u64p(pb), _u32p(nc), _u32p(pc), _u32p(nr), _u32p(pr), _u32p(cid), 0, 0, 0)
         no = np.ascontiguousarray(batch.no, dtype=np.uint32)
         pay = np.ascontiguousarray(batch.pay, dtype=np.uint32)
-        if ro is None:
+        dl_out = None
+        if dl is not None:
+            n_dl = int(dl.n)
+            dlb = np.ascontiguousarray(np.stack([np.asarray(getattr(dl, f), np.uint32)[:max(n_dl, 1)]
+                                                 if n_dl else np.zeros(1, np.uint32)
+                                                 for f in ("O_ID", "NO_W_ID", "NO_O_ID", "TID")]), np.uint32)
+            db = np.zeros(max(1, (n_dl + 63) // 64), np.uint64)
+            dc, dr = np.zeros(max(1, n_dl), np.uint32), np.zeros(max(1, 2 * n_dl), np.uint32)
+            dres = lib.TpccDlResult(_u64p(db), _u32p(dc), _u32p(dr), 0)
+        if ro is None and dl is None:
             rc = lib.oracle_tpcc_epoch(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
                                         _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
                                         ctypes.byref(res))
+            ro_out = None
+        elif ro is None:
+            z = np.zeros(3, np.uint32)
+            rres = lib.TpccRoResult()
+            rc = lib.oracle_tpcc_epoch_all(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
+                                           _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
+                                           _u32p(z), 1, 0, _u32p(z), 1, 0, _u32p(dlb), max(n_dl, 1), n_dl,
+                                           ctypes.byref(res), ctypes.byref(rres), ctypes.byref(dres))
             ro_out = None
         else:
             n_os, n_sl = int(ro.n_os), int(ro.n_sl)
@@ -243,15 +270,25 @@ class TpccOracle:
             oc, sc = np.zeros(max(1, n_os), np.uint32), np.zeros(max(1, n_sl), np.uint32)
             orr, srr = np.zeros(max(1, 3 * n_os), np.uint32), np.zeros(max(1, n_sl), np.uint32)
             rres = lib.TpccRoResult(_u64p(ob), _u64p(sb), _u32p(oc), _u32p(sc), _u32p(orr), _u32p(srr), 0, 0)
-            rc = lib.oracle_tpcc_epoch_ro(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
-                                           _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
-                                           _u32p(osb), max(n_os, 1), n_os, _u32p(slb), max(n_sl, 1), n_sl,
-                                           ctypes.byref(res), ctypes.byref(rres))
+            if dl is None:
+                rc = lib.oracle_tpcc_epoch_ro(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
+                                               _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
+                                               _u32p(osb), max(n_os, 1), n_os, _u32p(slb), max(n_sl, 1), n_sl,
+                                               ctypes.byref(res), ctypes.byref(rres))
+            else:
+                rc = lib.oracle_tpcc_epoch_all(self.h, lib.tables_t(*[_u32p(t) for t in tables]), rules, values,
+                                                _u32p(no), batch.no_cap, n_no, _u32p(pay), batch.pay_cap, n_pay,
+                                                _u32p(osb), max(n_os, 1), n_os, _u32p(slb), max(n_sl, 1), n_sl,
+                                                _u32p(dlb), max(n_dl, 1), n_dl, ctypes.byref(res),
+                                                ctypes.byref(rres), ctypes.byref(dres))
             ro_out = dict(os_bitmap=ob[:(n_os + 63) // 64], sl_bitmap=sb[:(n_sl + 63) // 64],
                           os_conflicts=oc[:n_os], sl_conflicts=sc[:n_sl], os_reads=orr[:3 * n_os],
                           sl_reads=srr[:n_sl], os_commits=rres.os_commits, sl_commits=rres.sl_commits)
         assert rc == 0
-        return TpccEpoch(no_bitmap=nb[:(n_no + 63) // 64], pay_bitmap=pb[:(n_pay + 63) // 64],
+        if dl is not None:
+            dl_out = dict(bitmap=db[:(n_dl + 63) // 64], conflicts=dc[:n_dl], reads=dr[:2 * n_dl],
+                          commits=dres.commits)
+        return TpccEpoch(dl=dl_out, no_bitmap=nb[:(n_no + 63) // 64], pay_bitmap=pb[:(n_pay + 63) // 64],
                          no_conflicts=nc[:n_no], pay_conflicts=pc[:n_pay], no_reads=nr[:51 * n_no],
                          pay_reads=pr[:24 * n_pay], pay_cid=cid[:n_pay], no_commits=res.no_commits,
                          pay_commits=res.pay_commits, range_errors=res.range_errors, ro=ro_out)

@@ -41,7 +41,7 @@ def test_ctypes_signatures_cover_the_header():
 
 
 def test_abi_version_and_config_defaults():
-    assert L.lib.ltpg_abi_version() == 1
+    assert L.lib.ltpg_abi_version() == 2
     cfg = L.default_config()
     assert cfg.size == ctypes.sizeof(L.Config)
     assert cfg.kv_hot_row == L.NO_HOT_ROW
@@ -52,6 +52,7 @@ def test_abi_version_and_config_defaults():
 C_STRUCTS = {"Config": "ltpg_config", "KvBatch": "ltpg_kv_batch", "KvGen": "ltpg_kv_gen",
              "NeworderBatch": "ltpg_neworder_batch", "PaymentBatch": "ltpg_payment_batch",
              "OrderstatusBatch": "ltpg_orderstatus_batch", "StocklevelBatch": "ltpg_stocklevel_batch",
+             "DeliveryBatch": "ltpg_delivery_batch",
              "TpccGenConfig": "ltpg_tpcc_gen_config", "BatchDesc": "ltpg_batch_desc",
              "TuningEntry": "ltpg_tuning_entry", "ServeConfig": "ltpg_serve_config", "ServeDone": "ltpg_serve_done",
              "ServeStats": "ltpg_serve_stats", "ServeDeviceStats": "ltpg_serve_device_stats"}

@@ -317,3 +317,15 @@ def test_autotuned_graph_matches_untuned_plain(gpu):
             np.testing.assert_array_equal(x, y)
     for x, y in zip(_tables(kv0, tp0), _tables(kv1, tp1)):
         np.testing.assert_array_equal(x, y)
+
+
+def test_sync_costs_are_measured(gpu):
+    """The grid-barrier and kernel-boundary costs behind the choice of
+    graphs over a persistent kernel: a cooperative launch of co-resident
+    workgroups passes its barriers (the poll limit is never reached), and
+    every cost is a positive number of microseconds."""
+    kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=1024, kv_fields=1, kv_field_bytes=4, max_txns=64, kv_max_slots=1)
+    for blocks in (64, 256):
+        c = L.sync_costs(kv, blocks, 200)
+        assert all(0.0 < x < 1000.0 for x in c.values()), c
+    kv.close()

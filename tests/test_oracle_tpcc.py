@@ -61,7 +61,7 @@ def _kat_readonly(case):
 
 
 def _check_kat(case, run):
-    """run(tables, batch, rules, values, ro) -> TpccEpoch; compares every
+    """run(tables, batch, rules, values, ro, dl) -> TpccEpoch; compares every
     expectation of the case, return False on the first mismatch."""
     W = case["W"]
     tables = _tables(W)
@@ -69,9 +69,10 @@ def _check_kat(case, run):
     initial = [t.copy() for t in tables]
     b = _kat_batch(case)
     ro = _kat_readonly(case)
+    dl = tc.kat_delivery(case)
     rules = 0 if case["rules"] == "reference" else 1
     values = 1 if case["values"] == "rmw" else 0
-    ep = run(tables, b, rules, values, ro)
+    ep = run(tables, b, rules, values, ro, dl)
     exp = case["expect"]
     ok = (tc.bits(ep.no_bitmap, b.n_no) == exp["no_commit"] and tc.bits(ep.pay_bitmap, b.n_pay) == exp["pay_commit"]
           and tc.unpack(ep.no_conflicts).tolist() == exp["no_conf"]
@@ -96,13 +97,19 @@ def _check_kat(case, run):
             ok = ok and int(r["os_reads"][v * ro.n_os + i]) == val
         for i, v, val in exp["sl_reads"]:
             ok = ok and int(r["sl_reads"][v * ro.n_sl + i]) == val
+    if dl is not None:
+        x = ep.dl
+        ok = ok and tc.bits(x["bitmap"], dl.n) == exp["dl_commit"] and x["commits"] == sum(exp["dl_commit"])
+        ok = ok and tc.unpack(x["conflicts"]).tolist() == exp["dl_conf"]
+        for i, v, val in exp["dl_reads"]:
+            ok = ok and int(x["reads"][v * dl.n + i]) == val
     return ok
 
 
 @pytest.mark.parametrize("case", KAT["cases"], ids=[c["name"] for c in KAT["cases"]])
 def test_oracle_known_answers(case):
     ora = oracle.TpccOracle(case["W"])
-    assert _check_kat(case, lambda tables, b, r, v, ro: ora.epoch(tables, b, r, v, ro=ro))
+    assert _check_kat(case, lambda tables, b, r, v, ro, dl: ora.epoch(tables, b, r, v, ro=ro, dl=dl))
 
 
 def test_loader_contents():
@@ -295,6 +302,112 @@ def test_oracle_readonly_bruteforce_tid_orders():
     assert checked == 480 and aborts > 0
 
 
+def _random_delivery(rng, W, n, tid_pool):
+    """Deliveries whose rows collide with _random_batch's: O_ID below 15
+    reaches its orders (0..4), order lines (0..14) and customers (0..3) of
+    warehouse 0, NO_O_ID below 5 its NEWORDER rows."""
+    dl = L.DeliveryBatch(n)
+    tids = rng.permutation(tid_pool)[:n]
+    for i in range(n):
+        dl.O_ID[i] = rng.integers(0, 15) if rng.random() < 0.9 else rng.integers(0, 40000)
+        dl.NO_W_ID[i], dl.NO_O_ID[i] = rng.integers(0, W + (rng.random() < 0.1)), rng.integers(0, 5)
+        dl.TID[i] = tids[i]
+    return dl
+
+
+def _pins_hold_dl(W, tables0, b, ro, dl, ep, rules, values=1):
+    """_pins_hold_ro with Deliveries: every type's counters against the
+    pairwise count from the dispatch tables, Aria's rule for every type, and
+    the serial execution of the committed set with every read value."""
+    no_c, pay_c, bad, os_c, sl_c, dl_c = tc.pairwise_conflicts(b, W, ep.pay_cid, rules, ro, dl)
+    got = {"no": tc.unpack(ep.no_conflicts), "pay": tc.unpack(ep.pay_conflicts), "dl": tc.unpack(ep.dl["conflicts"])}
+    want = {"no": no_c, "pay": pay_c, "dl": dl_c}
+    if ro is not None:
+        got.update(os=tc.unpack(ep.ro["os_conflicts"]), sl=tc.unpack(ep.ro["sl_conflicts"]))
+        want.update(os=os_c, sl=sl_c)
+    for name in got:
+        if not np.array_equal(got[name], want[name]):
+            return False, f"{name} conflict counters differ from the pairwise count"
+    if bad != ep.range_errors:
+        return False, "range errors"
+    commits = {"no": tc.bits(ep.no_bitmap, b.n_no), "pay": tc.bits(ep.pay_bitmap, b.n_pay),
+               "dl": tc.bits(ep.dl["bitmap"], dl.n)}
+    if ro is not None:
+        commits.update(os=tc.bits(ep.ro["os_bitmap"], ro.n_os), sl=tc.bits(ep.ro["sl_bitmap"], ro.n_sl))
+    for name in commits:
+        if commits[name] != tc.aria(want[name]).tolist():
+            return False, f"{name} commit is not waw == 0 && (raw == 0 || war == 0)"
+    ok, why, final, reads = tc.serial_check(
+        W, tables0, b, ep.pay_cid, commits["no"], commits["pay"], rmw=values == 1, ro=ro,
+        ro_commit={"os": commits["os"], "sl": commits["sl"]} if ro is not None else None, dl=dl,
+        dl_commit=commits["dl"])
+    if not ok:
+        return False, why
+    flats = {"no": (ep.no_reads, b.n_no, 51), "pay": (ep.pay_reads, b.n_pay, 24), "dl": (ep.dl["reads"], dl.n, 2)}
+    if ro is not None:
+        flats.update(os=(ep.ro["os_reads"], ro.n_os, 3), sl=(ep.ro["sl_reads"], ro.n_sl, 1))
+    for (typ, i), vals in reads.items():
+        flat, n, nv = flats[typ]
+        if [int(flat[v * n + i]) for v in range(nv)] != vals:
+            return False, f"read values of {typ} {i}"
+    return True, final
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_oracle_delivery_pairwise_and_serializable(seed):
+    """Deliveries mixed into colliding batches (with read-only transactions
+    in half of them): every type's counters match the pairwise count from
+    the dispatch tables (check_4_0 and check_5_0 never counted), every type
+    follows Aria's rule, the committed set executes serially to the oracle's
+    tables and read values."""
+    rng = np.random.default_rng(900 + seed)
+    W = 1 + seed % 2
+    n_no, n_pay, n_dl = int(rng.integers(0, 7)), int(rng.integers(0, 7)), int(rng.integers(1, 7))
+    b = _random_batch(rng, W, n_no, n_pay)
+    pool = np.arange(0, 2 * (n_no + n_pay) + 40, 2)
+    ro = _random_readonly(rng, W, int(rng.integers(0, 4)), int(rng.integers(0, 4)), pool) if seed % 2 else None
+    dl = _random_delivery(rng, W, n_dl, pool + 1)
+    tables0 = _tables(W)
+    tables = [t.copy() for t in tables0]
+    values = 0 if seed % 7 == 3 else 1
+    ep = oracle.TpccOracle(W).epoch(tables, b, (seed // 2) % 2, values, ro=ro, dl=dl)
+    ok, res = _pins_hold_dl(W, tables0, b, ro, dl, ep, (seed // 2) % 2, values)
+    assert ok, res
+    for t in range(9):
+        np.testing.assert_array_equal(tables[t], res[t])
+
+
+def test_oracle_delivery_bruteforce_tid_orders():
+    """Every TID order of small batches (2 NewOrders, 1 Payment, 2
+    Deliveries on their rows), both rule sets."""
+    rng = np.random.default_rng(93)
+    W = 1
+    tables0 = _tables(W)
+    ora = oracle.TpccOracle(W)
+    checked = 0
+    outcomes = set()
+    for pattern in range(4):
+        base = _random_batch(rng, W, 2, 1)
+        dl0 = _random_delivery(rng, W, 2, np.arange(100, 110))
+        # one Delivery on the first NewOrder's order and NEWORDER row, the
+        # other on the Payment's customer
+        dl0.O_ID[0] = base.nf("O_ID")[0] + 30000 * int(base.nf("W_ID")[0])
+        dl0.NO_W_ID[0], dl0.NO_O_ID[0] = base.nf("W_ID")[0], base.nf("N_O_ID")[0]
+        dl0.O_ID[1] = int(base.pf("C_W_ID")[0]) * 30000 + int(base.pf("C_D_ID")[0]) * 3000 + int(base.pf("C_ID")[0])
+        for perm in itertools.permutations(range(5)):
+            b = L.TpccBatch(base.no.copy(), base.no_cap, base.n_no, base.pay.copy(), base.pay_cap, base.n_pay)
+            b.nf("TID")[:] = np.array(perm[:2], dtype=np.uint32) * 5 + 3
+            b.pf("TID")[:] = np.array(perm[2:3], dtype=np.uint32) * 5 + 3
+            dl0.TID[:2] = np.array(perm[3:5], dtype=np.uint32) * 5 + 3
+            tables = [t.copy() for t in tables0]
+            ep = ora.epoch(tables, b, pattern % 2, 1, dl=dl0)
+            ok, res = _pins_hold_dl(W, tables0, b, None, dl0, ep, pattern % 2)
+            assert ok, res
+            outcomes.add(tuple(tc.bits(ep.dl["bitmap"], 2)) + tuple(tc.bits(ep.no_bitmap, 2)))
+            checked += 1
+    assert checked == 480 and len(outcomes) > 2
+
+
 def test_oracle_bruteforce_all_tid_orders():
     """Every TID order of small colliding batches (<= 6 transactions)."""
     rng = np.random.default_rng(77)
@@ -382,6 +495,18 @@ MISTAKES = {
     # an order line beyond the first not applied
     "first_line_only": (r"for \(uint32_t i = 0; i < nlines\(&nv, q\); i\+\+\) \{\n                const uint64_t ir = item_row\(&nv, q, i\), sr = stock_row\(&nv, q, i\), lr = line_row\(&nv, q, i\);\n                const uint32_t qty",
                         "for (uint32_t i = 0; i < 1 && i < nlines(&nv, q); i++) {\n                const uint64_t ir = item_row(&nv, q, i), sr = stock_row(&nv, q, i), lr = line_row(&nv, q, i);\n                const uint32_t qty"),
+    # Delivery: check_4_0 made reachable (war from ORDERLINE W)
+    "delivery_check_4_0": (r"/\* check_4_0 \(war, ORDERLINE W\) and check_5_0 \(raw, CUSTOMER R\)",
+                           "if (ok(o, OL, od)) war += lower(o, OL, W, od, me);\n            /* check_4_0 (war, ORDERLINE W) and check_5_0 (raw, CUSTOMER R)"),
+    # Delivery: the CUSTOMER waw (check_5_1) dropped
+    "delivery_no_customer_waw": (r"if \(ok\(o, CU, od\)\) waw \+= lower\(o, CU, W, od, me\);",
+                                 "if (ok(o, CU, od)) raw += 0;"),
+    # Delivery: NEWORDER row with the warehouse offset omitted
+    "delivery_neworder_row": (r"return \(uint64_t\)b\[\(uint64_t\)cap \+ d\] \* 30000 \+ b\[2 \* \(uint64_t\)cap \+ d\];",
+                              "return (uint64_t)b[2 * (uint64_t)cap + d];"),
+    # Delivery: reads ORDERLINE col 8 instead of col 3 (op_4_1)
+    "delivery_read_column": (r"get\(tbl, o, OL, 3, od\) : 0; /\* op_4_1 \*/",
+                             "get(tbl, o, OL, 8, od) : 0; /* op_4_1 */"),
 }
 
 
@@ -409,7 +534,7 @@ def test_pins_catch_planted_mistakes(tmp_path, mistake):
     caught = False
     for case in KAT["cases"]:
         ora = oracle.TpccOracle(case["W"], lib=lib)
-        if not _check_kat(case, lambda tables, b, r, v, ro: ora.epoch(tables, b, r, v, ro=ro)):
+        if not _check_kat(case, lambda tables, b, r, v, ro, dl: ora.epoch(tables, b, r, v, ro=ro, dl=dl)):
             caught = True
             break
     rng = np.random.default_rng(99)
@@ -418,8 +543,10 @@ def test_pins_catch_planted_mistakes(tmp_path, mistake):
         b = _random_batch(rng, W, int(rng.integers(1, 8)), int(rng.integers(1, 8)))
         tables0 = _tables(W)
         tables = [t.copy() for t in tables0]
-        ep = oracle.TpccOracle(W, lib=lib).epoch(tables, b, i % 2, 1)
-        ok, res = _pins_hold(W, tables0, b, ep, i % 2)
+        dl = _random_delivery(rng, W, int(rng.integers(1, 5)), np.arange(1, 60, 2)) if i % 3 == 2 else None
+        ep = oracle.TpccOracle(W, lib=lib).epoch(tables, b, i % 2, 1, dl=dl)
+        ok, res = _pins_hold(W, tables0, b, ep, i % 2) if dl is None else \
+            _pins_hold_dl(W, tables0, b, None, dl, ep, i % 2)
         if not ok or not all(np.array_equal(tables[t], res[t]) for t in range(9)):
             caught = True
             break

@@ -332,8 +332,9 @@ def _kat_readonly(case):
 KAT = tc.load_kat()
 
 
+# cases with Deliveries run in tests/test_delivery_gpu.py
 _KAT_PARAMS = [pytest.param(c, counts, id=f"{c['name']}-{'counting' if counts else 'fast'}")
-               for c in KAT["cases"]
+               for c in KAT["cases"] if "delivery" not in c
                for counts in ((False,) if "orderstatus" in c or "stocklevel" in c else (True, False))]
 
 

@@ -56,6 +56,17 @@ def make_batch(neworders, payments):
     return b
 
 
+def kat_delivery(case):
+    """DeliveryBatch of a case's 'delivery' list, or None."""
+    if "delivery" not in case:
+        return None
+    dl = L.DeliveryBatch(len(case["delivery"]))
+    for i, t in enumerate(case["delivery"]):
+        for f in L.DELIVERY_FIELDS:
+            getattr(dl, f)[i] = t.get(f, 0)
+    return dl
+
+
 def load_kat():
     with open(os.path.join(GOLDEN, "tpcc_kat.json")) as f:
         return json.load(f)
@@ -151,7 +162,35 @@ def readonly_ops(ro, typ, i, W):
     return regs, checks, bad
 
 
-def all_ops(b, W, cid, rules, ro=None):
+def delivery_rows(dl, d):
+    """(NEWORDER row, O_ID) of Delivery d: op_0_0 (TPCC-alltxn/Execute.h:
+    1780-1784) and the warehouse-less O_ID row of op_1_0 .. op_5_0."""
+    return int(dl.NO_W_ID[d]) * 30000 + int(dl.NO_O_ID[d]), int(dl.O_ID[d])
+
+
+def delivery_ops(dl, d, W):
+    """(registrations, checks, range errors) of Delivery d, slot by slot as
+    its execute() chain registers (:1929-1977) and its check() chain reaches
+    (:2108-2149; check_4_0 and check_5_0 sit behind repeated == 5 / == 6
+    tests and never run)."""
+    nr, o = delivery_rows(dl, d)
+    slots = [(NO, nr, "W", [("R", "raw"), ("W", "raw")]),     # op_0_0 / check_0_0, check_0_1
+             (OR, o, "R", [("W", "war")]),                     # op_1_0 / check_1_0
+             (OR, o, "W", [("R", "raw"), ("W", "waw")]),       # op_2_0 / check_2_0, check_2_1
+             (OL, o, "W", [("R", "raw"), ("W", "waw")]),       # op_3_0 / check_3_0, check_3_1
+             (OL, o, "R", []),                                 # op_4_0 / (check_4_0 unreachable)
+             (CU, o, "W", [("W", "waw")])]                     # op_5_0 / check_5_1 (check_5_0 unreachable)
+    regs, checks, bad = [], [], 0
+    for t, row, kind, cks in slots:
+        if row >= rows_of(W, t):
+            bad += 1
+            continue
+        regs.append((t, row, kind))
+        checks += [(t, row, k, ctr) for k, ctr in cks]
+    return regs, checks, bad
+
+
+def all_ops(b, W, cid, rules, ro=None, dl=None):
     """[(priority, type, index, regs, checks)] and the range-error count."""
     out, bad = [], 0
     for q in range(b.n_no):
@@ -169,26 +208,36 @@ def all_ops(b, W, cid, rules, ro=None):
                 r, c, e = readonly_ops(ro, typ, i, W)
                 out.append(((int(tids[i]), off + i), typ, i, r, c))
                 bad += e
+    if dl is not None:
+        base = b.n_no + b.n_pay + (ro.n_os + ro.n_sl if ro is not None else 0)
+        for d in range(dl.n):
+            r, c, e = delivery_ops(dl, d, W)
+            out.append(((int(dl.TID[d]), base + d), "dl", d, r, c))
+            bad += e
     return out, bad
 
 
-def pairwise_conflicts(b, W, cid, rules, ro=None):
+def pairwise_conflicts(b, W, cid, rules, ro=None, dl=None):
     """raw/war/waw per transaction by pairwise comparison: a check slot of t
     on (table, row, kind) counts iff some transaction of higher priority
     registered `kind` on that row.  With read-only transactions `ro` also
-    returns their counters: (no, pay, bad, os, sl)."""
-    txns, bad = all_ops(b, W, cid, rules, ro)
+    returns their counters: (no, pay, bad, os, sl); with Deliveries `dl`
+    (no, pay, bad, os, sl, dl)."""
+    txns, bad = all_ops(b, W, cid, rules, ro, dl)
     holders = {}
     for prio, _, _, regs, _ in txns:
         for t, row, kind in regs:
             holders.setdefault((t, row, kind), []).append(prio)
     conf = {"no": np.zeros((b.n_no, 3), np.uint32), "pay": np.zeros((b.n_pay, 3), np.uint32),
-            "os": np.zeros((ro.n_os if ro else 0, 3), np.uint32), "sl": np.zeros((ro.n_sl if ro else 0, 3), np.uint32)}
+            "os": np.zeros((ro.n_os if ro else 0, 3), np.uint32), "sl": np.zeros((ro.n_sl if ro else 0, 3), np.uint32),
+            "dl": np.zeros((dl.n if dl is not None else 0, 3), np.uint32)}
     col = {"raw": 0, "war": 1, "waw": 2}
     for prio, typ, i, _, checks in txns:
         for t, row, kind, ctr in checks:
             if any(h < prio for h in holders.get((t, row, kind), [])):
                 conf[typ][i, col[ctr]] += 1
+    if dl is not None:
+        return conf["no"], conf["pay"], bad, conf["os"], conf["sl"], conf["dl"]
     if ro is None:
         return conf["no"], conf["pay"], bad
     return conf["no"], conf["pay"], bad, conf["os"], conf["sl"]
@@ -301,7 +350,29 @@ def _payment_effects(tb, b, p, cid, rmw):
     return reads
 
 
-def _rw_sets(b, W, cid, ro=None):
+def _delivery_effects(tb, dl, d, rmw):
+    """Execute Delivery d on tb: returns its read values (ORDER col 3,
+    ORDERLINE col 3).  Writers of one NEWORDER row run in priority order,
+    so the last of them leaves the row."""
+    nr, o = delivery_rows(dl, d)
+    reads = [tb.get(OR, o, 3) if tb.ok(OR, o) else 0, tb.get(OL, o, 3) if tb.ok(OL, o) else 0]
+    if not rmw:
+        return reads
+    tid = int(dl.TID[d])
+    if tb.ok(NO, nr):
+        for k in range(3):
+            tb.set(NO, nr, k, 0)
+    if tb.ok(OR, o):
+        tb.set(OR, o, 5, tid % 10 + 1)
+    if tb.ok(OL, o):
+        tb.set(OL, o, 6, tid)
+    if tb.ok(CU, o):
+        tb.set(CU, o, 5, tb.get(CU, o, 5) + (tb.get(OL, o, 8) if tb.ok(OL, o) else 0))
+        tb.set(CU, o, 8, tb.get(CU, o, 8) + 1)
+    return reads
+
+
+def _rw_sets(b, W, cid, ro=None, dl=None):
     """Row-level read and write sets of every transaction (the YTD columns
     are commutative adds and do not order transactions)."""
     out = []
@@ -326,19 +397,29 @@ def _rw_sets(b, W, cid, ro=None):
         rd = {(CU, cu)}
         wr = {(CU, cu), (HI, w * 30000 + int(b.pf("H_ID")[p]))}
         out.append(((int(b.pf("TID")[p]), b.n_no + p), "pay", p, rd, wr))
+    if dl is not None:
+        base = b.n_no + b.n_pay + (ro.n_os + ro.n_sl if ro is not None else 0)
+        for d in range(dl.n):
+            nr, o = delivery_rows(dl, d)
+            out.append(((int(dl.TID[d]), base + d), "dl", d, {(OR, o), (OL, o), (CU, o)},
+                        {(NO, nr), (OR, o), (OL, o), (CU, o)}))
     return out
 
 
-def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True, ro=None, ro_commit=None):
+def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True, ro=None, ro_commit=None, dl=None,
+                 dl_commit=None):
     """(ok, message, final tables, reads) of a serial execution of the
     committed transactions in an order that honours every precedence
     constraint; reads are {('no', q): [..51], ('pay', p): [..24],
-    ('os', o): [..3], ('sl', k): [..1]}.  ro_commit: {'os': [...], 'sl':
-    [...]} commit lists of the read-only transactions `ro`."""
+    ('os', o): [..3], ('sl', k): [..1], ('dl', d): [..2]}.  ro_commit:
+    {'os': [...], 'sl': [...]} commit lists of the read-only transactions
+    `ro`; dl_commit the commit list of the Deliveries `dl`."""
     commits = {"no": no_commit, "pay": pay_commit}
     if ro is not None:
         commits.update(ro_commit)
-    txns = [t for t in _rw_sets(b, W, cid, ro) if commits[t[1]][t[2]]]
+    if dl is not None:
+        commits["dl"] = dl_commit
+    txns = [t for t in _rw_sets(b, W, cid, ro, dl) if commits[t[1]][t[2]]]
     n = len(txns)
     succ = [set() for _ in range(n)]
     indeg = [0] * n
@@ -376,6 +457,8 @@ def serial_check(W, tables0, b, cid, no_commit, pay_commit, rmw=True, ro=None, r
             reads[(typ, i)] = _neworder_effects(tb, b, i, rmw)
         elif typ == "pay":
             reads[(typ, i)] = _payment_effects(tb, b, i, cid[i], rmw)
+        elif typ == "dl":
+            reads[(typ, i)] = _delivery_effects(tb, dl, i, rmw)
         else:
             reads[(typ, i)] = [tb.get(t, r, c) if tb.ok(t, r) else 0 for t, r, c in readonly_rows(ro, typ, i)]
     return True, "", tb.a, reads
