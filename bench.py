@@ -124,14 +124,24 @@ def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw, rows=None, max_scan=0):
     return {"execute": execute, "check": check, "writeback": writeback}, committed
 
 
+def shape_flags(k):
+    """LTPG_TUNE_* flags of launch shape k (0-31): k & 15, with the
+    row-partitioned EXECUTE for k >= 16."""
+    import ltpg_amd as L
+
+    return (k & 15) | (L.TUNE_PARTITIONED if k >= 16 else 0)
+
+
 def choose_kv_shape(ctx, batch, reps=3):
-    """Time the 16 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
+    """Time the 32 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
     block-local row folding, split register arrays, 64-transaction chunks,
-    per-lane field copies) on one batch, one warm-up and `reps` epochs each,
-    and keep the fastest for the run.  The timed loop is the plain per-batch
-    loop either way."""
+    per-lane field copies, each with and without the row-partitioned
+    EXECUTE) on one batch, one warm-up and `reps` epochs each, and keep the
+    fastest for the run.  The timed loop is the plain per-batch loop either
+    way."""
     times = []
-    for flags in range(16):
+    for k in range(32):
+        flags = shape_flags(k)
         ctx.kv_set_tuning(flags)
         ctx.kv_submit_device(batch)
         ctx.epoch_run()
@@ -142,7 +152,7 @@ def choose_kv_shape(ctx, batch, reps=3):
             ctx.epoch_run()
         ctx.sync()
         times.append((time.perf_counter() - t0) * 1e3 / reps)
-    best = int(np.argmin(times))
+    best = shape_flags(int(np.argmin(times)))
     ctx.kv_set_tuning(best)
     return best, times
 
@@ -376,7 +386,8 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
                    "batch_txns": B, "ops_per_txn": S, "zipf_theta": w["theta"], "form": "plain per-batch",
                    "inputs": "resident in HBM", "distinct_batches": P,
                    "launch_shape": {"flags": shape, "epoch_ms_by_flags": shape_ms,
-                                    "chosen_by": "fastest of the 16 LTPG_TUNE_* shapes (by flags) on the first batch, "
+                                    "chosen_by": "fastest of the 32 LTPG_TUNE_* shapes (index k: flags k & 15, | PARTITIONED for k >= 16) "
+                                                 "on the first batch, "
                                                  "before the timed region"}},
         "abort_rate": 1.0 - commits / max(attempts, 1),
         "attempted_txns": attempts,

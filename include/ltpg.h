@@ -466,6 +466,15 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   instead of four or five; other epochs run as before.  Not one of the
  *   autotuner's combinations (LTPG_TUNE_ALL).  The served form runs its
  *   small key-value epochs this way whatever the context's flags.
+ * PARTITIONED: EXECUTE of the fast form (no LTPG_FLAG_CONFLICT_COUNTS) bins
+ *   the epoch's register operations by row range (8192 rows per bucket) and
+ *   keeps each range's W minima in LDS, instead of probing and updating the
+ *   per-row registers in HBM at random (EXEC_PER_OP then has no effect).
+ *   Two streaming passes over the ops and one over the binned operations
+ *   (12 bytes each); the autotuner tries it with every other combination.
+ *   Needs kv_rows <= 8192 * 16384 and kv_max_slots * max_txns *
+ *   max(1, kv_max_scan) < 2^32 (else LTPG_ERR_UNSUPPORTED); the first
+ *   selection allocates its buffers, 12 bytes per possible operation.
  * Results are identical for every combination; may change between any two
  * epochs. */
 #define LTPG_TUNE_EXEC_PER_OP 1u
@@ -474,6 +483,7 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
 #define LTPG_TUNE_LANE_COPY 8u
 #define LTPG_TUNE_ALL 15u
 #define LTPG_TUNE_SMALL_EPOCH 16u
+#define LTPG_TUNE_PARTITIONED 32u
 #define LTPG_SMALL_EPOCH_OPS 2048u
 int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags);
 
@@ -587,10 +597,16 @@ typedef struct ltpg_tuning_entry {
     uint32_t log2_txns;        /* the size class                                    */
     uint32_t flags;            /* chosen LTPG_TUNE_* combination                    */
     uint32_t reserved;
-    float ms[16];              /* epoch time of every combination, by flags         */
+    float ms[32];              /* epoch time of every combination: [flags] for flags
+                                  <= LTPG_TUNE_ALL, [16 + flags] for flags |
+                                  LTPG_TUNE_PARTITIONED                           */
 } ltpg_tuning_entry;
 int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
                           uint32_t *n_out);
+/* One launch shape (LTPG_TUNE_* flags, LTPG_TUNE_SMALL_EPOCH excluded) for
+ * every key-value batch of the runner, in place of the autotuner's choice;
+ * captures the graph again. */
+int ltpg_batched_set_tuning(ltpg_batched *b, uint32_t flags);
 
 /* ---- served form: a request loop over resident tables (replaces the
  *      producer/consumer threads around the tp loop, TPCC-allopt-final/

@@ -59,6 +59,7 @@ NEWORDER_WORDS, PAYMENT_WORDS = 53, 10
 RUN_PLAIN, RUN_GRAPH = 0, 1
 TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_LANE_COPY, TUNE_ALL = 1, 2, 4, 8, 15
 TUNE_SMALL_EPOCH, SMALL_EPOCH_OPS = 16, 2048
+TUNE_PARTITIONED = 32
 NOALLOC_MAX_ROWS = NOALLOC_MAX_TXNS = 1 << 20
 
 
@@ -697,6 +698,10 @@ class Batched:
         n = ctypes.c_uint32()
         _check(lib.ltpg_batched_autotune(self.handle, reps, out, 33, ctypes.byref(n)), "ltpg_batched_autotune")
         return [{"log2_txns": e.log2_txns, "flags": e.flags, "ms": list(e.ms)} for e in out[:n.value]]
+
+    def set_tuning(self, flags):
+        """One TUNE_* launch shape for every key-value batch (recaptures)."""
+        _check(lib.ltpg_batched_set_tuning(self.handle, flags), "ltpg_batched_set_tuning")
 
     def bitmap(self, i, type_):
         d = self.descs[i]

@@ -128,7 +128,7 @@ class TuningEntry(ctypes.Structure):
     """struct ltpg_tuning_entry."""
 
     _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 16)]
+                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 32)]
 
 
 class ServeConfig(ctypes.Structure):
@@ -233,6 +233,7 @@ SIGNATURES = [
     ("ltpg_batched_bitmap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_u64p, ctypes.c_uint64]),
     ("ltpg_batched_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ltpg_batched_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]),
+    ("ltpg_batched_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_batched_autotune", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(TuningEntry), ctypes.c_uint32, c_u32p]),
     ("ltpg_serve_config_init", None, [ctypes.POINTER(ServeConfig)]),

@@ -41,6 +41,12 @@ void free_kv(KvState &kv)
         dev_free(kv.tids_buf[k]);
     }
     dev_free(kv.hot_acc);
+    dev_free(kv.part.count);
+    dev_free(kv.part.start);
+    dev_free(kv.part.cursor);
+    dev_free(kv.part.slices);
+    dev_free(kv.part.ent_me);
+    dev_free(kv.part.ent_key);
 }
 
 int init_kv(ltpg_ctx *ctx)
@@ -194,6 +200,41 @@ int init_tpcc(ltpg_ctx *ctx)
 }
 
 }  // namespace
+
+// Buffers of the row-partitioned EXECUTE, allocated on its first selection
+// (outside any graph capture: the batched form prepares them before it
+// captures).
+int kv_part_prepare(ltpg_ctx *ctx)
+{
+    KvState &kv = ctx->kv;
+    if (kv.part.ent_me) return LTPG_OK;
+    const uint64_t nb = (kv.rows + (1ull << kPartShift) - 1) >> kPartShift;
+    const uint64_t cap = (uint64_t)kv.max_slots * ctx->cfg.max_txns * kv.read_slot;
+    if (nb > kPartMaxBuckets || cap >= (1ull << 32))
+        LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "row-partitioned EXECUTE: too many rows or operations per epoch");
+    KvState::Part p;
+    p.n_buckets = (uint32_t)nb;
+    p.cap_entries = cap;
+    p.cap_slices = (uint32_t)(nb + cap / kPartSlice + 1);
+    int rc;
+    if ((rc = dev_alloc(&p.count, nb, "partition counts")) || (rc = dev_alloc(&p.start, nb + 1, "partition starts")) ||
+        (rc = dev_alloc(&p.cursor, nb, "partition cursors")) ||
+        (rc = dev_alloc(&p.slices, 3ull * p.cap_slices + 1, "partition slices")) ||
+        (rc = dev_alloc(&p.ent_me, cap, "partitioned operations")) ||
+        (rc = dev_alloc(&p.ent_key, cap, "partitioned operation rows"))) {
+        dev_free(p.count);
+        dev_free(p.start);
+        dev_free(p.cursor);
+        dev_free(p.slices);
+        dev_free(p.ent_me);
+        dev_free(p.ent_key);
+        return rc;
+    }
+    // the bucket sizes start at zero; the binning leaves them at zero
+    LTPG_HIP_TRY(launch_fill_u8(reinterpret_cast<uint8_t *>(p.count), nb * 4, 0, ctx->stream));
+    kv.part = p;
+    return LTPG_OK;
+}
 
 int refill_registers(ltpg_ctx *ctx)
 {
@@ -767,7 +808,10 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_SMALL_EPOCH)) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_SMALL_EPOCH | LTPG_TUNE_PARTITIONED))
+        LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    if ((flags & LTPG_TUNE_PARTITIONED) && (rc = kv_part_prepare(ctx))) return rc;
+    ctx->kv.partitioned = (flags & LTPG_TUNE_PARTITIONED) ? 1 : 0;
     ctx->kv.exec_per_op = (flags & LTPG_TUNE_EXEC_PER_OP) ? 1 : 0;
     ctx->kv.split_regs = (flags & LTPG_TUNE_SPLIT_REGISTERS) ? 1 : 0;
     ctx->kv.small_chunks = (flags & LTPG_TUNE_SMALL_CHUNKS) ? 1 : 0;

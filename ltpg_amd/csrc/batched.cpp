@@ -391,8 +391,14 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     for (int c = 0; c < 33 && !rc; c++) {
         if (rep[c] < 0) continue;
         const ltpg_batch_desc &d = b->d[rep[c]];
-        float ms[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t cfg = 0; cfg <= LTPG_TUNE_ALL && !rc; cfg++) {
+        float ms[32] = {};
+        bool tried[32] = {};
+        // every combination, then each with the row-partitioned EXECUTE
+        // (skipped where the table or the epochs are too large for it)
+        const bool part_ok = kv_part_prepare(ctx) == LTPG_OK;
+        for (uint32_t k = 0; k < 32 && !rc; k++) {
+            if (k >= 16 && !part_ok) continue;
+            const uint32_t cfg = (k & LTPG_TUNE_ALL) | (k >= 16 ? LTPG_TUNE_PARTITIONED : 0u);
             rc = ltpg_kv_set_tuning(ctx, cfg);
             for (uint32_t k = 0; k <= reps && !rc; k++) {
                 if (k == 1) rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? LTPG_OK : LTPG_ERR_DEVICE;
@@ -403,13 +409,14 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
             if (!rc) rc = ltpg_sync(ctx);
             float t = 0;
             if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = LTPG_ERR_DEVICE;
-            ms[cfg] = t / (float)reps;
+            ms[k] = t / (float)reps;
+            tried[k] = true;
         }
         if (rc) break;
         uint32_t best = 0;
-        for (uint32_t cfg = 1; cfg <= LTPG_TUNE_ALL; cfg++)
-            if (ms[cfg] < ms[best]) best = cfg;
-        const uint8_t t = (uint8_t)best;
+        for (uint32_t k = 1; k < 32; k++)
+            if (tried[k] && ms[k] < ms[best]) best = k;
+        const uint8_t t = (uint8_t)((best & LTPG_TUNE_ALL) | (best >= 16 ? LTPG_TUNE_PARTITIONED : 0u));
         for (uint32_t i = 0; i < b->d.size(); i++)
             if (b->d[i].workload == LTPG_WORKLOAD_KV && b->d[i].kv.n_txns &&
                 31 - __builtin_clz(b->d[i].kv.n_txns) == c)
@@ -418,9 +425,9 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
             ltpg_tuning_entry &o = out[n];
             o.workload = LTPG_WORKLOAD_KV;
             o.log2_txns = (uint32_t)c;
-            o.flags = best;
+            o.flags = t;
             o.reserved = 0;
-            for (int k = 0; k < 16; k++) o.ms[k] = ms[k];
+            for (int k = 0; k < 32; k++) o.ms[k] = ms[k];
         }
         n++;
     }
@@ -431,6 +438,25 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
     if (rc) return rc;
     if (n_out) *n_out = n < max_out ? n : max_out;
     // capture the key-value sequence again with the chosen shapes
+    if (b->exec[0]) (void)hipGraphExecDestroy(b->exec[0]);
+    if (b->graph[0]) (void)hipGraphDestroy(b->graph[0]);
+    b->exec[0] = nullptr;
+    b->graph[0] = nullptr;
+    return capture(b, 0);
+}
+
+int ltpg_batched_set_tuning(ltpg_batched *b, uint32_t flags)
+{
+    if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null runner");
+    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_PARTITIONED)) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    ltpg_ctx *ctx = b->ctx[0];
+    if (!ctx) return LTPG_OK;  // no key-value batches
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    // validates the flags and prepares their buffers outside any capture
+    if ((rc = ltpg_kv_set_tuning(ctx, flags))) return rc;
+    for (uint32_t i = 0; i < b->d.size(); i++)
+        if (b->d[i].workload == LTPG_WORKLOAD_KV) b->tune[i] = (uint8_t)flags;
     if (b->exec[0]) (void)hipGraphExecDestroy(b->exec[0]);
     if (b->graph[0]) (void)hipGraphDestroy(b->graph[0]);
     b->exec[0] = nullptr;

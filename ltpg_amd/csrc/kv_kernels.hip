@@ -545,6 +545,306 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     if (bad) atomicAdd(&ctr->range_errors, bad);
 }
 
+// ---- row-partitioned EXECUTE (LTPG_TUNE_PARTITIONED, fast form) -----------
+//
+// The register phase touches one random register line per operation; the
+// lines of a 10M-row table overflow the L2s, so each 8-byte probe or atomic
+// moves a 128-byte line.  Here the epoch's register operations (every write
+// op, every row a read or scan op reads; hot-row ops never register) are
+// first binned by row range into buckets of 2^kPartShift rows:
+//   count    streaming pass over the ops: per-bucket sizes (LDS histogram,
+//            one global add per bucket and block), range errors;
+//   scan     one workgroup: bucket starts, cursors, and slices of at most
+//            kPartSlice operations (a skewed bucket gets several);
+//   scatter  second pass over the ops: each operation's (priority, row in
+//            bucket, is-write) to its bucket, a block reserving one run per
+//            bucket;
+//   process  one workgroup per slice: the W minima of the bucket's rows in
+//            LDS (probe, then a 64-bit LDS atomicMin), then the flags of the
+//            fast form from them -- a writer behind the minimum is doomed; a
+//            reader behind it has raw; a reader ahead of it gives the minimum
+//            writer war.  A bucket split over several slices folds each
+//            slice's minima into the global W registers instead;
+//   flags    one workgroup per slice of such a bucket: the same flags from
+//            the global W registers.
+// The flags equal those of kv_execute_kernel + kv_mark_reads_kernel: both
+// compare every operation with the minimum W priority of its row.
+constexpr uint32_t kPartBlock = 1024;
+constexpr uint32_t kPartPer = 16;  // operations per thread in the binning passes
+constexpr uint32_t kPartChunk = kPartBlock * kPartPer;
+constexpr uint32_t kPartRows = 1u << kPartShift;
+
+struct PartArgs {
+    const uint64_t *keys;
+    const uint32_t *meta;
+    const uint32_t *tids;
+    uint32_t B, S;
+    uint64_t rows;
+    uint32_t fields, max_scan;
+    uint64_t hot;
+    uint32_t nb, cap_slices;
+    uint32_t *count, *start, *cursor, *slices;
+    unsigned long long *ent_me;
+    uint32_t *ent_key;
+    unsigned long long *regs;
+    RegLayout rl;
+    uint32_t tag;
+    uint8_t epoch;
+    bool rm;
+    uint8_t *doomed, *raw_flag, *war_flag;
+    DeviceCounters *ctr;
+};
+
+// kPartGroup consecutive op slots of a thread (stride kPartBlock), all
+// loads issued before any is used.
+constexpr uint32_t kPartGroup = 4;
+
+template <uint32_t G>
+__device__ __forceinline__ void part_load(const PartArgs &a, uint64_t first, Op (&o)[G])
+{
+    const uint64_t n_ops = (uint64_t)a.S * a.B;
+    uint64_t r[G];
+    uint32_t m[G];
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) {
+        const uint64_t i = first + (uint64_t)u * kPartBlock;
+        const uint64_t j = i < n_ops ? i : 0ull;
+        r[u] = a.keys[j];
+        m[u] = a.meta[j];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) {
+        const bool live = first + (uint64_t)u * kPartBlock < n_ops;
+        o[u] = decode_op(live ? r[u] : 0ull, live ? m[u] : 0u, a.rows, a.fields, a.max_scan);
+    }
+}
+
+__global__ __launch_bounds__(kPartBlock) void kv_part_count_kernel(PartArgs a, bool zero_counts)
+{
+    __shared__ uint32_t s_hist[kPartMaxBuckets];
+    if (zero_counts && blockIdx.x == 0)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kPartBlock)
+            a.ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
+    for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) s_hist[b] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kPartChunk + threadIdx.x;
+    uint32_t bad = 0;
+#pragma unroll 1
+    for (uint32_t g = 0; g < kPartPer; g += kPartGroup) {
+        Op o[kPartGroup];
+        part_load(a, base + (uint64_t)g * kPartBlock, o);
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u) {
+            bad += (o[u].kind != LTPG_KV_NONE && !o[u].valid) ? 1u : 0u;
+            if (!o[u].valid) continue;
+            if (o[u].kind == LTPG_KV_WRITE) {
+                if (o[u].row != a.hot) atomicAdd(&s_hist[o[u].row >> kPartShift], 1u);
+            } else {
+                for (uint32_t j = 0; j < o[u].len; ++j)
+                    if (o[u].row + j != a.hot) atomicAdd(&s_hist[(o[u].row + j) >> kPartShift], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) {
+        const uint32_t c = s_hist[b];
+        if (c) atomicAdd(&a.count[b], c);
+    }
+    if (bad) atomicAdd(&a.ctr->range_errors, bad);
+}
+
+// Exclusive scan of the bucket sizes into starts, cursors and slices (one
+// workgroup; each thread owns a run of consecutive buckets).  Clears the
+// sizes for the next epoch.
+__global__ __launch_bounds__(kPartBlock) void kv_part_scan_kernel(PartArgs a)
+{
+    __shared__ uint32_t s_n[kPartBlock], s_sl[kPartBlock];
+    const uint32_t per = (a.nb + kPartBlock - 1) / kPartBlock;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t n = 0, ns = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t b = b0 + j;
+        if (b < a.nb) {
+            const uint32_t c = a.count[b];
+            n += c;
+            ns += (c + kPartSlice - 1) / kPartSlice;
+        }
+    }
+    s_n[threadIdx.x] = n;
+    s_sl[threadIdx.x] = ns;
+    __syncthreads();
+    // inclusive scans (Hillis-Steele over the block)
+    for (uint32_t d = 1; d < kPartBlock; d <<= 1) {
+        const uint32_t x = threadIdx.x >= d ? s_n[threadIdx.x - d] : 0u;
+        const uint32_t y = threadIdx.x >= d ? s_sl[threadIdx.x - d] : 0u;
+        __syncthreads();
+        s_n[threadIdx.x] += x;
+        s_sl[threadIdx.x] += y;
+        __syncthreads();
+    }
+    uint32_t at = s_n[threadIdx.x] - n, sl = s_sl[threadIdx.x] - ns;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t b = b0 + j;
+        if (b >= a.nb) break;
+        const uint32_t c = a.count[b];
+        a.start[b] = at;
+        a.cursor[b] = at;
+        for (uint32_t k = 0; k * kPartSlice < c; ++k, ++sl) {
+            if (sl >= a.cap_slices) continue;
+            a.slices[3 * sl] = b;
+            a.slices[3 * sl + 1] = at + k * kPartSlice;
+            a.slices[3 * sl + 2] = at + min(c, (k + 1) * kPartSlice);
+        }
+        at += c;
+        a.count[b] = 0;
+    }
+    if (threadIdx.x == kPartBlock - 1) {
+        a.start[a.nb] = s_n[kPartBlock - 1];
+        a.slices[3 * a.cap_slices] = min(s_sl[kPartBlock - 1], a.cap_slices);
+    }
+}
+
+__global__ __launch_bounds__(kPartBlock) void kv_part_scatter_kernel(PartArgs a)
+{
+    __shared__ uint32_t s_at[kPartMaxBuckets];
+    for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) s_at[b] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kPartChunk + threadIdx.x;
+#pragma unroll 1
+    for (uint32_t g = 0; g < kPartPer; g += kPartGroup) {
+        Op o[kPartGroup];
+        part_load(a, base + (uint64_t)g * kPartBlock, o);
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u) {
+            if (!o[u].valid) continue;
+            if (o[u].kind == LTPG_KV_WRITE) {
+                if (o[u].row != a.hot) atomicAdd(&s_at[o[u].row >> kPartShift], 1u);
+            } else {
+                for (uint32_t j = 0; j < o[u].len; ++j)
+                    if (o[u].row + j != a.hot) atomicAdd(&s_at[(o[u].row + j) >> kPartShift], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    // one run per bucket: the block's count becomes its run's start
+    for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) {
+        const uint32_t c = s_at[b];
+        if (c) s_at[b] = atomicAdd(&a.cursor[b], c);
+    }
+    __syncthreads();
+    const uint64_t n_ops = (uint64_t)a.S * a.B;
+    constexpr uint32_t G = kPartGroup;
+#pragma unroll 1
+    for (uint32_t g = 0; g < kPartPer; g += G) {
+        Op o[G];
+        unsigned long long me[G];
+        part_load(a, base + (uint64_t)g * kPartBlock, o);
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) {
+            // op indices stay below 2^32 (kv_part_prepare): 32-bit division
+            const uint64_t i = base + (uint64_t)(g + u) * kPartBlock;
+            const uint32_t i32 = i < n_ops ? (uint32_t)i : 0u;
+            const uint32_t t = a.rm ? i32 / a.S : i32 % a.B;
+            me[u] = priority(a.tag, a.tids[t], t);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) {
+            if (!o[u].valid) continue;
+            if (o[u].kind == LTPG_KV_WRITE) {
+                if (o[u].row == a.hot) continue;
+                const uint32_t at = atomicAdd(&s_at[o[u].row >> kPartShift], 1u);
+                a.ent_me[at] = me[u];
+                a.ent_key[at] = ((uint32_t)(o[u].row & (kPartRows - 1)) << 1) | 1u;
+            } else {
+                for (uint32_t j = 0; j < o[u].len; ++j) {
+                    const uint64_t r = o[u].row + j;
+                    if (r == a.hot) continue;
+                    const uint32_t at = atomicAdd(&s_at[r >> kPartShift], 1u);
+                    a.ent_me[at] = me[u];
+                    a.ent_key[at] = (uint32_t)(r & (kPartRows - 1)) << 1;
+                }
+            }
+        }
+    }
+}
+
+// The fast form's flags of one operation against its row's minimum W
+// priority w (~0: no writer this epoch).
+__device__ __forceinline__ void part_flag(const PartArgs &a, uint32_t key, unsigned long long me,
+                                          unsigned long long w)
+{
+    const uint32_t pos = (uint32_t)(me & kPosMask);
+    if (key & 1u) {
+        if (w < me) a.doomed[pos] = a.epoch;
+    } else if (w != ~0ull) {
+        if (w < me)
+            a.raw_flag[pos] = a.epoch;
+        else if (w > me)
+            a.war_flag[w & kPosMask] = a.epoch;
+    }
+}
+
+__global__ __launch_bounds__(kPartBlock) void kv_part_process_kernel(PartArgs a)
+{
+    __shared__ unsigned long long s_w[kPartRows];
+    if (blockIdx.x >= a.slices[3 * a.cap_slices]) return;
+    const uint32_t b = a.slices[3 * blockIdx.x], beg = a.slices[3 * blockIdx.x + 1], end = a.slices[3 * blockIdx.x + 2];
+    const bool multi = a.start[b + 1] - a.start[b] > kPartSlice;
+    for (uint32_t r = threadIdx.x; r < kPartRows; r += kPartBlock) s_w[r] = ~0ull;
+    __syncthreads();
+    for (uint32_t e0 = beg + threadIdx.x; e0 < end; e0 += kPartGroup * kPartBlock) {
+        uint32_t key[kPartGroup];
+        unsigned long long me[kPartGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u) {
+            const uint32_t e = e0 + u * kPartBlock;
+            key[u] = e < end ? a.ent_key[e] : 0u;
+            me[u] = e < end ? a.ent_me[e] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u)
+            if ((key[u] & 1u) && s_w[key[u] >> 1] > me[u]) atomicMin(&s_w[key[u] >> 1], me[u]);
+    }
+    __syncthreads();
+    if (multi) {
+        const uint64_t row0 = (uint64_t)b << kPartShift;
+        for (uint32_t r = threadIdx.x; r < kPartRows; r += kPartBlock) {
+            const unsigned long long v = s_w[r];
+            if (v == ~0ull) continue;
+            unsigned long long *p = &a.regs[a.rl.w(row0 + r)];
+            if (probe(p) > v) atomicMin(p, v);
+        }
+        return;
+    }
+    for (uint32_t e0 = beg + threadIdx.x; e0 < end; e0 += kPartGroup * kPartBlock) {
+        uint32_t key[kPartGroup];
+        unsigned long long me[kPartGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u) {
+            const uint32_t e = e0 + u * kPartBlock;
+            key[u] = e < end ? a.ent_key[e] : 0u;
+            me[u] = e < end ? a.ent_me[e] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u)
+            if (e0 + u * kPartBlock < end) part_flag(a, key[u], me[u], s_w[key[u] >> 1]);
+    }
+}
+
+__global__ __launch_bounds__(kPartBlock) void kv_part_flags_kernel(PartArgs a)
+{
+    if (blockIdx.x >= a.slices[3 * a.cap_slices]) return;
+    const uint32_t b = a.slices[3 * blockIdx.x], beg = a.slices[3 * blockIdx.x + 1], end = a.slices[3 * blockIdx.x + 2];
+    if (a.start[b + 1] - a.start[b] <= kPartSlice) return;  // decided in kv_part_process_kernel
+    const uint64_t row0 = (uint64_t)b << kPartShift;
+    for (uint32_t e = beg + threadIdx.x; e < end; e += kPartBlock) {
+        const uint32_t key = a.ent_key[e];
+        unsigned long long w = a.regs[a.rl.w(row0 + (key >> 1))];
+        if ((uint32_t)(w >> kTagShift) != a.tag) w = ~0ull;  // no writer this epoch
+        part_flag(a, key, a.ent_me[e], w);
+    }
+}
+
 // Read marking (fast form), one lane per op after every write has
 // registered.  A read of row x by t against w = W[x] of this epoch:
 //   w < me: t reads a row a higher-priority transaction writes -> raw(t);
@@ -2096,6 +2396,46 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
     if (b.n_txns == 0) return zero_counts ? launch_reset_epoch_counters(ctr, s) : hipSuccess;
     const uint64_t ops = (uint64_t)b.n_txns * b.n_slots;
     const dim3 op_grid((unsigned)((ops + kBlock - 1) / kBlock));
+    if (kv.partitioned && !counts && kv.part.ent_me) {
+        PartArgs a;
+        a.keys = b.keys;
+        a.meta = b.meta;
+        a.tids = b.tids;
+        a.B = b.n_txns;
+        a.S = b.n_slots;
+        a.rows = kv.rows;
+        a.fields = kv.fields;
+        a.max_scan = kv.max_scan;
+        a.hot = kv.hot_row;
+        a.nb = kv.part.n_buckets;
+        a.cap_slices = kv.part.cap_slices;
+        a.count = kv.part.count;
+        a.start = kv.part.start;
+        a.cursor = kv.part.cursor;
+        a.slices = kv.part.slices;
+        a.ent_me = kv.part.ent_me;
+        a.ent_key = kv.part.ent_key;
+        a.regs = kv.regs;
+        a.rl = reg_layout(kv);
+        a.tag = e.tag;
+        a.epoch = e.epoch;
+        a.rm = b.request_major;
+        a.doomed = kv.doomed;
+        a.raw_flag = kv.raw_flag;
+        a.war_flag = kv.war_flag;
+        a.ctr = ctr;
+        const dim3 chunks((unsigned)((ops + kPartChunk - 1) / kPartChunk));
+        // slices: one per non-empty bucket, plus one per kPartSlice operations
+        const uint64_t entries = ops * (kv.max_scan > 1 ? kv.max_scan : 1);
+        uint64_t slices = a.nb + entries / kPartSlice + 1;
+        if (slices > kv.part.cap_slices) slices = kv.part.cap_slices;
+        hipLaunchKernelGGL(kv_part_count_kernel, chunks, dim3(kPartBlock), 0, s, a, zero_counts);
+        hipLaunchKernelGGL(kv_part_scan_kernel, dim3(1), dim3(kPartBlock), 0, s, a);
+        hipLaunchKernelGGL(kv_part_scatter_kernel, chunks, dim3(kPartBlock), 0, s, a);
+        hipLaunchKernelGGL(kv_part_process_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
+        hipLaunchKernelGGL(kv_part_flags_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
+        return hipGetLastError();
+    }
     if (kv.exec_per_op && !counts) {
         hipLaunchKernelGGL(kv_execute_dedup_kernel, dim3((unsigned)((ops + kDedupOps - 1) / kDedupOps)), dim3(kBlock),
                            0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,

@@ -97,6 +97,21 @@ struct KvState {
     uint32_t small_chunks = 0;
     uint32_t lane_copy = 0;
     uint32_t small_epoch = 0;           // epochs of <= LTPG_SMALL_EPOCH_OPS op slots in one kernel
+    // row-partitioned EXECUTE (fast form): the epoch's register operations
+    // binned by row range, each range's W minima kept in LDS; buffers
+    // allocated when the shape is first selected (kv_part_prepare)
+    uint32_t partitioned = 0;
+    struct Part {
+        uint32_t n_buckets = 0;
+        uint64_t cap_entries = 0;             // register operations an epoch may bin
+        uint32_t cap_slices = 0;
+        uint32_t *count = nullptr;            // [n_buckets] operations per bucket (zero between epochs)
+        uint32_t *start = nullptr;            // [n_buckets + 1] first entry of each bucket
+        uint32_t *cursor = nullptr;           // [n_buckets] next free entry while binning
+        uint32_t *slices = nullptr;           // [3 * cap_slices + 1] (bucket, begin, end), then their number
+        unsigned long long *ent_me = nullptr; // [cap_entries] priority of each binned operation
+        uint32_t *ent_key = nullptr;          // [cap_entries] row within its bucket << 1 | is-write
+    } part;
     // no kernel has written regs / doomed / raw_flag / war_flag since the
     // last refill ran (the next epoch-tag wrap may skip the refill)
     bool regs_clean = false;
@@ -285,6 +300,7 @@ hipError_t launch_stamp(unsigned long long *out, hipStream_t s);
 // inserters to 0, on the context's stream (api.cpp).  Epoch-tag wrap and the
 // start of a captured batch sequence use it.
 int refill_registers(ltpg_ctx *ctx);
+int kv_part_prepare(ltpg_ctx *ctx);
 hipError_t launch_kv_fill_reference(const KvState &kv, hipStream_t s);
 // rows idx[0 .. n-1] (device, each < kv.rows) of every field, packed, to out (device)
 hipError_t launch_kv_gather_rows(const KvState &kv, const uint64_t *idx, uint64_t n, uint32_t *out, hipStream_t s);
@@ -292,6 +308,11 @@ hipError_t launch_repack(const uint32_t *src, uint32_t *dst, uint64_t n_fields, 
                          uint32_t fsw, int to_strided, hipStream_t s);
 hipError_t launch_kv_transpose(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                int to_row_major, hipStream_t s);
+// Row-partitioned EXECUTE limits: rows per bucket (1 << kPartShift), buckets,
+// and operations per slice (one workgroup).
+constexpr uint32_t kPartShift = 13;
+constexpr uint32_t kPartMaxBuckets = 16384;
+constexpr uint32_t kPartSlice = 32768;
 hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,
                              DeviceCounters *ctr, hipStream_t s, bool zero_counts = true);
 hipError_t launch_kv_check(const KvState &kv, const KvBatchDev &b, const EpochTag &e, bool counts,

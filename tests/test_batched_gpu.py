@@ -266,7 +266,7 @@ def test_empty_batches_change_nothing(gpu, mode):
 @pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("fw", [2, 25], ids=["8B", "100B"])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
-@pytest.mark.parametrize("flags", range(16))
+@pytest.mark.parametrize("flags", list(range(16)) + [L.TUNE_PARTITIONED | f for f in (0, 2, 4, 8, 14)])
 def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw, rm):
     """Every launch shape against the oracle, batches in either layout;
     100-byte fields (stride 32 words) take the per-lane copy under
@@ -309,7 +309,7 @@ def test_autotuned_graph_matches_untuned_plain(gpu):
     descs1, keep1, _ = _batches(np.random.default_rng(11), 30, kv1, tp1)
     tuned = L.Batched(kv1, tp1, descs1)
     table = tuned.autotune(reps=2)
-    assert table and all(len(e["ms"]) == 16 and min(e["ms"]) > 0 for e in table)
+    assert table and all(len(e["ms"]) == 32 and min(e["ms"]) > 0 for e in table)
     tuned.run(L.RUN_GRAPH)
     got = _bitmaps(tuned, meta)
     for a, b in zip(want, got):
@@ -329,3 +329,75 @@ def test_sync_costs_are_measured(gpu):
         c = L.sync_costs(kv, blocks, 200)
         assert all(0.0 < x < 1000.0 for x in c.values()), c
     kv.close()
+
+
+@pytest.mark.parametrize("rows", [5000, 200000], ids=["one-bucket", "25-buckets"])
+@pytest.mark.parametrize("theta", [0.6, 0.99])
+@pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
+@pytest.mark.parametrize("scan", [0, 4], ids=["point", "scans"])
+def test_kv_partitioned_execute_matches_oracle(gpu, rows, theta, hot_row, scan):
+    """The row-partitioned EXECUTE against the oracle: one bucket split over
+    several slices (global W folding) and many single-slice buckets (LDS
+    only), skew that overloads one bucket, the hot row, scans across bucket
+    edges, and epochs one after the other on the same registers."""
+    from oracle import oracle
+    from tests import kvcases as kc
+
+    fields, fw, B, S = 3, 2, 6000, 10
+    ctx = L.Context(workload=L.WORKLOAD_KV, kv_rows=rows, kv_fields=fields, kv_field_bytes=4 * fw, max_txns=B,
+                    kv_max_slots=S, kv_hot_row=hot_row, kv_max_scan=scan)
+    table = kc.initial_table(rows, fields, fw)
+    ctx.kv_table_load(table)
+    for ep in range(4):
+        ctx.kv_set_tuning(L.TUNE_PARTITIONED | (L.TUNE_SPLIT_REGISTERS if ep % 2 else 0))
+        keys, meta, tids = L.kv_generate(seed=700 + ep, rows=rows, fields=fields, n_txns=B, n_slots=S, min_ops=1,
+                                         max_ops=S, theta=theta, tid_base=ep * B, scan_len=scan)
+        if scan:
+            keys[::97] = rows - 2  # scans clipped at the end of the table
+        ctx.kv_submit(keys, meta, tids, S)
+        ctx.epoch_run()
+        ctx.sync()
+        ora = oracle.kv_epoch(table, rows, fields, fw, keys, meta, tids, S, inplace=True, hot_row=hot_row,
+                              max_scan=scan)
+        bm = ctx.commit_bitmap()
+        np.testing.assert_array_equal(bm, ora.bitmap)
+        assert ctx.commit_count() == ora.commit_count
+        com = np.unpackbits(bm.view(np.uint8), bitorder="little")[:B].astype(bool)
+        L_ = max(1, scan)
+        kinds = (meta & 3).astype(np.int64)
+        lens = np.where(kinds == 3, (meta >> 24).astype(np.int64) + 1, (kinds == 1).astype(np.int64))
+        j = np.arange(L_)[None, :]
+        # row j of a read / scan slot of a committed transaction, inside the table
+        valid = (j < lens[:, None]) & ((keys.astype(np.int64)[:, None] + j) < rows)
+        valid &= np.broadcast_to(com[None, :], (S, B)).reshape(-1)[:, None]
+        mask = np.repeat(valid.reshape(-1), fw)
+        np.testing.assert_array_equal(ctx.kv_read_values()[mask], ora.read_values[mask])
+    np.testing.assert_array_equal(ctx.kv_table_read(), table)
+
+
+@pytest.mark.parametrize("flags", [L.TUNE_PARTITIONED, L.TUNE_PARTITIONED | L.TUNE_SPLIT_REGISTERS | 4],
+                         ids=["partitioned", "partitioned-split-small"])
+def test_graph_form_with_partitioned_execute_matches_plain_loop(gpu, flags):
+    """Every key-value batch on the row-partitioned EXECUTE in the graph
+    form (EXECUTE of batch j + 1 overlapping CHECK of batch j, two register
+    sets, one partition scratch): the default-shape plain loop's commit sets
+    and tables."""
+    kv0, tp0 = _contexts()
+    descs0, keep0, meta = _batches(np.random.default_rng(13), 30, kv0, tp0)
+    plain = L.Batched(kv0, tp0, descs0)
+    plain.run(L.RUN_PLAIN)
+    want = _bitmaps(plain, meta)
+    kv1, tp1 = _contexts()
+    descs1, keep1, _ = _batches(np.random.default_rng(13), 30, kv1, tp1)
+    part = L.Batched(kv1, tp1, descs1)
+    part.set_tuning(flags)
+    part.run(L.RUN_GRAPH)
+    for a, b in zip(want, _bitmaps(part, meta)):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    for x, y in zip(_tables(kv0, tp0), _tables(kv1, tp1)):
+        np.testing.assert_array_equal(x, y)
+    part.run(L.RUN_GRAPH)  # a replay of the captured graph decides the same
+    for a, b in zip(want, _bitmaps(part, meta)):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
