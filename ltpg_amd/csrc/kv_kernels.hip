@@ -768,6 +768,124 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_scatter_kernel(PartArgs a)
     }
 }
 
+// Binning of point operations (no scans) through LDS: a block takes
+// kPartStaged ops, ranks each in its bucket with an LDS counter, turns the
+// counts into block-local starts, reserves one run per bucket in the global
+// bucket with one atomic, places the entries bucket by bucket in LDS and
+// writes them out in that order -- consecutive lanes write consecutive
+// entries of a run, so the stores coalesce instead of scattering 8- and
+// 4-byte writes over every bucket.  MaxB bounds the buckets: with at most
+// kPartFewBuckets the block's LDS (64 KB) leaves room for a second block on
+// the CU, whose loads overlap this one's LDS phases.
+constexpr uint32_t kPartStagedPer = 4;
+constexpr uint32_t kPartStaged = kPartBlock * kPartStagedPer;
+constexpr uint32_t kPartFewBuckets = 2048;
+
+// Exclusive prefix sum over the block of one value per thread; *total gets
+// the sum.  Wave scans by shuffles, then the 16 wave totals through LDS.
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t *s_wt, uint32_t *total)
+{
+    constexpr uint32_t kWaves = kPartBlock / 64;
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    uint32_t inc = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(inc, d);
+        if (lane >= d) inc += x;
+    }
+    if (lane == 63) s_wt[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t t = s_wt[w];
+        before += w < wave ? t : 0u;
+        all += t;
+    }
+    *total = all;
+    return before + inc - v;
+}
+
+template <uint32_t MaxB>
+__global__ __launch_bounds__(kPartBlock) void kv_part_scatter_staged_kernel(PartArgs a)
+{
+    constexpr uint32_t kPer = (MaxB + kPartBlock - 1) / kPartBlock;  // buckets per thread at most
+    __shared__ uint32_t s_at[MaxB];              // counts, then local starts, then global - local
+    __shared__ unsigned long long st_me[kPartStaged];
+    __shared__ uint32_t st_key[kPartStaged];
+    __shared__ uint16_t st_b[kPartStaged];
+    __shared__ uint32_t s_wt[kPartBlock / 64];
+    for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) s_at[b] = 0;
+    __syncthreads();
+    const uint64_t n_ops = (uint64_t)a.S * a.B;
+    const uint64_t base = (uint64_t)blockIdx.x * kPartStaged + threadIdx.x;
+    Op o[kPartStagedPer];
+    part_load(a, base, o);
+    uint32_t bk[kPartStagedPer], rank[kPartStagedPer];
+    bool use[kPartStagedPer];
+    unsigned long long me[kPartStagedPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kPartStagedPer; ++u) {
+        use[u] = o[u].valid && o[u].len <= 1 && o[u].row != a.hot;
+        bk[u] = (uint32_t)(o[u].row >> kPartShift);
+        const uint64_t i = base + (uint64_t)u * kPartBlock;
+        const uint32_t i32 = i < n_ops ? (uint32_t)i : 0u;
+        const uint32_t t = a.rm ? i32 / a.S : i32 % a.B;
+        me[u] = priority(a.tag, a.tids[t], t);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kPartStagedPer; ++u)
+        rank[u] = use[u] ? atomicAdd(&s_at[bk[u]], 1u) : 0u;
+    __syncthreads();
+    // thread j owns buckets [j * per, (j + 1) * per): their counts, then
+    // their local starts from the block-wide exclusive sum
+    const uint32_t per = (a.nb + kPartBlock - 1) / kPartBlock;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t cnt[kPer], st[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        cnt[j] = (j < per && b0 + j < a.nb) ? s_at[b0 + j] : 0u;
+        sum += cnt[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_sum(sum, s_wt, &total);
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        st[j] = run;
+        run += cnt[j];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j)
+        if (cnt[j]) s_at[b0 + j] = st[j];
+    __syncthreads();
+    // this thread's entries go to their bucket's local run
+    uint32_t slot[kPartStagedPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kPartStagedPer; ++u) slot[u] = use[u] ? s_at[bk[u]] + rank[u] : 0u;
+    // one global run per non-empty bucket; s_at[b] becomes global - local
+    uint32_t delta[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) delta[j] = cnt[j] ? atomicAdd(&a.cursor[b0 + j], cnt[j]) - st[j] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kPartStagedPer; ++u) {
+        if (!use[u]) continue;
+        st_me[slot[u]] = me[u];
+        st_key[slot[u]] = ((uint32_t)(o[u].row & (kPartRows - 1)) << 1) | (o[u].kind == LTPG_KV_WRITE ? 1u : 0u);
+        st_b[slot[u]] = (uint16_t)bk[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j)
+        if (cnt[j]) s_at[b0 + j] = delta[j];
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kPartBlock) {
+        const uint32_t at = j + s_at[st_b[j]];
+        a.ent_me[at] = st_me[j];
+        a.ent_key[at] = st_key[j];
+    }
+}
+
 // The fast form's flags of one operation against its row's minimum W
 // priority w (~0: no writer this epoch).
 __device__ __forceinline__ void part_flag(const PartArgs &a, uint32_t key, unsigned long long me,
@@ -2431,7 +2549,14 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
         if (slices > kv.part.cap_slices) slices = kv.part.cap_slices;
         hipLaunchKernelGGL(kv_part_count_kernel, chunks, dim3(kPartBlock), 0, s, a, zero_counts);
         hipLaunchKernelGGL(kv_part_scan_kernel, dim3(1), dim3(kPartBlock), 0, s, a);
-        hipLaunchKernelGGL(kv_part_scatter_kernel, chunks, dim3(kPartBlock), 0, s, a);
+        if (kv.max_scan > 1)
+            hipLaunchKernelGGL(kv_part_scatter_kernel, chunks, dim3(kPartBlock), 0, s, a);
+        else if (a.nb <= kPartFewBuckets)
+            hipLaunchKernelGGL(kv_part_scatter_staged_kernel<kPartFewBuckets>,
+                               dim3((unsigned)((ops + kPartStaged - 1) / kPartStaged)), dim3(kPartBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL(kv_part_scatter_staged_kernel<kPartMaxBuckets>,
+                               dim3((unsigned)((ops + kPartStaged - 1) / kPartStaged)), dim3(kPartBlock), 0, s, a);
         hipLaunchKernelGGL(kv_part_process_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
         hipLaunchKernelGGL(kv_part_flags_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
         return hipGetLastError();

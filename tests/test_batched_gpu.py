@@ -340,6 +340,16 @@ def test_kv_partitioned_execute_matches_oracle(gpu, rows, theta, hot_row, scan):
     several slices (global W folding) and many single-slice buckets (LDS
     only), skew that overloads one bucket, the hot row, scans across bucket
     edges, and epochs one after the other on the same registers."""
+    _partitioned_epochs(rows, theta, hot_row, scan)
+
+
+def test_kv_partitioned_execute_many_buckets_matches_oracle(gpu):
+    """More than 2048 buckets of 8192 rows (16.8M rows): the binning kernel
+    whose LDS holds every bucket's counter up to the 16384-bucket limit."""
+    _partitioned_epochs(2048 * 8192 + 5000, 0.6, L.NO_HOT_ROW, 0)
+
+
+def _partitioned_epochs(rows, theta, hot_row, scan):
     from oracle import oracle
     from tests import kvcases as kc
 
