@@ -769,16 +769,15 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_scatter_kernel(PartArgs a)
 }
 
 // Binning of point operations (no scans) through LDS: a block takes
-// kPartStaged ops, ranks each in its bucket with an LDS counter, turns the
+// P ops per thread, ranks each in its bucket with an LDS counter, turns the
 // counts into block-local starts, reserves one run per bucket in the global
 // bucket with one atomic, places the entries bucket by bucket in LDS and
 // writes them out in that order -- consecutive lanes write consecutive
 // entries of a run, so the stores coalesce instead of scattering 8- and
-// 4-byte writes over every bucket.  MaxB bounds the buckets: with at most
-// kPartFewBuckets the block's LDS (64 KB) leaves room for a second block on
-// the CU, whose loads overlap this one's LDS phases.
-constexpr uint32_t kPartStagedPer = 4;
-constexpr uint32_t kPartStaged = kPartBlock * kPartStagedPer;
+// 4-byte writes over every bucket.  MaxB bounds the buckets.  Runs are as
+// long as the block's share of a bucket, so a block takes as many ops as
+// its LDS holds: 8 per thread (122 KB) with at most kPartFewBuckets
+// buckets, 4 per thread beside the 64 KB of counters for 16384.
 constexpr uint32_t kPartFewBuckets = 2048;
 
 // Exclusive prefix sum over the block of one value per thread; *total gets
@@ -806,26 +805,26 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t *s_
     return before + inc - v;
 }
 
-template <uint32_t MaxB>
+template <uint32_t MaxB, uint32_t P>
 __global__ __launch_bounds__(kPartBlock) void kv_part_scatter_staged_kernel(PartArgs a)
 {
     constexpr uint32_t kPer = (MaxB + kPartBlock - 1) / kPartBlock;  // buckets per thread at most
     __shared__ uint32_t s_at[MaxB];              // counts, then local starts, then global - local
-    __shared__ unsigned long long st_me[kPartStaged];
-    __shared__ uint32_t st_key[kPartStaged];
-    __shared__ uint16_t st_b[kPartStaged];
+    __shared__ unsigned long long st_me[P * kPartBlock];
+    __shared__ uint32_t st_key[P * kPartBlock];
+    __shared__ uint16_t st_b[P * kPartBlock];
     __shared__ uint32_t s_wt[kPartBlock / 64];
     for (uint32_t b = threadIdx.x; b < a.nb; b += kPartBlock) s_at[b] = 0;
     __syncthreads();
     const uint64_t n_ops = (uint64_t)a.S * a.B;
-    const uint64_t base = (uint64_t)blockIdx.x * kPartStaged + threadIdx.x;
-    Op o[kPartStagedPer];
+    const uint64_t base = (uint64_t)blockIdx.x * (P * kPartBlock) + threadIdx.x;
+    Op o[P];
     part_load(a, base, o);
-    uint32_t bk[kPartStagedPer], rank[kPartStagedPer];
-    bool use[kPartStagedPer];
-    unsigned long long me[kPartStagedPer];
+    uint32_t bk[P], rank[P];
+    bool use[P];
+    unsigned long long me[P];
 #pragma unroll
-    for (uint32_t u = 0; u < kPartStagedPer; ++u) {
+    for (uint32_t u = 0; u < P; ++u) {
         use[u] = o[u].valid && o[u].len <= 1 && o[u].row != a.hot;
         bk[u] = (uint32_t)(o[u].row >> kPartShift);
         const uint64_t i = base + (uint64_t)u * kPartBlock;
@@ -834,7 +833,7 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_scatter_staged_kernel(Part
         me[u] = priority(a.tag, a.tids[t], t);
     }
 #pragma unroll
-    for (uint32_t u = 0; u < kPartStagedPer; ++u)
+    for (uint32_t u = 0; u < P; ++u)
         rank[u] = use[u] ? atomicAdd(&s_at[bk[u]], 1u) : 0u;
     __syncthreads();
     // thread j owns buckets [j * per, (j + 1) * per): their counts, then
@@ -860,15 +859,15 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_scatter_staged_kernel(Part
         if (cnt[j]) s_at[b0 + j] = st[j];
     __syncthreads();
     // this thread's entries go to their bucket's local run
-    uint32_t slot[kPartStagedPer];
+    uint32_t slot[P];
 #pragma unroll
-    for (uint32_t u = 0; u < kPartStagedPer; ++u) slot[u] = use[u] ? s_at[bk[u]] + rank[u] : 0u;
+    for (uint32_t u = 0; u < P; ++u) slot[u] = use[u] ? s_at[bk[u]] + rank[u] : 0u;
     // one global run per non-empty bucket; s_at[b] becomes global - local
     uint32_t delta[kPer];
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) delta[j] = cnt[j] ? atomicAdd(&a.cursor[b0 + j], cnt[j]) - st[j] : 0u;
 #pragma unroll
-    for (uint32_t u = 0; u < kPartStagedPer; ++u) {
+    for (uint32_t u = 0; u < P; ++u) {
         if (!use[u]) continue;
         st_me[slot[u]] = me[u];
         st_key[slot[u]] = ((uint32_t)(o[u].row & (kPartRows - 1)) << 1) | (o[u].kind == LTPG_KV_WRITE ? 1u : 0u);
@@ -2552,11 +2551,13 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
         if (kv.max_scan > 1)
             hipLaunchKernelGGL(kv_part_scatter_kernel, chunks, dim3(kPartBlock), 0, s, a);
         else if (a.nb <= kPartFewBuckets)
-            hipLaunchKernelGGL(kv_part_scatter_staged_kernel<kPartFewBuckets>,
-                               dim3((unsigned)((ops + kPartStaged - 1) / kPartStaged)), dim3(kPartBlock), 0, s, a);
+            hipLaunchKernelGGL((kv_part_scatter_staged_kernel<kPartFewBuckets, 8>),
+                               dim3((unsigned)((ops + 8 * kPartBlock - 1) / (8 * kPartBlock))), dim3(kPartBlock), 0, s,
+                               a);
         else
-            hipLaunchKernelGGL(kv_part_scatter_staged_kernel<kPartMaxBuckets>,
-                               dim3((unsigned)((ops + kPartStaged - 1) / kPartStaged)), dim3(kPartBlock), 0, s, a);
+            hipLaunchKernelGGL((kv_part_scatter_staged_kernel<kPartMaxBuckets, 4>),
+                               dim3((unsigned)((ops + 4 * kPartBlock - 1) / (4 * kPartBlock))), dim3(kPartBlock), 0, s,
+                               a);
         hipLaunchKernelGGL(kv_part_process_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
         hipLaunchKernelGGL(kv_part_flags_kernel, dim3((unsigned)slices), dim3(kPartBlock), 0, s, a);
         return hipGetLastError();
