@@ -471,10 +471,14 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
  *   keeps each range's W minima in LDS, instead of probing and updating the
  *   per-row registers in HBM at random (EXEC_PER_OP then has no effect).
  *   Two streaming passes over the ops and one over the binned operations
- *   (12 bytes each); the autotuner tries it with every other combination.
+ *   (12 bytes each); the autotuner tries it with SPLIT_REGISTERS,
+ *   SMALL_CHUNKS and LANE_COPY.
  *   Needs kv_rows <= 8192 * 16384 and kv_max_slots * max_txns *
  *   max(1, kv_max_scan) < 2^32 (else LTPG_ERR_UNSUPPORTED); the first
  *   selection allocates its buffers, 12 bytes per possible operation.
+ * MORE_ITEMS: items per thread -- the per-op EXECUTE (EXEC_PER_OP) takes 4
+ *   ops per lane instead of 2 (its LDS fold spans 1024 ops), and the read
+ *   marking 2 instead of 1; no effect with PARTITIONED.
  * Results are identical for every combination; may change between any two
  * epochs. */
 #define LTPG_TUNE_EXEC_PER_OP 1u
@@ -484,8 +488,18 @@ int ltpg_tpcc_gen_next(ltpg_tpcc_gen *g, uint32_t n_txns, uint32_t tid_base, uin
 #define LTPG_TUNE_ALL 15u
 #define LTPG_TUNE_SMALL_EPOCH 16u
 #define LTPG_TUNE_PARTITIONED 32u
+#define LTPG_TUNE_MORE_ITEMS 64u
 #define LTPG_SMALL_EPOCH_OPS 2048u
 int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags);
+/* Launch shape of an order-entry context's CID, EXECUTE, CHECK and
+ * WRITEBACK kernels (no counterpart in the reference, whose grid constants
+ * are fixed, TPCC-allopt-final/Predefine.h:35-46).  TPCC_SMALL_BLOCKS: 64-thread
+ * workgroups (4 NewOrders or 64 Payments each) instead of 256, which spreads
+ * a small batch over four times as many CUs.  Results are identical; may
+ * change between any two epochs.  LTPG_ERR_UNSUPPORTED on a key-value
+ * context. */
+#define LTPG_TUNE_TPCC_SMALL_BLOCKS 128u
+int ltpg_tpcc_set_tuning(ltpg_ctx *ctx, uint32_t flags);
 
 /* ---- epoch phases (KERNEL_EXECUTE/CHECK/WRITEBACK, TPCC-allopt-final/
  *      tp.cu:24-117, YCSB/0/ycsb.cu:9-58; launched by tp.cu:154-236) ------- */
@@ -586,26 +600,34 @@ int ltpg_batched_bitmap(ltpg_batched *b, uint32_t batch, uint32_t type, uint64_t
 int ltpg_batched_times(ltpg_batched *b, double *ms, uint32_t n);
 int ltpg_batched_destroy(ltpg_batched *b);
 
-/* Autotuner.  For every size class of the runner's key-value batches
- * (floor(log2(n_txns))) it times every LTPG_TUNE_* combination on the first
- * batch of the class (reps epochs after one warm-up, HIP events on the
- * context's stream, write-back in copy-back mode so the tables are left as
- * they were), keeps the fastest for every batch of the class, and captures
- * the graph again.  One entry per class is written to out. */
+/* Autotuner.  For every size class (floor(log2(n_txns))) of the runner's
+ * key-value batches and of its order-entry batches it times every distinct
+ * launch shape on the first batch of the class (reps epochs after one
+ * warm-up, HIP events on the context's stream, write-back in copy-back mode
+ * so the tables are left as they were), keeps the fastest for every batch of
+ * the class, and captures the graphs again.  Key-value shapes: the
+ * LTPG_TUNE_ALL combinations, each with and without MORE_ITEMS (EXEC_PER_OP
+ * and MORE_ITEMS act only without PARTITIONED), and PARTITIONED with the
+ * SPLIT_REGISTERS / SMALL_CHUNKS / LANE_COPY combinations; order-entry
+ * shapes: 0 and LTPG_TUNE_TPCC_SMALL_BLOCKS.  One entry per class and
+ * workload is written to out, key-value classes first. */
 typedef struct ltpg_tuning_entry {
-    uint32_t workload;         /* LTPG_WORKLOAD_KV                                  */
+    uint32_t workload;         /* LTPG_WORKLOAD_KV or LTPG_WORKLOAD_TPCC            */
     uint32_t log2_txns;        /* the size class                                    */
     uint32_t flags;            /* chosen LTPG_TUNE_* combination                    */
     uint32_t reserved;
-    float ms[32];              /* epoch time of every combination: [flags] for flags
-                                  <= LTPG_TUNE_ALL, [16 + flags] for flags |
-                                  LTPG_TUNE_PARTITIONED                           */
+    float ms[64];              /* epoch time of every shape tried, 0 for the others:
+                                  key-value at LTPG_TUNE_INDEX(flags), order-entry
+                                  [0] (256-thread blocks) and [1] (SMALL_BLOCKS) */
 } ltpg_tuning_entry;
+#define LTPG_TUNE_INDEX(f) (((f) & LTPG_TUNE_ALL) | (((f) & LTPG_TUNE_PARTITIONED) ? 16u : 0u) | \
+                            (((f) & LTPG_TUNE_MORE_ITEMS) ? 32u : 0u))
 int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out, uint32_t max_out,
                           uint32_t *n_out);
-/* One launch shape (LTPG_TUNE_* flags, LTPG_TUNE_SMALL_EPOCH excluded) for
- * every key-value batch of the runner, in place of the autotuner's choice;
- * captures the graph again. */
+/* One launch shape for every batch of the runner, in place of the
+ * autotuner's choice: the key-value bits of flags (LTPG_TUNE_SMALL_EPOCH
+ * excluded) for its key-value batches, LTPG_TUNE_TPCC_SMALL_BLOCKS for its
+ * order-entry batches; captures the graphs again. */
 int ltpg_batched_set_tuning(ltpg_batched *b, uint32_t flags);
 
 /* ---- served form: a request loop over resident tables (replaces the

@@ -60,6 +60,23 @@ RUN_PLAIN, RUN_GRAPH = 0, 1
 TUNE_EXEC_PER_OP, TUNE_SPLIT_REGISTERS, TUNE_SMALL_CHUNKS, TUNE_LANE_COPY, TUNE_ALL = 1, 2, 4, 8, 15
 TUNE_SMALL_EPOCH, SMALL_EPOCH_OPS = 16, 2048
 TUNE_PARTITIONED = 32
+TUNE_MORE_ITEMS = 64
+TUNE_TPCC_SMALL_BLOCKS = 128
+
+
+def tune_index(flags):
+    """Index of a key-value launch shape in a tuning entry's ms (LTPG_TUNE_INDEX)."""
+    return (flags & TUNE_ALL) | (16 if flags & TUNE_PARTITIONED else 0) | (32 if flags & TUNE_MORE_ITEMS else 0)
+
+
+def kv_shapes(partitioned=True):
+    """The distinct key-value launch shapes (the autotuner's set): every
+    TUNE_ALL combination with and without MORE_ITEMS, and PARTITIONED with
+    the SPLIT_REGISTERS / SMALL_CHUNKS / LANE_COPY combinations."""
+    out = [f | m for m in (0, TUNE_MORE_ITEMS) for f in range(16)]
+    if partitioned:
+        out += [TUNE_PARTITIONED | f for f in range(16) if not f & TUNE_EXEC_PER_OP]
+    return out
 NOALLOC_MAX_ROWS = NOALLOC_MAX_TXNS = 1 << 20
 
 
@@ -254,6 +271,10 @@ class Context:
     def kv_set_tuning(self, flags=0):
         """LTPG_TUNE_* launch shape of the key-value kernels (results never change)."""
         _check(lib.ltpg_kv_set_tuning(self.handle, int(flags)), "ltpg_kv_set_tuning")
+
+    def tpcc_set_tuning(self, flags=0):
+        """Launch shape of the order-entry kernels: 0 or TUNE_TPCC_SMALL_BLOCKS."""
+        _check(lib.ltpg_tpcc_set_tuning(self.handle, int(flags)), "ltpg_tpcc_set_tuning")
 
     def kv_fill_reference(self):
         _check(lib.ltpg_kv_table_fill_reference(self.handle), "ltpg_kv_table_fill_reference")
@@ -692,15 +713,18 @@ class Batched:
         return out[:len(self.descs)]
 
     def autotune(self, reps=3):
-        """Pick the key-value launch shape per size class; returns one dict
-        per class (log2 size, chosen TUNE_* flags, ms of every combination)."""
-        out = (_lib.TuningEntry * 33)()
+        """Pick the launch shape per workload and size class; returns one
+        dict per class (workload, log2 size, chosen TUNE_* flags, ms of every
+        shape tried: key-value at tune_index(flags), order-entry [0] and [1])."""
+        out = (_lib.TuningEntry * 66)()
         n = ctypes.c_uint32()
-        _check(lib.ltpg_batched_autotune(self.handle, reps, out, 33, ctypes.byref(n)), "ltpg_batched_autotune")
-        return [{"log2_txns": e.log2_txns, "flags": e.flags, "ms": list(e.ms)} for e in out[:n.value]]
+        _check(lib.ltpg_batched_autotune(self.handle, reps, out, 66, ctypes.byref(n)), "ltpg_batched_autotune")
+        return [{"workload": e.workload, "log2_txns": e.log2_txns, "flags": e.flags, "ms": list(e.ms)}
+                for e in out[:n.value]]
 
     def set_tuning(self, flags):
-        """One TUNE_* launch shape for every key-value batch (recaptures)."""
+        """One launch shape for every batch (recaptures): the key-value bits
+        for key-value batches, TUNE_TPCC_SMALL_BLOCKS for order-entry ones."""
         _check(lib.ltpg_batched_set_tuning(self.handle, flags), "ltpg_batched_set_tuning")
 
     def bitmap(self, i, type_):

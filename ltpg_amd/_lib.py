@@ -128,7 +128,7 @@ class TuningEntry(ctypes.Structure):
     """struct ltpg_tuning_entry."""
 
     _fields_ = [("workload", ctypes.c_uint32), ("log2_txns", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 32)]
+                ("reserved", ctypes.c_uint32), ("ms", ctypes.c_float * 64)]
 
 
 class ServeConfig(ctypes.Structure):
@@ -189,6 +189,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(KvBatch), ctypes.c_uint32,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("ltpg_kv_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("ltpg_tpcc_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_kv_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KvBatch), ctypes.c_uint32]),
     ("ltpg_epoch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("ltpg_epoch_reset", ctypes.c_int, [ctypes.c_void_p]),

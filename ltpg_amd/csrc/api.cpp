@@ -808,7 +808,7 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->kv.table) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no key-value table");
-    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_SMALL_EPOCH | LTPG_TUNE_PARTITIONED))
+    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_SMALL_EPOCH | LTPG_TUNE_PARTITIONED | LTPG_TUNE_MORE_ITEMS))
         LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
     if ((flags & LTPG_TUNE_PARTITIONED) && (rc = kv_part_prepare(ctx))) return rc;
     ctx->kv.partitioned = (flags & LTPG_TUNE_PARTITIONED) ? 1 : 0;
@@ -817,6 +817,17 @@ int ltpg_kv_set_tuning(ltpg_ctx *ctx, uint32_t flags)
     ctx->kv.small_chunks = (flags & LTPG_TUNE_SMALL_CHUNKS) ? 1 : 0;
     ctx->kv.lane_copy = (flags & LTPG_TUNE_LANE_COPY) ? 1 : 0;
     ctx->kv.small_epoch = (flags & LTPG_TUNE_SMALL_EPOCH) ? 1 : 0;
+    ctx->kv.more_items = (flags & LTPG_TUNE_MORE_ITEMS) ? 1 : 0;
+    return LTPG_OK;
+}
+
+int ltpg_tpcc_set_tuning(ltpg_ctx *ctx, uint32_t flags)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->cfg.workload != LTPG_WORKLOAD_TPCC) LTPG_FAIL(LTPG_ERR_UNSUPPORTED, "context has no order-entry tables");
+    if (flags & ~LTPG_TUNE_TPCC_SMALL_BLOCKS) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    ctx->tpcc.small_blocks = (flags & LTPG_TUNE_TPCC_SMALL_BLOCKS) != 0;
     return LTPG_OK;
 }
 
@@ -958,12 +969,12 @@ static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
         if ((rc = take_events(ctx, &ev))) return rc;
         LTPG_HIP_TRY(hipEventRecord(ev[0], s));
     }
-    if (phase_mask & LTPG_PHASE_CID) LTPG_HIP_TRY(launch_tpcc_cid(tp.d, tp.pay, s));
+    if (phase_mask & LTPG_PHASE_CID) LTPG_HIP_TRY(launch_tpcc_cid(tp.d, tp.pay, tp.small_blocks, s));
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[1], s));
     const bool ro = tp.ro.n_os + tp.ro.n_sl > 0, dl = tp.dl.n > 0;
     const uint32_t dl_base = tp.no.n + tp.pay.n + tp.ro.n_os + tp.ro.n_sl;
     if (phase_mask & LTPG_PHASE_EXECUTE) {
-        LTPG_HIP_TRY(launch_tpcc_execute(tp.d, tp.no, tp.pay, e, counts, ctx->counters, s));
+        LTPG_HIP_TRY(launch_tpcc_execute(tp.d, tp.no, tp.pay, e, counts, tp.small_blocks, ctx->counters, s));
         // Deliveries register after NewOrders and Payments, then read-only
         // transactions mark war on the registered writers
         if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_execute(tp.d, tp.dl, dl_base, e, ctx->counters, s));
@@ -972,7 +983,7 @@ static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[2], s));
     if (phase_mask & LTPG_PHASE_CHECK) {
         if (!(phase_mask & LTPG_PHASE_EXECUTE)) LTPG_HIP_TRY(launch_reset_epoch_counters(ctx->counters, s));
-        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ro, dl, ctx->counters, s));
+        LTPG_HIP_TRY(launch_tpcc_check(tp.d, tp.no, tp.pay, e, counts, ro, dl, tp.small_blocks, ctx->counters, s));
         if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_check(tp.d, tp.dl, dl_base, e, ctx->counters, s));
     }
     if (ev) {
@@ -980,7 +991,7 @@ static int tpcc_epoch_run(ltpg_ctx *ctx, uint32_t phase_mask)
         LTPG_HIP_TRY(hipEventRecord(ev[4], s));
     }
     if (phase_mask & LTPG_PHASE_WRITEBACK) {
-        LTPG_HIP_TRY(launch_tpcc_writeback(tp.d, tp.no, tp.pay, e, ctx->cfg.values, s));
+        LTPG_HIP_TRY(launch_tpcc_writeback(tp.d, tp.no, tp.pay, e, ctx->cfg.values, tp.small_blocks, s));
         if (dl) LTPG_HIP_TRY(launch_tpcc_delivery_writeback(tp.d, tp.dl, dl_base, e, ctx->cfg.values, s));
     }
     if (ev) LTPG_HIP_TRY(hipEventRecord(ev[5], s));

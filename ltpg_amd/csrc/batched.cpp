@@ -40,7 +40,7 @@ struct ltpg_batched {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t final_epoch[2] = {0, 0};       // host epoch counter after a replay
-    std::vector<uint8_t> tune;              // key-value batch i: LTPG_TUNE_* flags
+    std::vector<uint8_t> tune;              // batch i: its LTPG_TUNE_* flags (key-value or order-entry)
     unsigned long long *stamps = nullptr;   // graph runs: device clock after batch i; [n + k] start of graph k
     std::vector<double> plain_ms;           // plain runs: host ms from the run's start to batch i's readback
     // key-value pipelining: side stream for EXECUTE, second register / flag set
@@ -72,6 +72,7 @@ int enqueue_batch(ltpg_batched *b, uint32_t i, bool stamp, bool stepwise = false
         if ((rc = ltpg_kv_set_tuning(ctx, b->tune[i]))) return rc;
         if ((rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE))) return rc;
     } else {
+        if ((rc = ltpg_tpcc_set_tuning(ctx, b->tune[i]))) return rc;
         if ((rc = ltpg_tpcc_submit(ctx, &d.neworder, &d.payment, LTPG_MEM_DEVICE))) return rc;
         if ((d.orderstatus.n || d.stocklevel.n) &&
             (rc = ltpg_tpcc_submit_readonly(ctx, &d.orderstatus, &d.stocklevel, LTPG_MEM_DEVICE)))
@@ -367,101 +368,135 @@ int ltpg_batched_autotune(ltpg_batched *b, uint32_t reps, ltpg_tuning_entry *out
 {
     if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null runner");
     if (n_out) *n_out = 0;
-    ltpg_ctx *ctx = b->ctx[0];
-    if (!ctx) return LTPG_OK;  // no key-value batches: nothing to tune
-    int rc = check_ctx(ctx);
-    if (rc) return rc;
-    ctx->kv.regs_clean = false;
     if (reps == 0) reps = 1;
-    // size classes and their first batch
-    std::vector<int> rep(33, -1);
-    for (uint32_t i = 0; i < b->d.size(); i++) {
-        if (b->d[i].workload != LTPG_WORKLOAD_KV || b->d[i].kv.n_txns == 0) continue;
-        const int c = 31 - __builtin_clz(b->d[i].kv.n_txns);
-        if (rep[c] < 0) rep[c] = (int)i;
-    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     LTPG_HIP_TRY(hipEventCreate(&e0));
     LTPG_HIP_TRY(hipEventCreate(&e1));
-    const uint32_t values = ctx->cfg.values;
-    ctx->cfg.values = LTPG_VALUES_COPYBACK;  // timing runs leave the table as it is
-    const bool timing = ctx->timing;
-    ctx->timing = false;
+    int rc = LTPG_OK;
     uint32_t n = 0;
-    for (int c = 0; c < 33 && !rc; c++) {
-        if (rep[c] < 0) continue;
-        const ltpg_batch_desc &d = b->d[rep[c]];
-        float ms[32] = {};
-        bool tried[32] = {};
-        // every combination, then each with the row-partitioned EXECUTE
-        // (skipped where the table or the epochs are too large for it)
-        const bool part_ok = kv_part_prepare(ctx) == LTPG_OK;
-        for (uint32_t k = 0; k < 32 && !rc; k++) {
-            if (k >= 16 && !part_ok) continue;
-            const uint32_t cfg = (k & LTPG_TUNE_ALL) | (k >= 16 ? LTPG_TUNE_PARTITIONED : 0u);
-            rc = ltpg_kv_set_tuning(ctx, cfg);
-            for (uint32_t k = 0; k <= reps && !rc; k++) {
-                if (k == 1) rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? LTPG_OK : LTPG_ERR_DEVICE;
-                if (!rc) rc = ltpg_kv_submit(ctx, &d.kv, LTPG_MEM_DEVICE);
-                if (!rc) rc = ltpg_epoch_run(ctx, LTPG_PHASE_ALL);
+    for (int k = 0; k < 2 && !rc; k++) {
+        ltpg_ctx *ctx = b->ctx[k];
+        if (!ctx) continue;
+        if ((rc = check_ctx(ctx))) break;
+        const uint32_t workload = k == 0 ? LTPG_WORKLOAD_KV : LTPG_WORKLOAD_TPCC;
+        // size classes and their first batch
+        std::vector<int> rep(33, -1);
+        for (uint32_t i = 0; i < b->d.size(); i++) {
+            const ltpg_batch_desc &d = b->d[i];
+            if (d.workload != workload) continue;
+            const uint32_t txns = k == 0 ? d.kv.n_txns
+                                         : d.neworder.n + d.payment.n + d.orderstatus.n + d.stocklevel.n +
+                                               d.delivery.n;
+            if (txns == 0) continue;
+            const int c = 31 - __builtin_clz(txns);
+            if (rep[c] < 0) rep[c] = (int)i;
+        }
+        if (k == 0) ctx->kv.regs_clean = false;
+        const uint32_t values = ctx->cfg.values;
+        ctx->cfg.values = LTPG_VALUES_COPYBACK;  // timing runs leave the tables as they are
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        // the distinct launch shapes of the workload
+        std::vector<uint32_t> shapes;
+        if (k == 0) {
+            const bool part_ok = kv_part_prepare(ctx) == LTPG_OK;
+            for (uint32_t idx = 0; idx < 64; idx++) {
+                const bool part = idx & 16, more = idx & 32;
+                // EXEC_PER_OP and MORE_ITEMS shape only the register path
+                if (part && (!part_ok || (idx & LTPG_TUNE_EXEC_PER_OP) || more)) continue;
+                shapes.push_back((idx & LTPG_TUNE_ALL) | (part ? LTPG_TUNE_PARTITIONED : 0u) |
+                                 (more ? LTPG_TUNE_MORE_ITEMS : 0u));
             }
-            if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LTPG_ERR_DEVICE;
-            if (!rc) rc = ltpg_sync(ctx);
-            float t = 0;
-            if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = LTPG_ERR_DEVICE;
-            ms[k] = t / (float)reps;
-            tried[k] = true;
+        } else {
+            shapes = {0u, LTPG_TUNE_TPCC_SMALL_BLOCKS};
         }
-        if (rc) break;
-        uint32_t best = 0;
-        for (uint32_t k = 1; k < 32; k++)
-            if (tried[k] && ms[k] < ms[best]) best = k;
-        const uint8_t t = (uint8_t)((best & LTPG_TUNE_ALL) | (best >= 16 ? LTPG_TUNE_PARTITIONED : 0u));
-        for (uint32_t i = 0; i < b->d.size(); i++)
-            if (b->d[i].workload == LTPG_WORKLOAD_KV && b->d[i].kv.n_txns &&
-                31 - __builtin_clz(b->d[i].kv.n_txns) == c)
-                b->tune[i] = t;
-        if (out && n < max_out) {
-            ltpg_tuning_entry &o = out[n];
-            o.workload = LTPG_WORKLOAD_KV;
-            o.log2_txns = (uint32_t)c;
-            o.flags = t;
-            o.reserved = 0;
-            for (int k = 0; k < 32; k++) o.ms[k] = ms[k];
+        for (int c = 0; c < 33 && !rc; c++) {
+            if (rep[c] < 0) continue;
+            const uint32_t i = (uint32_t)rep[c];
+            float ms[64] = {};
+            uint32_t best = shapes[0];
+            float best_ms = -1.f;
+            for (uint32_t f : shapes) {
+                for (uint32_t r = 0; r <= reps && !rc; r++) {
+                    if (r == 1) rc = hipEventRecord(e0, ctx->stream) == hipSuccess ? LTPG_OK : LTPG_ERR_DEVICE;
+                    if (rc) break;
+                    // submit + phases of the batch under shape f (no archive copy)
+                    const uint8_t keep = b->tune[i];
+                    b->tune[i] = (uint8_t)f;
+                    rc = enqueue_batch(b, i, false);
+                    b->tune[i] = keep;
+                }
+                if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LTPG_ERR_DEVICE;
+                if (!rc) rc = ltpg_sync(ctx);
+                float t = 0;
+                if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = LTPG_ERR_DEVICE;
+                if (rc) break;
+                t /= (float)reps;
+                ms[k == 0 ? LTPG_TUNE_INDEX(f) : (f ? 1u : 0u)] = t;
+                if (best_ms < 0.f || t < best_ms) {
+                    best_ms = t;
+                    best = f;
+                }
+            }
+            if (rc) break;
+            for (uint32_t j = 0; j < b->d.size(); j++) {
+                const ltpg_batch_desc &d = b->d[j];
+                if (d.workload != workload) continue;
+                const uint32_t txns = k == 0 ? d.kv.n_txns
+                                             : d.neworder.n + d.payment.n + d.orderstatus.n + d.stocklevel.n +
+                                                   d.delivery.n;
+                if (txns && 31 - __builtin_clz(txns) == c) b->tune[j] = (uint8_t)best;
+            }
+            if (out && n < max_out) {
+                ltpg_tuning_entry &o = out[n];
+                o.workload = workload;
+                o.log2_txns = (uint32_t)c;
+                o.flags = best;
+                o.reserved = 0;
+                for (int m = 0; m < 64; m++) o.ms[m] = ms[m];
+            }
+            n++;
         }
-        n++;
+        ctx->cfg.values = values;
+        ctx->timing = timing;
     }
-    ctx->cfg.values = values;
-    ctx->timing = timing;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (n_out) *n_out = n < max_out ? n : max_out;
-    // capture the key-value sequence again with the chosen shapes
-    if (b->exec[0]) (void)hipGraphExecDestroy(b->exec[0]);
-    if (b->graph[0]) (void)hipGraphDestroy(b->graph[0]);
-    b->exec[0] = nullptr;
-    b->graph[0] = nullptr;
-    return capture(b, 0);
+    // capture both sequences again with the chosen shapes
+    for (int k = 0; k < 2; k++) {
+        if (!b->exec[k] && !b->graph[k]) continue;
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->exec[k] = nullptr;
+        b->graph[k] = nullptr;
+        if ((rc = capture(b, k))) return rc;
+    }
+    return LTPG_OK;
 }
 
 int ltpg_batched_set_tuning(ltpg_batched *b, uint32_t flags)
 {
     if (!b) LTPG_FAIL(LTPG_ERR_INVALID, "null runner");
-    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_PARTITIONED)) LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
-    ltpg_ctx *ctx = b->ctx[0];
-    if (!ctx) return LTPG_OK;  // no key-value batches
-    int rc = check_ctx(ctx);
-    if (rc) return rc;
+    if (flags & ~(LTPG_TUNE_ALL | LTPG_TUNE_PARTITIONED | LTPG_TUNE_MORE_ITEMS | LTPG_TUNE_TPCC_SMALL_BLOCKS))
+        LTPG_FAIL(LTPG_ERR_INVALID, "unknown tuning flag");
+    const uint32_t kv_flags = flags & ~LTPG_TUNE_TPCC_SMALL_BLOCKS, tp_flags = flags & LTPG_TUNE_TPCC_SMALL_BLOCKS;
+    int rc;
     // validates the flags and prepares their buffers outside any capture
-    if ((rc = ltpg_kv_set_tuning(ctx, flags))) return rc;
+    if (b->ctx[0] && ((rc = check_ctx(b->ctx[0])) || (rc = ltpg_kv_set_tuning(b->ctx[0], kv_flags)))) return rc;
+    if (b->ctx[1] && ((rc = check_ctx(b->ctx[1])) || (rc = ltpg_tpcc_set_tuning(b->ctx[1], tp_flags)))) return rc;
     for (uint32_t i = 0; i < b->d.size(); i++)
-        if (b->d[i].workload == LTPG_WORKLOAD_KV) b->tune[i] = (uint8_t)flags;
-    if (b->exec[0]) (void)hipGraphExecDestroy(b->exec[0]);
-    if (b->graph[0]) (void)hipGraphDestroy(b->graph[0]);
-    b->exec[0] = nullptr;
-    b->graph[0] = nullptr;
-    return capture(b, 0);
+        b->tune[i] = (uint8_t)(b->d[i].workload == LTPG_WORKLOAD_KV ? kv_flags : tp_flags);
+    for (int k = 0; k < 2; k++) {
+        if (!b->exec[k] && !b->graph[k]) continue;
+        if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->exec[k] = nullptr;
+        b->graph[k] = nullptr;
+        if ((rc = capture(b, k))) return rc;
+    }
+    return LTPG_OK;
 }
 
 int ltpg_batched_times(ltpg_batched *b, double *ms, uint32_t n)

@@ -454,15 +454,14 @@ __global__ __launch_bounds__(kBlock) void kv_execute_ops_kernel(
 // EXECUTE, fast form, one lane per op with the block's duplicate rows
 // folded first.  Under skew thousands of in-flight writes hit the same hot
 // register at once, all see it stale, and their atomics serialize on one
-// address.  Here each block takes kDedupOps consecutive ops (kDedupPer per lane),
+// address.  Here each block takes P * kBlock consecutive ops (P per lane),
 // finds the minimum writer priority of every row among them in an LDS hash
 // table, and only that writer goes to the global register; the block's
 // other writers of the row already have waw (a lower writer exists).  Every
 // non-minimum writer of a row is still doomed exactly once.
-constexpr uint32_t kDedupPer = 2;
-constexpr uint32_t kDedupOps = kDedupPer * kBlock;
-constexpr uint32_t kDedupSlots = 2 * kDedupOps;  // load factor <= 1/2: probing always ends
-
+// P (ops per lane) is 2, or 4 with LTPG_TUNE_MORE_ITEMS: a wider fold and
+// more loads in flight per lane, against half the blocks.
+template <uint32_t P>
 __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
@@ -470,6 +469,8 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     uint8_t *__restrict__ doomed, uint8_t epoch, bool zero_counts, uint32_t max_scan,
     DeviceCounters *__restrict__ ctr, bool rm)
 {
+    constexpr uint32_t kDedupOps = P * kBlock;
+    constexpr uint32_t kDedupSlots = 2 * kDedupOps;  // load factor <= 1/2: probing always ends
     __shared__ unsigned long long s_key[kDedupSlots];  // row + 1, 0 = empty
     __shared__ unsigned long long s_min[kDedupSlots];
     if (zero_counts && blockIdx.x == 0)
@@ -482,13 +483,13 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     __syncthreads();
     const uint64_t n_ops = (uint64_t)S * B;
     const uint64_t base = (uint64_t)blockIdx.x * kDedupOps;
-    uint64_t row[kDedupPer];
-    unsigned long long me[kDedupPer];
-    uint32_t tt[kDedupPer], slot[kDedupPer];
-    bool wr[kDedupPer];
+    uint64_t row[P];
+    unsigned long long me[P];
+    uint32_t tt[P], slot[P];
+    bool wr[P];
     uint32_t bad = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k) {
+    for (uint32_t k = 0; k < P; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         const bool live = i < n_ops;
         const uint32_t t = live ? txn_of(i, B, S, rm) : 0u;
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
         slot[k] = 0;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k) {
+    for (uint32_t k = 0; k < P; ++k) {
         if (!wr[k]) continue;
         const unsigned long long key = row[k] + 1ull;
         uint32_t h = (uint32_t)((row[k] * 0x9E3779B97F4A7C15ull) >> 40) & (kDedupSlots - 1);
@@ -516,21 +517,21 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
     __syncthreads();
     // register_write for the block minima, split into phases so that the
     // four probes, then the four atomics, are in flight together
-    bool lead[kDedupPer], waw[kDedupPer];
-    unsigned long long cur[kDedupPer], old[kDedupPer];
+    bool lead[P], waw[P];
+    unsigned long long cur[P], old[P];
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k) {
+    for (uint32_t k = 0; k < P; ++k) {
         waw[k] = wr[k] && s_min[slot[k]] < me[k];
         lead[k] = wr[k] && !waw[k];
         cur[k] = lead[k] ? probe(&regs[rl.w(row[k])]) : ~0ull;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k) {
+    for (uint32_t k = 0; k < P; ++k) {
         old[k] = ~0ull;
         if (lead[k] && cur[k] > me[k]) old[k] = atomicMin(&regs[rl.w(row[k])], me[k]);
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k) {
+    for (uint32_t k = 0; k < P; ++k) {
         if (!lead[k]) continue;
         if (cur[k] < me[k] || old[k] < me[k]) {
             waw[k] = true;
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void kv_execute_dedup_kernel(
         }
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kDedupPer; ++k)
+    for (uint32_t k = 0; k < P; ++k)
         if (waw[k]) doomed[tt[k]] = epoch;
     if (bad) atomicAdd(&ctr->range_errors, bad);
 }
@@ -971,29 +972,42 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_flags_kernel(PartArgs a)
 // gives every transaction that can still commit its exact war bit; raw(t)
 // is exact for every t.  Flags hold the epoch number, so equal stores race
 // harmlessly and nothing is cleared between epochs.
+template <uint32_t P>
 __global__ __launch_bounds__(kBlock) void kv_mark_reads_kernel(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ tids, uint32_t B, uint32_t S, uint64_t rows, uint32_t fields,
     const unsigned long long *__restrict__ regs, RegLayout rl, uint64_t hot, uint32_t tag,
     uint8_t *__restrict__ raw_flag, uint8_t *__restrict__ war_flag, uint8_t epoch, uint32_t max_scan, bool rm)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool live = i < (uint64_t)S * B;
-    const uint32_t t = live ? txn_of(i, B, S, rm) : 0u;
-    const Op o = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
-    if (o.len == 0) return;  // writes, empty and invalid slots
-    const unsigned long long me = priority(tag, tids[t], t);
-    bool raw = false;
-    for (uint32_t k = 0; k < o.len; ++k) {  // a scan: every row of its range
-        if (o.row + k == hot) continue;
-        const unsigned long long w = regs[rl.w(o.row + k)];
-        if ((uint32_t)(w >> kTagShift) != tag) continue;  // no writer this epoch
-        if (w < me)
-            raw = true;
-        else if (w > me)
-            war_flag[w & kPosMask] = epoch;
+    // P ops per lane (1, or 2 with LTPG_TUNE_MORE_ITEMS), stride kBlock; all
+    // their loads issued before any is used
+    const uint64_t base = (uint64_t)blockIdx.x * (P * kBlock) + threadIdx.x;
+    const uint64_t n_ops = (uint64_t)S * B;
+    Op o[P];
+    uint32_t t[P];
+#pragma unroll
+    for (uint32_t u = 0; u < P; ++u) {
+        const uint64_t i = base + (uint64_t)u * kBlock;
+        const bool live = i < n_ops;
+        t[u] = live ? txn_of(i, B, S, rm) : 0u;
+        o[u] = load_op(keys, meta, live ? i : 0, live, rows, fields, max_scan);
     }
-    if (raw) raw_flag[t] = epoch;
+#pragma unroll
+    for (uint32_t u = 0; u < P; ++u) {
+        if (o[u].len == 0) continue;  // writes, empty and invalid slots
+        const unsigned long long me = priority(tag, tids[t[u]], t[u]);
+        bool raw = false;
+        for (uint32_t k = 0; k < o[u].len; ++k) {  // a scan: every row of its range
+            if (o[u].row + k == hot) continue;
+            const unsigned long long w = regs[rl.w(o[u].row + k)];
+            if ((uint32_t)(w >> kTagShift) != tag) continue;  // no writer this epoch
+            if (w < me)
+                raw = true;
+            else if (w > me)
+                war_flag[w & kPosMask] = epoch;
+        }
+        if (raw) raw_flag[t[u]] = epoch;
+    }
 }
 
 // CHECK + COMMIT, counting form (check_r / check_w_0 / check_w_1 and
@@ -2563,10 +2577,16 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
         return hipGetLastError();
     }
     if (kv.exec_per_op && !counts) {
-        hipLaunchKernelGGL(kv_execute_dedup_kernel, dim3((unsigned)((ops + kDedupOps - 1) / kDedupOps)), dim3(kBlock),
-                           0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs,
-                           reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts, kv.max_scan, ctr,
-                           b.request_major);
+        if (kv.more_items)
+            hipLaunchKernelGGL(kv_execute_dedup_kernel<4>, dim3((unsigned)((ops + 4 * kBlock - 1) / (4 * kBlock))),
+                               dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,
+                               kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts,
+                               kv.max_scan, ctr, b.request_major);
+        else
+            hipLaunchKernelGGL(kv_execute_dedup_kernel<2>, dim3((unsigned)((ops + 2 * kBlock - 1) / (2 * kBlock))),
+                               dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,
+                               kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed, e.epoch, zero_counts,
+                               kv.max_scan, ctr, b.request_major);
     } else if (kv.exec_per_op) {
         hipLaunchKernelGGL(kv_execute_ops_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
                            b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.doomed,
@@ -2576,10 +2596,17 @@ hipError_t launch_kv_execute(const KvState &kv, const KvBatchDev &b, const Epoch
                            b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
                            kv.doomed, e.epoch, counts, zero_counts, kv.max_scan, ctr, b.request_major);
     }
-    if (!counts)
-        hipLaunchKernelGGL(kv_mark_reads_kernel, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns,
-                           b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
-                           kv.raw_flag, kv.war_flag, e.epoch, kv.max_scan, b.request_major);
+    if (!counts) {
+        if (kv.more_items)
+            hipLaunchKernelGGL(kv_mark_reads_kernel<2>, dim3((unsigned)((ops + 2 * kBlock - 1) / (2 * kBlock))),
+                               dim3(kBlock), 0, s, b.keys, b.meta, b.tids, b.n_txns, b.n_slots, kv.rows, kv.fields,
+                               kv.regs, reg_layout(kv), kv.hot_row, e.tag, kv.raw_flag, kv.war_flag, e.epoch,
+                               kv.max_scan, b.request_major);
+        else
+            hipLaunchKernelGGL(kv_mark_reads_kernel<1>, op_grid, dim3(kBlock), 0, s, b.keys, b.meta, b.tids,
+                               b.n_txns, b.n_slots, kv.rows, kv.fields, kv.regs, reg_layout(kv), kv.hot_row, e.tag,
+                               kv.raw_flag, kv.war_flag, e.epoch, kv.max_scan, b.request_major);
+    }
     return hipGetLastError();
 }
 

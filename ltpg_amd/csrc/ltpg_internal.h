@@ -97,6 +97,7 @@ struct KvState {
     uint32_t small_chunks = 0;
     uint32_t lane_copy = 0;
     uint32_t small_epoch = 0;           // epochs of <= LTPG_SMALL_EPOCH_OPS op slots in one kernel
+    uint32_t more_items = 0;            // per-op EXECUTE and read marking: twice the ops per lane
     // row-partitioned EXECUTE (fast form): the epoch's register operations
     // binned by row range, each range's W minima kept in LDS; buffers
     // allocated when the shape is first selected (kv_part_prepare)
@@ -208,6 +209,7 @@ struct TpccState {
     ReadOnlyDev ro;
     DeliveryDev dl;
     bool has_batch = false;
+    bool small_blocks = false;                  // LTPG_TUNE_TPCC_SMALL_BLOCKS
 };
 
 }  // namespace ltpg
@@ -413,9 +415,10 @@ hipError_t launch_tpcc_populate(const TpccDev &d, uint64_t seed, hipStream_t s);
 hipError_t launch_tpcc_name_index(const TpccDev &d, hipStream_t s);
 hipError_t launch_tpcc_relayout(const uint32_t *src, uint32_t *dst, uint64_t rows, uint32_t cols,
                                 uint32_t stride, uint32_t layout, int to_device, hipStream_t s);
-hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t s);
+// `small`: 64-thread blocks (LTPG_TUNE_TPCC_SMALL_BLOCKS) instead of 256.
+hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, bool small, hipStream_t s);
 hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
-                               const EpochTag &e, bool counts, DeviceCounters *ctr, hipStream_t s);
+                               const EpochTag &e, bool counts, bool small, DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_t base, const EpochTag &e,
                                 DeviceCounters *ctr, hipStream_t s);
 // Deliveries (base = their first batch position): EXECUTE registrations,
@@ -427,8 +430,8 @@ hipError_t launch_tpcc_delivery_check(const TpccDev &d, const DeliveryDev &dl, u
 hipError_t launch_tpcc_delivery_writeback(const TpccDev &d, const DeliveryDev &dl, uint32_t base, const EpochTag &e,
                                           uint32_t values, hipStream_t s);
 hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                             bool counts, bool ro, bool dl, DeviceCounters *ctr, hipStream_t s);
+                             bool counts, bool ro, bool dl, bool small, DeviceCounters *ctr, hipStream_t s);
 hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay,
-                                 const EpochTag &e, uint32_t values, hipStream_t s);
+                                 const EpochTag &e, uint32_t values, bool small, hipStream_t s);
 
 }  // namespace ltpg

@@ -185,9 +185,10 @@ __global__ __launch_bounds__(kBlock) void relayout_kernel(const uint32_t *__rest
 
 // C_ID of the k-th (0-based, k = C_LAST & 0xff) customer of district
 // C_W_ID * 10 + C_D_ID whose C_LAST is C_LAST >> 8; 0 when there is none.
-__global__ __launch_bounds__(kBlock) void cid_kernel(TpccDev d, PaymentDev pay)
+template <uint32_t BLK>
+__global__ __launch_bounds__(BLK) void cid_kernel(TpccDev d, PaymentDev pay)
 {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t p = blockIdx.x * BLK + threadIdx.x;
     if (p >= pay.n) return;
     uint32_t c = pay.C_ID[p];
     if (pay.isName[p]) {
@@ -259,22 +260,22 @@ __device__ __forceinline__ unsigned long long *cust_w(const TpccDev &d, uint64_t
 // ---- EXECUTE ---------------------------------------------------------------
 
 // NewOrders take 16 adjacent lanes each (lane j < 15: line j, lane 15: the
-// header slots), 16 orders per block: an order's consecutive ORDERLINE
+// header slots), BLK / 16 orders per block: an order's consecutive ORDERLINE
 // registers are touched by adjacent lanes, and all its roles share a CU.
 // Payments take one lane each, in the blocks after the NewOrder blocks.
-constexpr uint32_t kNoPerBlock = kBlock / 16;
-
-template <bool COUNTS>
-__global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
+// BLK is 256, or 64 for the small-blocks launch shape (LTPG_TUNE_TPCC_SMALL_BLOCKS),
+// which spreads a small batch over four times as many CUs.
+template <bool COUNTS, uint32_t BLK>
+__global__ __launch_bounds__(BLK) void execute_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
                                                         uint8_t epoch, uint32_t no_blocks, DeviceCounters *ctr)
 {
     // the epoch's commit counts start at zero (CHECK adds to them)
     if (blockIdx.x == 0)
-        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += kBlock)
+        for (uint32_t c = threadIdx.x; c < kCounterStripes * kCountedTypes; c += BLK)
             ctr->commits[c / kCountedTypes][c % kCountedTypes] = 0;
     uint32_t bad = 0;
     if (blockIdx.x < no_blocks) {
-        const uint32_t q = blockIdx.x * kNoPerBlock + threadIdx.x / 16;
+        const uint32_t q = blockIdx.x * (BLK / 16) + threadIdx.x / 16;
         const uint32_t j = threadIdx.x % 16;
         if (q < no.n) {
             const unsigned long long me = priority(tag, no.TID[q], q);
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void execute_kernel(TpccDev d, NeworderDev 
             if (waw) d.doomed[q] = epoch;
         }
     } else {
-        const uint32_t p = (blockIdx.x - no_blocks) * kBlock + threadIdx.x;
+        const uint32_t p = (blockIdx.x - no_blocks) * BLK + threadIdx.x;
         if (p < pay.n) {
             const unsigned long long me = priority(tag, pay.TID[p], no.n + p);
             const PayRows r = pay_rows(d, pay, p);
@@ -480,15 +481,15 @@ __global__ __launch_bounds__(kBlock) void readonly_kernel(TpccDev d, ReadOnlyDev
     }
 }
 
-template <bool COUNTS>
-__global__ __launch_bounds__(kBlock) void check_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
+template <bool COUNTS, uint32_t BLK>
+__global__ __launch_bounds__(BLK) void check_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint32_t tag,
                                                       uint8_t epoch, uint32_t no_waves, bool ro, bool dl,
                                                       DeviceCounters *ctr)
 {
     __shared__ uint32_t s_cnt[4];  // NewOrder commits, attempts, Payment commits, attempts
     if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / 64;
+    const uint32_t wave = blockIdx.x * (BLK / 64) + threadIdx.x / 64;
     const uint32_t lane = threadIdx.x % 64;
     bool commit = false, live = false;
     const bool is_no = wave < no_waves;
@@ -734,14 +735,15 @@ __global__ __launch_bounds__(kBlock) void delivery_writeback_kernel(TpccDev d, D
 // the rows are rewritten unchanged and the YTD update is not applied.  The
 // committed writer of a row is unique (waw), so every row read here still
 // holds its snapshot value, apart from a NewOrder's own earlier lines.
-__global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint8_t epoch,
+template <uint32_t BLK>
+__global__ __launch_bounds__(BLK) void writeback_kernel(TpccDev d, NeworderDev no, PaymentDev pay, uint8_t epoch,
                                                           uint32_t no_blocks, int rmw)
 {
     if (blockIdx.x < no_blocks) {
         // 16 lanes per NewOrder, as in EXECUTE: lane j < 15 writes line j,
         // lane 15 the NEWORDER and ORDER rows.  The group of an order is
         // active or idle as a whole, so the shuffles below read live lanes.
-        const uint32_t q = blockIdx.x * kNoPerBlock + threadIdx.x / 16;
+        const uint32_t q = blockIdx.x * (BLK / 16) + threadIdx.x / 16;
         const uint32_t j = threadIdx.x % 16;
         if (q >= no.n || !((d.bitmap_no[q / 64] >> (q % 64)) & 1ull)) return;
         const uint32_t oid = no.O_ID[q], did = no.D_ID[q], wid = no.W_ID[q];
@@ -812,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void writeback_kernel(TpccDev d, NeworderDe
             }
         }
     } else {
-        const uint32_t p = (blockIdx.x - no_blocks) * kBlock + threadIdx.x;
+        const uint32_t p = (blockIdx.x - no_blocks) * BLK + threadIdx.x;
         if (p >= pay.n || !((d.bitmap_pay[p / 64] >> (p % 64)) & 1ull)) return;
         const PayRows r = pay_rows(d, pay, p);
         const uint32_t amt = pay.H_AMOUNT[p];
@@ -919,24 +921,35 @@ hipError_t launch_tpcc_relayout(const uint32_t *src, uint32_t *dst, uint64_t row
     return hipGetLastError();
 }
 
-hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, hipStream_t s)
+hipError_t launch_tpcc_cid(const TpccDev &d, const PaymentDev &pay, bool small, hipStream_t s)
 {
     if (pay.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(cid_kernel, dim3((pay.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, pay);
+    if (small)
+        hipLaunchKernelGGL(cid_kernel<64>, dim3((pay.n + 63) / 64), dim3(64), 0, s, d, pay);
+    else
+        hipLaunchKernelGGL(cid_kernel<kBlock>, dim3((pay.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d, pay);
     return hipGetLastError();
 }
 
-hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                               bool counts, DeviceCounters *ctr, hipStream_t s)
+template <uint32_t BLK>
+void execute_at(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e, bool counts,
+                DeviceCounters *ctr, hipStream_t s)
 {
-    const uint32_t no_blocks = (no.n + kNoPerBlock - 1) / kNoPerBlock;
-    const uint32_t blocks = std::max(1u, no_blocks + (pay.n + kBlock - 1) / kBlock);
+    const uint32_t no_blocks = (no.n + BLK / 16 - 1) / (BLK / 16);
+    const uint32_t blocks = std::max(1u, no_blocks + (pay.n + BLK - 1) / BLK);
     if (counts)
-        hipLaunchKernelGGL(execute_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+        hipLaunchKernelGGL((execute_kernel<true, BLK>), dim3(blocks), dim3(BLK), 0, s, d, no, pay, e.tag, e.epoch,
                            no_blocks, ctr);
     else
-        hipLaunchKernelGGL(execute_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+        hipLaunchKernelGGL((execute_kernel<false, BLK>), dim3(blocks), dim3(BLK), 0, s, d, no, pay, e.tag, e.epoch,
                            no_blocks, ctr);
+}
+
+hipError_t launch_tpcc_execute(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                               bool counts, bool small, DeviceCounters *ctr, hipStream_t s)
+{
+    if (small) execute_at<64>(d, no, pay, e, counts, ctr, s);
+    else execute_at<kBlock>(d, no, pay, e, counts, ctr, s);
     return hipGetLastError();
 }
 
@@ -950,18 +963,26 @@ hipError_t launch_tpcc_readonly(const TpccDev &d, const ReadOnlyDev &ro, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                             bool counts, bool ro, bool dl, DeviceCounters *ctr, hipStream_t s)
+template <uint32_t BLK>
+void check_at(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e, bool counts, bool ro,
+              bool dl, DeviceCounters *ctr, hipStream_t s)
 {
     const uint32_t no_waves = waves64(no.n), waves = no_waves + waves64(pay.n);
-    if (waves == 0) return hipSuccess;
-    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+    const uint32_t blocks = (waves + BLK / 64 - 1) / (BLK / 64);
     if (counts)
-        hipLaunchKernelGGL(check_kernel<true>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+        hipLaunchKernelGGL((check_kernel<true, BLK>), dim3(blocks), dim3(BLK), 0, s, d, no, pay, e.tag, e.epoch,
                            no_waves, ro, dl, ctr);
     else
-        hipLaunchKernelGGL(check_kernel<false>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.tag, e.epoch,
+        hipLaunchKernelGGL((check_kernel<false, BLK>), dim3(blocks), dim3(BLK), 0, s, d, no, pay, e.tag, e.epoch,
                            no_waves, ro, dl, ctr);
+}
+
+hipError_t launch_tpcc_check(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
+                             bool counts, bool ro, bool dl, bool small, DeviceCounters *ctr, hipStream_t s)
+{
+    if (waves64(no.n) + waves64(pay.n) == 0) return hipSuccess;
+    if (small) check_at<64>(d, no, pay, e, counts, ro, dl, ctr, s);
+    else check_at<kBlock>(d, no, pay, e, counts, ro, dl, ctr, s);
     return hipGetLastError();
 }
 
@@ -993,13 +1014,18 @@ hipError_t launch_tpcc_delivery_writeback(const TpccDev &d, const DeliveryDev &d
 }
 
 hipError_t launch_tpcc_writeback(const TpccDev &d, const NeworderDev &no, const PaymentDev &pay, const EpochTag &e,
-                                 uint32_t values, hipStream_t s)
+                                 uint32_t values, bool small, hipStream_t s)
 {
-    const uint32_t no_blocks = (no.n + kNoPerBlock - 1) / kNoPerBlock;
-    const uint32_t blocks = no_blocks + (pay.n + kBlock - 1) / kBlock;
+    const uint32_t blk = small ? 64 : kBlock;
+    const uint32_t no_blocks = (no.n + blk / 16 - 1) / (blk / 16);
+    const uint32_t blocks = no_blocks + (pay.n + blk - 1) / blk;
     if (blocks == 0) return hipSuccess;
     const int rmw = values == LTPG_VALUES_RMW ? 1 : 0;
-    hipLaunchKernelGGL(writeback_kernel, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_blocks, rmw);
+    if (small)
+        hipLaunchKernelGGL(writeback_kernel<64>, dim3(blocks), dim3(64), 0, s, d, no, pay, e.epoch, no_blocks, rmw);
+    else
+        hipLaunchKernelGGL(writeback_kernel<kBlock>, dim3(blocks), dim3(kBlock), 0, s, d, no, pay, e.epoch, no_blocks,
+                           rmw);
     if (rmw && pay.n) {
         const uint32_t chunk = std::max<uint32_t>(4096, (pay.n + 255) / 256);
         hipLaunchKernelGGL(ytd_kernel, dim3((pay.n + chunk - 1) / chunk), dim3(kYtdBlock), 0, s, d, pay, chunk);

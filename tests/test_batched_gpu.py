@@ -266,7 +266,8 @@ def test_empty_batches_change_nothing(gpu, mode):
 @pytest.mark.parametrize("rm", [False, True], ids=["slot-major", "request-major"])
 @pytest.mark.parametrize("fw", [2, 25], ids=["8B", "100B"])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
-@pytest.mark.parametrize("flags", list(range(16)) + [L.TUNE_PARTITIONED | f for f in (0, 2, 4, 8, 14)])
+@pytest.mark.parametrize("flags", list(range(16)) + [L.TUNE_MORE_ITEMS | f for f in (0, 1, 3, 5, 15)] +
+                         [L.TUNE_PARTITIONED | f for f in (0, 2, 4, 8, 14)])
 def test_kv_launch_shapes_match_oracle(gpu, flags, hot_row, fw, rm):
     """Every launch shape against the oracle, batches in either layout;
     100-byte fields (stride 32 words) take the per-lane copy under
@@ -309,7 +310,17 @@ def test_autotuned_graph_matches_untuned_plain(gpu):
     descs1, keep1, _ = _batches(np.random.default_rng(11), 30, kv1, tp1)
     tuned = L.Batched(kv1, tp1, descs1)
     table = tuned.autotune(reps=2)
-    assert table and all(len(e["ms"]) == 32 and min(e["ms"]) > 0 for e in table)
+    assert {e["workload"] for e in table} == {L.WORKLOAD_KV, L.WORKLOAD_TPCC}
+    for e in table:
+        assert len(e["ms"]) == 64
+        if e["workload"] == L.WORKLOAD_KV:
+            tried = [L.tune_index(f) for f in L.kv_shapes()]
+            assert e["flags"] in L.kv_shapes()
+        else:
+            tried = [0, 1]
+            assert e["flags"] in (0, L.TUNE_TPCC_SMALL_BLOCKS)
+        assert all(e["ms"][k] > 0 for k in tried), e
+        assert all(e["ms"][k] == 0 for k in range(64) if k not in tried), e
     tuned.run(L.RUN_GRAPH)
     got = _bitmaps(tuned, meta)
     for a, b in zip(want, got):
@@ -385,13 +396,17 @@ def _partitioned_epochs(rows, theta, hot_row, scan):
     np.testing.assert_array_equal(ctx.kv_table_read(), table)
 
 
-@pytest.mark.parametrize("flags", [L.TUNE_PARTITIONED, L.TUNE_PARTITIONED | L.TUNE_SPLIT_REGISTERS | 4],
-                         ids=["partitioned", "partitioned-split-small"])
+@pytest.mark.parametrize("flags", [L.TUNE_PARTITIONED, L.TUNE_PARTITIONED | L.TUNE_SPLIT_REGISTERS | 4,
+                                   L.TUNE_MORE_ITEMS | L.TUNE_EXEC_PER_OP | L.TUNE_TPCC_SMALL_BLOCKS,
+                                   L.TUNE_PARTITIONED | L.TUNE_TPCC_SMALL_BLOCKS],
+                         ids=["partitioned", "partitioned-split-small", "more-items-small-blocks",
+                              "partitioned-small-blocks"])
 def test_graph_form_with_partitioned_execute_matches_plain_loop(gpu, flags):
-    """Every key-value batch on the row-partitioned EXECUTE in the graph
-    form (EXECUTE of batch j + 1 overlapping CHECK of batch j, two register
-    sets, one partition scratch): the default-shape plain loop's commit sets
-    and tables."""
+    """One launch shape for every batch in the graph form -- the
+    row-partitioned EXECUTE (EXECUTE of batch j + 1 overlapping CHECK of
+    batch j, two register sets, one partition scratch), more items per lane,
+    64-thread order-entry workgroups: the default-shape plain loop's commit
+    sets and tables."""
     kv0, tp0 = _contexts()
     descs0, keep0, meta = _batches(np.random.default_rng(13), 30, kv0, tp0)
     plain = L.Batched(kv0, tp0, descs0)

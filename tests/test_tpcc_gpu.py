@@ -112,6 +112,30 @@ def test_seeded_epochs_match_oracle(gpu, seed, rules, values, counts):
     _assert_tables(ctx, tables)
 
 
+@pytest.mark.parametrize("rules", [L.RULES_REFERENCE, L.RULES_INTENDED], ids=["reference", "intended"])
+@pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
+def test_small_blocks_shape_matches_oracle(gpu, rules, counts):
+    """TUNE_TPCC_SMALL_BLOCKS (64-thread workgroups: 4 NewOrders or 64
+    Payments each) against the oracle, switched on and off between epochs;
+    odd batch sizes leave partial workgroups of both roles."""
+    W = 2
+    ctx = _ctx(W, 8192, rules, L.VALUES_RMW, counts)
+    ctx.tpcc_populate(SEED)
+    tables = oracle.tpcc_populate(W, SEED)
+    ora = oracle.TpccOracle(W)
+    gen = L.TpccGenerator(warehouses=W, seed=321)
+    for ep, n in enumerate((3001, 777, 64, 2050)):
+        ctx.tpcc_set_tuning(L.TUNE_TPCC_SMALL_BLOCKS if ep != 2 else 0)
+        b = gen.next(n, tid_base=ep * 4000)
+        g = _gpu_epoch(ctx, b)
+        o = ora.epoch(tables, b, rules, L.VALUES_RMW)
+        _assert_epoch(g, o, b)
+    _assert_tables(ctx, tables)
+    with pytest.raises(L.LtpgError):
+        ctx.tpcc_set_tuning(L.TUNE_PARTITIONED)
+    ctx.close()
+
+
 @pytest.mark.parametrize("counts", [True, False], ids=["counting", "fast"])
 def test_empty_and_one_sided_epochs_match_oracle(gpu, counts):
     """Epochs with no transactions, only NewOrders or only Payments between
