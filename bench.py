@@ -124,24 +124,17 @@ def kv_kernel_bytes(keys, meta, tids, bitmap, B, S, fw, rows=None, max_scan=0):
     return {"execute": execute, "check": check, "writeback": writeback}, committed
 
 
-def shape_flags(k):
-    """LTPG_TUNE_* flags of launch shape k (0-31): k & 15, with the
-    row-partitioned EXECUTE for k >= 16."""
+def choose_kv_shape(ctx, batch, reps=3):
+    """Time the distinct key-value launch shapes (ltpg_amd.kv_shapes(): the
+    LTPG_TUNE_* sets of per-op execute with block-local row folding, split
+    register arrays, 64-transaction chunks, per-lane field copies and more
+    items per lane, and the row-partitioned EXECUTE with the ones it uses)
+    on one batch, one warm-up and `reps` epochs each, and keep the fastest for
+    the run.  The timed loop is the plain per-batch loop either way."""
     import ltpg_amd as L
 
-    return (k & 15) | (L.TUNE_PARTITIONED if k >= 16 else 0)
-
-
-def choose_kv_shape(ctx, batch, reps=3):
-    """Time the 32 launch shapes (LTPG_TUNE_* flag sets: per-op execute with
-    block-local row folding, split register arrays, 64-transaction chunks,
-    per-lane field copies, each with and without the row-partitioned
-    EXECUTE) on one batch, one warm-up and `reps` epochs each, and keep the
-    fastest for the run.  The timed loop is the plain per-batch loop either
-    way."""
-    times = []
-    for k in range(32):
-        flags = shape_flags(k)
+    times = {}
+    for flags in L.kv_shapes():
         ctx.kv_set_tuning(flags)
         ctx.kv_submit_device(batch)
         ctx.epoch_run()
@@ -151,10 +144,10 @@ def choose_kv_shape(ctx, batch, reps=3):
             ctx.kv_submit_device(batch)
             ctx.epoch_run()
         ctx.sync()
-        times.append((time.perf_counter() - t0) * 1e3 / reps)
-    best = shape_flags(int(np.argmin(times)))
+        times[flags] = (time.perf_counter() - t0) * 1e3 / reps
+    best = min(times, key=times.get)
     ctx.kv_set_tuning(best)
-    return best, times
+    return best, {str(k): v for k, v in times.items()}
 
 
 def cpu_baseline_kv(w, keys, meta, tids, budget_s=10.0):
@@ -247,7 +240,7 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
         per_batch.append(kv_kernel_bytes(gen[i][0], gen[i][1], gen[i][2], ctx.commit_bitmap(), B, S, fw, rows,
                                          scan))
 
-    shape, shape_ms = choose_kv_shape(ctx, dev[0]) if not args.default_shape else (0, [])
+    shape, shape_ms = choose_kv_shape(ctx, dev[0]) if not args.default_shape else (0, {})
     log(f"launch shape flags {shape}")
 
     for k in range(args.warmup):
@@ -386,8 +379,7 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
                    "batch_txns": B, "ops_per_txn": S, "zipf_theta": w["theta"], "form": "plain per-batch",
                    "inputs": "resident in HBM", "distinct_batches": P,
                    "launch_shape": {"flags": shape, "epoch_ms_by_flags": shape_ms,
-                                    "chosen_by": "fastest of the 32 LTPG_TUNE_* shapes (index k: flags k & 15, | PARTITIONED for k >= 16) "
-                                                 "on the first batch, "
+                                    "chosen_by": "fastest of the 40 distinct LTPG_TUNE_* shapes on the first batch, "
                                                  "before the timed region"}},
         "abort_rate": 1.0 - commits / max(attempts, 1),
         "attempted_txns": attempts,
@@ -1145,6 +1137,9 @@ def compact(v, name):
             out[k] = v[k]
     if "latency_ms" in v:
         out["latency_ms"] = {k: v["latency_ms"][k] for k in v["latency_ms"] if k != "what"}
+    shape = v.get("config", {}).get("launch_shape")
+    if shape:
+        out["launch_shape_flags"] = shape["flags"]
     if "roofline" in v:
         out["roofline"] = {k: v["roofline"].get(k) for k in ROOF_KEYS}
         ra = v["roofline"].get("random_access")
@@ -1207,7 +1202,7 @@ def main():
     ap.add_argument("--batches", type=int, default=2, help="distinct device-resident batches cycled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--default-shape", action="store_true",
-                    help="key-value plain runs keep the default launch shape instead of the fastest of the 16")
+                    help="key-value plain runs keep the default launch shape instead of the fastest distinct one")
     ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
                     help="file for every figure behind the line (empty: none)")
     ap.add_argument("--copy-calibration", type=float, default=None, metavar="GIB",
