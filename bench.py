@@ -328,7 +328,8 @@ def run_kv(args, theta, cpu_baseline=True, name=None, hot_row=None):
     # counted HBM bytes exist for the workloads profiled with --pmc
     traffic_src = {("kv-c2", 0.6): "kv_c2_traffic.json", ("kv-y1", 0.0): "kv_y1_traffic.json"}.get(
         (name or args.workload, w["theta"]))
-    counted = {k: measured_traffic(k, source=traffic_src) for k in KERNELS}
+    counted = {k: measured_traffic(k, source=traffic_src, partitioned=bool(shape & L.TUNE_PARTITIONED))
+               for k in KERNELS}
     # EXECUTE is bound by random register accesses, not bytes: one probe +
     # atomicMin per write op and one W load per read op, against the rates
     # this device sustains for them over an array of the registers' size
@@ -1091,17 +1092,19 @@ def _tpcc_select(b, idx):
     return out, perm
 
 
-def measured_traffic(phase, family="kv", source=None):
+def measured_traffic(phase, family="kv", source=None, partitioned=False):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc
     summary of the same workload (profiles/<source>), or None when none
-    exists for it."""
+    exists for it.  The row-partitioned EXECUTE has its own entry
+    (<family>_execute_partitioned)."""
     if source is None:
         source = "tpcc_c3_traffic.json" if family == "tpcc" else None
     if source is None:
         return None
+    key = f"{family}_{phase}" + ("_partitioned" if partitioned and phase == "execute" else "")
     try:
         with open(os.path.join(ROOT, "profiles", source)) as f:
-            return json.load(f).get(f"{family}_{phase}")
+            return json.load(f).get(key)
     except (OSError, ValueError):
         return None
 
