@@ -14,10 +14,11 @@ the dominant kernel (the device loop, per epoch) and the CPU baseline (the
 oracle replaying the served key-value lane on one host core).
 
 Compact variants on the same line: the key-value plain form (10M rows x 10
-x 100 B, 2^20 txns x 16 ops, Zipf 0.6 and 0.99), the order-entry plain form
-(W = 64, 2^18 txns) and the batched form (256 mixed batches of 1e3-1e6
-txns) against the plain loop, each with its roofline.  Every figure behind
-the line goes to --detail (gpurun_out/bench_detail.json).
+x 100 B, 2^20 txns x 16 ops, Zipf 0.6 and 0.99; and the YCSB/1 mix of 9
+reads + 1 RMW on uniform keys, where CHECK and WRITEBACK carry volume), the
+order-entry plain form (W = 64, 2^18 txns) and the batched form (256 mixed
+batches of 1e3-1e6 txns) against the plain loop, each with its roofline.
+Every figure behind the line goes to --detail (gpurun_out/bench_detail.json).
 """
 import argparse
 import json
