@@ -2198,8 +2198,26 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (t < B) s_cbefore[t] = before[k];
             if (c[k]) s_list[before[k]] = t;
         }
-        lds_sync();
+        // With kept reads the read-out below loads the table, so the last
+        // epoch's write-back atomics must have completed: a full barrier
+        // (every wave waits for its memory operations).  Without, nothing in
+        // the loop reads the table and an LDS barrier suffices.
+        if (reads)
+            __syncthreads();
+        else
+            lds_sync();
         stamp(4);
+        // the epoch's record, stored now by wave 0 (which takes no part in
+        // the write-back below) so that its PCIe writes are done by the
+        // time done_tail is released
+        if (threadIdx.x < 64) {
+            if (threadIdx.x == 0)
+                done[done_tail & done_mask] = (uint64_t)B | ((uint64_t)nc << 16) | (epoch << 32);
+            for (uint32_t t = threadIdx.x; t < B; t += 64) {
+                const bool ct = !s_waw[t] && !(s_raw[t] && s_war[t]);
+                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (ct ? kDoneCommitted : 0ull);
+            }
+        }
         // read-out (when kept) and write-back; reads see the snapshot, so
         // every read completes before the first write-back store
         const uint32_t per = L * fw;
@@ -2252,35 +2270,31 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
             // every read of the snapshot is done before the first write
             if (total_r) __syncthreads();
-            for (uint32_t x = threadIdx.x; x < total_w; x += kSmallBlock) {
-                uint64_t off;
-                uint32_t add;
-                if (write_item(x, off, add)) atomicAdd(&table[off], add);
-            }
+            // waves 1.. issue the write-back atomics and do not wait for
+            // them: the next wait is the next epoch's full barrier before its
+            // read-out (kept reads), else the end of the launch
+            if (threadIdx.x >= 64)
+                for (uint32_t x = threadIdx.x - 64; x < total_w; x += kSmallBlock - 64) {
+                    uint64_t off;
+                    uint32_t add;
+                    if (write_item(x, off, add)) atomicAdd(&table[off], add);
+                }
         }
         // committed reads went to host memory from every wave
         if (reads && total_r) __threadfence_system();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 64) {
             add_counts(a.ctr, (uint32_t)(epoch % kCounterStripes), LTPG_TYPE_KV, nc, B);
             if (s_bad) atomicAdd(&a.ctr->range_errors, s_bad);
         }
-        // a full barrier: the epoch's atomics are complete before the next
-        // epoch's coherent reads (by then the prefetch has long landed)
-        __syncthreads();
+        lds_sync();
         stamp(6);
-        // the epoch's record: one wave writes it, then releases done_tail
-        if (threadIdx.x < 64) {
-            if (threadIdx.x == 0)
-                done[done_tail & done_mask] = (uint64_t)B | ((uint64_t)nc << 16) | (epoch << 32);
-            for (uint32_t t = threadIdx.x; t < B; t += 64) {
-                const bool ct = !s_waw[t] && !(s_raw[t] && s_war[t]);
-                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (ct ? kDoneCommitted : 0ull);
-            }
+        // release the record (and the kept reads, fenced above by every
+        // wave before the barrier): wave 0's own writes are only the record
+        if (threadIdx.x == 0) {
             __threadfence_system();
-            if (threadIdx.x == 0) sys_store(&ctl->done_tail, done_tail + 1 + B);
+            sys_store(&ctl->done_tail, done_tail + 1 + B);
         }
         done_tail += 1 + B;
-        lds_sync();
         stamp(7);
         // requeue: the aborted requests move to the front, in position order
         // (everything is re-read from LDS: nothing of the phases above stays
