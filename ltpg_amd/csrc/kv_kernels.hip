@@ -1945,7 +1945,10 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __shared__ uint64_t s_st_head;
     __shared__ uint32_t s_st_n;
     __shared__ uint64_t s_busy;  // wall-clock ticks inside epochs
-    __shared__ uint64_t s_ctl[4];
+    // control words, double-buffered: thread 0 fills the half the loop is
+    // not reading, and every wave switches halves after the next barrier,
+    // so reading them needs no barrier of its own
+    __shared__ uint64_t s_ctl[2][4];
     __shared__ uint32_t s_bad;
 
     // kernel arguments as locals: the lambdas below capture by reference,
@@ -1962,15 +1965,16 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     // give the same answer in every wave)
     __shared__ uint64_t s_start;
     if (threadIdx.x == 0) {
-        s_ctl[0] = sys_load(&ctl->head);
-        s_ctl[1] = sys_load(&ctl->done_tail);
-        s_ctl[2] = sys_load(&ctl->epochs);
-        s_ctl[3] = (sys_load(&ctl->n_carry) & 0xffffffffull) | (sys_load(&ctl->cap) << 32);
+        s_ctl[0][0] = sys_load(&ctl->head);
+        s_ctl[0][1] = sys_load(&ctl->done_tail);
+        s_ctl[0][2] = sys_load(&ctl->epochs);
+        s_ctl[0][3] = (sys_load(&ctl->n_carry) & 0xffffffffull) | (sys_load(&ctl->cap) << 32);
         s_start = wall_clock64();
     }
     __syncthreads();
-    uint64_t head = s_ctl[0], done_tail = s_ctl[1], epoch = s_ctl[2];
-    uint32_t n_carry = (uint32_t)(s_ctl[3] & 0xffffffffull), cap = (uint32_t)(s_ctl[3] >> 32);
+    uint64_t head = s_ctl[0][0], done_tail = s_ctl[0][1], epoch = s_ctl[0][2];
+    uint32_t n_carry = (uint32_t)(s_ctl[0][3] & 0xffffffffull), cap = (uint32_t)(s_ctl[0][3] >> 32);
+    uint32_t cp = 0;  // the half of s_ctl being read
     const uint64_t t_start = s_start;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
@@ -2001,12 +2005,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     // values at most one epoch old without waiting for them; only an idle or
     // stalled loop waits for a fresh read.  (Ring entries are read only below
     // the tail so read: the host writes them before it advances the tail.)
+    // (fills the other half; the caller switches after a barrier)
     const auto poll = [&]() {
         if (threadIdx.x == 0) {
-            s_ctl[0] = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            s_ctl[1] = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            s_ctl[2] = __hip_atomic_load(&ctl->done_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            s_ctl[3] = wall_clock64();
+            uint64_t *c = s_ctl[cp ^ 1u];
+            c[0] = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            c[1] = __hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            c[2] = __hip_atomic_load(&ctl->done_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            c[3] = wall_clock64();
         }
     };
     poll();
@@ -2016,9 +2022,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         s_busy = 0;
     }
     __syncthreads();
+    cp ^= 1u;
     while (true) {
-        const uint64_t tail = s_ctl[0], stop = s_ctl[1], done_head = s_ctl[2], now = s_ctl[3];
-        lds_sync();
+        const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2], now = s_ctl[cp][3];
         if (stop) {
             code = kServeExitStop;
             break;
@@ -2039,6 +2045,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             __builtin_amdgcn_s_sleep(32);
             poll();
             lds_sync();
+            cp ^= 1u;
             stamp(9);
             continue;
         }
@@ -2050,6 +2057,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             __builtin_amdgcn_s_sleep(32);
             poll();
             lds_sync();
+            cp ^= 1u;
             stamp(9);
             continue;
         }
@@ -2183,20 +2191,36 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         }
         lds_sync();
         stamp(3);
-        // CHECK: commit bits and the committed list in position order
-        bool c[kSmallPer];
-        uint32_t before[kSmallPer];
+        // CHECK: commit bits and the committed list in position order.  An
+        // epoch of at most 64 transactions (the common case under the
+        // commit-based cap) is counted by wave 0 alone with one ballot.
+        uint32_t nc;
+        if (B <= 64) {
+            if (threadIdx.x < 64) {
+                const uint32_t t = threadIdx.x;
+                const bool ct = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
+                const unsigned long long m = __ballot(ct);
+                const uint32_t bf = (uint32_t)__popcll(m & (t ? (~0ull >> (64 - t)) : 0ull));
+                if (t < B) s_cbefore[t] = bf;
+                if (ct) s_list[bf] = t;
+                if (t == 0) s_w[0] = (uint32_t)__popcll(m);
+            }
+        } else {
+            bool c[kSmallPer];
+            uint32_t before[kSmallPer];
 #pragma unroll
-        for (uint32_t k = 0; k < kSmallPer; ++k) {
-            const uint32_t t = k * kSmallBlock + threadIdx.x;
-            c[k] = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
-        }
-        const uint32_t nc = block_count_before(c, before, s_w);
+            for (uint32_t k = 0; k < kSmallPer; ++k) {
+                const uint32_t t = k * kSmallBlock + threadIdx.x;
+                c[k] = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
+            }
+            const uint32_t total = block_count_before(c, before, s_w);
 #pragma unroll
-        for (uint32_t k = 0; k < kSmallPer; ++k) {
-            const uint32_t t = k * kSmallBlock + threadIdx.x;
-            if (t < B) s_cbefore[t] = before[k];
-            if (c[k]) s_list[before[k]] = t;
+            for (uint32_t k = 0; k < kSmallPer; ++k) {
+                const uint32_t t = k * kSmallBlock + threadIdx.x;
+                if (t < B) s_cbefore[t] = before[k];
+                if (c[k]) s_list[before[k]] = t;
+            }
+            if (threadIdx.x == 0) s_w[0] = total;
         }
         // With kept reads the read-out below loads the table, so the last
         // epoch's write-back atomics must have completed: a full barrier
@@ -2206,6 +2230,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             __syncthreads();
         else
             lds_sync();
+        nc = s_w[0];
         stamp(4);
         // the epoch's record, stored now by wave 0 (which takes no part in
         // the write-back below) so that its PCIe writes are done by the
@@ -2334,11 +2359,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         ++epoch;
         t_work = now;
         if (threadIdx.x == 0) {
-            s_ctl[0] = c_tail;
-            s_ctl[1] = c_stop;
-            s_ctl[2] = c_dh;
-            s_ctl[3] = wall_clock64();
-            s_busy += s_ctl[3] - now;
+            uint64_t *cn = s_ctl[cp ^ 1u];
+            cn[0] = c_tail;
+            cn[1] = c_stop;
+            cn[2] = c_dh;
+            cn[3] = wall_clock64();
+            s_busy += cn[3] - now;
             s_st_head = head;
             s_st_n = pf_n;
         }
@@ -2351,6 +2377,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
         }
         lds_sync();
+        cp ^= 1u;
         stamp(8);
         if (prof && threadIdx.x == 0) s_prof[kServeProfParts + 1] += 1;
     }
