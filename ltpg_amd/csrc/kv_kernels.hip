@@ -1929,8 +1929,10 @@ __device__ __forceinline__ uint32_t block_count_before(const bool (&f)[kSmallPer
 
 __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArgs a)
 {
-    __shared__ uint64_t s_row[kSmallOps];
-    __shared__ uint32_t s_meta[kSmallOps];
+    // op arrays, double-buffered: the requeue writes the carried requests
+    // into the half the next epoch reads while this epoch's half is read
+    __shared__ uint64_t s_rowb[2][kSmallOps];
+    __shared__ uint32_t s_metab[2][kSmallOps];
     __shared__ uint64_t s_seq[kLoopMaxT];            // arrival sequence number by position
     __shared__ uint32_t s_key[kSmallSlots];          // row + 1 of a written row, 0 = empty
     __shared__ uint32_t s_min[kSmallSlots];          // its writers' minimum position
@@ -1975,11 +1977,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     uint64_t head = s_ctl[0][0], done_tail = s_ctl[0][1], epoch = s_ctl[0][2];
     uint32_t n_carry = (uint32_t)(s_ctl[0][3] & 0xffffffffull), cap = (uint32_t)(s_ctl[0][3] >> 32);
     uint32_t cp = 0;  // the half of s_ctl being read
+    uint32_t op = 0;  // the half of s_rowb / s_metab being read
     const uint64_t t_start = s_start;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
-        s_row[i] = a.carry_keys[i];
-        s_meta[i] = a.carry_meta[i];
+        s_rowb[0][i] = a.carry_keys[i];
+        s_metab[0][i] = a.carry_meta[i];
     }
     for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seq[t] = a.carry_seq[t];
     uint64_t t_work = t_start, code = 0;
@@ -2025,6 +2028,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     cp ^= 1u;
     while (true) {
         const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2], now = s_ctl[cp][3];
+        uint64_t *const s_row = s_rowb[op];
+        uint32_t *const s_meta = s_metab[op];
         if (stop) {
             code = kServeExitStop;
             break;
@@ -2340,13 +2345,16 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             ro[k] = abop[k] ? s_row[i] : 0ull;
             mo[k] = abop[k] ? s_meta[i] : 0u;
         }
-        lds_sync();
+        // the ops go to the other half, so only s_seq is compacted in
+        // place: with at most 64 transactions wave 0 alone reads and writes
+        // it (its reads above precede its writes below), else a barrier
+        if (B > 64) lds_sync();
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             if (ab[k]) s_seq[dst_t[k]] = sq[k];
             if (abop[k]) {
-                s_row[dst_i[k]] = ro[k];
-                s_meta[dst_i[k]] = mo[k];
+                s_rowb[op ^ 1u][dst_i[k]] = ro[k];
+                s_metab[op ^ 1u][dst_i[k]] = mo[k];
             }
         }
         n_carry = B - nc;
@@ -2378,13 +2386,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         }
         lds_sync();
         cp ^= 1u;
+        op ^= 1u;
         stamp(8);
         if (prof && threadIdx.x == 0) s_prof[kServeProfParts + 1] += 1;
     }
     // save the carried requests for the next launch (same stream: ordered)
     for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
-        a.carry_keys[i] = s_row[i];
-        a.carry_meta[i] = s_meta[i];
+        a.carry_keys[i] = s_rowb[op][i];
+        a.carry_meta[i] = s_metab[op][i];
     }
     for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) a.carry_seq[t] = s_seq[t];
     __syncthreads();
