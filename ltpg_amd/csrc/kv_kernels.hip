@@ -1927,6 +1927,25 @@ __device__ __forceinline__ uint32_t block_count_before(const bool (&f)[kSmallPer
     return total;
 }
 
+// Unsigned division by a divisor fixed for the kernel's life (ops per
+// request, field words): q = high 64 bits of n * M with M = floor(2^64 / d)
+// + 1 (exactly 2^64 / d for a power of two), exact for every 32-bit n because
+// n * d < 2^64 (Granlund and Montgomery, "Division by invariant integers
+// using multiplication", 1994).  Two 32x32->64 multiplies instead of the
+// ~25-instruction division sequence.
+struct FastDiv {
+    uint64_t m;
+    uint32_t d;
+    __device__ explicit FastDiv(uint32_t d_) : m(d_ > 1 ? ~0ull / d_ + 1ull : 0ull), d(d_) {}
+    __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+        if (d == 1) return n;
+        const uint64_t hi = (uint64_t)n * (uint32_t)(m >> 32);
+        const uint64_t lo = ((uint64_t)n * (uint32_t)m) >> 32;
+        return (uint32_t)((hi + lo) >> 32);
+    }
+};
+
 __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArgs a)
 {
     // op arrays, double-buffered: the requeue writes the carried requests
@@ -1957,6 +1976,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     // and a reference to the argument struct would force it into scratch
     const uint32_t S = a.S, fields = a.fields, fw = a.fw, fsw = a.fsw, L = a.L, max_scan = a.max_scan;
     const uint32_t values = a.values;
+    const FastDiv dS(S), dfw(fw), dper(L * fw);
     const uint64_t rows = a.rows, hot = a.hot, done_mask = a.done_mask;
     ServeCtl *ctl = a.ctl;
     uint32_t *table = a.table;
@@ -2081,7 +2101,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t i = k * kSmallBlock + threadIdx.x;
                 const bool live = i >= c0 && i < n;
-                const uint32_t t = live ? i / S : 0u;
+                const uint32_t t = live ? dS.div(i) : 0u;
                 const uint32_t jn = live ? t - n_carry : 0u;
                 from_ring[k] = live && jn >= staged;
                 const uint64_t at = from_ring[k] ? ((head + jn) & a.ring_mask) * S + (i - t * S) : 0ull;
@@ -2132,7 +2152,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t i = k * kSmallBlock + threadIdx.x;
-            const uint32_t q = i / S;
+            const uint32_t q = dS.div(i);
             if (i < pf_n * S) {
                 const uint64_t at = ((head + q) & a.ring_mask) * S + (i - q * S);
                 pf_r[k] = a.ring_keys[at];
@@ -2163,7 +2183,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 if (prev == 0u || prev == key) break;
                 h = (h + 1) & (kSmallSlots - 1);
             }
-            atomicMin(&s_min[h], i / S);
+            atomicMin(&s_min[h], dS.div(i));
         }
         if (bad) atomicAdd(&s_bad, bad);
         lds_sync();
@@ -2176,7 +2196,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (i >= n) continue;
             const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
             if (!o.valid) continue;
-            const uint32_t t = i / S;
+            const uint32_t t = dS.div(i);
             if (o.kind == LTPG_KV_WRITE) {
                 if (o.row != hot && s_min[small_find(s_key, o.row)] < t) s_waw[t] = 1;
                 continue;
@@ -2255,8 +2275,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         const uint32_t total_w = values == LTPG_VALUES_RMW ? nc * S * fw : 0u;
         const uint64_t entry_words = (uint64_t)S * per;
         const auto read_item = [&](uint32_t x, uint64_t &src, uint64_t &dst) -> bool {
-            const uint32_t j = x / per, q = (x % per) / fw, w = x % fw;
-            const uint32_t t = s_list[j / S], sl = j % S;
+            const uint32_t j = dper.div(x), rem = x - j * per, q = dfw.div(rem), w = rem - q * fw;
+            const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
             const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
             const bool ok = q < o.len;
@@ -2269,8 +2289,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         // the writes to the hot row, which never conflict) then need no
         // folding, and the sum is the sequential one (modular addition)
         const auto write_item = [&](uint32_t x, uint64_t &off, uint32_t &add) -> bool {
-            const uint32_t j = x / fw, w = x % fw;
-            const uint32_t t = s_list[j / S], sl = j % S;
+            const uint32_t j = dfw.div(x), w = x - j * fw;
+            const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
             const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
             const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
@@ -2339,7 +2359,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             dst_t[k] = ab[k] ? t - s_cbefore[t] : 0u;
             sq[k] = ab[k] ? s_seq[t] : 0ull;
             const uint32_t i = t;
-            const uint32_t ti = i < n ? i / S : 0u;
+            const uint32_t ti = i < n ? dS.div(i) : 0u;
             abop[k] = i < n && (s_waw[ti] || (s_raw[ti] && s_war[ti]));
             dst_i[k] = (ti - s_cbefore[ti]) * S + (i - ti * S);
             ro[k] = abop[k] ? s_row[i] : 0ull;
