@@ -2005,6 +2005,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         s_metab[0][i] = a.carry_meta[i];
     }
     for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seq[t] = a.carry_seq[t];
+    // the write hash starts empty; each epoch empties the slots its writes
+    // took once its marking is done
+    for (uint32_t h = threadIdx.x; h < kSmallSlots; h += kSmallBlock) {
+        s_key[h] = 0u;
+        s_min[h] = ~0u;
+    }
     uint64_t t_work = t_start, code = 0;
     // profile: thread 0 stamps the end of each part (after its barrier)
     // (sums in LDS: a register array would be spilled)
@@ -2121,10 +2127,6 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
         }
         for (uint32_t t = n_carry + threadIdx.x; t < B; t += kSmallBlock) s_seq[t] = head + (t - n_carry);
-        for (uint32_t k = threadIdx.x; k < kSmallSlots; k += kSmallBlock) {
-            s_key[k] = 0u;
-            s_min[k] = ~0u;
-        }
         for (uint32_t t = threadIdx.x; t < B; t += kSmallBlock) {
             s_waw[t] = 0;
             s_raw[t] = 0;
@@ -2167,10 +2169,13 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             r[k] = i < n ? s_row[i] : 0ull;
             m[k] = i < n ? s_meta[i] : 0u;
         }
-        // EXECUTE: W registrations into the hash (priority = position)
+        // EXECUTE: W registrations into the hash (priority = position); a
+        // write keeps its slot for the marking below
         uint32_t bad = 0;
+        uint32_t wslot[kSmallPer];  // ~0: no slot
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
+            wslot[k] = ~0u;
             const uint32_t i = k * kSmallBlock + threadIdx.x;
             if (i >= n) continue;
             const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
@@ -2183,6 +2188,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 if (prev == 0u || prev == key) break;
                 h = (h + 1) & (kSmallSlots - 1);
             }
+            wslot[k] = h;
             atomicMin(&s_min[h], dS.div(i));
         }
         if (bad) atomicAdd(&s_bad, bad);
@@ -2198,7 +2204,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (!o.valid) continue;
             const uint32_t t = dS.div(i);
             if (o.kind == LTPG_KV_WRITE) {
-                if (o.row != hot && s_min[small_find(s_key, o.row)] < t) s_waw[t] = 1;
+                if (o.row != hot && s_min[wslot[k]] < t) s_waw[t] = 1;
                 continue;
             }
             bool raw = false;
@@ -2257,6 +2263,13 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             lds_sync();
         nc = s_w[0];
         stamp(4);
+        // the marking is done: empty the hash slots this epoch's writes took
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k)
+            if (wslot[k] != ~0u) {
+                s_key[wslot[k]] = 0u;
+                s_min[wslot[k]] = ~0u;
+            }
         // the epoch's record, stored now by wave 0 (which takes no part in
         // the write-back below) so that its PCIe writes are done by the
         // time done_tail is released
