@@ -1957,7 +1957,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __shared__ uint32_t s_min[kSmallSlots];          // its writers' minimum position
     __shared__ uint8_t s_waw[kLoopMaxT], s_raw[kLoopMaxT], s_war[kLoopMaxT];
     __shared__ uint32_t s_list[kLoopMaxT];           // committed transactions, in position order
-    __shared__ uint32_t s_cbefore[kLoopMaxT];        // committed transactions before each position
+    __shared__ uint32_t s_dst[kLoopMaxT];            // an aborted position's place in the next epoch, ~0 if committed
     __shared__ uint32_t s_w[16 * kSmallPer + 1];
     // arrivals prefetched during the last epoch: requests st_head ..
     // st_head + st_n - 1 of the ring, request-major
@@ -2232,7 +2232,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 const bool ct = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
                 const unsigned long long m = __ballot(ct);
                 const uint32_t bf = (uint32_t)__popcll(m & (t ? (~0ull >> (64 - t)) : 0ull));
-                if (t < B) s_cbefore[t] = bf;
+                if (t < B) s_dst[t] = ct ? ~0u : t - bf;
                 if (ct) s_list[bf] = t;
                 if (t == 0) s_w[0] = (uint32_t)__popcll(m);
             }
@@ -2248,7 +2248,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
 #pragma unroll
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t t = k * kSmallBlock + threadIdx.x;
-                if (t < B) s_cbefore[t] = before[k];
+                if (t < B) s_dst[t] = c[k] ? ~0u : t - before[k];
                 if (c[k]) s_list[before[k]] = t;
             }
             if (threadIdx.x == 0) s_w[0] = total;
@@ -2276,10 +2276,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         if (threadIdx.x < 64) {
             if (threadIdx.x == 0)
                 done[done_tail & done_mask] = (uint64_t)B | ((uint64_t)nc << 16) | (epoch << 32);
-            for (uint32_t t = threadIdx.x; t < B; t += 64) {
-                const bool ct = !s_waw[t] && !(s_raw[t] && s_war[t]);
-                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (ct ? kDoneCommitted : 0ull);
-            }
+            for (uint32_t t = threadIdx.x; t < B; t += 64)
+                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (s_dst[t] == ~0u ? kDoneCommitted : 0ull);
         }
         // read-out (when kept) and write-back; reads see the snapshot, so
         // every read completes before the first write-back store
@@ -2368,13 +2366,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t t = k * kSmallBlock + threadIdx.x;
-            ab[k] = t < B && (s_waw[t] || (s_raw[t] && s_war[t]));
-            dst_t[k] = ab[k] ? t - s_cbefore[t] : 0u;
+            dst_t[k] = t < B ? s_dst[t] : ~0u;
+            ab[k] = dst_t[k] != ~0u;
             sq[k] = ab[k] ? s_seq[t] : 0ull;
             const uint32_t i = t;
             const uint32_t ti = i < n ? dS.div(i) : 0u;
-            abop[k] = i < n && (s_waw[ti] || (s_raw[ti] && s_war[ti]));
-            dst_i[k] = (ti - s_cbefore[ti]) * S + (i - ti * S);
+            const uint32_t di = i < n ? s_dst[ti] : ~0u;
+            abop[k] = di != ~0u;
+            dst_i[k] = di * S + (i - ti * S);
             ro[k] = abop[k] ? s_row[i] : 0ull;
             mo[k] = abop[k] ? s_meta[i] : 0u;
         }
