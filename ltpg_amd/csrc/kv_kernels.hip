@@ -1883,6 +1883,15 @@ __device__ __forceinline__ void sys_store(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// A request-ring entry, written by the host: a system-scope load, which
+// bypasses the GPU caches, so an entry of a ring slot the host has reused is
+// never served from a line cached on an earlier lap.
+template <typename T>
+__device__ __forceinline__ T ring_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Workgroup barrier for LDS only: it orders LDS accesses like __syncthreads
 // but does not wait for the wave's global loads, which stay in flight across
 // it (the served loop keeps PCIe reads in flight over several phases).
@@ -2032,9 +2041,10 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     // epoch thread 0 issues their loads after admission and stores the
     // values at the end of the epoch, so the next admission decides on
     // values at most one epoch old without waiting for them; only an idle or
-    // stalled loop waits for a fresh read.  (Ring entries are read only below
-    // the tail so read: the host writes them before it advances the tail.)
-    // (fills the other half; the caller switches after a barrier)
+    // stalled loop waits for a fresh read.  Ring entries are read only below
+    // the tail so read (the host writes them before it advances the tail),
+    // with ring_load, after the tail's load has returned.  poll() fills the
+    // other half of s_ctl; the caller switches halves after a barrier.
     const auto poll = [&]() {
         if (threadIdx.x == 0) {
             uint64_t *c = s_ctl[cp ^ 1u];
@@ -2112,8 +2122,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 from_ring[k] = live && jn >= staged;
                 const uint64_t at = from_ring[k] ? ((head + jn) & a.ring_mask) * S + (i - t * S) : 0ull;
                 if (any_ring) {
-                    r[k] = a.ring_keys[at];
-                    m[k] = a.ring_meta[at];
+                    r[k] = ring_load(&a.ring_keys[at]);
+                    m[k] = ring_load(&a.ring_meta[at]);
                 }
             }
 #pragma unroll
@@ -2157,8 +2167,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t q = dS.div(i);
             if (i < pf_n * S) {
                 const uint64_t at = ((head + q) & a.ring_mask) * S + (i - q * S);
-                pf_r[k] = a.ring_keys[at];
-                pf_m[k] = a.ring_meta[at];
+                pf_r[k] = ring_load(&a.ring_keys[at]);
+                pf_m[k] = ring_load(&a.ring_meta[at]);
             }
         }
         uint64_t r[kSmallPer];
@@ -2349,11 +2359,15 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         }
         lds_sync();
         stamp(6);
-        // release the record (and the kept reads, fenced above by every
-        // wave before the barrier): wave 0's own writes are only the record
-        if (threadIdx.x == 0) {
-            __threadfence_system();
-            sys_store(&ctl->done_tail, done_tail + 1 + B);
+        // release the record (the kept reads were fenced above by every wave
+        // before the barrier): every lane of wave 0 that stored part of it
+        // issues a system-scope release fence, then lane 0 stores done_tail.
+        // A release-only fence: the acquire half of __threadfence_system
+        // would also invalidate the L2.
+        if (threadIdx.x < 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&ctl->done_tail, done_tail + 1 + B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         done_tail += 1 + B;
         stamp(7);
