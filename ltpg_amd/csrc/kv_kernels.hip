@@ -949,17 +949,36 @@ __global__ __launch_bounds__(kPartBlock) void kv_part_process_kernel(PartArgs a)
     }
 }
 
+// The bucket's final W minima are read from the registers once, in row
+// order, into LDS; each operation then finds its row's minimum there.
 __global__ __launch_bounds__(kPartBlock) void kv_part_flags_kernel(PartArgs a)
 {
+    __shared__ unsigned long long s_w[kPartRows];
     if (blockIdx.x >= a.slices[3 * a.cap_slices]) return;
     const uint32_t b = a.slices[3 * blockIdx.x], beg = a.slices[3 * blockIdx.x + 1], end = a.slices[3 * blockIdx.x + 2];
     if (a.start[b + 1] - a.start[b] <= kPartSlice) return;  // decided in kv_part_process_kernel
     const uint64_t row0 = (uint64_t)b << kPartShift;
-    for (uint32_t e = beg + threadIdx.x; e < end; e += kPartBlock) {
-        const uint32_t key = a.ent_key[e];
-        unsigned long long w = a.regs[a.rl.w(row0 + (key >> 1))];
-        if ((uint32_t)(w >> kTagShift) != a.tag) w = ~0ull;  // no writer this epoch
-        part_flag(a, key, a.ent_me[e], w);
+    for (uint32_t r = threadIdx.x; r < kPartRows; r += kPartBlock) {
+        unsigned long long w = ~0ull;
+        if (row0 + r < a.rows) {
+            w = a.regs[a.rl.w(row0 + r)];
+            if ((uint32_t)(w >> kTagShift) != a.tag) w = ~0ull;  // no writer this epoch
+        }
+        s_w[r] = w;
+    }
+    __syncthreads();
+    for (uint32_t e0 = beg + threadIdx.x; e0 < end; e0 += kPartGroup * kPartBlock) {
+        uint32_t key[kPartGroup];
+        unsigned long long me[kPartGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u) {
+            const uint32_t e = e0 + u * kPartBlock;
+            key[u] = e < end ? a.ent_key[e] : 0u;
+            me[u] = e < end ? a.ent_me[e] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPartGroup; ++u)
+            if (e0 + u * kPartBlock < end) part_flag(a, key[u], me[u], s_w[key[u] >> 1]);
     }
 }
 
