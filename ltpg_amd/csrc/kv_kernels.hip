@@ -1879,10 +1879,12 @@ __global__ __launch_bounds__(kBlock) void kv_gather_rows_kernel(const uint32_t *
 //              write-back of committed writes, the RMW delta being the low
 //              32 bits of the arrival sequence number | 1;
 //   results    one record per epoch in the done ring (pinned host memory),
-//              written by wave 0 and published by a system-scope release of
-//              ctl->done_tail;
-//   requeue    the aborted requests are compacted to the front of the LDS
-//              op arrays, in position order, and head the next epoch.
+//              written by wave 0 right after CHECK (wave 0 takes no part in
+//              the write-back) and published by a system-scope release of
+//              ctl->done_tail after it;
+//   requeue    the aborted requests are compacted, in position order, to the
+//              front of the other half of the double-buffered LDS op arrays
+//              and head the next epoch.
 //
 // Every wait is bounded: the loop ends when the host sets ctl->stop, after
 // idle_ticks without work, after stall_ticks with the done ring full and
