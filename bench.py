@@ -953,6 +953,32 @@ def c5_inputs(L, rate, seconds, rows, fields, S, gen, rng, seed, min_requests=20
     return n, keys, meta, tb, at, kind
 
 
+# The device loop's epoch (at most 64 requests, no kept reads) as a chain of
+# steps, counted from kv_serve_loop_kernel along its longest lane: 6 LDS-only
+# workgroup barriers (after admission, registration, marking, CHECK,
+# write-back and requeue); 22 dependent LDS accesses (control words 1;
+# admission: staging read, op write 2; registration: op words 1; marking:
+# hash probe, minimum, flag store 3; CHECK: flags, count and list writes, the
+# count read back 3; write-back: list, op, sequence number per round, 3 rounds
+# of ~2,900 field words over 960 lanes at C5, 9; requeue: destination, op,
+# compacted write 3); 2 dependent LDS atomics (the registration's
+# compare-and-swap and minimum); one release publish.  The ring's PCIe reads
+# are prefetched an epoch ahead and the write-back's HBM atomics are never
+# waited for, so neither is on the chain.
+CHAIN_STEPS = {"barrier_ns": 6, "lds_load_ns": 22, "lds_atomic_ns": 2, "publish_ns": 1}
+
+
+def chain_bound(lat, epoch_us):
+    """Dependent-chain roofline of one device-loop epoch: the measured step
+    latencies (ltpg_latency_costs) times the steps the epoch chains."""
+    bound_us = sum(lat[k] * c for k, c in CHAIN_STEPS.items()) / 1e3
+    return {"steps": CHAIN_STEPS, "step_ns": {k: round(v, 1) for k, v in lat.items()}, "bound_us": bound_us,
+            "epoch_us": epoch_us, "frac": bound_us / epoch_us if epoch_us else None,
+            "what": "sum of step latencies along the epoch's dependent chain, measured live in one workgroup; "
+                    "frac = bound / measured epoch time (the rest is ALU work, bank conflicts and same-address "
+                    "serialization inside each step)"}
+
+
 def run_served_c5(args):
     """Served form under Poisson arrivals: a 50/50 mix of key-value requests
     (1-64 ops, Zipf 0.99, 100M rows x 10 x 100 B) and order-entry requests
@@ -1014,6 +1040,9 @@ def run_served_c5(args):
     # headline rate (profiling adds ~1.5 us per epoch, so it is not timed)
     n, keys, meta, tb, at, kind = inputs[head_rate]
     prof = _c5_rate(args, L, kv, tp, S, head_rate, n, keys, meta, tb, at, kind, 0, profile=True)
+    # the step latencies the epoch chains (barrier, LDS load / atomic, PCIe
+    # round trip, release publish), measured in one 1024-thread workgroup
+    lat = L.latency_costs(kv)
     # the reference's ZIPFIAN mode: row 0 as the hot key (its writes reduce
     # commutatively and never conflict), on a fresh table
     hot = {}
@@ -1056,7 +1085,8 @@ def run_served_c5(args):
             "latency_chain": {"what": "the epoch is a dependent chain of LDS phases separated by workgroup barriers "
                                       "plus one PCIe-published record; per-part device time of a profiled pass "
                                       "(profiling adds ~1.5 us per epoch)", "parts_us": parts,
-                              "profiled_epoch_us": sum(parts.values()) if parts else None}}
+                              "profiled_epoch_us": sum(parts.values()) if parts else None},
+            "chain_bound": chain_bound(lat, epoch_us)}
     res = {"value": h["committed_txn_per_s"], "unit": "txn/s", "span_s": h["span_s"], "served": served,
            "plain": plain, "profiled_pass": prof, "hot_row_0": hot, "roofline": roof,
            "config": {"workload": desc, "offered_txn_per_s": head_rate * 1e6, "rates_mtxn_s": rates,
@@ -1257,7 +1287,10 @@ def main():
         result["roofline"] = {k: c5["roofline"].get(k) for k in ROOF_KEYS}
         result["roofline"].update({"unit_of_work": "epoch", "epoch_us": c5["roofline"]["epoch_us"],
                                    "epochs": c5["roofline"]["epochs"],
-                                   "chain_parts_us": c5["roofline"]["latency_chain"]["parts_us"]})
+                                   "chain_parts_us": c5["roofline"]["latency_chain"]["parts_us"],
+                                   "chain_bound_us": c5["roofline"]["chain_bound"]["bound_us"],
+                                   "chain_frac": c5["roofline"]["chain_bound"]["frac"],
+                                   "chain_step_ns": c5["roofline"]["chain_bound"]["step_ns"]})
         if "cpu_baseline" in c5:
             result["cpu_baseline"] = c5["cpu_baseline"]
         variants = []

@@ -244,6 +244,16 @@ int ltpg_random_gather_rate(ltpg_ctx *ctx, uint64_t bytes, uint32_t field_bytes,
  * LTPG_ERR_DEVICE if the workgroups cannot all be resident. */
 int ltpg_sync_costs(ltpg_ctx *ctx, uint32_t blocks, uint32_t reps, double out[3]);
 
+/* Latencies of the steps an epoch of the served key-value device loop
+ * chains (no counterpart in the reference; the terms of that loop's
+ * dependent-chain roofline), in nanoseconds per step, measured in one
+ * 1024-thread workgroup: out_ns[0] an LDS-only workgroup barrier, [1] a
+ * dependent LDS load, [2] a dependent LDS atomic with return (every lane
+ * running such a chain), [3] a dependent load of pinned host memory (a PCIe
+ * round trip), [4] a system-scope release fence plus a store to pinned host
+ * memory, back to back.  Synchronous; allocates and frees its buffers. */
+int ltpg_latency_costs(ltpg_ctx *ctx, double out_ns[5]);
+
 /* ---- key-value tables (YCSB/0/Database.h:14-131) ------------------------ */
 /* Load rows x kv_fields x kv_field_bytes.  COLUMN_MAJOR requires
  * kv_field_bytes == 4 (the reference's u32 snapshot). */

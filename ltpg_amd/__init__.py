@@ -146,6 +146,18 @@ def sync_costs(ctx, blocks, reps=1000):
     return {"grid_barrier_us": out[0], "graph_boundary_us": out[1], "stream_boundary_us": out[2]}
 
 
+LATENCY_NAMES = ("barrier_ns", "lds_load_ns", "lds_atomic_ns", "pcie_load_ns", "publish_ns")
+
+
+def latency_costs(ctx):
+    """Step latencies of the served device loop's epoch chain, in ns
+    (ltpg_latency_costs): LDS-only workgroup barrier, dependent LDS load,
+    dependent LDS atomic, PCIe round trip, release publish to host memory."""
+    out = (ctypes.c_double * 5)()
+    _check(lib.ltpg_latency_costs(ctx.handle, out), "ltpg_latency_costs")
+    return dict(zip(LATENCY_NAMES, out))
+
+
 class DeviceBuffer:
     """A library-owned device allocation (for LTPG_MEM_DEVICE batches)."""
 

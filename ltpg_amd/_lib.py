@@ -214,6 +214,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(NeworderBatch), ctypes.POINTER(PaymentBatch), ctypes.c_uint32]),
     ("ltpg_tpcc_submit_readonly", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(OrderstatusBatch), ctypes.POINTER(StocklevelBatch), ctypes.c_uint32]),
+    ("ltpg_latency_costs", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     ("ltpg_sync_costs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_double)]),
     ("ltpg_random_gather_rate", ctypes.c_int,

@@ -706,6 +706,39 @@ int ltpg_sync_costs(ltpg_ctx *ctx, uint32_t blocks, uint32_t reps, double out[3]
     return LTPG_OK;
 }
 
+int ltpg_latency_costs(ltpg_ctx *ctx, double out_ns[5])
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out_ns) LTPG_FAIL(LTPG_ERR_INVALID, "null output");
+    constexpr uint32_t kChain = 4096, kReps = 4096, kHostReps = 512;
+    uint64_t *host = nullptr;
+    unsigned long long *d_out = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void **>(&host), (kChain + 1) * sizeof(uint64_t),
+                      hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        LTPG_FAIL(LTPG_ERR_NOMEM, "pinned host allocation failed");
+    for (uint32_t i = 0; i < kChain; i++) host[i] = (i * 2053u + 1u) & (kChain - 1);
+    host[kChain] = 0;
+    if ((rc = dev_alloc(&d_out, 8, "latency probe output"))) {
+        (void)hipHostFree(host);
+        return rc;
+    }
+    unsigned long long ticks[8] = {};
+    int khz = 0;
+    hipError_t h = launch_latency_probe(host, kChain, d_out, kReps, kHostReps, ctx->stream);
+    if (h == hipSuccess) h = hipStreamSynchronize(ctx->stream);
+    if (h == hipSuccess) h = hipMemcpy(ticks, d_out, sizeof(ticks), hipMemcpyDeviceToHost);
+    if (h == hipSuccess) h = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
+    dev_free(d_out);
+    (void)hipHostFree(host);
+    if (h != hipSuccess) LTPG_FAIL(LTPG_ERR_DEVICE, std::string("latency probe: ") + hipGetErrorString(h));
+    if (khz <= 0) LTPG_FAIL(LTPG_ERR_DEVICE, "no wall-clock rate");
+    const double ns_per_tick = 1e6 / khz;
+    const uint32_t reps[5] = {kReps, kReps, kReps, kHostReps, kHostReps};
+    for (int k = 0; k < 5; k++) out_ns[k] = ticks[k] * ns_per_tick / reps[k];
+    return LTPG_OK;
+}
+
 int ltpg_device_copy(ltpg_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 {
     int rc = check_ctx(ctx);

@@ -2479,6 +2479,70 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     }
 }
 
+// Latencies of the steps an epoch of the device loop chains (the terms of
+// its dependent-chain roofline), in wall-clock ticks per step, measured by
+// thread 0 of one 1024-thread workgroup while every lane runs the same chain:
+//   [0] an LDS-only workgroup barrier (lds_sync) over 16 waves;
+//   [1] a dependent LDS load (each lane chases its own index chain);
+//   [2] a dependent LDS atomic with return (a compare-and-swap that never
+//       swaps, along the chain);
+//   [3] a dependent system-scope load of pinned host memory (a PCIe round
+//       trip), thread 0 alone;
+//   [4] a publish -- a system-scope release fence, then a relaxed
+//       system-scope store to pinned host memory -- back to back, thread 0
+//       alone (each fence also waits for the previous store).
+// Every index stays inside its array: LDS chain values are masked to 4096,
+// host chain values are masked to chain_words (a power of two).
+__global__ __launch_bounds__(kSmallBlock) void latency_probe_kernel(uint64_t *host, uint32_t chain_words,
+                                                                   unsigned long long *out, uint32_t reps,
+                                                                   uint32_t host_reps, uint32_t mask,
+                                                                   uint64_t sentinel)
+{
+    // mask (4095) and sentinel (never reached) come from the host, so the
+    // compiler can neither drop the chains nor the waits for their ends
+    __shared__ uint32_t s_chain[4096];
+    for (uint32_t i = threadIdx.x; i < 4096; i += kSmallBlock) s_chain[i] = (i * 2053u + 1u) & 4095u;
+    __syncthreads();
+    unsigned long long t0 = 0;
+    // [0] barriers
+    if (threadIdx.x == 0) t0 = wall_clock64();
+    for (uint32_t r = 0; r < reps; ++r) lds_sync();
+    if (threadIdx.x == 0) out[0] = wall_clock64() - t0;
+    __syncthreads();
+    // [1] dependent LDS loads
+    uint32_t idx = (threadIdx.x * 4u) & mask;
+    if (threadIdx.x == 0) t0 = wall_clock64();
+    for (uint32_t r = 0; r < reps; ++r) idx = s_chain[idx] & mask;
+    if (idx == (uint32_t)sentinel) out[5] = idx;  // waits for the last load
+    if (threadIdx.x == 0) out[1] = wall_clock64() - t0;
+    __syncthreads();
+    // [2] dependent LDS atomics with return
+    idx = (threadIdx.x * 4u) & mask;
+    if (threadIdx.x == 0) t0 = wall_clock64();
+    // a compare-and-swap whose compare value is never stored: it returns the
+    // slot unchanged (an add of 0 would be turned into a plain load)
+    for (uint32_t r = 0; r < reps; ++r) idx = atomicCAS(&s_chain[idx], (uint32_t)sentinel, 0u) & mask;
+    if (idx == (uint32_t)sentinel) out[5] = idx;
+    if (threadIdx.x == 0) out[2] = wall_clock64() - t0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // [3] PCIe round trips
+        uint64_t j = 0;
+        t0 = wall_clock64();
+        for (uint32_t r = 0; r < host_reps; ++r) j = ring_load(&host[j]) & (chain_words - 1);
+        if (j == sentinel) out[5] = j;
+        out[3] = wall_clock64() - t0;
+        // [4] publishes
+        t0 = wall_clock64();
+        for (uint32_t r = 0; r < host_reps; ++r) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            __hip_atomic_store(&host[chain_words], (uint64_t)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        out[4] = wall_clock64() - t0;
+    }
+}
+
 int lanes_for(uint32_t fw)
 {
     if (fw > 64) return 0;
@@ -2862,6 +2926,15 @@ bool kv_serve_loop_ok(const KvState &kv, uint32_t S)
 {
     return S >= 1 && S <= 64 && S <= kSmallOps && (kv.hot_row == LTPG_NO_HOT_ROW || kv.fields <= kSmallHotFields) &&
            kv.rows < 0xFFFFFFFFull;
+}
+
+hipError_t launch_latency_probe(uint64_t *host, uint32_t chain_words, unsigned long long *out, uint32_t reps,
+                                uint32_t host_reps, hipStream_t s)
+{
+    if (chain_words == 0 || (chain_words & (chain_words - 1))) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(latency_probe_kernel, dim3(1), dim3(kSmallBlock), 0, s, host, chain_words, out, reps, host_reps,
+                       4095u, ~0ull);
+    return hipGetLastError();
 }
 
 hipError_t launch_kv_serve_loop(const ServeLoopArgs &a, hipStream_t s)

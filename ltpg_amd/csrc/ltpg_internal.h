@@ -277,6 +277,9 @@ hipError_t launch_copy_u128(void *dst, const void *src, uint64_t n16, hipStream_
 hipError_t launch_grid_barrier(uint32_t blocks, unsigned int *count, unsigned int *gen, uint32_t reps,
                                uint32_t spin_limit, unsigned int *err, hipStream_t s);
 hipError_t launch_boundary(uint32_t blocks, unsigned int *p, hipStream_t s);
+// Step latencies of the served device loop's epoch chain (ticks): out[0..4].
+hipError_t launch_latency_probe(uint64_t *host, uint32_t chain_words, unsigned long long *out, uint32_t reps,
+                                uint32_t host_reps, hipStream_t s);
 hipError_t launch_gather_calibration(const uint32_t *src, uint64_t slots, uint32_t stride_w, uint32_t fw,
                                      uint32_t *dst, uint64_t n, hipStream_t s);
 hipError_t launch_calibration(unsigned long long *a, uint64_t len, uint64_t n, int atomics,

@@ -342,6 +342,18 @@ def test_sync_costs_are_measured(gpu):
     kv.close()
 
 
+def test_latency_costs_are_measured(gpu):
+    """The step latencies behind the served device loop's chain roofline:
+    every chain ends (fixed counts, in-bounds indices) and gives a positive
+    time per step; a PCIe round trip costs more than an LDS load."""
+    kv = L.Context(workload=L.WORKLOAD_KV, kv_rows=1024, kv_fields=1, kv_field_bytes=4, max_txns=64, kv_max_slots=1)
+    c = L.latency_costs(kv)
+    assert set(c) == set(L.LATENCY_NAMES)
+    assert all(0.0 < x < 100000.0 for x in c.values()), c
+    assert c["pcie_load_ns"] > c["lds_load_ns"], c
+    kv.close()
+
+
 @pytest.mark.parametrize("rows", [5000, 200000], ids=["one-bucket", "25-buckets"])
 @pytest.mark.parametrize("theta", [0.6, 0.99])
 @pytest.mark.parametrize("hot_row", [L.NO_HOT_ROW, 0], ids=["no-hot", "hot0"])
