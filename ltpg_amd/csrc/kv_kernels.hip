@@ -1863,6 +1863,42 @@ __global__ __launch_bounds__(kBlock) void kv_gather_rows_kernel(const uint32_t *
 
 // ---- served key-value epochs in one persistent workgroup ------------------
 //
+// The device loop's op decode and write hash in 32-bit row arithmetic (its
+// tables have fewer than 2^32 - 1 rows): the same fields, validity and
+// scan length as decode_op, with a multiplicative hash on 32 bits.
+struct Op32 {
+    uint32_t row, kind, field, len;
+    bool valid;
+};
+
+__device__ __forceinline__ Op32 decode_op32(uint64_t key, uint32_t m, uint32_t rows, uint32_t fields,
+                                            uint32_t max_scan)
+{
+    Op32 o;
+    o.kind = m & 3u;
+    o.field = (m >> 8) & 0xffffu;
+    const uint32_t sl = (m >> 24) + 1u;
+    o.row = (uint32_t)key;
+    o.valid = o.kind != LTPG_KV_NONE && (key >> 32) == 0ull && o.row < rows && o.field < fields &&
+              (o.kind != LTPG_KV_SCAN || sl <= max_scan);
+    const uint32_t left = o.row < rows ? rows - o.row : 0u;
+    o.len = !o.valid || o.kind == LTPG_KV_WRITE ? 0u : o.kind == LTPG_KV_READ ? 1u : (sl < left ? sl : left);
+    return o;
+}
+
+static_assert(kSmallSlots == 4096, "loop_hash yields 12 bits");
+__device__ __forceinline__ uint32_t loop_hash(uint32_t row) { return (row * 0x9E3779B1u) >> 20; }
+
+__device__ __forceinline__ int loop_find(const uint32_t *s_key, uint32_t row)
+{
+    const uint32_t key = row + 1u;
+    for (uint32_t h = loop_hash(row);; h = (h + 1) & (kSmallSlots - 1)) {
+        const uint32_t k = s_key[h];
+        if (k == key) return (int)h;
+        if (k == 0u) return -1;
+    }
+}
+
 // kv_serve_loop_kernel runs the served form's key-value epochs back to back
 // in one 1024-thread workgroup, without a kernel launch or a host round trip
 // between them:
@@ -1883,8 +1919,11 @@ __global__ __launch_bounds__(kBlock) void kv_gather_rows_kernel(const uint32_t *
 //              the write-back) and published by a system-scope release of
 //              ctl->done_tail after it;
 //   requeue    the aborted requests are compacted, in position order, to the
-//              front of the other half of the double-buffered LDS op arrays
-//              and head the next epoch.
+//              front of the other half of the double-buffered LDS op and
+//              sequence arrays and head the next epoch.  Without kept reads
+//              no barrier separates the write-back, the publish and the
+//              requeue: wave 0 publishes while waves 1.. issue write-back
+//              atomics.
 //
 // Every wait is bounded: the loop ends when the host sets ctl->stop, after
 // idle_ticks without work, after stall_ticks with the done ring full and
@@ -1893,6 +1932,9 @@ __global__ __launch_bounds__(kBlock) void kv_gather_rows_kernel(const uint32_t *
 // values thread 0 broadcasts through LDS, so every wave leaves the loop at
 // the same iteration.
 constexpr uint32_t kLoopMaxT = kSmallOps;  // transactions per epoch at one op slot each
+// s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): wait for the wave's vector memory
+// accesses only (gfx9 encoding: vmcnt in bits 3:0 and 15:14)
+constexpr int kWaitVmcnt0 = 0x0F70;
 
 __device__ __forceinline__ uint64_t sys_load(const uint64_t *p)
 {
@@ -1982,7 +2024,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     // into the half the next epoch reads while this epoch's half is read
     __shared__ uint64_t s_rowb[2][kSmallOps];
     __shared__ uint32_t s_metab[2][kSmallOps];
-    __shared__ uint64_t s_seq[kLoopMaxT];            // arrival sequence number by position
+    __shared__ uint64_t s_seqb[2][kLoopMaxT];        // arrival sequence number by position, double-buffered
     __shared__ uint32_t s_key[kSmallSlots];          // row + 1 of a written row, 0 = empty
     __shared__ uint32_t s_min[kSmallSlots];          // its writers' minimum position
     __shared__ uint8_t s_waw[kLoopMaxT], s_raw[kLoopMaxT], s_war[kLoopMaxT];
@@ -2008,6 +2050,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     const uint32_t values = a.values;
     const FastDiv dS(S), dfw(fw), dper(L * fw);
     const uint64_t rows = a.rows, hot = a.hot, done_mask = a.done_mask;
+    // rows < 2^32 - 1 here (kv_serve_loop_ok): 32-bit rows; a hot row
+    // outside the table becomes a value no row takes
+    const uint32_t rows32 = (uint32_t)rows, hot32 = hot < rows ? (uint32_t)hot : 0xFFFFFFFFu;
     ServeCtl *ctl = a.ctl;
     uint32_t *table = a.table;
     uint32_t *reads = a.reads;
@@ -2034,7 +2079,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         s_rowb[0][i] = a.carry_keys[i];
         s_metab[0][i] = a.carry_meta[i];
     }
-    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seq[t] = a.carry_seq[t];
+    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seqb[0][t] = a.carry_seq[t];
     // the write hash starts empty; each epoch empties the slots its writes
     // took once its marking is done
     for (uint32_t h = threadIdx.x; h < kSmallSlots; h += kSmallBlock) {
@@ -2087,6 +2132,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2], now = s_ctl[cp][3];
         uint64_t *const s_row = s_rowb[op];
         uint32_t *const s_meta = s_metab[op];
+        uint64_t *const s_seq = s_seqb[op];
         if (stop) {
             code = kServeExitStop;
             break;
@@ -2209,11 +2255,11 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             wslot[k] = ~0u;
             const uint32_t i = k * kSmallBlock + threadIdx.x;
             if (i >= n) continue;
-            const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+            const Op32 o = decode_op32(r[k], m[k], rows32, fields, max_scan);
             bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
-            if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot) continue;
-            const uint32_t key = (uint32_t)o.row + 1u;
-            uint32_t h = small_hash(o.row);
+            if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot32) continue;
+            const uint32_t key = o.row + 1u;
+            uint32_t h = loop_hash(o.row);
             while (true) {
                 const uint32_t prev = atomicCAS(&s_key[h], 0u, key);
                 if (prev == 0u || prev == key) break;
@@ -2231,17 +2277,17 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t i = k * kSmallBlock + threadIdx.x;
             if (i >= n) continue;
-            const Op o = decode_op(r[k], m[k], rows, fields, max_scan);
+            const Op32 o = decode_op32(r[k], m[k], rows32, fields, max_scan);
             if (!o.valid) continue;
             const uint32_t t = dS.div(i);
             if (o.kind == LTPG_KV_WRITE) {
-                if (o.row != hot && s_min[wslot[k]] < t) s_waw[t] = 1;
+                if (o.row != hot32 && s_min[wslot[k]] < t) s_waw[t] = 1;
                 continue;
             }
             bool raw = false;
             for (uint32_t q = 0; q < o.len; ++q) {
-                if (o.row + q == hot) continue;
-                const int h = small_find(s_key, o.row + q);
+                if (o.row + q == hot32) continue;
+                const int h = loop_find(s_key, o.row + q);
                 if (h < 0) continue;
                 const uint32_t w = s_min[h];
                 if (w < t)
@@ -2285,15 +2331,35 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (threadIdx.x == 0) s_w[0] = total;
         }
         // With kept reads the read-out below loads the table, so the last
-        // epoch's write-back atomics must have completed: a full barrier
-        // (every wave waits for its memory operations).  Without, nothing in
-        // the loop reads the table and an LDS barrier suffices.
+        // epoch's write-back atomics must be ordered before it: a barrier
+        // with workgroup-scope release / acquire of global memory.  Without,
+        // nothing in the loop reads the table and an LDS barrier suffices.
         if (reads)
             __syncthreads();
         else
             lds_sync();
         nc = s_w[0];
         stamp(4);
+        // the prefetched arrivals and the control words go to LDS now, before
+        // the write-back: their loads returned during the phases above, and
+        // no load result is then left pending behind the write-back atomics
+        // (its first use would also wait for every atomic issued after it)
+        if (threadIdx.x == 0) {
+            uint64_t *cn = s_ctl[cp ^ 1u];
+            cn[0] = c_tail;
+            cn[1] = c_stop;
+            cn[2] = c_dh;
+            s_st_head = head;
+            s_st_n = pf_n;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSmallPer; ++k) {
+            const uint32_t i = k * kSmallBlock + threadIdx.x;
+            if (i < pf_n * S) {
+                st_row[i] = pf_r[k];
+                st_meta[i] = pf_m[k];
+            }
+        }
         // the marking is done: empty the hash slots this epoch's writes took
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k)
@@ -2320,9 +2386,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t j = dper.div(x), rem = x - j * per, q = dfw.div(rem), w = rem - q * fw;
             const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
-            const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+            const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
             const bool ok = q < o.len;
-            src = ok ? ((o.row + q) * fields + o.field) * fsw + w : 0ull;
+            src = ok ? ((uint64_t)(o.row + q) * fields + o.field) * fsw + w : 0ull;
             dst = ((done_tail + 1 + t) & done_mask) * entry_words + ((uint64_t)sl * L + q) * fw + w;
             return ok;
         };
@@ -2334,15 +2400,17 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t j = dfw.div(x), w = x - j * fw;
             const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
-            const Op o = decode_op(s_row[i], s_meta[i], rows, fields, max_scan);
+            const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
             const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
             add = (uint32_t)s_seq[t] | 1u;
-            off = wr ? (o.row * fields + o.field) * fsw + w : 0ull;
+            off = wr ? ((uint64_t)o.row * fields + o.field) * fsw + w : 0ull;
             return wr;
         };
         // the read-out (kept reads only) in rounds, then a barrier, then the
-        // write-back; without kept reads an epoch's writes are one round
-        {
+        // write-back; without kept reads an epoch's writes are one round.
+        // (The barrier closes every path through the read-out, so no table
+        // load is left pending after it.)
+        if (total_r) {
             for (uint32_t b0 = 0; b0 < total_r; b0 += kSmallUnroll * kSmallBlock) {
                 uint64_t src[kSmallUnroll], dst[kSmallUnroll];
                 bool ok[kSmallUnroll];
@@ -2360,11 +2428,19 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 for (uint32_t u = 0; u < kSmallUnroll; ++u)
                     if (ok[u]) reads[dst[u]] = v[u];
             }
-            // every read of the snapshot is done before the first write
-            if (total_r) __syncthreads();
-            // waves 1.. issue the write-back atomics and do not wait for
-            // them: the next wait is the next epoch's full barrier before its
-            // read-out (kept reads), else the end of the launch
+            // every table load has returned (a load whose value is not kept
+            // is otherwise never waited for, and the first later write of
+            // its register would wait for every access issued after it,
+            // the write-back atomics included), then every read of the
+            // snapshot is done before the first write
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+            __syncthreads();
+        }
+        {
+            // waves 1.. issue the write-back atomics and never wait for
+            // them inside the loop (no later load result is pending behind
+            // them); the next epoch's read-out is ordered after them by its
+            // barrier (kept reads), and the launch ends after them
             if (threadIdx.x >= 64)
                 for (uint32_t x = threadIdx.x - 64; x < total_w; x += kSmallBlock - 64) {
                     uint64_t off;
@@ -2378,10 +2454,15 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             add_counts(a.ctr, (uint32_t)(epoch % kCounterStripes), LTPG_TYPE_KV, nc, B);
             if (s_bad) atomicAdd(&a.ctr->range_errors, s_bad);
         }
-        lds_sync();
+        // Kept reads: every wave fenced its stores above, and the barrier
+        // orders those fences before the release below.  Without, the record
+        // is wave 0's alone and wave 0 publishes it while waves 1.. are still
+        // issuing the write-back (nothing after this point reads what they
+        // write: the requeue reads this epoch's LDS halves and writes the
+        // other ones).
+        if (reads) lds_sync();
         stamp(6);
-        // release the record (the kept reads were fenced above by every wave
-        // before the barrier): every lane of wave 0 that stored part of it
+        // release the record: every lane of wave 0 that stored part of it
         // issues a system-scope release fence, then lane 0 stores done_tail.
         // A release-only fence: the acquire half of __threadfence_system
         // would also invalidate the L2.
@@ -2412,13 +2493,11 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             ro[k] = abop[k] ? s_row[i] : 0ull;
             mo[k] = abop[k] ? s_meta[i] : 0u;
         }
-        // the ops go to the other half, so only s_seq is compacted in
-        // place: with at most 64 transactions wave 0 alone reads and writes
-        // it (its reads above precede its writes below), else a barrier
-        if (B > 64) lds_sync();
+        // everything goes to the other halves, which no wave reads in this
+        // epoch: no barrier between the reads above and the stores below
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
-            if (ab[k]) s_seq[dst_t[k]] = sq[k];
+            if (ab[k]) s_seqb[op ^ 1u][dst_t[k]] = sq[k];
             if (abop[k]) {
                 s_rowb[op ^ 1u][dst_i[k]] = ro[k];
                 s_metab[op ^ 1u][dst_i[k]] = mo[k];
@@ -2435,21 +2514,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         t_work = now;
         if (threadIdx.x == 0) {
             uint64_t *cn = s_ctl[cp ^ 1u];
-            cn[0] = c_tail;
-            cn[1] = c_stop;
-            cn[2] = c_dh;
             cn[3] = wall_clock64();
             s_busy += cn[3] - now;
-            s_st_head = head;
-            s_st_n = pf_n;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kSmallPer; ++k) {
-            const uint32_t i = k * kSmallBlock + threadIdx.x;
-            if (i < pf_n * S) {
-                st_row[i] = pf_r[k];
-                st_meta[i] = pf_m[k];
-            }
         }
         lds_sync();
         cp ^= 1u;
@@ -2462,7 +2528,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         a.carry_keys[i] = s_rowb[op][i];
         a.carry_meta[i] = s_metab[op][i];
     }
-    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) a.carry_seq[t] = s_seq[t];
+    for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) a.carry_seq[t] = s_seqb[op][t];
     __syncthreads();
     if (threadIdx.x == 0) {
         if (prof) {
