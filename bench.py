@@ -1084,7 +1084,8 @@ def run_served_c5(args):
             "epoch_us": epoch_us, "epochs": dl.get("epochs", 0), "kernel_busy_ms": dl.get("busy_ns", 0) / 1e6,
             "latency_chain": {"what": "the epoch is a dependent chain of LDS phases separated by workgroup barriers "
                                       "plus one PCIe-published record; per-part device time of a profiled pass "
-                                      "(profiling adds ~1.5 us per epoch)", "parts_us": parts,
+                                      "(thread 0's clock stamps; the profiled epoch is longer than the plain one "
+                                      "by the stamps' own cost)", "parts_us": parts,
                               "profiled_epoch_us": sum(parts.values()) if parts else None},
             "chain_bound": chain_bound(lat, epoch_us)}
     res = {"value": h["committed_txn_per_s"], "unit": "txn/s", "span_s": h["span_s"], "served": served,

@@ -1908,7 +1908,12 @@ __device__ __forceinline__ int loop_find(const uint32_t *s_key, uint32_t row)
 //              taken from the request ring in arrival order, up to the epoch
 //              cap.  Priority inside an epoch is batch position, which is
 //              arrival order: every carried request arrived before every new
-//              one, and carried requests keep their relative order;
+//              one, and carried requests keep their relative order.  The
+//              next epoch is decided and admitted right after an epoch's
+//              CHECK, from arrivals prefetched during the epoch, into the
+//              other halves of the double-buffered LDS arrays, so one
+//              barrier separates two epochs; after an idle or stalled turn
+//              (or an exit test) the loop's top admits from the ring;
 //   phases     those of kv_small_epoch_kernel: W registrations in an LDS
 //              hash of the written rows, read marking, Aria's commit rule,
 //              read-out of committed reads (when the host keeps them) and
@@ -2031,12 +2036,6 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __shared__ uint32_t s_list[kLoopMaxT];           // committed transactions, in position order
     __shared__ uint32_t s_dst[kLoopMaxT];            // an aborted position's place in the next epoch, ~0 if committed
     __shared__ uint32_t s_w[16 * kSmallPer + 1];
-    // arrivals prefetched during the last epoch: requests st_head ..
-    // st_head + st_n - 1 of the ring, request-major
-    __shared__ uint64_t st_row[kSmallOps];
-    __shared__ uint32_t st_meta[kSmallOps];
-    __shared__ uint64_t s_st_head;
-    __shared__ uint32_t s_st_n;
     __shared__ uint64_t s_busy;  // wall-clock ticks inside epochs
     // control words, double-buffered: thread 0 fills the half the loop is
     // not reading, and every wave switches halves after the next barrier,
@@ -2120,19 +2119,40 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             c[3] = wall_clock64();
         }
     };
+    // Releases the records written so far: every lane of wave 0 (the lanes
+    // that store records) issues a system-scope release fence, then lane 0
+    // stores done_tail (with kept reads, every wave fenced its stores before
+    // the barrier that precedes this).  A release-only fence: the acquire
+    // half of __threadfence_system would also invalidate the L2.
+    const auto publish = [&](uint64_t tail_value) {
+        if (threadIdx.x < 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&ctl->done_tail, tail_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    };
     poll();
     if (threadIdx.x == 0) {
-        s_st_head = head;
-        s_st_n = 0;
         s_busy = 0;
+        s_bad = 0;
     }
     __syncthreads();
     cp ^= 1u;
+    // The epoch about to run, uniform in every wave: admitted (its ops,
+    // sequence numbers and zeroed flags already in half op, admitted by the
+    // last epoch right after its CHECK), B transactions of which n_new new
+    // arrivals, now its start time and tail_seen the ring tail it knows.
+    bool admitted = false;
+    uint32_t B = 0, n_new = 0;
+    uint64_t now = 0, tail_seen = 0;
     while (true) {
-        const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2], now = s_ctl[cp][3];
         uint64_t *const s_row = s_rowb[op];
         uint32_t *const s_meta = s_metab[op];
         uint64_t *const s_seq = s_seqb[op];
+        if (!admitted) {
+        const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2];
+        now = s_ctl[cp][3];
+        tail_seen = tail;
         if (stop) {
             code = kServeExitStop;
             break;
@@ -2143,8 +2163,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         }
         const uint64_t avail = tail > head ? tail - head : 0ull;
         const uint32_t room = cap > n_carry ? cap - n_carry : 0u;
-        const uint32_t n_new = (uint32_t)(avail < room ? avail : (uint64_t)room);
-        const uint32_t B = n_carry + n_new;
+        n_new = (uint32_t)(avail < room ? avail : (uint64_t)room);
+        B = n_carry + n_new;
         if (B == 0) {
             if (now - t_work > a.idle_ticks) {
                 code = kServeExitIdle;
@@ -2170,25 +2190,20 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             continue;
         }
         stamp(0);
-        // admission: arrivals head .. head + n_new - 1 behind the carried
-        // requests, from the prefetched staging when it holds them, else from
-        // the ring (every ring load issued before the first LDS store)
-        const uint32_t n = B * S, c0 = n_carry * S;
+        // admission after an idle turn, an exit test or a full done ring:
+        // arrivals head .. head + n_new - 1 behind the carried requests, read
+        // from the ring (every ring load issued before the first LDS store)
         {
-            const uint32_t staged = s_st_head == head ? s_st_n : 0u;
-            const bool any_ring = n_new > staged;  // uniform: skip the PCIe reads when all are staged
+            const uint32_t n = B * S, c0 = n_carry * S;
             uint64_t r[kSmallPer] = {};
             uint32_t m[kSmallPer] = {};
-            bool from_ring[kSmallPer];
 #pragma unroll
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t i = k * kSmallBlock + threadIdx.x;
                 const bool live = i >= c0 && i < n;
-                const uint32_t t = live ? dS.div(i) : 0u;
-                const uint32_t jn = live ? t - n_carry : 0u;
-                from_ring[k] = live && jn >= staged;
-                const uint64_t at = from_ring[k] ? ((head + jn) & a.ring_mask) * S + (i - t * S) : 0ull;
-                if (any_ring) {
+                const uint32_t jn = live ? dS.div(i - c0) : 0u;
+                const uint64_t at = live ? ((head + jn) & a.ring_mask) * S + (i - c0 - jn * S) : 0ull;
+                if (n_new) {  // uniform: no PCIe read for an epoch of carried requests
                     r[k] = ring_load(&a.ring_keys[at]);
                     m[k] = ring_load(&a.ring_meta[at]);
                 }
@@ -2197,9 +2212,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t i = k * kSmallBlock + threadIdx.x;
                 if (i >= c0 && i < n) {
-                    const uint32_t st = i - c0;  // staging index: request jn, slot i % S
-                    s_row[i] = from_ring[k] ? r[k] : st_row[st];
-                    s_meta[i] = from_ring[k] ? m[k] : st_meta[st];
+                    s_row[i] = r[k];
+                    s_meta[i] = m[k];
                 }
             }
         }
@@ -2209,12 +2223,13 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             s_raw[t] = 0;
             s_war[t] = 0;
         }
-        if (threadIdx.x == 0) s_bad = 0;
+        head += n_new;
         lds_sync();
         stamp(1);
-        // the ring entries are in LDS: their slots may be reused by the host
-        head += n_new;
-        // (relaxed: the entries' loads completed before the barrier)
+        }  // !admitted
+        const uint32_t n = B * S;
+        // the arrivals this epoch took are in LDS (in every wave, before the
+        // barrier above): their ring slots may be reused by the host
         if (threadIdx.x == 0 && n_new) __hip_atomic_store(&ctl->head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         // the next admission's control words: loads in flight during the epoch
         uint64_t c_tail = 0, c_stop = 0, c_dh = 0;
@@ -2224,8 +2239,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             c_dh = __hip_atomic_load(&ctl->done_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         // and its arrivals: the published requests after this admission, up
-        // to a full epoch, loaded now and staged in LDS at the end
-        const uint32_t pf_n = (uint32_t)(tail - head < a.cap_max ? tail - head : (uint64_t)a.cap_max);
+        // to a full epoch, loaded now and admitted into the next epoch right
+        // after this one's CHECK
+        const uint32_t pf_n = (uint32_t)(tail_seen - head < a.cap_max ? tail_seen - head : (uint64_t)a.cap_max);
         uint64_t pf_r[kSmallPer] = {};
         uint32_t pf_m[kSmallPer] = {};
 #pragma unroll
@@ -2330,6 +2346,16 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
             if (threadIdx.x == 0) s_w[0] = total;
         }
+        // the control words loaded during the epoch, and the time, to LDS
+        // before the barrier: every wave decides on them right after it
+        if (threadIdx.x == 0) {
+            uint64_t *cn = s_ctl[cp ^ 1u];
+            cn[0] = c_tail;
+            cn[1] = c_stop;
+            cn[2] = c_dh;
+            cn[3] = wall_clock64();
+            s_busy += cn[3] - now;
+        }
         // With kept reads the read-out below loads the table, so the last
         // epoch's write-back atomics must be ordered before it: a barrier
         // with workgroup-scope release / acquire of global memory.  Without,
@@ -2340,24 +2366,49 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             lds_sync();
         nc = s_w[0];
         stamp(4);
-        // the prefetched arrivals and the control words go to LDS now, before
-        // the write-back: their loads returned during the phases above, and
-        // no load result is then left pending behind the write-back atomics
-        // (its first use would also wait for every atomic issued after it)
-        if (threadIdx.x == 0) {
-            uint64_t *cn = s_ctl[cp ^ 1u];
-            cn[0] = c_tail;
-            cn[1] = c_stop;
-            cn[2] = c_dh;
-            s_st_head = head;
-            s_st_n = pf_n;
+        // The next epoch, decided now in every wave alike: the aborted
+        // requests head it (the requeue below moves them), followed by the
+        // prefetched arrivals up to its cap.  It is admitted now, into the
+        // other halves, unless the host asked to stop, the slice is over, it
+        // would be empty or the done ring has no room for its record; the
+        // loop's top then decides again on fresh values.  The arrivals go to
+        // LDS before the write-back: their loads returned during the phases
+        // above, and no load result is then left pending behind the
+        // write-back atomics (its first use would wait for those as well).
+        const uint32_t carry_n = B - nc;
+        uint32_t cap_n = a.cap_max;
+        if (a.growth) {
+            const uint64_t g = (uint64_t)a.growth * nc;
+            cap_n = (uint32_t)(g < a.floor ? a.floor : (g > a.cap_max ? a.cap_max : g));
         }
+        const uint64_t n_tail = s_ctl[cp ^ 1u][0], n_stop = s_ctl[cp ^ 1u][1], n_dh = s_ctl[cp ^ 1u][2],
+                       n_now = s_ctl[cp ^ 1u][3];
+        uint32_t new_n;
+        {
+            const uint64_t avail = n_tail > head ? n_tail - head : 0ull;
+            const uint32_t room = cap_n > carry_n ? cap_n - carry_n : 0u;
+            new_n = (uint32_t)(avail < room ? avail : (uint64_t)room);
+            new_n = new_n < pf_n ? new_n : pf_n;
+        }
+        const uint32_t B_n = carry_n + new_n;
+        const bool go = !n_stop && n_now - t_start <= a.slice_ticks && B_n > 0 &&
+                        done_tail + 1 + B + 1 + B_n - n_dh <= done_mask + 1;
+        if (go) {
+            uint64_t *const nx_row = s_rowb[op ^ 1u] + (uint64_t)carry_n * S;
+            uint32_t *const nx_meta = s_metab[op ^ 1u] + (uint64_t)carry_n * S;
 #pragma unroll
-        for (uint32_t k = 0; k < kSmallPer; ++k) {
-            const uint32_t i = k * kSmallBlock + threadIdx.x;
-            if (i < pf_n * S) {
-                st_row[i] = pf_r[k];
-                st_meta[i] = pf_m[k];
+            for (uint32_t k = 0; k < kSmallPer; ++k) {
+                const uint32_t i = k * kSmallBlock + threadIdx.x;
+                if (i < new_n * S) {
+                    nx_row[i] = pf_r[k];
+                    nx_meta[i] = pf_m[k];
+                }
+            }
+            for (uint32_t t = threadIdx.x; t < new_n; t += kSmallBlock) s_seqb[op ^ 1u][carry_n + t] = head + t;
+            for (uint32_t t = threadIdx.x; t < B_n; t += kSmallBlock) {
+                s_waw[t] = 0;
+                s_raw[t] = 0;
+                s_war[t] = 0;
             }
         }
         // the marking is done: empty the hash slots this epoch's writes took
@@ -2452,7 +2503,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         if (reads && total_r) __threadfence_system();
         if (threadIdx.x == 64) {
             add_counts(a.ctr, (uint32_t)(epoch % kCounterStripes), LTPG_TYPE_KV, nc, B);
-            if (s_bad) atomicAdd(&a.ctr->range_errors, s_bad);
+            const uint32_t bad_ops = s_bad;
+            s_bad = 0;  // (the next EXECUTE adds to it after the barrier below)
+            if (bad_ops) atomicAdd(&a.ctr->range_errors, bad_ops);
         }
         // Kept reads: every wave fenced its stores above, and the barrier
         // orders those fences before the release below.  Without, the record
@@ -2462,15 +2515,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         // other ones).
         if (reads) lds_sync();
         stamp(6);
-        // release the record: every lane of wave 0 that stored part of it
-        // issues a system-scope release fence, then lane 0 stores done_tail.
-        // A release-only fence: the acquire half of __threadfence_system
-        // would also invalidate the L2.
-        if (threadIdx.x < 64) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-            if (threadIdx.x == 0)
-                __hip_atomic_store(&ctl->done_tail, done_tail + 1 + B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        publish(done_tail + 1 + B);
         done_tail += 1 + B;
         stamp(7);
         // requeue: the aborted requests move to the front, in position order
@@ -2503,19 +2548,17 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 s_metab[op ^ 1u][dst_i[k]] = mo[k];
             }
         }
-        n_carry = B - nc;
-        if (a.growth) {
-            const uint64_t g = (uint64_t)a.growth * nc;
-            cap = (uint32_t)(g < a.floor ? a.floor : (g > a.cap_max ? a.cap_max : g));
-        } else {
-            cap = a.cap_max;
-        }
+        n_carry = carry_n;
+        cap = cap_n;
         ++epoch;
         t_work = now;
-        if (threadIdx.x == 0) {
-            uint64_t *cn = s_ctl[cp ^ 1u];
-            cn[3] = wall_clock64();
-            s_busy += cn[3] - now;
+        admitted = go;
+        if (go) {
+            B = B_n;
+            n_new = new_n;
+            head += new_n;
+            now = n_now;
+            tail_seen = n_tail;
         }
         lds_sync();
         cp ^= 1u;
