@@ -2032,9 +2032,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     __shared__ uint64_t s_seqb[2][kLoopMaxT];        // arrival sequence number by position, double-buffered
     __shared__ uint32_t s_key[kSmallSlots];          // row + 1 of a written row, 0 = empty
     __shared__ uint32_t s_min[kSmallSlots];          // its writers' minimum position
-    __shared__ uint8_t s_waw[kLoopMaxT], s_raw[kLoopMaxT], s_war[kLoopMaxT];
+    // conflict flags by position, double-buffered: the next epoch's are
+    // cleared while waves may still read this epoch's
+    __shared__ uint8_t s_wawb[2][kLoopMaxT], s_rawb[2][kLoopMaxT], s_warb[2][kLoopMaxT];
     __shared__ uint32_t s_list[kLoopMaxT];           // committed transactions, in position order
     __shared__ uint32_t s_dst[kLoopMaxT];            // an aborted position's place in the next epoch, ~0 if committed
+    // the same, one copy per wave, for epochs of at most 64 transactions
+    // (each wave counts the commits itself)
+    __shared__ uint32_t s_listw[kSmallBlock / 64][64], s_dstw[kSmallBlock / 64][64];
     __shared__ uint32_t s_w[16 * kSmallPer + 1];
     __shared__ uint64_t s_busy;  // wall-clock ticks inside epochs
     // control words, double-buffered: thread 0 fills the half the loop is
@@ -2149,6 +2154,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         uint64_t *const s_row = s_rowb[op];
         uint32_t *const s_meta = s_metab[op];
         uint64_t *const s_seq = s_seqb[op];
+        uint8_t *const s_waw = s_wawb[op], *const s_raw = s_rawb[op], *const s_war = s_warb[op];
         if (!admitted) {
         const uint64_t tail = s_ctl[cp][0], stop = s_ctl[cp][1], done_head = s_ctl[cp][2];
         now = s_ctl[cp][3];
@@ -2313,12 +2319,42 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
             if (raw) s_raw[t] = 1;
         }
+        // the control words loaded during the epoch, and the time, to LDS
+        // before the barrier: every wave decides on them after CHECK
+        if (threadIdx.x == 0) {
+            uint64_t *cn = s_ctl[cp ^ 1u];
+            cn[0] = c_tail;
+            cn[1] = c_stop;
+            cn[2] = c_dh;
+            cn[3] = wall_clock64();
+            s_busy += cn[3] - now;
+        }
         lds_sync();
         stamp(3);
         // CHECK: commit bits and the committed list in position order.  An
         // epoch of at most 64 transactions (the common case under the
-        // commit-based cap) is counted by wave 0 alone with one ballot.
+        // commit-based cap) is counted with one ballot: by every wave into
+        // its own copies of the arrays, with no barrier after it, when the
+        // host keeps no reads; else by wave 0 alone.
         uint32_t nc;
+        const uint32_t *dstv = s_dst, *listv = s_list;  // the arrays this wave reads below
+        if (B <= 64 && !reads) {
+            const uint32_t t = (uint32_t)lane_id(), wave = threadIdx.x / 64;
+            const bool ct = t < B && !s_waw[t] && !(s_raw[t] && s_war[t]);
+            const unsigned long long m = __ballot(ct);
+            const uint32_t bf = (uint32_t)__popcll(m & (t ? (~0ull >> (64 - t)) : 0ull));
+            uint32_t *const dw = s_dstw[wave], *const lw = s_listw[wave];
+            if (t < B) dw[t] = ct ? ~0u : t - bf;
+            if (ct) lw[bf] = t;
+            nc = (uint32_t)__popcll(m);
+            dstv = dw;
+            listv = lw;
+            // the wave's lanes read each other's entries below: LDS accesses
+            // of one wave complete in order; keep the compiler's order too
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            stamp(4);
+        } else {
         if (B <= 64) {
             if (threadIdx.x < 64) {
                 const uint32_t t = threadIdx.x;
@@ -2346,16 +2382,6 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
             if (threadIdx.x == 0) s_w[0] = total;
         }
-        // the control words loaded during the epoch, and the time, to LDS
-        // before the barrier: every wave decides on them right after it
-        if (threadIdx.x == 0) {
-            uint64_t *cn = s_ctl[cp ^ 1u];
-            cn[0] = c_tail;
-            cn[1] = c_stop;
-            cn[2] = c_dh;
-            cn[3] = wall_clock64();
-            s_busy += cn[3] - now;
-        }
         // With kept reads the read-out below loads the table, so the last
         // epoch's write-back atomics must be ordered before it: a barrier
         // with workgroup-scope release / acquire of global memory.  Without,
@@ -2366,6 +2392,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             lds_sync();
         nc = s_w[0];
         stamp(4);
+        }
         // The next epoch, decided now in every wave alike: the aborted
         // requests head it (the requeue below moves them), followed by the
         // prefetched arrivals up to its cap.  It is admitted now, into the
@@ -2406,9 +2433,9 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             }
             for (uint32_t t = threadIdx.x; t < new_n; t += kSmallBlock) s_seqb[op ^ 1u][carry_n + t] = head + t;
             for (uint32_t t = threadIdx.x; t < B_n; t += kSmallBlock) {
-                s_waw[t] = 0;
-                s_raw[t] = 0;
-                s_war[t] = 0;
+                s_wawb[op ^ 1u][t] = 0;
+                s_rawb[op ^ 1u][t] = 0;
+                s_warb[op ^ 1u][t] = 0;
             }
         }
         // the marking is done: empty the hash slots this epoch's writes took
@@ -2425,7 +2452,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (threadIdx.x == 0)
                 done[done_tail & done_mask] = (uint64_t)B | ((uint64_t)nc << 16) | (epoch << 32);
             for (uint32_t t = threadIdx.x; t < B; t += 64)
-                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (s_dst[t] == ~0u ? kDoneCommitted : 0ull);
+                done[(done_tail + 1 + t) & done_mask] = s_seq[t] | (dstv[t] == ~0u ? kDoneCommitted : 0ull);
         }
         // read-out (when kept) and write-back; reads see the snapshot, so
         // every read completes before the first write-back store
@@ -2435,7 +2462,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         const uint64_t entry_words = (uint64_t)S * per;
         const auto read_item = [&](uint32_t x, uint64_t &src, uint64_t &dst) -> bool {
             const uint32_t j = dper.div(x), rem = x - j * per, q = dfw.div(rem), w = rem - q * fw;
-            const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
+            const uint32_t jt = dS.div(j), t = listv[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
             const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
             const bool ok = q < o.len;
@@ -2449,7 +2476,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         // folding, and the sum is the sequential one (modular addition)
         const auto write_item = [&](uint32_t x, uint64_t &off, uint32_t &add) -> bool {
             const uint32_t j = dfw.div(x), w = x - j * fw;
-            const uint32_t jt = dS.div(j), t = s_list[jt], sl = j - jt * S;
+            const uint32_t jt = dS.div(j), t = listv[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
             const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
             const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
@@ -2527,12 +2554,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t t = k * kSmallBlock + threadIdx.x;
-            dst_t[k] = t < B ? s_dst[t] : ~0u;
+            dst_t[k] = t < B ? dstv[t] : ~0u;
             ab[k] = dst_t[k] != ~0u;
             sq[k] = ab[k] ? s_seq[t] : 0ull;
             const uint32_t i = t;
             const uint32_t ti = i < n ? dS.div(i) : 0u;
-            const uint32_t di = i < n ? s_dst[ti] : ~0u;
+            const uint32_t di = i < n ? dstv[ti] : ~0u;
             abop[k] = di != ~0u;
             dst_i[k] = di * S + (i - ti * S);
             ro[k] = abop[k] ? s_row[i] : 0ull;
