@@ -1886,6 +1886,31 @@ __device__ __forceinline__ Op32 decode_op32(uint64_t key, uint32_t m, uint32_t r
     return o;
 }
 
+// The loop keeps an op as (row, meta) in 32 bits each, validated once when
+// the request is admitted: meta bit 2 (never set in an op's meta by its
+// layout: kind in bits 0-1, field in 8-23, scan length in 24-31) records that
+// the op is valid.
+constexpr uint32_t kLoopValid = 4u;
+
+__device__ __forceinline__ uint32_t loop_pack_meta(uint64_t key, uint32_t m, uint32_t rows, uint32_t fields,
+                                                   uint32_t max_scan)
+{
+    const Op32 o = decode_op32(key, m, rows, fields, max_scan);
+    return (m & ~kLoopValid) | (o.valid ? kLoopValid : 0u);
+}
+
+__device__ __forceinline__ Op32 loop_decode(uint32_t row, uint32_t m, uint32_t rows)
+{
+    Op32 o;
+    o.kind = m & 3u;
+    o.field = (m >> 8) & 0xffffu;
+    o.row = row;
+    o.valid = (m & kLoopValid) != 0u;
+    const uint32_t sl = (m >> 24) + 1u, left = rows - row;  // (row < rows when valid)
+    o.len = !o.valid || o.kind == LTPG_KV_WRITE ? 0u : o.kind == LTPG_KV_READ ? 1u : (sl < left ? sl : left);
+    return o;
+}
+
 static_assert(kSmallSlots == 4096, "loop_hash yields 12 bits");
 __device__ __forceinline__ uint32_t loop_hash(uint32_t row) { return (row * 0x9E3779B1u) >> 20; }
 
@@ -2027,7 +2052,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
 {
     // op arrays, double-buffered: the requeue writes the carried requests
     // into the half the next epoch reads while this epoch's half is read
-    __shared__ uint64_t s_rowb[2][kSmallOps];
+    // (rows and validated metas: loop_pack_meta)
+    __shared__ uint32_t s_rowb[2][kSmallOps];
     __shared__ uint32_t s_metab[2][kSmallOps];
     __shared__ uint64_t s_seqb[2][kLoopMaxT];        // arrival sequence number by position, double-buffered
     __shared__ uint32_t s_key[kSmallSlots];          // row + 1 of a written row, 0 = empty
@@ -2080,7 +2106,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     const uint64_t t_start = s_start;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
-        s_rowb[0][i] = a.carry_keys[i];
+        s_rowb[0][i] = (uint32_t)a.carry_keys[i];  // saved by the last launch, already validated
         s_metab[0][i] = a.carry_meta[i];
     }
     for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) s_seqb[0][t] = a.carry_seq[t];
@@ -2151,7 +2177,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     uint32_t B = 0, n_new = 0;
     uint64_t now = 0, tail_seen = 0;
     while (true) {
-        uint64_t *const s_row = s_rowb[op];
+        uint32_t *const s_row = s_rowb[op];
         uint32_t *const s_meta = s_metab[op];
         uint64_t *const s_seq = s_seqb[op];
         uint8_t *const s_waw = s_wawb[op], *const s_raw = s_rawb[op], *const s_war = s_warb[op];
@@ -2218,8 +2244,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t i = k * kSmallBlock + threadIdx.x;
                 if (i >= c0 && i < n) {
-                    s_row[i] = r[k];
-                    s_meta[i] = m[k];
+                    s_row[i] = (uint32_t)r[k];
+                    s_meta[i] = loop_pack_meta(r[k], m[k], rows32, fields, max_scan);
                 }
             }
         }
@@ -2260,12 +2286,12 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 pf_m[k] = ring_load(&a.ring_meta[at]);
             }
         }
-        uint64_t r[kSmallPer];
+        uint32_t r[kSmallPer];
         uint32_t m[kSmallPer];
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t i = k * kSmallBlock + threadIdx.x;
-            r[k] = i < n ? s_row[i] : 0ull;
+            r[k] = i < n ? s_row[i] : 0u;
             m[k] = i < n ? s_meta[i] : 0u;
         }
         // EXECUTE: W registrations into the hash (priority = position); a
@@ -2277,7 +2303,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             wslot[k] = ~0u;
             const uint32_t i = k * kSmallBlock + threadIdx.x;
             if (i >= n) continue;
-            const Op32 o = decode_op32(r[k], m[k], rows32, fields, max_scan);
+            const Op32 o = loop_decode(r[k], m[k], rows32);
             bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
             if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot32) continue;
             const uint32_t key = o.row + 1u;
@@ -2299,7 +2325,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         for (uint32_t k = 0; k < kSmallPer; ++k) {
             const uint32_t i = k * kSmallBlock + threadIdx.x;
             if (i >= n) continue;
-            const Op32 o = decode_op32(r[k], m[k], rows32, fields, max_scan);
+            const Op32 o = loop_decode(r[k], m[k], rows32);
             if (!o.valid) continue;
             const uint32_t t = dS.div(i);
             if (o.kind == LTPG_KV_WRITE) {
@@ -2421,14 +2447,14 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         const bool go = !n_stop && n_now - t_start <= a.slice_ticks && B_n > 0 &&
                         done_tail + 1 + B + 1 + B_n - n_dh <= done_mask + 1;
         if (go) {
-            uint64_t *const nx_row = s_rowb[op ^ 1u] + (uint64_t)carry_n * S;
+            uint32_t *const nx_row = s_rowb[op ^ 1u] + (uint64_t)carry_n * S;
             uint32_t *const nx_meta = s_metab[op ^ 1u] + (uint64_t)carry_n * S;
 #pragma unroll
             for (uint32_t k = 0; k < kSmallPer; ++k) {
                 const uint32_t i = k * kSmallBlock + threadIdx.x;
                 if (i < new_n * S) {
-                    nx_row[i] = pf_r[k];
-                    nx_meta[i] = pf_m[k];
+                    nx_row[i] = (uint32_t)pf_r[k];
+                    nx_meta[i] = loop_pack_meta(pf_r[k], pf_m[k], rows32, fields, max_scan);
                 }
             }
             for (uint32_t t = threadIdx.x; t < new_n; t += kSmallBlock) s_seqb[op ^ 1u][carry_n + t] = head + t;
@@ -2464,7 +2490,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t j = dper.div(x), rem = x - j * per, q = dfw.div(rem), w = rem - q * fw;
             const uint32_t jt = dS.div(j), t = listv[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
-            const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
+            const Op32 o = loop_decode(s_row[i], s_meta[i], rows32);
             const bool ok = q < o.len;
             src = ok ? ((uint64_t)(o.row + q) * fields + o.field) * fsw + w : 0ull;
             dst = ((done_tail + 1 + t) & done_mask) * entry_words + ((uint64_t)sl * L + q) * fw + w;
@@ -2478,7 +2504,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t j = dfw.div(x), w = x - j * fw;
             const uint32_t jt = dS.div(j), t = listv[jt], sl = j - jt * S;
             const uint32_t i = t * S + sl;
-            const Op32 o = decode_op32(s_row[i], s_meta[i], rows32, fields, max_scan);
+            const Op32 o = loop_decode(s_row[i], s_meta[i], rows32);
             const bool wr = o.valid && o.kind == LTPG_KV_WRITE;
             add = (uint32_t)s_seq[t] | 1u;
             off = wr ? ((uint64_t)o.row * fields + o.field) * fsw + w : 0ull;
@@ -2548,8 +2574,8 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
         // requeue: the aborted requests move to the front, in position order
         // (everything is re-read from LDS: nothing of the phases above stays
         // in registers across the read-out and write-back)
-        uint64_t sq[kSmallPer], ro[kSmallPer];
-        uint32_t mo[kSmallPer], dst_t[kSmallPer], dst_i[kSmallPer];
+        uint64_t sq[kSmallPer];
+        uint32_t ro[kSmallPer], mo[kSmallPer], dst_t[kSmallPer], dst_i[kSmallPer];
         bool ab[kSmallPer], abop[kSmallPer];
 #pragma unroll
         for (uint32_t k = 0; k < kSmallPer; ++k) {
@@ -2562,7 +2588,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             const uint32_t di = i < n ? dstv[ti] : ~0u;
             abop[k] = di != ~0u;
             dst_i[k] = di * S + (i - ti * S);
-            ro[k] = abop[k] ? s_row[i] : 0ull;
+            ro[k] = abop[k] ? s_row[i] : 0u;
             mo[k] = abop[k] ? s_meta[i] : 0u;
         }
         // everything goes to the other halves, which no wave reads in this
@@ -2595,7 +2621,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
     }
     // save the carried requests for the next launch (same stream: ordered)
     for (uint32_t i = threadIdx.x; i < n_carry * S; i += kSmallBlock) {
-        a.carry_keys[i] = s_rowb[op][i];
+        a.carry_keys[i] = s_rowb[op][i];  // (the packed form; only this kernel reads it back)
         a.carry_meta[i] = s_metab[op][i];
     }
     for (uint32_t t = threadIdx.x; t < n_carry; t += kSmallBlock) a.carry_seq[t] = s_seqb[op][t];
