@@ -2294,8 +2294,11 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             r[k] = i < n ? s_row[i] : 0u;
             m[k] = i < n ? s_meta[i] : 0u;
         }
-        // EXECUTE: W registrations into the hash (priority = position); a
-        // write keeps its slot for the marking below
+        // EXECUTE: W registrations into the hash (priority = position).  A
+        // single-row read takes a slot for its row as well (its minimum
+        // stays ~0 unless a write registers there), so that the marking
+        // below reads the row's minimum without probing; every slot taken is
+        // kept for the marking.
         uint32_t bad = 0;
         uint32_t wslot[kSmallPer];  // ~0: no slot
 #pragma unroll
@@ -2305,7 +2308,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
             if (i >= n) continue;
             const Op32 o = loop_decode(r[k], m[k], rows32);
             bad += (o.kind != LTPG_KV_NONE && !o.valid) ? 1u : 0u;
-            if (!o.valid || o.kind != LTPG_KV_WRITE || o.row == hot32) continue;
+            if (!o.valid || o.kind == LTPG_KV_SCAN || o.row == hot32) continue;
             const uint32_t key = o.row + 1u;
             uint32_t h = loop_hash(o.row);
             while (true) {
@@ -2314,7 +2317,7 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 h = (h + 1) & (kSmallSlots - 1);
             }
             wslot[k] = h;
-            atomicMin(&s_min[h], dS.div(i));
+            if (o.kind == LTPG_KV_WRITE) atomicMin(&s_min[h], dS.div(i));
         }
         if (bad) atomicAdd(&s_bad, bad);
         lds_sync();
@@ -2332,16 +2335,27 @@ __global__ __launch_bounds__(kSmallBlock) void kv_serve_loop_kernel(ServeLoopArg
                 if (o.row != hot32 && s_min[wslot[k]] < t) s_waw[t] = 1;
                 continue;
             }
+            // (a minimum of ~0: no write registered the row)
             bool raw = false;
-            for (uint32_t q = 0; q < o.len; ++q) {
-                if (o.row + q == hot32) continue;
-                const int h = loop_find(s_key, o.row + q);
-                if (h < 0) continue;
-                const uint32_t w = s_min[h];
-                if (w < t)
-                    raw = true;
-                else if (w > t)
-                    s_war[w] = 1;
+            if (o.kind == LTPG_KV_READ) {
+                if (wslot[k] != ~0u) {
+                    const uint32_t w = s_min[wslot[k]];
+                    if (w < t)
+                        raw = true;
+                    else if (w > t && w != ~0u)
+                        s_war[w] = 1;
+                }
+            } else {
+                for (uint32_t q = 0; q < o.len; ++q) {
+                    if (o.row + q == hot32) continue;
+                    const int h = loop_find(s_key, o.row + q);
+                    if (h < 0) continue;
+                    const uint32_t w = s_min[h];
+                    if (w < t)
+                        raw = true;
+                    else if (w > t && w != ~0u)
+                        s_war[w] = 1;
+                }
             }
             if (raw) s_raw[t] = 1;
         }
