@@ -953,19 +953,20 @@ def c5_inputs(L, rate, seconds, rows, fields, S, gen, rng, seed, min_requests=20
     return n, keys, meta, tb, at, kind
 
 
-# The device loop's epoch (at most 64 requests, no kept reads) as a chain of
-# steps, counted from kv_serve_loop_kernel along its longest lane: 6 LDS-only
-# workgroup barriers (after admission, registration, marking, CHECK,
-# write-back and requeue); 22 dependent LDS accesses (control words 1;
-# admission: staging read, op write 2; registration: op words 1; marking:
-# hash probe, minimum, flag store 3; CHECK: flags, count and list writes, the
-# count read back 3; write-back: list, op, sequence number per round, 3 rounds
-# of ~2,900 field words over 960 lanes at C5, 9; requeue: destination, op,
-# compacted write 3); 2 dependent LDS atomics (the registration's
-# compare-and-swap and minimum); one release publish.  The ring's PCIe reads
-# are prefetched an epoch ahead and the write-back's HBM atomics are never
-# waited for, so neither is on the chain.
-CHAIN_STEPS = {"barrier_ns": 6, "lds_load_ns": 22, "lds_atomic_ns": 2, "publish_ns": 1}
+# The device loop's epoch (at most 64 requests, no kept reads, admitted by
+# the epoch before it) as a chain of steps, counted from kv_serve_loop_kernel
+# along its longest lane: 3 LDS-only workgroup barriers (after registration,
+# marking and requeue); 18 dependent LDS accesses (registration: op words 1;
+# marking: a row's minimum and the flag store, for two op slots, 3; CHECK:
+# flags and the wave's commit arrays 2; next admission: control words and
+# op writes 2; write-back: commit list, then op and sequence number, 3 rounds
+# of ~2,900 field words over 960 lanes at C5, 6; requeue: destination, op,
+# its write 3; the record's destination 1); 2 dependent LDS atomics (the
+# registration's compare-and-swap and minimum); one release publish (wave 0,
+# while the other waves issue the write-back).  The ring's PCIe reads are
+# prefetched an epoch ahead and the write-back's HBM atomics are never waited
+# for, so neither is on the chain.
+CHAIN_STEPS = {"barrier_ns": 3, "lds_load_ns": 18, "lds_atomic_ns": 2, "publish_ns": 1}
 
 
 def chain_bound(lat, epoch_us):
@@ -1037,7 +1038,7 @@ def run_served_c5(args):
                 f"{r['completed_fraction'] * 100:.0f}% in the drain cap, submit {r['submit_s']:.3f} s for "
                 f"{r['arrival_window_s']:.3f} s of arrivals")
     # the per-part chain of the device loop's epochs: one profiled pass at the
-    # headline rate (profiling adds ~1.5 us per epoch, so it is not timed)
+    # headline rate (its clock stamps add to each epoch, so it is not timed)
     n, keys, meta, tb, at, kind = inputs[head_rate]
     prof = _c5_rate(args, L, kv, tp, S, head_rate, n, keys, meta, tb, at, kind, 0, profile=True)
     # the step latencies the epoch chains (barrier, LDS load / atomic, PCIe
