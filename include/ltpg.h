@@ -723,7 +723,8 @@ typedef struct ltpg_serve_device_stats {
                                  time in each part of the epochs (control read,
                                  admission, EXECUTE, read marking, CHECK, unused,
                                  read-out and write-back, publication, requeue,
-                                 idle) -- a profiled epoch takes ~1.5 us longer */
+                                 idle) -- a profiled epoch takes a few tenths
+                                 of a microsecond longer                         */
     uint64_t prof_epochs;     /* epochs profiled                                  */
 } ltpg_serve_device_stats;
 

@@ -438,10 +438,15 @@ int alloc_loop(ltpg_server *sv)
     a.growth = sv->cfg.epoch_growth;
     a.idle_ticks = kLoopIdleUs * per_us;
     a.stall_ticks = kLoopStallUs * per_us;
-    a.slice_ticks = kLoopSliceUs * per_us;
+    // environment LTPG_SERVE_SLICE_US: a shorter time slice (10 us at least),
+    // so that tests see many launches end with requests carried over
+    uint64_t slice_us = kLoopSliceUs;
+    if (const char *e = std::getenv("LTPG_SERVE_SLICE_US"))
+        slice_us = std::min<uint64_t>(std::max<uint64_t>(std::strtoull(e, nullptr, 10), 10), kLoopSliceUs);
+    a.slice_ticks = slice_us * per_us;
     a.ctr = L.ctx->counters;
     a.profile = std::getenv("LTPG_SERVE_PROFILE") != nullptr;
-    lp->slice_ns = kLoopSliceUs * 1000;
+    lp->slice_ns = slice_us * 1000;
     lp->ticks_per_us = (double)per_us;
     return LTPG_OK;
 }

@@ -56,22 +56,31 @@ def _submit_in_waves(sv, keys, meta, waves=12, pause=0.002):
         time.sleep(pause)
 
 
-@pytest.mark.parametrize("form,keep", [(f, 1) for f in KV_FORMS] + [("device-loop", 0)],
-                         ids=list(KV_FORMS) + ["device-loop-no-reads"])
-def test_served_kv_epochs_replay_on_the_oracle(gpu, form, keep):
+@pytest.mark.parametrize("form,keep,slice_us",
+                         [(f, 1, None) for f in KV_FORMS] + [("device-loop", 0, None), ("device-loop", 0, 40),
+                                                             ("device-loop", 1, 40)],
+                         ids=list(KV_FORMS) + ["device-loop-no-reads", "device-loop-no-reads-short-slices",
+                                               "device-loop-short-slices"])
+def test_served_kv_epochs_replay_on_the_oracle(gpu, form, keep, slice_us, monkeypatch):
     """The device loop, host-formed epochs and the plain per-batch form
     (same admission): every recorded epoch replayed by the oracle; the
     device loop also without kept reads (no table loads in the loop, so its
-    write-back atomics are never waited for inside an epoch)."""
+    write-back atomics are never waited for inside an epoch), and with 40 us
+    time slices, so that many launches end with aborted requests carried
+    over to the next one (saved to device memory and admitted again)."""
     rows, fields, fw, S, n = 3000, 3, 2, 8, 6000
     ctx = _kv_ctx(rows, fields, fw, S, 1024)
     keys, meta = _kv_requests(31, n, S, rows, fields)
+    if slice_us:
+        monkeypatch.setenv("LTPG_SERVE_SLICE_US", str(slice_us))
     sv = L.Server(ctx, None, kv_ops=S, slots=2, max_epoch_txns=700, keep_reads=keep, record_epochs=1,
                   **KV_FORMS[form])
+    monkeypatch.delenv("LTPG_SERVE_SLICE_US", raising=False)
     _submit_in_waves(sv, keys, meta)
     sv.drain()
     done = {d["id"]: d for d in sv.poll()}
     st = sv.stats()
+    dev = sv.device_stats()
     epochs = [sv.epoch(L.WORKLOAD_KV, e) for e in range(st["epochs"][0])]
     reads = {rid: sv.read_values(rid) for rid in done} if keep else None
     sv.stop()
@@ -95,6 +104,9 @@ def test_served_kv_epochs_replay_on_the_oracle(gpu, form, keep):
     np.testing.assert_array_equal(ctx.kv_table_read(), table)
     # every request committed after at least one attempt, latency is positive
     assert all(d["attempts"] >= 1 and d["commit_ns"] >= d["arrival_ns"] for d in done.values())
+    if slice_us:
+        # the short slices ended launches while requests were carried over
+        assert dev["exits"]["slice"] >= 10, dev
 
 
 @pytest.mark.parametrize("form", list(KV_FORMS))
