@@ -56,19 +56,25 @@ def _submit_in_waves(sv, keys, meta, waves=12, pause=0.002):
         time.sleep(pause)
 
 
-@pytest.mark.parametrize("form,keep,slice_us",
-                         [(f, 1, None) for f in KV_FORMS] + [("device-loop", 0, None), ("device-loop", 0, 40),
-                                                             ("device-loop", 1, 40)],
+@pytest.mark.parametrize("form,keep,slice_us,S",
+                         [(f, 1, None, 8) for f in KV_FORMS] +
+                         [("device-loop", 0, None, 8), ("device-loop", 0, 40, 8), ("device-loop", 1, 40, 8),
+                          ("device-loop", 1, None, 64), ("device-loop", 0, None, 64), ("device-loop", 0, 40, 64),
+                          ("host-formed", 1, None, 64)],
                          ids=list(KV_FORMS) + ["device-loop-no-reads", "device-loop-no-reads-short-slices",
-                                               "device-loop-short-slices"])
-def test_served_kv_epochs_replay_on_the_oracle(gpu, form, keep, slice_us, monkeypatch):
+                                               "device-loop-short-slices", "device-loop-64-ops",
+                                               "device-loop-64-ops-no-reads", "device-loop-64-ops-short-slices",
+                                               "host-formed-64-ops"])
+def test_served_kv_epochs_replay_on_the_oracle(gpu, form, keep, slice_us, S, monkeypatch):
     """The device loop, host-formed epochs and the plain per-batch form
     (same admission): every recorded epoch replayed by the oracle; the
     device loop also without kept reads (no table loads in the loop, so its
     write-back atomics are never waited for inside an epoch), and with 40 us
     time slices, so that many launches end with aborted requests carried
-    over to the next one (saved to device memory and admitted again)."""
-    rows, fields, fw, S, n = 3000, 3, 2, 8, 6000
+    over to the next one (saved to device memory and admitted again).  With
+    64 op slots per request (1-64 ops, the served workload's shape) an epoch
+    holds at most 32 requests."""
+    rows, fields, fw, n = 3000, 3, 2, 6000
     ctx = _kv_ctx(rows, fields, fw, S, 1024)
     keys, meta = _kv_requests(31, n, S, rows, fields)
     if slice_us:
